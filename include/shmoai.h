@@ -1,0 +1,202 @@
+/*
+ * shmoai.h — C ABI of the MI355X-native CKKS engine (libshmoai.so).
+ *
+ * This is the drop-in boundary for ShMOAI's encrypted-BERT hot path. The
+ * reference exposes no C/FFI interface: its API is the C++ `moai::` facade
+ * (src/include/source/ckks_evaluator_parallel.cuh:13-680) over PhantomFHE free
+ * functions (thirdparty/phantom-fhe/include/evaluate.cuh:32-304) and classes
+ * (context.cuh:136, secretkey.h:18-309, ciphertext.h:21, ckks.h). Each entry
+ * point below names the reference interface it replaces.
+ *
+ * Conventions
+ *  - Every function returns SHM_OK (0) or a negative SHM_ERR_* code; the
+ *    message of the last failure on the calling thread is shm_last_error().
+ *    Error classes mirror the reference's exceptions: std::invalid_argument ->
+ *    SHM_ERR_INVALID_ARGUMENT, std::logic_error -> SHM_ERR_LOGIC,
+ *    std::runtime_error -> SHM_ERR_RUNTIME; the engine never calls exit().
+ *  - Streams are hipStream_t passed as void*; NULL is the legacy default
+ *    stream. Operations only enqueue work, except where noted (import/export,
+ *    decode, key generation synchronise the stream before returning).
+ *  - chain_index follows the reference: 1 = all data primes, +1 per dropped
+ *    prime (context.cu:151-174).
+ *  - Device data layout: a ciphertext is uint64 [size][Ql][N] in NTT form, a
+ *    plaintext uint64 [Ql][N]; Ql = size_q - chain_index + 1.
+ *  - Handles are owned by the caller and released with the matching
+ *    *_destroy. Contexts, keys and plaintexts may be shared read-only between
+ *    threads that use different streams.
+ */
+#ifndef SHMOAI_H
+#define SHMOAI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SHM_OK 0
+#define SHM_ERR_INVALID_ARGUMENT (-1)
+#define SHM_ERR_LOGIC (-2)
+#define SHM_ERR_RUNTIME (-3)
+#define SHM_ERR_OUT_OF_MEMORY (-4)
+#define SHM_ERR_DEVICE (-5)
+
+/* ct-pt matmul variants (shm_ct_pt_matrix_mul `variant`) */
+#define SHM_MATMUL_U64 0    /* batched 64-bit contraction + batched rescale */
+#define SHM_MATMUL_PER_CT 1 /* reference structure: multiply_plain + add per term */
+#define SHM_MATMUL_INT8 2   /* batched INT8-MFMA contraction (bit-identical) */
+
+typedef struct shm_context shm_context;
+typedef struct shm_ciphertext shm_ciphertext;
+typedef struct shm_plaintext shm_plaintext;
+typedef struct shm_secret_key shm_secret_key;
+typedef struct shm_kswitch_key shm_kswitch_key;
+typedef struct shm_galois_keys shm_galois_keys;
+typedef void *shm_stream_t;
+
+typedef struct shm_context_info {
+    uint32_t poly_degree;  /* N */
+    uint32_t size_q;       /* data primes */
+    uint32_t size_p;       /* special primes (alpha) */
+    uint32_t dnum;         /* size_q / size_p key digits */
+} shm_context_info;
+
+/* ---- errors / version ---------------------------------------------------- */
+const char *shm_last_error(void);
+const char *shm_version(void);
+
+/* ---- context --------------------------------------------------------------
+ * Replaces EncryptionParameters + CoeffModulus::Create (modulus.cu:79-108) +
+ * PhantomContext(parms) (context.cu:125-238). coeff_bit_sizes lists the data
+ * primes followed by the special primes, exactly as passed to
+ * CoeffModulus::Create; special_modulus_size = set_special_modulus_size. */
+int shm_context_create(uint32_t log_poly_degree, const int *coeff_bit_sizes, uint32_t n_coeff,
+                       uint32_t special_modulus_size, shm_context **out);
+int shm_context_destroy(shm_context *ctx);
+int shm_context_get_info(const shm_context *ctx, shm_context_info *info);
+/* all primes, data then special (size_q + size_p words) */
+int shm_context_coeff_modulus(const shm_context *ctx, uint64_t *out, size_t n);
+/* galois.cu:47-90 PhantomGaloisTool::get_elt_from_step */
+int shm_galois_elt_from_step(const shm_context *ctx, int step, uint32_t *elt);
+
+/* ---- raw limb transforms ------------------------------------------------------
+ * Replace nwt_2d_radix8_forward_inplace / nwt_2d_radix8_backward_inplace
+ * (fntt_2d.cu:620, intt_2d.cu:724): n_polys x n_limbs limbs, limb k uses prime
+ * first_prime + k. `data` is a device pointer. */
+int shm_ntt_forward(shm_context *ctx, uint64_t *data, uint32_t n_polys, uint32_t n_limbs, uint32_t first_prime,
+                    shm_stream_t stream);
+int shm_ntt_inverse(shm_context *ctx, uint64_t *data, uint32_t n_polys, uint32_t n_limbs, uint32_t first_prime,
+                    shm_stream_t stream);
+
+/* ---- ciphertexts / plaintexts (PhantomCiphertext ciphertext.h:21-288,
+ * PhantomPlaintext plaintext.h) -------------------------------------------- */
+int shm_ciphertext_create(shm_context *ctx, uint32_t size, uint32_t chain_index, shm_ciphertext **out);
+int shm_ciphertext_destroy(shm_ciphertext *ct);
+int shm_ciphertext_info(const shm_ciphertext *ct, uint32_t *size, uint32_t *chain_index, uint32_t *coeff_mod_size,
+                        double *scale);
+int shm_ciphertext_set_scale(shm_ciphertext *ct, double scale);
+uint64_t *shm_ciphertext_data(shm_ciphertext *ct);
+int shm_ciphertext_import(shm_ciphertext *ct, const uint64_t *host, size_t n_words, shm_stream_t stream);
+int shm_ciphertext_export(const shm_ciphertext *ct, uint64_t *host, size_t n_words, shm_stream_t stream);
+int shm_ciphertext_copy(const shm_ciphertext *src, shm_ciphertext *dst, shm_stream_t stream);
+
+int shm_plaintext_create(shm_context *ctx, uint32_t chain_index, shm_plaintext **out);
+int shm_plaintext_destroy(shm_plaintext *pt);
+int shm_plaintext_info(const shm_plaintext *pt, uint32_t *chain_index, uint32_t *coeff_mod_size, double *scale);
+int shm_plaintext_set_scale(shm_plaintext *pt, double scale);
+uint64_t *shm_plaintext_data(shm_plaintext *pt);
+int shm_plaintext_import(shm_plaintext *pt, const uint64_t *host, size_t n_words, shm_stream_t stream);
+int shm_plaintext_export(const shm_plaintext *pt, uint64_t *host, size_t n_words, shm_stream_t stream);
+
+/* ---- keys (PhantomSecretKey secretkey.cu:431-646) -------------------------
+ * Key material is injected: a ternary secret from explicit coefficients, or
+ * from a seeded exact-Hamming-weight sampler; uniform `a` polynomials and
+ * discrete-Gaussian (sigma 3.2) errors come from SplitMix64 counter streams of
+ * the given seed, so the same seed reproduces the same keys on any host. */
+int shm_secret_key_create(shm_context *ctx, const int8_t *coeffs, shm_secret_key **out);
+int shm_secret_key_generate(shm_context *ctx, uint64_t seed, uint32_t hamming_weight, shm_secret_key **out);
+int shm_secret_key_destroy(shm_secret_key *sk);
+/* NTT-form secret over all QP primes (size_q + size_p limbs) */
+int shm_secret_key_export(const shm_secret_key *sk, uint64_t *host, size_t n_words);
+/* secretkey.cu:539-565 gen_relinkey */
+int shm_relin_key_generate(shm_context *ctx, const shm_secret_key *sk, uint64_t seed, shm_kswitch_key **out);
+/* secretkey.cu:567-575 gen_kswitch_key: switch from new_key (NTT form over QP, host) to sk */
+int shm_kswitch_key_generate(shm_context *ctx, const shm_secret_key *sk, const uint64_t *new_key_qp, uint64_t seed,
+                             shm_kswitch_key **out);
+int shm_kswitch_key_destroy(shm_kswitch_key *key);
+/* dnum x [2][QP][N] words */
+int shm_kswitch_key_export(const shm_kswitch_key *key, uint64_t *host, size_t n_words);
+/* secretkey.cu:634-646 create_galois_keys_from_steps; the key of element elt
+ * uses seed splitmix64(seed, elt) */
+int shm_galois_keys_generate(shm_context *ctx, const shm_secret_key *sk, const int *steps, size_t n_steps,
+                             uint64_t seed, shm_galois_keys **out);
+int shm_galois_keys_destroy(shm_galois_keys *gk);
+int shm_galois_keys_export(const shm_galois_keys *gk, uint32_t elt, uint64_t *host, size_t n_words);
+
+/* ---- encryption (secretkey.cu:260-326 symmetric, 725-766 decrypt) -------- */
+int shm_encrypt_symmetric(shm_context *ctx, const shm_secret_key *sk, const shm_plaintext *pt, uint64_t seed,
+                          shm_ciphertext *out, shm_stream_t stream);
+int shm_decrypt(shm_context *ctx, const shm_secret_key *sk, const shm_ciphertext *ct, shm_plaintext *out,
+                shm_stream_t stream);
+
+/* ---- encoder (PhantomCKKSEncoder ckks.cu:89-246; moai::Encoder
+ * ckks_evaluator_parallel.cuh:34-103) --------------------------------------- */
+int shm_encode(shm_context *ctx, const double *re, const double *im, size_t n_values, uint32_t chain_index,
+               double scale, shm_plaintext *out, shm_stream_t stream);
+/* moai::Encoder::encode(double value, chain_index, scale, pt): all slots = value */
+int shm_encode_scalar(shm_context *ctx, double value, uint32_t chain_index, double scale, shm_plaintext *out,
+                      shm_stream_t stream);
+/* synchronises the stream; n <= N/2 slots written to re/im (im may be NULL) */
+int shm_decode(shm_context *ctx, const shm_plaintext *pt, double *re, double *im, size_t n, shm_stream_t stream);
+
+/* ---- evaluator (evaluate.cuh:32-304) ---------------------------------------- */
+int shm_add_inplace(shm_context *ctx, shm_ciphertext *a, const shm_ciphertext *b, shm_stream_t stream);
+int shm_sub_inplace(shm_context *ctx, shm_ciphertext *a, const shm_ciphertext *b, shm_stream_t stream);
+int shm_negate_inplace(shm_context *ctx, shm_ciphertext *a, shm_stream_t stream);
+int shm_add_plain_inplace(shm_context *ctx, shm_ciphertext *a, const shm_plaintext *pt, shm_stream_t stream);
+int shm_sub_plain_inplace(shm_context *ctx, shm_ciphertext *a, const shm_plaintext *pt, shm_stream_t stream);
+/* evaluate.cu:1237-1266 multiply_plain_ntt */
+int shm_multiply_plain_inplace(shm_context *ctx, shm_ciphertext *ct, const shm_plaintext *pt, shm_stream_t stream);
+/* evaluate.cu:350-402 (size-3 result) */
+int shm_multiply_inplace(shm_context *ctx, shm_ciphertext *a, const shm_ciphertext *b, shm_stream_t stream);
+int shm_square_inplace(shm_context *ctx, shm_ciphertext *a, shm_stream_t stream);
+/* evaluate.cu:1358-1390 */
+int shm_relinearize_inplace(shm_context *ctx, shm_ciphertext *ct, const shm_kswitch_key *relin_key,
+                            shm_stream_t stream);
+/* eval_key_switch.cu:91-178 keyswitch_inplace with an arbitrary key (size-2 ct) */
+int shm_apply_kswitch_key_inplace(shm_context *ctx, shm_ciphertext *ct, const shm_kswitch_key *key,
+                                  shm_stream_t stream);
+/* evaluate.cu:1563-1584 rescale_to_next (floor division by the last prime) */
+int shm_rescale_to_next_inplace(shm_context *ctx, shm_ciphertext *ct, shm_stream_t stream);
+/* evaluate.cu:1445-1561 mod_switch_to_next / mod_switch_to (drop limbs) */
+int shm_mod_switch_to_next_inplace(shm_context *ctx, shm_ciphertext *ct, shm_stream_t stream);
+int shm_mod_switch_to_inplace(shm_context *ctx, shm_ciphertext *ct, uint32_t chain_index, shm_stream_t stream);
+int shm_plain_mod_switch_to_inplace(shm_context *ctx, shm_plaintext *pt, uint32_t chain_index, shm_stream_t stream);
+/* evaluate.cu:1586-1647 apply_galois_inplace */
+int shm_apply_galois_inplace(shm_context *ctx, shm_ciphertext *ct, uint32_t galois_elt, const shm_galois_keys *gk,
+                             shm_stream_t stream);
+/* evaluate.cu:1650-1715 rotate_vector_inplace (NAF split when a key is missing) */
+int shm_rotate_vector_inplace(shm_context *ctx, shm_ciphertext *ct, int step, const shm_galois_keys *gk,
+                              shm_stream_t stream);
+/* evaluate.cu:1717-1725 complex_conjugate_inplace */
+int shm_complex_conjugate_inplace(shm_context *ctx, shm_ciphertext *ct, const shm_galois_keys *gk,
+                                  shm_stream_t stream);
+
+/* ---- MOAI ct-pt matmul ---------------------------------------------------------
+ * Replaces ct_pt_matrix_mul_wo_pre(enc_X, W, col_X, col_W, row_W, context)
+ * (src/include/source/matrix_mul/Ct_pt_matrix_mul.cuh:18-185):
+ * out[i] = rescale(sum_{j<row_W} X[j] (.) encode(W[j][i], X[j].scale)),
+ * out[i].scale = X[0].scale. W is row-major row_W x col_W (W[j*col_W + i]).
+ * out[] must hold col_W ciphertexts; they are resized as needed.
+ * Synchronises the stream. */
+int shm_ct_pt_matrix_mul(shm_context *ctx, const shm_ciphertext *const *enc_X, uint32_t row_W, const double *W,
+                         uint32_t col_W, shm_ciphertext *const *out, int variant, shm_stream_t stream);
+/* duration of the dominant kernel of the last matmul on this process, ms (HIP events) */
+double shm_last_matmul_kernel_ms(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SHMOAI_H */

@@ -1,0 +1,326 @@
+"""ctypes front end of the oracle (oracle/librefcpu.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the shmoai_amd product package.
+Functions mirror refcpu.h; arrays are numpy uint64/int64/float64, C-contiguous.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "librefcpu.so")
+_lib = None
+
+u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+def build(force: bool = False) -> str:
+    """Compile oracle/librefcpu.so with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        vp = C.c_void_p
+        sz = C.c_size_t
+        sig = {
+            "ref_ctx_new": (vp, [C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int]),
+            "ref_ctx_free": (None, [vp]),
+            "ref_ctx_nq": (sz, [vp]),
+            "ref_ctx_np": (sz, [vp]),
+            "ref_ctx_primes": (None, [vp, u64p]),
+            "ref_ctx_psi": (None, [vp, u64p]),
+            "ref_ctx_tables": (None, [vp, sz, u64p, u64p]),
+            "ref_set_threads": (None, [C.c_int]),
+            "ref_is_prime": (C.c_int, [C.c_uint64]),
+            "ref_get_primes": (None, [sz, C.c_int, sz, u64p]),
+            "ref_ntt": (None, [vp, u64p, sz, u32p]),
+            "ref_intt": (None, [vp, u64p, sz, u32p]),
+            "ref_add": (None, [vp, u64p, u64p, u64p, sz, sz]),
+            "ref_sub": (None, [vp, u64p, u64p, u64p, sz, sz]),
+            "ref_negate": (None, [vp, u64p, u64p, sz, sz]),
+            "ref_multiply_plain": (None, [vp, u64p, u64p, sz, sz]),
+            "ref_tensor": (None, [vp, u64p, u64p, u64p, sz]),
+            "ref_rescale": (None, [vp, u64p, u64p, sz, sz]),
+            "ref_mod_switch_drop": (None, [vp, u64p, u64p, sz, sz]),
+            "ref_galois_elt_from_step": (C.c_uint32, [vp, C.c_int]),
+            "ref_galois_table": (None, [vp, C.c_uint32, u32p]),
+            "ref_apply_galois_ntt": (None, [vp, C.c_uint32, u64p, u64p, sz]),
+            "ref_beta": (sz, [vp, sz]),
+            "ref_modup": (None, [vp, u64p, sz, u64p]),
+            "ref_ks_inner": (None, [vp, u64p, u64p, sz, u64p]),
+            "ref_moddown": (None, [vp, u64p, sz, u64p]),
+            "ref_keyswitch": (None, [vp, u64p, u64p, sz, u64p, u64p]),
+            "ref_rotate": (None, [vp, u64p, sz, C.c_uint32, u64p, u64p]),
+            "ref_relinearize": (None, [vp, u64p, sz, u64p, u64p]),
+            "ref_splitmix64": (C.c_uint64, [C.c_uint64, C.c_uint64]),
+            "ref_sample_uniform": (None, [vp, C.c_uint64, sz, u32p, u64p]),
+            "ref_sample_gaussian": (None, [vp, C.c_uint64, i64p]),
+            "ref_sample_ternary": (None, [vp, C.c_uint64, sz, i64p]),
+            "ref_sk_ntt": (None, [vp, i64p, u64p]),
+            "ref_gen_kswitch_key": (None, [vp, u64p, u64p, C.c_uint64, u64p]),
+            "ref_gen_relin_key": (None, [vp, u64p, C.c_uint64, u64p]),
+            "ref_gen_galois_key": (None, [vp, u64p, C.c_uint32, C.c_uint64, u64p]),
+            "ref_encrypt_symmetric": (None, [vp, u64p, u64p, sz, C.c_uint64, u64p]),
+            "ref_decrypt": (None, [vp, u64p, u64p, sz, sz, u64p]),
+            "ref_encode": (None, [vp, f64p, C.c_void_p, sz, sz, sz, C.c_double, u64p]),
+            "ref_encode_scalar": (None, [vp, C.c_double, sz, C.c_double, u64p]),
+            "ref_decode": (None, [vp, u64p, sz, sz, C.c_double, f64p, f64p]),
+            "ref_ctpt_matmul": (None, [vp, C.POINTER(C.c_void_p), sz, f64p, sz, sz, f64p, sz, sz,
+                                       C.POINTER(C.c_void_p)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def set_threads(n: int) -> None:
+    lib().ref_set_threads(int(n))
+
+
+class Ref:
+    """One CKKS parameter set of the oracle (primes, NTT tables, encoder roots)."""
+
+    def __init__(self, logN: int, bit_sizes, n_special: int):
+        self.L = lib()
+        arr = (C.c_int * len(bit_sizes))(*bit_sizes)
+        self.h = self.L.ref_ctx_new(logN, len(bit_sizes), arr, n_special)
+        self.logN = logN
+        self.N = 1 << logN
+        self.nQ = int(self.L.ref_ctx_nq(self.h))
+        self.nP = int(self.L.ref_ctx_np(self.h))
+        self.primes = np.zeros(self.nQ + self.nP, np.uint64)
+        self.L.ref_ctx_primes(self.h, self.primes)
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.ref_ctx_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    # helpers ----------------------------------------------------------------
+    @property
+    def nQP(self):
+        return self.nQ + self.nP
+
+    def dnum(self):
+        return self.nQ // self.nP
+
+    def beta(self, Ql):
+        return int(self.L.ref_beta(self.h, Ql))
+
+    def pidx(self, Ql, withP=False):
+        v = list(range(Ql)) + (list(range(self.nQ, self.nQ + self.nP)) if withP else [])
+        return np.array(v, np.uint32)
+
+    def psi(self):
+        out = np.zeros(self.nQP, np.uint64)
+        self.L.ref_ctx_psi(self.h, out)
+        return out
+
+    def tables(self, prime):
+        tw = np.zeros(self.N, np.uint64)
+        itw = np.zeros(self.N, np.uint64)
+        self.L.ref_ctx_tables(self.h, prime, tw, itw)
+        return tw, itw
+
+    # transforms ---------------------------------------------------------------
+    def ntt(self, data, pidx):
+        d = np.ascontiguousarray(data, np.uint64).copy()
+        self.L.ref_ntt(self.h, d, len(pidx), np.ascontiguousarray(pidx, np.uint32))
+        return d
+
+    def intt(self, data, pidx):
+        d = np.ascontiguousarray(data, np.uint64).copy()
+        self.L.ref_intt(self.h, d, len(pidx), np.ascontiguousarray(pidx, np.uint32))
+        return d
+
+    # elementwise ----------------------------------------------------------------
+    def add(self, a, b, Ql, npolys):
+        out = np.empty_like(a)
+        self.L.ref_add(self.h, a, b, out, Ql, npolys)
+        return out
+
+    def sub(self, a, b, Ql, npolys):
+        out = np.empty_like(a)
+        self.L.ref_sub(self.h, a, b, out, Ql, npolys)
+        return out
+
+    def negate(self, a, Ql, npolys):
+        out = np.empty_like(a)
+        self.L.ref_negate(self.h, a, out, Ql, npolys)
+        return out
+
+    def multiply_plain(self, ct, pt, Ql, npolys=2):
+        out = np.ascontiguousarray(ct, np.uint64).copy()
+        self.L.ref_multiply_plain(self.h, out, np.ascontiguousarray(pt, np.uint64), Ql, npolys)
+        return out
+
+    def tensor(self, a, b, Ql):
+        out = np.zeros(3 * Ql * self.N, np.uint64)
+        self.L.ref_tensor(self.h, a, b, out, Ql)
+        return out
+
+    def rescale(self, ct, Ql, npolys=2):
+        out = np.zeros(npolys * (Ql - 1) * self.N, np.uint64)
+        self.L.ref_rescale(self.h, np.ascontiguousarray(ct, np.uint64), out, Ql, npolys)
+        return out
+
+    def mod_switch_drop(self, ct, Ql, npolys=2):
+        out = np.zeros(npolys * (Ql - 1) * self.N, np.uint64)
+        self.L.ref_mod_switch_drop(self.h, np.ascontiguousarray(ct, np.uint64), out, Ql, npolys)
+        return out
+
+    # galois ----------------------------------------------------------------------
+    def elt(self, step):
+        return int(self.L.ref_galois_elt_from_step(self.h, step))
+
+    def galois_table(self, elt):
+        t = np.zeros(self.N, np.uint32)
+        self.L.ref_galois_table(self.h, elt, t)
+        return t
+
+    def apply_galois(self, data, elt, nlimbs):
+        out = np.zeros(nlimbs * self.N, np.uint64)
+        self.L.ref_apply_galois_ntt(self.h, elt, np.ascontiguousarray(data, np.uint64), out, nlimbs)
+        return out
+
+    # key switching -----------------------------------------------------------------
+    def modup(self, c2, Ql):
+        out = np.zeros(self.beta(Ql) * (Ql + self.nP) * self.N, np.uint64)
+        self.L.ref_modup(self.h, np.ascontiguousarray(c2, np.uint64), Ql, out)
+        return out
+
+    def ks_inner(self, modup, key, Ql):
+        out = np.zeros(2 * (Ql + self.nP) * self.N, np.uint64)
+        self.L.ref_ks_inner(self.h, modup, key, Ql, out)
+        return out
+
+    def moddown(self, cx, Ql):
+        cx = np.ascontiguousarray(cx, np.uint64).copy()
+        out = np.zeros(Ql * self.N, np.uint64)
+        self.L.ref_moddown(self.h, cx, Ql, out)
+        return out
+
+    def keyswitch(self, c2, key, Ql):
+        k0 = np.zeros(Ql * self.N, np.uint64)
+        k1 = np.zeros(Ql * self.N, np.uint64)
+        self.L.ref_keyswitch(self.h, np.ascontiguousarray(c2, np.uint64), key, Ql, k0, k1)
+        return k0, k1
+
+    def rotate(self, ct, Ql, elt, key):
+        out = np.zeros(2 * Ql * self.N, np.uint64)
+        self.L.ref_rotate(self.h, np.ascontiguousarray(ct, np.uint64), Ql, elt, key, out)
+        return out
+
+    def relinearize(self, ct3, Ql, key):
+        out = np.zeros(2 * Ql * self.N, np.uint64)
+        self.L.ref_relinearize(self.h, np.ascontiguousarray(ct3, np.uint64), Ql, key, out)
+        return out
+
+    # samplers / keys / encryption -----------------------------------------------------
+    def splitmix64(self, key, ctr):
+        return int(self.L.ref_splitmix64(key, ctr))
+
+    def sample_uniform(self, seed, pidx):
+        out = np.zeros(len(pidx) * self.N, np.uint64)
+        self.L.ref_sample_uniform(self.h, seed, len(pidx), np.ascontiguousarray(pidx, np.uint32), out)
+        return out
+
+    def sample_gaussian(self, seed):
+        out = np.zeros(self.N, np.int64)
+        self.L.ref_sample_gaussian(self.h, seed, out)
+        return out
+
+    def sample_ternary(self, seed, hw):
+        out = np.zeros(self.N, np.int64)
+        self.L.ref_sample_ternary(self.h, seed, hw, out)
+        return out
+
+    def sk_ntt(self, coeffs):
+        out = np.zeros(self.nQP * self.N, np.uint64)
+        self.L.ref_sk_ntt(self.h, np.ascontiguousarray(coeffs, np.int64), out)
+        return out
+
+    def key_size(self):
+        return self.dnum() * 2 * self.nQP * self.N
+
+    def gen_kswitch_key(self, sk_qp, newkey_qp, seed):
+        out = np.zeros(self.key_size(), np.uint64)
+        self.L.ref_gen_kswitch_key(self.h, sk_qp, np.ascontiguousarray(newkey_qp, np.uint64), seed, out)
+        return out
+
+    def gen_relin_key(self, sk_qp, seed):
+        out = np.zeros(self.key_size(), np.uint64)
+        self.L.ref_gen_relin_key(self.h, sk_qp, seed, out)
+        return out
+
+    def gen_galois_key(self, sk_qp, elt, seed):
+        out = np.zeros(self.key_size(), np.uint64)
+        self.L.ref_gen_galois_key(self.h, sk_qp, elt, seed, out)
+        return out
+
+    def encrypt_symmetric(self, sk_qp, pt, Ql, seed):
+        out = np.zeros(2 * Ql * self.N, np.uint64)
+        self.L.ref_encrypt_symmetric(self.h, sk_qp, np.ascontiguousarray(pt, np.uint64), Ql, seed, out)
+        return out
+
+    def decrypt(self, sk_qp, ct, size, Ql):
+        out = np.zeros(Ql * self.N, np.uint64)
+        self.L.ref_decrypt(self.h, sk_qp, np.ascontiguousarray(ct, np.uint64), size, Ql, out)
+        return out
+
+    # encoder -------------------------------------------------------------------------
+    def encode(self, re, Ql, scale, im=None, sparse_slots=None):
+        re = np.ascontiguousarray(re, np.float64)
+        n = sparse_slots or self.N // 2
+        out = np.zeros(Ql * self.N, np.uint64)
+        imp = None
+        if im is not None:
+            im = np.ascontiguousarray(im, np.float64)
+            imp = im.ctypes.data_as(C.c_void_p)
+        self.L.ref_encode(self.h, re, imp, len(re), n, Ql, float(scale), out)
+        return out
+
+    def encode_scalar(self, value, Ql, scale):
+        out = np.zeros(Ql * self.N, np.uint64)
+        self.L.ref_encode_scalar(self.h, float(value), Ql, float(scale), out)
+        return out
+
+    def decode(self, pt, Ql, scale, sparse_slots=None):
+        n = sparse_slots or self.N // 2
+        re = np.zeros(n, np.float64)
+        im = np.zeros(n, np.float64)
+        self.L.ref_decode(self.h, np.ascontiguousarray(pt, np.uint64), Ql, n, float(scale), re, im)
+        return re, im
+
+    # MOAI ct-pt matmul ------------------------------------------------------------
+    def ctpt_matmul(self, X, W, Ql, scales, col_begin=0, col_end=None):
+        row_w, col_w = W.shape
+        col_end = col_w if col_end is None else col_end
+        Xs = [np.ascontiguousarray(x, np.uint64) for x in X]
+        xp = (C.c_void_p * row_w)(*[x.ctypes.data for x in Xs])
+        outs = [np.zeros(2 * (Ql - 1) * self.N, np.uint64) for _ in range(col_begin, col_end)]
+        op = (C.c_void_p * len(outs))(*[o.ctypes.data for o in outs])
+        self.L.ref_ctpt_matmul(self.h, xp, row_w, np.ascontiguousarray(W, np.float64), col_w, Ql,
+                               np.ascontiguousarray(scales, np.float64), col_begin, col_end, op)
+        return outs

@@ -1,0 +1,417 @@
+"""shmoai_amd — MI355X-native CKKS engine for ShMOAI's encrypted-BERT hot path.
+
+Python mirror of the reference's evaluator API (the `moai::` facade in
+src/include/source/ckks_evaluator_parallel.cuh and the PhantomFHE free
+functions in thirdparty/phantom-fhe/include/evaluate.cuh), layered over the C
+ABI of libshmoai.so (include/shmoai.h). Every operation runs in hand-written
+HIP kernels for gfx950; nothing here computes on the host.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Sequence
+
+import numpy as np
+
+from . import capi
+from .capi import check, MATMUL_U64, MATMUL_PER_CT, MATMUL_INT8  # noqa: F401
+
+__all__ = [
+    "Context", "Ciphertext", "Plaintext", "SecretKey", "KSwitchKey", "GaloisKeys", "Encoder", "Evaluator",
+    "Decryptor", "ct_pt_matrix_mul_wo_pre", "MATMUL_U64", "MATMUL_PER_CT", "MATMUL_INT8",
+]
+
+
+def _lib():
+    return capi.load()
+
+
+def _u64p(a: np.ndarray):
+    return a.ctypes.data_as(capi.pu64)
+
+
+class Context:
+    """EncryptionParameters + CoeffModulus::Create + PhantomContext (context.cu:125).
+
+    bit_sizes lists the data primes then the special primes, as passed to
+    CoeffModulus::Create; special_modulus_size is set_special_modulus_size.
+    """
+
+    def __init__(self, log_n: int, bit_sizes: Sequence[int], special_modulus_size: int):
+        L = _lib()
+        arr = (C.c_int * len(bit_sizes))(*bit_sizes)
+        h = C.c_void_p()
+        check(L.shm_context_create(log_n, arr, len(bit_sizes), special_modulus_size, C.byref(h)))
+        self.h = h
+        info = capi.ContextInfo()
+        check(L.shm_context_get_info(self.h, C.byref(info)))
+        self.N = info.poly_degree
+        self.size_q = info.size_q
+        self.size_p = info.size_p
+        self.dnum = info.dnum
+        self.coeff_modulus = np.zeros(self.size_q + self.size_p, np.uint64)
+        check(L.shm_context_coeff_modulus(self.h, _u64p(self.coeff_modulus), self.coeff_modulus.size))
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib().shm_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def ql(self, chain_index: int) -> int:
+        return self.size_q - chain_index + 1
+
+    def chain(self, ql: int) -> int:
+        return self.size_q - ql + 1
+
+    def galois_elt_from_step(self, step: int) -> int:
+        e = C.c_uint32()
+        check(_lib().shm_galois_elt_from_step(self.h, step, C.byref(e)))
+        return e.value
+
+    # raw limb transforms on device memory (e.g. a torch tensor's data_ptr())
+    def ntt_forward(self, dev_ptr: int, n_polys: int, n_limbs: int, first_prime: int = 0, stream=None):
+        check(_lib().shm_ntt_forward(self.h, C.c_void_p(dev_ptr), n_polys, n_limbs, first_prime, stream))
+
+    def ntt_inverse(self, dev_ptr: int, n_polys: int, n_limbs: int, first_prime: int = 0, stream=None):
+        check(_lib().shm_ntt_inverse(self.h, C.c_void_p(dev_ptr), n_polys, n_limbs, first_prime, stream))
+
+
+class Ciphertext:
+    """PhantomCiphertext (ciphertext.h:21-288): [size][Ql][N] uint64 in NTT form."""
+
+    def __init__(self, ctx: Context, size: int = 2, chain_index: int = 1):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(_lib().shm_ciphertext_create(ctx.h, size, chain_index, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                _lib().shm_ciphertext_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def info(self):
+        s, ch, q = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        sc = C.c_double()
+        check(_lib().shm_ciphertext_info(self.h, C.byref(s), C.byref(ch), C.byref(q), C.byref(sc)))
+        return s.value, ch.value, q.value, sc.value
+
+    @property
+    def size(self):
+        return self.info()[0]
+
+    @property
+    def chain_index(self):
+        return self.info()[1]
+
+    @property
+    def coeff_modulus_size(self):
+        return self.info()[2]
+
+    @property
+    def scale(self):
+        return self.info()[3]
+
+    @scale.setter
+    def scale(self, v: float):
+        check(_lib().shm_ciphertext_set_scale(self.h, float(v)))
+
+    def data_ptr(self) -> int:
+        return _lib().shm_ciphertext_data(self.h)
+
+    def words(self) -> int:
+        s, _, q, _ = self.info()
+        return s * q * self.ctx.N
+
+    def load(self, limbs: np.ndarray, stream=None) -> "Ciphertext":
+        a = np.ascontiguousarray(limbs, np.uint64).ravel()
+        check(_lib().shm_ciphertext_import(self.h, _u64p(a), a.size, stream))
+        return self
+
+    def numpy(self, stream=None) -> np.ndarray:
+        out = np.zeros(self.words(), np.uint64)
+        check(_lib().shm_ciphertext_export(self.h, _u64p(out), out.size, stream))
+        return out
+
+    def copy(self, stream=None) -> "Ciphertext":
+        s, ch, _, _ = self.info()
+        out = Ciphertext(self.ctx, s, ch)
+        check(_lib().shm_ciphertext_copy(self.h, out.h, stream))
+        return out
+
+
+class Plaintext:
+    """PhantomPlaintext: [Ql][N] uint64 in NTT form."""
+
+    def __init__(self, ctx: Context, chain_index: int = 1):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(_lib().shm_plaintext_create(ctx.h, chain_index, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                _lib().shm_plaintext_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def info(self):
+        ch, q = C.c_uint32(), C.c_uint32()
+        sc = C.c_double()
+        check(_lib().shm_plaintext_info(self.h, C.byref(ch), C.byref(q), C.byref(sc)))
+        return ch.value, q.value, sc.value
+
+    @property
+    def chain_index(self):
+        return self.info()[0]
+
+    @property
+    def scale(self):
+        return self.info()[2]
+
+    @scale.setter
+    def scale(self, v: float):
+        check(_lib().shm_plaintext_set_scale(self.h, float(v)))
+
+    def load(self, limbs: np.ndarray, stream=None) -> "Plaintext":
+        a = np.ascontiguousarray(limbs, np.uint64).ravel()
+        check(_lib().shm_plaintext_import(self.h, _u64p(a), a.size, stream))
+        return self
+
+    def numpy(self, stream=None) -> np.ndarray:
+        _, q, _ = self.info()
+        out = np.zeros(q * self.ctx.N, np.uint64)
+        check(_lib().shm_plaintext_export(self.h, _u64p(out), out.size, stream))
+        return out
+
+
+class KSwitchKey:
+    """PhantomRelinKey / one Galois key: dnum x [2][QP][N]."""
+
+    def __init__(self, ctx: Context, h):
+        self.ctx, self.h = ctx, h
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                _lib().shm_kswitch_key_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def numpy(self) -> np.ndarray:
+        c = self.ctx
+        out = np.zeros(c.dnum * 2 * (c.size_q + c.size_p) * c.N, np.uint64)
+        check(_lib().shm_kswitch_key_export(self.h, _u64p(out), out.size))
+        return out
+
+
+class GaloisKeys:
+    """PhantomGaloisKey for a set of rotation steps."""
+
+    def __init__(self, ctx: Context, h, steps):
+        self.ctx, self.h, self.steps = ctx, h, list(steps)
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                _lib().shm_galois_keys_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def numpy(self, elt: int) -> np.ndarray:
+        c = self.ctx
+        out = np.zeros(c.dnum * 2 * (c.size_q + c.size_p) * c.N, np.uint64)
+        check(_lib().shm_galois_keys_export(self.h, elt, _u64p(out), out.size))
+        return out
+
+
+class SecretKey:
+    """PhantomSecretKey (secretkey.cu:431-646) with injected, seeded randomness."""
+
+    def __init__(self, ctx: Context, coeffs: np.ndarray | None = None, seed: int = 0, hamming_weight: int = 64):
+        self.ctx = ctx
+        h = C.c_void_p()
+        if coeffs is not None:
+            a = np.ascontiguousarray(coeffs, np.int8)
+            check(_lib().shm_secret_key_create(ctx.h, a.ctypes.data_as(C.POINTER(C.c_int8)), C.byref(h)))
+        else:
+            check(_lib().shm_secret_key_generate(ctx.h, seed, hamming_weight, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                _lib().shm_secret_key_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def numpy(self) -> np.ndarray:
+        c = self.ctx
+        out = np.zeros((c.size_q + c.size_p) * c.N, np.uint64)
+        check(_lib().shm_secret_key_export(self.h, _u64p(out), out.size))
+        return out
+
+    def gen_relinkey(self, seed: int) -> KSwitchKey:
+        h = C.c_void_p()
+        check(_lib().shm_relin_key_generate(self.ctx.h, self.h, seed, C.byref(h)))
+        return KSwitchKey(self.ctx, h)
+
+    def gen_kswitch_key(self, new_key_qp: np.ndarray, seed: int) -> KSwitchKey:
+        a = np.ascontiguousarray(new_key_qp, np.uint64)
+        h = C.c_void_p()
+        check(_lib().shm_kswitch_key_generate(self.ctx.h, self.h, _u64p(a), seed, C.byref(h)))
+        return KSwitchKey(self.ctx, h)
+
+    def create_galois_keys_from_steps(self, steps: Sequence[int], seed: int) -> GaloisKeys:
+        arr = (C.c_int * len(steps))(*steps)
+        h = C.c_void_p()
+        check(_lib().shm_galois_keys_generate(self.ctx.h, self.h, arr, len(steps), seed, C.byref(h)))
+        return GaloisKeys(self.ctx, h, steps)
+
+    def encrypt_symmetric(self, pt: Plaintext, seed: int, stream=None) -> Ciphertext:
+        ct = Ciphertext(self.ctx, 2, pt.chain_index)
+        check(_lib().shm_encrypt_symmetric(self.ctx.h, self.h, pt.h, seed, ct.h, stream))
+        return ct
+
+
+class Decryptor:
+    """moai::Decryptor (ckks_evaluator_parallel.cuh:564-598)."""
+
+    def __init__(self, ctx: Context, sk: SecretKey):
+        self.ctx, self.sk = ctx, sk
+
+    def decrypt(self, ct: Ciphertext, stream=None) -> Plaintext:
+        pt = Plaintext(self.ctx, ct.chain_index)
+        check(_lib().shm_decrypt(self.ctx.h, self.sk.h, ct.h, pt.h, stream))
+        return pt
+
+
+class Encoder:
+    """moai::Encoder over PhantomCKKSEncoder (ckks.cu:89-246)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+
+    def slot_count(self) -> int:
+        return self.ctx.N // 2
+
+    def encode(self, values, scale: float, chain_index: int = 1, stream=None) -> Plaintext:
+        pt = Plaintext(self.ctx, chain_index)
+        if np.isscalar(values):
+            check(_lib().shm_encode_scalar(self.ctx.h, float(values), chain_index, float(scale), pt.h, stream))
+            return pt
+        v = np.asarray(values)
+        re = np.ascontiguousarray(v.real, np.float64)
+        im = np.ascontiguousarray(v.imag, np.float64) if np.iscomplexobj(v) else None
+        check(_lib().shm_encode(self.ctx.h, re.ctypes.data_as(C.POINTER(C.c_double)),
+                                im.ctypes.data_as(C.POINTER(C.c_double)) if im is not None else None,
+                                re.size, chain_index, float(scale), pt.h, stream))
+        return pt
+
+    def decode(self, pt: Plaintext, stream=None, complex_out: bool = False):
+        n = self.ctx.N // 2
+        re = np.zeros(n, np.float64)
+        im = np.zeros(n, np.float64)
+        check(_lib().shm_decode(self.ctx.h, pt.h, re.ctypes.data_as(C.POINTER(C.c_double)),
+                                im.ctypes.data_as(C.POINTER(C.c_double)), n, stream))
+        return re + 1j * im if complex_out else re
+
+
+class Evaluator:
+    """Phantom evaluator free functions (evaluate.cuh:32-304) as methods."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.L = _lib()
+
+    def add_inplace(self, a, b, stream=None):
+        check(self.L.shm_add_inplace(self.ctx.h, a.h, b.h, stream))
+
+    def sub_inplace(self, a, b, stream=None):
+        check(self.L.shm_sub_inplace(self.ctx.h, a.h, b.h, stream))
+
+    def negate_inplace(self, a, stream=None):
+        check(self.L.shm_negate_inplace(self.ctx.h, a.h, stream))
+
+    def add_plain_inplace(self, a, pt, stream=None):
+        check(self.L.shm_add_plain_inplace(self.ctx.h, a.h, pt.h, stream))
+
+    def sub_plain_inplace(self, a, pt, stream=None):
+        check(self.L.shm_sub_plain_inplace(self.ctx.h, a.h, pt.h, stream))
+
+    def multiply_plain_inplace(self, ct, pt, stream=None):
+        check(self.L.shm_multiply_plain_inplace(self.ctx.h, ct.h, pt.h, stream))
+
+    def multiply_inplace(self, a, b, stream=None):
+        check(self.L.shm_multiply_inplace(self.ctx.h, a.h, b.h, stream))
+
+    def square_inplace(self, a, stream=None):
+        check(self.L.shm_square_inplace(self.ctx.h, a.h, stream))
+
+    def relinearize_inplace(self, ct, relin_key: KSwitchKey, stream=None):
+        check(self.L.shm_relinearize_inplace(self.ctx.h, ct.h, relin_key.h, stream))
+
+    def apply_kswitch_key_inplace(self, ct, key: KSwitchKey, stream=None):
+        check(self.L.shm_apply_kswitch_key_inplace(self.ctx.h, ct.h, key.h, stream))
+
+    def rescale_to_next_inplace(self, ct, stream=None):
+        check(self.L.shm_rescale_to_next_inplace(self.ctx.h, ct.h, stream))
+
+    def mod_switch_to_next_inplace(self, ct, stream=None):
+        check(self.L.shm_mod_switch_to_next_inplace(self.ctx.h, ct.h, stream))
+
+    def mod_switch_to_inplace(self, ct, chain_index: int, stream=None):
+        check(self.L.shm_mod_switch_to_inplace(self.ctx.h, ct.h, chain_index, stream))
+
+    def plain_mod_switch_to_inplace(self, pt, chain_index: int, stream=None):
+        check(self.L.shm_plain_mod_switch_to_inplace(self.ctx.h, pt.h, chain_index, stream))
+
+    def apply_galois_inplace(self, ct, elt: int, gk: GaloisKeys, stream=None):
+        check(self.L.shm_apply_galois_inplace(self.ctx.h, ct.h, elt, gk.h, stream))
+
+    def rotate_vector_inplace(self, ct, step: int, gk: GaloisKeys, stream=None):
+        check(self.L.shm_rotate_vector_inplace(self.ctx.h, ct.h, step, gk.h, stream))
+
+    def complex_conjugate_inplace(self, ct, gk: GaloisKeys, stream=None):
+        check(self.L.shm_complex_conjugate_inplace(self.ctx.h, ct.h, gk.h, stream))
+
+
+def ct_pt_matrix_mul_wo_pre(enc_X: Sequence[Ciphertext], W: np.ndarray, col_X: int, col_W: int, row_W: int,
+                            context: Context, variant: int = MATMUL_INT8, out=None, stream=None):
+    """Ct_pt_matrix_mul.cuh:18-185: out[i] = rescale(sum_j X[j] (.) encode(W[j][i])).
+
+    W is indexed W[j][i] (row_W x col_W). Returns col_W ciphertexts one level
+    down with the scale of enc_X[0]. Raises ShmInvalidArgument on the
+    reference's dimension errors.
+    """
+    if col_X != row_W:
+        raise capi.ShmInvalidArgument(capi.SHM_ERR_INVALID_ARGUMENT, "bad dimensions of X or W")
+    Wm = np.ascontiguousarray(np.asarray(W, np.float64)[:row_W, :col_W])
+    if len(enc_X) < row_W:
+        raise capi.ShmInvalidArgument(capi.SHM_ERR_INVALID_ARGUMENT, "enc_X size < row_W")
+    chain = enc_X[0].chain_index
+    if out is None:
+        out = [Ciphertext(context, 2, chain + 1) for _ in range(col_W)]
+    xs = (C.c_void_p * row_W)(*[x.h.value for x in enc_X[:row_W]])
+    os_ = (C.c_void_p * col_W)(*[o.h.value for o in out])
+    check(_lib().shm_ct_pt_matrix_mul(context.h, xs, row_W, Wm.ctypes.data_as(C.POINTER(C.c_double)), col_W, os_,
+                                      variant, stream))
+    return out
+
+
+def last_matmul_kernel_ms() -> float:
+    return float(_lib().shm_last_matmul_kernel_ms())

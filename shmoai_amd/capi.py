@@ -1,0 +1,149 @@
+"""ctypes binding of libshmoai.so (the C ABI declared in include/shmoai.h).
+
+The library is built in-tree by shmoai_amd/build.py. Loading fails loudly when
+it is missing: there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libshmoai.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "shmoai.h")
+
+SHM_OK = 0
+SHM_ERR_INVALID_ARGUMENT = -1
+SHM_ERR_LOGIC = -2
+SHM_ERR_RUNTIME = -3
+SHM_ERR_OUT_OF_MEMORY = -4
+SHM_ERR_DEVICE = -5
+
+MATMUL_U64 = 0
+MATMUL_PER_CT = 1
+MATMUL_INT8 = 2
+
+vp = C.c_void_p
+u32 = C.c_uint32
+u64 = C.c_uint64
+sz = C.c_size_t
+i32 = C.c_int
+dbl = C.c_double
+pu64 = C.POINTER(C.c_uint64)
+pvp = C.POINTER(C.c_void_p)
+
+
+class ContextInfo(C.Structure):
+    _fields_ = [("poly_degree", u32), ("size_q", u32), ("size_p", u32), ("dnum", u32)]
+
+
+_SIGS = {
+    "shm_last_error": (C.c_char_p, []),
+    "shm_version": (C.c_char_p, []),
+    "shm_context_create": (i32, [u32, C.POINTER(C.c_int), u32, u32, pvp]),
+    "shm_context_destroy": (i32, [vp]),
+    "shm_context_get_info": (i32, [vp, C.POINTER(ContextInfo)]),
+    "shm_context_coeff_modulus": (i32, [vp, pu64, sz]),
+    "shm_galois_elt_from_step": (i32, [vp, i32, C.POINTER(u32)]),
+    "shm_ntt_forward": (i32, [vp, vp, u32, u32, u32, vp]),
+    "shm_ntt_inverse": (i32, [vp, vp, u32, u32, u32, vp]),
+    "shm_ciphertext_create": (i32, [vp, u32, u32, pvp]),
+    "shm_ciphertext_destroy": (i32, [vp]),
+    "shm_ciphertext_info": (i32, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(dbl)]),
+    "shm_ciphertext_set_scale": (i32, [vp, dbl]),
+    "shm_ciphertext_data": (vp, [vp]),
+    "shm_ciphertext_import": (i32, [vp, pu64, sz, vp]),
+    "shm_ciphertext_export": (i32, [vp, pu64, sz, vp]),
+    "shm_ciphertext_copy": (i32, [vp, vp, vp]),
+    "shm_plaintext_create": (i32, [vp, u32, pvp]),
+    "shm_plaintext_destroy": (i32, [vp]),
+    "shm_plaintext_info": (i32, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(dbl)]),
+    "shm_plaintext_set_scale": (i32, [vp, dbl]),
+    "shm_plaintext_data": (vp, [vp]),
+    "shm_plaintext_import": (i32, [vp, pu64, sz, vp]),
+    "shm_plaintext_export": (i32, [vp, pu64, sz, vp]),
+    "shm_secret_key_create": (i32, [vp, C.POINTER(C.c_int8), pvp]),
+    "shm_secret_key_generate": (i32, [vp, u64, u32, pvp]),
+    "shm_secret_key_destroy": (i32, [vp]),
+    "shm_secret_key_export": (i32, [vp, pu64, sz]),
+    "shm_relin_key_generate": (i32, [vp, vp, u64, pvp]),
+    "shm_kswitch_key_generate": (i32, [vp, vp, pu64, u64, pvp]),
+    "shm_kswitch_key_destroy": (i32, [vp]),
+    "shm_kswitch_key_export": (i32, [vp, pu64, sz]),
+    "shm_galois_keys_generate": (i32, [vp, vp, C.POINTER(C.c_int), sz, u64, pvp]),
+    "shm_galois_keys_destroy": (i32, [vp]),
+    "shm_galois_keys_export": (i32, [vp, u32, pu64, sz]),
+    "shm_encrypt_symmetric": (i32, [vp, vp, vp, u64, vp, vp]),
+    "shm_decrypt": (i32, [vp, vp, vp, vp, vp]),
+    "shm_encode": (i32, [vp, C.POINTER(dbl), C.POINTER(dbl), sz, u32, dbl, vp, vp]),
+    "shm_encode_scalar": (i32, [vp, dbl, u32, dbl, vp, vp]),
+    "shm_decode": (i32, [vp, vp, C.POINTER(dbl), C.POINTER(dbl), sz, vp]),
+    "shm_add_inplace": (i32, [vp, vp, vp, vp]),
+    "shm_sub_inplace": (i32, [vp, vp, vp, vp]),
+    "shm_negate_inplace": (i32, [vp, vp, vp]),
+    "shm_add_plain_inplace": (i32, [vp, vp, vp, vp]),
+    "shm_sub_plain_inplace": (i32, [vp, vp, vp, vp]),
+    "shm_multiply_plain_inplace": (i32, [vp, vp, vp, vp]),
+    "shm_multiply_inplace": (i32, [vp, vp, vp, vp]),
+    "shm_square_inplace": (i32, [vp, vp, vp]),
+    "shm_relinearize_inplace": (i32, [vp, vp, vp, vp]),
+    "shm_apply_kswitch_key_inplace": (i32, [vp, vp, vp, vp]),
+    "shm_rescale_to_next_inplace": (i32, [vp, vp, vp]),
+    "shm_mod_switch_to_next_inplace": (i32, [vp, vp, vp]),
+    "shm_mod_switch_to_inplace": (i32, [vp, vp, u32, vp]),
+    "shm_plain_mod_switch_to_inplace": (i32, [vp, vp, u32, vp]),
+    "shm_apply_galois_inplace": (i32, [vp, vp, u32, vp, vp]),
+    "shm_rotate_vector_inplace": (i32, [vp, vp, i32, vp, vp]),
+    "shm_complex_conjugate_inplace": (i32, [vp, vp, vp, vp]),
+    "shm_ct_pt_matrix_mul": (i32, [vp, pvp, u32, C.POINTER(dbl), u32, pvp, i32, vp]),
+    "shm_last_matmul_kernel_ms": (dbl, []),
+}
+
+_lib = None
+
+
+class ShmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class ShmInvalidArgument(ShmError, ValueError):
+    pass
+
+
+class ShmLogicError(ShmError):
+    pass
+
+
+_ERRS = {SHM_ERR_INVALID_ARGUMENT: ShmInvalidArgument, SHM_ERR_LOGIC: ShmLogicError}
+
+
+def header_functions() -> list[str]:
+    """Every function name declared in include/shmoai.h."""
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(shm_[a-z0-9_]+)\s*\(", txt)))
+
+
+def load(path: str = LIB_PATH):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with `python -m shmoai_amd.build` "
+                          "(there is no fallback implementation)")
+    L = C.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != SHM_OK:
+        msg = load().shm_last_error().decode(errors="replace")
+        raise _ERRS.get(rc, ShmError)(rc, msg)

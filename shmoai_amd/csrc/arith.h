@@ -1,0 +1,82 @@
+// Exact 64-bit modular arithmetic for CDNA4 (gfx950) kernels.
+//
+// CDNA4 has no 64x64->128 multiply instruction; __umul64hi and 64-bit
+// products lower to 32-bit v_mul_hi/v_mad_u64_u32 chains. Every helper keeps
+// operands below 2^62 so that lazy [0, 4q) butterflies never wrap (all primes
+// are at most 61 bits).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace shm {
+
+typedef unsigned __int128 u128;
+
+// One RNS prime with its Barrett constant floor(2^128 / q) = (mu_hi, mu_lo).
+struct DevMod {
+    uint64_t q;
+    uint64_t mu_hi;
+    uint64_t mu_lo;
+};
+
+__device__ __forceinline__ uint64_t csub(uint64_t x, uint64_t q) { return x >= q ? x - q : x; }
+
+__device__ __forceinline__ uint64_t add_mod(uint64_t a, uint64_t b, uint64_t q) { return csub(a + b, q); }
+
+__device__ __forceinline__ uint64_t sub_mod(uint64_t a, uint64_t b, uint64_t q) { return csub(a + q - b, q); }
+
+__device__ __forceinline__ uint64_t neg_mod(uint64_t a, uint64_t q) { return a == 0 ? 0 : q - a; }
+
+// Shoup product x * w mod q with ws = floor(w * 2^64 / q): result in [0, 2q)
+// for any x < 2^64.
+__device__ __forceinline__ uint64_t mul_shoup_lazy(uint64_t x, uint64_t w, uint64_t ws, uint64_t q) {
+    return x * w - __umul64hi(x, ws) * q;
+}
+
+__device__ __forceinline__ uint64_t mul_shoup(uint64_t x, uint64_t w, uint64_t ws, uint64_t q) {
+    return csub(mul_shoup_lazy(x, w, ws, q), q);
+}
+
+// Barrett reduction of a 128-bit value (hi, lo) with mu = floor(2^128/q).
+// The quotient estimate is at most 2 short, so the remainder is in [0, 3q)
+// before the final corrections.
+__device__ __forceinline__ uint64_t barrett128(uint64_t hi, uint64_t lo, const DevMod &m) {
+    // floor((hi*2^64 + lo) * mu / 2^128), dropping the lo*mu_lo low part.
+    const uint64_t a = __umul64hi(lo, m.mu_lo);
+    const uint64_t b_lo = lo * m.mu_hi, b_hi = __umul64hi(lo, m.mu_hi);
+    const uint64_t c_lo = hi * m.mu_lo, c_hi = __umul64hi(hi, m.mu_lo);
+    uint64_t s = a + b_lo;
+    uint64_t carry = s < a;
+    const uint64_t s2 = s + c_lo;
+    carry += s2 < s;
+    const uint64_t qest = hi * m.mu_hi + b_hi + c_hi + carry;
+    uint64_t r = lo - qest * m.q;
+    r = csub(r, m.q);
+    return csub(r, m.q);
+}
+
+__device__ __forceinline__ uint64_t mul_mod(uint64_t a, uint64_t b, const DevMod &m) {
+    return barrett128(__umul64hi(a, b), a * b, m);
+}
+
+// Reduce a single 64-bit word (any value) mod q using mu_hi = floor(2^64/q).
+__device__ __forceinline__ uint64_t reduce64(uint64_t x, const DevMod &m) {
+    const uint64_t qe = __umul64hi(x, m.mu_hi);
+    return csub(x - qe * m.q, m.q);
+}
+
+// 128-bit accumulator helpers (explicit carries; no __int128 division).
+struct Acc128 {
+    uint64_t lo, hi;
+};
+
+__device__ __forceinline__ void mac128(Acc128 &acc, uint64_t a, uint64_t b) {
+    const uint64_t lo = a * b, hi = __umul64hi(a, b);
+    acc.lo += lo;
+    acc.hi += hi + (acc.lo < lo);
+}
+
+__device__ __forceinline__ uint64_t reduce128(const Acc128 &acc, const DevMod &m) { return barrett128(acc.hi, acc.lo, m); }
+
+}  // namespace shm

@@ -1,0 +1,854 @@
+// C ABI (include/shmoai.h): thin, exception-safe wrappers over the engine.
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/shmoai.h"
+#include "engine.h"
+
+using namespace shm;
+
+struct shm_context {
+    std::unique_ptr<Context> c;
+};
+struct shm_ciphertext {
+    Ciphertext ct;
+};
+struct shm_plaintext {
+    Plaintext pt;
+};
+struct shm_secret_key {
+    SecretKey sk;
+};
+struct shm_kswitch_key {
+    KSwitchKey k;
+};
+struct shm_galois_keys {
+    GaloisKeys g;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guard(F &&f) {
+    try {
+        f();
+        return SHM_OK;
+    } catch (const InvalidArgument &e) {
+        g_err = e.what();
+        return SHM_ERR_INVALID_ARGUMENT;
+    } catch (const LogicError &e) {
+        g_err = e.what();
+        return SHM_ERR_LOGIC;
+    } catch (const DeviceError &e) {
+        g_err = e.what();
+        return SHM_ERR_DEVICE;
+    } catch (const std::bad_alloc &) {
+        g_err = "out of device or host memory";
+        return SHM_ERR_OUT_OF_MEMORY;
+    } catch (const std::invalid_argument &e) {
+        g_err = e.what();
+        return SHM_ERR_INVALID_ARGUMENT;
+    } catch (const std::logic_error &e) {
+        g_err = e.what();
+        return SHM_ERR_LOGIC;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return SHM_ERR_RUNTIME;
+    } catch (...) {
+        g_err = "unknown error";
+        return SHM_ERR_RUNTIME;
+    }
+}
+
+inline hipStream_t S(shm_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+void need(const void *p, const char *what) {
+    if (!p) throw InvalidArgument(std::string(what) + " is null");
+}
+
+uint64_t splitmix64(uint64_t key, uint64_t ctr) {
+    uint64_t z = key + (ctr + 1) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+void check_chain(const Context &c, uint32_t chain) {
+    if (chain < 1 || chain > c.nQ) throw InvalidArgument("chain_index out of range");
+}
+
+void same_level(const Ciphertext &a, const Ciphertext &b) {
+    if (a.chain != b.chain) throw InvalidArgument("encrypted1 and encrypted2 parameter mismatch");
+}
+
+// NTT-form secret over QP from signed coefficients (host int8 -> device).
+void load_secret(Context &c, SecretKey &sk, const int8_t *coeffs) {
+    const size_t N = c.N, QP = c.nQP();
+    Scratch dc(N, nullptr);
+    SHM_HIP(hipMemcpy(dc.as<void>(), coeffs, N, hipMemcpyHostToDevice));
+    sk.s.alloc(QP * N * sizeof(uint64_t));
+    launch_signed_to_rns(c, dc.as<int8_t>(), sk.s.as<uint64_t>(), c.nQ, true, nullptr);
+    ntt_forward(c, sk.s.as<uint64_t>(), 1, QP, 0, nullptr);
+    SHM_HIP(hipDeviceSynchronize());
+}
+
+// secretkey.cu:328-373: dnum digits of (-(a s + e) + P [newkey]_{digit}, a).
+void gen_kswitch(Context &c, const SecretKey &sk, const uint64_t *newkey_dev, uint64_t seed, KSwitchKey &out) {
+    const size_t N = c.N, QP = c.nQP(), dnum = c.dnum;
+    out.ctx = &c;
+    out.k.alloc(dnum * 2 * QP * N * sizeof(uint64_t));
+    Scratch e(QP * N * sizeof(uint64_t), nullptr);
+    for (size_t g = 0; g < dnum; g++) {
+        uint64_t *b = out.k.as<uint64_t>() + g * 2 * QP * N, *a = b + QP * N;
+        launch_sample_uniform(c, splitmix64(seed, 2 * g), a, c.nQ, true, nullptr);
+        launch_sample_gaussian_rns(c, splitmix64(seed, 2 * g + 1), e.as<uint64_t>(), c.nQ, true, nullptr);
+        ntt_forward(c, e.as<uint64_t>(), 1, QP, 0, nullptr);
+        launch_ks_key_digit(c, sk.s.as<uint64_t>(), newkey_dev, b, a, e.as<uint64_t>(), g, nullptr);
+    }
+    SHM_HIP(hipDeviceSynchronize());
+}
+
+void apply_galois(Context &c, Ciphertext &ct, uint32_t elt, const GaloisKeys &gk, hipStream_t s) {
+    if (ct.size != 2) throw InvalidArgument("encrypted size must be 2");
+    auto it = gk.keys.find(elt);
+    if (it == gk.keys.end()) throw InvalidArgument("Galois elt not present");
+    const size_t Ql = ct.Ql(), N = c.N;
+    const uint32_t *table = c.galois_table(elt, s);
+    Scratch t0(Ql * N * sizeof(uint64_t), s), t1(Ql * N * sizeof(uint64_t), s);
+    launch_permute(c, ct.poly(0), t0.as<uint64_t>(), table, Ql, s);
+    launch_permute(c, ct.poly(1), t1.as<uint64_t>(), table, Ql, s);
+    SHM_HIP(hipMemcpyAsync(ct.poly(0), t0.as<uint64_t>(), Ql * N * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    keyswitch(c, t1.as<uint64_t>(), it->second->k.as<uint64_t>(), Ql, ct.poly(0), ct.poly(1), false, s);
+}
+
+// evaluate.cu:1650-1706 rotate_internal (NAF split, numth.h:17-34).
+void rotate_internal(Context &c, Ciphertext &ct, int step, const GaloisKeys &gk, hipStream_t s) {
+    if (step == 0) return;
+    const uint32_t elt = c.elt_from_step(step);
+    if (gk.keys.count(elt)) {
+        apply_galois(c, ct, elt, gk, s);
+        return;
+    }
+    std::vector<int> naf;
+    {
+        const bool sign = step < 0;
+        int v = std::abs(step);
+        for (int i = 0; v; i++) {
+            const int zi = (v & 1) ? 2 - (v & 3) : 0;
+            v = (v - zi) >> 1;
+            if (zi) naf.push_back((sign ? -zi : zi) * (1 << i));
+        }
+    }
+    if (naf.size() == 1) throw InvalidArgument("Galois key not present");
+    for (int t : naf)
+        if ((size_t)std::abs(t) != (c.N >> 1)) rotate_internal(c, ct, t, gk, s);
+}
+
+void rescale_inplace(Context &c, Ciphertext &ct, hipStream_t s) {
+    const size_t Ql = ct.Ql();
+    if (Ql < 2) throw LogicError("end of modulus switching chain reached");
+    if (!ct.ntt) throw InvalidArgument("CKKS encrypted must be in NTT form");
+    launch_rescale(c, ct.data(), ct.data(), Ql, ct.size, s);
+    ct.chain += 1;
+    ct.scale = ct.scale / (double)c.q[Ql - 1];
+}
+
+void mod_switch_drop(Context &c, Ciphertext &ct, size_t new_chain, hipStream_t s) {
+    const size_t Ql = ct.Ql(), nl = c.ql_of_chain(new_chain);
+    if (new_chain < ct.chain) throw InvalidArgument("cannot switch to a higher level");
+    if (new_chain == ct.chain) return;
+    // compact [size][Ql][N] -> [size][nl][N] through a staging copy (ranges overlap)
+    if (ct.size > 1) {
+        Scratch tmp(ct.size * nl * c.N * sizeof(uint64_t), s);
+        launch_copy_limbs(c, ct.data(), Ql, tmp.as<uint64_t>(), nl, ct.size, s);
+        SHM_HIP(hipMemcpyAsync(ct.data(), tmp.as<uint64_t>(), ct.size * nl * c.N * sizeof(uint64_t),
+                               hipMemcpyDeviceToDevice, s));
+    }
+    ct.chain = new_chain;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *shm_last_error(void) { return g_err.c_str(); }
+const char *shm_version(void) { return "shmoai-mi355x 0.1 (gfx950)"; }
+
+int shm_context_create(uint32_t logN, const int *bits, uint32_t n, uint32_t nP, shm_context **out) {
+    return guard([&] {
+        need(bits, "coeff_bit_sizes");
+        need(out, "out");
+        std::vector<int> b(bits, bits + n);
+        auto h = new shm_context();
+        try {
+            h->c = make_context((int)logN, b, nP);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int shm_context_destroy(shm_context *ctx) {
+    return guard([&] {
+        if (ctx) {
+            (void)hipDeviceSynchronize();
+            delete ctx;
+        }
+    });
+}
+
+int shm_context_get_info(const shm_context *ctx, shm_context_info *info) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(info, "info");
+        info->poly_degree = (uint32_t)ctx->c->N;
+        info->size_q = (uint32_t)ctx->c->nQ;
+        info->size_p = (uint32_t)ctx->c->nP;
+        info->dnum = (uint32_t)ctx->c->dnum;
+    });
+}
+
+int shm_context_coeff_modulus(const shm_context *ctx, uint64_t *out, size_t n) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(out, "out");
+        if (n < ctx->c->q.size()) throw InvalidArgument("output too small");
+        std::memcpy(out, ctx->c->q.data(), ctx->c->q.size() * sizeof(uint64_t));
+    });
+}
+
+int shm_galois_elt_from_step(const shm_context *ctx, int step, uint32_t *elt) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(elt, "elt");
+        *elt = ctx->c->elt_from_step(step);
+    });
+}
+
+int shm_ntt_forward(shm_context *ctx, uint64_t *data, uint32_t n_polys, uint32_t n_limbs, uint32_t first,
+                    shm_stream_t s) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(data, "data");
+        if (first + n_limbs > ctx->c->nQP()) throw InvalidArgument("prime range out of bounds");
+        ntt_forward(*ctx->c, data, n_polys, n_limbs, first, S(s));
+    });
+}
+
+int shm_ntt_inverse(shm_context *ctx, uint64_t *data, uint32_t n_polys, uint32_t n_limbs, uint32_t first,
+                    shm_stream_t s) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(data, "data");
+        if (first + n_limbs > ctx->c->nQP()) throw InvalidArgument("prime range out of bounds");
+        ntt_inverse(*ctx->c, data, n_polys, n_limbs, first, S(s));
+    });
+}
+
+// ---- ciphertexts / plaintexts ------------------------------------------------
+int shm_ciphertext_create(shm_context *ctx, uint32_t size, uint32_t chain, shm_ciphertext **out) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(out, "out");
+        check_chain(*ctx->c, chain);
+        if (size < 2 || size > 3) throw InvalidArgument("ciphertext size must be 2 or 3");
+        auto h = new shm_ciphertext();
+        h->ct.ctx = ctx->c.get();
+        try {
+            h->ct.resize(size, chain);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int shm_ciphertext_destroy(shm_ciphertext *ct) {
+    return guard([&] { delete ct; });
+}
+
+int shm_ciphertext_info(const shm_ciphertext *ct, uint32_t *size, uint32_t *chain, uint32_t *cms, double *scale) {
+    return guard([&] {
+        need(ct, "ct");
+        if (size) *size = (uint32_t)ct->ct.size;
+        if (chain) *chain = (uint32_t)ct->ct.chain;
+        if (cms) *cms = (uint32_t)ct->ct.Ql();
+        if (scale) *scale = ct->ct.scale;
+    });
+}
+
+int shm_ciphertext_set_scale(shm_ciphertext *ct, double scale) {
+    return guard([&] {
+        need(ct, "ct");
+        ct->ct.scale = scale;
+    });
+}
+
+uint64_t *shm_ciphertext_data(shm_ciphertext *ct) { return ct ? ct->ct.data() : nullptr; }
+
+int shm_ciphertext_import(shm_ciphertext *ct, const uint64_t *host, size_t n, shm_stream_t s) {
+    return guard([&] {
+        need(ct, "ct");
+        need(host, "host");
+        const size_t words = ct->ct.size * ct->ct.Ql() * ct->ct.ctx->N;
+        if (n != words) throw InvalidArgument("import size mismatch");
+        SHM_HIP(hipMemcpyAsync(ct->ct.data(), host, words * 8, hipMemcpyHostToDevice, S(s)));
+        SHM_HIP(hipStreamSynchronize(S(s)));
+    });
+}
+
+int shm_ciphertext_export(const shm_ciphertext *ct, uint64_t *host, size_t n, shm_stream_t s) {
+    return guard([&] {
+        need(ct, "ct");
+        need(host, "host");
+        const size_t words = ct->ct.size * ct->ct.Ql() * ct->ct.ctx->N;
+        if (n != words) throw InvalidArgument("export size mismatch");
+        SHM_HIP(hipMemcpyAsync(host, ct->ct.data(), words * 8, hipMemcpyDeviceToHost, S(s)));
+        SHM_HIP(hipStreamSynchronize(S(s)));
+    });
+}
+
+int shm_ciphertext_copy(const shm_ciphertext *src, shm_ciphertext *dst, shm_stream_t s) {
+    return guard([&] {
+        need(src, "src");
+        need(dst, "dst");
+        dst->ct.ctx = src->ct.ctx;
+        dst->ct.resize(src->ct.size, src->ct.chain);
+        dst->ct.scale = src->ct.scale;
+        dst->ct.ntt = src->ct.ntt;
+        SHM_HIP(hipMemcpyAsync(dst->ct.data(), src->ct.data(), src->ct.size * src->ct.Ql() * src->ct.ctx->N * 8,
+                               hipMemcpyDeviceToDevice, S(s)));
+    });
+}
+
+int shm_plaintext_create(shm_context *ctx, uint32_t chain, shm_plaintext **out) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(out, "out");
+        check_chain(*ctx->c, chain);
+        auto h = new shm_plaintext();
+        h->pt.ctx = ctx->c.get();
+        h->pt.chain = chain;
+        try {
+            h->pt.buf.alloc(ctx->c->nQ * ctx->c->N * sizeof(uint64_t));
+            SHM_HIP(hipMemset(h->pt.buf.as<void>(), 0, ctx->c->nQ * ctx->c->N * sizeof(uint64_t)));
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int shm_plaintext_destroy(shm_plaintext *pt) {
+    return guard([&] { delete pt; });
+}
+
+int shm_plaintext_info(const shm_plaintext *pt, uint32_t *chain, uint32_t *cms, double *scale) {
+    return guard([&] {
+        need(pt, "pt");
+        if (chain) *chain = (uint32_t)pt->pt.chain;
+        if (cms) *cms = (uint32_t)pt->pt.Ql();
+        if (scale) *scale = pt->pt.scale;
+    });
+}
+
+int shm_plaintext_set_scale(shm_plaintext *pt, double scale) {
+    return guard([&] {
+        need(pt, "pt");
+        pt->pt.scale = scale;
+    });
+}
+
+uint64_t *shm_plaintext_data(shm_plaintext *pt) { return pt ? pt->pt.data() : nullptr; }
+
+int shm_plaintext_import(shm_plaintext *pt, const uint64_t *host, size_t n, shm_stream_t s) {
+    return guard([&] {
+        need(pt, "pt");
+        need(host, "host");
+        const size_t words = pt->pt.Ql() * pt->pt.ctx->N;
+        if (n != words) throw InvalidArgument("import size mismatch");
+        SHM_HIP(hipMemcpyAsync(pt->pt.data(), host, words * 8, hipMemcpyHostToDevice, S(s)));
+        SHM_HIP(hipStreamSynchronize(S(s)));
+    });
+}
+
+int shm_plaintext_export(const shm_plaintext *pt, uint64_t *host, size_t n, shm_stream_t s) {
+    return guard([&] {
+        need(pt, "pt");
+        need(host, "host");
+        const size_t words = pt->pt.Ql() * pt->pt.ctx->N;
+        if (n != words) throw InvalidArgument("export size mismatch");
+        SHM_HIP(hipMemcpyAsync(host, pt->pt.data(), words * 8, hipMemcpyDeviceToHost, S(s)));
+        SHM_HIP(hipStreamSynchronize(S(s)));
+    });
+}
+
+// ---- keys ----------------------------------------------------------------------
+int shm_secret_key_create(shm_context *ctx, const int8_t *coeffs, shm_secret_key **out) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(coeffs, "coeffs");
+        need(out, "out");
+        for (size_t j = 0; j < ctx->c->N; j++)
+            if (coeffs[j] < -1 || coeffs[j] > 1) throw InvalidArgument("secret key must be ternary");
+        auto h = new shm_secret_key();
+        h->sk.ctx = ctx->c.get();
+        try {
+            load_secret(*ctx->c, h->sk, coeffs);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int shm_secret_key_generate(shm_context *ctx, uint64_t seed, uint32_t hw, shm_secret_key **out) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(out, "out");
+        const size_t N = ctx->c->N;
+        if (hw == 0 || hw > N) throw InvalidArgument("hamming weight out of range");
+        // exact-weight ternary secret by partial Fisher-Yates (SplitMix64 draws)
+        std::vector<uint32_t> perm(N);
+        for (size_t i = 0; i < N; i++) perm[i] = (uint32_t)i;
+        std::vector<int8_t> coeff(N, 0);
+        for (size_t i = 0; i < hw; i++) {
+            const size_t j = i + (size_t)(splitmix64(seed, i) % (uint64_t)(N - i));
+            std::swap(perm[i], perm[j]);
+            coeff[perm[i]] = (splitmix64(seed, N + i) & 1) ? -1 : 1;
+        }
+        auto h = new shm_secret_key();
+        h->sk.ctx = ctx->c.get();
+        try {
+            load_secret(*ctx->c, h->sk, coeff.data());
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int shm_secret_key_destroy(shm_secret_key *sk) {
+    return guard([&] { delete sk; });
+}
+
+int shm_secret_key_export(const shm_secret_key *sk, uint64_t *host, size_t n) {
+    return guard([&] {
+        need(sk, "sk");
+        need(host, "host");
+        const size_t words = sk->sk.ctx->nQP() * sk->sk.ctx->N;
+        if (n != words) throw InvalidArgument("export size mismatch");
+        SHM_HIP(hipMemcpy(host, sk->sk.s.as<void>(), words * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+int shm_relin_key_generate(shm_context *ctx, const shm_secret_key *sk, uint64_t seed, shm_kswitch_key **out) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(sk, "sk");
+        need(out, "out");
+        Context &c = *ctx->c;
+        const size_t words = c.nQP() * c.N;
+        Scratch s2(words * sizeof(uint64_t), nullptr);
+        launch_mul_elementwise(c, sk->sk.s.as<uint64_t>(), sk->sk.s.as<uint64_t>(), s2.as<uint64_t>(), c.nQP(), true,
+                               c.nQ, nullptr);
+        auto h = new shm_kswitch_key();
+        try {
+            gen_kswitch(c, sk->sk, s2.as<uint64_t>(), seed, h->k);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int shm_kswitch_key_generate(shm_context *ctx, const shm_secret_key *sk, const uint64_t *new_key, uint64_t seed,
+                             shm_kswitch_key **out) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(sk, "sk");
+        need(new_key, "new_key");
+        need(out, "out");
+        Context &c = *ctx->c;
+        const size_t words = c.nQP() * c.N;
+        Scratch nk(words * sizeof(uint64_t), nullptr);
+        SHM_HIP(hipMemcpy(nk.as<void>(), new_key, words * 8, hipMemcpyHostToDevice));
+        auto h = new shm_kswitch_key();
+        try {
+            gen_kswitch(c, sk->sk, nk.as<uint64_t>(), seed, h->k);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int shm_kswitch_key_destroy(shm_kswitch_key *key) {
+    return guard([&] { delete key; });
+}
+
+int shm_kswitch_key_export(const shm_kswitch_key *key, uint64_t *host, size_t n) {
+    return guard([&] {
+        need(key, "key");
+        need(host, "host");
+        const Context &c = *key->k.ctx;
+        const size_t words = c.dnum * 2 * c.nQP() * c.N;
+        if (n != words) throw InvalidArgument("export size mismatch");
+        SHM_HIP(hipMemcpy(host, key->k.k.as<void>(), words * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+int shm_galois_keys_generate(shm_context *ctx, const shm_secret_key *sk, const int *steps, size_t n_steps,
+                             uint64_t seed, shm_galois_keys **out) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(sk, "sk");
+        need(out, "out");
+        if (n_steps) need(steps, "steps");
+        Context &c = *ctx->c;
+        const size_t QP = c.nQP(), N = c.N;
+        auto h = new shm_galois_keys();
+        h->g.ctx = &c;
+        try {
+            Scratch rs(QP * N * sizeof(uint64_t), nullptr);
+            for (size_t i = 0; i < n_steps; i++) {
+                const uint32_t elt = c.elt_from_step(steps[i]);
+                if (h->g.keys.count(elt)) continue;
+                const uint32_t *table = c.galois_table(elt, nullptr);
+                launch_permute(c, sk->sk.s.as<uint64_t>(), rs.as<uint64_t>(), table, QP, nullptr);
+                auto k = std::make_unique<KSwitchKey>();
+                gen_kswitch(c, sk->sk, rs.as<uint64_t>(), splitmix64(seed, elt), *k);
+                h->g.keys[elt] = std::move(k);
+            }
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int shm_galois_keys_destroy(shm_galois_keys *gk) {
+    return guard([&] { delete gk; });
+}
+
+int shm_galois_keys_export(const shm_galois_keys *gk, uint32_t elt, uint64_t *host, size_t n) {
+    return guard([&] {
+        need(gk, "gk");
+        need(host, "host");
+        auto it = gk->g.keys.find(elt);
+        if (it == gk->g.keys.end()) throw InvalidArgument("Galois elt not present");
+        const Context &c = *gk->g.ctx;
+        const size_t words = c.dnum * 2 * c.nQP() * c.N;
+        if (n != words) throw InvalidArgument("export size mismatch");
+        SHM_HIP(hipMemcpy(host, it->second->k.as<void>(), words * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+// ---- encryption -------------------------------------------------------------------
+int shm_encrypt_symmetric(shm_context *ctx, const shm_secret_key *sk, const shm_plaintext *pt, uint64_t seed,
+                          shm_ciphertext *out, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(sk, "sk");
+        need(pt, "pt");
+        need(out, "out");
+        Context &c = *ctx->c;
+        const hipStream_t s = S(st);
+        const size_t Ql = pt->pt.Ql(), N = c.N;
+        out->ct.ctx = &c;
+        out->ct.resize(2, pt->pt.chain);
+        out->ct.scale = pt->pt.scale;
+        out->ct.ntt = true;
+        uint64_t *c0 = out->ct.poly(0), *c1 = out->ct.poly(1);
+        launch_sample_uniform(c, splitmix64(seed, 0), c1, Ql, false, s);
+        Scratch e(Ql * N * sizeof(uint64_t), s);
+        launch_sample_gaussian_rns(c, splitmix64(seed, 1), e.as<uint64_t>(), Ql, false, s);
+        ntt_forward(c, e.as<uint64_t>(), 1, Ql, 0, s);
+        launch_encrypt_combine(c, sk->sk.s.as<uint64_t>(), c1, e.as<uint64_t>(), pt->pt.data(), c0, Ql, s);
+    });
+}
+
+int shm_decrypt(shm_context *ctx, const shm_secret_key *sk, const shm_ciphertext *ct, shm_plaintext *out,
+                shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(sk, "sk");
+        need(ct, "ct");
+        need(out, "out");
+        Context &c = *ctx->c;
+        out->pt.chain = ct->ct.chain;
+        out->pt.scale = ct->ct.scale;
+        launch_decrypt(c, ct->ct.data(), ct->ct.size, sk->sk.s.as<uint64_t>(), out->pt.data(), ct->ct.Ql(), S(st));
+    });
+}
+
+// ---- encoder ----------------------------------------------------------------------
+int shm_encode(shm_context *ctx, const double *re, const double *im, size_t n_values, uint32_t chain, double scale,
+               shm_plaintext *out, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(out, "out");
+        if (n_values) need(re, "re");
+        Context &c = *ctx->c;
+        check_chain(c, chain);
+        if (n_values == 1) {
+            // moai::Encoder forwards single values to the scalar path
+            encode_scalar(c, re[0], c.ql_of_chain(chain), scale, out->pt.data(), S(st));
+        } else {
+            encode(c, re, im, n_values, c.ql_of_chain(chain), scale, out->pt.data(), S(st));
+        }
+        out->pt.chain = chain;
+        out->pt.scale = scale;
+    });
+}
+
+int shm_encode_scalar(shm_context *ctx, double value, uint32_t chain, double scale, shm_plaintext *out,
+                      shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(out, "out");
+        Context &c = *ctx->c;
+        check_chain(c, chain);
+        encode_scalar(c, value, c.ql_of_chain(chain), scale, out->pt.data(), S(st));
+        out->pt.chain = chain;
+        out->pt.scale = scale;
+    });
+}
+
+int shm_decode(shm_context *ctx, const shm_plaintext *pt, double *re, double *im, size_t n, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(pt, "pt");
+        need(re, "re");
+        decode(*ctx->c, pt->pt.data(), pt->pt.Ql(), pt->pt.scale, re, im, n, S(st));
+    });
+}
+
+// ---- evaluator ----------------------------------------------------------------------
+int shm_add_inplace(shm_context *ctx, shm_ciphertext *a, const shm_ciphertext *b, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(a, "a");
+        need(b, "b");
+        same_level(a->ct, b->ct);
+        if (a->ct.size != b->ct.size) throw InvalidArgument("ciphertext sizes differ");
+        launch_add(*ctx->c, a->ct.data(), b->ct.data(), a->ct.Ql(), a->ct.size, S(st));
+    });
+}
+
+int shm_sub_inplace(shm_context *ctx, shm_ciphertext *a, const shm_ciphertext *b, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(a, "a");
+        need(b, "b");
+        same_level(a->ct, b->ct);
+        if (a->ct.size != b->ct.size) throw InvalidArgument("ciphertext sizes differ");
+        if (std::fabs(a->ct.scale - b->ct.scale) > 1e-6 * std::fabs(a->ct.scale))
+            throw InvalidArgument("scale mismatch");
+        launch_sub(*ctx->c, a->ct.data(), b->ct.data(), a->ct.Ql(), a->ct.size, S(st));
+    });
+}
+
+int shm_negate_inplace(shm_context *ctx, shm_ciphertext *a, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(a, "a");
+        launch_negate(*ctx->c, a->ct.data(), a->ct.Ql(), a->ct.size, S(st));
+    });
+}
+
+int shm_add_plain_inplace(shm_context *ctx, shm_ciphertext *a, const shm_plaintext *pt, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(a, "a");
+        need(pt, "pt");
+        if (a->ct.chain != pt->pt.chain) throw InvalidArgument("encrypted and plain parameter mismatch");
+        if (std::fabs(a->ct.scale - pt->pt.scale) > 1e-6 * std::fabs(a->ct.scale))
+            throw InvalidArgument("scale mismatch");
+        launch_add(*ctx->c, a->ct.data(), pt->pt.data(), a->ct.Ql(), 1, S(st));
+    });
+}
+
+int shm_sub_plain_inplace(shm_context *ctx, shm_ciphertext *a, const shm_plaintext *pt, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(a, "a");
+        need(pt, "pt");
+        if (a->ct.chain != pt->pt.chain) throw InvalidArgument("encrypted and plain parameter mismatch");
+        if (std::fabs(a->ct.scale - pt->pt.scale) > 1e-6 * std::fabs(a->ct.scale))
+            throw InvalidArgument("scale mismatch");
+        launch_sub(*ctx->c, a->ct.data(), pt->pt.data(), a->ct.Ql(), 1, S(st));
+    });
+}
+
+int shm_multiply_plain_inplace(shm_context *ctx, shm_ciphertext *ct, const shm_plaintext *pt, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(ct, "ct");
+        need(pt, "pt");
+        if (ct->ct.chain != pt->pt.chain) throw InvalidArgument("encrypted and plain parameter mismatch");
+        launch_mul_plain(*ctx->c, ct->ct.data(), pt->pt.data(), ct->ct.Ql(), ct->ct.size, S(st));
+        ct->ct.scale = ct->ct.scale * pt->pt.scale;
+    });
+}
+
+int shm_multiply_inplace(shm_context *ctx, shm_ciphertext *a, const shm_ciphertext *b, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(a, "a");
+        need(b, "b");
+        same_level(a->ct, b->ct);
+        if (a->ct.size != 2 || b->ct.size != 2) throw InvalidArgument("multiply supports size-2 ciphertexts");
+        Context &c = *ctx->c;
+        const size_t Ql = a->ct.Ql(), S3 = 3 * Ql * c.N;
+        const hipStream_t s = S(st);
+        Scratch out(S3 * sizeof(uint64_t), s);
+        if (a == b)
+            launch_square(c, a->ct.data(), out.as<uint64_t>(), Ql, s);
+        else
+            launch_tensor(c, a->ct.data(), b->ct.data(), out.as<uint64_t>(), Ql, s);
+        a->ct.resize(3, a->ct.chain);
+        SHM_HIP(hipMemcpyAsync(a->ct.data(), out.as<uint64_t>(), S3 * 8, hipMemcpyDeviceToDevice, s));
+        a->ct.scale = a->ct.scale * b->ct.scale;
+    });
+}
+
+int shm_square_inplace(shm_context *ctx, shm_ciphertext *a, shm_stream_t st) {
+    return shm_multiply_inplace(ctx, a, a, st);
+}
+
+int shm_relinearize_inplace(shm_context *ctx, shm_ciphertext *ct, const shm_kswitch_key *rk, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(ct, "ct");
+        need(rk, "relin_key");
+        if (ct->ct.size != 3) throw InvalidArgument("destination_size must be 3");
+        Context &c = *ctx->c;
+        keyswitch(c, ct->ct.poly(2), rk->k.k.as<uint64_t>(), ct->ct.Ql(), ct->ct.poly(0), ct->ct.poly(1), true, S(st));
+        ct->ct.size = 2;
+    });
+}
+
+int shm_apply_kswitch_key_inplace(shm_context *ctx, shm_ciphertext *ct, const shm_kswitch_key *key,
+                                  shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(ct, "ct");
+        need(key, "key");
+        if (ct->ct.size != 2) throw InvalidArgument("encrypted size must be 2");
+        Context &c = *ctx->c;
+        const size_t Ql = ct->ct.Ql();
+        const hipStream_t s = S(st);
+        Scratch t(Ql * c.N * sizeof(uint64_t), s);
+        SHM_HIP(hipMemcpyAsync(t.as<uint64_t>(), ct->ct.poly(1), Ql * c.N * 8, hipMemcpyDeviceToDevice, s));
+        keyswitch(c, t.as<uint64_t>(), key->k.k.as<uint64_t>(), Ql, ct->ct.poly(0), ct->ct.poly(1), false, s);
+    });
+}
+
+int shm_rescale_to_next_inplace(shm_context *ctx, shm_ciphertext *ct, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(ct, "ct");
+        rescale_inplace(*ctx->c, ct->ct, S(st));
+    });
+}
+
+int shm_mod_switch_to_next_inplace(shm_context *ctx, shm_ciphertext *ct, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(ct, "ct");
+        if (ct->ct.Ql() < 2) throw LogicError("end of modulus switching chain reached");
+        mod_switch_drop(*ctx->c, ct->ct, ct->ct.chain + 1, S(st));
+    });
+}
+
+int shm_mod_switch_to_inplace(shm_context *ctx, shm_ciphertext *ct, uint32_t chain, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(ct, "ct");
+        check_chain(*ctx->c, chain);
+        mod_switch_drop(*ctx->c, ct->ct, chain, S(st));
+    });
+}
+
+int shm_plain_mod_switch_to_inplace(shm_context *ctx, shm_plaintext *pt, uint32_t chain, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(pt, "pt");
+        check_chain(*ctx->c, chain);
+        if (chain < pt->pt.chain) throw InvalidArgument("cannot switch to a higher level");
+        (void)st;
+        pt->pt.chain = chain;  // limb-major: dropping trailing limbs needs no data movement
+    });
+}
+
+int shm_apply_galois_inplace(shm_context *ctx, shm_ciphertext *ct, uint32_t elt, const shm_galois_keys *gk,
+                             shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(ct, "ct");
+        need(gk, "galois_keys");
+        apply_galois(*ctx->c, ct->ct, elt, gk->g, S(st));
+    });
+}
+
+int shm_rotate_vector_inplace(shm_context *ctx, shm_ciphertext *ct, int step, const shm_galois_keys *gk,
+                              shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(ct, "ct");
+        need(gk, "galois_keys");
+        rotate_internal(*ctx->c, ct->ct, step, gk->g, S(st));
+    });
+}
+
+int shm_complex_conjugate_inplace(shm_context *ctx, shm_ciphertext *ct, const shm_galois_keys *gk, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(ct, "ct");
+        need(gk, "galois_keys");
+        apply_galois(*ctx->c, ct->ct, ctx->c->elt_from_step(0), gk->g, S(st));
+    });
+}
+
+// ---- MOAI matmul ------------------------------------------------------------------------
+int shm_ct_pt_matrix_mul(shm_context *ctx, const shm_ciphertext *const *X, uint32_t row_W, const double *W,
+                         uint32_t col_W, shm_ciphertext *const *out, int variant, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(X, "enc_X");
+        need(W, "W");
+        need(out, "out");
+        if (variant < 0 || variant > 2) throw InvalidArgument("unknown matmul variant");
+        std::vector<const Ciphertext *> xs(row_W);
+        for (uint32_t j = 0; j < row_W; j++) {
+            need(X[j], "enc_X[j]");
+            xs[j] = &X[j]->ct;
+        }
+        std::vector<Ciphertext *> os(col_W);
+        for (uint32_t i = 0; i < col_W; i++) {
+            need(out[i], "out[i]");
+            out[i]->ct.ctx = ctx->c.get();
+            os[i] = &out[i]->ct;
+        }
+        ctpt_matmul(*ctx->c, xs, W, row_W, col_W, os, variant, S(st));
+    });
+}
+
+double shm_last_matmul_kernel_ms(void) { return (double)last_matmul_kernel_ms(); }
+
+}  // extern "C"

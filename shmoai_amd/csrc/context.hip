@@ -1,0 +1,385 @@
+// Context construction: RNS primes, NTT twiddles, per-level base-conversion
+// and rescale constants, encoder roots. Host-side precompute (as in the
+// reference: modulus.cu, util/ntt.cu, rns.cu, ckks.cu), uploaded once to HBM.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <map>
+
+#include "engine.h"
+
+namespace shm {
+
+namespace {
+
+typedef unsigned __int128 hu128;
+
+uint64_t h_mulmod(uint64_t a, uint64_t b, uint64_t q) { return (uint64_t)(((hu128)a * b) % q); }
+uint64_t h_powmod(uint64_t b, uint64_t e, uint64_t q) {
+    uint64_t r = 1 % q;
+    b %= q;
+    for (; e; e >>= 1) {
+        if (e & 1) r = h_mulmod(r, b, q);
+        b = h_mulmod(b, b, q);
+    }
+    return r;
+}
+uint64_t h_inv(uint64_t a, uint64_t q) { return h_powmod(a % q, q - 2, q); }
+uint64_t h_shoup(uint64_t w, uint64_t q) { return (uint64_t)(((hu128)w << 64) / q); }
+
+uint32_t h_brv(uint32_t x, int bits) {
+    uint32_t r = 0;
+    for (int i = 0; i < bits; i++) r = (r << 1) | ((x >> i) & 1u);
+    return r;
+}
+
+// Miller-Rabin with the first twelve prime bases: exact for all 64-bit inputs.
+bool h_is_prime(uint64_t n) {
+    static const uint64_t bases[] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+    if (n < 2) return false;
+    for (uint64_t p : bases)
+        if (n % p == 0) return n == p;
+    uint64_t d = n - 1;
+    int r = 0;
+    while (!(d & 1)) d >>= 1, r++;
+    for (uint64_t a : bases) {
+        uint64_t x = h_powmod(a, d, n);
+        if (x == 1 || x == n - 1) continue;
+        int i = 1;
+        for (; i < r; i++) {
+            x = h_mulmod(x, x, n);
+            if (x == n - 1) break;
+        }
+        if (i == r) return false;
+    }
+    return true;
+}
+
+// modulus.cu:79-108 + util/numth.cu:209-235: candidates 2^b - 2N*k + 1,
+// descending; each bit-size class is consumed from the smallest found prime.
+std::vector<uint64_t> make_primes(size_t N, const std::vector<int> &bits) {
+    std::map<int, size_t> need;
+    for (int b : bits) {
+        if (b < 2 || b > 61) throw InvalidArgument("coeff bit size must be in [2, 61]");
+        need[b]++;
+    }
+    std::map<int, std::vector<uint64_t>> found;
+    for (auto &kv : need) {
+        const uint64_t step = 2 * (uint64_t)N;
+        const uint64_t top = uint64_t(1) << kv.first;
+        if (top <= step) throw InvalidArgument("bit size too small for poly degree");
+        uint64_t v = top - step + 1;
+        const uint64_t floor_v = uint64_t(1) << (kv.first - 1);
+        std::vector<uint64_t> list;
+        while (list.size() < kv.second && v > floor_v) {
+            if (h_is_prime(v)) list.push_back(v);
+            v -= step;
+        }
+        if (list.size() < kv.second) throw LogicError("failed to find enough qualifying primes");
+        found[kv.first] = list;
+    }
+    std::vector<uint64_t> out;
+    for (int b : bits) {
+        out.push_back(found[b].back());
+        found[b].pop_back();
+    }
+    return out;
+}
+
+// util/numth.cu:311-333: the smallest primitive 2N-th root of unity.
+uint64_t min_root_2n(uint64_t q, uint64_t two_n) {
+    const uint64_t cof = (q - 1) / two_n;
+    uint64_t g = 0;
+    for (uint64_t x = 2;; x++) {
+        uint64_t c = h_powmod(x, cof, q);
+        if (h_powmod(c, two_n / 2, q) == q - 1) {
+            g = c;
+            break;
+        }
+    }
+    const uint64_t g2 = h_mulmod(g, g, q);
+    uint64_t best = g, cur = g;
+    for (uint64_t i = 0; i < two_n / 2; i++) {
+        best = std::min(best, cur);
+        cur = h_mulmod(cur, g2, q);
+    }
+    return best;
+}
+
+template <class T>
+void upload(DevBuf &b, const std::vector<T> &v) {
+    b.alloc(std::max<size_t>(v.size() * sizeof(T), 8));
+    if (!v.empty()) SHM_HIP(hipMemcpy(b.as<void>(), v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+}  // namespace
+
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void DevBuf::alloc(size_t bytes) {
+    release();
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(&ptr_, bytes);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        throw std::bad_alloc();
+    }
+    hip_check(e, "hipMalloc");
+    bytes_ = bytes;
+}
+
+void DevBuf::release() {
+    if (ptr_) (void)hipFree(ptr_);
+    ptr_ = nullptr;
+    bytes_ = 0;
+}
+
+Scratch::Scratch(size_t bytes, hipStream_t s) : s_(s) {
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMallocAsync(&ptr_, bytes, s);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        throw std::bad_alloc();
+    }
+    hip_check(e, "hipMallocAsync");
+}
+
+Scratch::~Scratch() {
+    if (ptr_) (void)hipFreeAsync(ptr_, s_);
+}
+
+uint32_t Context::elt_from_step(int step) const {
+    // galois.cu:47-90
+    const uint64_t m = 2 * (uint64_t)N;
+    if (step == 0) return (uint32_t)(m - 1);
+    const uint32_t mag = (uint32_t)std::abs(step);
+    if (mag >= N / 2) throw InvalidArgument("step count too large");
+    uint64_t e = 1;
+    uint32_t k = step < 0 ? (uint32_t)(N / 2) - mag : mag;
+    for (uint32_t i = 0; i < k; i++) e = (e * 5) & (m - 1);
+    return (uint32_t)e;
+}
+
+const uint32_t *Context::galois_table(uint32_t elt, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(galois_mu);
+    auto it = galois_tables.find(elt);
+    if (it != galois_tables.end()) return it->second->as<uint32_t>();
+    if (!(elt & 1) || elt >= 2 * N) throw InvalidArgument("Galois element is not valid");
+    // galois.cuh: NTT-domain permutation for X -> X^elt on bit-reversed slots.
+    std::vector<uint32_t> t(N);
+    for (size_t i = N; i < 2 * N; i++) {
+        uint32_t r = h_brv((uint32_t)i, logN + 1);
+        uint64_t raw = (((uint64_t)elt * r) >> 1) & (N - 1);
+        t[i - N] = h_brv((uint32_t)raw, logN);
+    }
+    auto b = std::make_unique<DevBuf>(N * sizeof(uint32_t));
+    SHM_HIP(hipMemcpyAsync(b->as<void>(), t.data(), N * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    SHM_HIP(hipStreamSynchronize(s));
+    const uint32_t *p = b->as<uint32_t>();
+    galois_tables[elt] = std::move(b);
+    return p;
+}
+
+std::unique_ptr<Context> make_context(int logN, const std::vector<int> &bits, size_t nP) {
+    if (logN < 10 || logN > 17) throw InvalidArgument("poly_modulus_degree must be 2^10 .. 2^17");
+    if (nP == 0 || nP >= bits.size()) throw InvalidArgument("special_modulus_size must be in [1, #primes)");
+    auto c = std::make_unique<Context>();
+    SHM_HIP(hipGetDevice(&c->device));
+    c->logN = logN;
+    c->N = size_t(1) << logN;
+    c->nP = nP;
+    c->nQ = bits.size() - nP;
+    c->dnum = c->nQ / c->nP;
+    c->q = make_primes(c->N, bits);
+    const size_t N = c->N, QP = c->q.size();
+
+    std::vector<DevMod> mods(QP);
+    std::vector<uint64_t> tw(QP * N), tws(QP * N), itw(QP * N), itws(QP * N), ninv(QP), ninvs(QP);
+    c->psi.resize(QP);
+    for (size_t p = 0; p < QP; p++) {
+        const uint64_t q = c->q[p];
+        hu128 mu = (~(hu128)0) / q;  // floor((2^128 - 1)/q) == floor(2^128/q) for non-power-of-two q
+        mods[p] = DevMod{q, (uint64_t)(mu >> 64), (uint64_t)mu};
+        const uint64_t psi = min_root_2n(q, 2 * N), ipsi = h_inv(psi, q);
+        c->psi[p] = psi;
+        uint64_t pw = 1, ipw = 1;
+        for (size_t i = 0; i < N; i++) {
+            const uint32_t r = h_brv((uint32_t)i, logN);
+            tw[p * N + r] = pw;
+            itw[p * N + r] = ipw;
+            pw = h_mulmod(pw, psi, q);
+            ipw = h_mulmod(ipw, ipsi, q);
+        }
+        for (size_t i = 0; i < N; i++) {
+            tws[p * N + i] = h_shoup(tw[p * N + i], q);
+            itws[p * N + i] = h_shoup(itw[p * N + i], q);
+        }
+        ninv[p] = h_inv(N, q);
+        ninvs[p] = h_shoup(ninv[p], q);
+    }
+    upload(c->d_mod, mods);
+    upload(c->tw, tw);
+    upload(c->tw_s, tws);
+    upload(c->itw, itw);
+    upload(c->itw_s, itws);
+    upload(c->ninv, ninv);
+    upload(c->ninv_s, ninvs);
+
+    // ModDown constants (rns.cu:95-128, 187-190): P mod q_i, P^-1 mod q_i,
+    // (P/p_j)^-1 mod p_j, (P/p_j) mod q_i.
+    const size_t nQ = c->nQ;
+    std::vector<uint64_t> pinv(nQ), pinvs(nQ), pmod(nQ), pmods(nQ), phinv(nP), phinvs(nP), phmod(nQ * nP);
+    for (size_t i = 0; i < nQ; i++) {
+        const uint64_t q = c->q[i];
+        uint64_t P = 1;
+        for (size_t j = 0; j < nP; j++) P = h_mulmod(P, c->q[nQ + j] % q, q);
+        pmod[i] = P;
+        pmods[i] = h_shoup(P, q);
+        pinv[i] = h_inv(P, q);
+        pinvs[i] = h_shoup(pinv[i], q);
+        for (size_t j = 0; j < nP; j++) {
+            uint64_t h = 1;
+            for (size_t k = 0; k < nP; k++)
+                if (k != j) h = h_mulmod(h, c->q[nQ + k] % q, q);
+            phmod[i * nP + j] = h;
+        }
+    }
+    for (size_t j = 0; j < nP; j++) {
+        const uint64_t p = c->q[nQ + j];
+        uint64_t h = 1;
+        for (size_t k = 0; k < nP; k++)
+            if (k != j) h = h_mulmod(h, c->q[nQ + k] % p, p);
+        phinv[j] = h_inv(h, p);
+        phinvs[j] = h_shoup(phinv[j], p);
+    }
+    upload(c->pinv, pinv);
+    upload(c->pinv_s, pinvs);
+    upload(c->pmod, pmod);
+    upload(c->pmod_s, pmods);
+    upload(c->phat_inv, phinv);
+    upload(c->phat_inv_s, phinvs);
+    upload(c->phat_mod, phmod);
+
+    // Per-level constants (rns.cu:63-94 rescale; rns.cu:131-186 modup).
+    c->levels.resize(nQ + 1);
+    const size_t alpha = nP;
+    for (size_t Ql = 1; Ql <= nQ; Ql++) {
+        auto L = std::make_unique<LevelConsts>();
+        L->Ql = Ql;
+        L->beta = (Ql + alpha - 1) / alpha;
+        const size_t QlP = Ql + nP;
+        std::vector<uint64_t> qi(std::max<size_t>(Ql, 2) - 1), qis(std::max<size_t>(Ql, 2) - 1);
+        for (size_t i = 0; i + 1 < Ql; i++) {
+            qi[i] = h_inv(c->q[Ql - 1] % c->q[i], c->q[i]);
+            qis[i] = h_shoup(qi[i], c->q[i]);
+        }
+        std::vector<uint64_t> hi(Ql), his(Ql), hm(L->beta * alpha * QlP, 0);
+        for (size_t g = 0; g < L->beta; g++) {
+            const size_t start = g * alpha, size = std::min(alpha, Ql - start);
+            for (size_t i = 0; i < size; i++) {
+                const uint64_t qv = c->q[start + i];
+                uint64_t h = 1;
+                for (size_t k = 0; k < size; k++)
+                    if (k != i) h = h_mulmod(h, c->q[start + k] % qv, qv);
+                hi[start + i] = h_inv(h, qv);
+                his[start + i] = h_shoup(hi[start + i], qv);
+                for (size_t k = 0; k < QlP; k++) {
+                    if (k >= start && k < start + size) continue;
+                    const uint64_t p = c->q[k < Ql ? k : nQ + (k - Ql)];
+                    uint64_t hp = 1;
+                    for (size_t kk = 0; kk < size; kk++)
+                        if (kk != i) hp = h_mulmod(hp, c->q[start + kk] % p, p);
+                    hm[(g * alpha + i) * QlP + k] = hp;
+                }
+            }
+        }
+        upload(L->qlast_inv, qi);
+        upload(L->qlast_inv_s, qis);
+        upload(L->hat_inv, hi);
+        upload(L->hat_inv_s, his);
+        upload(L->hat_mod, hm);
+        c->levels[Ql] = std::move(L);
+    }
+
+    // Encoder roots: ckks.cu:30-87, fft.cu:11-40 (1/8 of the 2N-th roots by
+    // polar form, the rest by symmetry).
+    const size_t m = 2 * N;
+    const double PI_ = 3.1415926535897932384626433832795028842;
+    std::vector<double> base_re(m / 8 + 1), base_im(m / 8 + 1);
+    for (size_t i = 0; i <= m / 8; i++) {
+        const double th = 2 * PI_ * (double)i / (double)m;
+        base_re[i] = std::cos(th);
+        base_im[i] = std::sin(th);
+    }
+    std::vector<double> roots(2 * m);
+    for (size_t i0 = 0; i0 < m; i0++) {
+        // resolve the symmetry chain iteratively
+        // Map the index into [0, m/8] through conj(m - i), -(i - m/2),
+        // -conj(m/2 - i) and mirror(m/4 - i), then undo the maps in reverse.
+        size_t idx = i0;
+        double re = 0, im = 0;
+        {
+            std::vector<int> ops;  // 1 conj, 2 neg, 3 negconj, 4 mirror
+            for (int hop = 0; hop < 4; hop++) {
+                idx &= m - 1;
+                if (idx <= m / 8) break;
+                if (idx <= m / 4) {
+                    ops.push_back(4);
+                    idx = m / 4 - idx;
+                    break;
+                }
+                if (idx <= m / 2) {
+                    ops.push_back(3);
+                    idx = m / 2 - idx;
+                } else if (idx <= 3 * m / 4) {
+                    ops.push_back(2);
+                    idx = idx - m / 2;
+                } else {
+                    ops.push_back(1);
+                    idx = m - idx;
+                }
+            }
+            re = base_re[idx];
+            im = base_im[idx];
+            for (size_t k = ops.size(); k-- > 0;) {
+                switch (ops[k]) {
+                    case 4: std::swap(re, im); break;
+                    case 3: re = 0.0 - re; im = 0.0 - (-im); break;
+                    case 2: re = 0.0 - re; im = 0.0 - im; break;
+                    case 1: im = -im; break;
+                }
+            }
+        }
+        roots[2 * i0] = re;
+        roots[2 * i0 + 1] = im;
+    }
+    upload(c->enc_roots, roots);
+    std::vector<uint32_t> grp(std::max<size_t>(N / 4, 1));
+    uint64_t pos = 1;
+    for (size_t i = 0; i < N / 4; i++) {
+        grp[i] = (uint32_t)pos;
+        pos = (pos * 5) & (m - 1);
+    }
+    upload(c->enc_group, grp);
+    SHM_HIP(hipDeviceSynchronize());
+    return c;
+}
+
+void Ciphertext::resize(size_t new_size, size_t new_chain) {
+    const size_t nl = ctx->ql_of_chain(new_chain);
+    if (new_size > cap_size || nl > cap_limbs) {
+        if (buf.bytes()) SHM_HIP(hipDeviceSynchronize());  // old buffer may still be read by queued work
+        DevBuf nb(new_size * nl * ctx->N * sizeof(uint64_t));
+        SHM_HIP(hipMemset(nb.as<void>(), 0, new_size * nl * ctx->N * sizeof(uint64_t)));
+        buf = std::move(nb);
+        cap_size = new_size;
+        cap_limbs = nl;
+    }
+    size = new_size;
+    chain = new_chain;
+}
+
+}  // namespace shm

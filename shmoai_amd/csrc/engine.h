@@ -1,0 +1,243 @@
+// Host-side engine structures shared by the HIP translation units.
+//
+// Layouts (HBM):
+//   polynomial at level Ql        uint64 [Ql][N]              (limb-major)
+//   ciphertext                     uint64 [size][Ql][N]       NTT form
+//   key-switching key              uint64 [dnum][2][QP][N]    b-part, a-part
+//   twiddles per prime             uint64 [QP][N]             bit-reversed powers
+// Prime index convention: 0..nQ-1 data primes, nQ..nQ+nP-1 special primes.
+// Limb k of a "QlP" polynomial uses prime k for k < Ql and nQ + (k - Ql) after.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "arith.h"
+
+namespace shm {
+
+// Error classes surfaced through the C ABI (mirrors the reference's exception
+// types: std::invalid_argument, std::logic_error, std::runtime_error).
+struct InvalidArgument : std::invalid_argument {
+    using std::invalid_argument::invalid_argument;
+};
+struct LogicError : std::logic_error {
+    using std::logic_error::logic_error;
+};
+struct RuntimeError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char *what);
+#define SHM_HIP(call) ::shm::hip_check((call), #call)
+
+// Device buffer owned by the host object (freed with the object).
+class DevBuf {
+  public:
+    DevBuf() = default;
+    explicit DevBuf(size_t bytes) { alloc(bytes); }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : ptr_(o.ptr_), bytes_(o.bytes_) {
+        o.ptr_ = nullptr;
+        o.bytes_ = 0;
+    }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) {
+            release();
+            ptr_ = o.ptr_;
+            bytes_ = o.bytes_;
+            o.ptr_ = nullptr;
+            o.bytes_ = 0;
+        }
+        return *this;
+    }
+    void alloc(size_t bytes);
+    void release();
+    template <class T>
+    T *as() const {
+        return reinterpret_cast<T *>(ptr_);
+    }
+    size_t bytes() const { return bytes_; }
+
+  private:
+    void *ptr_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+// Stream-ordered scratch (hipMallocAsync / hipFreeAsync on the op's stream).
+class Scratch {
+  public:
+    Scratch(size_t bytes, hipStream_t s);
+    ~Scratch();
+    Scratch(const Scratch &) = delete;
+    Scratch &operator=(const Scratch &) = delete;
+    template <class T>
+    T *as() const {
+        return reinterpret_cast<T *>(ptr_);
+    }
+
+  private:
+    void *ptr_ = nullptr;
+    hipStream_t s_;
+};
+
+// Per-level constants (level = number of data limbs Ql).
+struct LevelConsts {
+    size_t Ql = 0, beta = 0;
+    // rescale: q_last^-1 mod q_i, i < Ql-1 (value, shoup)
+    DevBuf qlast_inv, qlast_inv_s;
+    // modup: digit-local hat inverse per data limb i (value, shoup)
+    DevBuf hat_inv, hat_inv_s;
+    // modup: [qhat_i]_{p_k} for digit g, local i, out limb k: [beta][alpha][QlP]
+    DevBuf hat_mod;
+};
+
+struct Context {
+    int logN = 0;
+    size_t N = 0, nQ = 0, nP = 0, dnum = 0;
+    std::vector<uint64_t> q;  // QP primes
+    std::vector<uint64_t> psi;
+    int device = 0;
+
+    // device tables
+    DevBuf d_mod;           // DevMod[QP]
+    DevBuf tw, tw_s;        // [QP][N]
+    DevBuf itw, itw_s;      // [QP][N]
+    DevBuf ninv, ninv_s;    // [QP]
+    DevBuf pinv, pinv_s;    // P^-1 mod q_i, i < nQ
+    DevBuf pmod, pmod_s;    // P mod q_i, i < nQ
+    DevBuf phat_inv, phat_inv_s;  // [nP] (P/p_j)^-1 mod p_j
+    DevBuf phat_mod;        // [nQ][nP] (P/p_j) mod q_i
+    std::vector<std::unique_ptr<LevelConsts>> levels;  // index Ql (0 unused)
+
+    // encoder
+    DevBuf enc_roots;       // complex<double>[2N] (re, im interleaved)
+    DevBuf enc_group;       // uint32[N/4]
+
+    // Galois permutation tables cache: elt -> uint32[N]
+    std::mutex galois_mu;
+    std::map<uint32_t, std::unique_ptr<DevBuf>> galois_tables;
+
+    size_t nQP() const { return nQ + nP; }
+    size_t ql_of_chain(size_t chain) const { return nQ - chain + 1; }
+    size_t chain_of_ql(size_t Ql) const { return nQ - Ql + 1; }
+    const DevMod *mods() const { return d_mod.as<DevMod>(); }
+    const LevelConsts &level(size_t Ql) const { return *levels.at(Ql); }
+    const uint32_t *galois_table(uint32_t elt, hipStream_t s);
+    uint32_t elt_from_step(int step) const;
+};
+
+std::unique_ptr<Context> make_context(int logN, const std::vector<int> &bits, size_t nP);
+
+struct Ciphertext {
+    Context *ctx = nullptr;
+    size_t size = 0;      // polynomials (2 or 3)
+    size_t chain = 1;     // chain index (1 = all of Q)
+    size_t cap_limbs = 0; // allocated limbs per polynomial
+    size_t cap_size = 0;
+    double scale = 1.0;
+    bool ntt = true;
+    DevBuf buf;
+    size_t Ql() const { return ctx->ql_of_chain(chain); }
+    uint64_t *data() const { return buf.as<uint64_t>(); }
+    uint64_t *poly(size_t i) const { return data() + i * Ql() * ctx->N; }
+    void resize(size_t new_size, size_t new_chain);  // keeps contents of the leading limbs only
+};
+
+struct Plaintext {
+    Context *ctx = nullptr;
+    size_t chain = 1;
+    double scale = 1.0;
+    DevBuf buf;
+    size_t Ql() const { return ctx->ql_of_chain(chain); }
+    uint64_t *data() const { return buf.as<uint64_t>(); }
+};
+
+struct SecretKey {
+    Context *ctx = nullptr;
+    DevBuf s;  // NTT form over QP: [QP][N]
+};
+
+struct KSwitchKey {
+    Context *ctx = nullptr;
+    DevBuf k;  // [dnum][2][QP][N]
+};
+
+struct GaloisKeys {
+    Context *ctx = nullptr;
+    std::map<uint32_t, std::unique_ptr<KSwitchKey>> keys;
+};
+
+// ---- kernels / launchers (defined in the .hip files) ----------------------
+// NTT over n_limbs consecutive limbs whose primes are given by prime_idx
+// (device array) or by a contiguous range starting at first_prime.
+void ntt_forward(const Context &c, uint64_t *data, size_t n_polys, size_t n_limbs, size_t first_prime,
+                 hipStream_t s);
+void ntt_inverse(const Context &c, uint64_t *data, size_t n_polys, size_t n_limbs, size_t first_prime,
+                 hipStream_t s);
+// Same, for a QlP-shaped polynomial: limbs [0, Ql) use primes 0..Ql-1 and the
+// rest use special primes; only limbs [lo, hi) are transformed, skipping
+// [skip_lo, skip_hi).
+void ntt_forward_qlp(const Context &c, uint64_t *data, size_t Ql, size_t lo, size_t hi, size_t skip_lo,
+                     size_t skip_hi, size_t n_polys, size_t poly_stride, hipStream_t s);
+void ntt_inverse_qlp(const Context &c, uint64_t *data, size_t Ql, size_t lo, size_t hi, size_t n_polys,
+                     size_t poly_stride, hipStream_t s);
+
+// elementwise (poly.hip)
+void launch_add(const Context &c, uint64_t *a, const uint64_t *b, size_t Ql, size_t npolys, hipStream_t s);
+void launch_sub(const Context &c, uint64_t *a, const uint64_t *b, size_t Ql, size_t npolys, hipStream_t s);
+void launch_negate(const Context &c, uint64_t *a, size_t Ql, size_t npolys, hipStream_t s);
+void launch_mul_plain(const Context &c, uint64_t *ct, const uint64_t *pt, size_t Ql, size_t npolys, hipStream_t s);
+void launch_tensor(const Context &c, const uint64_t *a, const uint64_t *b, uint64_t *out3, size_t Ql, hipStream_t s);
+void launch_square(const Context &c, const uint64_t *a, uint64_t *out3, size_t Ql, hipStream_t s);
+void launch_rescale(const Context &c, const uint64_t *in, uint64_t *out, size_t Ql, size_t npolys, hipStream_t s);
+void launch_permute(const Context &c, const uint64_t *in, uint64_t *out, const uint32_t *table, size_t nlimbs,
+                    hipStream_t s);
+void launch_copy_limbs(const Context &c, const uint64_t *in, size_t in_limbs, uint64_t *out, size_t out_limbs,
+                       size_t npolys, hipStream_t s);
+// samplers
+void launch_sample_uniform(const Context &c, uint64_t seed, uint64_t *out, size_t Ql, bool withP, hipStream_t s);
+void launch_sample_gaussian_rns(const Context &c, uint64_t seed, uint64_t *out, size_t Ql, bool withP,
+                                hipStream_t s);
+void launch_signed_to_rns(const Context &c, const int8_t *coeffs, uint64_t *out, size_t Ql, bool withP,
+                          hipStream_t s);
+void launch_ks_key_digit(const Context &c, const uint64_t *sk, const uint64_t *newkey, uint64_t *b,
+                         const uint64_t *a, const uint64_t *e, size_t digit, hipStream_t s);
+void launch_encrypt_combine(const Context &c, const uint64_t *sk, const uint64_t *a, const uint64_t *e,
+                            const uint64_t *pt, uint64_t *c0, size_t Ql, hipStream_t s);
+void launch_decrypt(const Context &c, const uint64_t *ct, size_t size, const uint64_t *sk, uint64_t *pt,
+                    size_t Ql, hipStream_t s);
+void launch_mul_elementwise(const Context &c, const uint64_t *a, const uint64_t *b, uint64_t *out, size_t nlimbs,
+                            bool withP, size_t Ql, hipStream_t s);
+
+// key switching (keyswitch.hip)
+void keyswitch(const Context &c, const uint64_t *c2, const uint64_t *key, size_t Ql, uint64_t *ct0, uint64_t *ct1,
+               bool add_to_ct1, hipStream_t s);
+
+// encoder (encoder.hip)
+void encode(const Context &c, const double *re, const double *im, size_t n_values, size_t Ql, double scale,
+            uint64_t *out, hipStream_t s);
+void encode_scalar(const Context &c, double value, size_t Ql, double scale, uint64_t *out, hipStream_t s);
+void decode(const Context &c, const uint64_t *pt, size_t Ql, double scale, double *re, double *im, size_t n,
+            hipStream_t s);
+
+// MOAI matmul (matmul.hip)
+void ctpt_matmul(const Context &c, const std::vector<const Ciphertext *> &X, const double *W, size_t row_w,
+                 size_t col_w, const std::vector<Ciphertext *> &out, int variant, hipStream_t s);
+
+// timing hook for the dominant kernel of the last matmul (HIP events)
+float last_matmul_kernel_ms();
+
+}  // namespace shm

@@ -1,0 +1,243 @@
+// MOAI column-packed ciphertext x plaintext matmul
+// (Ct_pt_matrix_mul.cuh:18-185 ct_pt_matrix_mul_wo_pre):
+//   out_i = rescale( sum_{j < row_W} X_j (.) encode(W[j][i], X_j.scale) ),
+//   out_i.scale = X_0.scale.
+// encode() of a scalar fills every slot, so the plaintext is the constant
+// polynomial round(fl(W[j][i] * scale_j)) and, in NTT form, every slot holds
+// its residue: the contraction is, per limb l and coefficient column c,
+//   acc_l[i][c] = sum_j [w_ji]_{q_l} * X_j[l][c]  (mod q_l).
+//
+// Variants:
+//   MATMUL_PER_CT  - the reference's structure: for each (i, j) one constant
+//                    plaintext, one multiply_plain and one add over the whole
+//                    ciphertext, then a rescale per output.
+//   MATMUL_U64     - one batched contraction kernel (64-bit modular MACs with
+//                    128-bit accumulation) and one batched rescale.
+//   MATMUL_INT8    - the same contraction on INT8 MFMA: X limbs split into
+//                    8-bit digits, weights into balanced signed 8-bit digits,
+//                    exact int32 accumulation, recombined with 128-bit integer
+//                    arithmetic and one Barrett reduction (bit-identical).
+#include <cmath>
+#include <vector>
+
+#include "engine.h"
+
+namespace shm {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(size_t work) {
+    size_t g = (work + kBlock - 1) / kBlock;
+    if (g > (1u << 20)) g = 1u << 20;
+    return (unsigned)(g ? g : 1);
+}
+
+float g_last_ms = 0.f;
+
+// Weight residues: wres[l][j][i] = [round(fl(W[j][i] * scale_j))]_{q_l}.
+__global__ void k_weight_residues(const double *W, const double *scales, uint64_t *wres, const DevMod *mods,
+                                  size_t row_w, size_t col_w, size_t Ql) {
+    const size_t total = Ql * row_w * col_w;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t l = t / (row_w * col_w), r = t % (row_w * col_w), j = r / col_w, i = r % col_w;
+        const double v = round(W[j * col_w + i] * scales[j]);
+        const DevMod m = mods[l];
+        const bool neg = signbit(v);
+        const double a = fabs(v);
+        uint64_t res;
+        if (a < 18446744073709551616.0) {
+            res = reduce64((uint64_t)a, m);
+        } else {
+            const uint64_t lo = (uint64_t)fmod(a, 18446744073709551616.0);
+            const uint64_t hi = (uint64_t)(a / 18446744073709551616.0);
+            res = barrett128(hi, lo, m);
+        }
+        if (neg && res) res = m.q - res;
+        wres[t] = res;
+    }
+}
+
+// Fused contraction (64-bit VALU). Block = (limb l, poly p, 32-column tile);
+// X tile staged in LDS as [j][32]; 8 groups of 32 lanes each own col_w/8
+// outputs, 128-bit accumulators, one Barrett reduction per output.
+template <int OUT_PER_THREAD>
+__global__ __launch_bounds__(256) void k_contract_u64(const uint64_t *const *__restrict__ X,
+                                                      const uint64_t *__restrict__ wres,
+                                                      uint64_t *__restrict__ acc_out, const DevMod *__restrict__ mods,
+                                                      size_t row_w, size_t col_w, size_t Ql, size_t N) {
+    constexpr int TN = 32;
+    extern __shared__ uint64_t xs[];  // [row_w][TN]
+    const size_t tiles = N / TN;
+    const size_t b = blockIdx.x;
+    const size_t tile = b % tiles, lp = b / tiles;  // lp = p * Ql + l
+    const size_t l = lp % Ql;
+    const size_t off = lp * N + tile * TN;
+    for (size_t t = threadIdx.x; t < row_w * TN; t += blockDim.x) {
+        const size_t j = t / TN, n = t % TN;
+        xs[t] = X[j][off + n];
+    }
+    __syncthreads();
+    const DevMod m = mods[l];
+    const int n = threadIdx.x % TN;
+    const int grp = threadIdx.x / TN;  // 8 groups
+    const uint64_t *wl = wres + l * row_w * col_w;
+    for (size_t i0 = (size_t)grp * OUT_PER_THREAD; i0 < col_w; i0 += 8 * OUT_PER_THREAD) {
+        Acc128 acc[OUT_PER_THREAD];
+#pragma unroll
+        for (int k = 0; k < OUT_PER_THREAD; k++) acc[k] = Acc128{0, 0};
+        for (size_t j = 0; j < row_w; j++) {
+            const uint64_t x = xs[j * TN + n];
+#pragma unroll
+            for (int k = 0; k < OUT_PER_THREAD; k++)
+                if (i0 + k < col_w) mac128(acc[k], x, wl[j * col_w + i0 + k]);
+        }
+#pragma unroll
+        for (int k = 0; k < OUT_PER_THREAD; k++)
+            if (i0 + k < col_w) acc_out[(i0 + k) * 2 * Ql * N + off + n] = reduce128(acc[k], m);
+    }
+}
+
+// acc += x * w (constant) for the per-ciphertext reference structure.
+__global__ void k_mul_const_add(uint64_t *acc, const uint64_t *x, const uint64_t *pt, const DevMod *mods, size_t Ql,
+                                size_t N, size_t total, int first) {
+    const size_t S = Ql * N;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t r = t % S;
+        const DevMod m = mods[r / N];
+        const uint64_t v = mul_mod(x[t], pt[r], m);
+        acc[t] = first ? v : add_mod(acc[t], v, m.q);
+    }
+}
+
+// Batched rescale helpers over many ciphertexts (each [2][Ql][N]).
+__global__ void k_gather_last(const uint64_t *acc, uint64_t *last, size_t Ql, size_t N, size_t npolys) {
+    const size_t total = npolys * N;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t p = t / N, j = t % N;
+        last[t] = acc[(p * Ql + Ql - 1) * N + j];
+    }
+}
+
+__global__ void k_reduce_last(const uint64_t *last, uint64_t *delta, const DevMod *mods, size_t L1, size_t N,
+                              size_t npolys) {
+    const size_t total = npolys * L1 * N;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t p = t / (L1 * N), r = t % (L1 * N);
+        delta[t] = reduce64(last[p * N + r % N], mods[r / N]);
+    }
+}
+
+__global__ void k_rescale_scatter(const uint64_t *acc, const uint64_t *delta, uint64_t *const *outs,
+                                  const DevMod *mods, const uint64_t *inv, const uint64_t *inv_s, size_t Ql, size_t N,
+                                  size_t nct) {
+    const size_t L1 = Ql - 1, total = nct * 2 * L1 * N;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t ct = t / (2 * L1 * N), r = t % (2 * L1 * N), p = r / (L1 * N), r2 = r % (L1 * N), i = r2 / N;
+        const uint64_t q = mods[i].q;
+        const uint64_t d = sub_mod(acc[(ct * 2 + p) * Ql * N + r2], delta[t], q);
+        outs[ct][r] = mul_shoup(d, inv[i], inv_s[i], q);
+    }
+}
+
+void batched_rescale(const Context &c, const uint64_t *acc, const std::vector<Ciphertext *> &out, size_t Ql,
+                     hipStream_t s) {
+    const size_t N = c.N, L1 = Ql - 1, nct = out.size(), np = 2 * nct;
+    Scratch last(np * N * sizeof(uint64_t), s);
+    Scratch delta(np * L1 * N * sizeof(uint64_t), s);
+    k_gather_last<<<grid_for(np * N), kBlock, 0, s>>>(acc, last.as<uint64_t>(), Ql, N, np);
+    ntt_inverse(c, last.as<uint64_t>(), np, 1, L1, s);
+    k_reduce_last<<<grid_for(np * L1 * N), kBlock, 0, s>>>(last.as<uint64_t>(), delta.as<uint64_t>(), c.mods(), L1,
+                                                           N, np);
+    ntt_forward(c, delta.as<uint64_t>(), np, L1, 0, s);
+    std::vector<uint64_t *> ptrs(nct);
+    for (size_t i = 0; i < nct; i++) ptrs[i] = out[i]->data();
+    Scratch dptrs(nct * sizeof(uint64_t *), s);
+    SHM_HIP(hipMemcpyAsync(dptrs.as<void>(), ptrs.data(), nct * sizeof(uint64_t *), hipMemcpyHostToDevice, s));
+    const LevelConsts &L = c.level(Ql);
+    k_rescale_scatter<<<grid_for(nct * 2 * L1 * N), kBlock, 0, s>>>(
+        acc, delta.as<uint64_t>(), dptrs.as<uint64_t *>(), c.mods(), L.qlast_inv.as<uint64_t>(),
+        L.qlast_inv_s.as<uint64_t>(), Ql, N, nct);
+    SHM_HIP(hipGetLastError());
+    // the host vector of pointers must outlive the async copy
+    SHM_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+void ctpt_matmul_int8(const Context &c, const std::vector<const Ciphertext *> &X, const double *W, size_t row_w,
+                      size_t col_w, uint64_t *acc, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1);
+
+float last_matmul_kernel_ms() { return g_last_ms; }
+
+void ctpt_matmul(const Context &c, const std::vector<const Ciphertext *> &X, const double *W, size_t row_w,
+                 size_t col_w, const std::vector<Ciphertext *> &out, int variant, hipStream_t s) {
+    if (X.size() < row_w || out.size() != col_w || row_w == 0 || col_w == 0)
+        throw InvalidArgument("bad dimensions of X or W");
+    const size_t Ql = X[0]->Ql(), N = c.N, S = Ql * N;
+    if (Ql < 2) throw LogicError("end of modulus switching chain reached");
+    for (size_t j = 0; j < row_w; j++) {
+        if (X[j]->chain != X[0]->chain || X[j]->size != 2)
+            throw InvalidArgument("encrypted and plain parameter mismatch");
+    }
+    const double out_scale = X[0]->scale;
+    for (Ciphertext *o : out) o->resize(2, X[0]->chain + 1);
+
+    std::vector<double> scales(row_w);
+    for (size_t j = 0; j < row_w; j++) scales[j] = X[j]->scale;
+    hipEvent_t e0, e1;
+    SHM_HIP(hipEventCreate(&e0));
+    SHM_HIP(hipEventCreate(&e1));
+
+    if (variant == 1) {
+        // Reference structure: per output, per input: encode scalar, multiply_plain, add; rescale.
+        Scratch acc(2 * S * sizeof(uint64_t), s);
+        Scratch pt(S * sizeof(uint64_t), s);
+        SHM_HIP(hipEventRecord(e0, s));
+        for (size_t i = 0; i < col_w; i++) {
+            for (size_t j = 0; j < row_w; j++) {
+                encode_scalar(c, W[j * col_w + i], Ql, scales[j], pt.as<uint64_t>(), s);
+                k_mul_const_add<<<grid_for(2 * S), kBlock, 0, s>>>(acc.as<uint64_t>(), X[j]->data(),
+                                                                   pt.as<uint64_t>(), c.mods(), Ql, N, 2 * S,
+                                                                   j == 0 ? 1 : 0);
+            }
+            launch_rescale(c, acc.as<uint64_t>(), out[i]->data(), Ql, 2, s);
+        }
+        SHM_HIP(hipEventRecord(e1, s));
+    } else {
+        Scratch acc(col_w * 2 * S * sizeof(uint64_t), s);
+        if (variant == 2) {
+            ctpt_matmul_int8(c, X, W, row_w, col_w, acc.as<uint64_t>(), s, e0, e1);
+        } else {
+            Scratch dW(row_w * col_w * sizeof(double), s);
+            Scratch dscale(row_w * sizeof(double), s);
+            Scratch wres(Ql * row_w * col_w * sizeof(uint64_t), s);
+            Scratch xptr(row_w * sizeof(uint64_t *), s);
+            std::vector<const uint64_t *> xp(row_w);
+            for (size_t j = 0; j < row_w; j++) xp[j] = X[j]->data();
+            SHM_HIP(hipMemcpyAsync(dW.as<void>(), W, row_w * col_w * sizeof(double), hipMemcpyHostToDevice, s));
+            SHM_HIP(hipMemcpyAsync(dscale.as<void>(), scales.data(), row_w * sizeof(double), hipMemcpyHostToDevice, s));
+            SHM_HIP(hipMemcpyAsync(xptr.as<void>(), xp.data(), row_w * sizeof(uint64_t *), hipMemcpyHostToDevice, s));
+            k_weight_residues<<<grid_for(Ql * row_w * col_w), kBlock, 0, s>>>(
+                dW.as<double>(), dscale.as<double>(), wres.as<uint64_t>(), c.mods(), row_w, col_w, Ql);
+            const size_t blocks = 2 * Ql * (N / 32);
+            const size_t lds = row_w * 32 * sizeof(uint64_t);
+            if (lds > 64 * 1024) throw InvalidArgument("row_W too large for the 64-bit contraction");
+            SHM_HIP(hipEventRecord(e0, s));
+            k_contract_u64<8><<<(unsigned)blocks, 256, lds, s>>>(xptr.as<const uint64_t *>(), wres.as<uint64_t>(),
+                                                                 acc.as<uint64_t>(), c.mods(), row_w, col_w, Ql, N);
+            SHM_HIP(hipGetLastError());
+            SHM_HIP(hipEventRecord(e1, s));
+            SHM_HIP(hipStreamSynchronize(s));
+        }
+        batched_rescale(c, acc.as<uint64_t>(), out, Ql, s);
+    }
+    SHM_HIP(hipEventSynchronize(e1));
+    SHM_HIP(hipEventElapsedTime(&g_last_ms, e0, e1));
+    SHM_HIP(hipEventDestroy(e0));
+    SHM_HIP(hipEventDestroy(e1));
+    for (Ciphertext *o : out) o->scale = out_scale;
+}
+
+}  // namespace shm

@@ -1,0 +1,255 @@
+// INT8-MFMA contraction for the MOAI ct-pt matmul (bit-identical to the
+// 64-bit path).
+//
+// Per limb l (prime q) and coefficient column c:
+//   Y[i][c] = sum_j w_ij X_j[c]  (mod q),  w_ij = round(fl(W[j][i] * scale_j))
+// (X_j (.) [w]_q == X_j * w mod q, so the signed integer weight is shared by
+// all limbs). Split
+//   X = sum_a 256^a (x'_a + 128),  x'_a in [-128, 127]   (DX unsigned digits)
+//   w = sum_b 256^b w_b,           w_b in [-128, 127]    (DW balanced digits)
+// so Y = sum_d 256^d C_d + 128 * R * rowsum_i with C_d = sum_{a+b=d} W_b X'_a
+// and R = sum_a 256^a. Each W_b X'_a is a 16x16x64 int8 MFMA GEMM
+// accumulating straight into the int32 plane C_{a+b} (|C_d| < 2^24, exact).
+// The planes are recombined by Horner's rule in signed 128-bit integers and
+// reduced once with Barrett; the rowsum correction is added mod q.
+//
+// Mapping: block = 4 waves = 64 coefficient columns of one (poly, limb); a
+// wave owns 16 columns and keeps their X digits as MFMA B fragments in
+// registers (DX * KS fragments), then sweeps the output rows 16 at a time,
+// streaming the W digit fragments (A operand) from L2.
+#include <cmath>
+#include <vector>
+
+#include "engine.h"
+
+namespace shm {
+
+namespace {
+
+typedef int8_t i8x16 __attribute__((ext_vector_type(16)));
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(size_t work) {
+    size_t g = (work + kBlock - 1) / kBlock;
+    if (g > (1u << 20)) g = 1u << 20;
+    return (unsigned)(g ? g : 1);
+}
+
+__device__ __forceinline__ int64_t weight_int(const double *W, const double *scales, size_t col_w, size_t j,
+                                              size_t i) {
+    return (int64_t)round(W[j * col_w + i] * scales[j]);
+}
+
+// A fragments: frag[((b * MB + ib) * KS + ks) * 64 + lane] = 16 bytes,
+// byte e = digit b of w(row m = 16 ib + lane % 16, k = 64 ks + 16 (lane / 16) + e).
+__global__ void k_weight_frags(const double *W, const double *scales, i8x16 *frag, size_t row_w, size_t col_w,
+                               int DW, int MB, int KS) {
+    const size_t total = (size_t)DW * MB * KS * 64;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const int lane = (int)(t % 64);
+        size_t r = t / 64;
+        const int ks = (int)(r % KS);
+        r /= KS;
+        const int ib = (int)(r % MB);
+        const int b = (int)(r / MB);
+        const size_t m = 16 * (size_t)ib + lane % 16;
+        i8x16 v;
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const size_t k = 64 * (size_t)ks + 16 * (lane / 16) + e;
+            int8_t d = 0;
+            if (m < col_w && k < row_w) {
+                int64_t w = weight_int(W, scales, col_w, k, m);
+                int64_t dig = 0;
+                for (int bb = 0; bb <= b; bb++) {
+                    dig = w & 255;
+                    if (dig >= 128) dig -= 256;
+                    w = (w - dig) >> 8;
+                }
+                d = (int8_t)dig;
+            }
+            v[e] = d;
+        }
+        frag[t] = v;
+    }
+}
+
+// corr[l][i] = [128 R]_q * [sum_j w_ij]_q mod q
+__global__ void k_weight_corr(const double *W, const double *scales, uint64_t *corr, const DevMod *mods,
+                              size_t row_w, size_t col_w, size_t Ql, int DX) {
+    const size_t total = Ql * col_w;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t l = t / col_w, i = t % col_w;
+        const DevMod m = mods[l];
+        int64_t s = 0;
+        for (size_t j = 0; j < row_w; j++) s += weight_int(W, scales, col_w, j, i);
+        uint64_t rs = reduce64((uint64_t)(s < 0 ? -s : s), m);
+        if (s < 0 && rs) rs = m.q - rs;
+        uint64_t R = 0;
+        for (int a = 0; a < DX; a++) R = add_mod(mul_mod(R, 256, m), 1, m.q);
+        const uint64_t c = mul_mod(128 % m.q, R, m);
+        corr[t] = mul_mod(c, rs, m);
+    }
+}
+
+template <int DX, int DW, int KS>
+__global__ __launch_bounds__(256) void k_contract_int8(const uint64_t *const *__restrict__ X,
+                                                       const i8x16 *__restrict__ frag,
+                                                       const uint64_t *__restrict__ corr,
+                                                       uint64_t *__restrict__ acc_out,
+                                                       const DevMod *__restrict__ mods, int row_w, int col_w,
+                                                       int MB, size_t Ql, size_t N) {
+    constexpr int ND = DX + DW - 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const size_t tiles = N / 64;
+    const size_t tile = blockIdx.x % tiles, lp = blockIdx.x / tiles;  // lp = p * Ql + l
+    const size_t l = lp % Ql;
+    const size_t col = tile * 64 + 16 * wave + (lane & 15);
+    const size_t off = lp * N + col;
+    const DevMod m = mods[l];
+
+    // B fragments: lane holds X'_a[k = 64 ks + 16 (lane/16) + e][col]
+    i8x16 B[DX][KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const int k = 64 * ks + 16 * (lane >> 4) + e;
+            const uint64_t x = k < row_w ? X[k][off] : 0;
+#pragma unroll
+            for (int a = 0; a < DX; a++) B[a][ks][e] = (int8_t)((int)((x >> (8 * a)) & 255) - 128);
+        }
+    }
+    // rows of k >= row_w carry digit -128 in B; their A entries are 0 so they add nothing.
+
+    for (int ib = 0; ib < MB; ib++) {
+        i32x4 C[ND];
+#pragma unroll
+        for (int d = 0; d < ND; d++) C[d] = i32x4{0, 0, 0, 0};
+#pragma unroll
+        for (int b = 0; b < DW; b++) {
+#pragma unroll
+            for (int ks = 0; ks < KS; ks++) {
+                const i8x16 A = frag[(((size_t)b * MB + ib) * KS + ks) * 64 + lane];
+#pragma unroll
+                for (int a = 0; a < DX; a++) C[a + b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, B[a][ks], C[a + b], 0, 0, 0);
+            }
+        }
+        // lane holds rows 16 ib + 4 (lane/16) + r, column col
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int row = 16 * ib + 4 * (lane >> 4) + r;
+            // Horner in signed 128-bit: S = sum_d C_d 256^d
+            int64_t c = C[ND - 1][r];
+            uint64_t lo = (uint64_t)c, hi = (uint64_t)(c >> 63);
+#pragma unroll
+            for (int d = ND - 2; d >= 0; d--) {
+                hi = (hi << 8) | (lo >> 56);
+                lo <<= 8;
+                const int64_t cd = C[d][r];
+                const uint64_t add = (uint64_t)cd;
+                const uint64_t nlo = lo + add;
+                hi += (uint64_t)(cd >> 63) + (nlo < lo ? 1 : 0);
+                lo = nlo;
+            }
+            uint64_t res;
+            if ((int64_t)hi < 0) {
+                // |S| = -S
+                const uint64_t nlo = ~lo + 1;
+                const uint64_t nhi = ~hi + (nlo == 0 ? 1 : 0);
+                res = barrett128(nhi, nlo, m);
+                res = res ? m.q - res : 0;
+            } else {
+                res = barrett128(hi, lo, m);
+            }
+            if (row < col_w) {
+                res = add_mod(res, corr[l * col_w + row], m.q);
+                acc_out[(size_t)row * 2 * Ql * N + off] = res;
+            }
+        }
+    }
+}
+
+typedef void (*ContractFn)(const uint64_t *const *, const i8x16 *, const uint64_t *, uint64_t *, const DevMod *,
+                           int, int, int, size_t, size_t);
+
+ContractFn pick(int DX, int DW, int KS) {
+#define SHM_PICK(dx, dw)                                          \
+    if (DX == dx && DW == dw) {                                   \
+        if (KS == 1) return k_contract_int8<dx, dw, 1>;           \
+        if (KS == 2) return k_contract_int8<dx, dw, 2>;           \
+        if (KS == 3) return k_contract_int8<dx, dw, 3>;           \
+        if (KS == 4) return k_contract_int8<dx, dw, 4>;           \
+    }
+    SHM_PICK(7, 4)
+    SHM_PICK(7, 5)
+    SHM_PICK(7, 6)
+    SHM_PICK(8, 4)
+    SHM_PICK(8, 5)
+    SHM_PICK(8, 6)
+#undef SHM_PICK
+    return nullptr;
+}
+
+}  // namespace
+
+// Returns false when the shape or the weights fall outside the INT8 path.
+bool int8_supported(const Context &c, size_t Ql, const double *W, const double *scales, size_t row_w, size_t col_w,
+                    int *dx_out, int *dw_out) {
+    if (row_w > 256 || c.N % 64) return false;
+    int qbits = 0;
+    for (size_t l = 0; l < Ql; l++) {
+        int b = 64 - __builtin_clzll(c.q[l]);
+        qbits = std::max(qbits, b);
+    }
+    const int DX = (qbits + 7) / 8;
+    double mx = 0;
+    for (size_t j = 0; j < row_w; j++)
+        for (size_t i = 0; i < col_w; i++) mx = std::max(mx, std::fabs(std::round(W[j * col_w + i] * scales[j])));
+    int DW = 1;
+    // balanced digits: D digits cover |w| <= 127 * (256^D - 1) / 255
+    while (DW < 9 && mx > 127.0 * (std::pow(256.0, DW) - 1) / 255.0) DW++;
+    if (DW < 4) DW = 4;
+    if (!pick(DX, DW, (int)((row_w + 63) / 64))) return false;
+    *dx_out = DX;
+    *dw_out = DW;
+    return true;
+}
+
+void ctpt_matmul_int8(const Context &c, const std::vector<const Ciphertext *> &X, const double *W, size_t row_w,
+                      size_t col_w, uint64_t *acc, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+    const size_t Ql = X[0]->Ql(), N = c.N;
+    std::vector<double> scales(row_w);
+    for (size_t j = 0; j < row_w; j++) scales[j] = X[j]->scale;
+    int DX = 0, DW = 0;
+    if (!int8_supported(c, Ql, W, scales.data(), row_w, col_w, &DX, &DW))
+        throw InvalidArgument("weights or shape outside the INT8 MFMA contraction");
+    const int MB = (int)((col_w + 15) / 16), KS = (int)((row_w + 63) / 64);
+    Scratch dW(row_w * col_w * sizeof(double), s);
+    Scratch dscale(row_w * sizeof(double), s);
+    Scratch frag((size_t)DW * MB * KS * 64 * sizeof(i8x16), s);
+    Scratch corr(Ql * col_w * sizeof(uint64_t), s);
+    Scratch xptr(row_w * sizeof(uint64_t *), s);
+    std::vector<const uint64_t *> xp(row_w);
+    for (size_t j = 0; j < row_w; j++) xp[j] = X[j]->data();
+    SHM_HIP(hipMemcpyAsync(dW.as<void>(), W, row_w * col_w * sizeof(double), hipMemcpyHostToDevice, s));
+    SHM_HIP(hipMemcpyAsync(dscale.as<void>(), scales.data(), row_w * sizeof(double), hipMemcpyHostToDevice, s));
+    SHM_HIP(hipMemcpyAsync(xptr.as<void>(), xp.data(), row_w * sizeof(uint64_t *), hipMemcpyHostToDevice, s));
+    k_weight_frags<<<grid_for((size_t)DW * MB * KS * 64), kBlock, 0, s>>>(
+        dW.as<double>(), dscale.as<double>(), frag.as<i8x16>(), row_w, col_w, DW, MB, KS);
+    k_weight_corr<<<grid_for(Ql * col_w), kBlock, 0, s>>>(dW.as<double>(), dscale.as<double>(), corr.as<uint64_t>(),
+                                                          c.mods(), row_w, col_w, Ql, DX);
+    SHM_HIP(hipGetLastError());
+    const unsigned blocks = (unsigned)(2 * Ql * (N / 64));
+    ContractFn fn = pick(DX, DW, KS);
+    SHM_HIP(hipEventRecord(ev0, s));
+    hipLaunchKernelGGL(fn, dim3(blocks), dim3(256), 0, s, xptr.as<const uint64_t *>(), frag.as<i8x16>(),
+                       corr.as<uint64_t>(), acc, c.mods(), (int)row_w, (int)col_w, MB, Ql, N);
+    SHM_HIP(hipGetLastError());
+    SHM_HIP(hipEventRecord(ev1, s));
+    SHM_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace shm

@@ -1,0 +1,357 @@
+// Negacyclic NTT / iNTT on uint64 RNS limbs for CDNA4.
+//
+// Transform (same convention as the reference, fntt_2d.cu / intt_2d.cu):
+//   forward Cooley-Tukey, stage s (m = 2^s groups, distance t = N/2^(s+1)),
+//   group i uses tw[m + i] with tw[brv(k)] = psi^k; the output is the
+//   evaluation at psi^(2 brv(j) + 1) in slot j, fully reduced to [0, q).
+//   inverse Gentleman-Sande with itw[brv(k)] = psi^-k, then x N^-1.
+//
+// Decomposition: N = N1 * N2. The first log N1 stages only pair elements whose
+// indices differ by multiples of N2, so each "column" {c + N2 r} is an
+// independent N1-point transform (phase A); the last log N2 stages act on
+// contiguous blocks of N2 elements (phase B). Each phase stages a 4096-element
+// tile in LDS (transposed with one word of padding so the 64 lanes hit distinct
+// banks), then runs rounds of up to 4 butterfly stages in registers (radix-16
+// units). A sub-transform whose first-stage group id is G uses twiddle
+// tw[(G << u) + i'] at its local stage u for local group i'.
+//
+// Lazy ranges (Harvey): forward keeps [0, 4q) between stages, inverse [0, 2q);
+// primes are < 2^61 so nothing wraps.
+#include "engine.h"
+
+namespace shm {
+
+struct LimbMap {
+    uint32_t n_active;    // transformed limbs per polynomial
+    uint32_t lo;          // first limb
+    uint32_t skip_lo;     // limbs >= skip_lo are shifted by skip_width
+    uint32_t skip_width;
+    uint32_t Ql;          // limbs >= Ql use special primes nQ + (k - Ql)
+    uint32_t nQ;
+    uint32_t first_prime; // prime of limb k < Ql is first_prime + k
+    uint64_t poly_stride; // words between polynomials
+};
+
+__device__ __forceinline__ void limb_of(const LimbMap &m, uint32_t z, uint32_t &poly, uint32_t &k,
+                                        uint32_t &prime) {
+    poly = z / m.n_active;
+    const uint32_t j = z - poly * m.n_active;
+    k = m.lo + j;
+    if (k >= m.skip_lo) k += m.skip_width;
+    prime = k < m.Ql ? m.first_prime + k : m.nQ + (k - m.Ql);
+}
+
+namespace {
+
+constexpr int kTile = 4096;
+constexpr int kThreads = 256;
+
+template <int R>
+__device__ __forceinline__ void ct_unit(uint64_t *x, uint32_t G, const uint64_t *__restrict__ tw,
+                                        const uint64_t *__restrict__ tws, uint64_t q) {
+    const uint64_t q2 = q << 1;
+#pragma unroll
+    for (int u = 0; u < R; u++) {
+        const int h = 1 << (R - 1 - u);
+#pragma unroll
+        for (int l = 0; l < (1 << R); l++) {
+            if ((l & h) == 0) {
+                const uint32_t ti = (G << u) + (uint32_t)(l >> (R - u));
+                const uint64_t w = tw[ti], ws = tws[ti];
+                uint64_t a = x[l];
+                a = a >= q2 ? a - q2 : a;
+                const uint64_t t = mul_shoup_lazy(x[l + h], w, ws, q);
+                x[l] = a + t;
+                x[l + h] = a + q2 - t;
+            }
+        }
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void gs_unit(uint64_t *x, uint32_t G, const uint64_t *__restrict__ tw,
+                                        const uint64_t *__restrict__ tws, uint64_t q) {
+    const uint64_t q2 = q << 1;
+#pragma unroll
+    for (int u = R - 1; u >= 0; u--) {
+        const int h = 1 << (R - 1 - u);
+#pragma unroll
+        for (int l = 0; l < (1 << R); l++) {
+            if ((l & h) == 0) {
+                const uint32_t ti = (G << u) + (uint32_t)(l >> (R - u));
+                const uint64_t w = tw[ti], ws = tws[ti];
+                const uint64_t a = x[l], b = x[l + h];
+                uint64_t s = a + b;
+                s = s >= q2 ? s - q2 : s;
+                x[l] = s;
+                x[l + h] = mul_shoup_lazy(a + q2 - b, w, ws, q);
+            }
+        }
+    }
+}
+
+// Round split of LOGT stages into chunks of at most 4.
+__host__ __device__ constexpr int n_rounds(int logt) { return (logt + 3) / 4; }
+__host__ __device__ constexpr int round_len(int logt, int r) {
+    // distribute as evenly as possible, larger rounds first
+    return logt / n_rounds(logt) + (r < logt % n_rounds(logt) ? 1 : 0);
+}
+__host__ __device__ constexpr int round_start(int logt, int r) {
+    int s = 0;
+    for (int i = 0; i < r; i++) s += round_len(logt, i);
+    return s;
+}
+
+// One round over the LDS tile: SUBS independent sub-transforms of size 2^LOGT
+// (row stride TP words). Sub-transform `sub` has first-stage group id
+// gbase(sub) at local stage 0.
+template <int LOGT, int R, bool FWD>
+__device__ __forceinline__ void lds_round(uint64_t *lds, int s0, uint32_t gmul, uint32_t gadd_step, uint32_t gadd0,
+                                          const uint64_t *__restrict__ tw, const uint64_t *__restrict__ tws,
+                                          uint64_t q) {
+    constexpr int T = 1 << LOGT;
+    constexpr int SUBS = kTile / T;
+    constexpr int TP = T + 1;
+    const int T0 = T >> s0;          // size of the sub-transforms at this round
+    const int ups = T0 >> R;         // units per sub-transform
+    const int units = (T >> R) * SUBS;
+    for (int w = threadIdx.x; w < units; w += kThreads) {
+        const int sub = w % SUBS;
+        const int w2 = w / SUBS;
+        const int kk = w2 / ups;
+        const int o = w2 - kk * ups;
+        // group id of the whole phase for this sub-transform, shifted by s0
+        const uint32_t G = ((gmul + gadd0 + gadd_step * (uint32_t)sub) << s0) + (uint32_t)kk;
+        uint64_t *p = lds + sub * TP + kk * T0 + o;
+        uint64_t x[1 << R];
+#pragma unroll
+        for (int l = 0; l < (1 << R); l++) x[l] = p[l * ups];
+        if (FWD)
+            ct_unit<R>(x, G, tw, tws, q);
+        else
+            gs_unit<R>(x, G, tw, tws, q);
+#pragma unroll
+        for (int l = 0; l < (1 << R); l++) p[l * ups] = x[l];
+    }
+}
+
+template <int LOGT, bool FWD>
+__device__ __forceinline__ void run_rounds(uint64_t *lds, uint32_t gmul, uint32_t gadd_step, uint32_t gadd0,
+                                           const uint64_t *tw, const uint64_t *tws, uint64_t q) {
+    constexpr int NR = n_rounds(LOGT);
+    if (FWD) {
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int R = round_len(LOGT, r), s0 = round_start(LOGT, r);
+            switch (R) {
+                case 4: lds_round<LOGT, 4, true>(lds, s0, gmul, gadd_step, gadd0, tw, tws, q); break;
+                case 3: lds_round<LOGT, 3, true>(lds, s0, gmul, gadd_step, gadd0, tw, tws, q); break;
+                case 2: lds_round<LOGT, 2, true>(lds, s0, gmul, gadd_step, gadd0, tw, tws, q); break;
+                default: lds_round<LOGT, 1, true>(lds, s0, gmul, gadd_step, gadd0, tw, tws, q); break;
+            }
+            __syncthreads();
+        }
+    } else {
+#pragma unroll
+        for (int r = NR - 1; r >= 0; r--) {
+            const int R = round_len(LOGT, r), s0 = round_start(LOGT, r);
+            switch (R) {
+                case 4: lds_round<LOGT, 4, false>(lds, s0, gmul, gadd_step, gadd0, tw, tws, q); break;
+                case 3: lds_round<LOGT, 3, false>(lds, s0, gmul, gadd_step, gadd0, tw, tws, q); break;
+                case 2: lds_round<LOGT, 2, false>(lds, s0, gmul, gadd_step, gadd0, tw, tws, q); break;
+                default: lds_round<LOGT, 1, false>(lds, s0, gmul, gadd_step, gadd0, tw, tws, q); break;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Phase A (columns, first log N1 stages). LOGT = log N1.
+// FWD: forward stages 0..logN1-1 (output stays lazy [0,4q)).
+// !FWD: inverse stages logN1-1..0, then x N^-1, output canonical.
+template <int LOGT, bool FWD>
+__global__ __launch_bounds__(kThreads) void k_ntt_cols(uint64_t *__restrict__ data, LimbMap mp,
+                                                       const DevMod *__restrict__ mods,
+                                                       const uint64_t *__restrict__ tw_all,
+                                                       const uint64_t *__restrict__ tws_all,
+                                                       const uint64_t *__restrict__ ninv,
+                                                       const uint64_t *__restrict__ ninv_s, int logN) {
+    constexpr int T = 1 << LOGT;
+    constexpr int COLS = kTile / T;
+    constexpr int TP = T + 1;
+    __shared__ uint64_t lds[COLS * TP];
+    const size_t N = size_t(1) << logN;
+    const int N2 = (int)(N >> LOGT);
+    const int groups = N2 / COLS;
+    const uint32_t z = blockIdx.x / groups;
+    const int c0 = (blockIdx.x % groups) * COLS;
+    uint32_t poly, k, prime;
+    limb_of(mp, z, poly, k, prime);
+    uint64_t *base = data + poly * mp.poly_stride + (size_t)k * N + c0;
+    const uint64_t q = mods[prime].q;
+    const uint64_t *tw = tw_all + (size_t)prime * N;
+    const uint64_t *tws = tws_all + (size_t)prime * N;
+    for (int i = threadIdx.x; i < kTile; i += kThreads) {
+        const int c = i % COLS, r = i / COLS;
+        lds[c * TP + r] = base[(size_t)r * N2 + c];
+    }
+    __syncthreads();
+    // group id of every column at local stage 0 is 1 (phase A starts at s = 0)
+    run_rounds<LOGT, FWD>(lds, 1u, 0u, 0u, tw, tws, q);
+    if (FWD) {
+        for (int i = threadIdx.x; i < kTile; i += kThreads) {
+            const int c = i % COLS, r = i / COLS;
+            base[(size_t)r * N2 + c] = lds[c * TP + r];
+        }
+    } else {
+        const uint64_t ni = ninv[prime], nis = ninv_s[prime];
+        for (int i = threadIdx.x; i < kTile; i += kThreads) {
+            const int c = i % COLS, r = i / COLS;
+            base[(size_t)r * N2 + c] = mul_shoup(lds[c * TP + r], ni, nis, q);
+        }
+    }
+}
+
+// Phase B (contiguous blocks of N2, last log N2 stages). LOGT = log N2.
+// FWD: input lazy [0,4q), output canonical. !FWD: input canonical (< 2q
+// accepted), output lazy [0,2q).
+template <int LOGT, bool FWD>
+__global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ data, LimbMap mp,
+                                                         const DevMod *__restrict__ mods,
+                                                         const uint64_t *__restrict__ tw_all,
+                                                         const uint64_t *__restrict__ tws_all, int logN) {
+    constexpr int T = 1 << LOGT;
+    constexpr int BLKS = kTile / T;
+    constexpr int TP = T + 1;
+    __shared__ uint64_t lds[BLKS * TP];
+    const size_t N = size_t(1) << logN;
+    const int tiles = (int)(N / kTile);
+    const uint32_t z = blockIdx.x / tiles;
+    const int t0 = (blockIdx.x % tiles) * kTile;
+    uint32_t poly, k, prime;
+    limb_of(mp, z, poly, k, prime);
+    uint64_t *base = data + poly * mp.poly_stride + (size_t)k * N + t0;
+    const uint64_t q = mods[prime].q;
+    const uint64_t *tw = tw_all + (size_t)prime * N;
+    const uint64_t *tws = tws_all + (size_t)prime * N;
+    for (int i = threadIdx.x; i < kTile; i += kThreads) {
+        const int b = i / T, r = i % T;
+        lds[b * TP + r] = base[i];
+    }
+    __syncthreads();
+    // block b (global index t0/T + b) has group id N1 + b at its first stage
+    const uint32_t N1 = (uint32_t)(N >> LOGT);
+    run_rounds<LOGT, FWD>(lds, N1, 1u, (uint32_t)(t0 / T), tw, tws, q);
+    if (FWD) {
+        const uint64_t q2 = q << 1;
+        for (int i = threadIdx.x; i < kTile; i += kThreads) {
+            const int b = i / T, r = i % T;
+            uint64_t v = lds[b * TP + r];
+            v = v >= q2 ? v - q2 : v;
+            base[i] = v >= q ? v - q : v;
+        }
+    } else {
+        for (int i = threadIdx.x; i < kTile; i += kThreads) {
+            const int b = i / T, r = i % T;
+            base[i] = lds[b * TP + r];
+        }
+    }
+}
+
+template <bool FWD>
+void launch_ntt(const Context &c, uint64_t *data, const LimbMap &mp, size_t total_limbs, hipStream_t s) {
+    if (total_limbs == 0) return;
+    const int logN = c.logN;
+    const int log2b = (logN + 1) / 2;  // blocks (contiguous) get the larger half
+    const int log1 = logN - log2b;
+    const size_t N = c.N;
+    if (N < (size_t)kTile) throw InvalidArgument("NTT requires N >= 4096");
+    const unsigned grid_cols = (unsigned)(total_limbs * ((N >> log1) / (kTile >> log1)));
+    const unsigned grid_blk = (unsigned)(total_limbs * (N / kTile));
+    const DevMod *mods = c.mods();
+#define SHM_COLS(L)                                                                                         \
+    case L:                                                                                                 \
+        if (FWD)                                                                                            \
+            k_ntt_cols<L, true><<<grid_cols, kThreads, 0, s>>>(data, mp, mods, c.tw.as<uint64_t>(),         \
+                                                               c.tw_s.as<uint64_t>(), c.ninv.as<uint64_t>(), \
+                                                               c.ninv_s.as<uint64_t>(), logN);             \
+        else                                                                                                \
+            k_ntt_cols<L, false><<<grid_cols, kThreads, 0, s>>>(data, mp, mods, c.itw.as<uint64_t>(),       \
+                                                                c.itw_s.as<uint64_t>(), c.ninv.as<uint64_t>(), \
+                                                                c.ninv_s.as<uint64_t>(), logN);            \
+        break;
+#define SHM_BLKS(L)                                                                                          \
+    case L:                                                                                                  \
+        if (FWD)                                                                                             \
+            k_ntt_blocks<L, true><<<grid_blk, kThreads, 0, s>>>(data, mp, mods, c.tw.as<uint64_t>(),         \
+                                                                c.tw_s.as<uint64_t>(), logN);                \
+        else                                                                                                 \
+            k_ntt_blocks<L, false><<<grid_blk, kThreads, 0, s>>>(data, mp, mods, c.itw.as<uint64_t>(),       \
+                                                                 c.itw_s.as<uint64_t>(), logN);              \
+        break;
+    auto cols = [&]() {
+        switch (log1) {
+            SHM_COLS(6)
+            SHM_COLS(7)
+            SHM_COLS(8)
+            default: throw InvalidArgument("unsupported ring degree for NTT");
+        }
+    };
+    auto blks = [&]() {
+        switch (log2b) {
+            SHM_BLKS(6)
+            SHM_BLKS(7)
+            SHM_BLKS(8)
+            SHM_BLKS(9)
+            default: throw InvalidArgument("unsupported ring degree for NTT");
+        }
+    };
+#undef SHM_COLS
+#undef SHM_BLKS
+    if (FWD) {
+        cols();
+        blks();
+    } else {
+        blks();
+        cols();
+    }
+    SHM_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void ntt_forward(const Context &c, uint64_t *data, size_t n_polys, size_t n_limbs, size_t first_prime,
+                 hipStream_t s) {
+    LimbMap mp{(uint32_t)n_limbs, 0, 0xffffffffu, 0, 0xffffffffu, (uint32_t)c.nQ, (uint32_t)first_prime,
+               (uint64_t)n_limbs * c.N};
+    launch_ntt<true>(c, data, mp, n_polys * n_limbs, s);
+}
+
+void ntt_inverse(const Context &c, uint64_t *data, size_t n_polys, size_t n_limbs, size_t first_prime,
+                 hipStream_t s) {
+    LimbMap mp{(uint32_t)n_limbs, 0, 0xffffffffu, 0, 0xffffffffu, (uint32_t)c.nQ, (uint32_t)first_prime,
+               (uint64_t)n_limbs * c.N};
+    launch_ntt<false>(c, data, mp, n_polys * n_limbs, s);
+}
+
+void ntt_forward_qlp(const Context &c, uint64_t *data, size_t Ql, size_t lo, size_t hi, size_t skip_lo,
+                     size_t skip_hi, size_t n_polys, size_t poly_stride, hipStream_t s) {
+    size_t active = hi - lo;
+    uint32_t sl = 0xffffffffu, sw = 0;
+    if (skip_hi > skip_lo && skip_lo >= lo && skip_hi <= hi) {
+        active -= skip_hi - skip_lo;
+        sl = (uint32_t)skip_lo;
+        sw = (uint32_t)(skip_hi - skip_lo);
+    }
+    LimbMap mp{(uint32_t)active, (uint32_t)lo, sl, sw, (uint32_t)Ql, (uint32_t)c.nQ, 0, (uint64_t)poly_stride};
+    launch_ntt<true>(c, data, mp, n_polys * active, s);
+}
+
+void ntt_inverse_qlp(const Context &c, uint64_t *data, size_t Ql, size_t lo, size_t hi, size_t n_polys,
+                     size_t poly_stride, hipStream_t s) {
+    LimbMap mp{(uint32_t)(hi - lo), (uint32_t)lo, 0xffffffffu, 0, (uint32_t)Ql, (uint32_t)c.nQ, 0,
+               (uint64_t)poly_stride};
+    launch_ntt<false>(c, data, mp, n_polys * (hi - lo), s);
+}
+
+}  // namespace shm

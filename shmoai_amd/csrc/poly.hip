@@ -1,0 +1,337 @@
+// Elementwise RNS kernels, rescale, Galois permutation, samplers and key/
+// encryption combiners. Grid-stride over [poly][limb][N]; each thread handles
+// two consecutive coefficients (16-byte loads/stores).
+#include "engine.h"
+
+namespace shm {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(size_t work) {
+    size_t g = (work + kBlock - 1) / kBlock;
+    if (g > (1u << 20)) g = 1u << 20;
+    return (unsigned)(g ? g : 1);
+}
+
+__global__ void k_add(uint64_t *a, const uint64_t *b, const DevMod *mods, size_t Ql, size_t N, size_t total) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t limb = (t / N) % Ql;
+        a[t] = add_mod(a[t], b[t], mods[limb].q);
+    }
+}
+
+__global__ void k_sub(uint64_t *a, const uint64_t *b, const DevMod *mods, size_t Ql, size_t N, size_t total) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t limb = (t / N) % Ql;
+        a[t] = sub_mod(a[t], b[t], mods[limb].q);
+    }
+}
+
+__global__ void k_negate(uint64_t *a, const DevMod *mods, size_t Ql, size_t N, size_t total) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t limb = (t / N) % Ql;
+        a[t] = neg_mod(a[t], mods[limb].q);
+    }
+}
+
+// polymath.cu:161 multiply_rns_poly: ct (.) pt per limb, pt broadcast over polys.
+__global__ void k_mul_plain(uint64_t *ct, const uint64_t *pt, const DevMod *mods, size_t Ql, size_t N, size_t total) {
+    const size_t S = Ql * N;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t r = t % S;
+        const DevMod m = mods[r / N];
+        ct[t] = mul_mod(ct[t], pt[r], m);
+    }
+}
+
+// polymath.cu:468-503 tensor_prod_2x2_rns_poly (Karatsuba middle term).
+__global__ void k_tensor(const uint64_t *a, const uint64_t *b, uint64_t *out, const DevMod *mods, size_t Ql,
+                         size_t N) {
+    const size_t S = Ql * N;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < S; t += (size_t)gridDim.x * blockDim.x) {
+        const DevMod m = mods[t / N];
+        const uint64_t a0 = a[t], a1 = a[t + S], b0 = b[t], b1 = b[t + S];
+        const uint64_t d0 = mul_mod(a0, b0, m), d2 = mul_mod(a1, b1, m);
+        uint64_t d1 = mul_mod(a0 + a1, b0 + b1, m);
+        d1 = d1 + 2 * m.q - d0 - d2;
+        d1 = csub(d1, m.q);
+        d1 = csub(d1, m.q);
+        out[t] = d0;
+        out[t + S] = d1;
+        out[t + 2 * S] = d2;
+    }
+}
+
+// polymath.cu:505-550 tensor_square_2x2_rns_poly.
+__global__ void k_square(const uint64_t *a, uint64_t *out, const DevMod *mods, size_t Ql, size_t N) {
+    const size_t S = Ql * N;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < S; t += (size_t)gridDim.x * blockDim.x) {
+        const DevMod m = mods[t / N];
+        const uint64_t c0 = a[t], c1 = a[t + S];
+        const uint64_t d0 = mul_mod(c0, c0, m), d2 = mul_mod(c1, c1, m);
+        const uint64_t lo = c0 * c1, hi = __umul64hi(c0, c1);
+        const uint64_t d1 = barrett128((hi << 1) | (lo >> 63), lo << 1, m);
+        out[t] = d0;
+        out[t + S] = d1;
+        out[t + 2 * S] = d2;
+    }
+}
+
+// rns.cu:1125-1137: [c_last]_{q_j} for j < Ql-1 (c_last in coefficient form).
+__global__ void k_rescale_reduce(const uint64_t *last, uint64_t *out, const DevMod *mods, size_t L1, size_t N,
+                                 size_t npolys) {
+    const size_t total = npolys * L1 * N;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t p = t / (L1 * N), r = t % (L1 * N);
+        out[t] = reduce64(last[p * N + r % N], mods[r / N]);
+    }
+}
+
+// rns.cu:1138-1155: (c_j - delta_j) * q_last^-1 mod q_j, written into delta.
+__global__ void k_rescale_combine(const uint64_t *in, uint64_t *delta, const DevMod *mods, const uint64_t *inv,
+                                  const uint64_t *inv_s, size_t Ql, size_t N, size_t npolys) {
+    const size_t L1 = Ql - 1, total = npolys * L1 * N;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t p = t / (L1 * N), r = t % (L1 * N), i = r / N;
+        const uint64_t q = mods[i].q;
+        const uint64_t d = sub_mod(in[p * Ql * N + r], delta[t], q);
+        delta[t] = mul_shoup(d, inv[i], inv_s[i], q);
+    }
+}
+
+// galois.cu:8-20 apply_galois_ntt_permutation.
+__global__ void k_permute(const uint64_t *in, uint64_t *out, const uint32_t *table, size_t N, size_t total) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t limb = t / N;
+        out[t] = in[limb * N + table[t % N]];
+    }
+}
+
+__global__ void k_copy_limbs(const uint64_t *in, size_t in_limbs, uint64_t *out, size_t out_limbs, size_t N,
+                             size_t npolys) {
+    const size_t total = npolys * out_limbs * N;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t p = t / (out_limbs * N), r = t % (out_limbs * N);
+        out[t] = in[p * in_limbs * N + r];
+    }
+}
+
+// ---- samplers (SplitMix64 counters; identical streams on host and device) --
+__device__ __forceinline__ uint64_t splitmix64(uint64_t key, uint64_t ctr) {
+    uint64_t z = key + (ctr + 1) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ uint32_t limb_prime(size_t limb, size_t Ql, size_t nQ) {
+    return (uint32_t)(limb < Ql ? limb : nQ + (limb - Ql));
+}
+
+// Uniform residue per (prime, coefficient): (draw0 * 2^64 + draw1) mod q.
+__global__ void k_sample_uniform(uint64_t seed, uint64_t *out, const DevMod *mods, size_t Ql, size_t nQ, size_t N,
+                                 size_t total) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t limb = t / N, j = t % N;
+        const uint32_t pi = limb_prime(limb, Ql, nQ);
+        const uint64_t ctr = 2 * ((uint64_t)pi * N + j);
+        out[t] = barrett128(splitmix64(seed, ctr), splitmix64(seed, ctr + 1), mods[pi]);
+    }
+}
+
+// Discrete Gaussian sigma = 3.2 via its cumulative table (|e| <= 19): the
+// 63 high bits of a draw select the magnitude, bit 0 the sign.
+__constant__ uint64_t kCDT[19] = {
+    0x0ff52b40a5917f1dULL, 0x2e5a25d4bf0e400eULL, 0x489ae26955b04bd6ULL, 0x5d2bc20f621bf185ULL,
+    0x6bc8694c3cc80ff4ULL, 0x7532d89ac6ba7dceULL, 0x7ab396cb74436798ULL, 0x7d9e4e916643eb07ULL,
+    0x7f05495819eb2051ULL, 0x7fa1ce9c0039a957ULL, 0x7fdfb3f212e8c4e8ULL, 0x7ff5e6f9d2314fccULL,
+    0x7ffd1f97bc4406a2ULL, 0x7fff40fa0088d11dULL, 0x7fffd2e835e1c57dULL, 0x7ffff6524386ff1eULL,
+    0x7ffffe1db4769da5ULL, 0x7fffffac0a1dcb08ULL, 0x7ffffff428673853ULL};
+
+__global__ void k_sample_gaussian(uint64_t seed, uint64_t *out, const DevMod *mods, size_t Ql, size_t nQ, size_t N,
+                                  size_t total) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t limb = t / N, j = t % N;
+        const uint64_t u = splitmix64(seed, j), v = u >> 1;
+        uint32_t mag = 0;
+#pragma unroll
+        for (int i = 0; i < 19; i++) mag += v >= kCDT[i] ? 1u : 0u;
+        const uint64_t q = mods[limb_prime(limb, Ql, nQ)].q;
+        out[t] = (u & 1) ? (mag ? q - mag : 0) : mag;
+    }
+}
+
+__global__ void k_signed_to_rns(const int8_t *coeffs, uint64_t *out, const DevMod *mods, size_t Ql, size_t nQ,
+                                size_t N, size_t total) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t limb = t / N, j = t % N;
+        const int64_t v = coeffs[j];
+        const uint64_t q = mods[limb_prime(limb, Ql, nQ)].q;
+        out[t] = v >= 0 ? (uint64_t)v : q - (uint64_t)(-v);
+    }
+}
+
+// secretkey.cu:328-373: b = -(a s + e) (+ P * newkey on the digit's limbs), over QP.
+__global__ void k_ks_key_digit(const uint64_t *sk, const uint64_t *newkey, uint64_t *b, const uint64_t *a,
+                               const uint64_t *e, const DevMod *mods, const uint64_t *pmod, const uint64_t *pmod_s,
+                               size_t dlo, size_t dhi, size_t N, size_t total) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t limb = t / N;
+        const DevMod m = mods[limb];
+        uint64_t v = neg_mod(add_mod(mul_mod(a[t], sk[t], m), e[t], m.q), m.q);
+        if (limb >= dlo && limb < dhi) v = add_mod(v, mul_shoup(newkey[t], pmod[limb], pmod_s[limb], m.q), m.q);
+        b[t] = v;
+    }
+}
+
+// secretkey.cu:260-326 + add pt: c0 = -(a s + e) + m.
+__global__ void k_encrypt_combine(const uint64_t *sk, const uint64_t *a, const uint64_t *e, const uint64_t *pt,
+                                  uint64_t *c0, const DevMod *mods, size_t N, size_t total) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const DevMod m = mods[t / N];
+        const uint64_t v = neg_mod(add_mod(mul_mod(a[t], sk[t], m), e[t], m.q), m.q);
+        c0[t] = add_mod(v, pt[t], m.q);
+    }
+}
+
+// secretkey.cu:725-766: m = c0 + c1 s (+ c2 s^2).
+__global__ void k_decrypt(const uint64_t *ct, size_t size, const uint64_t *sk, uint64_t *pt, const DevMod *mods,
+                          size_t N, size_t S) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < S; t += (size_t)gridDim.x * blockDim.x) {
+        const DevMod m = mods[t / N];
+        const uint64_t s = sk[t];
+        uint64_t v = add_mod(ct[t], mul_mod(ct[S + t], s, m), m.q);
+        if (size > 2) v = add_mod(v, mul_mod(ct[2 * S + t], mul_mod(s, s, m), m), m.q);
+        pt[t] = v;
+    }
+}
+
+__global__ void k_mul_elementwise(const uint64_t *a, const uint64_t *b, uint64_t *out, const DevMod *mods, size_t Ql,
+                                  size_t nQ, size_t N, size_t total) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const DevMod m = mods[limb_prime(t / N, Ql, nQ)];
+        out[t] = mul_mod(a[t], b[t], m);
+    }
+}
+
+}  // namespace
+
+void launch_add(const Context &c, uint64_t *a, const uint64_t *b, size_t Ql, size_t npolys, hipStream_t s) {
+    const size_t total = npolys * Ql * c.N;
+    k_add<<<grid_for(total), kBlock, 0, s>>>(a, b, c.mods(), Ql, c.N, total);
+    SHM_HIP(hipGetLastError());
+}
+
+void launch_sub(const Context &c, uint64_t *a, const uint64_t *b, size_t Ql, size_t npolys, hipStream_t s) {
+    const size_t total = npolys * Ql * c.N;
+    k_sub<<<grid_for(total), kBlock, 0, s>>>(a, b, c.mods(), Ql, c.N, total);
+    SHM_HIP(hipGetLastError());
+}
+
+void launch_negate(const Context &c, uint64_t *a, size_t Ql, size_t npolys, hipStream_t s) {
+    const size_t total = npolys * Ql * c.N;
+    k_negate<<<grid_for(total), kBlock, 0, s>>>(a, c.mods(), Ql, c.N, total);
+    SHM_HIP(hipGetLastError());
+}
+
+void launch_mul_plain(const Context &c, uint64_t *ct, const uint64_t *pt, size_t Ql, size_t npolys, hipStream_t s) {
+    const size_t total = npolys * Ql * c.N;
+    k_mul_plain<<<grid_for(total), kBlock, 0, s>>>(ct, pt, c.mods(), Ql, c.N, total);
+    SHM_HIP(hipGetLastError());
+}
+
+void launch_tensor(const Context &c, const uint64_t *a, const uint64_t *b, uint64_t *out3, size_t Ql, hipStream_t s) {
+    k_tensor<<<grid_for(Ql * c.N), kBlock, 0, s>>>(a, b, out3, c.mods(), Ql, c.N);
+    SHM_HIP(hipGetLastError());
+}
+
+void launch_square(const Context &c, const uint64_t *a, uint64_t *out3, size_t Ql, hipStream_t s) {
+    k_square<<<grid_for(Ql * c.N), kBlock, 0, s>>>(a, out3, c.mods(), Ql, c.N);
+    SHM_HIP(hipGetLastError());
+}
+
+void launch_rescale(const Context &c, const uint64_t *in, uint64_t *out, size_t Ql, size_t npolys, hipStream_t s) {
+    if (Ql < 2) throw LogicError("end of modulus switching chain reached");
+    const size_t N = c.N, L1 = Ql - 1;
+    Scratch last(npolys * N * sizeof(uint64_t), s);
+    Scratch delta(npolys * L1 * N * sizeof(uint64_t), s);
+    for (size_t p = 0; p < npolys; p++)
+        SHM_HIP(hipMemcpyAsync(last.as<uint64_t>() + p * N, in + (p * Ql + L1) * N, N * sizeof(uint64_t),
+                               hipMemcpyDeviceToDevice, s));
+    ntt_inverse(c, last.as<uint64_t>(), npolys, 1, L1, s);
+    k_rescale_reduce<<<grid_for(npolys * L1 * N), kBlock, 0, s>>>(last.as<uint64_t>(), delta.as<uint64_t>(),
+                                                                  c.mods(), L1, N, npolys);
+    ntt_forward(c, delta.as<uint64_t>(), npolys, L1, 0, s);
+    const LevelConsts &L = c.level(Ql);
+    k_rescale_combine<<<grid_for(npolys * L1 * N), kBlock, 0, s>>>(
+        in, delta.as<uint64_t>(), c.mods(), L.qlast_inv.as<uint64_t>(), L.qlast_inv_s.as<uint64_t>(), Ql, N, npolys);
+    SHM_HIP(hipMemcpyAsync(out, delta.as<uint64_t>(), npolys * L1 * N * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    SHM_HIP(hipGetLastError());
+}
+
+void launch_permute(const Context &c, const uint64_t *in, uint64_t *out, const uint32_t *table, size_t nlimbs,
+                    hipStream_t s) {
+    const size_t total = nlimbs * c.N;
+    k_permute<<<grid_for(total), kBlock, 0, s>>>(in, out, table, c.N, total);
+    SHM_HIP(hipGetLastError());
+}
+
+void launch_copy_limbs(const Context &c, const uint64_t *in, size_t in_limbs, uint64_t *out, size_t out_limbs,
+                       size_t npolys, hipStream_t s) {
+    k_copy_limbs<<<grid_for(npolys * out_limbs * c.N), kBlock, 0, s>>>(in, in_limbs, out, out_limbs, c.N, npolys);
+    SHM_HIP(hipGetLastError());
+}
+
+void launch_sample_uniform(const Context &c, uint64_t seed, uint64_t *out, size_t Ql, bool withP, hipStream_t s) {
+    const size_t limbs = Ql + (withP ? c.nP : 0), total = limbs * c.N;
+    k_sample_uniform<<<grid_for(total), kBlock, 0, s>>>(seed, out, c.mods(), Ql, c.nQ, c.N, total);
+    SHM_HIP(hipGetLastError());
+}
+
+void launch_sample_gaussian_rns(const Context &c, uint64_t seed, uint64_t *out, size_t Ql, bool withP,
+                                hipStream_t s) {
+    const size_t limbs = Ql + (withP ? c.nP : 0), total = limbs * c.N;
+    k_sample_gaussian<<<grid_for(total), kBlock, 0, s>>>(seed, out, c.mods(), Ql, c.nQ, c.N, total);
+    SHM_HIP(hipGetLastError());
+}
+
+void launch_signed_to_rns(const Context &c, const int8_t *coeffs, uint64_t *out, size_t Ql, bool withP,
+                          hipStream_t s) {
+    const size_t limbs = Ql + (withP ? c.nP : 0), total = limbs * c.N;
+    k_signed_to_rns<<<grid_for(total), kBlock, 0, s>>>(coeffs, out, c.mods(), Ql, c.nQ, c.N, total);
+    SHM_HIP(hipGetLastError());
+}
+
+void launch_ks_key_digit(const Context &c, const uint64_t *sk, const uint64_t *newkey, uint64_t *b,
+                         const uint64_t *a, const uint64_t *e, size_t digit, hipStream_t s) {
+    const size_t total = c.nQP() * c.N;
+    k_ks_key_digit<<<grid_for(total), kBlock, 0, s>>>(sk, newkey, b, a, e, c.mods(), c.pmod.as<uint64_t>(),
+                                                      c.pmod_s.as<uint64_t>(), digit * c.nP, (digit + 1) * c.nP,
+                                                      c.N, total);
+    SHM_HIP(hipGetLastError());
+}
+
+void launch_encrypt_combine(const Context &c, const uint64_t *sk, const uint64_t *a, const uint64_t *e,
+                            const uint64_t *pt, uint64_t *c0, size_t Ql, hipStream_t s) {
+    const size_t total = Ql * c.N;
+    k_encrypt_combine<<<grid_for(total), kBlock, 0, s>>>(sk, a, e, pt, c0, c.mods(), c.N, total);
+    SHM_HIP(hipGetLastError());
+}
+
+void launch_decrypt(const Context &c, const uint64_t *ct, size_t size, const uint64_t *sk, uint64_t *pt, size_t Ql,
+                    hipStream_t s) {
+    k_decrypt<<<grid_for(Ql * c.N), kBlock, 0, s>>>(ct, size, sk, pt, c.mods(), c.N, Ql * c.N);
+    SHM_HIP(hipGetLastError());
+}
+
+void launch_mul_elementwise(const Context &c, const uint64_t *a, const uint64_t *b, uint64_t *out, size_t nlimbs,
+                            bool withP, size_t Ql, hipStream_t s) {
+    (void)withP;
+    const size_t total = nlimbs * c.N;
+    k_mul_elementwise<<<grid_for(total), kBlock, 0, s>>>(a, b, out, c.mods(), Ql, c.nQ, c.N, total);
+    SHM_HIP(hipGetLastError());
+}
+
+}  // namespace shm

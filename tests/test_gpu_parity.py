@@ -1,0 +1,120 @@
+"""GPU parity: the HIP engine (through the C ABI) against the oracle, bit-exact.
+
+Sizes are chosen so the oracle finishes in seconds; full-size shapes are
+checked through size-independent properties in test_gpu_matmul.py.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+C1_BITS = [50] * 8 + [60]
+
+
+@pytest.fixture(scope="module")
+def c1(oracle_lib):
+    import shmoai_amd as shm
+    ctx = shm.Context(13, C1_BITS, 1)
+    ref = oracle_lib.Ref(13, C1_BITS, 1)
+    return ctx, ref
+
+
+def _rand_limbs(ref, rng, npolys, Ql, withP=False):
+    idx = ref.pidx(Ql, withP)
+    out = np.empty((npolys, len(idx), ref.N), np.uint64)
+    for k, pi in enumerate(idx):
+        out[:, k, :] = rng.integers(0, int(ref.primes[pi]), size=(npolys, ref.N), dtype=np.uint64)
+    return out.ravel()
+
+
+def test_primes_match(c1):
+    ctx, ref = c1
+    assert np.array_equal(ctx.coeff_modulus, ref.primes)
+
+
+def test_ntt_roundtrip_bitexact(c1):
+    import shmoai_amd as shm
+    ctx, ref = c1
+    rng = np.random.default_rng(11)
+    x = _rand_limbs(ref, rng, 2, 8)
+    ct = shm.Ciphertext(ctx, 2, 1).load(x)
+    ctx.ntt_forward(ct.data_ptr(), 2, 8, 0)
+    got = ct.numpy()
+    exp = ref.ntt(x, np.tile(ref.pidx(8), 2))
+    assert np.array_equal(got, exp)
+    ctx.ntt_inverse(ct.data_ptr(), 2, 8, 0)
+    assert np.array_equal(ct.numpy(), x)
+
+
+def test_secret_and_keys_bitexact(c1):
+    import shmoai_amd as shm
+    ctx, ref = c1
+    sk = shm.SecretKey(ctx, seed=1, hamming_weight=64)
+    s_ref = ref.sk_ntt(ref.sample_ternary(1, 64))
+    assert np.array_equal(sk.numpy(), s_ref)
+    rk = sk.gen_relinkey(seed=5)
+    assert np.array_equal(rk.numpy(), ref.gen_relin_key(s_ref, 5))
+    gk = sk.create_galois_keys_from_steps([1, -1], seed=9)
+    for step in (1, -1):
+        elt = ctx.galois_elt_from_step(step)
+        assert elt == ref.elt(step)
+        kref = ref.gen_galois_key(s_ref, elt, ref.splitmix64(9, elt))
+        assert np.array_equal(gk.numpy(elt), kref)
+
+
+def test_c1_pipeline_bitexact(c1):
+    """NTT -> rotate(1) -> multiply_plain -> rescale -> iNTT."""
+    import shmoai_amd as shm
+    ctx, ref = c1
+    rng = np.random.default_rng(3)
+    Ql = 8
+    x = _rand_limbs(ref, rng, 2, Ql)
+    sk = shm.SecretKey(ctx, seed=1, hamming_weight=64)
+    s_ref = ref.sk_ntt(ref.sample_ternary(1, 64))
+    gk = sk.create_galois_keys_from_steps([1], seed=9)
+    elt = ref.elt(1)
+    key = ref.gen_galois_key(s_ref, elt, ref.splitmix64(9, elt))
+    ev = shm.Evaluator(ctx)
+    ct = shm.Ciphertext(ctx, 2, 1).load(x)
+    ctx.ntt_forward(ct.data_ptr(), 2, Ql, 0)
+    exp = ref.ntt(x, np.tile(ref.pidx(Ql), 2))
+    assert np.array_equal(ct.numpy(), exp)
+    ev.rotate_vector_inplace(ct, 1, gk)
+    exp = ref.rotate(exp, Ql, elt, key)
+    assert np.array_equal(ct.numpy(), exp)
+    c = float(np.random.default_rng(4).uniform(-1, 1))
+    pt = shm.Encoder(ctx).encode(c, 2.0 ** 50, 1)
+    pt_ref = ref.encode_scalar(c, Ql, 2.0 ** 50)
+    assert np.array_equal(pt.numpy(), pt_ref)
+    ev.multiply_plain_inplace(ct, pt)
+    exp = ref.multiply_plain(exp, pt_ref, Ql)
+    assert np.array_equal(ct.numpy(), exp)
+    ev.rescale_to_next_inplace(ct)
+    exp = ref.rescale(exp, Ql)
+    assert np.array_equal(ct.numpy(), exp)
+    ctx.ntt_inverse(ct.data_ptr(), 2, Ql - 1, 0)
+    exp = ref.intt(exp, np.tile(ref.pidx(Ql - 1), 2))
+    assert np.array_equal(ct.numpy(), exp)
+
+
+def test_encrypt_decrypt_encode_bitexact(c1):
+    import shmoai_amd as shm
+    ctx, ref = c1
+    rng = np.random.default_rng(21)
+    v = rng.uniform(-1, 1, ctx.N // 2)
+    enc = shm.Encoder(ctx)
+    pt = enc.encode(v, 2.0 ** 40, 1)
+    pt_ref = ref.encode(v, 8, 2.0 ** 40)
+    assert np.array_equal(pt.numpy(), pt_ref)
+    sk = shm.SecretKey(ctx, seed=1, hamming_weight=64)
+    s_ref = ref.sk_ntt(ref.sample_ternary(1, 64))
+    ct = sk.encrypt_symmetric(pt, seed=77)
+    ct_ref = ref.encrypt_symmetric(s_ref, pt_ref, 8, 77)
+    assert np.array_equal(ct.numpy(), ct_ref)
+    dec = shm.Decryptor(ctx, sk).decrypt(ct)
+    dec_ref = ref.decrypt(s_ref, ct_ref, 2, 8)
+    assert np.array_equal(dec.numpy(), dec_ref)
+    out = enc.decode(dec)
+    re_ref, _ = ref.decode(dec_ref, 8, 2.0 ** 40)
+    assert np.max(np.abs(out - re_ref)) <= 1e-12
+    assert np.max(np.abs(out - v)) < 1e-6
