@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Benchmark: the quarter-width MOAI ct-pt matmul on one MI355X.
+
+Workload (one step): 192 column-packed input ciphertexts (N = 2^16, 24 x 50-bit
+primes, 256 inputs x 128 tokens = 32768 slots each) times a 192 x 192 weight
+matrix W ~ N(0, 0.02) encoded at scale 2^40, then rescale
+(Ct_pt_matrix_mul.cuh:18-185 semantics). Inputs are resident in HBM before
+the timed region; limbs are uniform random residues (synthetic data).
+
+Prints one JSON line: value = amortised seconds per input (step time / 256
+inputs), plus the roofline of the dominant kernel (the INT8-MFMA contraction,
+timed with HIP events on its own stream) and a CPU baseline (the oracle, on a
+bounded sample of the same workload, extrapolated).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "amortized s/input per BERT layer + 768-ct batched bootstrap s; HBM GB/s % peak"
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+INT8_PEAK_TOPS = 5000.0        # MI355X dense INT8 MFMA (POP/s x 1e3)
+LOG_N, L, ROW_W, COL_W, INPUTS = 16, 24, 192, 192, 256
+BITS = [50] * L + [60]
+SCALE = 2.0 ** 40
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_inputs(ctx, shm, rng):
+    N = ctx.N
+    xs, hs = [], []
+    for _ in range(ROW_W):
+        a = np.empty((2, L, N), np.uint64)
+        for k in range(L):
+            a[:, k, :] = rng.integers(0, int(ctx.coeff_modulus[k]), size=(2, N), dtype=np.uint64)
+        a = a.ravel()
+        ct = shm.Ciphertext(ctx, 2, 1).load(a)
+        ct.scale = SCALE
+        xs.append(a)
+        hs.append(ct)
+    return xs, hs
+
+
+def cpu_baseline(xs, W, threads: int):
+    from oracle import refcpu
+    refcpu.build()
+    refcpu.set_threads(threads)
+    ref = refcpu.Ref(LOG_N, BITS, 1)
+    t0 = time.perf_counter()
+    ref.ctpt_matmul(xs, W, L, np.full(ROW_W, SCALE), 0, 1)
+    dt = time.perf_counter() - t0
+    per_matmul = dt * COL_W
+    return {"value": per_matmul / INPUTS, "unit": "s/input", "cores": threads, "kind": "port",
+            "sample": f"1 of {COL_W} output columns of the same 192x192 matmul at full shape "
+                      f"(N=2^16, L=24, incl. rescale) on {threads} threads = {dt:.2f} s, extrapolated x{COL_W}",
+            "seconds_per_full_matmul": per_matmul}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--variant", choices=["int8", "u64"], default="int8")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the one-off per-ciphertext form timing")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
+              "(the matmul shards by coefficient columns, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import shmoai_amd as shm
+    variant = shm.MATMUL_INT8 if args.variant == "int8" else shm.MATMUL_U64
+    ctx = shm.Context(LOG_N, BITS, 1)
+    rng = np.random.default_rng(5)
+    log("bench: generating 192 input ciphertexts (N=2^16, L=24) and uploading to HBM")
+    xs, hs = make_inputs(ctx, shm, rng)
+    W = np.random.default_rng(6).normal(0.0, 0.02, (ROW_W, COL_W))
+    out = [shm.Ciphertext(ctx, 2, 2) for _ in range(COL_W)]
+
+    for w in range(args.warmup):
+        shm.ct_pt_matrix_mul_wo_pre(hs, W, ROW_W, COL_W, ROW_W, ctx, variant=variant, out=out)
+        log(f"bench: warmup step {w + 1}/{args.warmup} done")
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        shm.ct_pt_matrix_mul_wo_pre(hs, W, ROW_W, COL_W, ROW_W, ctx, variant=variant, out=out)
+        kernel_ms.append(shm.last_matmul_kernel_ms())
+    dt = time.perf_counter() - t0  # every call synchronises its stream before returning
+    ms_per_step = dt / args.steps * 1e3
+    k_ms = float(np.mean(kernel_ms))
+    log(f"bench: {args.steps} steps, {ms_per_step:.3f} ms/step, contraction kernel {k_ms:.3f} ms")
+
+    N = ctx.N
+    bytes_alg = ROW_W * 2 * L * N * 8 + COL_W * 2 * L * N * 8        # read X limbs + write accumulators
+    macs = ROW_W * COL_W * 2 * L * N                                  # modular MACs
+    hbm_gbs = bytes_alg / (k_ms * 1e-3) / 1e9
+    digit_ops = 2 * 35 * macs                                         # 7 x 5 int8 digit products
+    tops = digit_ops / (k_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma" if variant == shm.MATMUL_INT8 else "hbm",
+                "achieved": round(tops if variant == shm.MATMUL_INT8 else hbm_gbs, 2),
+                "peak": INT8_PEAK_TOPS if variant == shm.MATMUL_INT8 else HBM_PEAK_GBS,
+                "unit": "TOP/s" if variant == shm.MATMUL_INT8 else "GB/s",
+                "traffic": None,
+                "kernel": "k_contract_int8" if variant == shm.MATMUL_INT8 else "k_contract_u64",
+                "kernel_ms": round(k_ms, 4),
+                "hbm_achieved_gbs": round(hbm_gbs, 1), "hbm_frac": round(hbm_gbs / HBM_PEAK_GBS, 4),
+                "algorithmic_bytes_per_launch": bytes_alg, "modular_macs_per_launch": macs}
+    roofline["frac"] = round(roofline["achieved"] / roofline["peak"], 4)
+
+    extra = {}
+    if not args.no_plain:
+        t1 = time.perf_counter()
+        shm.ct_pt_matrix_mul_wo_pre(hs, W, ROW_W, COL_W, ROW_W, ctx, variant=shm.MATMUL_PER_CT, out=out)
+        plain_ms = (time.perf_counter() - t1) * 1e3
+        extra = {"plain_per_ct_ms_per_step": round(plain_ms, 2),
+                 "batched_speedup_vs_plain": round(plain_ms / ms_per_step, 2)}
+        log(f"bench: plain per-ciphertext form {plain_ms:.1f} ms")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"bench: CPU baseline (oracle) on {threads} threads")
+        cpu = cpu_baseline(xs, W, threads)
+
+    line = {
+        "metric": METRIC,
+        "value": ms_per_step * 1e-3 / INPUTS,
+        "unit": "s/input",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 (int8 MFMA digits, exact)" if variant == shm.MATMUL_INT8 else "u64",
+        "data": "synthetic (uniform random limbs, W ~ N(0,0.02))",
+        "config": {"workload": "quarter-width ct-pt matmul 192x192 (MOAI column packing), N=2^16, "
+                               "L=24 x 50-bit primes, 256 inputs x 128 tokens per ciphertext, incl. rescale",
+                   "variant": args.variant, "inputs_per_step": INPUTS},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        **extra,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,90 @@
+"""GPU parity of the MOAI ct-pt matmul (Ct_pt_matrix_mul.cuh:18-185).
+
+- small shape: every variant bit-exact against the oracle on all outputs;
+- full quarter-width shape (N=2^16, L=24, 192 x 192): INT8-MFMA == 64-bit
+  path on every limb of every output, and sampled output columns bit-exact
+  against the oracle;
+- closed-form known answer from the reference's own test
+  (test_ct_pt_matrix_mul.cuh: X[i][j][k] = j+1, W = 1/128 -> slot of token j
+  decodes to (row_W/128) * (j+1)).
+"""
+import numpy as np
+import pytest
+
+import shmoai_amd as shm
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand_cts(ctx, ref, rng, n, Ql):
+    xs, hs = [], []
+    for _ in range(n):
+        a = np.empty((2, Ql, ref.N), np.uint64)
+        for k in range(Ql):
+            a[:, k, :] = rng.integers(0, int(ref.primes[k]), size=(2, ref.N), dtype=np.uint64)
+        a = a.ravel()
+        ct = shm.Ciphertext(ctx, 2, ctx.chain(Ql)).load(a)
+        ct.scale = 2.0 ** 40
+        xs.append(a)
+        hs.append(ct)
+    return xs, hs
+
+
+@pytest.mark.parametrize("variant", [shm.MATMUL_U64, shm.MATMUL_INT8, shm.MATMUL_PER_CT])
+def test_matmul_small_bitexact(oracle_lib, variant):
+    bits = [50] * 5 + [60]
+    ctx = shm.Context(13, bits, 1)
+    ref = oracle_lib.Ref(13, bits, 1)
+    rng = np.random.default_rng(5)
+    row_w, col_w, Ql = 20, 12, 5
+    xs, hs = _rand_cts(ctx, ref, rng, row_w, Ql)
+    W = np.random.default_rng(6).normal(0, 0.02, (row_w, col_w))
+    out = shm.ct_pt_matrix_mul_wo_pre(hs, W, row_w, col_w, row_w, ctx, variant=variant)
+    exp = ref.ctpt_matmul(xs, W, Ql, np.full(row_w, 2.0 ** 40))
+    for i in range(col_w):
+        assert out[i].chain_index == ctx.chain(Ql - 1)
+        assert out[i].scale == 2.0 ** 40
+        assert np.array_equal(out[i].numpy(), exp[i]), f"output {i}"
+
+
+def test_matmul_quarter_width_int8_equals_u64_and_oracle(oracle_lib):
+    bits = [50] * 24 + [60]
+    ctx = shm.Context(16, bits, 1)
+    ref = oracle_lib.Ref(16, bits, 1)
+    rng = np.random.default_rng(5)
+    row_w = col_w = 192
+    Ql = 24
+    xs, hs = _rand_cts(ctx, ref, rng, row_w, Ql)
+    W = np.random.default_rng(6).normal(0, 0.02, (row_w, col_w))
+    o8 = shm.ct_pt_matrix_mul_wo_pre(hs, W, row_w, col_w, row_w, ctx, variant=shm.MATMUL_INT8)
+    o64 = shm.ct_pt_matrix_mul_wo_pre(hs, W, row_w, col_w, row_w, ctx, variant=shm.MATMUL_U64)
+    for i in range(col_w):
+        assert np.array_equal(o8[i].numpy(), o64[i].numpy()), f"output {i}"
+    oracle_lib.set_threads(16)
+    for i in (0, 191):
+        exp = ref.ctpt_matmul(xs, W, Ql, np.full(row_w, 2.0 ** 40), i, i + 1)[0]
+        assert np.array_equal(o8[i].numpy(), exp), f"output {i}"
+
+
+def test_matmul_known_answer():
+    """X[i][j][k] = j+1 (token j), W = 1/128: every slot of token j decodes to (row_W/128)(j+1)."""
+    bits = [60] + [40] * 4 + [60]
+    ctx = shm.Context(13, bits, 1)
+    n_slots = ctx.N // 2
+    num_X, num_row, row_w, col_w = 32, 128, 64, 8
+    assert num_X * num_row == n_slots
+    # batch_input packing (Batch_encode_encrypt.cuh:12-43): slot[num_X*k + j] = X[j][k][i]
+    slot = np.empty(n_slots)
+    for k in range(num_row):
+        slot[num_X * k:num_X * (k + 1)] = k + 1.0
+    sk = shm.SecretKey(ctx, seed=3, hamming_weight=64)
+    enc = shm.Encoder(ctx)
+    cts = [sk.encrypt_symmetric(enc.encode(slot, 2.0 ** 40, 1), seed=100 + i) for i in range(row_w)]
+    W = np.full((row_w, col_w), 1.0 / 128.0)
+    dec = shm.Decryptor(ctx, sk)
+    for variant in (shm.MATMUL_INT8, shm.MATMUL_U64):
+        out = shm.ct_pt_matrix_mul_wo_pre(cts, W, row_w, col_w, row_w, ctx, variant=variant)
+        for o in out[:2]:
+            v = enc.decode(dec.decrypt(o))
+            expect = np.repeat(np.arange(1, num_row + 1) * (row_w / 128.0), num_X)
+            assert np.max(np.abs(v - expect)) < 1e-3
