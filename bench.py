@@ -75,6 +75,8 @@ def main() -> int:
     ap.add_argument("--variant", choices=["int8", "u64"], default="int8")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-off per-ciphertext form timing")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one device-to-device copy of a 24 MiB ciphertext (profiler byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
@@ -93,11 +95,15 @@ def main() -> int:
     for w in range(args.warmup):
         shm.ct_pt_matrix_mul_wo_pre(hs, W, ROW_W, COL_W, ROW_W, ctx, variant=variant, out=out)
         log(f"bench: warmup step {w + 1}/{args.warmup} done")
+    if args.copy_calibration:
+        hs[0].copy()
+        log(f"bench: calibration copy of {hs[0].words() * 8} bytes done")
     kernel_ms = []
     t0 = time.perf_counter()
     for s in range(args.steps):
         shm.ct_pt_matrix_mul_wo_pre(hs, W, ROW_W, COL_W, ROW_W, ctx, variant=variant, out=out)
         kernel_ms.append(shm.last_matmul_kernel_ms())
+        assert shm.last_matmul_variant() == variant, "matmul fell back to another variant"
     dt = time.perf_counter() - t0  # every call synchronises its stream before returning
     ms_per_step = dt / args.steps * 1e3
     k_ms = float(np.mean(kernel_ms))

@@ -194,6 +194,10 @@ int shm_ct_pt_matrix_mul(shm_context *ctx, const shm_ciphertext *const *enc_X, u
                          uint32_t col_W, shm_ciphertext *const *out, int variant, shm_stream_t stream);
 /* duration of the dominant kernel of the last matmul on this process, ms (HIP events) */
 double shm_last_matmul_kernel_ms(void);
+/* variant the last matmul actually ran: SHM_MATMUL_INT8 falls back to SHM_MATMUL_U64
+ * when a prime is outside (2^46, 2^61], row_W > 256 or a weight needs more than
+ * 5 balanced digits (4 for 8-digit limbs) */
+int shm_last_matmul_variant(void);
 
 #ifdef __cplusplus
 }

@@ -415,3 +415,7 @@ def ct_pt_matrix_mul_wo_pre(enc_X: Sequence[Ciphertext], W: np.ndarray, col_X: i
 
 def last_matmul_kernel_ms() -> float:
     return float(_lib().shm_last_matmul_kernel_ms())
+
+
+def last_matmul_variant() -> int:
+    return int(_lib().shm_last_matmul_variant())

@@ -98,6 +98,7 @@ _SIGS = {
     "shm_complex_conjugate_inplace": (i32, [vp, vp, vp, vp]),
     "shm_ct_pt_matrix_mul": (i32, [vp, pvp, u32, C.POINTER(dbl), u32, pvp, i32, vp]),
     "shm_last_matmul_kernel_ms": (dbl, []),
+    "shm_last_matmul_variant": (i32, []),
 }
 
 _lib = None

@@ -850,5 +850,6 @@ int shm_ct_pt_matrix_mul(shm_context *ctx, const shm_ciphertext *const *X, uint3
 }
 
 double shm_last_matmul_kernel_ms(void) { return (double)last_matmul_kernel_ms(); }
+int shm_last_matmul_variant(void) { return last_matmul_variant(); }
 
 }  // extern "C"

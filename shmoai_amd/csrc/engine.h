@@ -195,6 +195,13 @@ void ntt_forward_qlp(const Context &c, uint64_t *data, size_t Ql, size_t lo, siz
 void ntt_inverse_qlp(const Context &c, uint64_t *data, size_t Ql, size_t lo, size_t hi, size_t n_polys,
                      size_t poly_stride, hipStream_t s);
 
+// Rescale tail fused into one forward NTT: limb k (< L1) of polynomial p is
+// NTT([last_p]_{q_k}); the block pass stores (acc[p][k] - that) * qinv_k to
+// outs[p / ppo] (or out_single) at ((p % ppo) * L1 + k) * N.
+void rescale_ntt_fused(const Context &c, const uint64_t *last, const uint64_t *acc, size_t acc_limbs,
+                       uint64_t *delta, uint64_t *const *outs, uint64_t *out_single, size_t ppo, size_t n_polys,
+                       size_t L1, const uint64_t *qinv, const uint64_t *qinv_s, hipStream_t s);
+
 // elementwise (poly.hip)
 void launch_add(const Context &c, uint64_t *a, const uint64_t *b, size_t Ql, size_t npolys, hipStream_t s);
 void launch_sub(const Context &c, uint64_t *a, const uint64_t *b, size_t Ql, size_t npolys, hipStream_t s);
@@ -239,5 +246,6 @@ void ctpt_matmul(const Context &c, const std::vector<const Ciphertext *> &X, con
 
 // timing hook for the dominant kernel of the last matmul (HIP events)
 float last_matmul_kernel_ms();
+int last_matmul_variant();
 
 }  // namespace shm

@@ -120,27 +120,6 @@ __global__ void k_gather_last(const uint64_t *acc, uint64_t *last, size_t Ql, si
     }
 }
 
-__global__ void k_reduce_last(const uint64_t *last, uint64_t *delta, const DevMod *mods, size_t L1, size_t N,
-                              size_t npolys) {
-    const size_t total = npolys * L1 * N;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t p = t / (L1 * N), r = t % (L1 * N);
-        delta[t] = reduce64(last[p * N + r % N], mods[r / N]);
-    }
-}
-
-__global__ void k_rescale_scatter(const uint64_t *acc, const uint64_t *delta, uint64_t *const *outs,
-                                  const DevMod *mods, const uint64_t *inv, const uint64_t *inv_s, size_t Ql, size_t N,
-                                  size_t nct) {
-    const size_t L1 = Ql - 1, total = nct * 2 * L1 * N;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t ct = t / (2 * L1 * N), r = t % (2 * L1 * N), p = r / (L1 * N), r2 = r % (L1 * N), i = r2 / N;
-        const uint64_t q = mods[i].q;
-        const uint64_t d = sub_mod(acc[(ct * 2 + p) * Ql * N + r2], delta[t], q);
-        outs[ct][r] = mul_shoup(d, inv[i], inv_s[i], q);
-    }
-}
-
 void batched_rescale(const Context &c, const uint64_t *acc, const std::vector<Ciphertext *> &out, size_t Ql,
                      hipStream_t s) {
     const size_t N = c.N, L1 = Ql - 1, nct = out.size(), np = 2 * nct;
@@ -148,17 +127,13 @@ void batched_rescale(const Context &c, const uint64_t *acc, const std::vector<Ci
     Scratch delta(np * L1 * N * sizeof(uint64_t), s);
     k_gather_last<<<grid_for(np * N), kBlock, 0, s>>>(acc, last.as<uint64_t>(), Ql, N, np);
     ntt_inverse(c, last.as<uint64_t>(), np, 1, L1, s);
-    k_reduce_last<<<grid_for(np * L1 * N), kBlock, 0, s>>>(last.as<uint64_t>(), delta.as<uint64_t>(), c.mods(), L1,
-                                                           N, np);
-    ntt_forward(c, delta.as<uint64_t>(), np, L1, 0, s);
     std::vector<uint64_t *> ptrs(nct);
     for (size_t i = 0; i < nct; i++) ptrs[i] = out[i]->data();
     Scratch dptrs(nct * sizeof(uint64_t *), s);
     SHM_HIP(hipMemcpyAsync(dptrs.as<void>(), ptrs.data(), nct * sizeof(uint64_t *), hipMemcpyHostToDevice, s));
     const LevelConsts &L = c.level(Ql);
-    k_rescale_scatter<<<grid_for(nct * 2 * L1 * N), kBlock, 0, s>>>(
-        acc, delta.as<uint64_t>(), dptrs.as<uint64_t *>(), c.mods(), L.qlast_inv.as<uint64_t>(),
-        L.qlast_inv_s.as<uint64_t>(), Ql, N, nct);
+    rescale_ntt_fused(c, last.as<uint64_t>(), acc, Ql, delta.as<uint64_t>(), dptrs.as<uint64_t *>(), nullptr, 2, np,
+                      L1, L.qlast_inv.as<uint64_t>(), L.qlast_inv_s.as<uint64_t>(), s);
     SHM_HIP(hipGetLastError());
     // the host vector of pointers must outlive the async copy
     SHM_HIP(hipStreamSynchronize(s));
@@ -168,8 +143,13 @@ void batched_rescale(const Context &c, const uint64_t *acc, const std::vector<Ci
 
 void ctpt_matmul_int8(const Context &c, const std::vector<const Ciphertext *> &X, const double *W, size_t row_w,
                       size_t col_w, uint64_t *acc, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1);
+bool int8_supported(const Context &c, size_t Ql, const double *W, const double *scales, size_t row_w, size_t col_w,
+                    int *dx_out, int *dw_out);
+
+int g_last_variant = -1;
 
 float last_matmul_kernel_ms() { return g_last_ms; }
+int last_matmul_variant() { return g_last_variant; }
 
 void ctpt_matmul(const Context &c, const std::vector<const Ciphertext *> &X, const double *W, size_t row_w,
                  size_t col_w, const std::vector<Ciphertext *> &out, int variant, hipStream_t s) {
@@ -186,6 +166,12 @@ void ctpt_matmul(const Context &c, const std::vector<const Ciphertext *> &X, con
 
     std::vector<double> scales(row_w);
     for (size_t j = 0; j < row_w; j++) scales[j] = X[j]->scale;
+    if (variant == 2) {
+        // shapes, primes or weights outside the INT8 digit bounds run the 64-bit contraction
+        int dx = 0, dw = 0;
+        if (!int8_supported(c, Ql, W, scales.data(), row_w, col_w, &dx, &dw)) variant = 0;
+    }
+    g_last_variant = variant;
     hipEvent_t e0, e1;
     SHM_HIP(hipEventCreate(&e0));
     SHM_HIP(hipEventCreate(&e1));

@@ -42,19 +42,25 @@ __device__ __forceinline__ int64_t weight_int(const double *W, const double *sca
     return (int64_t)round(W[j * col_w + i] * scales[j]);
 }
 
-// A fragments: frag[((b * MB + ib) * KS + ks) * 64 + lane] = 16 bytes,
-// byte e = digit b of w(row m = 16 ib + lane % 16, k = 64 ks + 16 (lane / 16) + e).
+// A fragments in phase-major order, so that the fragments of one row phase
+// (RB blocks of 16 output rows) are one contiguous run:
+// frag[(((ph * DW + b) * RB + ibl) * KS + ks) * 64 + lane] = 16 bytes,
+// byte e = digit b of w(row m = 16 (ph RB + ibl) + lane % 16,
+//                       k = 64 ks + 16 (lane / 16) + e); zero outside the matrix.
 __global__ void k_weight_frags(const double *W, const double *scales, i8x16 *frag, size_t row_w, size_t col_w,
-                               int DW, int MB, int KS) {
-    const size_t total = (size_t)DW * MB * KS * 64;
+                               int DW, int MB, int KS, int RB) {
+    const int NPH = (MB + RB - 1) / RB;
+    const size_t total = (size_t)NPH * DW * RB * KS * 64;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
         const int lane = (int)(t % 64);
         size_t r = t / 64;
         const int ks = (int)(r % KS);
         r /= KS;
-        const int ib = (int)(r % MB);
-        const int b = (int)(r / MB);
-        const size_t m = 16 * (size_t)ib + lane % 16;
+        const int ibl = (int)(r % RB);
+        r /= RB;
+        const int b = (int)(r % DW);
+        const int ph = (int)(r / DW);
+        const size_t m = 16 * ((size_t)ph * RB + ibl) + lane % 16;
         i8x16 v;
 #pragma unroll
         for (int e = 0; e < 16; e++) {
@@ -76,9 +82,10 @@ __global__ void k_weight_frags(const double *W, const double *scales, i8x16 *fra
     }
 }
 
-// corr[l][i] = [128 R]_q * [sum_j w_ij]_q mod q
-__global__ void k_weight_corr(const double *W, const double *scales, uint64_t *corr, const DevMod *mods,
-                              size_t row_w, size_t col_w, size_t Ql, int DX) {
+// corr[l][i] = [128 R rowsum_i - 2^105]_q (the 2^105 offset keeps the
+// recombined plane sum non-negative); mu[l] = floor(2^108 / q_l).
+__global__ void k_weight_corr(const double *W, const double *scales, uint64_t *corr, uint64_t *mu,
+                              const DevMod *mods, size_t row_w, size_t col_w, size_t Ql, int DX) {
     const size_t total = Ql * col_w;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
         const size_t l = t / col_w, i = t % col_w;
@@ -90,105 +97,153 @@ __global__ void k_weight_corr(const double *W, const double *scales, uint64_t *c
         uint64_t R = 0;
         for (int a = 0; a < DX; a++) R = add_mod(mul_mod(R, 256, m), 1, m.q);
         const uint64_t c = mul_mod(128 % m.q, R, m);
-        corr[t] = mul_mod(c, rs, m);
+        uint64_t p105 = 1;
+        for (int e = 0; e < 105; e++) p105 = add_mod(p105, p105, m.q);
+        corr[t] = sub_mod(mul_mod(c, rs, m), p105, m.q);
+        if (i == 0) {
+            // floor(2^108 / q) from floor(2^128 / q): shift by 20 bits
+            mu[l] = (m.mu_hi << 44) | (m.mu_lo >> 20);
+        }
     }
 }
 
-template <int DX, int DW, int KS>
+__device__ __forceinline__ uint32_t bperm(uint32_t hi_src, uint32_t lo_src, uint32_t sel) {
+    return __builtin_amdgcn_perm(hi_src, lo_src, sel);
+}
+
+// 4x4 byte transpose: o_k = (w0.byte k, w1.byte k, w2.byte k, w3.byte k).
+__device__ __forceinline__ void transpose4(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, uint32_t (&o)[4]) {
+    const uint32_t t0 = bperm(w1, w0, 0x06020400u);  // w0.0 w1.0 w0.2 w1.2
+    const uint32_t t1 = bperm(w1, w0, 0x07030501u);  // w0.1 w1.1 w0.3 w1.3
+    const uint32_t t2 = bperm(w3, w2, 0x06020400u);  // w2.0 w3.0 w2.2 w3.2
+    const uint32_t t3 = bperm(w3, w2, 0x07030501u);  // w2.1 w3.1 w2.3 w3.3
+    o[0] = bperm(t2, t0, 0x05040100u);
+    o[1] = bperm(t3, t1, 0x05040100u);
+    o[2] = bperm(t2, t0, 0x07060302u);
+    o[3] = bperm(t3, t1, 0x07060302u);
+}
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+template <int DX, int DW, int KS, int RB>
 __global__ __launch_bounds__(256) void k_contract_int8(const uint64_t *const *__restrict__ X,
                                                        const i8x16 *__restrict__ frag,
                                                        const uint64_t *__restrict__ corr,
+                                                       const uint64_t *__restrict__ mu_arr,
                                                        uint64_t *__restrict__ acc_out,
                                                        const DevMod *__restrict__ mods, int row_w, int col_w,
                                                        int MB, size_t Ql, size_t N) {
     constexpr int ND = DX + DW - 1;
+    // |C_d| < 2^24.2, so |S| < 2^104.3 and 0 < S + 2^105 < 2^106 needs ND <= 11.
+    static_assert(ND >= 10 && ND <= 11, "plane recombination assumes 10 or 11 planes");
+    static_assert(DX <= 8, "at most 8 digits per limb");
+    constexpr int FR = DW * RB * KS * 64;  // fragments staged per row phase
+    __shared__ i8x16 wl[FR];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const size_t tiles = N / 64;
     const size_t tile = blockIdx.x % tiles, lp = blockIdx.x / tiles;  // lp = p * Ql + l
     const size_t l = lp % Ql;
     const size_t col = tile * 64 + 16 * wave + (lane & 15);
     const size_t off = lp * N + col;
-    const DevMod m = mods[l];
+    const uint64_t q = mods[l].q, mu = mu_arr[l];
 
-    // B fragments: lane holds X'_a[k = 64 ks + 16 (lane/16) + e][col]
-    i8x16 B[DX][KS];
+    // B fragments: lane holds X'_a[k = 64 ks + 16 (lane/16) + e][col] for
+    // e = 0..15, built by byte transposes of the little-endian limb words.
+    u32x4 B[DX][KS];
 #pragma unroll
     for (int ks = 0; ks < KS; ks++) {
+        uint64_t x[16];
 #pragma unroll
         for (int e = 0; e < 16; e++) {
             const int k = 64 * ks + 16 * (lane >> 4) + e;
-            const uint64_t x = k < row_w ? X[k][off] : 0;
+            x[e] = k < row_w ? X[k][off] : 0;
+        }
 #pragma unroll
-            for (int a = 0; a < DX; a++) B[a][ks][e] = (int8_t)((int)((x >> (8 * a)) & 255) - 128);
+        for (int g = 0; g < 4; g++) {
+            uint32_t lo[4], hi[4];
+            transpose4((uint32_t)x[4 * g], (uint32_t)x[4 * g + 1], (uint32_t)x[4 * g + 2], (uint32_t)x[4 * g + 3], lo);
+            transpose4((uint32_t)(x[4 * g] >> 32), (uint32_t)(x[4 * g + 1] >> 32), (uint32_t)(x[4 * g + 2] >> 32),
+                       (uint32_t)(x[4 * g + 3] >> 32), hi);
+#pragma unroll
+            for (int a = 0; a < DX; a++) B[a][ks][g] = (a < 4 ? lo[a & 3] : hi[a & 3]) ^ 0x80808080u;
         }
     }
-    // rows of k >= row_w carry digit -128 in B; their A entries are 0 so they add nothing.
+    // rows k >= row_w carry digit -128 in B; their A entries are 0 so they add nothing.
 
-    for (int ib = 0; ib < MB; ib++) {
-        i32x4 C[ND];
+    const int nph = (MB + RB - 1) / RB;
+    for (int ph = 0; ph < nph; ph++) {
+        __syncthreads();
+        const i8x16 *src = frag + (size_t)ph * FR;
+        for (int t = threadIdx.x; t < FR; t += 256) wl[t] = src[t];
+        __syncthreads();
+#pragma unroll 1
+        for (int ibl = 0; ibl < RB; ibl++) {
+            const int ib = ph * RB + ibl;
+            if (ib >= MB) break;
+            i32x4 C[ND];
 #pragma unroll
-        for (int d = 0; d < ND; d++) C[d] = i32x4{0, 0, 0, 0};
+            for (int d = 0; d < ND; d++) C[d] = i32x4{0, 0, 0, 0};
 #pragma unroll
-        for (int b = 0; b < DW; b++) {
+            for (int b = 0; b < DW; b++) {
 #pragma unroll
-            for (int ks = 0; ks < KS; ks++) {
-                const i8x16 A = frag[(((size_t)b * MB + ib) * KS + ks) * 64 + lane];
+                for (int ks = 0; ks < KS; ks++) {
+                    const i8x16 A = wl[((b * RB + ibl) * KS + ks) * 64 + lane];
 #pragma unroll
-                for (int a = 0; a < DX; a++) C[a + b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, B[a][ks], C[a + b], 0, 0, 0);
+                    for (int a = 0; a < DX; a++)
+                        C[a + b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, __builtin_bit_cast(i8x16, B[a][ks]),
+                                                                         C[a + b], 0, 0, 0);
+                }
             }
-        }
-        // lane holds rows 16 ib + 4 (lane/16) + r, column col
+            // lane holds rows 16 ib + 4 (lane/16) + r, column col
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int row = 16 * ib + 4 * (lane >> 4) + r;
-            // Horner in signed 128-bit: S = sum_d C_d 256^d
-            int64_t c = C[ND - 1][r];
-            uint64_t lo = (uint64_t)c, hi = (uint64_t)(c >> 63);
+            for (int r = 0; r < 4; r++) {
+                const int row = 16 * ib + 4 * (lane >> 4) + r;
+                // S = sum_d C_d 256^d as two int64 Horner halves plus the top planes
+                int64_t lv = C[4][r];
 #pragma unroll
-            for (int d = ND - 2; d >= 0; d--) {
-                hi = (hi << 8) | (lo >> 56);
-                lo <<= 8;
-                const int64_t cd = C[d][r];
-                const uint64_t add = (uint64_t)cd;
-                const uint64_t nlo = lo + add;
-                hi += (uint64_t)(cd >> 63) + (nlo < lo ? 1 : 0);
-                lo = nlo;
-            }
-            uint64_t res;
-            if ((int64_t)hi < 0) {
-                // |S| = -S
-                const uint64_t nlo = ~lo + 1;
-                const uint64_t nhi = ~hi + (nlo == 0 ? 1 : 0);
-                res = barrett128(nhi, nlo, m);
-                res = res ? m.q - res : 0;
-            } else {
-                res = barrett128(hi, lo, m);
-            }
-            if (row < col_w) {
-                res = add_mod(res, corr[l * col_w + row], m.q);
-                acc_out[(size_t)row * 2 * Ql * N + off] = res;
+                for (int d = 3; d >= 0; d--) lv = lv * 256 + C[d][r];
+                int64_t h1 = C[9][r];
+#pragma unroll
+                for (int d = 8; d >= 5; d--) h1 = h1 * 256 + C[d][r];
+                int64_t h2 = 0;
+#pragma unroll
+                for (int d = ND - 1; d >= 10; d--) h2 = h2 * 256 + C[d][r];
+                // T = S + 2^105 = lv + h1 2^40 + h2 2^80 + 2^105 (0 < T < 2^106)
+                const uint64_t a_lo = (uint64_t)h1 << 40;
+                const uint64_t lo = (uint64_t)lv + a_lo;
+                const uint64_t hi = (uint64_t)(lv >> 63) + (uint64_t)(h1 >> 24) + (lo < a_lo ? 1 : 0) +
+                                    ((uint64_t)h2 << 16) + (uint64_t(1) << 41);
+                // quotient from the top 62 bits: T' = T >> 44, qe = floor(T' mu / 2^64)
+                const uint64_t tp = (hi << 20) | (lo >> 44);
+                const uint64_t qe = __umul64hi(tp, mu);
+                uint64_t res = lo - qe * q;  // T - qe q < 4q
+                res = res >= 2 * q ? res - 2 * q : res;
+                res = res >= q ? res - q : res;
+                if (row < col_w) {
+                    res = add_mod(res, corr[l * col_w + row], q);
+                    acc_out[(size_t)row * 2 * Ql * N + off] = res;
+                }
             }
         }
     }
 }
 
-typedef void (*ContractFn)(const uint64_t *const *, const i8x16 *, const uint64_t *, uint64_t *, const DevMod *,
-                           int, int, int, size_t, size_t);
+typedef void (*ContractFn)(const uint64_t *const *, const i8x16 *, const uint64_t *, const uint64_t *, uint64_t *,
+                           const DevMod *, int, int, int, size_t, size_t);
+
+constexpr int rb_for(int KS) { return KS == 1 ? 4 : KS == 2 ? 4 : KS == 3 ? 3 : 2; }
 
 ContractFn pick(int DX, int DW, int KS) {
-#define SHM_PICK(dx, dw)                                          \
-    if (DX == dx && DW == dw) {                                   \
-        if (KS == 1) return k_contract_int8<dx, dw, 1>;           \
-        if (KS == 2) return k_contract_int8<dx, dw, 2>;           \
-        if (KS == 3) return k_contract_int8<dx, dw, 3>;           \
-        if (KS == 4) return k_contract_int8<dx, dw, 4>;           \
+#define SHM_PICK(dx, dw)                                                           \
+    if (DX == dx && DW == dw) {                                                    \
+        if (KS == 1) return k_contract_int8<dx, dw, 1, rb_for(1)>;                 \
+        if (KS == 2) return k_contract_int8<dx, dw, 2, rb_for(2)>;                 \
+        if (KS == 3) return k_contract_int8<dx, dw, 3, rb_for(3)>;                 \
+        if (KS == 4) return k_contract_int8<dx, dw, 4, rb_for(4)>;                 \
     }
     SHM_PICK(7, 4)
     SHM_PICK(7, 5)
-    SHM_PICK(7, 6)
     SHM_PICK(8, 4)
-    SHM_PICK(8, 5)
-    SHM_PICK(8, 6)
 #undef SHM_PICK
     return nullptr;
 }
@@ -199,11 +254,14 @@ ContractFn pick(int DX, int DW, int KS) {
 bool int8_supported(const Context &c, size_t Ql, const double *W, const double *scales, size_t row_w, size_t col_w,
                     int *dx_out, int *dw_out) {
     if (row_w > 256 || c.N % 64) return false;
-    int qbits = 0;
+    int qbits = 0, qmin = 64;
     for (size_t l = 0; l < Ql; l++) {
         int b = 64 - __builtin_clzll(c.q[l]);
         qbits = std::max(qbits, b);
+        qmin = std::min(qmin, b);
     }
+    // the single-word quotient estimate needs 2^46 < q < 2^62
+    if (qmin < 47 || qbits > 61) return false;
     const int DX = (qbits + 7) / 8;
     double mx = 0;
     for (size_t j = 0; j < row_w; j++)
@@ -226,27 +284,30 @@ void ctpt_matmul_int8(const Context &c, const std::vector<const Ciphertext *> &X
     int DX = 0, DW = 0;
     if (!int8_supported(c, Ql, W, scales.data(), row_w, col_w, &DX, &DW))
         throw InvalidArgument("weights or shape outside the INT8 MFMA contraction");
-    const int MB = (int)((col_w + 15) / 16), KS = (int)((row_w + 63) / 64);
+    const int MB = (int)((col_w + 15) / 16), KS = (int)((row_w + 63) / 64), RB = rb_for(KS);
+    const int NPH = (MB + RB - 1) / RB;
+    const size_t nfrag = (size_t)NPH * DW * RB * KS * 64;
     Scratch dW(row_w * col_w * sizeof(double), s);
     Scratch dscale(row_w * sizeof(double), s);
-    Scratch frag((size_t)DW * MB * KS * 64 * sizeof(i8x16), s);
+    Scratch frag(nfrag * sizeof(i8x16), s);
     Scratch corr(Ql * col_w * sizeof(uint64_t), s);
+    Scratch mu(Ql * sizeof(uint64_t), s);
     Scratch xptr(row_w * sizeof(uint64_t *), s);
     std::vector<const uint64_t *> xp(row_w);
     for (size_t j = 0; j < row_w; j++) xp[j] = X[j]->data();
     SHM_HIP(hipMemcpyAsync(dW.as<void>(), W, row_w * col_w * sizeof(double), hipMemcpyHostToDevice, s));
     SHM_HIP(hipMemcpyAsync(dscale.as<void>(), scales.data(), row_w * sizeof(double), hipMemcpyHostToDevice, s));
     SHM_HIP(hipMemcpyAsync(xptr.as<void>(), xp.data(), row_w * sizeof(uint64_t *), hipMemcpyHostToDevice, s));
-    k_weight_frags<<<grid_for((size_t)DW * MB * KS * 64), kBlock, 0, s>>>(
-        dW.as<double>(), dscale.as<double>(), frag.as<i8x16>(), row_w, col_w, DW, MB, KS);
+    k_weight_frags<<<grid_for(nfrag), kBlock, 0, s>>>(dW.as<double>(), dscale.as<double>(), frag.as<i8x16>(), row_w,
+                                                     col_w, DW, MB, KS, RB);
     k_weight_corr<<<grid_for(Ql * col_w), kBlock, 0, s>>>(dW.as<double>(), dscale.as<double>(), corr.as<uint64_t>(),
-                                                          c.mods(), row_w, col_w, Ql, DX);
+                                                          mu.as<uint64_t>(), c.mods(), row_w, col_w, Ql, DX);
     SHM_HIP(hipGetLastError());
     const unsigned blocks = (unsigned)(2 * Ql * (N / 64));
     ContractFn fn = pick(DX, DW, KS);
     SHM_HIP(hipEventRecord(ev0, s));
     hipLaunchKernelGGL(fn, dim3(blocks), dim3(256), 0, s, xptr.as<const uint64_t *>(), frag.as<i8x16>(),
-                       corr.as<uint64_t>(), acc, c.mods(), (int)row_w, (int)col_w, MB, Ql, N);
+                       corr.as<uint64_t>(), mu.as<uint64_t>(), acc, c.mods(), (int)row_w, (int)col_w, MB, Ql, N);
     SHM_HIP(hipGetLastError());
     SHM_HIP(hipEventRecord(ev1, s));
     SHM_HIP(hipStreamSynchronize(s));

@@ -166,16 +166,37 @@ __device__ __forceinline__ void run_rounds(uint64_t *lds, uint32_t gmul, uint32_
     }
 }
 
+// Extra I/O of fused variants.
+//   IO_PLAIN   : in-place transform of `data`.
+//   IO_REDUCE  : (phase A, forward) the input of limb k of polynomial p is
+//                src[p * N + idx] reduced mod the limb's prime (a broadcast of
+//                one coefficient-form limb to other primes, rns.cu:1125-1137).
+//   IO_RESCALE : (phase B, forward) instead of storing v = NTT(delta), store
+//                (acc - v) * qinv mod q to outs[p / ppo] + (p % ppo) L1 N + k N
+//                (rns.cu:1138-1155 fused with the final NTT pass).
+enum { IO_PLAIN = 0, IO_REDUCE = 1, IO_RESCALE = 2 };
+
+struct NttIO {
+    const uint64_t *src;      // IO_REDUCE: [n_polys][N]
+    const uint64_t *acc;      // IO_RESCALE: [n_polys][acc_limbs][N]
+    uint64_t *const *outs;    // IO_RESCALE: output base per group of ppo polys (device array)
+    uint64_t *out_single;     // IO_RESCALE: used instead of outs when outs is null
+    const uint64_t *qinv;     // IO_RESCALE: per prime
+    const uint64_t *qinv_s;
+    uint32_t acc_limbs;
+    uint32_t ppo;
+};
+
 // Phase A (columns, first log N1 stages). LOGT = log N1.
 // FWD: forward stages 0..logN1-1 (output stays lazy [0,4q)).
 // !FWD: inverse stages logN1-1..0, then x N^-1, output canonical.
-template <int LOGT, bool FWD>
+template <int LOGT, bool FWD, int IO>
 __global__ __launch_bounds__(kThreads) void k_ntt_cols(uint64_t *__restrict__ data, LimbMap mp,
                                                        const DevMod *__restrict__ mods,
                                                        const uint64_t *__restrict__ tw_all,
                                                        const uint64_t *__restrict__ tws_all,
                                                        const uint64_t *__restrict__ ninv,
-                                                       const uint64_t *__restrict__ ninv_s, int logN) {
+                                                       const uint64_t *__restrict__ ninv_s, int logN, NttIO io) {
     constexpr int T = 1 << LOGT;
     constexpr int COLS = kTile / T;
     constexpr int TP = T + 1;
@@ -191,9 +212,18 @@ __global__ __launch_bounds__(kThreads) void k_ntt_cols(uint64_t *__restrict__ da
     const uint64_t q = mods[prime].q;
     const uint64_t *tw = tw_all + (size_t)prime * N;
     const uint64_t *tws = tws_all + (size_t)prime * N;
-    for (int i = threadIdx.x; i < kTile; i += kThreads) {
-        const int c = i % COLS, r = i / COLS;
-        lds[c * TP + r] = base[(size_t)r * N2 + c];
+    if (IO == IO_REDUCE) {
+        const DevMod m = mods[prime];
+        const uint64_t *src = io.src + (size_t)poly * N + c0;
+        for (int i = threadIdx.x; i < kTile; i += kThreads) {
+            const int c = i % COLS, r = i / COLS;
+            lds[c * TP + r] = reduce64(src[(size_t)r * N2 + c], m);
+        }
+    } else {
+        for (int i = threadIdx.x; i < kTile; i += kThreads) {
+            const int c = i % COLS, r = i / COLS;
+            lds[c * TP + r] = base[(size_t)r * N2 + c];
+        }
     }
     __syncthreads();
     // group id of every column at local stage 0 is 1 (phase A starts at s = 0)
@@ -215,11 +245,11 @@ __global__ __launch_bounds__(kThreads) void k_ntt_cols(uint64_t *__restrict__ da
 // Phase B (contiguous blocks of N2, last log N2 stages). LOGT = log N2.
 // FWD: input lazy [0,4q), output canonical. !FWD: input canonical (< 2q
 // accepted), output lazy [0,2q).
-template <int LOGT, bool FWD>
+template <int LOGT, bool FWD, int IO>
 __global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ data, LimbMap mp,
                                                          const DevMod *__restrict__ mods,
                                                          const uint64_t *__restrict__ tw_all,
-                                                         const uint64_t *__restrict__ tws_all, int logN) {
+                                                         const uint64_t *__restrict__ tws_all, int logN, NttIO io) {
     constexpr int T = 1 << LOGT;
     constexpr int BLKS = kTile / T;
     constexpr int TP = T + 1;
@@ -244,11 +274,25 @@ __global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ 
     run_rounds<LOGT, FWD>(lds, N1, 1u, (uint32_t)(t0 / T), tw, tws, q);
     if (FWD) {
         const uint64_t q2 = q << 1;
-        for (int i = threadIdx.x; i < kTile; i += kThreads) {
-            const int b = i / T, r = i % T;
-            uint64_t v = lds[b * TP + r];
-            v = v >= q2 ? v - q2 : v;
-            base[i] = v >= q ? v - q : v;
+        if (IO == IO_RESCALE) {
+            const uint64_t *acc = io.acc + ((size_t)poly * io.acc_limbs + k) * N + t0;
+            uint64_t *ob = io.outs ? io.outs[poly / io.ppo] : io.out_single;
+            uint64_t *out = ob + ((size_t)(poly % io.ppo) * mp.n_active + k) * N + t0;
+            const uint64_t qi = io.qinv[prime], qis = io.qinv_s[prime];
+            for (int i = threadIdx.x; i < kTile; i += kThreads) {
+                const int b = i / T, r = i % T;
+                uint64_t v = lds[b * TP + r];
+                v = v >= q2 ? v - q2 : v;
+                v = v >= q ? v - q : v;
+                out[i] = mul_shoup(sub_mod(acc[i], v, q), qi, qis, q);
+            }
+        } else {
+            for (int i = threadIdx.x; i < kTile; i += kThreads) {
+                const int b = i / T, r = i % T;
+                uint64_t v = lds[b * TP + r];
+                v = v >= q2 ? v - q2 : v;
+                base[i] = v >= q ? v - q : v;
+            }
         }
     } else {
         for (int i = threadIdx.x; i < kTile; i += kThreads) {
@@ -258,8 +302,9 @@ __global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ 
     }
 }
 
-template <bool FWD>
-void launch_ntt(const Context &c, uint64_t *data, const LimbMap &mp, size_t total_limbs, hipStream_t s) {
+template <bool FWD, int IOA, int IOB>
+void launch_ntt(const Context &c, uint64_t *data, const LimbMap &mp, size_t total_limbs, hipStream_t s,
+                const NttIO &io = NttIO{}) {
     if (total_limbs == 0) return;
     const int logN = c.logN;
     const int log2b = (logN + 1) / 2;  // blocks (contiguous) get the larger half
@@ -269,25 +314,16 @@ void launch_ntt(const Context &c, uint64_t *data, const LimbMap &mp, size_t tota
     const unsigned grid_cols = (unsigned)(total_limbs * ((N >> log1) / (kTile >> log1)));
     const unsigned grid_blk = (unsigned)(total_limbs * (N / kTile));
     const DevMod *mods = c.mods();
-#define SHM_COLS(L)                                                                                         \
-    case L:                                                                                                 \
-        if (FWD)                                                                                            \
-            k_ntt_cols<L, true><<<grid_cols, kThreads, 0, s>>>(data, mp, mods, c.tw.as<uint64_t>(),         \
-                                                               c.tw_s.as<uint64_t>(), c.ninv.as<uint64_t>(), \
-                                                               c.ninv_s.as<uint64_t>(), logN);             \
-        else                                                                                                \
-            k_ntt_cols<L, false><<<grid_cols, kThreads, 0, s>>>(data, mp, mods, c.itw.as<uint64_t>(),       \
-                                                                c.itw_s.as<uint64_t>(), c.ninv.as<uint64_t>(), \
-                                                                c.ninv_s.as<uint64_t>(), logN);            \
+    const uint64_t *twa = FWD ? c.tw.as<uint64_t>() : c.itw.as<uint64_t>();
+    const uint64_t *twsa = FWD ? c.tw_s.as<uint64_t>() : c.itw_s.as<uint64_t>();
+#define SHM_COLS(L)                                                                                             \
+    case L:                                                                                                     \
+        k_ntt_cols<L, FWD, IOA><<<grid_cols, kThreads, 0, s>>>(data, mp, mods, twa, twsa, c.ninv.as<uint64_t>(), \
+                                                               c.ninv_s.as<uint64_t>(), logN, io);             \
         break;
-#define SHM_BLKS(L)                                                                                          \
-    case L:                                                                                                  \
-        if (FWD)                                                                                             \
-            k_ntt_blocks<L, true><<<grid_blk, kThreads, 0, s>>>(data, mp, mods, c.tw.as<uint64_t>(),         \
-                                                                c.tw_s.as<uint64_t>(), logN);                \
-        else                                                                                                 \
-            k_ntt_blocks<L, false><<<grid_blk, kThreads, 0, s>>>(data, mp, mods, c.itw.as<uint64_t>(),       \
-                                                                 c.itw_s.as<uint64_t>(), logN);              \
+#define SHM_BLKS(L)                                                                                   \
+    case L:                                                                                           \
+        k_ntt_blocks<L, FWD, IOB><<<grid_blk, kThreads, 0, s>>>(data, mp, mods, twa, twsa, logN, io); \
         break;
     auto cols = [&]() {
         switch (log1) {
@@ -324,14 +360,14 @@ void ntt_forward(const Context &c, uint64_t *data, size_t n_polys, size_t n_limb
                  hipStream_t s) {
     LimbMap mp{(uint32_t)n_limbs, 0, 0xffffffffu, 0, 0xffffffffu, (uint32_t)c.nQ, (uint32_t)first_prime,
                (uint64_t)n_limbs * c.N};
-    launch_ntt<true>(c, data, mp, n_polys * n_limbs, s);
+    launch_ntt<true, IO_PLAIN, IO_PLAIN>(c, data, mp, n_polys * n_limbs, s);
 }
 
 void ntt_inverse(const Context &c, uint64_t *data, size_t n_polys, size_t n_limbs, size_t first_prime,
                  hipStream_t s) {
     LimbMap mp{(uint32_t)n_limbs, 0, 0xffffffffu, 0, 0xffffffffu, (uint32_t)c.nQ, (uint32_t)first_prime,
                (uint64_t)n_limbs * c.N};
-    launch_ntt<false>(c, data, mp, n_polys * n_limbs, s);
+    launch_ntt<false, IO_PLAIN, IO_PLAIN>(c, data, mp, n_polys * n_limbs, s);
 }
 
 void ntt_forward_qlp(const Context &c, uint64_t *data, size_t Ql, size_t lo, size_t hi, size_t skip_lo,
@@ -344,14 +380,22 @@ void ntt_forward_qlp(const Context &c, uint64_t *data, size_t Ql, size_t lo, siz
         sw = (uint32_t)(skip_hi - skip_lo);
     }
     LimbMap mp{(uint32_t)active, (uint32_t)lo, sl, sw, (uint32_t)Ql, (uint32_t)c.nQ, 0, (uint64_t)poly_stride};
-    launch_ntt<true>(c, data, mp, n_polys * active, s);
+    launch_ntt<true, IO_PLAIN, IO_PLAIN>(c, data, mp, n_polys * active, s);
 }
 
 void ntt_inverse_qlp(const Context &c, uint64_t *data, size_t Ql, size_t lo, size_t hi, size_t n_polys,
                      size_t poly_stride, hipStream_t s) {
     LimbMap mp{(uint32_t)(hi - lo), (uint32_t)lo, 0xffffffffu, 0, (uint32_t)Ql, (uint32_t)c.nQ, 0,
                (uint64_t)poly_stride};
-    launch_ntt<false>(c, data, mp, n_polys * (hi - lo), s);
+    launch_ntt<false, IO_PLAIN, IO_PLAIN>(c, data, mp, n_polys * (hi - lo), s);
+}
+
+void rescale_ntt_fused(const Context &c, const uint64_t *last, const uint64_t *acc, size_t acc_limbs,
+                       uint64_t *delta, uint64_t *const *outs, uint64_t *out_single, size_t ppo, size_t n_polys,
+                       size_t L1, const uint64_t *qinv, const uint64_t *qinv_s, hipStream_t s) {
+    LimbMap mp{(uint32_t)L1, 0, 0xffffffffu, 0, 0xffffffffu, (uint32_t)c.nQ, 0, (uint64_t)L1 * c.N};
+    NttIO io{last, acc, outs, out_single, qinv, qinv_s, (uint32_t)acc_limbs, (uint32_t)ppo};
+    launch_ntt<true, IO_REDUCE, IO_RESCALE>(c, delta, mp, n_polys * L1, s, io);
 }
 
 }  // namespace shm

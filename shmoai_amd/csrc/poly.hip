@@ -1,6 +1,6 @@
 // Elementwise RNS kernels, rescale, Galois permutation, samplers and key/
-// encryption combiners. Grid-stride over [poly][limb][N]; each thread handles
-// two consecutive coefficients (16-byte loads/stores).
+// encryption combiners. Grid-stride loops over [poly][limb][N], one
+// coefficient per thread per iteration.
 #include "engine.h"
 
 namespace shm {
@@ -79,25 +79,12 @@ __global__ void k_square(const uint64_t *a, uint64_t *out, const DevMod *mods, s
     }
 }
 
-// rns.cu:1125-1137: [c_last]_{q_j} for j < Ql-1 (c_last in coefficient form).
-__global__ void k_rescale_reduce(const uint64_t *last, uint64_t *out, const DevMod *mods, size_t L1, size_t N,
-                                 size_t npolys) {
-    const size_t total = npolys * L1 * N;
+// limb `limb` of every polynomial [npolys][Ql][N] -> [npolys][N]
+__global__ void k_gather_limb(const uint64_t *in, uint64_t *out, size_t Ql, size_t limb, size_t N, size_t npolys) {
+    const size_t total = npolys * N;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t p = t / (L1 * N), r = t % (L1 * N);
-        out[t] = reduce64(last[p * N + r % N], mods[r / N]);
-    }
-}
-
-// rns.cu:1138-1155: (c_j - delta_j) * q_last^-1 mod q_j, written into delta.
-__global__ void k_rescale_combine(const uint64_t *in, uint64_t *delta, const DevMod *mods, const uint64_t *inv,
-                                  const uint64_t *inv_s, size_t Ql, size_t N, size_t npolys) {
-    const size_t L1 = Ql - 1, total = npolys * L1 * N;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t p = t / (L1 * N), r = t % (L1 * N), i = r / N;
-        const uint64_t q = mods[i].q;
-        const uint64_t d = sub_mod(in[p * Ql * N + r], delta[t], q);
-        delta[t] = mul_shoup(d, inv[i], inv_s[i], q);
+        const size_t p = t / N, j = t % N;
+        out[t] = in[(p * Ql + limb) * N + j];
     }
 }
 
@@ -257,17 +244,16 @@ void launch_rescale(const Context &c, const uint64_t *in, uint64_t *out, size_t 
     const size_t N = c.N, L1 = Ql - 1;
     Scratch last(npolys * N * sizeof(uint64_t), s);
     Scratch delta(npolys * L1 * N * sizeof(uint64_t), s);
-    for (size_t p = 0; p < npolys; p++)
-        SHM_HIP(hipMemcpyAsync(last.as<uint64_t>() + p * N, in + (p * Ql + L1) * N, N * sizeof(uint64_t),
-                               hipMemcpyDeviceToDevice, s));
+    k_gather_limb<<<grid_for(npolys * N), kBlock, 0, s>>>(in, last.as<uint64_t>(), Ql, L1, N, npolys);
     ntt_inverse(c, last.as<uint64_t>(), npolys, 1, L1, s);
-    k_rescale_reduce<<<grid_for(npolys * L1 * N), kBlock, 0, s>>>(last.as<uint64_t>(), delta.as<uint64_t>(),
-                                                                  c.mods(), L1, N, npolys);
-    ntt_forward(c, delta.as<uint64_t>(), npolys, L1, 0, s);
     const LevelConsts &L = c.level(Ql);
-    k_rescale_combine<<<grid_for(npolys * L1 * N), kBlock, 0, s>>>(
-        in, delta.as<uint64_t>(), c.mods(), L.qlast_inv.as<uint64_t>(), L.qlast_inv_s.as<uint64_t>(), Ql, N, npolys);
-    SHM_HIP(hipMemcpyAsync(out, delta.as<uint64_t>(), npolys * L1 * N * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    // in-place: write into delta (each element is read then written by one thread), then copy back
+    uint64_t *dst = (in == out) ? delta.as<uint64_t>() : out;
+    rescale_ntt_fused(c, last.as<uint64_t>(), in, Ql, delta.as<uint64_t>(), nullptr, dst, npolys, npolys, L1,
+                      L.qlast_inv.as<uint64_t>(), L.qlast_inv_s.as<uint64_t>(), s);
+    if (in == out)
+        SHM_HIP(hipMemcpyAsync(out, delta.as<uint64_t>(), npolys * L1 * N * sizeof(uint64_t), hipMemcpyDeviceToDevice,
+                               s));
     SHM_HIP(hipGetLastError());
 }
 

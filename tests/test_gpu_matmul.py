@@ -40,6 +40,7 @@ def test_matmul_small_bitexact(oracle_lib, variant):
     xs, hs = _rand_cts(ctx, ref, rng, row_w, Ql)
     W = np.random.default_rng(6).normal(0, 0.02, (row_w, col_w))
     out = shm.ct_pt_matrix_mul_wo_pre(hs, W, row_w, col_w, row_w, ctx, variant=variant)
+    assert shm.last_matmul_variant() == variant
     exp = ref.ctpt_matmul(xs, W, Ql, np.full(row_w, 2.0 ** 40))
     for i in range(col_w):
         assert out[i].chain_index == ctx.chain(Ql - 1)
@@ -57,6 +58,7 @@ def test_matmul_quarter_width_int8_equals_u64_and_oracle(oracle_lib):
     xs, hs = _rand_cts(ctx, ref, rng, row_w, Ql)
     W = np.random.default_rng(6).normal(0, 0.02, (row_w, col_w))
     o8 = shm.ct_pt_matrix_mul_wo_pre(hs, W, row_w, col_w, row_w, ctx, variant=shm.MATMUL_INT8)
+    assert shm.last_matmul_variant() == shm.MATMUL_INT8
     o64 = shm.ct_pt_matrix_mul_wo_pre(hs, W, row_w, col_w, row_w, ctx, variant=shm.MATMUL_U64)
     for i in range(col_w):
         assert np.array_equal(o8[i].numpy(), o64[i].numpy()), f"output {i}"
