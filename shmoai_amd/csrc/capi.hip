@@ -71,105 +71,12 @@ void need(const void *p, const char *what) {
     if (!p) throw InvalidArgument(std::string(what) + " is null");
 }
 
-uint64_t splitmix64(uint64_t key, uint64_t ctr) {
-    uint64_t z = key + (ctr + 1) * 0x9E3779B97F4A7C15ULL;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-    return z ^ (z >> 31);
-}
-
 void check_chain(const Context &c, uint32_t chain) {
     if (chain < 1 || chain > c.nQ) throw InvalidArgument("chain_index out of range");
 }
 
 void same_level(const Ciphertext &a, const Ciphertext &b) {
     if (a.chain != b.chain) throw InvalidArgument("encrypted1 and encrypted2 parameter mismatch");
-}
-
-// NTT-form secret over QP from signed coefficients (host int8 -> device).
-void load_secret(Context &c, SecretKey &sk, const int8_t *coeffs) {
-    const size_t N = c.N, QP = c.nQP();
-    Scratch dc(N, nullptr);
-    SHM_HIP(hipMemcpy(dc.as<void>(), coeffs, N, hipMemcpyHostToDevice));
-    sk.s.alloc(QP * N * sizeof(uint64_t));
-    launch_signed_to_rns(c, dc.as<int8_t>(), sk.s.as<uint64_t>(), c.nQ, true, nullptr);
-    ntt_forward(c, sk.s.as<uint64_t>(), 1, QP, 0, nullptr);
-    SHM_HIP(hipDeviceSynchronize());
-}
-
-// secretkey.cu:328-373: dnum digits of (-(a s + e) + P [newkey]_{digit}, a).
-void gen_kswitch(Context &c, const SecretKey &sk, const uint64_t *newkey_dev, uint64_t seed, KSwitchKey &out) {
-    const size_t N = c.N, QP = c.nQP(), dnum = c.dnum;
-    out.ctx = &c;
-    out.k.alloc(dnum * 2 * QP * N * sizeof(uint64_t));
-    Scratch e(QP * N * sizeof(uint64_t), nullptr);
-    for (size_t g = 0; g < dnum; g++) {
-        uint64_t *b = out.k.as<uint64_t>() + g * 2 * QP * N, *a = b + QP * N;
-        launch_sample_uniform(c, splitmix64(seed, 2 * g), a, c.nQ, true, nullptr);
-        launch_sample_gaussian_rns(c, splitmix64(seed, 2 * g + 1), e.as<uint64_t>(), c.nQ, true, nullptr);
-        ntt_forward(c, e.as<uint64_t>(), 1, QP, 0, nullptr);
-        launch_ks_key_digit(c, sk.s.as<uint64_t>(), newkey_dev, b, a, e.as<uint64_t>(), g, nullptr);
-    }
-    SHM_HIP(hipDeviceSynchronize());
-}
-
-void apply_galois(Context &c, Ciphertext &ct, uint32_t elt, const GaloisKeys &gk, hipStream_t s) {
-    if (ct.size != 2) throw InvalidArgument("encrypted size must be 2");
-    auto it = gk.keys.find(elt);
-    if (it == gk.keys.end()) throw InvalidArgument("Galois elt not present");
-    const size_t Ql = ct.Ql(), N = c.N;
-    const uint32_t *table = c.galois_table(elt, s);
-    Scratch t0(Ql * N * sizeof(uint64_t), s), t1(Ql * N * sizeof(uint64_t), s);
-    launch_permute(c, ct.poly(0), t0.as<uint64_t>(), table, Ql, s);
-    launch_permute(c, ct.poly(1), t1.as<uint64_t>(), table, Ql, s);
-    SHM_HIP(hipMemcpyAsync(ct.poly(0), t0.as<uint64_t>(), Ql * N * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
-    keyswitch(c, t1.as<uint64_t>(), it->second->k.as<uint64_t>(), Ql, ct.poly(0), ct.poly(1), false, s);
-}
-
-// evaluate.cu:1650-1706 rotate_internal (NAF split, numth.h:17-34).
-void rotate_internal(Context &c, Ciphertext &ct, int step, const GaloisKeys &gk, hipStream_t s) {
-    if (step == 0) return;
-    const uint32_t elt = c.elt_from_step(step);
-    if (gk.keys.count(elt)) {
-        apply_galois(c, ct, elt, gk, s);
-        return;
-    }
-    std::vector<int> naf;
-    {
-        const bool sign = step < 0;
-        int v = std::abs(step);
-        for (int i = 0; v; i++) {
-            const int zi = (v & 1) ? 2 - (v & 3) : 0;
-            v = (v - zi) >> 1;
-            if (zi) naf.push_back((sign ? -zi : zi) * (1 << i));
-        }
-    }
-    if (naf.size() == 1) throw InvalidArgument("Galois key not present");
-    for (int t : naf)
-        if ((size_t)std::abs(t) != (c.N >> 1)) rotate_internal(c, ct, t, gk, s);
-}
-
-void rescale_inplace(Context &c, Ciphertext &ct, hipStream_t s) {
-    const size_t Ql = ct.Ql();
-    if (Ql < 2) throw LogicError("end of modulus switching chain reached");
-    if (!ct.ntt) throw InvalidArgument("CKKS encrypted must be in NTT form");
-    launch_rescale(c, ct.data(), ct.data(), Ql, ct.size, s);
-    ct.chain += 1;
-    ct.scale = ct.scale / (double)c.q[Ql - 1];
-}
-
-void mod_switch_drop(Context &c, Ciphertext &ct, size_t new_chain, hipStream_t s) {
-    const size_t Ql = ct.Ql(), nl = c.ql_of_chain(new_chain);
-    if (new_chain < ct.chain) throw InvalidArgument("cannot switch to a higher level");
-    if (new_chain == ct.chain) return;
-    // compact [size][Ql][N] -> [size][nl][N] through a staging copy (ranges overlap)
-    if (ct.size > 1) {
-        Scratch tmp(ct.size * nl * c.N * sizeof(uint64_t), s);
-        launch_copy_limbs(c, ct.data(), Ql, tmp.as<uint64_t>(), nl, ct.size, s);
-        SHM_HIP(hipMemcpyAsync(ct.data(), tmp.as<uint64_t>(), ct.size * nl * c.N * sizeof(uint64_t),
-                               hipMemcpyDeviceToDevice, s));
-    }
-    ct.chain = new_chain;
 }
 
 }  // namespace
@@ -423,9 +330,9 @@ int shm_secret_key_generate(shm_context *ctx, uint64_t seed, uint32_t hw, shm_se
         for (size_t i = 0; i < N; i++) perm[i] = (uint32_t)i;
         std::vector<int8_t> coeff(N, 0);
         for (size_t i = 0; i < hw; i++) {
-            const size_t j = i + (size_t)(splitmix64(seed, i) % (uint64_t)(N - i));
+            const size_t j = i + (size_t)(host_splitmix64(seed, i) % (uint64_t)(N - i));
             std::swap(perm[i], perm[j]);
-            coeff[perm[i]] = (splitmix64(seed, N + i) & 1) ? -1 : 1;
+            coeff[perm[i]] = (host_splitmix64(seed, N + i) & 1) ? -1 : 1;
         }
         auto h = new shm_secret_key();
         h->sk.ctx = ctx->c.get();
@@ -530,7 +437,7 @@ int shm_galois_keys_generate(shm_context *ctx, const shm_secret_key *sk, const i
                 const uint32_t *table = c.galois_table(elt, nullptr);
                 launch_permute(c, sk->sk.s.as<uint64_t>(), rs.as<uint64_t>(), table, QP, nullptr);
                 auto k = std::make_unique<KSwitchKey>();
-                gen_kswitch(c, sk->sk, rs.as<uint64_t>(), splitmix64(seed, elt), *k);
+                gen_kswitch(c, sk->sk, rs.as<uint64_t>(), host_splitmix64(seed, elt), *k);
                 h->g.keys[elt] = std::move(k);
             }
         } catch (...) {
@@ -574,9 +481,9 @@ int shm_encrypt_symmetric(shm_context *ctx, const shm_secret_key *sk, const shm_
         out->ct.scale = pt->pt.scale;
         out->ct.ntt = true;
         uint64_t *c0 = out->ct.poly(0), *c1 = out->ct.poly(1);
-        launch_sample_uniform(c, splitmix64(seed, 0), c1, Ql, false, s);
+        launch_sample_uniform(c, host_splitmix64(seed, 0), c1, Ql, false, s);
         Scratch e(Ql * N * sizeof(uint64_t), s);
-        launch_sample_gaussian_rns(c, splitmix64(seed, 1), e.as<uint64_t>(), Ql, false, s);
+        launch_sample_gaussian_rns(c, host_splitmix64(seed, 1), e.as<uint64_t>(), Ql, false, s);
         ntt_forward(c, e.as<uint64_t>(), 1, Ql, 0, s);
         launch_encrypt_combine(c, sk->sk.s.as<uint64_t>(), c1, e.as<uint64_t>(), pt->pt.data(), c0, Ql, s);
     });

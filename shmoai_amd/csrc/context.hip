@@ -221,6 +221,13 @@ std::unique_ptr<Context> make_context(int logN, const std::vector<int> &bits, si
         ninvs[p] = h_shoup(ninv[p], q);
     }
     upload(c->d_mod, mods);
+    std::vector<uint64_t> io(QP), ios(QP);
+    for (size_t p = 0; p < QP; p++) {
+        io[p] = h_powmod(c->psi[p], N / 2, c->q[p]);
+        ios[p] = h_shoup(io[p], c->q[p]);
+    }
+    upload(c->iota, io);
+    upload(c->iota_s, ios);
     upload(c->tw, tw);
     upload(c->tw_s, tws);
     upload(c->itw, itw);

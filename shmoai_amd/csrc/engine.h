@@ -122,6 +122,9 @@ struct Context {
     DevBuf phat_mod;        // [nQ][nP] (P/p_j) mod q_i
     std::vector<std::unique_ptr<LevelConsts>> levels;  // index Ql (0 unused)
 
+    // psi^(N/2) per prime (value, shoup): the NTT-slot factor of X^(N/2)
+    DevBuf iota, iota_s;
+
     // encoder
     DevBuf enc_roots;       // complex<double>[2N] (re, im interleaved)
     DevBuf enc_group;       // uint32[N/4]
@@ -142,6 +145,9 @@ struct Context {
 std::unique_ptr<Context> make_context(int logN, const std::vector<int> &bits, size_t nP);
 
 struct Ciphertext {
+    Ciphertext() = default;
+    Ciphertext(Ciphertext &&) = default;
+    Ciphertext &operator=(Ciphertext &&) = default;
     Context *ctx = nullptr;
     size_t size = 0;      // polynomials (2 or 3)
     size_t chain = 1;     // chain index (1 = all of Q)
@@ -243,6 +249,25 @@ void decode(const Context &c, const uint64_t *pt, size_t Ql, double scale, doubl
 // MOAI matmul (matmul.hip)
 void ctpt_matmul(const Context &c, const std::vector<const Ciphertext *> &X, const double *W, size_t row_w,
                  size_t col_w, const std::vector<Ciphertext *> &out, int variant, hipStream_t s);
+
+// evaluator-level operations (ops.hip)
+uint64_t host_splitmix64(uint64_t key, uint64_t ctr);
+void load_secret(Context &c, SecretKey &sk, const int8_t *coeffs);
+void gen_kswitch(Context &c, const SecretKey &sk, const uint64_t *newkey_dev, uint64_t seed, KSwitchKey &out);
+void apply_galois(Context &c, Ciphertext &ct, uint32_t elt, const GaloisKeys &gk, hipStream_t s);
+void rotate_internal(Context &c, Ciphertext &ct, int step, const GaloisKeys &gk, hipStream_t s);
+void rescale_inplace(Context &c, Ciphertext &ct, hipStream_t s);
+void mod_switch_drop(Context &c, Ciphertext &ct, size_t new_chain, hipStream_t s);
+void ct_clone(Ciphertext &dst, const Ciphertext &src, hipStream_t s);
+void ct_add(Context &c, Ciphertext &a, const Ciphertext &b, hipStream_t s);
+void ct_sub(Context &c, Ciphertext &a, const Ciphertext &b, hipStream_t s);
+void ct_add_const(Context &c, Ciphertext &a, double v, hipStream_t s);
+void ct_mul_const_rescale(Context &c, Ciphertext &a, double v, double sigma, hipStream_t s);
+void ct_align(Context &c, Ciphertext &a, size_t chain, double scale, hipStream_t s);
+void ct_mul_relin_rescale(Context &c, Ciphertext &out, const Ciphertext &a, const Ciphertext &b,
+                          const KSwitchKey &rk, hipStream_t s);
+void ct_mul_monomial_half(Context &c, Ciphertext &a, bool negate, hipStream_t s);
+void ct_modraise(Context &c, Ciphertext &a, hipStream_t s);
 
 // timing hook for the dominant kernel of the last matmul (HIP events)
 float last_matmul_kernel_ms();

@@ -1,0 +1,257 @@
+// Evaluator-level operations shared by the C ABI and the bootstrapping
+// pipeline: key generation, Galois automorphisms with key switching, rescale,
+// level dropping, and the helpers that keep levels and scales aligned.
+#include <cmath>
+#include <vector>
+
+#include "engine.h"
+
+namespace shm {
+
+uint64_t host_splitmix64(uint64_t key, uint64_t ctr) {
+    uint64_t z = key + (ctr + 1) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+// NTT-form secret over QP from signed coefficients (host int8 -> device).
+void load_secret(Context &c, SecretKey &sk, const int8_t *coeffs) {
+    const size_t N = c.N, QP = c.nQP();
+    Scratch dc(N, nullptr);
+    SHM_HIP(hipMemcpy(dc.as<void>(), coeffs, N, hipMemcpyHostToDevice));
+    sk.s.alloc(QP * N * sizeof(uint64_t));
+    launch_signed_to_rns(c, dc.as<int8_t>(), sk.s.as<uint64_t>(), c.nQ, true, nullptr);
+    ntt_forward(c, sk.s.as<uint64_t>(), 1, QP, 0, nullptr);
+    SHM_HIP(hipDeviceSynchronize());
+}
+
+// secretkey.cu:328-373: dnum digits of (-(a s + e) + P [newkey]_{digit}, a).
+void gen_kswitch(Context &c, const SecretKey &sk, const uint64_t *newkey_dev, uint64_t seed, KSwitchKey &out) {
+    const size_t N = c.N, QP = c.nQP(), dnum = c.dnum;
+    out.ctx = &c;
+    out.k.alloc(dnum * 2 * QP * N * sizeof(uint64_t));
+    Scratch e(QP * N * sizeof(uint64_t), nullptr);
+    for (size_t g = 0; g < dnum; g++) {
+        uint64_t *b = out.k.as<uint64_t>() + g * 2 * QP * N, *a = b + QP * N;
+        launch_sample_uniform(c, host_splitmix64(seed, 2 * g), a, c.nQ, true, nullptr);
+        launch_sample_gaussian_rns(c, host_splitmix64(seed, 2 * g + 1), e.as<uint64_t>(), c.nQ, true, nullptr);
+        ntt_forward(c, e.as<uint64_t>(), 1, QP, 0, nullptr);
+        launch_ks_key_digit(c, sk.s.as<uint64_t>(), newkey_dev, b, a, e.as<uint64_t>(), g, nullptr);
+    }
+    SHM_HIP(hipDeviceSynchronize());
+}
+
+void apply_galois(Context &c, Ciphertext &ct, uint32_t elt, const GaloisKeys &gk, hipStream_t s) {
+    if (ct.size != 2) throw InvalidArgument("encrypted size must be 2");
+    auto it = gk.keys.find(elt);
+    if (it == gk.keys.end()) throw InvalidArgument("Galois elt not present");
+    const size_t Ql = ct.Ql(), N = c.N;
+    const uint32_t *table = c.galois_table(elt, s);
+    Scratch t0(Ql * N * sizeof(uint64_t), s), t1(Ql * N * sizeof(uint64_t), s);
+    launch_permute(c, ct.poly(0), t0.as<uint64_t>(), table, Ql, s);
+    launch_permute(c, ct.poly(1), t1.as<uint64_t>(), table, Ql, s);
+    SHM_HIP(hipMemcpyAsync(ct.poly(0), t0.as<uint64_t>(), Ql * N * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    keyswitch(c, t1.as<uint64_t>(), it->second->k.as<uint64_t>(), Ql, ct.poly(0), ct.poly(1), false, s);
+}
+
+// evaluate.cu:1650-1706 rotate_internal (NAF split, numth.h:17-34).
+void rotate_internal(Context &c, Ciphertext &ct, int step, const GaloisKeys &gk, hipStream_t s) {
+    if (step == 0) return;
+    const uint32_t elt = c.elt_from_step(step);
+    if (gk.keys.count(elt)) {
+        apply_galois(c, ct, elt, gk, s);
+        return;
+    }
+    std::vector<int> naf;
+    {
+        const bool sign = step < 0;
+        int v = std::abs(step);
+        for (int i = 0; v; i++) {
+            const int zi = (v & 1) ? 2 - (v & 3) : 0;
+            v = (v - zi) >> 1;
+            if (zi) naf.push_back((sign ? -zi : zi) * (1 << i));
+        }
+    }
+    if (naf.size() == 1) throw InvalidArgument("Galois key not present");
+    for (int t : naf)
+        if ((size_t)std::abs(t) != (c.N >> 1)) rotate_internal(c, ct, t, gk, s);
+}
+
+void rescale_inplace(Context &c, Ciphertext &ct, hipStream_t s) {
+    const size_t Ql = ct.Ql();
+    if (Ql < 2) throw LogicError("end of modulus switching chain reached");
+    if (!ct.ntt) throw InvalidArgument("CKKS encrypted must be in NTT form");
+    launch_rescale(c, ct.data(), ct.data(), Ql, ct.size, s);
+    ct.chain += 1;
+    ct.scale = ct.scale / (double)c.q[Ql - 1];
+}
+
+void mod_switch_drop(Context &c, Ciphertext &ct, size_t new_chain, hipStream_t s) {
+    const size_t Ql = ct.Ql(), nl = c.ql_of_chain(new_chain);
+    if (new_chain < ct.chain) throw InvalidArgument("cannot switch to a higher level");
+    if (new_chain == ct.chain) return;
+    // compact [size][Ql][N] -> [size][nl][N] through a staging copy (ranges overlap)
+    if (ct.size > 1) {
+        Scratch tmp(ct.size * nl * c.N * sizeof(uint64_t), s);
+        launch_copy_limbs(c, ct.data(), Ql, tmp.as<uint64_t>(), nl, ct.size, s);
+        SHM_HIP(hipMemcpyAsync(ct.data(), tmp.as<uint64_t>(), ct.size * nl * c.N * sizeof(uint64_t),
+                               hipMemcpyDeviceToDevice, s));
+    }
+    ct.chain = new_chain;
+}
+
+
+// ---- level / scale management ---------------------------------------------------
+namespace {
+
+constexpr int kBlockOps = 256;
+inline unsigned grid_ops(size_t work) {
+    size_t g = (work + kBlockOps - 1) / kBlockOps;
+    if (g > (1u << 20)) g = 1u << 20;
+    return (unsigned)(g ? g : 1);
+}
+
+// Bootstrapper.cu:3271-3284 kernel_modraise_inplace: limb 0 (coefficient form,
+// prime q0) lifted to every prime; values above floor(q0/2) represent x - q0.
+__global__ void k_modraise(const uint64_t *src0, uint64_t *dst, const DevMod *mods, size_t nl, size_t N) {
+    const uint64_t q0 = mods[0].q;
+    const size_t total = nl * N;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t j = t / N, i = t % N;
+        const DevMod m = mods[j];
+        const uint64_t x = src0[i];
+        uint64_t v = reduce64(x, m);
+        if (x > (q0 >> 1)) v = sub_mod(v, reduce64(q0, m), m.q);
+        dst[t] = v;
+    }
+}
+
+// Multiply by the monomial X^(N/2): in NTT slot j the factor is
+// psi^((2 brv(j) + 1) N/2) = iota (-1)^brv(j) with iota = psi^(N/2); brv(j) is odd
+// exactly for j >= N/2. `negate` multiplies by X^(3N/2) = -X^(N/2).
+__global__ void k_mono_half(uint64_t *a, const DevMod *mods, const uint64_t *iota, const uint64_t *iota_s, size_t Ql,
+                            size_t N, size_t total, int negate) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t limb = (t / N) % Ql, j = t % N;
+        const uint64_t q = mods[limb].q;
+        uint64_t v = mul_shoup(a[t], iota[limb], iota_s[limb], q);
+        const bool neg = (j >= N / 2) != (negate != 0);
+        a[t] = neg ? neg_mod(v, q) : v;
+    }
+}
+
+}  // namespace
+
+void ct_clone(Ciphertext &dst, const Ciphertext &src, hipStream_t s) {
+    dst.ctx = src.ctx;
+    dst.resize(src.size, src.chain);
+    dst.scale = src.scale;
+    dst.ntt = src.ntt;
+    SHM_HIP(hipMemcpyAsync(dst.data(), src.data(), src.size * src.Ql() * src.ctx->N * sizeof(uint64_t),
+                           hipMemcpyDeviceToDevice, s));
+}
+
+void ct_add(Context &c, Ciphertext &a, const Ciphertext &b, hipStream_t s) {
+    if (a.chain != b.chain || a.size != b.size) throw InvalidArgument("ct_add: level or size mismatch");
+    launch_add(c, a.data(), b.data(), a.Ql(), a.size, s);
+}
+
+void ct_sub(Context &c, Ciphertext &a, const Ciphertext &b, hipStream_t s) {
+    if (a.chain != b.chain || a.size != b.size) throw InvalidArgument("ct_sub: level or size mismatch");
+    launch_sub(c, a.data(), b.data(), a.Ql(), a.size, s);
+}
+
+// a += constant v (encoded at a's exact scale).
+void ct_add_const(Context &c, Ciphertext &a, double v, hipStream_t s) {
+    const size_t Ql = a.Ql();
+    Scratch pt(Ql * c.N * sizeof(uint64_t), s);
+    encode_scalar(c, v, Ql, a.scale, pt.as<uint64_t>(), s);
+    launch_add(c, a.data(), pt.as<uint64_t>(), Ql, 1, s);
+}
+
+// a <- rescale(a * [v encoded at scale sigma]); scale becomes a.scale * sigma / q_last.
+void ct_mul_const_rescale(Context &c, Ciphertext &a, double v, double sigma, hipStream_t s) {
+    const size_t Ql = a.Ql();
+    if (Ql < 2) throw LogicError("end of modulus switching chain reached");
+    Scratch pt(Ql * c.N * sizeof(uint64_t), s);
+    encode_scalar(c, v, Ql, sigma, pt.as<uint64_t>(), s);
+    launch_mul_plain(c, a.data(), pt.as<uint64_t>(), Ql, a.size, s);
+    a.scale *= sigma;
+    rescale_inplace(c, a, s);
+}
+
+// Bring a to (chain, scale) exactly: one constant multiply + rescale at a's
+// level (the constant 1 encoded at scale * q_last / a.scale), then drop limbs.
+void ct_align(Context &c, Ciphertext &a, size_t chain, double scale, hipStream_t s) {
+    if (a.chain > chain) throw LogicError("ct_align: ciphertext is already below the target level");
+    if (a.chain == chain) {
+        if (std::fabs(a.scale / scale - 1.0) > 1e-12) throw LogicError("ct_align: scales differ at equal levels");
+        a.scale = scale;
+        return;
+    }
+    const double qlast = (double)c.q[a.Ql() - 1];
+    ct_mul_const_rescale(c, a, 1.0, scale * qlast / a.scale, s);
+    a.scale = scale;  // equal up to the encoding rounding of the constant
+    if (a.chain < chain) mod_switch_drop(c, a, chain, s);
+}
+
+// out = rescale(relinearize(a * b)); operands are brought to the lower level first.
+void ct_mul_relin_rescale(Context &c, Ciphertext &out, const Ciphertext &a, const Ciphertext &b,
+                          const KSwitchKey &rk, hipStream_t s) {
+    if (a.size != 2 || b.size != 2) throw InvalidArgument("multiply supports size-2 ciphertexts");
+    const size_t chain = std::max(a.chain, b.chain);
+    Ciphertext ta, tb;
+    const Ciphertext *pa = &a, *pb = &b;
+    if (a.chain != chain) {
+        ct_clone(ta, a, s);
+        mod_switch_drop(c, ta, chain, s);
+        pa = &ta;
+    }
+    if (b.chain != chain) {
+        ct_clone(tb, b, s);
+        mod_switch_drop(c, tb, chain, s);
+        pb = &tb;
+    }
+    const size_t Ql = c.ql_of_chain(chain);
+    out.ctx = &c;
+    out.resize(3, chain);
+    if (pa == pb || (pa->data() == pb->data()))
+        launch_square(c, pa->data(), out.data(), Ql, s);
+    else
+        launch_tensor(c, pa->data(), pb->data(), out.data(), Ql, s);
+    out.scale = pa->scale * pb->scale;
+    out.ntt = true;
+    keyswitch(c, out.poly(2), rk.k.as<uint64_t>(), Ql, out.poly(0), out.poly(1), true, s);
+    out.size = 2;
+    rescale_inplace(c, out, s);
+}
+
+void ct_mul_monomial_half(Context &c, Ciphertext &a, bool negate, hipStream_t s) {
+    const size_t total = a.size * a.Ql() * c.N;
+    k_mono_half<<<grid_ops(total), kBlockOps, 0, s>>>(a.data(), c.mods(), c.iota.as<uint64_t>(),
+                                                      c.iota_s.as<uint64_t>(), a.Ql(), c.N, total, negate ? 1 : 0);
+    SHM_HIP(hipGetLastError());
+}
+
+// Bootstrapper.cu:3287-3356 modraise_inplace: bottom-level (1 prime) ciphertext
+// lifted to all data primes; the result is in NTT form at chain index 1.
+void ct_modraise(Context &c, Ciphertext &a, hipStream_t s) {
+    if (a.size != 2) throw InvalidArgument("Ciphertexts of size 2 are supported only!");
+    if (a.Ql() != 1) throw InvalidArgument("Ciphertexts in the lowest level are supported only!");
+    const size_t N = c.N, nl = c.nQ;
+    Scratch low(2 * N * sizeof(uint64_t), s);
+    SHM_HIP(hipMemcpyAsync(low.as<uint64_t>(), a.data(), 2 * N * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    ntt_inverse(c, low.as<uint64_t>(), 2, 1, 0, s);
+    Ciphertext out;
+    out.ctx = &c;
+    out.resize(2, 1);
+    for (size_t p = 0; p < 2; p++)
+        k_modraise<<<grid_ops(nl * N), kBlockOps, 0, s>>>(low.as<uint64_t>() + p * N, out.poly(p), c.mods(), nl, N);
+    SHM_HIP(hipGetLastError());
+    ntt_forward(c, out.data(), 2, nl, 0, s);
+    out.scale = a.scale;
+    a = std::move(out);
+}
+
+}  // namespace shm
