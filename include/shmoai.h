@@ -199,6 +199,74 @@ double shm_last_matmul_kernel_ms(void);
  * 5 balanced digits (4 for 8-digit limbs) */
 int shm_last_matmul_variant(void);
 
+/* ---- bootstrapping (slot-to-coefficient first) ------------------------------------
+ * Replaces Bootstrapper (src/include/source/bootstrapping/Bootstrapper.cuh:63) and
+ * the ordinary path of BatchBootstrapper (BatchBootstrap.cuh:470-820). Full slots
+ * (n = N/2). Transform layers follow sfl_full_3 / coefftoslot_full_3
+ * (Bootstrapper.cu:2737-2777, 3048-3067); EvalMod is a Chebyshev cosine of degree
+ * `deg` on [-K, K] followed by `double_angles` double-angle steps. */
+typedef struct shm_bootstrapper shm_bootstrapper;
+
+typedef struct shm_boot_params {
+    uint32_t s2c_input_primes; /* must be 4 (BatchBootstrap.cuh:727-728) */
+    double layer3_factor;      /* F of the third S2C layer (kLowS2cLayer3Factor = 1/16) */
+    int32_t K;                 /* EvalMod range: |integer part| < K (boundary_K = 25) */
+    int32_t deg;               /* Chebyshev degree, 16..63 */
+    int32_t double_angles;     /* r (num_double_formula = 2) */
+    double input_bound;        /* bound on |Re z|, |Im z| of the slots (1.0) */
+} shm_boot_params;
+
+/* Bootstrapper ctor + generate_LT_coefficient_3 + prepare_mod_polynomial
+ * (Bootstrapper.cu:3-20, 2135-2147; coefficients 634-727, 1356-2119); host only */
+int shm_bootstrapper_create(shm_context *ctx, const shm_boot_params *params, shm_bootstrapper **out);
+/* generate_LT_coefficient_3 + prepare_mod_polynomial alone (Bootstrapper.cu:2135-2147):
+ * the host plan for N = 2^log_poly_degree and size_q data primes, without a context;
+ * it can be inspected (steps, layers, diagonals, coefficients) but not run */
+int shm_bootstrapper_create_plan(uint32_t log_poly_degree, uint32_t size_q, const shm_boot_params *params,
+                                 shm_bootstrapper **out);
+int shm_bootstrapper_destroy(shm_bootstrapper *b);
+/* addBootKeys_3 (Bootstrapper.cu:456-479): rotation steps the transforms use, step 0 =
+ * complex conjugation. Pass to shm_galois_keys_generate. `count` receives the total. */
+int shm_bootstrapper_rotation_steps(const shm_bootstrapper *b, int *steps, size_t cap, size_t *count);
+/* Layer schedule: transform 0 = slot-to-coefficient, 1 = coefficient-to-slot; layer
+ * 0..2 in application order. info = {base step, k0, babies g, giants, chain_index}. */
+int shm_bootstrapper_layer_info(const shm_bootstrapper *b, int transform, int layer, int64_t info[5]);
+/* Pre-rotated diagonal (giant i, baby j) of a layer, n = N/2 complex values
+ * (fftcoeff1/2/3 after rotation(); Bootstrapper.cu:2196-2197). Returns zeros for an
+ * absent diagonal. */
+int shm_bootstrapper_layer_diagonal(const shm_bootstrapper *b, int transform, int layer, int giant, int baby,
+                                    double *re, double *im, size_t n);
+/* EvalMod Chebyshev coefficients (deg + 1 values) */
+int shm_bootstrapper_evalmod_coeffs(const shm_bootstrapper *b, double *out, size_t cap, size_t *count);
+/* Encode every diagonal on the device at the scale of the prime its rescale drops
+ * (BatchBootstrapper::prepare, BatchBootstrap.cuh:492). Synchronises the stream. */
+int shm_bootstrapper_prepare(shm_bootstrapper *b, shm_stream_t stream);
+
+/* Bootstrapper::modraise_inplace (Bootstrapper.cu:3287-3356): a 1-prime ciphertext
+ * lifted to every data prime (centred: residues above q0/2 stand for x - q0). The
+ * scale is left unchanged; callers set it to q0. */
+int shm_modraise_inplace(shm_context *ctx, shm_ciphertext *ct, shm_stream_t stream);
+/* Multiply by X^(N/2) (every slot times i), or by X^(3N/2) (times -i) when negate != 0:
+ * the exact form of the encode(i) multiply_plain in coefftoslot_full_3 /
+ * ordinary_evalmod_combine_batch (Bootstrapper.cu:3053-3062; BatchBootstrap.cuh:216-224) */
+int shm_multiply_by_i_inplace(shm_context *ctx, shm_ciphertext *ct, int negate, shm_stream_t stream);
+/* sfl_full_3 (Bootstrapper.cu:2737-2777) at the 4-prime level: 3 levels consumed */
+int shm_bootstrap_slot_to_coeff(shm_bootstrapper *b, shm_ciphertext *ct, const shm_galois_keys *gk,
+                                shm_stream_t stream);
+/* coefftoslot_full_3 (Bootstrapper.cu:3048-3067): ct (top level, after modraise) ->
+ * real and imaginary parts, each divided by K (the EvalMod input form) */
+int shm_bootstrap_coeff_to_slot(shm_bootstrapper *b, shm_ciphertext *ct, shm_ciphertext *re, shm_ciphertext *im,
+                                const shm_galois_keys *gk, shm_stream_t stream);
+/* ModularReducer::modular_reduction (ModularReducer.cu:64-84): slots y -> sin(2 pi K y) */
+int shm_bootstrap_eval_mod(shm_bootstrapper *b, shm_ciphertext *ct, const shm_kswitch_key *relin,
+                           shm_stream_t stream);
+/* BatchBootstrapper::bootstrap_ordinary_s2c_first (BatchBootstrap.cuh:674-709) over
+ * `count` ciphertexts with exactly 4 primes; out[i] receives the refreshed
+ * ciphertext at scale out_scale (0 = the input scale). Inputs are not modified. */
+int shm_bootstrap_s2c_first(shm_bootstrapper *b, const shm_ciphertext *const *in, size_t count,
+                            shm_ciphertext *const *out, const shm_galois_keys *gk, const shm_kswitch_key *relin,
+                            double out_scale, shm_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

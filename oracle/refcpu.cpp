@@ -1040,4 +1040,52 @@ void ref_ctpt_matmul(const RefCtx *c, const uint64_t *const *X, size_t row_w, co
     }
 }
 
+
+/* Bootstrapper.cu:3271-3284 kernel_modraise_inplace, 3287-3356 modraise_inplace:
+ * in = [2][N] (prime 0, NTT form) -> out = [2][nQ][N] (NTT form): iNTT, lift each
+ * coefficient x of [0, q0) to x (x <= q0/2) or x - q0 (x > q0/2) mod every data
+ * prime, NTT. */
+void ref_modraise(const RefCtx *c, const uint64_t *in, uint64_t *out) {
+    const size_t N = c->N, nq = c->nQ;
+    const uint64_t q0 = c->q[0];
+    for (size_t p = 0; p < 2; p++) {
+        std::vector<uint64_t> a(in + p * N, in + (p + 1) * N);
+        intt_limb(*c, a.data(), 0);
+        for (size_t k = 0; k < nq; k++) {
+            const uint64_t q = c->q[k];
+            uint64_t *o = out + (p * nq + k) * N;
+            for (size_t i = 0; i < N; i++) {
+                uint64_t v = a[i] % q;
+                if (a[i] > (q0 >> 1)) v = submod(v, q0 % q, q);
+                o[i] = v;
+            }
+            ntt_limb(*c, o, k);
+        }
+    }
+}
+
+/* Multiply npolys x [nlimbs][N] NTT-form polynomials (primes 0..nlimbs-1) by the
+ * monomial X^power (0 <= power < 2N) in the coefficient domain: negacyclic shift. */
+void ref_mul_monomial(const RefCtx *c, uint64_t *data, size_t nlimbs, size_t npolys, size_t power) {
+    const size_t N = c->N;
+    for (size_t p = 0; p < npolys; p++) {
+        for (size_t k = 0; k < nlimbs; k++) {
+            uint64_t *a = data + (p * nlimbs + k) * N;
+            const uint64_t q = c->q[k];
+            intt_limb(*c, a, k);
+            std::vector<uint64_t> r(N);
+            for (size_t i = 0; i < N; i++) {
+                const size_t e = (i + power) % (2 * N);
+                const uint64_t v = a[i];
+                if (e < N)
+                    r[e] = v;
+                else
+                    r[e - N] = v ? q - v : 0;
+            }
+            std::copy(r.begin(), r.end(), a);
+            ntt_limb(*c, a, k);
+        }
+    }
+}
+
 }  // extern "C"

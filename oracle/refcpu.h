@@ -91,6 +91,8 @@ void ref_decode(const RefCtx *c, const uint64_t *pt, size_t Ql, size_t sparse_sl
                 double *re, double *im);
 
 /* MOAI ct-pt matmul --------------------------------------------------------- */
+void ref_modraise(const RefCtx *c, const uint64_t *in, uint64_t *out);
+void ref_mul_monomial(const RefCtx *c, uint64_t *data, size_t nlimbs, size_t npolys, size_t power);
 void ref_ctpt_matmul(const RefCtx *c, const uint64_t *const *X, size_t row_w, const double *W, size_t col_w,
                      size_t Ql, const double *scales, size_t col_begin, size_t col_end, uint64_t *const *out);
 

@@ -82,6 +82,8 @@ def lib():
             "ref_decode": (None, [vp, u64p, sz, sz, C.c_double, f64p, f64p]),
             "ref_ctpt_matmul": (None, [vp, C.POINTER(C.c_void_p), sz, f64p, sz, sz, f64p, sz, sz,
                                        C.POINTER(C.c_void_p)]),
+            "ref_modraise": (None, [vp, u64p, u64p]),
+            "ref_mul_monomial": (None, [vp, u64p, sz, sz, sz]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -324,3 +326,15 @@ class Ref:
         self.L.ref_ctpt_matmul(self.h, xp, row_w, np.ascontiguousarray(W, np.float64), col_w, Ql,
                                np.ascontiguousarray(scales, np.float64), col_begin, col_end, op)
         return outs
+
+    # bootstrapping pieces -------------------------------------------------------------
+    def modraise(self, ct_low):
+        """[2][N] at prime 0 (NTT form) -> [2][nQ][N] (Bootstrapper.cu:3287-3356)."""
+        out = np.zeros(2 * self.nQ * self.N, np.uint64)
+        self.L.ref_modraise(self.h, np.ascontiguousarray(ct_low, np.uint64), out)
+        return out
+
+    def mul_monomial(self, data, nlimbs, npolys, power):
+        d = np.ascontiguousarray(data, np.uint64).copy()
+        self.L.ref_mul_monomial(self.h, d, nlimbs, npolys, power)
+        return d

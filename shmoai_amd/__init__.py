@@ -18,7 +18,7 @@ from .capi import check, MATMUL_U64, MATMUL_PER_CT, MATMUL_INT8  # noqa: F401
 
 __all__ = [
     "Context", "Ciphertext", "Plaintext", "SecretKey", "KSwitchKey", "GaloisKeys", "Encoder", "Evaluator",
-    "Decryptor", "ct_pt_matrix_mul_wo_pre", "MATMUL_U64", "MATMUL_PER_CT", "MATMUL_INT8",
+    "Decryptor", "ct_pt_matrix_mul_wo_pre", "MATMUL_U64", "MATMUL_PER_CT", "MATMUL_INT8", "Bootstrapper",
 ]
 
 
@@ -388,6 +388,97 @@ class Evaluator:
 
     def complex_conjugate_inplace(self, ct, gk: GaloisKeys, stream=None):
         check(self.L.shm_complex_conjugate_inplace(self.ctx.h, ct.h, gk.h, stream))
+
+    def multiply_by_i_inplace(self, ct, negate: bool = False, stream=None):
+        """Multiply every slot by i (X^(N/2)), or by -i (X^(3N/2)) when negate."""
+        check(self.L.shm_multiply_by_i_inplace(self.ctx.h, ct.h, 1 if negate else 0, stream))
+
+
+class Bootstrapper:
+    """Slot-to-coefficient-first bootstrapping (Bootstrapper.cuh:63 +
+    BatchBootstrap.cuh:674-709 bootstrap_ordinary_s2c_first).
+
+    Inputs carry exactly 4 primes. rotation_steps() lists the Galois steps
+    the transforms need (0 = conjugation) for create_galois_keys_from_steps.
+    """
+
+    def __init__(self, ctx: Context | None, layer3_factor: float = 1.0 / 16, K: int = 25, deg: int = 63,
+                 double_angles: int = 2, input_bound: float = 1.0, s2c_input_primes: int = 4,
+                 plan_only: tuple[int, int] | None = None):
+        """plan_only=(log_n, size_q) builds the host plan without a context (no device)."""
+        self.ctx = ctx
+        p = capi.BootParams(s2c_input_primes, layer3_factor, K, deg, double_angles, input_bound)
+        h = C.c_void_p()
+        if plan_only is not None:
+            check(_lib().shm_bootstrapper_create_plan(plan_only[0], plan_only[1], C.byref(p), C.byref(h)))
+            self.n = 1 << (plan_only[0] - 1)
+        else:
+            check(_lib().shm_bootstrapper_create(ctx.h, C.byref(p), C.byref(h)))
+            self.n = ctx.N // 2
+        self.h = h
+        self.K = K
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                _lib().shm_bootstrapper_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def rotation_steps(self) -> list[int]:
+        n = C.c_size_t()
+        check(_lib().shm_bootstrapper_rotation_steps(self.h, None, 0, C.byref(n)))
+        arr = (C.c_int * n.value)()
+        check(_lib().shm_bootstrapper_rotation_steps(self.h, arr, n.value, C.byref(n)))
+        return list(arr)
+
+    def layer_info(self, transform: int, layer: int) -> dict:
+        info = (C.c_int64 * 5)()
+        check(_lib().shm_bootstrapper_layer_info(self.h, transform, layer, info))
+        return {"base": info[0], "k0": info[1], "g": info[2], "giants": info[3], "chain": info[4]}
+
+    def layer_diagonal(self, transform: int, layer: int, giant: int, baby: int) -> np.ndarray:
+        n = self.n
+        re = np.zeros(n)
+        im = np.zeros(n)
+        check(_lib().shm_bootstrapper_layer_diagonal(self.h, transform, layer, giant, baby,
+                                                     re.ctypes.data_as(C.POINTER(C.c_double)),
+                                                     im.ctypes.data_as(C.POINTER(C.c_double)), n))
+        return re + 1j * im
+
+    def evalmod_coeffs(self) -> np.ndarray:
+        n = C.c_size_t()
+        check(_lib().shm_bootstrapper_evalmod_coeffs(self.h, None, 0, C.byref(n)))
+        out = np.zeros(n.value)
+        check(_lib().shm_bootstrapper_evalmod_coeffs(self.h, out.ctypes.data_as(C.POINTER(C.c_double)), n.value,
+                                                     C.byref(n)))
+        return out
+
+    def prepare(self, stream=None):
+        check(_lib().shm_bootstrapper_prepare(self.h, stream))
+
+    def slot_to_coeff(self, ct: Ciphertext, gk: GaloisKeys, stream=None):
+        check(_lib().shm_bootstrap_slot_to_coeff(self.h, ct.h, gk.h, stream))
+
+    def modraise_inplace(self, ct: Ciphertext, stream=None):
+        check(_lib().shm_modraise_inplace(self.ctx.h, ct.h, stream))
+
+    def coeff_to_slot(self, ct: Ciphertext, gk: GaloisKeys, stream=None):
+        re, im = Ciphertext(self.ctx, 2, 1), Ciphertext(self.ctx, 2, 1)
+        check(_lib().shm_bootstrap_coeff_to_slot(self.h, ct.h, re.h, im.h, gk.h, stream))
+        return re, im
+
+    def eval_mod(self, ct: Ciphertext, relin_key: KSwitchKey, stream=None):
+        check(_lib().shm_bootstrap_eval_mod(self.h, ct.h, relin_key.h, stream))
+
+    def bootstrap_s2c_first(self, cts: Sequence[Ciphertext], gk: GaloisKeys, relin_key: KSwitchKey,
+                            out_scale: float = 0.0, stream=None) -> list:
+        out = [Ciphertext(self.ctx, 2, 1) for _ in cts]
+        ins = (C.c_void_p * len(cts))(*[c.h.value for c in cts])
+        outs = (C.c_void_p * len(cts))(*[o.h.value for o in out])
+        check(_lib().shm_bootstrap_s2c_first(self.h, ins, len(cts), outs, gk.h, relin_key.h, out_scale, stream))
+        return out
 
 
 def ct_pt_matrix_mul_wo_pre(enc_X: Sequence[Ciphertext], W: np.ndarray, col_X: int, col_W: int, row_W: int,

@@ -34,6 +34,11 @@ pu64 = C.POINTER(C.c_uint64)
 pvp = C.POINTER(C.c_void_p)
 
 
+class BootParams(C.Structure):
+    _fields_ = [("s2c_input_primes", u32), ("layer3_factor", C.c_double), ("K", C.c_int32), ("deg", C.c_int32),
+                ("double_angles", C.c_int32), ("input_bound", C.c_double)]
+
+
 class ContextInfo(C.Structure):
     _fields_ = [("poly_degree", u32), ("size_q", u32), ("size_p", u32), ("dnum", u32)]
 
@@ -99,6 +104,20 @@ _SIGS = {
     "shm_ct_pt_matrix_mul": (i32, [vp, pvp, u32, C.POINTER(dbl), u32, pvp, i32, vp]),
     "shm_last_matmul_kernel_ms": (dbl, []),
     "shm_last_matmul_variant": (i32, []),
+    "shm_bootstrapper_create": (i32, [vp, C.POINTER(BootParams), pvp]),
+    "shm_bootstrapper_create_plan": (i32, [u32, u32, C.POINTER(BootParams), pvp]),
+    "shm_bootstrapper_destroy": (i32, [vp]),
+    "shm_bootstrapper_rotation_steps": (i32, [vp, C.POINTER(C.c_int), sz, C.POINTER(sz)]),
+    "shm_bootstrapper_layer_info": (i32, [vp, i32, i32, C.POINTER(C.c_int64)]),
+    "shm_bootstrapper_layer_diagonal": (i32, [vp, i32, i32, i32, i32, C.POINTER(dbl), C.POINTER(dbl), sz]),
+    "shm_bootstrapper_evalmod_coeffs": (i32, [vp, C.POINTER(dbl), sz, C.POINTER(sz)]),
+    "shm_bootstrapper_prepare": (i32, [vp, vp]),
+    "shm_modraise_inplace": (i32, [vp, vp, vp]),
+    "shm_multiply_by_i_inplace": (i32, [vp, vp, i32, vp]),
+    "shm_bootstrap_slot_to_coeff": (i32, [vp, vp, vp, vp]),
+    "shm_bootstrap_coeff_to_slot": (i32, [vp, vp, vp, vp, vp, vp]),
+    "shm_bootstrap_eval_mod": (i32, [vp, vp, vp, vp]),
+    "shm_bootstrap_s2c_first": (i32, [vp, pvp, sz, pvp, vp, vp, dbl, vp]),
 }
 
 _lib = None

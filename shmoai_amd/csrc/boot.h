@@ -1,0 +1,102 @@
+// CKKS bootstrapping (slot-to-coefficient-first): plan (host FP64) and the
+// device pipeline that executes it.
+//
+// Slot convention of the encoder: slot j of a plaintext m(X) is
+// m(w_j) / scale with w_j = exp(2 pi i 5^j / 2N), j < n = N/2. Writing
+// m = m_lo + X^n m_hi gives z = U0 (m_lo + i m_hi), U0[j][k] = w_j^k.
+// U0 = B_logn ... B_1 R, where R is the bit reversal and B_s the radix-2
+// butterfly stage of block length 2^s (fft.cu special FFT, fftSpecial
+// form). Slot-to-coefficient applies T = B_logn ... B_1 (so the
+// coefficients come out bit-reversed) and coefficient-to-slot applies
+// T^-1; the bit reversals cancel because EvalMod works slot by slot.
+// Each transform is split into three layers of consecutive stages
+// (Bootstrapper.cu:2737-2777 sfl_full_3, 2891-2913 sflinv_full_3 level
+// split), evaluated with baby-step / giant-step rotations.
+#pragma once
+
+#include <complex>
+#include <map>
+#include <vector>
+
+#include "engine.h"
+
+namespace shm {
+namespace boot {
+
+using cplx = std::complex<double>;
+
+// Matrix on slot vectors by generalised diagonals:
+// (A z)[j] = sum_d diag[d][j] * z[(j + d) mod n].
+struct DiagMat {
+    size_t n = 0;
+    std::map<size_t, std::vector<cplx>> d;
+};
+
+DiagMat fft_stage(size_t n, int s, bool inverse);
+DiagMat compose(const DiagMat &a, const DiagMat &b);  // a * b (b applied first)
+DiagMat stages_product(size_t n, int first, int last, bool inverse);
+
+// One linear-transform layer: terms k = k0 + g*i + j (j < g, i < n_giants)
+// with slot offset k*base; plaintext (i, j) holds rot(diag_k, -(k0 + g*i)*base).
+struct LtLayer {
+    size_t base = 1;
+    long k0 = 0;
+    int g = 1;
+    int n_giants = 1;
+    double factor = 1.0;
+    size_t chain = 0;          // chain index the layer is applied at
+    std::vector<std::vector<std::vector<cplx>>> pre;  // [i][j] (empty = zero)
+    std::vector<std::vector<DevBuf>> pts;             // encoded at scale q_last of `chain`
+    int baby_step(int j) const { return (int)(j * base); }
+    long giant_offset(int i) const { return (k0 + (long)g * i) * (long)base; }
+};
+
+// Build one layer from stages [first, last] (1-based, inclusive).
+LtLayer make_layer(size_t n, int first, int last, bool inverse, double factor);
+
+// p(y) ~ cos(2 pi (K y - 1/4) / 2^r) on [-1, 1]: Chebyshev coefficients.
+std::vector<double> evalmod_chebyshev(int K, int r, int deg);
+// p = q + T_m r (Chebyshev division by T_m); requires deg p < 2m.
+void cheb_split(const std::vector<double> &p, size_t m, std::vector<double> &q, std::vector<double> &r);
+
+struct BootParams {
+    uint32_t s2c_input_primes = 4;  // lS2C + 1 (BatchBootstrap.cuh:727-728)
+    double layer3_factor = 1.0 / 16.0;
+    int K = 25;
+    int deg = 63;
+    int double_angles = 2;
+    double input_bound = 1.0;  // bound on |Re z|, |Im z| of the input slots
+};
+
+struct Bootstrapper {
+    Context *ctx = nullptr;
+    BootParams prm;
+    size_t n = 0;
+    int logn = 0;
+    LtLayer s2c[3], c2s[3];
+    std::vector<double> cheb;
+    bool prepared = false;
+
+    std::vector<int> rotation_steps() const;  // 0 = conjugation
+    void prepare(hipStream_t s);              // encode every diagonal on the device
+};
+
+std::unique_ptr<Bootstrapper> make_bootstrapper(Context &c, const BootParams &p);
+// Host-only plan (no context: diagonals, schedule and coefficients, no device work).
+std::unique_ptr<Bootstrapper> make_plan(size_t N, size_t nQ, const BootParams &p);
+
+// Device pipeline.
+void lt_apply(Context &c, const LtLayer &L, Ciphertext &ct, const GaloisKeys &gk, hipStream_t s);
+void slot_to_coeff(Bootstrapper &b, Ciphertext &ct, const GaloisKeys &gk, hipStream_t s);
+void coeff_to_slot(Bootstrapper &b, Ciphertext &ct, Ciphertext &re, Ciphertext &im, const GaloisKeys &gk,
+                   hipStream_t s);
+void eval_mod(Bootstrapper &b, Ciphertext &x, const KSwitchKey &rk, hipStream_t s);
+void bootstrap_s2c_first(Bootstrapper &b, const Ciphertext &in, Ciphertext &out, const GaloisKeys &gk,
+                         const KSwitchKey &rk, double out_scale, hipStream_t s);
+// Multiply by the constant `value` encoded at scale `sigma` and accumulate:
+// acc (+)= src * [round(value * sigma)]_q per limb over Ql limbs of src.
+void const_lincomb(Context &c, Ciphertext &acc, const std::vector<const Ciphertext *> &src,
+                   const std::vector<double> &values, const std::vector<double> &sigmas, size_t Ql, hipStream_t s);
+
+}  // namespace boot
+}  // namespace shm

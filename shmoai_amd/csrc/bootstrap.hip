@@ -1,0 +1,350 @@
+// Slot-to-coefficient-first bootstrapping on the device.
+//
+// Pipeline (BatchBootstrap.cuh:674-709 bootstrap_ordinary_s2c_first):
+//   S2C (3 diagonal layers at the 4-prime input level, Bootstrapper.cu:2737-2777)
+//   -> modraise to all data primes, scale := q0 (Bootstrapper.cu:3271-3356)
+//   -> C2S (3 layers at the top, then w + conj(w) and -i (w - conj(w)),
+//      Bootstrapper.cu:3048-3067)
+//   -> EvalMod on both halves (ModularReducer.cu:64-84: cosine polynomial,
+//      double angles) -> recombine re + i im -> amplitude correction
+//      (BatchBootstrap.cuh:444-460).
+// Scale bookkeeping is exact: diagonals are encoded at the scale of the prime
+// the next rescale divides by, and every constant that meets a ciphertext of
+// another scale is encoded at the scale that lands the product on the target.
+#include <cmath>
+
+#include "arith.h"
+#include "boot.h"
+
+namespace shm {
+namespace boot {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxTerms = 16;
+
+inline unsigned grid_for(size_t work) {
+    size_t g = (work + kBlock - 1) / kBlock;
+    if (g > (1u << 20)) g = 1u << 20;
+    return (unsigned)(g ? g : 1);
+}
+
+struct PtPack {
+    const uint64_t *x[kMaxTerms];
+    const uint64_t *pt[kMaxTerms];
+};
+
+// acc[p][l][i] = sum_t x_t[p][l][i] * pt_t[l][i] mod q_l over both polynomials;
+// products accumulate in 128 bits (at most 16 terms below 2^122).
+__global__ void k_lt_mac(uint64_t *acc, PtPack pk, int nterms, const DevMod *mods, size_t Ql, size_t N) {
+    const size_t S = Ql * N, total = 2 * S;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t r = t % S;
+        Acc128 a{0, 0};
+        for (int k = 0; k < nterms; k++) mac128(a, pk.x[k][t], pk.pt[k][r]);
+        acc[t] = reduce128(a, mods[r / N]);
+    }
+}
+
+struct SrcPack {
+    const uint64_t *x[kMaxTerms];
+    uint32_t ql[kMaxTerms];  // limb count (stride) of each source ciphertext
+};
+
+// acc[p][l][i] (+)= sum_k x_k[p][l][i] * cst[k][l]: a linear combination of
+// ciphertexts at different levels, each read over the first Ql limbs.
+__global__ void k_const_lincomb(uint64_t *acc, SrcPack sp, int nterms, const uint64_t *cst, const DevMod *mods,
+                                size_t Ql, size_t N, int accumulate) {
+    const size_t S = Ql * N, total = 2 * S;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t p = t / S, r = t % S, l = r / N, i = r % N;
+        Acc128 a{accumulate ? acc[t] : 0, 0};
+        for (int k = 0; k < nterms; k++) mac128(a, sp.x[k][(p * sp.ql[k] + l) * N + i], cst[k * Ql + l]);
+        acc[t] = reduce128(a, mods[l]);
+    }
+}
+
+uint64_t h_mulmod(uint64_t a, uint64_t b, uint64_t q) { return (uint64_t)((unsigned __int128)a * b % q); }
+
+// [v]_q for an integral double v (any magnitude), negative values wrap.
+uint64_t double_to_residue(double v, uint64_t q) {
+    const bool neg = v < 0;
+    double a = std::fabs(v);
+    uint64_t r;
+    if (a < 18446744073709551616.0) {
+        r = (uint64_t)a % q;
+    } else {
+        int e;
+        const double m = std::frexp(a, &e);                 // a = m 2^e, m in [0.5, 1)
+        const uint64_t mant = (uint64_t)std::ldexp(m, 53);  // exact
+        uint64_t p2 = 1 % q, b = 2 % q;
+        for (int x = e - 53; x > 0; x >>= 1) {
+            if (x & 1) p2 = h_mulmod(p2, b, q);
+            b = h_mulmod(b, b, q);
+        }
+        r = h_mulmod(mant % q, p2, q);
+    }
+    return (neg && r) ? q - r : r;
+}
+
+void add_checked(Context &c, Ciphertext &a, const Ciphertext &b, hipStream_t s) {
+    if (std::fabs(a.scale / b.scale - 1.0) > 1e-9) throw LogicError("scale mismatch");
+    ct_add(c, a, b, s);
+}
+
+// a and b to a common (level, scale): the operand with more primes is aligned
+// onto the other one.
+void add_aligned(Context &c, Ciphertext &a, Ciphertext &b, hipStream_t s) {
+    if (a.chain < b.chain)
+        ct_align(c, a, b.chain, b.scale, s);
+    else if (b.chain < a.chain)
+        ct_align(c, b, a.chain, a.scale, s);
+    add_checked(c, a, b, s);
+}
+
+}  // namespace
+
+void const_lincomb(Context &c, Ciphertext &acc, const std::vector<const Ciphertext *> &src,
+                   const std::vector<double> &values, const std::vector<double> &sigmas, size_t Ql, hipStream_t s) {
+    if (src.empty() || src.size() > (size_t)kMaxTerms || values.size() != src.size() || sigmas.size() != src.size())
+        throw InvalidArgument("const_lincomb: 1..16 terms");
+    std::vector<uint64_t> cst(src.size() * Ql);
+    SrcPack sp{};
+    for (size_t k = 0; k < src.size(); k++) {
+        if (src[k]->Ql() < Ql || src[k]->size != 2) throw LogicError("const_lincomb: source below the target level");
+        sp.x[k] = src[k]->data();
+        sp.ql[k] = (uint32_t)src[k]->Ql();
+        const double v = std::round(values[k] * sigmas[k]);
+        for (size_t l = 0; l < Ql; l++) cst[k * Ql + l] = double_to_residue(v, c.q[l]);
+    }
+    Scratch d(cst.size() * sizeof(uint64_t), s);
+    SHM_HIP(hipMemcpyAsync(d.as<uint64_t>(), cst.data(), cst.size() * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    acc.ctx = &c;
+    acc.resize(2, c.nQ - Ql + 1);
+    acc.ntt = true;
+    k_const_lincomb<<<grid_for(2 * Ql * c.N), kBlock, 0, s>>>(acc.data(), sp, (int)src.size(), d.as<uint64_t>(),
+                                                              c.mods(), Ql, c.N, 0);
+    SHM_HIP(hipGetLastError());
+    SHM_HIP(hipStreamSynchronize(s));  // cst is host memory
+}
+
+// bsgs_linear_transform (Bootstrapper.cu:2168-2251): babies rot(x, j*base),
+// inner sums with pre-rotated diagonals, giant rotation of each inner sum.
+void lt_apply(Context &c, const LtLayer &L, Ciphertext &ct, const GaloisKeys &gk, hipStream_t s) {
+    if (ct.size != 2) throw InvalidArgument("linear transform takes size-2 ciphertexts");
+    if (ct.chain != L.chain) throw InvalidArgument("linear transform applied at the wrong level");
+    if (L.pts.empty()) throw LogicError("bootstrapper is not prepared");
+    const size_t Ql = ct.Ql(), n = c.N / 2;
+    std::vector<Ciphertext> baby(L.g);
+    for (int j = 1; j < L.g; j++) {
+        bool used = false;
+        for (int i = 0; i < L.n_giants; i++) used |= !L.pre[i][j].empty();
+        if (!used) continue;
+        ct_clone(baby[j], ct, s);
+        rotate_internal(c, baby[j], L.baby_step(j), gk, s);
+    }
+    Ciphertext out;
+    bool have = false;
+    for (int i = 0; i < L.n_giants; i++) {
+        PtPack pk{};
+        int nt = 0;
+        for (int j = 0; j < L.g; j++) {
+            if (L.pre[i][j].empty()) continue;
+            pk.x[nt] = j == 0 ? ct.data() : baby[j].data();
+            pk.pt[nt] = L.pts[i][j].as<uint64_t>();
+            nt++;
+        }
+        if (!nt) continue;
+        Ciphertext inner;
+        inner.ctx = &c;
+        inner.resize(2, ct.chain);
+        inner.ntt = true;
+        inner.scale = ct.scale;
+        k_lt_mac<<<grid_for(2 * Ql * c.N), kBlock, 0, s>>>(inner.data(), pk, nt, c.mods(), Ql, c.N);
+        SHM_HIP(hipGetLastError());
+        const long G = ((L.giant_offset(i) % (long)n) + (long)n) % (long)n;
+        if (G) rotate_internal(c, inner, (int)G, gk, s);
+        if (!have) {
+            out = std::move(inner);
+            have = true;
+        } else {
+            ct_add(c, out, inner, s);
+        }
+    }
+    if (!have) throw LogicError("empty linear transform");
+    const double sc = ct.scale;
+    out.scale = sc * (double)c.q[Ql - 1];
+    rescale_inplace(c, out, s);
+    out.scale = sc;  // diagonals carry exactly the scale q_last
+    ct = std::move(out);
+}
+
+void slot_to_coeff(Bootstrapper &b, Ciphertext &ct, const GaloisKeys &gk, hipStream_t s) {
+    for (int l = 0; l < 3; l++) lt_apply(*b.ctx, b.s2c[l], ct, gk, s);
+}
+
+void coeff_to_slot(Bootstrapper &b, Ciphertext &ct, Ciphertext &re, Ciphertext &im, const GaloisKeys &gk,
+                   hipStream_t s) {
+    Context &c = *b.ctx;
+    for (int l = 0; l < 3; l++) lt_apply(c, b.c2s[l], ct, gk, s);
+    Ciphertext conj;
+    ct_clone(conj, ct, s);
+    apply_galois(c, conj, (uint32_t)(2 * c.N - 1), gk, s);
+    ct_clone(re, ct, s);
+    ct_add(c, re, conj, s);  // w + conj(w)
+    ct_clone(im, ct, s);
+    ct_sub(c, im, conj, s);
+    ct_mul_monomial_half(c, im, true, s);  // -i (w - conj(w)) = 2 Im(w)
+}
+
+namespace {
+
+struct ChebEval {
+    Context &c;
+    const KSwitchKey &rk;
+    hipStream_t s;
+    std::vector<std::unique_ptr<Ciphertext>> T;
+
+    ChebEval(Context &c_, const KSwitchKey &rk_, hipStream_t s_, Ciphertext &y) : c(c_), rk(rk_), s(s_), T(65) {
+        T[1] = std::make_unique<Ciphertext>();
+        ct_clone(*T[1], y, s);
+    }
+
+    // T_k = 2 T_a T_b - T_{a-b}, a the largest power of two below k (T_{2a} = 2 T_a^2 - 1).
+    Ciphertext &get(int k) {
+        if (T[k]) return *T[k];
+        int a = 1;
+        while (2 * a < k) a *= 2;
+        const int b = k - a;
+        Ciphertext &ta = get(a);
+        Ciphertext &tb = get(b);
+        auto out = std::make_unique<Ciphertext>();
+        ct_mul_relin_rescale(c, *out, ta, tb, rk, s);
+        ct_add(c, *out, *out, s);
+        if (a == b) {
+            ct_add_const(c, *out, -1.0, s);
+        } else {
+            Ciphertext tmp;
+            ct_clone(tmp, get(a - b), s);
+            ct_align(c, tmp, out->chain, out->scale, s);
+            ct_sub(c, *out, tmp, s);
+        }
+        T[k] = std::move(out);
+        return *T[k];
+    }
+
+    // sum_k p[k] T_k landing at (chain, scale): products at chain-1 with each
+    // constant encoded at scale * q_last / scale(T_k), one rescale.
+    void baby(const std::vector<double> &p, size_t chain, double scale, Ciphertext &out) {
+        std::vector<const Ciphertext *> src;
+        std::vector<double> vals, sig;
+        const size_t Qc = c.ql_of_chain(chain - 1);
+        const double ql = (double)c.q[Qc - 1];
+        for (size_t k = 1; k < p.size(); k++) {
+            if (p[k] == 0.0) continue;
+            Ciphertext &t = get((int)k);
+            if (t.chain > chain - 1) throw LogicError("Chebyshev basis element below the combination level");
+            src.push_back(&t);
+            vals.push_back(p[k]);
+            sig.push_back(scale * ql / t.scale);
+        }
+        if (src.empty()) throw LogicError("constant-only Chebyshev block");
+        const_lincomb(c, out, src, vals, sig, Qc, s);
+        out.scale = scale * ql;
+        rescale_inplace(c, out, s);
+        out.scale = scale;
+        if (p[0] != 0.0) ct_add_const(c, out, p[0], s);
+    }
+
+    size_t natural_chain(const std::vector<double> &p) {
+        size_t ch = 0;
+        for (size_t k = 1; k < p.size(); k++)
+            if (p[k] != 0.0) ch = std::max(ch, get((int)k).chain);
+        return ch + 1;
+    }
+
+    // Homomorphic_poly_evaluation (common/Polynomial.cu:253-476) in Chebyshev
+    // form: p = q + T_m r with m in {32, 16}, babies T_1..T_15.
+    void eval(const std::vector<double> &p, Ciphertext &out) {
+        const size_t deg = p.size() - 1;
+        if (deg < 16) {
+            baby(p, natural_chain(p), T[1]->scale, out);
+            return;
+        }
+        const size_t m = deg >= 32 ? 32 : 16;
+        std::vector<double> q, r;
+        cheb_split(p, m, q, r);
+        Ciphertext R;
+        eval(r, R);
+        ct_mul_relin_rescale(c, out, get((int)m), R, rk, s);
+        bool q_terms = false;
+        for (size_t k = 1; k < q.size(); k++) q_terms |= q[k] != 0.0;
+        if (!q_terms) {
+            if (q[0] != 0.0) ct_add_const(c, out, q[0], s);
+            return;
+        }
+        Ciphertext Q;
+        if (q.size() <= 16)
+            baby(q, out.chain, out.scale, Q);
+        else
+            eval(q, Q);
+        add_aligned(c, out, Q, s);
+    }
+};
+
+}  // namespace
+
+// ModularReducer::modular_reduction (ModularReducer.cu:64-84): x holds y = x'/K
+// with x' = eps + integer; result cos(2 pi (x' - 1/4)) = sin(2 pi x').
+void eval_mod(Bootstrapper &b, Ciphertext &x, const KSwitchKey &rk, hipStream_t s) {
+    Context &c = *b.ctx;
+    Ciphertext out;
+    {
+        ChebEval ev(c, rk, s, x);
+        ev.eval(b.cheb, out);
+    }
+    for (int i = 0; i < b.prm.double_angles; i++) {
+        Ciphertext sq;
+        ct_mul_relin_rescale(c, sq, out, out, rk, s);
+        ct_add(c, sq, sq, s);
+        ct_add_const(c, sq, -1.0, s);
+        out = std::move(sq);
+    }
+    x = std::move(out);
+}
+
+void bootstrap_s2c_first(Bootstrapper &b, const Ciphertext &in, Ciphertext &out, const GaloisKeys &gk,
+                         const KSwitchKey &rk, double out_scale, hipStream_t s) {
+    Context &c = *b.ctx;
+    if (in.size != 2) throw InvalidArgument("bootstrapping takes size-2 ciphertexts");
+    if (in.Ql() != b.prm.s2c_input_primes)
+        throw InvalidArgument("bootstrap_s2c_first: inputs must have exactly 4 primes (lS2C+1)");
+    if (!b.prepared) b.prepare(s);
+    const double d_in = in.scale;
+    Ciphertext x;
+    ct_clone(x, in, s);
+    slot_to_coeff(b, x, gk, s);
+    ct_modraise(c, x, s);
+    const double q0 = (double)c.q[0];
+    x.scale = q0;
+    Ciphertext re, im;
+    coeff_to_slot(b, x, re, im, gk, s);
+    eval_mod(b, re, rk, s);
+    eval_mod(b, im, rk, s);
+    ct_mul_monomial_half(c, im, false, s);  // i * sin(2 pi x_im)
+    add_checked(c, re, im, s);
+    // sin(2 pi eps)/(2 pi) ~ eps; the linear minimax constant over |eps| <= E,
+    // E = F * scale_in * bound / q0, absorbs the cubic term (error b E^3 / 4).
+    const double F = b.prm.layer3_factor;
+    const double E = F * d_in * b.prm.input_bound / q0;
+    const double pi = 3.14159265358979323846;
+    const double cst = (1.0 + 0.5 * pi * pi * E * E) / (2.0 * pi) * q0 / (F * d_in);
+    const double ql = (double)c.q[re.Ql() - 1];
+    ct_mul_const_rescale(c, re, cst, out_scale * ql / re.scale, s);
+    re.scale = out_scale;
+    out = std::move(re);
+}
+
+}  // namespace boot
+}  // namespace shm

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "../../include/shmoai.h"
+#include "boot.h"
 #include "engine.h"
 
 using namespace shm;
@@ -760,3 +761,191 @@ double shm_last_matmul_kernel_ms(void) { return (double)last_matmul_kernel_ms();
 int shm_last_matmul_variant(void) { return last_matmul_variant(); }
 
 }  // extern "C"
+
+// ---- bootstrapping ----------------------------------------------------------------------
+struct shm_bootstrapper {
+    std::unique_ptr<boot::Bootstrapper> b;
+};
+
+namespace {
+const boot::LtLayer &pick_layer(const shm_bootstrapper *b, int transform, int layer) {
+    if (transform < 0 || transform > 1 || layer < 0 || layer > 2)
+        throw InvalidArgument("transform must be 0 or 1 and layer 0..2");
+    return transform == 0 ? b->b->s2c[layer] : b->b->c2s[layer];
+}
+}  // namespace
+
+namespace {
+boot::BootParams boot_params(const shm_boot_params *p) {
+    boot::BootParams bp;
+    bp.s2c_input_primes = p->s2c_input_primes;
+    bp.layer3_factor = p->layer3_factor;
+    bp.K = p->K;
+    bp.deg = p->deg;
+    bp.double_angles = p->double_angles;
+    bp.input_bound = p->input_bound;
+    return bp;
+}
+
+boot::Bootstrapper &ready(shm_bootstrapper *b, hipStream_t s) {
+    need(b, "bootstrapper");
+    if (!b->b->ctx) throw InvalidArgument("bootstrapping plan without a context cannot run");
+    if (!b->b->prepared) b->b->prepare(s);
+    return *b->b;
+}
+}  // namespace
+
+int shm_bootstrapper_create(shm_context *ctx, const shm_boot_params *p, shm_bootstrapper **out) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(p, "params");
+        need(out, "out");
+        auto h = std::make_unique<shm_bootstrapper>();
+        h->b = boot::make_bootstrapper(*ctx->c, boot_params(p));
+        *out = h.release();
+    });
+}
+
+int shm_bootstrapper_create_plan(uint32_t log_poly_degree, uint32_t size_q, const shm_boot_params *p,
+                                 shm_bootstrapper **out) {
+    return guard([&] {
+        need(p, "params");
+        need(out, "out");
+        if (log_poly_degree < 4 || log_poly_degree > 17) throw InvalidArgument("log_poly_degree out of range");
+        auto h = std::make_unique<shm_bootstrapper>();
+        h->b = boot::make_plan(size_t(1) << log_poly_degree, size_q, boot_params(p));
+        *out = h.release();
+    });
+}
+
+int shm_bootstrapper_destroy(shm_bootstrapper *b) {
+    return guard([&] {
+        if (b && b->b->prepared) SHM_HIP(hipDeviceSynchronize());
+        delete b;
+    });
+}
+
+int shm_bootstrapper_rotation_steps(const shm_bootstrapper *b, int *steps, size_t cap, size_t *count) {
+    return guard([&] {
+        need(b, "bootstrapper");
+        const auto v = b->b->rotation_steps();
+        if (count) *count = v.size();
+        if (steps) {
+            if (cap < v.size()) throw InvalidArgument("steps buffer too small");
+            std::copy(v.begin(), v.end(), steps);
+        }
+    });
+}
+
+int shm_bootstrapper_layer_info(const shm_bootstrapper *b, int transform, int layer, int64_t info[5]) {
+    return guard([&] {
+        need(b, "bootstrapper");
+        need(info, "info");
+        const auto &L = pick_layer(b, transform, layer);
+        info[0] = (int64_t)L.base;
+        info[1] = L.k0;
+        info[2] = L.g;
+        info[3] = L.n_giants;
+        info[4] = (int64_t)L.chain;
+    });
+}
+
+int shm_bootstrapper_layer_diagonal(const shm_bootstrapper *b, int transform, int layer, int giant, int baby,
+                                    double *re, double *im, size_t n) {
+    return guard([&] {
+        need(b, "bootstrapper");
+        need(re, "re");
+        need(im, "im");
+        const auto &L = pick_layer(b, transform, layer);
+        if (giant < 0 || giant >= L.n_giants || baby < 0 || baby >= L.g) throw InvalidArgument("diagonal index");
+        if (n != b->b->n) throw InvalidArgument("n must be N/2");
+        const auto &v = L.pre[giant][baby];
+        for (size_t t = 0; t < n; t++) {
+            re[t] = v.empty() ? 0.0 : v[t].real();
+            im[t] = v.empty() ? 0.0 : v[t].imag();
+        }
+    });
+}
+
+int shm_bootstrapper_evalmod_coeffs(const shm_bootstrapper *b, double *out, size_t cap, size_t *count) {
+    return guard([&] {
+        need(b, "bootstrapper");
+        const auto &c = b->b->cheb;
+        if (count) *count = c.size();
+        if (out) {
+            if (cap < c.size()) throw InvalidArgument("buffer too small");
+            std::copy(c.begin(), c.end(), out);
+        }
+    });
+}
+
+int shm_bootstrapper_prepare(shm_bootstrapper *b, shm_stream_t st) {
+    return guard([&] {
+        ready(b, S(st));
+        SHM_HIP(hipStreamSynchronize(S(st)));
+    });
+}
+
+int shm_modraise_inplace(shm_context *ctx, shm_ciphertext *ct, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(ct, "ct");
+        ct_modraise(*ctx->c, ct->ct, S(st));
+    });
+}
+
+int shm_multiply_by_i_inplace(shm_context *ctx, shm_ciphertext *ct, int negate, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(ct, "ct");
+        if (!ct->ct.ntt) throw InvalidArgument("CKKS encrypted must be in NTT form");
+        ct_mul_monomial_half(*ctx->c, ct->ct, negate != 0, S(st));
+    });
+}
+
+int shm_bootstrap_slot_to_coeff(shm_bootstrapper *b, shm_ciphertext *ct, const shm_galois_keys *gk,
+                                shm_stream_t st) {
+    return guard([&] {
+        need(ct, "ct");
+        need(gk, "galois_keys");
+        boot::slot_to_coeff(ready(b, S(st)), ct->ct, gk->g, S(st));
+    });
+}
+
+int shm_bootstrap_coeff_to_slot(shm_bootstrapper *b, shm_ciphertext *ct, shm_ciphertext *re, shm_ciphertext *im,
+                                const shm_galois_keys *gk, shm_stream_t st) {
+    return guard([&] {
+        need(ct, "ct");
+        need(re, "re");
+        need(im, "im");
+        need(gk, "galois_keys");
+        boot::coeff_to_slot(ready(b, S(st)), ct->ct, re->ct, im->ct, gk->g, S(st));
+    });
+}
+
+int shm_bootstrap_eval_mod(shm_bootstrapper *b, shm_ciphertext *ct, const shm_kswitch_key *relin, shm_stream_t st) {
+    return guard([&] {
+        need(ct, "ct");
+        need(relin, "relin_keys");
+        boot::eval_mod(ready(b, S(st)), ct->ct, relin->k, S(st));
+    });
+}
+
+int shm_bootstrap_s2c_first(shm_bootstrapper *b, const shm_ciphertext *const *in, size_t count,
+                            shm_ciphertext *const *out, const shm_galois_keys *gk, const shm_kswitch_key *relin,
+                            double out_scale, shm_stream_t st) {
+    return guard([&] {
+        boot::Bootstrapper &bs = ready(b, S(st));
+        need(gk, "galois_keys");
+        need(relin, "relin_keys");
+        if (count && (!in || !out)) throw InvalidArgument("in/out is null");
+        for (size_t i = 0; i < count; i++) {
+            need(in[i], "in[i]");
+            need(out[i], "out[i]");
+        }
+        for (size_t i = 0; i < count; i++)
+            boot::bootstrap_s2c_first(bs, in[i]->ct, out[i]->ct, gk->g, relin->k,
+                                      out_scale > 0 ? out_scale : in[i]->ct.scale, S(st));
+        SHM_HIP(hipStreamSynchronize(S(st)));
+    });
+}

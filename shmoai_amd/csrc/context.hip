@@ -132,7 +132,11 @@ void DevBuf::alloc(size_t bytes) {
 }
 
 void DevBuf::release() {
-    if (ptr_) (void)hipFree(ptr_);
+    // queued work may still read the buffer: drain the device before freeing
+    if (ptr_) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(ptr_);
+    }
     ptr_ = nullptr;
     bytes_ = 0;
 }

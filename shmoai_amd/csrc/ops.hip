@@ -181,19 +181,20 @@ void ct_mul_const_rescale(Context &c, Ciphertext &a, double v, double sigma, hip
     rescale_inplace(c, a, s);
 }
 
-// Bring a to (chain, scale) exactly: one constant multiply + rescale at a's
-// level (the constant 1 encoded at scale * q_last / a.scale), then drop limbs.
+// Bring a to (chain, scale): drop to one level above the target, then one
+// constant multiply + rescale (the constant 1 encoded at scale * q_last / a.scale,
+// so the result carries `scale` up to the rounding of that constant).
 void ct_align(Context &c, Ciphertext &a, size_t chain, double scale, hipStream_t s) {
     if (a.chain > chain) throw LogicError("ct_align: ciphertext is already below the target level");
     if (a.chain == chain) {
-        if (std::fabs(a.scale / scale - 1.0) > 1e-12) throw LogicError("ct_align: scales differ at equal levels");
+        if (std::fabs(a.scale / scale - 1.0) > 1e-9) throw LogicError("ct_align: scales differ at equal levels");
         a.scale = scale;
         return;
     }
+    if (a.chain + 1 < chain) mod_switch_drop(c, a, chain - 1, s);
     const double qlast = (double)c.q[a.Ql() - 1];
     ct_mul_const_rescale(c, a, 1.0, scale * qlast / a.scale, s);
-    a.scale = scale;  // equal up to the encoding rounding of the constant
-    if (a.chain < chain) mod_switch_drop(c, a, chain, s);
+    a.scale = scale;
 }
 
 // out = rescale(relinearize(a * b)); operands are brought to the lower level first.

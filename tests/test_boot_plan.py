@@ -1,0 +1,131 @@
+"""Bootstrapping plan (host only, no GPU): linear-transform diagonals, BSGS
+schedule, rotation steps and EvalMod coefficients.
+
+Pins (independent of how the plan is built):
+- the three slot-to-coefficient layers, applied through their own
+  baby-step/giant-step schedule (babies rot(z, j*base), giant rotation of each
+  inner sum), equal F * U0 * R with U0[j][k] = w_j^k, w_j = exp(2 pi i 5^j / 2N)
+  built from the definition (Bootstrapper.cu:634-727 genorigcoeff defines the
+  same matrix) and R the bit reversal;
+- the three coefficient-to-slot layers equal (0.5 / K) * R * U0^-1 with
+  U0^-1 = U0^H / n (a closed form: rows of U0 are orthogonal);
+- every baby and giant offset the schedule uses is a listed rotation step;
+- the EvalMod polynomial approximates cos(2 pi (K y - 1/4) / 2^r) and, after r
+  double angles, sin(2 pi K y) near the integers (ModularReducer.cu:64-84).
+"""
+import numpy as np
+import pytest
+from numpy.polynomial import chebyshev as Ch
+
+import shmoai_amd as shm
+
+
+def u0_matrix(n):
+    M = 4 * n
+    w = np.exp(2j * np.pi * np.array([pow(5, j, M) for j in range(n)]) / M)
+    return w[:, None] ** np.arange(n)[None, :]
+
+
+def bitrev_perm(n):
+    b = n.bit_length() - 1
+    return np.array([int(format(i, f"0{b}b")[::-1], 2) for i in range(n)])
+
+
+def rot(v, r):
+    """rot(v, r)[t] = v[(t + r) mod n] (rotation by r slots)."""
+    return np.roll(v, -r)
+
+
+def apply_layer(boot, transform, layer, z):
+    info = boot.layer_info(transform, layer)
+    base, k0, g, ng = info["base"], info["k0"], info["g"], info["giants"]
+    babies = [rot(z, j * base) for j in range(g)]
+    out = np.zeros_like(z)
+    for i in range(ng):
+        inner = sum(boot.layer_diagonal(transform, layer, i, j) * babies[j] for j in range(g))
+        out += rot(inner, (k0 + g * i) * base)
+    return out
+
+
+@pytest.mark.parametrize("log_n", [7, 10, 12])
+def test_transform_layers_match_dft(log_n):
+    F, K = 1.0 / 16, 25
+    boot = shm.Bootstrapper(None, layer3_factor=F, K=K, plan_only=(log_n, 20))
+    n = 1 << (log_n - 1)
+    U = u0_matrix(n)
+    perm = bitrev_perm(n)
+    rng = np.random.default_rng(log_n)
+    z = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+    # slot-to-coefficient: F * U0 * R
+    x = z.copy()
+    for layer in range(3):
+        x = apply_layer(boot, 0, layer, x)
+    expect = F * (U @ z[perm])
+    assert np.max(np.abs(x - expect)) < 1e-12 * n
+    # coefficient-to-slot: (0.5/K) * R * U0^H / n
+    y = z.copy()
+    for layer in range(3):
+        y = apply_layer(boot, 1, layer, y)
+    expect = (0.5 / K) * ((U.conj().T @ z) / n)[perm]
+    assert np.max(np.abs(y - expect)) < 1e-13 * n
+    # round trip through both transforms is the identity up to the factors
+    r = z.copy()
+    for layer in range(3):
+        r = apply_layer(boot, 0, layer, r)
+    for layer in range(3):
+        r = apply_layer(boot, 1, layer, r)
+    assert np.max(np.abs(r * (2 * K / F) - z)) < 1e-11
+
+
+@pytest.mark.parametrize("log_n", [10, 16])
+def test_layer_schedule_and_rotation_steps(log_n):
+    boot = shm.Bootstrapper(None, plan_only=(log_n, 35))
+    n = 1 << (log_n - 1)
+    steps = set(boot.rotation_steps())
+    assert 0 in steps  # conjugation
+    chains = []
+    for t in range(2):
+        for layer in range(3):
+            info = boot.layer_info(t, layer)
+            chains.append(info["chain"])
+            for j in range(1, info["g"]):
+                assert (j * info["base"]) % n in steps
+            for i in range(info["giants"]):
+                off = ((info["k0"] + info["g"] * i) * info["base"]) % n
+                if off:
+                    assert off in steps
+    # S2C at the 4-prime level (chain 32..34 of 35), C2S at the top (chain 1..3)
+    assert chains == [32, 33, 34, 1, 2, 3]
+    if log_n == 16:
+        # logn = 15: three 5-stage layers; 63 diagonals in 8 x 8, the wrapped
+        # layer 32 diagonals; 14 / 14 / 10 rotations per transform + conjugation
+        s2c = [boot.layer_info(0, l) for l in range(3)]
+        assert [i["base"] for i in s2c] == [1, 32, 1024]
+        assert [i["g"] for i in s2c] == [8, 8, 8]
+        assert [i["giants"] for i in s2c] == [8, 8, 4]
+        assert len(steps) <= 1 + 14 + 14 + 10 + 14 + 14 + 10
+
+
+def test_evalmod_coefficients():
+    for K, r in [(25, 2), (16, 3)]:
+        boot = shm.Bootstrapper(None, K=K, double_angles=r, deg=63, plan_only=(10, 20))
+        c = boot.evalmod_coeffs()
+        assert c.size == 64
+        y = np.linspace(-1, 1, 20001)
+        f = np.cos(2 * np.pi * (K * y - 0.25) / 2 ** r)
+        assert np.max(np.abs(Ch.chebval(y, c) - f)) < 5e-9
+        # after r double angles: sin(2 pi x) for x = K y near the integers
+        x = (np.arange(-K + 1, K)[:, None] + np.linspace(-1 / 64, 1 / 64, 33)[None, :]).ravel()
+        v = Ch.chebval(x / K, c)
+        for _ in range(r):
+            v = 2 * v * v - 1
+        assert np.max(np.abs(v - np.sin(2 * np.pi * x))) < 1e-7
+
+
+def test_plan_rejects_bad_parameters():
+    with pytest.raises(shm.capi.ShmInvalidArgument):
+        shm.Bootstrapper(None, s2c_input_primes=5, plan_only=(10, 20))
+    with pytest.raises(shm.capi.ShmInvalidArgument):
+        shm.Bootstrapper(None, plan_only=(10, 8))  # too few primes for the level budget
+    with pytest.raises(shm.capi.ShmInvalidArgument):
+        shm.Bootstrapper(None, deg=64, plan_only=(10, 20))
