@@ -254,10 +254,11 @@ int shm_multiply_by_i_inplace(shm_context *ctx, shm_ciphertext *ct, int negate, 
 int shm_bootstrap_slot_to_coeff(shm_bootstrapper *b, shm_ciphertext *ct, const shm_galois_keys *gk,
                                 shm_stream_t stream);
 /* coefftoslot_full_3 (Bootstrapper.cu:3048-3067): ct (top level, after modraise) ->
- * real and imaginary parts, each divided by K (the EvalMod input form) */
+ * real and imaginary parts of its coefficient vector (bit-reversed), 3 levels */
 int shm_bootstrap_coeff_to_slot(shm_bootstrapper *b, shm_ciphertext *ct, shm_ciphertext *re, shm_ciphertext *im,
                                 const shm_galois_keys *gk, shm_stream_t stream);
-/* ModularReducer::modular_reduction (ModularReducer.cu:64-84): slots y -> sin(2 pi K y) */
+/* ModularReducer::modular_reduction (ModularReducer.cu:64-84): slots x with
+ * |x| < K -> sin(2 pi x), same scale; 1 + ceil(log2(deg+1)) + 1 + double_angles levels */
 int shm_bootstrap_eval_mod(shm_bootstrapper *b, shm_ciphertext *ct, const shm_kswitch_key *relin,
                            shm_stream_t stream);
 /* BatchBootstrapper::bootstrap_ordinary_s2c_first (BatchBootstrap.cuh:674-709) over

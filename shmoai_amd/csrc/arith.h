@@ -79,4 +79,20 @@ __device__ __forceinline__ void mac128(Acc128 &acc, uint64_t a, uint64_t b) {
 
 __device__ __forceinline__ uint64_t reduce128(const Acc128 &acc, const DevMod &m) { return barrett128(acc.hi, acc.lo, m); }
 
+// [v]_q of an integral double v with |v| < 2^128; negative v maps to q - (|v| mod q),
+// which is q itself when |v| is a multiple of q (callers reduce when they need [0, q)).
+__device__ __forceinline__ uint64_t residue_of_rounded(double v, const DevMod &m) {
+    const bool neg = signbit(v);
+    const double a = fabs(v);
+    uint64_t r;
+    if (a < 18446744073709551616.0) {
+        r = reduce64((uint64_t)a, m);
+    } else {
+        const uint64_t lo = (uint64_t)fmod(a, 18446744073709551616.0);
+        const uint64_t hi = (uint64_t)(a / 18446744073709551616.0);
+        r = barrett128(hi, lo, m);
+    }
+    return neg ? m.q - r : r;
+}
+
 }  // namespace shm

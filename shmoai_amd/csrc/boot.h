@@ -90,7 +90,8 @@ void lt_apply(Context &c, const LtLayer &L, Ciphertext &ct, const GaloisKeys &gk
 void slot_to_coeff(Bootstrapper &b, Ciphertext &ct, const GaloisKeys &gk, hipStream_t s);
 void coeff_to_slot(Bootstrapper &b, Ciphertext &ct, Ciphertext &re, Ciphertext &im, const GaloisKeys &gk,
                    hipStream_t s);
-void eval_mod(Bootstrapper &b, Ciphertext &x, const KSwitchKey &rk, hipStream_t s);
+void eval_mod(Bootstrapper &b, Ciphertext &x, const KSwitchKey &rk, double out_const, double out_scale,
+              hipStream_t s);
 void bootstrap_s2c_first(Bootstrapper &b, const Ciphertext &in, Ciphertext &out, const GaloisKeys &gk,
                          const KSwitchKey &rk, double out_scale, hipStream_t s);
 // Multiply by the constant `value` encoded at scale `sigma` and accumulate:

@@ -270,7 +270,7 @@ std::unique_ptr<Bootstrapper> make_plan(size_t N, size_t nQ, const BootParams &p
     while ((size_t(1) << b->logn) < b->n) b->logn++;
     const int logn = b->logn;
     if (logn < 3) throw InvalidArgument("ring degree too small for three transform layers");
-    // level budget: S2C 3, C2S 3, EvalMod ceil(log2(deg+1)) + 1 + r, amplitude 1, output >= 1 prime
+    // level budget: S2C 3, C2S 3, EvalMod 1 (x/K) + ceil(log2(deg+1)) + 1 + r, output >= 1 prime
     int dpoly = 0;
     while ((1 << dpoly) < p.deg + 1) dpoly++;
     const size_t need = 3 + (size_t)(dpoly + 1 + p.double_angles) + 1 + 1;
@@ -284,9 +284,9 @@ std::unique_ptr<Bootstrapper> make_plan(size_t N, size_t nQ, const BootParams &p
     b->s2c[2] = make_layer(b->n, p1 + p2 + 1, logn, false, p.layer3_factor);
     for (int i = 0; i < 3; i++) b->s2c[i].chain = chain_in + i;
     // coefficient-to-slot split (coefftoslot_full_3): q1, q2 top stages first; the
-    // first layer also carries 1/2 (real-part extraction) and 1/K (EvalMod domain)
+    // first layer also carries the 1/2 of the real / imaginary part extraction
     const int q1 = logn / 3, q2 = (logn - q1) / 2;
-    b->c2s[0] = make_layer(b->n, logn - q1 + 1, logn, true, 0.5 / (double)p.K);
+    b->c2s[0] = make_layer(b->n, logn - q1 + 1, logn, true, 0.5);
     b->c2s[1] = make_layer(b->n, logn - q1 - q2 + 1, logn - q1, true, 1.0);
     b->c2s[2] = make_layer(b->n, 1, logn - q1 - q2, true, 1.0);
     for (int i = 0; i < 3; i++) b->c2s[i].chain = 1 + i;

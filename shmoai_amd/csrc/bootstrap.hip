@@ -50,42 +50,25 @@ __global__ void k_lt_mac(uint64_t *acc, PtPack pk, int nterms, const DevMod *mod
 struct SrcPack {
     const uint64_t *x[kMaxTerms];
     uint32_t ql[kMaxTerms];  // limb count (stride) of each source ciphertext
+    double v[kMaxTerms];     // integral constants round(value * sigma)
 };
 
-// acc[p][l][i] (+)= sum_k x_k[p][l][i] * cst[k][l]: a linear combination of
-// ciphertexts at different levels, each read over the first Ql limbs.
-__global__ void k_const_lincomb(uint64_t *acc, SrcPack sp, int nterms, const uint64_t *cst, const DevMod *mods,
-                                size_t Ql, size_t N, int accumulate) {
+// acc[p][l][i] = sum_k x_k[p][l][i] * [v_k]_{q_l}: a linear combination of
+// ciphertexts at different levels, each read over the first Ql limbs; the
+// constant residues are formed in-thread from the integral doubles.
+__global__ void k_const_lincomb(uint64_t *acc, SrcPack sp, int nterms, const DevMod *mods, size_t Ql, size_t N) {
     const size_t S = Ql * N, total = 2 * S;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
         const size_t p = t / S, r = t % S, l = r / N, i = r % N;
-        Acc128 a{accumulate ? acc[t] : 0, 0};
-        for (int k = 0; k < nterms; k++) mac128(a, sp.x[k][(p * sp.ql[k] + l) * N + i], cst[k * Ql + l]);
-        acc[t] = reduce128(a, mods[l]);
-    }
-}
-
-uint64_t h_mulmod(uint64_t a, uint64_t b, uint64_t q) { return (uint64_t)((unsigned __int128)a * b % q); }
-
-// [v]_q for an integral double v (any magnitude), negative values wrap.
-uint64_t double_to_residue(double v, uint64_t q) {
-    const bool neg = v < 0;
-    double a = std::fabs(v);
-    uint64_t r;
-    if (a < 18446744073709551616.0) {
-        r = (uint64_t)a % q;
-    } else {
-        int e;
-        const double m = std::frexp(a, &e);                 // a = m 2^e, m in [0.5, 1)
-        const uint64_t mant = (uint64_t)std::ldexp(m, 53);  // exact
-        uint64_t p2 = 1 % q, b = 2 % q;
-        for (int x = e - 53; x > 0; x >>= 1) {
-            if (x & 1) p2 = h_mulmod(p2, b, q);
-            b = h_mulmod(b, b, q);
+        const DevMod m = mods[l];
+        Acc128 a{0, 0};
+        for (int k = 0; k < nterms; k++) {
+            uint64_t c = residue_of_rounded(sp.v[k], m);
+            c = c >= m.q ? c - m.q : c;
+            mac128(a, sp.x[k][(p * sp.ql[k] + l) * N + i], c);
         }
-        r = h_mulmod(mant % q, p2, q);
+        acc[t] = reduce128(a, m);
     }
-    return (neg && r) ? q - r : r;
 }
 
 void add_checked(Context &c, Ciphertext &a, const Ciphertext &b, hipStream_t s) {
@@ -109,24 +92,19 @@ void const_lincomb(Context &c, Ciphertext &acc, const std::vector<const Cipherte
                    const std::vector<double> &values, const std::vector<double> &sigmas, size_t Ql, hipStream_t s) {
     if (src.empty() || src.size() > (size_t)kMaxTerms || values.size() != src.size() || sigmas.size() != src.size())
         throw InvalidArgument("const_lincomb: 1..16 terms");
-    std::vector<uint64_t> cst(src.size() * Ql);
     SrcPack sp{};
     for (size_t k = 0; k < src.size(); k++) {
         if (src[k]->Ql() < Ql || src[k]->size != 2) throw LogicError("const_lincomb: source below the target level");
         sp.x[k] = src[k]->data();
         sp.ql[k] = (uint32_t)src[k]->Ql();
-        const double v = std::round(values[k] * sigmas[k]);
-        for (size_t l = 0; l < Ql; l++) cst[k * Ql + l] = double_to_residue(v, c.q[l]);
+        sp.v[k] = std::round(values[k] * sigmas[k]);
+        if (!(std::fabs(sp.v[k]) < 3.4e38)) throw InvalidArgument("constant out of range");
     }
-    Scratch d(cst.size() * sizeof(uint64_t), s);
-    SHM_HIP(hipMemcpyAsync(d.as<uint64_t>(), cst.data(), cst.size() * sizeof(uint64_t), hipMemcpyHostToDevice, s));
     acc.ctx = &c;
-    acc.resize(2, c.nQ - Ql + 1);
+    acc.resize(2, c.nQ - Ql + 1, s);
     acc.ntt = true;
-    k_const_lincomb<<<grid_for(2 * Ql * c.N), kBlock, 0, s>>>(acc.data(), sp, (int)src.size(), d.as<uint64_t>(),
-                                                              c.mods(), Ql, c.N, 0);
+    k_const_lincomb<<<grid_for(2 * Ql * c.N), kBlock, 0, s>>>(acc.data(), sp, (int)src.size(), c.mods(), Ql, c.N);
     SHM_HIP(hipGetLastError());
-    SHM_HIP(hipStreamSynchronize(s));  // cst is host memory
 }
 
 // bsgs_linear_transform (Bootstrapper.cu:2168-2251): babies rot(x, j*base),
@@ -158,7 +136,7 @@ void lt_apply(Context &c, const LtLayer &L, Ciphertext &ct, const GaloisKeys &gk
         if (!nt) continue;
         Ciphertext inner;
         inner.ctx = &c;
-        inner.resize(2, ct.chain);
+        inner.resize(2, ct.chain, s);
         inner.ntt = true;
         inner.scale = ct.scale;
         k_lt_mac<<<grid_for(2 * Ql * c.N), kBlock, 0, s>>>(inner.data(), pk, nt, c.mods(), Ql, c.N);
@@ -206,17 +184,27 @@ struct ChebEval {
     hipStream_t s;
     std::vector<std::unique_ptr<Ciphertext>> T;
 
-    ChebEval(Context &c_, const KSwitchKey &rk_, hipStream_t s_, Ciphertext &y) : c(c_), rk(rk_), s(s_), T(65) {
-        T[1] = std::make_unique<Ciphertext>();
-        ct_clone(*T[1], y, s);
+    ChebEval(Context &c_, const KSwitchKey &rk_, hipStream_t s_, Ciphertext &&t1) : c(c_), rk(rk_), s(s_), T(65) {
+        T[1] = std::make_unique<Ciphertext>(std::move(t1));
     }
+
+    static int split_a(int k) {
+        int a = 1;
+        while (2 * a < k) a *= 2;
+        return a;
+    }
+    // multiplicative depth of T_k above T_1 under the rule used by get()
+    static int depth(int k) {
+        if (k <= 1) return 0;
+        const int a = split_a(k);
+        return std::max(depth(a), depth(k - a)) + 1;
+    }
+    size_t tchain(int k) const { return T[1]->chain + depth(k); }
 
     // T_k = 2 T_a T_b - T_{a-b}, a the largest power of two below k (T_{2a} = 2 T_a^2 - 1).
     Ciphertext &get(int k) {
         if (T[k]) return *T[k];
-        int a = 1;
-        while (2 * a < k) a *= 2;
-        const int b = k - a;
+        const int a = split_a(k), b = k - a;
         Ciphertext &ta = get(a);
         Ciphertext &tb = get(b);
         auto out = std::make_unique<Ciphertext>();
@@ -230,12 +218,30 @@ struct ChebEval {
             ct_align(c, tmp, out->chain, out->scale, s);
             ct_sub(c, *out, tmp, s);
         }
+        if (out->chain != tchain(k)) throw LogicError("Chebyshev basis level bookkeeping");
         T[k] = std::move(out);
         return *T[k];
     }
 
-    // sum_k p[k] T_k landing at (chain, scale): products at chain-1 with each
-    // constant encoded at scale * q_last / scale(T_k), one rescale.
+    size_t baby_chain(const std::vector<double> &p) const {
+        size_t ch = 0;
+        for (size_t k = 1; k < p.size(); k++)
+            if (p[k] != 0.0) ch = std::max(ch, tchain((int)k));
+        return ch + 1;
+    }
+
+    // chain index eval() lands on
+    size_t eval_chain(const std::vector<double> &p) const {
+        const size_t deg = p.size() - 1;
+        if (deg < 16) return baby_chain(p);
+        const size_t m = deg >= 32 ? 32 : 16;
+        std::vector<double> q, r;
+        cheb_split(p, m, q, r);
+        return std::max(tchain((int)m), eval_chain(r)) + 1;
+    }
+
+    // sum_k p[k] T_k landing exactly at (chain, scale): products at chain-1 with
+    // each constant encoded at scale * q_last / scale(T_k), one rescale.
     void baby(const std::vector<double> &p, size_t chain, double scale, Ciphertext &out) {
         std::vector<const Ciphertext *> src;
         std::vector<double> vals, sig;
@@ -257,27 +263,26 @@ struct ChebEval {
         if (p[0] != 0.0) ct_add_const(c, out, p[0], s);
     }
 
-    size_t natural_chain(const std::vector<double> &p) {
-        size_t ch = 0;
-        for (size_t k = 1; k < p.size(); k++)
-            if (p[k] != 0.0) ch = std::max(ch, get((int)k).chain);
-        return ch + 1;
-    }
-
     // Homomorphic_poly_evaluation (common/Polynomial.cu:253-476) in Chebyshev
-    // form: p = q + T_m r with m in {32, 16}, babies T_1..T_15.
-    void eval(const std::vector<double> &p, Ciphertext &out) {
+    // form: p = q + T_m r with m in {32, 16}, babies T_1..T_15. The result
+    // carries exactly `target` as its scale: r is evaluated at the scale that
+    // the product with T_m maps onto `target`.
+    void eval(const std::vector<double> &p, double target, Ciphertext &out) {
         const size_t deg = p.size() - 1;
         if (deg < 16) {
-            baby(p, natural_chain(p), T[1]->scale, out);
+            baby(p, baby_chain(p), target, out);
             return;
         }
         const size_t m = deg >= 32 ? 32 : 16;
         std::vector<double> q, r;
         cheb_split(p, m, q, r);
+        const size_t pc = std::max(tchain((int)m), eval_chain(r));
+        const double qp = (double)c.q[c.ql_of_chain(pc) - 1];
+        Ciphertext &Tm = get((int)m);
         Ciphertext R;
-        eval(r, R);
-        ct_mul_relin_rescale(c, out, get((int)m), R, rk, s);
+        eval(r, target * qp / Tm.scale, R);
+        ct_mul_relin_rescale(c, out, Tm, R, rk, s);
+        out.scale = target;
         bool q_terms = false;
         for (size_t k = 1; k < q.size(); k++) q_terms |= q[k] != 0.0;
         if (!q_terms) {
@@ -285,30 +290,56 @@ struct ChebEval {
             return;
         }
         Ciphertext Q;
-        if (q.size() <= 16)
-            baby(q, out.chain, out.scale, Q);
-        else
-            eval(q, Q);
-        add_aligned(c, out, Q, s);
+        if (q.size() <= 16) {
+            baby(q, out.chain, target, Q);
+        } else {
+            eval(q, target, Q);
+            if (Q.chain < out.chain) ct_align(c, Q, out.chain, target, s);
+        }
+        add_checked(c, out, Q, s);
     }
 };
 
 }  // namespace
 
-// ModularReducer::modular_reduction (ModularReducer.cu:64-84): x holds y = x'/K
-// with x' = eps + integer; result cos(2 pi (x' - 1/4)) = sin(2 pi x').
-void eval_mod(Bootstrapper &b, Ciphertext &x, const KSwitchKey &rk, hipStream_t s) {
+// ModularReducer::modular_reduction (ModularReducer.cu:64-84). x holds slots
+// x' = eps + integer with |x'| < K at any scale; the result holds
+// out_const * sin(2 pi x') = out_const * cos(2 pi (x' - 1/4)) at scale out_scale.
+// Steps: y = x'/K (one constant multiply, so that the noise keeps its size
+// relative to x'), the Chebyshev polynomial of cos(2 pi (K y - 1/4) / 2^r), then
+// r double angles c -> 2c^2 - 1. Each double angle reads its square at half the
+// scale (2c^2 for free) and the last one also absorbs out_const; the polynomial
+// targets the scale that this chain of squarings maps onto out_scale.
+void eval_mod(Bootstrapper &b, Ciphertext &x, const KSwitchKey &rk, double out_const, double out_scale,
+              hipStream_t s) {
     Context &c = *b.ctx;
-    Ciphertext out;
-    {
-        ChebEval ev(c, rk, s, x);
-        ev.eval(b.cheb, out);
+    const int r = b.prm.double_angles;
+    Ciphertext t1;
+    ct_clone(t1, x, s);
+    const double s1 = x.scale;
+    ct_mul_const_rescale(c, t1, 1.0 / (double)b.prm.K, s1 * (double)c.q[t1.Ql() - 1] / t1.scale, s);
+    t1.scale = s1;
+    ChebEval ev(c, rk, s, std::move(t1));
+    std::vector<double> coeffs = b.cheb;
+    if (r == 0)
+        for (auto &v : coeffs) v *= out_const;
+    const size_t cp = ev.eval_chain(coeffs);
+    if (c.ql_of_chain(cp) < (size_t)r + 1) throw LogicError("end of modulus switching chain reached");
+    std::vector<double> sig(r + 1);
+    sig[r] = out_scale;
+    for (int j = r - 1; j >= 0; j--) {
+        const double qj = (double)c.q[c.ql_of_chain(cp + j) - 1];
+        sig[j] = std::sqrt(2.0 * (j == r - 1 ? out_const : 1.0) * qj * sig[j + 1]);
     }
-    for (int i = 0; i < b.prm.double_angles; i++) {
+    Ciphertext out;
+    ev.eval(coeffs, sig[0], out);
+    if (out.chain != cp) throw LogicError("EvalMod level bookkeeping");
+    for (int j = 0; j < r; j++) {
         Ciphertext sq;
         ct_mul_relin_rescale(c, sq, out, out, rk, s);
-        ct_add(c, sq, sq, s);
-        ct_add_const(c, sq, -1.0, s);
+        const double cst = j == r - 1 ? out_const : 1.0;
+        sq.scale = sig[j + 1];  // = sig[j]^2 / (2 cst q_j): the square read as 2 cst c^2
+        ct_add_const(c, sq, -cst, s);
         out = std::move(sq);
     }
     x = std::move(out);
@@ -330,18 +361,18 @@ void bootstrap_s2c_first(Bootstrapper &b, const Ciphertext &in, Ciphertext &out,
     x.scale = q0;
     Ciphertext re, im;
     coeff_to_slot(b, x, re, im, gk, s);
-    eval_mod(b, re, rk, s);
-    eval_mod(b, im, rk, s);
-    ct_mul_monomial_half(c, im, false, s);  // i * sin(2 pi x_im)
-    add_checked(c, re, im, s);
-    // sin(2 pi eps)/(2 pi) ~ eps; the linear minimax constant over |eps| <= E,
-    // E = F * scale_in * bound / q0, absorbs the cubic term (error b E^3 / 4).
+    // amplitude correction (BatchBootstrap.cuh:444-460) folded into EvalMod:
+    // sin(2 pi eps)/(2 pi) ~ eps, with the linear minimax constant over
+    // |eps| <= E = F * scale_in * bound / q0 absorbing the cubic term (error
+    // (2 pi)^2 E^3 / 24), times q0 / (F * scale_in) to return to the message.
     const double F = b.prm.layer3_factor;
     const double E = F * d_in * b.prm.input_bound / q0;
     const double pi = 3.14159265358979323846;
     const double cst = (1.0 + 0.5 * pi * pi * E * E) / (2.0 * pi) * q0 / (F * d_in);
-    const double ql = (double)c.q[re.Ql() - 1];
-    ct_mul_const_rescale(c, re, cst, out_scale * ql / re.scale, s);
+    eval_mod(b, re, rk, cst, out_scale, s);
+    eval_mod(b, im, rk, cst, out_scale, s);
+    ct_mul_monomial_half(c, im, false, s);  // i * m_im
+    add_checked(c, re, im, s);
     re.scale = out_scale;
     out = std::move(re);
 }

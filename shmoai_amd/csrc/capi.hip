@@ -180,7 +180,12 @@ int shm_ciphertext_create(shm_context *ctx, uint32_t size, uint32_t chain, shm_c
 }
 
 int shm_ciphertext_destroy(shm_ciphertext *ct) {
-    return guard([&] { delete ct; });
+    return guard([&] {
+        // the block returns to the stream-ordered pool; work the caller queued on
+        // other streams may still read it, so drain first
+        if (ct) SHM_HIP(hipDeviceSynchronize());
+        delete ct;
+    });
 }
 
 int shm_ciphertext_info(const shm_ciphertext *ct, uint32_t *size, uint32_t *chain, uint32_t *cms, double *scale) {
@@ -229,7 +234,7 @@ int shm_ciphertext_copy(const shm_ciphertext *src, shm_ciphertext *dst, shm_stre
         need(src, "src");
         need(dst, "dst");
         dst->ct.ctx = src->ct.ctx;
-        dst->ct.resize(src->ct.size, src->ct.chain);
+        dst->ct.resize(src->ct.size, src->ct.chain, S(s));
         dst->ct.scale = src->ct.scale;
         dst->ct.ntt = src->ct.ntt;
         SHM_HIP(hipMemcpyAsync(dst->ct.data(), src->ct.data(), src->ct.size * src->ct.Ql() * src->ct.ctx->N * 8,
@@ -478,7 +483,7 @@ int shm_encrypt_symmetric(shm_context *ctx, const shm_secret_key *sk, const shm_
         const hipStream_t s = S(st);
         const size_t Ql = pt->pt.Ql(), N = c.N;
         out->ct.ctx = &c;
-        out->ct.resize(2, pt->pt.chain);
+        out->ct.resize(2, pt->pt.chain, s);
         out->ct.scale = pt->pt.scale;
         out->ct.ntt = true;
         uint64_t *c0 = out->ct.poly(0), *c1 = out->ct.poly(1);
@@ -629,7 +634,7 @@ int shm_multiply_inplace(shm_context *ctx, shm_ciphertext *a, const shm_cipherte
             launch_square(c, a->ct.data(), out.as<uint64_t>(), Ql, s);
         else
             launch_tensor(c, a->ct.data(), b->ct.data(), out.as<uint64_t>(), Ql, s);
-        a->ct.resize(3, a->ct.chain);
+        a->ct.resize(3, a->ct.chain, s);
         SHM_HIP(hipMemcpyAsync(a->ct.data(), out.as<uint64_t>(), S3 * 8, hipMemcpyDeviceToDevice, s));
         a->ct.scale = a->ct.scale * b->ct.scale;
     });
@@ -927,7 +932,8 @@ int shm_bootstrap_eval_mod(shm_bootstrapper *b, shm_ciphertext *ct, const shm_ks
     return guard([&] {
         need(ct, "ct");
         need(relin, "relin_keys");
-        boot::eval_mod(ready(b, S(st)), ct->ct, relin->k, S(st));
+        const double sc = ct->ct.scale;
+        boot::eval_mod(ready(b, S(st)), ct->ct, relin->k, 1.0, sc, S(st));
     });
 }
 

@@ -131,12 +131,43 @@ void DevBuf::alloc(size_t bytes) {
     bytes_ = bytes;
 }
 
-void DevBuf::release() {
-    // queued work may still read the buffer: drain the device before freeing
-    if (ptr_) {
-        (void)hipDeviceSynchronize();
-        (void)hipFree(ptr_);
+void DevBuf::alloc_async(size_t bytes, hipStream_t s) {
+    release();
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMallocAsync(&ptr_, bytes, s);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        ptr_ = nullptr;
+        throw std::bad_alloc();
     }
+    hip_check(e, "hipMallocAsync");
+    bytes_ = bytes;
+    s_ = s;
+    async_ = true;
+}
+
+void DevBuf::use_on(hipStream_t s) {
+    if (!ptr_ || !async_ || s == s_) return;
+    hipEvent_t ev;
+    SHM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    SHM_HIP(hipEventRecord(ev, s_));
+    SHM_HIP(hipStreamWaitEvent(s, ev, 0));
+    SHM_HIP(hipEventDestroy(ev));
+    s_ = s;
+}
+
+void DevBuf::release() {
+    if (ptr_) {
+        if (async_) {
+            (void)hipFreeAsync(ptr_, s_);
+        } else {
+            // queued work may still read the buffer: drain the device before freeing
+            (void)hipDeviceSynchronize();
+            (void)hipFree(ptr_);
+        }
+    }
+    async_ = false;
+    s_ = nullptr;
     ptr_ = nullptr;
     bytes_ = 0;
 }
@@ -192,6 +223,13 @@ std::unique_ptr<Context> make_context(int logN, const std::vector<int> &bits, si
     if (nP == 0 || nP >= bits.size()) throw InvalidArgument("special_modulus_size must be in [1, #primes)");
     auto c = std::make_unique<Context>();
     SHM_HIP(hipGetDevice(&c->device));
+    {
+        // keep freed pool blocks cached instead of returning them at every sync
+        hipMemPool_t pool;
+        SHM_HIP(hipDeviceGetDefaultMemPool(&pool, c->device));
+        uint64_t thr = UINT64_MAX;
+        SHM_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
+    }
     c->logN = logN;
     c->N = size_t(1) << logN;
     c->nP = nP;
@@ -379,15 +417,19 @@ std::unique_ptr<Context> make_context(int logN, const std::vector<int> &bits, si
     return c;
 }
 
-void Ciphertext::resize(size_t new_size, size_t new_chain) {
+void Ciphertext::resize(size_t new_size, size_t new_chain, hipStream_t s) {
     const size_t nl = ctx->ql_of_chain(new_chain);
     if (new_size > cap_size || nl > cap_limbs) {
-        if (buf.bytes()) SHM_HIP(hipDeviceSynchronize());  // old buffer may still be read by queued work
-        DevBuf nb(new_size * nl * ctx->N * sizeof(uint64_t));
-        SHM_HIP(hipMemset(nb.as<void>(), 0, new_size * nl * ctx->N * sizeof(uint64_t)));
-        buf = std::move(nb);
+        const size_t bytes = new_size * nl * ctx->N * sizeof(uint64_t);
+        DevBuf nb;
+        nb.alloc_async(bytes, s);
+        SHM_HIP(hipMemsetAsync(nb.as<void>(), 0, bytes, s));
+        buf.use_on(s);
+        buf = std::move(nb);  // the old block returns to the pool after the work queued on s
         cap_size = new_size;
         cap_limbs = nl;
+    } else {
+        buf.use_on(s);
     }
     size = new_size;
     chain = new_chain;

@@ -89,20 +89,6 @@ __global__ void k_fft_stage(double2 *buf, const double2 *roots, const uint32_t *
     }
 }
 
-__device__ __forceinline__ uint64_t residue_of_rounded(double v, const DevMod &m) {
-    const bool neg = signbit(v);
-    const double a = fabs(v);
-    uint64_t r;
-    if (a < 18446744073709551616.0) {
-        r = reduce64((uint64_t)a, m);
-    } else {
-        const uint64_t lo = (uint64_t)fmod(a, 18446744073709551616.0);
-        const uint64_t hi = (uint64_t)(a / 18446744073709551616.0);
-        r = barrett128(hi, lo, m);
-    }
-    return neg ? m.q - r : r;
-}
-
 // rns_base.cu:51-130 decompose_array (pt zero-filled beforehand).
 __global__ void k_decompose(const double2 *buf, uint64_t *pt, const DevMod *mods, uint32_t n, size_t ratio, size_t Ql,
                             size_t N) {

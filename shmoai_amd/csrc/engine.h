@@ -41,7 +41,10 @@ struct DeviceError : std::runtime_error {
 void hip_check(hipError_t e, const char *what);
 #define SHM_HIP(call) ::shm::hip_check((call), #call)
 
-// Device buffer owned by the host object (freed with the object).
+// Device buffer owned by the host object (freed with the object). alloc() is a
+// plain hipMalloc (freed after draining the device); alloc_async() takes the
+// block from the stream-ordered pool and returns it with hipFreeAsync on the
+// last stream the buffer was used on, so no host synchronisation is needed.
 class DevBuf {
   public:
     DevBuf() = default;
@@ -49,7 +52,7 @@ class DevBuf {
     ~DevBuf() { release(); }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : ptr_(o.ptr_), bytes_(o.bytes_) {
+    DevBuf(DevBuf &&o) noexcept : ptr_(o.ptr_), bytes_(o.bytes_), s_(o.s_), async_(o.async_) {
         o.ptr_ = nullptr;
         o.bytes_ = 0;
     }
@@ -58,12 +61,17 @@ class DevBuf {
             release();
             ptr_ = o.ptr_;
             bytes_ = o.bytes_;
+            s_ = o.s_;
+            async_ = o.async_;
             o.ptr_ = nullptr;
             o.bytes_ = 0;
         }
         return *this;
     }
     void alloc(size_t bytes);
+    void alloc_async(size_t bytes, hipStream_t s);
+    // order the eventual free after the work already queued on s
+    void use_on(hipStream_t s);
     void release();
     template <class T>
     T *as() const {
@@ -74,6 +82,8 @@ class DevBuf {
   private:
     void *ptr_ = nullptr;
     size_t bytes_ = 0;
+    hipStream_t s_ = nullptr;
+    bool async_ = false;
 };
 
 // Stream-ordered scratch (hipMallocAsync / hipFreeAsync on the op's stream).
@@ -159,7 +169,9 @@ struct Ciphertext {
     size_t Ql() const { return ctx->ql_of_chain(chain); }
     uint64_t *data() const { return buf.as<uint64_t>(); }
     uint64_t *poly(size_t i) const { return data() + i * Ql() * ctx->N; }
-    void resize(size_t new_size, size_t new_chain);  // keeps contents of the leading limbs only
+    // (re)allocates from the stream-ordered pool on s when the capacity is too small;
+    // contents are kept only when no reallocation happens
+    void resize(size_t new_size, size_t new_chain, hipStream_t s = nullptr);
 };
 
 struct Plaintext {

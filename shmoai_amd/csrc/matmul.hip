@@ -162,7 +162,7 @@ void ctpt_matmul(const Context &c, const std::vector<const Ciphertext *> &X, con
             throw InvalidArgument("encrypted and plain parameter mismatch");
     }
     const double out_scale = X[0]->scale;
-    for (Ciphertext *o : out) o->resize(2, X[0]->chain + 1);
+    for (Ciphertext *o : out) o->resize(2, X[0]->chain + 1, s);
 
     std::vector<double> scales(row_w);
     for (size_t j = 0; j < row_w; j++) scales[j] = X[j]->scale;

@@ -145,7 +145,7 @@ __global__ void k_mono_half(uint64_t *a, const DevMod *mods, const uint64_t *iot
 
 void ct_clone(Ciphertext &dst, const Ciphertext &src, hipStream_t s) {
     dst.ctx = src.ctx;
-    dst.resize(src.size, src.chain);
+    dst.resize(src.size, src.chain, s);
     dst.scale = src.scale;
     dst.ntt = src.ntt;
     SHM_HIP(hipMemcpyAsync(dst.data(), src.data(), src.size * src.Ql() * src.ctx->N * sizeof(uint64_t),
@@ -216,7 +216,7 @@ void ct_mul_relin_rescale(Context &c, Ciphertext &out, const Ciphertext &a, cons
     }
     const size_t Ql = c.ql_of_chain(chain);
     out.ctx = &c;
-    out.resize(3, chain);
+    out.resize(3, chain, s);
     if (pa == pb || (pa->data() == pb->data()))
         launch_square(c, pa->data(), out.data(), Ql, s);
     else
@@ -246,7 +246,7 @@ void ct_modraise(Context &c, Ciphertext &a, hipStream_t s) {
     ntt_inverse(c, low.as<uint64_t>(), 2, 1, 0, s);
     Ciphertext out;
     out.ctx = &c;
-    out.resize(2, 1);
+    out.resize(2, 1, s);
     for (size_t p = 0; p < 2; p++)
         k_modraise<<<grid_ops(nl * N), kBlockOps, 0, s>>>(low.as<uint64_t>() + p * N, out.poly(p), c.mods(), nl, N);
     SHM_HIP(hipGetLastError());

@@ -7,7 +7,7 @@ Pins (independent of how the plan is built):
   inner sum), equal F * U0 * R with U0[j][k] = w_j^k, w_j = exp(2 pi i 5^j / 2N)
   built from the definition (Bootstrapper.cu:634-727 genorigcoeff defines the
   same matrix) and R the bit reversal;
-- the three coefficient-to-slot layers equal (0.5 / K) * R * U0^-1 with
+- the three coefficient-to-slot layers equal 0.5 * R * U0^-1 with
   U0^-1 = U0^H / n (a closed form: rows of U0 are orthogonal);
 - every baby and giant offset the schedule uses is a listed rotation step;
 - the EvalMod polynomial approximates cos(2 pi (K y - 1/4) / 2^r) and, after r
@@ -62,11 +62,11 @@ def test_transform_layers_match_dft(log_n):
         x = apply_layer(boot, 0, layer, x)
     expect = F * (U @ z[perm])
     assert np.max(np.abs(x - expect)) < 1e-12 * n
-    # coefficient-to-slot: (0.5/K) * R * U0^H / n
+    # coefficient-to-slot: 0.5 * R * U0^H / n
     y = z.copy()
     for layer in range(3):
         y = apply_layer(boot, 1, layer, y)
-    expect = (0.5 / K) * ((U.conj().T @ z) / n)[perm]
+    expect = 0.5 * ((U.conj().T @ z) / n)[perm]
     assert np.max(np.abs(y - expect)) < 1e-13 * n
     # round trip through both transforms is the identity up to the factors
     r = z.copy()
@@ -74,7 +74,7 @@ def test_transform_layers_match_dft(log_n):
         r = apply_layer(boot, 0, layer, r)
     for layer in range(3):
         r = apply_layer(boot, 1, layer, r)
-    assert np.max(np.abs(r * (2 * K / F) - z)) < 1e-11
+    assert np.max(np.abs(r * (2 / F) - z)) < 1e-11
 
 
 @pytest.mark.parametrize("log_n", [10, 16])

@@ -95,8 +95,9 @@ def test_eval_mod_sine(small):
     eps = rng.uniform(-1 / 512, 1 / 512, n)
     x = ints + eps
     q0 = float(ctx.coeff_modulus[0])
-    ct = _encrypt(ctx, sk, x / K, q0, ctx.size_q - 3, 36)
+    ct = _encrypt(ctx, sk, x, q0, ctx.size_q - 3, 36)
     boot.eval_mod(ct, rk)
+    assert ct.scale == q0 and ct.coeff_modulus_size == ctx.size_q - 13
     got = shm.Encoder(ctx).decode(shm.Decryptor(ctx, sk).decrypt(ct))
     err = np.max(np.abs(got - np.sin(2 * np.pi * x)))
     assert err < 1e-6, err
@@ -123,3 +124,26 @@ def test_bootstrap_s2c_first_small(small):
     bad = _encrypt(ctx, sk, zs[0], scale, 5, 50)
     with pytest.raises(shm.capi.ShmInvalidArgument):
         boot.bootstrap_s2c_first([bad], gk, rk)
+
+
+def test_bootstrap_s2c_first_reference_params():
+    """N = 2^16, Q = 1 x 51 + 20 x 46 + 14 x 51 bits, P = 5 x 51 bits (dnum 7),
+    Hamming weight 192, scale 2^46, inputs with 4 primes
+    (test_shared_a.cuh:1207-1274 s2c_first_bootstrap_test parameters)."""
+    bits = [51] + [46] * 20 + [51] * 14 + [51] * 5
+    ctx = shm.Context(16, bits, 5)
+    sk = shm.SecretKey(ctx, seed=61, hamming_weight=192)
+    boot = shm.Bootstrapper(ctx)
+    gk = sk.create_galois_keys_from_steps(boot.rotation_steps(), seed=62)
+    rk = sk.gen_relinkey(seed=63)
+    n = ctx.N // 2
+    rng = np.random.default_rng(64)
+    zs = [rng.uniform(-1, 1, n), rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)]
+    cts = [_encrypt(ctx, sk, z, 2.0 ** 46, 4, 70 + i) for i, z in enumerate(zs)]
+    out = boot.bootstrap_s2c_first(cts, gk, rk)
+    dec, enc = shm.Decryptor(ctx, sk), shm.Encoder(ctx)
+    for z, o in zip(zs, out):
+        assert o.coeff_modulus_size == 22 and o.scale == 2.0 ** 46
+        err = np.max(np.abs(enc.decode(dec.decrypt(o), complex_out=True) - z))
+        print(f"bootstrap N=2^16 max err {err:.3e}", flush=True)
+        assert err <= 2e-5, err
