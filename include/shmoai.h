@@ -268,6 +268,59 @@ int shm_bootstrap_s2c_first(shm_bootstrapper *b, const shm_ciphertext *const *in
                             shm_ciphertext *const *out, const shm_galois_keys *gk, const shm_kswitch_key *relin,
                             double out_scale, shm_stream_t stream);
 
+/* ---- shared-a batches (SharedACiphertextBatch, SharedAKeyGen, FmtSwitch, Sh-Rot) ------
+ * A batch holds n members (b_i, a) that share the a-part; member i decrypts under the
+ * auxiliary secret s'_i. Batches of 1..16 members. */
+typedef struct shm_shared_a_keys shm_shared_a_keys;
+typedef struct shm_shared_a_batch shm_shared_a_batch;
+
+/* SharedAKeyGen(context, sk, n) + gen_format_key(chain_index) (FmtSwitch.cu:154-270):
+ * auxiliary secrets (Hamming weight hw) from splitmix64(seed, i), the format key at
+ * chain_index (the bootstrap input level) and the backward keys s'_i -> sk. Synchronises. */
+int shm_shared_a_keys_create(shm_context *ctx, const shm_secret_key *sk, uint32_t n, uint32_t hw, uint64_t seed,
+                             uint32_t chain_index, shm_shared_a_keys **out);
+int shm_shared_a_keys_destroy(shm_shared_a_keys *keys);
+/* gen_shared_galois_key / acquire_shared_galois_key (ShRotate.cu:54-195) for each step
+ * (0 = conjugation); keys stay resident (the reference's amortised mode). Synchronises. */
+int shm_shared_a_keys_add_galois(shm_shared_a_keys *keys, const int *steps, size_t n_steps, uint64_t seed);
+/* auxiliary secret i as N signed coefficients */
+int shm_shared_a_keys_aux_coeffs(const shm_shared_a_keys *keys, uint32_t i, int8_t *out, size_t n);
+/* format key words: a_fmt n x (Ql+P) x N, B n x n x (Ql+P) x N (index j*n + i), local limb order */
+int shm_shared_a_keys_export_format(const shm_shared_a_keys *keys, uint64_t *a_fmt, uint64_t *B, size_t n_words_B);
+/* shared Galois key of element elt: A dnum x QP x N, B dnum x n x QP x N */
+int shm_shared_a_keys_export_galois(const shm_shared_a_keys *keys, uint32_t elt, uint64_t *A, uint64_t *B,
+                                    size_t n_words_B);
+
+int shm_shared_a_batch_create(shm_context *ctx, uint32_t n, uint32_t chain_index, shm_shared_a_batch **out);
+int shm_shared_a_batch_destroy(shm_shared_a_batch *b);
+int shm_shared_a_batch_info(const shm_shared_a_batch *b, uint32_t *n, uint32_t *chain_index,
+                            uint32_t *coeff_mod_size, double *scale);
+int shm_shared_a_batch_set_scale(shm_shared_a_batch *b, double scale);
+/* words [n+1][Ql][N]: b_0..b_{n-1}, then a */
+int shm_shared_a_batch_import(shm_shared_a_batch *b, const uint64_t *host, size_t n_words, shm_stream_t stream);
+int shm_shared_a_batch_export(const shm_shared_a_batch *b, uint64_t *host, size_t n_words, shm_stream_t stream);
+/* SharedACiphertextBatch::assemble (SharedACiphertext.cuh:74-99): member i as (b_i, a) */
+int shm_shared_a_batch_member(const shm_shared_a_batch *b, uint32_t i, shm_ciphertext *out, shm_stream_t stream);
+
+/* fmt_switch_forward (FmtSwitch.cu:276-394): n ciphertexts at the key's level -> batch */
+int shm_fmt_switch_forward(shm_shared_a_keys *keys, const shm_ciphertext *const *cts, size_t n,
+                           shm_shared_a_batch *out, shm_stream_t stream);
+/* fmt_switch_backward (FmtSwitch.cu:396-429): batch -> n ciphertexts under sk */
+int shm_fmt_switch_backward(shm_shared_a_keys *keys, const shm_shared_a_batch *batch, shm_ciphertext *const *out,
+                            size_t n, shm_stream_t stream);
+/* shared_a_rotate (ShRotate.cu:201-257) by a slot step (0 = conjugation) */
+int shm_shared_a_rotate_inplace(shm_shared_a_keys *keys, shm_shared_a_batch *b, int step, shm_stream_t stream);
+/* SharedABatchOps.cu:16-166: add, multiply_plain, rescale_to_next, modraise */
+int shm_shared_a_add_inplace(shm_shared_a_batch *a, const shm_shared_a_batch *b, shm_stream_t stream);
+int shm_shared_a_multiply_plain_inplace(shm_shared_a_batch *b, const shm_plaintext *pt, shm_stream_t stream);
+int shm_shared_a_rescale_to_next_inplace(shm_shared_a_batch *b, shm_stream_t stream);
+int shm_shared_a_modraise_inplace(shm_shared_a_batch *b, shm_stream_t stream);
+/* BatchBootstrapper::bootstrap_shared_a_s2c_first (BatchBootstrap.cuh:719-820): exactly n
+ * ciphertexts with 4 primes (n = the keys' batch size); out_scale 0 = the input scale */
+int shm_bootstrap_shared_a_s2c_first(shm_bootstrapper *b, shm_shared_a_keys *keys, const shm_ciphertext *const *in,
+                                     size_t n, shm_ciphertext *const *out, const shm_kswitch_key *relin,
+                                     double out_scale, shm_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1088,4 +1088,160 @@ void ref_mul_monomial(const RefCtx *c, uint64_t *data, size_t nlimbs, size_t npo
     }
 }
 
+
+/* ---- shared-a format (FmtSwitch.cu, ShRotate.cu) --------------------------------------- */
+
+namespace {
+uint64_t prod_special_mod(const RefCtx *c, uint64_t q) {
+    uint64_t P = 1;
+    for (size_t jj = 0; jj < c->nP; jj++) P = mulmod(P, c->q[c->nQ + jj] % q, q);
+    return P;
+}
+}  // namespace
+
+/* FmtSwitch.cu:82-108 extend_Ql_to_P_kernel: coefficient residues x_g (g < Ql) ->
+ * residues mod every special prime of x = sum_g y_g QHat_g - v Ql, y_g =
+ * [x_g QHat_g^-1]_{q_g}, v = floor(sum_g y_g / q_g) summed in double in g order,
+ * rounded up when within 1e-9 below an integer. outP: [nP][N]. */
+void ref_extend_exact(const RefCtx *c, const uint64_t *coef, size_t Ql, uint64_t *outP) {
+    const size_t N = c->N, nP = c->nP;
+    std::vector<uint64_t> inv(Ql);
+    std::vector<uint32_t> S;
+    for (size_t g = 0; g < Ql; g++) S.push_back((uint32_t)g);
+    for (size_t g = 0; g < Ql; g++) inv[g] = invmod(punctured_prod_mod(*c, S, g, c->q[g]), c->q[g]);
+    for (size_t j = 0; j < N; j++) {
+        std::vector<uint64_t> y(Ql);
+        double v = 0.0;
+        for (size_t g = 0; g < Ql; g++) {
+            y[g] = mulmod(coef[g * N + j], inv[g], c->q[g]);
+            v += (double)y[g] / (double)c->q[g];
+        }
+        uint64_t alpha = (uint64_t)v;
+        if (v - (double)alpha > 1.0 - 1e-9) alpha += 1;
+        for (size_t k = 0; k < nP; k++) {
+            const uint64_t p = c->q[c->nQ + k];
+            u128 acc = 0;
+            for (size_t g = 0; g < Ql; g++) acc += (u128)y[g] * punctured_prod_mod(*c, S, g, p);
+            uint64_t Qlp = 1;
+            for (size_t g = 0; g < Ql; g++) Qlp = mulmod(Qlp, c->q[g] % p, p);
+            outP[k * N + j] = submod((uint64_t)(acc % p), mulmod(alpha % p, Qlp, p), p);
+        }
+    }
+}
+
+/* FmtSwitch.cu:164-270 gen_format_key with injected randomness (fseed =
+ * splitmix(seed, n); a_fmt[j] uniform from splitmix(fseed, j); e_{j,i} from
+ * splitmix(fseed, n + j n + i)): over QlP (local limb order),
+ * B[j][i] = -(a_fmt[j] s'_i + e_{j,i}) + [i == j] P s on the Ql limbs. */
+void ref_gen_fmt_key(const RefCtx *c, const uint64_t *sk_qp, const uint64_t *const *aux_qp, size_t n, size_t Ql,
+                     uint64_t seed, uint64_t *a_fmt, uint64_t *B) {
+    const size_t N = c->N, QlP = Ql + c->nP;
+    auto idx = idx_Ql(*c, Ql, true);
+    const uint64_t fseed = sm64(seed, n);
+    std::vector<int64_t> e(N);
+    std::vector<uint64_t> er(QlP * N);
+    for (size_t j = 0; j < n; j++) {
+        uint64_t *aj = a_fmt + j * QlP * N;
+        ref_sample_uniform(c, sm64(fseed, j), QlP, idx.data(), aj);
+        for (size_t i = 0; i < n; i++) {
+            ref_sample_gaussian(c, sm64(fseed, n + j * n + i), e.data());
+            for (size_t k = 0; k < QlP; k++)
+                for (size_t t = 0; t < N; t++) er[k * N + t] = from_signed(e[t], c->q[idx[k]]);
+            ntt_limbs(*c, er.data(), QlP, idx.data());
+            uint64_t *b = B + (j * n + i) * QlP * N;
+            for (size_t k = 0; k < QlP; k++) {
+                const uint32_t pi = idx[k];
+                const uint64_t q = c->q[pi], Pq = prod_special_mod(c, q);
+                for (size_t t = 0; t < N; t++) {
+                    uint64_t v = negmod(addmod(mulmod(aj[k * N + t], aux_qp[i][pi * N + t], q), er[k * N + t], q), q);
+                    if (i == j && k < Ql) v = addmod(v, mulmod(sk_qp[pi * N + t], Pq, q), q);
+                    b[k * N + t] = v;
+                }
+            }
+        }
+    }
+}
+
+/* FmtSwitch.cu:276-394 fmt_switch_forward: cts n x [2][Ql][N] -> out [n+1][Ql][N]
+ * (b'_0..b'_{n-1}, a'). */
+void ref_fmt_switch_forward(const RefCtx *c, const uint64_t *const *cts, size_t n, size_t Ql, const uint64_t *a_fmt,
+                            const uint64_t *B, uint64_t *out) {
+    const size_t N = c->N, nP = c->nP, QlP = Ql + nP;
+    auto idx = idx_Ql(*c, Ql, true);
+    auto idxQ = idx_Ql(*c, Ql, false);
+    std::vector<uint32_t> pP;
+    for (size_t k = 0; k < nP; k++) pP.push_back((uint32_t)(c->nQ + k));
+    std::vector<uint64_t> ext(n * QlP * N);
+    for (size_t j = 0; j < n; j++) {
+        const uint64_t *aj = cts[j] + Ql * N;
+        uint64_t *ej = ext.data() + j * QlP * N;
+        std::memcpy(ej, aj, Ql * N * sizeof(uint64_t));
+        std::vector<uint64_t> coef(aj, aj + Ql * N);
+        intt_limbs(*c, coef.data(), Ql, idxQ.data());
+        ref_extend_exact(c, coef.data(), Ql, ej + Ql * N);
+        ntt_limbs(*c, ej + Ql * N, nP, pP.data());
+    }
+    std::vector<uint64_t> cx(QlP * N), md(Ql * N);
+    for (size_t o = 0; o <= n; o++) {
+        for (size_t k = 0; k < QlP; k++) {
+            const uint64_t q = c->q[idx[k]];
+            for (size_t t = 0; t < N; t++) {
+                u128 acc = 0;
+                for (size_t j = 0; j < n; j++) {
+                    const uint64_t *key = o < n ? B + (j * n + o) * QlP * N : a_fmt + j * QlP * N;
+                    acc += (u128)ext[(j * QlP + k) * N + t] * key[k * N + t];
+                }
+                cx[k * N + t] = (uint64_t)(acc % q);
+            }
+        }
+        ref_moddown(c, cx.data(), Ql, md.data());
+        uint64_t *dst = out + o * Ql * N;
+        for (size_t k = 0; k < Ql; k++)
+            for (size_t t = 0; t < N; t++) {
+                const size_t i = k * N + t;
+                dst[i] = o < n ? addmod(cts[o][i], md[i], c->q[k]) : md[i];
+            }
+    }
+}
+
+/* ShRotate.cu:54-162 gen_shared_galois_key with injected randomness (kseed =
+ * splitmix(seed, elt); A_g from splitmix(kseed, g (n+1)); e_{g,i} from
+ * splitmix(kseed, g (n+1) + 1 + i)): A [dnum][QP][N], B [dnum][n][QP][N] with
+ * B_{g,i} = -(A_g s'_i + e) + P [sigma(s'_i)]_{digit g}. */
+void ref_gen_shared_galois_key(const RefCtx *c, const uint64_t *const *aux_qp, size_t n, uint32_t elt, uint64_t seed,
+                               uint64_t *A, uint64_t *B) {
+    const size_t N = c->N, QP = c->nQP(), alpha = c->nP, dnum = c->nQ / c->nP;
+    auto idx = idx_Ql(*c, c->nQ, true);
+    const uint64_t kseed = sm64(seed, elt);
+    std::vector<std::vector<uint64_t>> sig(n, std::vector<uint64_t>(QP * N));
+    for (size_t i = 0; i < n; i++) ref_apply_galois_ntt(c, elt, aux_qp[i], sig[i].data(), QP);
+    std::vector<int64_t> e(N);
+    std::vector<uint64_t> er(QP * N);
+    for (size_t g = 0; g < dnum; g++) {
+        uint64_t *Ag = A + g * QP * N;
+        ref_sample_uniform(c, sm64(kseed, g * (n + 1)), QP, idx.data(), Ag);
+        for (size_t i = 0; i < n; i++) {
+            ref_sample_gaussian(c, sm64(kseed, g * (n + 1) + 1 + i), e.data());
+            for (size_t k = 0; k < QP; k++)
+                for (size_t t = 0; t < N; t++) er[k * N + t] = from_signed(e[t], c->q[k]);
+            ntt_limbs(*c, er.data(), QP, idx.data());
+            uint64_t *b = B + (g * n + i) * QP * N;
+            for (size_t k = 0; k < QP; k++) {
+                const uint64_t q = c->q[k];
+                for (size_t t = 0; t < N; t++) {
+                    const size_t x = k * N + t;
+                    b[x] = negmod(addmod(mulmod(Ag[x], aux_qp[i][x], q), er[x], q), q);
+                }
+            }
+            for (size_t k = g * alpha; k < (g + 1) * alpha; k++) {
+                const uint64_t q = c->q[k], Pq = prod_special_mod(c, q);
+                for (size_t t = 0; t < N; t++) {
+                    const size_t x = k * N + t;
+                    b[x] = addmod(b[x], mulmod(sig[i][x], Pq, q), q);
+                }
+            }
+        }
+    }
+}
+
 }  // extern "C"

@@ -93,6 +93,13 @@ void ref_decode(const RefCtx *c, const uint64_t *pt, size_t Ql, size_t sparse_sl
 /* MOAI ct-pt matmul --------------------------------------------------------- */
 void ref_modraise(const RefCtx *c, const uint64_t *in, uint64_t *out);
 void ref_mul_monomial(const RefCtx *c, uint64_t *data, size_t nlimbs, size_t npolys, size_t power);
+void ref_extend_exact(const RefCtx *c, const uint64_t *coef, size_t Ql, uint64_t *outP);
+void ref_gen_fmt_key(const RefCtx *c, const uint64_t *sk_qp, const uint64_t *const *aux_qp, size_t n, size_t Ql,
+                     uint64_t seed, uint64_t *a_fmt, uint64_t *B);
+void ref_fmt_switch_forward(const RefCtx *c, const uint64_t *const *cts, size_t n, size_t Ql, const uint64_t *a_fmt,
+                            const uint64_t *B, uint64_t *out);
+void ref_gen_shared_galois_key(const RefCtx *c, const uint64_t *const *aux_qp, size_t n, uint32_t elt, uint64_t seed,
+                               uint64_t *A, uint64_t *B);
 void ref_ctpt_matmul(const RefCtx *c, const uint64_t *const *X, size_t row_w, const double *W, size_t col_w,
                      size_t Ql, const double *scales, size_t col_begin, size_t col_end, uint64_t *const *out);
 

@@ -84,6 +84,11 @@ def lib():
                                        C.POINTER(C.c_void_p)]),
             "ref_modraise": (None, [vp, u64p, u64p]),
             "ref_mul_monomial": (None, [vp, u64p, sz, sz, sz]),
+            "ref_extend_exact": (None, [vp, u64p, sz, u64p]),
+            "ref_gen_fmt_key": (None, [vp, u64p, C.POINTER(C.c_void_p), sz, sz, C.c_uint64, u64p, u64p]),
+            "ref_fmt_switch_forward": (None, [vp, C.POINTER(C.c_void_p), sz, sz, u64p, u64p, u64p]),
+            "ref_gen_shared_galois_key": (None, [vp, C.POINTER(C.c_void_p), sz, C.c_uint32, C.c_uint64, u64p,
+                                                 u64p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -338,3 +343,40 @@ class Ref:
         d = np.ascontiguousarray(data, np.uint64).copy()
         self.L.ref_mul_monomial(self.h, d, nlimbs, npolys, power)
         return d
+
+    # shared-a format ------------------------------------------------------------------
+    @staticmethod
+    def _pp(arrs):
+        return (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+
+    def extend_exact(self, coef, Ql):
+        """FmtSwitch.cu:82-108: coefficient residues over Ql primes -> residues mod P primes."""
+        out = np.zeros(self.nP * self.N, np.uint64)
+        self.L.ref_extend_exact(self.h, np.ascontiguousarray(coef, np.uint64), Ql, out)
+        return out
+
+    def gen_fmt_key(self, sk_qp, aux_qp, Ql, seed):
+        n = len(aux_qp)
+        QlP = Ql + self.nP
+        aux = [np.ascontiguousarray(a, np.uint64) for a in aux_qp]
+        a_fmt = np.zeros(n * QlP * self.N, np.uint64)
+        B = np.zeros(n * n * QlP * self.N, np.uint64)
+        self.L.ref_gen_fmt_key(self.h, np.ascontiguousarray(sk_qp, np.uint64), self._pp(aux), n, Ql, seed, a_fmt, B)
+        return a_fmt, B
+
+    def fmt_switch_forward(self, cts, Ql, a_fmt, B):
+        n = len(cts)
+        cs = [np.ascontiguousarray(c, np.uint64) for c in cts]
+        out = np.zeros((n + 1) * Ql * self.N, np.uint64)
+        self.L.ref_fmt_switch_forward(self.h, self._pp(cs), n, Ql, np.ascontiguousarray(a_fmt, np.uint64),
+                                      np.ascontiguousarray(B, np.uint64), out)
+        return out
+
+    def gen_shared_galois_key(self, aux_qp, elt, seed):
+        n = len(aux_qp)
+        aux = [np.ascontiguousarray(a, np.uint64) for a in aux_qp]
+        dnum = self.nQ // self.nP
+        A = np.zeros(dnum * self.nQP * self.N, np.uint64)
+        B = np.zeros(dnum * n * self.nQP * self.N, np.uint64)
+        self.L.ref_gen_shared_galois_key(self.h, self._pp(aux), n, elt, seed, A, B)
+        return A, B

@@ -19,6 +19,7 @@ from .capi import check, MATMUL_U64, MATMUL_PER_CT, MATMUL_INT8  # noqa: F401
 __all__ = [
     "Context", "Ciphertext", "Plaintext", "SecretKey", "KSwitchKey", "GaloisKeys", "Encoder", "Evaluator",
     "Decryptor", "ct_pt_matrix_mul_wo_pre", "MATMUL_U64", "MATMUL_PER_CT", "MATMUL_INT8", "Bootstrapper",
+    "SharedAKeys", "SharedABatch",
 ]
 
 
@@ -510,3 +511,144 @@ def last_matmul_kernel_ms() -> float:
 
 def last_matmul_variant() -> int:
     return int(_lib().shm_last_matmul_variant())
+
+
+def _ptrs(objs):
+    return (C.c_void_p * len(objs))(*[o.h.value for o in objs])
+
+
+class SharedABatch:
+    """SharedACiphertextBatch (SharedACiphertext.cuh:22-99): n members (b_i, a)."""
+
+    def __init__(self, ctx: Context, n: int, chain_index: int = 1):
+        self.ctx, self.n = ctx, n
+        h = C.c_void_p()
+        check(_lib().shm_shared_a_batch_create(ctx.h, n, chain_index, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                _lib().shm_shared_a_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def info(self):
+        n, ch, cm, sc = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_double()
+        check(_lib().shm_shared_a_batch_info(self.h, C.byref(n), C.byref(ch), C.byref(cm), C.byref(sc)))
+        return n.value, ch.value, cm.value, sc.value
+
+    @property
+    def chain_index(self):
+        return self.info()[1]
+
+    @property
+    def coeff_modulus_size(self):
+        return self.info()[2]
+
+    @property
+    def scale(self):
+        return self.info()[3]
+
+    @scale.setter
+    def scale(self, v: float):
+        check(_lib().shm_shared_a_batch_set_scale(self.h, v))
+
+    def words(self) -> int:
+        return (self.n + 1) * self.coeff_modulus_size * self.ctx.N
+
+    def load(self, limbs: np.ndarray, stream=None) -> "SharedABatch":
+        a = np.ascontiguousarray(limbs, np.uint64)
+        check(_lib().shm_shared_a_batch_import(self.h, _u64p(a), a.size, stream))
+        return self
+
+    def numpy(self, stream=None) -> np.ndarray:
+        out = np.zeros(self.words(), np.uint64)
+        check(_lib().shm_shared_a_batch_export(self.h, _u64p(out), out.size, stream))
+        return out
+
+    def member(self, i: int, stream=None) -> Ciphertext:
+        ct = Ciphertext(self.ctx, 2, 1)
+        check(_lib().shm_shared_a_batch_member(self.h, i, ct.h, stream))
+        return ct
+
+    def add_inplace(self, other: "SharedABatch", stream=None):
+        check(_lib().shm_shared_a_add_inplace(self.h, other.h, stream))
+
+    def multiply_plain_inplace(self, pt: Plaintext, stream=None):
+        check(_lib().shm_shared_a_multiply_plain_inplace(self.h, pt.h, stream))
+
+    def rescale_to_next_inplace(self, stream=None):
+        check(_lib().shm_shared_a_rescale_to_next_inplace(self.h, stream))
+
+    def modraise_inplace(self, stream=None):
+        check(_lib().shm_shared_a_modraise_inplace(self.h, stream))
+
+
+class SharedAKeys:
+    """SharedAKeyGen + SharedAFormatKey (FmtSwitch.cuh:28-105) and shared-a Galois
+    keys (ShRotate.cuh:25-35), all resident in HBM."""
+
+    def __init__(self, ctx: Context, sk: SecretKey, n: int, hamming_weight: int, seed: int, chain_index: int):
+        self.ctx, self.n, self.chain_index = ctx, n, chain_index
+        h = C.c_void_p()
+        check(_lib().shm_shared_a_keys_create(ctx.h, sk.h, n, hamming_weight, seed, chain_index, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                _lib().shm_shared_a_keys_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def add_galois(self, steps: Sequence[int], seed: int):
+        arr = (C.c_int * len(steps))(*steps)
+        check(_lib().shm_shared_a_keys_add_galois(self.h, arr, len(steps), seed))
+
+    def aux_coeffs(self, i: int) -> np.ndarray:
+        out = np.zeros(self.ctx.N, np.int8)
+        check(_lib().shm_shared_a_keys_aux_coeffs(self.h, i, out.ctypes.data_as(C.POINTER(C.c_int8)), out.size))
+        return out
+
+    def aux_secret(self, i: int) -> SecretKey:
+        return SecretKey(self.ctx, coeffs=self.aux_coeffs(i))
+
+    def export_format(self):
+        c = self.ctx
+        qlp = c.ql(self.chain_index) + c.size_p
+        a = np.zeros(self.n * qlp * c.N, np.uint64)
+        B = np.zeros(self.n * self.n * qlp * c.N, np.uint64)
+        check(_lib().shm_shared_a_keys_export_format(self.h, _u64p(a), _u64p(B), B.size))
+        return a, B
+
+    def export_galois(self, elt: int):
+        c = self.ctx
+        A = np.zeros(c.dnum * (c.size_q + c.size_p) * c.N, np.uint64)
+        B = np.zeros(c.dnum * self.n * (c.size_q + c.size_p) * c.N, np.uint64)
+        check(_lib().shm_shared_a_keys_export_galois(self.h, elt, _u64p(A), _u64p(B), B.size))
+        return A, B
+
+    def fmt_switch_forward(self, cts: Sequence[Ciphertext], stream=None) -> SharedABatch:
+        out = SharedABatch(self.ctx, self.n, self.chain_index)
+        check(_lib().shm_fmt_switch_forward(self.h, _ptrs(cts), len(cts), out.h, stream))
+        return out
+
+    def fmt_switch_backward(self, batch: SharedABatch, stream=None) -> list:
+        out = [Ciphertext(self.ctx, 2, 1) for _ in range(batch.n)]
+        check(_lib().shm_fmt_switch_backward(self.h, batch.h, _ptrs(out), len(out), stream))
+        return out
+
+    def rotate_inplace(self, batch: SharedABatch, step: int, stream=None):
+        check(_lib().shm_shared_a_rotate_inplace(self.h, batch.h, step, stream))
+
+
+def bootstrap_shared_a_s2c_first(boot: "Bootstrapper", keys: SharedAKeys, cts: Sequence[Ciphertext],
+                                 relin_key: KSwitchKey, out_scale: float = 0.0, stream=None) -> list:
+    """BatchBootstrap.cuh:719-820: shared-a S2C-first bootstrap of exactly keys.n ciphertexts."""
+    out = [Ciphertext(keys.ctx, 2, 1) for _ in cts]
+    check(_lib().shm_bootstrap_shared_a_s2c_first(boot.h, keys.h, _ptrs(cts), len(cts), _ptrs(out), relin_key.h,
+                                                  out_scale, stream))
+    return out

@@ -118,6 +118,27 @@ _SIGS = {
     "shm_bootstrap_coeff_to_slot": (i32, [vp, vp, vp, vp, vp, vp]),
     "shm_bootstrap_eval_mod": (i32, [vp, vp, vp, vp]),
     "shm_bootstrap_s2c_first": (i32, [vp, pvp, sz, pvp, vp, vp, dbl, vp]),
+    "shm_shared_a_keys_create": (i32, [vp, vp, u32, u32, u64, u32, pvp]),
+    "shm_shared_a_keys_destroy": (i32, [vp]),
+    "shm_shared_a_keys_add_galois": (i32, [vp, C.POINTER(C.c_int), sz, u64]),
+    "shm_shared_a_keys_aux_coeffs": (i32, [vp, u32, C.POINTER(C.c_int8), sz]),
+    "shm_shared_a_keys_export_format": (i32, [vp, pu64, pu64, sz]),
+    "shm_shared_a_keys_export_galois": (i32, [vp, u32, pu64, pu64, sz]),
+    "shm_shared_a_batch_create": (i32, [vp, u32, u32, pvp]),
+    "shm_shared_a_batch_destroy": (i32, [vp]),
+    "shm_shared_a_batch_info": (i32, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(dbl)]),
+    "shm_shared_a_batch_set_scale": (i32, [vp, dbl]),
+    "shm_shared_a_batch_import": (i32, [vp, pu64, sz, vp]),
+    "shm_shared_a_batch_export": (i32, [vp, pu64, sz, vp]),
+    "shm_shared_a_batch_member": (i32, [vp, u32, vp, vp]),
+    "shm_fmt_switch_forward": (i32, [vp, pvp, sz, vp, vp]),
+    "shm_fmt_switch_backward": (i32, [vp, vp, pvp, sz, vp]),
+    "shm_shared_a_rotate_inplace": (i32, [vp, vp, i32, vp]),
+    "shm_shared_a_add_inplace": (i32, [vp, vp, vp]),
+    "shm_shared_a_multiply_plain_inplace": (i32, [vp, vp, vp]),
+    "shm_shared_a_rescale_to_next_inplace": (i32, [vp, vp]),
+    "shm_shared_a_modraise_inplace": (i32, [vp, vp]),
+    "shm_bootstrap_shared_a_s2c_first": (i32, [vp, vp, pvp, sz, pvp, vp, dbl, vp]),
 }
 
 _lib = None

@@ -15,6 +15,7 @@
 #pragma once
 
 #include <complex>
+#include <functional>
 #include <map>
 #include <vector>
 
@@ -85,15 +86,63 @@ std::unique_ptr<Bootstrapper> make_bootstrapper(Context &c, const BootParams &p)
 // Host-only plan (no context: diagonals, schedule and coefficients, no device work).
 std::unique_ptr<Bootstrapper> make_plan(size_t N, size_t nQ, const BootParams &p);
 
-// Device pipeline.
-void lt_apply(Context &c, const LtLayer &L, Ciphertext &ct, const GaloisKeys &gk, hipStream_t s);
+// Device pipeline. A RotateFn rotates a ciphertext (or shared-a batch) in place
+// by a slot step; step 0 is complex conjugation.
+using RotateFn = std::function<void(Ciphertext &, int)>;
+void lt_apply(Context &c, const LtLayer &L, Ciphertext &ct, const RotateFn &rot, hipStream_t s);
+void slot_to_coeff(Bootstrapper &b, Ciphertext &ct, const RotateFn &rot, hipStream_t s);
 void slot_to_coeff(Bootstrapper &b, Ciphertext &ct, const GaloisKeys &gk, hipStream_t s);
+void coeff_to_slot(Bootstrapper &b, Ciphertext &ct, Ciphertext &re, Ciphertext &im, const RotateFn &rot,
+                   hipStream_t s);
 void coeff_to_slot(Bootstrapper &b, Ciphertext &ct, Ciphertext &re, Ciphertext &im, const GaloisKeys &gk,
                    hipStream_t s);
 void eval_mod(Bootstrapper &b, Ciphertext &x, const KSwitchKey &rk, double out_const, double out_scale,
               hipStream_t s);
 void bootstrap_s2c_first(Bootstrapper &b, const Ciphertext &in, Ciphertext &out, const GaloisKeys &gk,
                          const KSwitchKey &rk, double out_scale, hipStream_t s);
+// ---- shared-a batches (shared_a.hip) -------------------------------------------------
+// SharedACiphertextBatch (SharedACiphertext.cuh:22-99): ct has size n + 1;
+// polynomials 0..n-1 are the b_i, polynomial n the shared a (NTT form).
+struct SharedABatch {
+    size_t n = 0;
+    Ciphertext ct;
+};
+
+// SharedAGaloisKey (ShRotate.cuh:25-35): A [dnum][QP][N]; B [dnum][n][QP][N].
+struct SharedGaloisKey {
+    uint32_t elt = 0;
+    DevBuf A, B;
+};
+
+// SharedAKeyGen + SharedAFormatKey (FmtSwitch.cuh:28-105): auxiliary secrets,
+// the forward format key at fmt_chain (QlP, local limb order), the backward
+// switching keys s'_i -> s and the shared-a Galois keys (all kept resident).
+struct SharedAKeys {
+    Context *ctx = nullptr;
+    size_t n = 0, fmt_chain = 0;
+    uint32_t hw = 0;
+    uint64_t seed = 0;
+    std::vector<SecretKey> aux;
+    std::vector<std::vector<int8_t>> aux_coeffs;
+    DevBuf a_fmt, B_fmt;                         // [n][QlP][N], [n][n][QlP][N] (index j * n + i)
+    DevBuf ext_inv, ext_inv_s, ext_cg, ext_qmodp;  // exact Ql -> P extension constants
+    std::vector<std::unique_ptr<KSwitchKey>> backward;
+    std::map<uint32_t, std::unique_ptr<SharedGaloisKey>> galois;
+};
+
+std::unique_ptr<SharedAKeys> make_shared_keys(Context &c, const SecretKey &sk, size_t n, uint32_t hw,
+                                              uint64_t seed, size_t fmt_chain, hipStream_t s);
+void add_shared_galois_key(SharedAKeys &K, int step, uint64_t seed, hipStream_t s);
+void fmt_switch_forward(SharedAKeys &K, const std::vector<const Ciphertext *> &cts, SharedABatch &out,
+                        hipStream_t s);
+void fmt_switch_backward(SharedAKeys &K, const SharedABatch &in, const std::vector<Ciphertext *> &out,
+                         hipStream_t s);
+void shared_apply_galois(SharedAKeys &K, SharedABatch &x, uint32_t elt, hipStream_t s);
+void shared_rotate(SharedAKeys &K, SharedABatch &x, int step, hipStream_t s);
+void bootstrap_shared_a_s2c_first(Bootstrapper &b, SharedAKeys &K, const std::vector<const Ciphertext *> &in,
+                                  const std::vector<Ciphertext *> &out, const KSwitchKey &rk, double out_scale,
+                                  hipStream_t s);
+
 // Multiply by the constant `value` encoded at scale `sigma` and accumulate:
 // acc (+)= src * [round(value * sigma)]_q per limb over Ql limbs of src.
 void const_lincomb(Context &c, Ciphertext &acc, const std::vector<const Ciphertext *> &src,

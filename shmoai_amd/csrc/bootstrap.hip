@@ -35,10 +35,11 @@ struct PtPack {
     const uint64_t *pt[kMaxTerms];
 };
 
-// acc[p][l][i] = sum_t x_t[p][l][i] * pt_t[l][i] mod q_l over both polynomials;
+// acc[p][l][i] = sum_t x_t[p][l][i] * pt_t[l][i] mod q_l over npolys polynomials;
 // products accumulate in 128 bits (at most 16 terms below 2^122).
-__global__ void k_lt_mac(uint64_t *acc, PtPack pk, int nterms, const DevMod *mods, size_t Ql, size_t N) {
-    const size_t S = Ql * N, total = 2 * S;
+__global__ void k_lt_mac(uint64_t *acc, PtPack pk, int nterms, const DevMod *mods, size_t Ql, size_t N,
+                         size_t npolys) {
+    const size_t S = Ql * N, total = npolys * S;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
         const size_t r = t % S;
         Acc128 a{0, 0};
@@ -108,9 +109,11 @@ void const_lincomb(Context &c, Ciphertext &acc, const std::vector<const Cipherte
 }
 
 // bsgs_linear_transform (Bootstrapper.cu:2168-2251): babies rot(x, j*base),
-// inner sums with pre-rotated diagonals, giant rotation of each inner sum.
-void lt_apply(Context &c, const LtLayer &L, Ciphertext &ct, const GaloisKeys &gk, hipStream_t s) {
-    if (ct.size != 2) throw InvalidArgument("linear transform takes size-2 ciphertexts");
+// inner sums with pre-rotated diagonals, giant rotation of each inner sum. `ct`
+// is an ordinary ciphertext or a shared-a batch (every polynomial is multiplied
+// by the same diagonal); `rot` rotates it in place.
+void lt_apply(Context &c, const LtLayer &L, Ciphertext &ct, const RotateFn &rot, hipStream_t s) {
+    if (ct.size < 2) throw InvalidArgument("linear transform takes ciphertexts of size >= 2");
     if (ct.chain != L.chain) throw InvalidArgument("linear transform applied at the wrong level");
     if (L.pts.empty()) throw LogicError("bootstrapper is not prepared");
     const size_t Ql = ct.Ql(), n = c.N / 2;
@@ -120,7 +123,7 @@ void lt_apply(Context &c, const LtLayer &L, Ciphertext &ct, const GaloisKeys &gk
         for (int i = 0; i < L.n_giants; i++) used |= !L.pre[i][j].empty();
         if (!used) continue;
         ct_clone(baby[j], ct, s);
-        rotate_internal(c, baby[j], L.baby_step(j), gk, s);
+        rot(baby[j], L.baby_step(j));
     }
     Ciphertext out;
     bool have = false;
@@ -136,13 +139,13 @@ void lt_apply(Context &c, const LtLayer &L, Ciphertext &ct, const GaloisKeys &gk
         if (!nt) continue;
         Ciphertext inner;
         inner.ctx = &c;
-        inner.resize(2, ct.chain, s);
+        inner.resize(ct.size, ct.chain, s);
         inner.ntt = true;
         inner.scale = ct.scale;
-        k_lt_mac<<<grid_for(2 * Ql * c.N), kBlock, 0, s>>>(inner.data(), pk, nt, c.mods(), Ql, c.N);
+        k_lt_mac<<<grid_for(ct.size * Ql * c.N), kBlock, 0, s>>>(inner.data(), pk, nt, c.mods(), Ql, c.N, ct.size);
         SHM_HIP(hipGetLastError());
         const long G = ((L.giant_offset(i) % (long)n) + (long)n) % (long)n;
-        if (G) rotate_internal(c, inner, (int)G, gk, s);
+        if (G) rot(inner, (int)G);
         if (!have) {
             out = std::move(inner);
             have = true;
@@ -158,22 +161,43 @@ void lt_apply(Context &c, const LtLayer &L, Ciphertext &ct, const GaloisKeys &gk
     ct = std::move(out);
 }
 
-void slot_to_coeff(Bootstrapper &b, Ciphertext &ct, const GaloisKeys &gk, hipStream_t s) {
-    for (int l = 0; l < 3; l++) lt_apply(*b.ctx, b.s2c[l], ct, gk, s);
+void slot_to_coeff(Bootstrapper &b, Ciphertext &ct, const RotateFn &rot, hipStream_t s) {
+    for (int l = 0; l < 3; l++) lt_apply(*b.ctx, b.s2c[l], ct, rot, s);
 }
 
-void coeff_to_slot(Bootstrapper &b, Ciphertext &ct, Ciphertext &re, Ciphertext &im, const GaloisKeys &gk,
+void slot_to_coeff(Bootstrapper &b, Ciphertext &ct, const GaloisKeys &gk, hipStream_t s) {
+    Context &c = *b.ctx;
+    slot_to_coeff(b, ct, [&](Ciphertext &x, int step) { rotate_internal(c, x, step, gk, s); }, s);
+}
+
+// coefftoslot_full_3 (Bootstrapper.cu:3048-3067): re = w + conj(w) and
+// im = -i (w - conj(w)) with one conjugation; -i is the exact monomial X^(3N/2).
+void coeff_to_slot(Bootstrapper &b, Ciphertext &ct, Ciphertext &re, Ciphertext &im, const RotateFn &rot,
                    hipStream_t s) {
     Context &c = *b.ctx;
-    for (int l = 0; l < 3; l++) lt_apply(c, b.c2s[l], ct, gk, s);
+    for (int l = 0; l < 3; l++) lt_apply(c, b.c2s[l], ct, rot, s);
     Ciphertext conj;
     ct_clone(conj, ct, s);
-    apply_galois(c, conj, (uint32_t)(2 * c.N - 1), gk, s);
+    rot(conj, 0);  // step 0 = complex conjugation
     ct_clone(re, ct, s);
     ct_add(c, re, conj, s);  // w + conj(w)
     ct_clone(im, ct, s);
     ct_sub(c, im, conj, s);
     ct_mul_monomial_half(c, im, true, s);  // -i (w - conj(w)) = 2 Im(w)
+}
+
+void coeff_to_slot(Bootstrapper &b, Ciphertext &ct, Ciphertext &re, Ciphertext &im, const GaloisKeys &gk,
+                   hipStream_t s) {
+    Context &c = *b.ctx;
+    coeff_to_slot(
+        b, ct, re, im,
+        [&](Ciphertext &x, int step) {
+            if (step == 0)
+                apply_galois(c, x, (uint32_t)(2 * c.N - 1), gk, s);
+            else
+                rotate_internal(c, x, step, gk, s);
+        },
+        s);
 }
 
 namespace {

@@ -329,17 +329,7 @@ int shm_secret_key_generate(shm_context *ctx, uint64_t seed, uint32_t hw, shm_se
     return guard([&] {
         need(ctx, "ctx");
         need(out, "out");
-        const size_t N = ctx->c->N;
-        if (hw == 0 || hw > N) throw InvalidArgument("hamming weight out of range");
-        // exact-weight ternary secret by partial Fisher-Yates (SplitMix64 draws)
-        std::vector<uint32_t> perm(N);
-        for (size_t i = 0; i < N; i++) perm[i] = (uint32_t)i;
-        std::vector<int8_t> coeff(N, 0);
-        for (size_t i = 0; i < hw; i++) {
-            const size_t j = i + (size_t)(host_splitmix64(seed, i) % (uint64_t)(N - i));
-            std::swap(perm[i], perm[j]);
-            coeff[perm[i]] = (host_splitmix64(seed, N + i) & 1) ? -1 : 1;
-        }
+        const std::vector<int8_t> coeff = ternary_secret(ctx->c->N, seed, hw);
         auto h = new shm_secret_key();
         h->sk.ctx = ctx->c.get();
         try {
@@ -952,6 +942,250 @@ int shm_bootstrap_s2c_first(shm_bootstrapper *b, const shm_ciphertext *const *in
         for (size_t i = 0; i < count; i++)
             boot::bootstrap_s2c_first(bs, in[i]->ct, out[i]->ct, gk->g, relin->k,
                                       out_scale > 0 ? out_scale : in[i]->ct.scale, S(st));
+        SHM_HIP(hipStreamSynchronize(S(st)));
+    });
+}
+
+// ---- shared-a batches ------------------------------------------------------------------------
+struct shm_shared_a_keys {
+    std::unique_ptr<boot::SharedAKeys> k;
+};
+struct shm_shared_a_batch {
+    boot::SharedABatch b;
+};
+
+int shm_shared_a_keys_create(shm_context *ctx, const shm_secret_key *sk, uint32_t n, uint32_t hw, uint64_t seed,
+                             uint32_t chain, shm_shared_a_keys **out) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(sk, "sk");
+        need(out, "out");
+        check_chain(*ctx->c, chain);
+        auto h = std::make_unique<shm_shared_a_keys>();
+        h->k = boot::make_shared_keys(*ctx->c, sk->sk, n, hw, seed, chain, nullptr);
+        *out = h.release();
+    });
+}
+
+int shm_shared_a_keys_destroy(shm_shared_a_keys *keys) {
+    return guard([&] {
+        if (keys) SHM_HIP(hipDeviceSynchronize());
+        delete keys;
+    });
+}
+
+int shm_shared_a_keys_add_galois(shm_shared_a_keys *keys, const int *steps, size_t n_steps, uint64_t seed) {
+    return guard([&] {
+        need(keys, "keys");
+        if (n_steps) need(steps, "steps");
+        for (size_t i = 0; i < n_steps; i++) boot::add_shared_galois_key(*keys->k, steps[i], seed, nullptr);
+    });
+}
+
+int shm_shared_a_keys_aux_coeffs(const shm_shared_a_keys *keys, uint32_t i, int8_t *out, size_t n) {
+    return guard([&] {
+        need(keys, "keys");
+        need(out, "out");
+        if (i >= keys->k->n) throw InvalidArgument("member index out of range");
+        const auto &v = keys->k->aux_coeffs[i];
+        if (n != v.size()) throw InvalidArgument("n must be N");
+        std::copy(v.begin(), v.end(), out);
+    });
+}
+
+int shm_shared_a_keys_export_format(const shm_shared_a_keys *keys, uint64_t *a_fmt, uint64_t *B, size_t nw) {
+    return guard([&] {
+        need(keys, "keys");
+        const auto &K = *keys->k;
+        const Context &c = *K.ctx;
+        const size_t QlP = c.ql_of_chain(K.fmt_chain) + c.nP, words = K.n * K.n * QlP * c.N;
+        if (nw != words) throw InvalidArgument("n_words_B must be n*n*(Ql+P)*N");
+        if (a_fmt)
+            SHM_HIP(hipMemcpy(a_fmt, K.a_fmt.as<void>(), K.n * QlP * c.N * 8, hipMemcpyDeviceToHost));
+        if (B) SHM_HIP(hipMemcpy(B, K.B_fmt.as<void>(), words * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+int shm_shared_a_keys_export_galois(const shm_shared_a_keys *keys, uint32_t elt, uint64_t *A, uint64_t *B,
+                                    size_t nw) {
+    return guard([&] {
+        need(keys, "keys");
+        const auto &K = *keys->k;
+        const Context &c = *K.ctx;
+        auto it = K.galois.find(elt);
+        if (it == K.galois.end()) throw InvalidArgument("shared-a Galois key not present");
+        const size_t words = c.dnum * K.n * c.nQP() * c.N;
+        if (nw != words) throw InvalidArgument("n_words_B must be dnum*n*QP*N");
+        if (A) SHM_HIP(hipMemcpy(A, it->second->A.as<void>(), c.dnum * c.nQP() * c.N * 8, hipMemcpyDeviceToHost));
+        if (B) SHM_HIP(hipMemcpy(B, it->second->B.as<void>(), words * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+int shm_shared_a_batch_create(shm_context *ctx, uint32_t n, uint32_t chain, shm_shared_a_batch **out) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(out, "out");
+        check_chain(*ctx->c, chain);
+        if (n == 0 || n > 16) throw InvalidArgument("shared-a batches hold 1..16 members");
+        auto h = std::make_unique<shm_shared_a_batch>();
+        h->b.n = n;
+        h->b.ct.ctx = ctx->c.get();
+        h->b.ct.resize(n + 1, chain);
+        *out = h.release();
+    });
+}
+
+int shm_shared_a_batch_destroy(shm_shared_a_batch *b) {
+    return guard([&] {
+        if (b) SHM_HIP(hipDeviceSynchronize());
+        delete b;
+    });
+}
+
+int shm_shared_a_batch_info(const shm_shared_a_batch *b, uint32_t *n, uint32_t *chain, uint32_t *cms, double *scale) {
+    return guard([&] {
+        need(b, "batch");
+        if (n) *n = (uint32_t)b->b.n;
+        if (chain) *chain = (uint32_t)b->b.ct.chain;
+        if (cms) *cms = (uint32_t)b->b.ct.Ql();
+        if (scale) *scale = b->b.ct.scale;
+    });
+}
+
+int shm_shared_a_batch_set_scale(shm_shared_a_batch *b, double scale) {
+    return guard([&] {
+        need(b, "batch");
+        b->b.ct.scale = scale;
+    });
+}
+
+int shm_shared_a_batch_import(shm_shared_a_batch *b, const uint64_t *host, size_t nw, shm_stream_t st) {
+    return guard([&] {
+        need(b, "batch");
+        need(host, "host");
+        const Ciphertext &ct = b->b.ct;
+        if (nw != ct.size * ct.Ql() * ct.ctx->N) throw InvalidArgument("n_words must be (n+1)*Ql*N");
+        SHM_HIP(hipMemcpyAsync(ct.data(), host, nw * 8, hipMemcpyHostToDevice, S(st)));
+        SHM_HIP(hipStreamSynchronize(S(st)));
+    });
+}
+
+int shm_shared_a_batch_export(const shm_shared_a_batch *b, uint64_t *host, size_t nw, shm_stream_t st) {
+    return guard([&] {
+        need(b, "batch");
+        need(host, "host");
+        const Ciphertext &ct = b->b.ct;
+        if (nw != ct.size * ct.Ql() * ct.ctx->N) throw InvalidArgument("n_words must be (n+1)*Ql*N");
+        SHM_HIP(hipMemcpyAsync(host, ct.data(), nw * 8, hipMemcpyDeviceToHost, S(st)));
+        SHM_HIP(hipStreamSynchronize(S(st)));
+    });
+}
+
+int shm_shared_a_batch_member(const shm_shared_a_batch *b, uint32_t i, shm_ciphertext *out, shm_stream_t st) {
+    return guard([&] {
+        need(b, "batch");
+        need(out, "out");
+        if (i >= b->b.n) throw InvalidArgument("SharedACiphertextBatch::assemble: index out of range");
+        const Ciphertext &ct = b->b.ct;
+        const size_t w = ct.Ql() * ct.ctx->N;
+        out->ct.ctx = ct.ctx;
+        out->ct.resize(2, ct.chain, S(st));
+        out->ct.scale = ct.scale;
+        out->ct.ntt = true;
+        SHM_HIP(hipMemcpyAsync(out->ct.poly(0), ct.poly(i), w * 8, hipMemcpyDeviceToDevice, S(st)));
+        SHM_HIP(hipMemcpyAsync(out->ct.poly(1), ct.poly(b->b.n), w * 8, hipMemcpyDeviceToDevice, S(st)));
+    });
+}
+
+int shm_fmt_switch_forward(shm_shared_a_keys *keys, const shm_ciphertext *const *cts, size_t n,
+                           shm_shared_a_batch *out, shm_stream_t st) {
+    return guard([&] {
+        need(keys, "keys");
+        need(out, "out");
+        if (n && !cts) throw InvalidArgument("cts is null");
+        std::vector<const Ciphertext *> v(n);
+        for (size_t i = 0; i < n; i++) {
+            need(cts[i], "cts[i]");
+            v[i] = &cts[i]->ct;
+        }
+        boot::fmt_switch_forward(*keys->k, v, out->b, S(st));
+    });
+}
+
+int shm_fmt_switch_backward(shm_shared_a_keys *keys, const shm_shared_a_batch *batch, shm_ciphertext *const *out,
+                            size_t n, shm_stream_t st) {
+    return guard([&] {
+        need(keys, "keys");
+        need(batch, "batch");
+        if (n && !out) throw InvalidArgument("out is null");
+        std::vector<Ciphertext *> v(n);
+        for (size_t i = 0; i < n; i++) {
+            need(out[i], "out[i]");
+            v[i] = &out[i]->ct;
+        }
+        boot::fmt_switch_backward(*keys->k, batch->b, v, S(st));
+    });
+}
+
+int shm_shared_a_rotate_inplace(shm_shared_a_keys *keys, shm_shared_a_batch *b, int step, shm_stream_t st) {
+    return guard([&] {
+        need(keys, "keys");
+        need(b, "batch");
+        boot::shared_rotate(*keys->k, b->b, step, S(st));
+    });
+}
+
+int shm_shared_a_add_inplace(shm_shared_a_batch *a, const shm_shared_a_batch *b, shm_stream_t st) {
+    return guard([&] {
+        need(a, "a");
+        need(b, "b");
+        if (a->b.n != b->b.n) throw InvalidArgument("batch sizes differ");
+        ct_add(*a->b.ct.ctx, a->b.ct, b->b.ct, S(st));
+    });
+}
+
+int shm_shared_a_multiply_plain_inplace(shm_shared_a_batch *b, const shm_plaintext *pt, shm_stream_t st) {
+    return guard([&] {
+        need(b, "batch");
+        need(pt, "pt");
+        Ciphertext &ct = b->b.ct;
+        if (pt->pt.chain != ct.chain) throw InvalidArgument("encrypted and plain parameter mismatch");
+        launch_mul_plain(*ct.ctx, ct.data(), pt->pt.data(), ct.Ql(), ct.size, S(st));
+        ct.scale *= pt->pt.scale;
+    });
+}
+
+int shm_shared_a_rescale_to_next_inplace(shm_shared_a_batch *b, shm_stream_t st) {
+    return guard([&] {
+        need(b, "batch");
+        rescale_inplace(*b->b.ct.ctx, b->b.ct, S(st));
+    });
+}
+
+int shm_shared_a_modraise_inplace(shm_shared_a_batch *b, shm_stream_t st) {
+    return guard([&] {
+        need(b, "batch");
+        ct_modraise(*b->b.ct.ctx, b->b.ct, S(st));
+    });
+}
+
+int shm_bootstrap_shared_a_s2c_first(shm_bootstrapper *b, shm_shared_a_keys *keys, const shm_ciphertext *const *in,
+                                     size_t n, shm_ciphertext *const *out, const shm_kswitch_key *relin,
+                                     double out_scale, shm_stream_t st) {
+    return guard([&] {
+        boot::Bootstrapper &bs = ready(b, S(st));
+        need(keys, "keys");
+        need(relin, "relin_keys");
+        if (n && (!in || !out)) throw InvalidArgument("in/out is null");
+        std::vector<const Ciphertext *> vi(n);
+        std::vector<Ciphertext *> vo(n);
+        for (size_t i = 0; i < n; i++) {
+            need(in[i], "in[i]");
+            need(out[i], "out[i]");
+            vi[i] = &in[i]->ct;
+            vo[i] = &out[i]->ct;
+        }
+        boot::bootstrap_shared_a_s2c_first(bs, *keys->k, vi, vo, relin->k, out_scale, S(st));
         SHM_HIP(hipStreamSynchronize(S(st)));
     });
 }

@@ -251,6 +251,22 @@ void launch_mul_elementwise(const Context &c, const uint64_t *a, const uint64_t 
 void keyswitch(const Context &c, const uint64_t *c2, const uint64_t *key, size_t Ql, uint64_t *ct0, uint64_t *ct1,
                bool add_to_ct1, hipStream_t s);
 
+// key-switching pieces (keyswitch.hip)
+constexpr int kMaxKeys = 34;
+constexpr int kMaxDigits = 16;
+struct KeyRefs {
+    const uint64_t *base[kMaxKeys];  // output o reads digit g at base[o] + g * stride[o]
+    size_t stride[kMaxKeys];
+};
+// ModUp of one Ql-limb NTT polynomial into [beta][QlP][N] (NTT form)
+void ks_modup(const Context &c, const uint64_t *c2, size_t Ql, uint64_t *d, hipStream_t s);
+// cx[o] = sum_{g < ndig} d_g * key_o(g) over QlP limbs, o < nout
+void ks_inner_multi(const Context &c, const uint64_t *d, size_t Ql, size_t ndig, const KeyRefs &kr, size_t nout,
+                    bool local, uint64_t *cx, hipStream_t s);
+// ModDown of npolys [QlP][N] polynomials into out + p * out_stride (bit p of acc_mask: accumulate)
+void ks_moddown(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, uint64_t *out, size_t out_stride,
+                uint64_t acc_mask, hipStream_t s);
+
 // encoder (encoder.hip)
 void encode(const Context &c, const double *re, const double *im, size_t n_values, size_t Ql, double scale,
             uint64_t *out, hipStream_t s);
@@ -264,6 +280,7 @@ void ctpt_matmul(const Context &c, const std::vector<const Ciphertext *> &X, con
 
 // evaluator-level operations (ops.hip)
 uint64_t host_splitmix64(uint64_t key, uint64_t ctr);
+std::vector<int8_t> ternary_secret(size_t N, uint64_t seed, uint32_t hw);
 void load_secret(Context &c, SecretKey &sk, const int8_t *coeffs);
 void gen_kswitch(Context &c, const SecretKey &sk, const uint64_t *newkey_dev, uint64_t seed, KSwitchKey &out);
 void apply_galois(Context &c, Ciphertext &ct, uint32_t elt, const GaloisKeys &gk, hipStream_t s);

@@ -77,6 +77,34 @@ __global__ void k_ks_inner(const uint64_t *__restrict__ d, const uint64_t *__res
     }
 }
 
+// cx[o][k][j] = sum_g d[g][k][j] * key_o(g)[limb(k)][j] for every output o; the
+// digits of one coefficient are read once and reused for all outputs. Key limbs
+// are addressed by their global prime (QP layout) or, with `local`, by the QlP
+// position (keys generated for one level).
+__global__ void k_ks_inner_multi(const uint64_t *__restrict__ d, KeyRefs kr, int nout, uint64_t *__restrict__ cx,
+                                 const DevMod *__restrict__ mods, size_t Ql, size_t nQ, size_t alpha, size_t ndig,
+                                 size_t N, int local) {
+    const size_t QlP = Ql + alpha, total = QlP * N;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t k = t / N, j = t % N;
+        const uint32_t pi = qlp_prime(k, Ql, nQ);
+        const size_t kl = (local ? k : (size_t)pi) * N + j;
+        uint64_t dv[kMaxDigits];  // fully unrolled: stays in registers
+#pragma unroll
+        for (int g = 0; g < kMaxDigits; g++) dv[g] = (size_t)g < ndig ? d[g * total + t] : 0;
+        const DevMod m = mods[pi];
+        for (int o = 0; o < nout; o++) {
+            Acc128 a{0, 0};
+            const uint64_t *kb = kr.base[o];
+            const size_t st = kr.stride[o];
+#pragma unroll
+            for (int g = 0; g < kMaxDigits; g++)
+                if ((size_t)g < ndig) mac128(a, dv[g], kb[g * st + kl]);
+            cx[o * total + t] = reduce128(a, m);
+        }
+    }
+}
+
 // delta[i][j] = fastbconv of the coefficient-form P limbs (cxP) to q_i.
 __global__ void k_moddown_bconv(const uint64_t *__restrict__ cxP, uint64_t *__restrict__ delta,
                                 const DevMod *__restrict__ mods, const uint64_t *__restrict__ phat_inv,
@@ -100,60 +128,87 @@ __global__ void k_moddown_bconv(const uint64_t *__restrict__ cxP, uint64_t *__re
     }
 }
 
-// out = (cx - delta) P^-1 mod q; then ct (+)= out.
+// out = (cx - delta) P^-1 mod q; then ct (+)= out. Over npolys polynomials:
+// cx [p][QlP][N], delta [p][Ql][N], ct [p] at ct + p * ct_stride; bit p of
+// acc_mask selects accumulation.
 __global__ void k_moddown_combine(const uint64_t *__restrict__ cx, const uint64_t *__restrict__ delta,
                                   uint64_t *__restrict__ ct, const DevMod *__restrict__ mods,
-                                  const uint64_t *__restrict__ pinv, const uint64_t *__restrict__ pinv_s,
-                                  size_t N, size_t total, int accumulate) {
+                                  const uint64_t *__restrict__ pinv, const uint64_t *__restrict__ pinv_s, size_t Ql,
+                                  size_t nP, size_t N, size_t npolys, size_t ct_stride, uint64_t acc_mask) {
+    const size_t S = Ql * N, total = npolys * S;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t i = t / N;
+        const size_t p = t / S, r = t % S, i = r / N;
         const uint64_t q = mods[i].q;
-        const uint64_t v = mul_shoup(sub_mod(cx[t], delta[t], q), pinv[i], pinv_s[i], q);
-        ct[t] = accumulate ? add_mod(ct[t], v, q) : v;
+        const uint64_t v = mul_shoup(sub_mod(cx[p * (S + nP * N) + r], delta[t], q), pinv[i], pinv_s[i], q);
+        uint64_t *o = ct + p * ct_stride + r;
+        *o = ((acc_mask >> p) & 1) ? add_mod(*o, v, q) : v;
     }
-}
-
-void moddown_into(const Context &c, uint64_t *cx, size_t Ql, uint64_t *ct, bool accumulate, uint64_t *delta,
-                  hipStream_t s) {
-    const size_t N = c.N, nP = c.nP, QlP = Ql + nP;
-    ntt_inverse_qlp(c, cx, Ql, Ql, QlP, 1, QlP * N, s);
-    k_moddown_bconv<<<grid_for(Ql * N), kBlock, 0, s>>>(cx + Ql * N, delta, c.mods(), c.phat_inv.as<uint64_t>(),
-                                                       c.phat_inv_s.as<uint64_t>(), c.phat_mod.as<uint64_t>(), Ql,
-                                                       c.nQ, nP, N);
-    SHM_HIP(hipGetLastError());
-    ntt_forward(c, delta, 1, Ql, 0, s);
-    k_moddown_combine<<<grid_for(Ql * N), kBlock, 0, s>>>(cx, delta, ct, c.mods(), c.pinv.as<uint64_t>(),
-                                                         c.pinv_s.as<uint64_t>(), N, Ql * N, accumulate ? 1 : 0);
-    SHM_HIP(hipGetLastError());
 }
 
 }  // namespace
 
-void keyswitch(const Context &c, const uint64_t *c2, const uint64_t *key, size_t Ql, uint64_t *ct0, uint64_t *ct1,
-               bool add_to_ct1, hipStream_t s) {
+void ks_modup(const Context &c, const uint64_t *c2, size_t Ql, uint64_t *d, hipStream_t s) {
     const size_t N = c.N, alpha = c.nP, QlP = Ql + alpha;
     const LevelConsts &L = c.level(Ql);
     const size_t beta = L.beta;
-    if (beta > c.dnum) throw LogicError("key switching key has too few digits for this level");
     Scratch tcoef(Ql * N * sizeof(uint64_t), s);
-    Scratch d(beta * QlP * N * sizeof(uint64_t), s);
-    Scratch cx(2 * QlP * N * sizeof(uint64_t), s);
-    Scratch delta(Ql * N * sizeof(uint64_t), s);
     SHM_HIP(hipMemcpyAsync(tcoef.as<uint64_t>(), c2, Ql * N * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
     ntt_inverse(c, tcoef.as<uint64_t>(), 1, Ql, 0, s);
     k_modup_bconv<<<grid_for(beta * QlP * N), kBlock, 0, s>>>(
-        c2, tcoef.as<uint64_t>(), d.as<uint64_t>(), c.mods(), L.hat_inv.as<uint64_t>(), L.hat_inv_s.as<uint64_t>(),
+        c2, tcoef.as<uint64_t>(), d, c.mods(), L.hat_inv.as<uint64_t>(), L.hat_inv_s.as<uint64_t>(),
         L.hat_mod.as<uint64_t>(), Ql, c.nQ, alpha, beta, N);
     SHM_HIP(hipGetLastError());
     for (size_t g = 0; g < beta; g++) {
         const size_t start = g * alpha, size = std::min(alpha, Ql - start);
-        ntt_forward_qlp(c, d.as<uint64_t>() + g * QlP * N, Ql, 0, QlP, start, start + size, 1, QlP * N, s);
+        ntt_forward_qlp(c, d + g * QlP * N, Ql, 0, QlP, start, start + size, 1, QlP * N, s);
     }
+}
+
+void ks_inner_multi(const Context &c, const uint64_t *d, size_t Ql, size_t ndig, const KeyRefs &kr, size_t nout,
+                    bool local, uint64_t *cx, hipStream_t s) {
+    if (nout == 0 || nout > (size_t)kMaxKeys) throw InvalidArgument("ks_inner_multi: 1..34 outputs");
+    if (ndig == 0 || ndig > (size_t)kMaxDigits) throw InvalidArgument("ks_inner_multi: 1..16 digits");
+    const size_t QlP = Ql + c.nP;
+    k_ks_inner_multi<<<grid_for(QlP * c.N), kBlock, 0, s>>>(d, kr, (int)nout, cx, c.mods(), Ql, c.nQ, c.nP, ndig,
+                                                           c.N, local ? 1 : 0);
+    SHM_HIP(hipGetLastError());
+}
+
+void ks_moddown(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, uint64_t *out, size_t out_stride,
+                uint64_t acc_mask, hipStream_t s) {
+    const size_t N = c.N, nP = c.nP, QlP = Ql + nP;
+    if (npolys == 0 || npolys > 64) throw InvalidArgument("ks_moddown: 1..64 polynomials");
+    Scratch delta(npolys * Ql * N * sizeof(uint64_t), s);
+    ntt_inverse_qlp(c, cx, Ql, Ql, QlP, npolys, QlP * N, s);
+    for (size_t p = 0; p < npolys; p++)
+        k_moddown_bconv<<<grid_for(Ql * N), kBlock, 0, s>>>(
+            cx + p * QlP * N + Ql * N, delta.as<uint64_t>() + p * Ql * N, c.mods(), c.phat_inv.as<uint64_t>(),
+            c.phat_inv_s.as<uint64_t>(), c.phat_mod.as<uint64_t>(), Ql, c.nQ, nP, N);
+    SHM_HIP(hipGetLastError());
+    ntt_forward(c, delta.as<uint64_t>(), npolys, Ql, 0, s);
+    k_moddown_combine<<<grid_for(npolys * Ql * N), kBlock, 0, s>>>(cx, delta.as<uint64_t>(), out, c.mods(),
+                                                                   c.pinv.as<uint64_t>(), c.pinv_s.as<uint64_t>(),
+                                                                   Ql, nP, N, npolys, out_stride, acc_mask);
+    SHM_HIP(hipGetLastError());
+}
+
+void keyswitch(const Context &c, const uint64_t *c2, const uint64_t *key, size_t Ql, uint64_t *ct0, uint64_t *ct1,
+               bool add_to_ct1, hipStream_t s) {
+    const size_t N = c.N, alpha = c.nP, QlP = Ql + alpha;
+    const size_t beta = c.level(Ql).beta;
+    if (beta > c.dnum) throw LogicError("key switching key has too few digits for this level");
+    Scratch d(beta * QlP * N * sizeof(uint64_t), s);
+    Scratch cx(2 * QlP * N * sizeof(uint64_t), s);
+    ks_modup(c, c2, Ql, d.as<uint64_t>(), s);
     k_ks_inner<<<grid_for(QlP * N), kBlock, 0, s>>>(d.as<uint64_t>(), key, cx.as<uint64_t>(), c.mods(), Ql, c.nQ,
                                                     c.nQP(), alpha, beta, N);
     SHM_HIP(hipGetLastError());
-    moddown_into(c, cx.as<uint64_t>(), Ql, ct0, true, delta.as<uint64_t>(), s);
-    moddown_into(c, cx.as<uint64_t>() + QlP * N, Ql, ct1, add_to_ct1, delta.as<uint64_t>(), s);
+    if (ct1 == ct0 + Ql * N) {
+        ks_moddown(c, cx.as<uint64_t>(), 2, Ql, ct0, Ql * N, add_to_ct1 ? 3 : 1, s);
+    } else {
+        ks_moddown(c, cx.as<uint64_t>(), 1, Ql, ct0, Ql * N, 1, s);
+        ks_moddown(c, cx.as<uint64_t>() + QlP * N, 1, Ql, ct1, Ql * N, add_to_ct1 ? 1 : 0, s);
+    }
 }
 
 }  // namespace shm

@@ -15,6 +15,21 @@ uint64_t host_splitmix64(uint64_t key, uint64_t ctr) {
     return z ^ (z >> 31);
 }
 
+// Exact-weight ternary secret by a partial Fisher-Yates shuffle of SplitMix64
+// draws: position draw i from counter i, sign from counter N + i.
+std::vector<int8_t> ternary_secret(size_t N, uint64_t seed, uint32_t hw) {
+    if (hw == 0 || hw > N) throw InvalidArgument("hamming weight out of range");
+    std::vector<uint32_t> perm(N);
+    for (size_t i = 0; i < N; i++) perm[i] = (uint32_t)i;
+    std::vector<int8_t> coeff(N, 0);
+    for (size_t i = 0; i < hw; i++) {
+        const size_t j = i + (size_t)(host_splitmix64(seed, i) % (uint64_t)(N - i));
+        std::swap(perm[i], perm[j]);
+        coeff[perm[i]] = (host_splitmix64(seed, N + i) & 1) ? -1 : 1;
+    }
+    return coeff;
+}
+
 // NTT-form secret over QP from signed coefficients (host int8 -> device).
 void load_secret(Context &c, SecretKey &sk, const int8_t *coeffs) {
     const size_t N = c.N, QP = c.nQP();
@@ -236,21 +251,23 @@ void ct_mul_monomial_half(Context &c, Ciphertext &a, bool negate, hipStream_t s)
 }
 
 // Bootstrapper.cu:3287-3356 modraise_inplace: bottom-level (1 prime) ciphertext
-// lifted to all data primes; the result is in NTT form at chain index 1.
+// lifted to all data primes; the result is in NTT form at chain index 1. Every
+// polynomial is lifted, so a shared-a batch (SharedABatchOps.cu modraise) goes
+// through the same path.
 void ct_modraise(Context &c, Ciphertext &a, hipStream_t s) {
-    if (a.size != 2) throw InvalidArgument("Ciphertexts of size 2 are supported only!");
+    if (a.size < 2) throw InvalidArgument("Ciphertexts of size 2 are supported only!");
     if (a.Ql() != 1) throw InvalidArgument("Ciphertexts in the lowest level are supported only!");
-    const size_t N = c.N, nl = c.nQ;
-    Scratch low(2 * N * sizeof(uint64_t), s);
-    SHM_HIP(hipMemcpyAsync(low.as<uint64_t>(), a.data(), 2 * N * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
-    ntt_inverse(c, low.as<uint64_t>(), 2, 1, 0, s);
+    const size_t N = c.N, nl = c.nQ, np = a.size;
+    Scratch low(np * N * sizeof(uint64_t), s);
+    SHM_HIP(hipMemcpyAsync(low.as<uint64_t>(), a.data(), np * N * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    ntt_inverse(c, low.as<uint64_t>(), np, 1, 0, s);
     Ciphertext out;
     out.ctx = &c;
-    out.resize(2, 1, s);
-    for (size_t p = 0; p < 2; p++)
+    out.resize(np, 1, s);
+    for (size_t p = 0; p < np; p++)
         k_modraise<<<grid_ops(nl * N), kBlockOps, 0, s>>>(low.as<uint64_t>() + p * N, out.poly(p), c.mods(), nl, N);
     SHM_HIP(hipGetLastError());
-    ntt_forward(c, out.data(), 2, nl, 0, s);
+    ntt_forward(c, out.data(), np, nl, 0, s);
     out.scale = a.scale;
     a = std::move(out);
 }
