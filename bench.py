@@ -11,6 +11,12 @@ Prints one JSON line: value = amortised seconds per input (step time / 256
 inputs), plus the roofline of the dominant kernel (the INT8-MFMA contraction,
 timed with HIP events on its own stream) and a CPU baseline (the oracle, on a
 bounded sample of the same workload, extrapolated).
+
+The line also carries `bootstrap_768`: S2C-first bootstrapping at the reference
+parameters (N = 2^16, Q = 1 x 51 + 20 x 46 + 14 x 51 bits, P = 5 x 51 bits,
+HW 192, inputs with 4 primes, slots U(-1, 1)), timed on a bounded number of
+ciphertexts and extrapolated to 768: ordinary (one ciphertext at a time) and
+shared-a batches of n = 4, with the decryption error of the timed outputs.
 """
 from __future__ import annotations
 
@@ -67,6 +73,57 @@ def cpu_baseline(xs, W, threads: int):
             "seconds_per_full_matmul": per_matmul}
 
 
+BOOT_BITS = [51] + [46] * 20 + [51] * 14 + [51] * 5
+
+
+def bootstrap_bench(shm, n_ord: int = 8, n_shared: int = 4, chunks: int = 2):
+    """Seconds per ciphertext of ordinary and shared-a S2C-first bootstrapping."""
+    ctx = shm.Context(16, BOOT_BITS, 5)
+    sk = shm.SecretKey(ctx, seed=1, hamming_weight=192)
+    boot = shm.Bootstrapper(ctx)
+    t0 = time.perf_counter()
+    steps = boot.rotation_steps()
+    gk = sk.create_galois_keys_from_steps(steps, seed=2)
+    rk = sk.gen_relinkey(seed=3)
+    boot.prepare()
+    keys = shm.SharedAKeys(ctx, sk, n_shared, 192, 4, ctx.chain(4))
+    keys.add_galois(steps, seed=5)
+    setup_s = time.perf_counter() - t0
+    log(f"bench: bootstrap keys + diagonals ready in {setup_s:.1f} s")
+    n = ctx.N // 2
+    rng = np.random.default_rng(7)
+    total = max(n_ord, n_shared * chunks)
+    zs = [rng.uniform(-1, 1, n) for _ in range(total)]
+    enc, dec = shm.Encoder(ctx), shm.Decryptor(ctx, sk)
+    cts = [sk.encrypt_symmetric(enc.encode(z, 2.0 ** 46, ctx.chain(4)), seed=100 + i) for i, z in enumerate(zs)]
+    boot.bootstrap_s2c_first(cts[:1], gk, rk)  # warm-up
+    t0 = time.perf_counter()
+    out = boot.bootstrap_s2c_first(cts[:n_ord], gk, rk)  # synchronises
+    ord_s = (time.perf_counter() - t0) / n_ord
+    err_ord = max(float(np.max(np.abs(enc.decode(dec.decrypt(o), complex_out=True) - z)))
+                  for o, z in zip(out[:2], zs[:2]))
+    log(f"bench: ordinary bootstrap {ord_s * 1e3:.1f} ms/ct, max err {err_ord:.2e}")
+    shm.bootstrap_shared_a_s2c_first(boot, keys, cts[:n_shared], rk)  # warm-up
+    t0 = time.perf_counter()
+    outs = []
+    for c in range(chunks):
+        outs += shm.bootstrap_shared_a_s2c_first(boot, keys, cts[c * n_shared:(c + 1) * n_shared], rk)
+    sh_s = (time.perf_counter() - t0) / (n_shared * chunks)
+    err_sh = max(float(np.max(np.abs(enc.decode(dec.decrypt(o), complex_out=True) - z)))
+                 for o, z in zip(outs[:2], zs[:2]))
+    log(f"bench: shared-a n={n_shared} bootstrap {sh_s * 1e3:.1f} ms/ct, max err {err_sh:.2e}")
+    return {"params": "N=2^16, logn=15, Q=1x51+20x46+14x51, P=5x51 (dnum 7), HW 192, scale 2^46, "
+                      "S2C-first, EvalMod deg-63 Chebyshev + 2 double angles, output 22 primes",
+            "ordinary_s_per_ct": round(ord_s, 5), "ordinary_768_s_extrapolated": round(ord_s * 768, 2),
+            f"shared_a_n{n_shared}_s_per_ct": round(sh_s, 5),
+            f"shared_a_n{n_shared}_768_s_extrapolated": round(sh_s * 768, 2),
+            "batched_speedup_vs_sequential": round(ord_s / sh_s, 3),
+            "max_abs_err_ordinary": err_ord, "max_abs_err_shared_a": err_sh,
+            "keys_resident_setup_s": round(setup_s, 2),
+            "sample": f"{n_ord} ordinary ciphertexts and {chunks} shared-a chunks of {n_shared}, "
+                      "wall clock incl. host overhead, extrapolated to 768"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +132,7 @@ def main() -> int:
     ap.add_argument("--variant", choices=["int8", "u64"], default="int8")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-off per-ciphertext form timing")
+    ap.add_argument("--no-bootstrap", action="store_true", help="skip the bootstrap timing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one device-to-device copy of a 24 MiB ciphertext (profiler byte-counter calibration)")
     args = ap.parse_args()
@@ -135,6 +193,10 @@ def main() -> int:
                  "batched_speedup_vs_plain": round(plain_ms / ms_per_step, 2)}
         log(f"bench: plain per-ciphertext form {plain_ms:.1f} ms")
 
+    boot = None
+    if not args.no_bootstrap:
+        boot = bootstrap_bench(shm)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -159,6 +221,7 @@ def main() -> int:
                    "variant": args.variant, "inputs_per_step": INPUTS},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "bootstrap_768": boot,
         **extra,
     }
     print(json.dumps(line), flush=True)

@@ -133,12 +133,13 @@ struct SharedAKeys {
 std::unique_ptr<SharedAKeys> make_shared_keys(Context &c, const SecretKey &sk, size_t n, uint32_t hw,
                                               uint64_t seed, size_t fmt_chain, hipStream_t s);
 void add_shared_galois_key(SharedAKeys &K, int step, uint64_t seed, hipStream_t s);
+// `rounded`: unbiased ModDown (used inside bootstrapping); default = reference semantics
 void fmt_switch_forward(SharedAKeys &K, const std::vector<const Ciphertext *> &cts, SharedABatch &out,
-                        hipStream_t s);
+                        hipStream_t s, bool rounded = false);
 void fmt_switch_backward(SharedAKeys &K, const SharedABatch &in, const std::vector<Ciphertext *> &out,
-                         hipStream_t s);
-void shared_apply_galois(SharedAKeys &K, SharedABatch &x, uint32_t elt, hipStream_t s);
-void shared_rotate(SharedAKeys &K, SharedABatch &x, int step, hipStream_t s);
+                         hipStream_t s, bool rounded = false);
+void shared_apply_galois(SharedAKeys &K, SharedABatch &x, uint32_t elt, hipStream_t s, bool rounded = false);
+void shared_rotate(SharedAKeys &K, SharedABatch &x, int step, hipStream_t s, bool rounded = false);
 void bootstrap_shared_a_s2c_first(Bootstrapper &b, SharedAKeys &K, const std::vector<const Ciphertext *> &in,
                                   const std::vector<Ciphertext *> &out, const KSwitchKey &rk, double out_scale,
                                   hipStream_t s);

@@ -81,9 +81,9 @@ void add_checked(Context &c, Ciphertext &a, const Ciphertext &b, hipStream_t s) 
 // onto the other one.
 void add_aligned(Context &c, Ciphertext &a, Ciphertext &b, hipStream_t s) {
     if (a.chain < b.chain)
-        ct_align(c, a, b.chain, b.scale, s);
+        ct_align(c, a, b.chain, b.scale, s, true);
     else if (b.chain < a.chain)
-        ct_align(c, b, a.chain, a.scale, s);
+        ct_align(c, b, a.chain, a.scale, s, true);
     add_checked(c, a, b, s);
 }
 
@@ -156,7 +156,7 @@ void lt_apply(Context &c, const LtLayer &L, Ciphertext &ct, const RotateFn &rot,
     if (!have) throw LogicError("empty linear transform");
     const double sc = ct.scale;
     out.scale = sc * (double)c.q[Ql - 1];
-    rescale_inplace(c, out, s);
+    rescale_inplace(c, out, s, true);
     out.scale = sc;  // diagonals carry exactly the scale q_last
     ct = std::move(out);
 }
@@ -167,7 +167,7 @@ void slot_to_coeff(Bootstrapper &b, Ciphertext &ct, const RotateFn &rot, hipStre
 
 void slot_to_coeff(Bootstrapper &b, Ciphertext &ct, const GaloisKeys &gk, hipStream_t s) {
     Context &c = *b.ctx;
-    slot_to_coeff(b, ct, [&](Ciphertext &x, int step) { rotate_internal(c, x, step, gk, s); }, s);
+    slot_to_coeff(b, ct, [&](Ciphertext &x, int step) { rotate_internal(c, x, step, gk, s, true); }, s);
 }
 
 // coefftoslot_full_3 (Bootstrapper.cu:3048-3067): re = w + conj(w) and
@@ -193,9 +193,9 @@ void coeff_to_slot(Bootstrapper &b, Ciphertext &ct, Ciphertext &re, Ciphertext &
         b, ct, re, im,
         [&](Ciphertext &x, int step) {
             if (step == 0)
-                apply_galois(c, x, (uint32_t)(2 * c.N - 1), gk, s);
+                apply_galois(c, x, (uint32_t)(2 * c.N - 1), gk, s, true);
             else
-                rotate_internal(c, x, step, gk, s);
+                rotate_internal(c, x, step, gk, s, true);
         },
         s);
 }
@@ -232,14 +232,14 @@ struct ChebEval {
         Ciphertext &ta = get(a);
         Ciphertext &tb = get(b);
         auto out = std::make_unique<Ciphertext>();
-        ct_mul_relin_rescale(c, *out, ta, tb, rk, s);
+        ct_mul_relin_rescale(c, *out, ta, tb, rk, s, true);
         ct_add(c, *out, *out, s);
         if (a == b) {
             ct_add_const(c, *out, -1.0, s);
         } else {
             Ciphertext tmp;
             ct_clone(tmp, get(a - b), s);
-            ct_align(c, tmp, out->chain, out->scale, s);
+            ct_align(c, tmp, out->chain, out->scale, s, true);
             ct_sub(c, *out, tmp, s);
         }
         if (out->chain != tchain(k)) throw LogicError("Chebyshev basis level bookkeeping");
@@ -282,7 +282,7 @@ struct ChebEval {
         if (src.empty()) throw LogicError("constant-only Chebyshev block");
         const_lincomb(c, out, src, vals, sig, Qc, s);
         out.scale = scale * ql;
-        rescale_inplace(c, out, s);
+        rescale_inplace(c, out, s, true);
         out.scale = scale;
         if (p[0] != 0.0) ct_add_const(c, out, p[0], s);
     }
@@ -305,7 +305,7 @@ struct ChebEval {
         Ciphertext &Tm = get((int)m);
         Ciphertext R;
         eval(r, target * qp / Tm.scale, R);
-        ct_mul_relin_rescale(c, out, Tm, R, rk, s);
+        ct_mul_relin_rescale(c, out, Tm, R, rk, s, true);
         out.scale = target;
         bool q_terms = false;
         for (size_t k = 1; k < q.size(); k++) q_terms |= q[k] != 0.0;
@@ -318,7 +318,7 @@ struct ChebEval {
             baby(q, out.chain, target, Q);
         } else {
             eval(q, target, Q);
-            if (Q.chain < out.chain) ct_align(c, Q, out.chain, target, s);
+            if (Q.chain < out.chain) ct_align(c, Q, out.chain, target, s, true);
         }
         add_checked(c, out, Q, s);
     }
@@ -341,7 +341,7 @@ void eval_mod(Bootstrapper &b, Ciphertext &x, const KSwitchKey &rk, double out_c
     Ciphertext t1;
     ct_clone(t1, x, s);
     const double s1 = x.scale;
-    ct_mul_const_rescale(c, t1, 1.0 / (double)b.prm.K, s1 * (double)c.q[t1.Ql() - 1] / t1.scale, s);
+    ct_mul_const_rescale(c, t1, 1.0 / (double)b.prm.K, s1 * (double)c.q[t1.Ql() - 1] / t1.scale, s, true);
     t1.scale = s1;
     ChebEval ev(c, rk, s, std::move(t1));
     std::vector<double> coeffs = b.cheb;
@@ -360,7 +360,7 @@ void eval_mod(Bootstrapper &b, Ciphertext &x, const KSwitchKey &rk, double out_c
     if (out.chain != cp) throw LogicError("EvalMod level bookkeeping");
     for (int j = 0; j < r; j++) {
         Ciphertext sq;
-        ct_mul_relin_rescale(c, sq, out, out, rk, s);
+        ct_mul_relin_rescale(c, sq, out, out, rk, s, true);
         const double cst = j == r - 1 ? out_const : 1.0;
         sq.scale = sig[j + 1];  // = sig[j]^2 / (2 cst q_j): the square read as 2 cst c^2
         ct_add_const(c, sq, -cst, s);

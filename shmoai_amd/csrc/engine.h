@@ -216,9 +216,10 @@ void ntt_inverse_qlp(const Context &c, uint64_t *data, size_t Ql, size_t lo, siz
 // Rescale tail fused into one forward NTT: limb k (< L1) of polynomial p is
 // NTT([last_p]_{q_k}); the block pass stores (acc[p][k] - that) * qinv_k to
 // outs[p / ppo] (or out_single) at ((p % ppo) * L1 + k) * N.
+// rounded: divide with rounding to nearest (centred last limb) instead of the floor.
 void rescale_ntt_fused(const Context &c, const uint64_t *last, const uint64_t *acc, size_t acc_limbs,
                        uint64_t *delta, uint64_t *const *outs, uint64_t *out_single, size_t ppo, size_t n_polys,
-                       size_t L1, const uint64_t *qinv, const uint64_t *qinv_s, hipStream_t s);
+                       size_t L1, const uint64_t *qinv, const uint64_t *qinv_s, hipStream_t s, bool rounded = false);
 
 // elementwise (poly.hip)
 void launch_add(const Context &c, uint64_t *a, const uint64_t *b, size_t Ql, size_t npolys, hipStream_t s);
@@ -227,7 +228,8 @@ void launch_negate(const Context &c, uint64_t *a, size_t Ql, size_t npolys, hipS
 void launch_mul_plain(const Context &c, uint64_t *ct, const uint64_t *pt, size_t Ql, size_t npolys, hipStream_t s);
 void launch_tensor(const Context &c, const uint64_t *a, const uint64_t *b, uint64_t *out3, size_t Ql, hipStream_t s);
 void launch_square(const Context &c, const uint64_t *a, uint64_t *out3, size_t Ql, hipStream_t s);
-void launch_rescale(const Context &c, const uint64_t *in, uint64_t *out, size_t Ql, size_t npolys, hipStream_t s);
+void launch_rescale(const Context &c, const uint64_t *in, uint64_t *out, size_t Ql, size_t npolys, hipStream_t s,
+                    bool rounded = false);
 void launch_permute(const Context &c, const uint64_t *in, uint64_t *out, const uint32_t *table, size_t nlimbs,
                     hipStream_t s);
 void launch_copy_limbs(const Context &c, const uint64_t *in, size_t in_limbs, uint64_t *out, size_t out_limbs,
@@ -248,8 +250,10 @@ void launch_mul_elementwise(const Context &c, const uint64_t *a, const uint64_t 
                             bool withP, size_t Ql, hipStream_t s);
 
 // key switching (keyswitch.hip)
+// rounded: ModDown rounds to nearest with an exact base conversion (unbiased) instead
+// of the reference's floor with the approximate conversion.
 void keyswitch(const Context &c, const uint64_t *c2, const uint64_t *key, size_t Ql, uint64_t *ct0, uint64_t *ct1,
-               bool add_to_ct1, hipStream_t s);
+               bool add_to_ct1, hipStream_t s, bool rounded = false);
 
 // key-switching pieces (keyswitch.hip)
 constexpr int kMaxKeys = 34;
@@ -265,7 +269,7 @@ void ks_inner_multi(const Context &c, const uint64_t *d, size_t Ql, size_t ndig,
                     bool local, uint64_t *cx, hipStream_t s);
 // ModDown of npolys [QlP][N] polynomials into out + p * out_stride (bit p of acc_mask: accumulate)
 void ks_moddown(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, uint64_t *out, size_t out_stride,
-                uint64_t acc_mask, hipStream_t s);
+                uint64_t acc_mask, hipStream_t s, bool rounded = false);
 
 // encoder (encoder.hip)
 void encode(const Context &c, const double *re, const double *im, size_t n_values, size_t Ql, double scale,
@@ -283,18 +287,20 @@ uint64_t host_splitmix64(uint64_t key, uint64_t ctr);
 std::vector<int8_t> ternary_secret(size_t N, uint64_t seed, uint32_t hw);
 void load_secret(Context &c, SecretKey &sk, const int8_t *coeffs);
 void gen_kswitch(Context &c, const SecretKey &sk, const uint64_t *newkey_dev, uint64_t seed, KSwitchKey &out);
-void apply_galois(Context &c, Ciphertext &ct, uint32_t elt, const GaloisKeys &gk, hipStream_t s);
-void rotate_internal(Context &c, Ciphertext &ct, int step, const GaloisKeys &gk, hipStream_t s);
-void rescale_inplace(Context &c, Ciphertext &ct, hipStream_t s);
+// `rounded` selects the unbiased (round-to-nearest) rescale / ModDown used inside
+// bootstrapping; the default is the reference's floor semantics.
+void apply_galois(Context &c, Ciphertext &ct, uint32_t elt, const GaloisKeys &gk, hipStream_t s, bool rounded = false);
+void rotate_internal(Context &c, Ciphertext &ct, int step, const GaloisKeys &gk, hipStream_t s, bool rounded = false);
+void rescale_inplace(Context &c, Ciphertext &ct, hipStream_t s, bool rounded = false);
 void mod_switch_drop(Context &c, Ciphertext &ct, size_t new_chain, hipStream_t s);
 void ct_clone(Ciphertext &dst, const Ciphertext &src, hipStream_t s);
 void ct_add(Context &c, Ciphertext &a, const Ciphertext &b, hipStream_t s);
 void ct_sub(Context &c, Ciphertext &a, const Ciphertext &b, hipStream_t s);
 void ct_add_const(Context &c, Ciphertext &a, double v, hipStream_t s);
-void ct_mul_const_rescale(Context &c, Ciphertext &a, double v, double sigma, hipStream_t s);
-void ct_align(Context &c, Ciphertext &a, size_t chain, double scale, hipStream_t s);
+void ct_mul_const_rescale(Context &c, Ciphertext &a, double v, double sigma, hipStream_t s, bool rounded = false);
+void ct_align(Context &c, Ciphertext &a, size_t chain, double scale, hipStream_t s, bool rounded = false);
 void ct_mul_relin_rescale(Context &c, Ciphertext &out, const Ciphertext &a, const Ciphertext &b,
-                          const KSwitchKey &rk, hipStream_t s);
+                          const KSwitchKey &rk, hipStream_t s, bool rounded = false);
 void ct_mul_monomial_half(Context &c, Ciphertext &a, bool negate, hipStream_t s);
 void ct_modraise(Context &c, Ciphertext &a, hipStream_t s);
 

@@ -105,25 +105,39 @@ __global__ void k_ks_inner_multi(const uint64_t *__restrict__ d, KeyRefs kr, int
     }
 }
 
-// delta[i][j] = fastbconv of the coefficient-form P limbs (cxP) to q_i.
+// delta[i][j] = fastbconv of the coefficient-form P limbs (cxP) to q_i. With
+// `rounded`, the conversion is exact and centred: it subtracts
+// round(sum_j y_j / p_j) P, so delta is the representative of cx mod P in
+// (-P/2, P/2] and (cx - delta) / P rounds to nearest.
 __global__ void k_moddown_bconv(const uint64_t *__restrict__ cxP, uint64_t *__restrict__ delta,
                                 const DevMod *__restrict__ mods, const uint64_t *__restrict__ phat_inv,
                                 const uint64_t *__restrict__ phat_inv_s, const uint64_t *__restrict__ phat_mod,
-                                size_t Ql, size_t nQ, size_t nP, size_t N) {
+                                const uint64_t *__restrict__ pmod, size_t Ql, size_t nQ, size_t nP, size_t N,
+                                int rounded) {
     const size_t total = Ql * N;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
         const size_t i = t / N, j = t % N;
         const DevMod mq = mods[i];
         if (nP == 1) {
-            delta[t] = reduce64(cxP[j], mq);
+            const uint64_t x = cxP[j];
+            uint64_t d = reduce64(x, mq);
+            if (rounded && x > (mods[nQ].q >> 1)) d = sub_mod(d, reduce64(mods[nQ].q, mq), mq.q);
+            delta[t] = d;
         } else {
             Acc128 acc{0, 0};
+            double v = 0.0;
             for (size_t jj = 0; jj < nP; jj++) {
                 const uint64_t p = mods[nQ + jj].q;
                 const uint64_t x = mul_shoup(cxP[jj * N + j], phat_inv[jj], phat_inv_s[jj], p);
                 mac128(acc, x, phat_mod[i * nP + jj]);
+                if (rounded) v = __dadd_rn(v, __ddiv_rn((double)x, (double)p));
             }
-            delta[t] = reduce128(acc, mq);
+            uint64_t d = reduce128(acc, mq);
+            if (rounded) {
+                const uint64_t r = (uint64_t)__dadd_rn(v, 0.5);  // round(sum y_j / p_j)
+                d = sub_mod(d, mul_mod(r, pmod[i], mq), mq.q);
+            }
+            delta[t] = d;
         }
     }
 }
@@ -175,7 +189,7 @@ void ks_inner_multi(const Context &c, const uint64_t *d, size_t Ql, size_t ndig,
 }
 
 void ks_moddown(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, uint64_t *out, size_t out_stride,
-                uint64_t acc_mask, hipStream_t s) {
+                uint64_t acc_mask, hipStream_t s, bool rounded) {
     const size_t N = c.N, nP = c.nP, QlP = Ql + nP;
     if (npolys == 0 || npolys > 64) throw InvalidArgument("ks_moddown: 1..64 polynomials");
     Scratch delta(npolys * Ql * N * sizeof(uint64_t), s);
@@ -183,7 +197,8 @@ void ks_moddown(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, uint64
     for (size_t p = 0; p < npolys; p++)
         k_moddown_bconv<<<grid_for(Ql * N), kBlock, 0, s>>>(
             cx + p * QlP * N + Ql * N, delta.as<uint64_t>() + p * Ql * N, c.mods(), c.phat_inv.as<uint64_t>(),
-            c.phat_inv_s.as<uint64_t>(), c.phat_mod.as<uint64_t>(), Ql, c.nQ, nP, N);
+            c.phat_inv_s.as<uint64_t>(), c.phat_mod.as<uint64_t>(), c.pmod.as<uint64_t>(), Ql, c.nQ, nP, N,
+            rounded ? 1 : 0);
     SHM_HIP(hipGetLastError());
     ntt_forward(c, delta.as<uint64_t>(), npolys, Ql, 0, s);
     k_moddown_combine<<<grid_for(npolys * Ql * N), kBlock, 0, s>>>(cx, delta.as<uint64_t>(), out, c.mods(),
@@ -193,7 +208,7 @@ void ks_moddown(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, uint64
 }
 
 void keyswitch(const Context &c, const uint64_t *c2, const uint64_t *key, size_t Ql, uint64_t *ct0, uint64_t *ct1,
-               bool add_to_ct1, hipStream_t s) {
+               bool add_to_ct1, hipStream_t s, bool rounded) {
     const size_t N = c.N, alpha = c.nP, QlP = Ql + alpha;
     const size_t beta = c.level(Ql).beta;
     if (beta > c.dnum) throw LogicError("key switching key has too few digits for this level");
@@ -204,10 +219,10 @@ void keyswitch(const Context &c, const uint64_t *c2, const uint64_t *key, size_t
                                                     c.nQP(), alpha, beta, N);
     SHM_HIP(hipGetLastError());
     if (ct1 == ct0 + Ql * N) {
-        ks_moddown(c, cx.as<uint64_t>(), 2, Ql, ct0, Ql * N, add_to_ct1 ? 3 : 1, s);
+        ks_moddown(c, cx.as<uint64_t>(), 2, Ql, ct0, Ql * N, add_to_ct1 ? 3 : 1, s, rounded);
     } else {
-        ks_moddown(c, cx.as<uint64_t>(), 1, Ql, ct0, Ql * N, 1, s);
-        ks_moddown(c, cx.as<uint64_t>() + QlP * N, 1, Ql, ct1, Ql * N, add_to_ct1 ? 1 : 0, s);
+        ks_moddown(c, cx.as<uint64_t>(), 1, Ql, ct0, Ql * N, 1, s, rounded);
+        ks_moddown(c, cx.as<uint64_t>() + QlP * N, 1, Ql, ct1, Ql * N, add_to_ct1 ? 1 : 0, s, rounded);
     }
 }
 

@@ -185,6 +185,7 @@ struct NttIO {
     const uint64_t *qinv_s;
     uint32_t acc_limbs;
     uint32_t ppo;
+    uint64_t center;          // IO_REDUCE: 0, or h = q_last >> 1: values above h stand for v - q_last
 };
 
 // Phase A (columns, first log N1 stages). LOGT = log N1.
@@ -217,7 +218,10 @@ __global__ __launch_bounds__(kThreads) void k_ntt_cols(uint64_t *__restrict__ da
         const uint64_t *src = io.src + (size_t)poly * N + c0;
         for (int i = threadIdx.x; i < kTile; i += kThreads) {
             const int c = i % COLS, r = i / COLS;
-            lds[c * TP + r] = reduce64(src[(size_t)r * N2 + c], m);
+            const uint64_t v = src[(size_t)r * N2 + c];
+            uint64_t x = reduce64(v, m);
+            if (io.center && v > io.center) x = sub_mod(x, reduce64(2 * io.center + 1, m), m.q);
+            lds[c * TP + r] = x;
         }
     } else {
         for (int i = threadIdx.x; i < kTile; i += kThreads) {
@@ -392,9 +396,10 @@ void ntt_inverse_qlp(const Context &c, uint64_t *data, size_t Ql, size_t lo, siz
 
 void rescale_ntt_fused(const Context &c, const uint64_t *last, const uint64_t *acc, size_t acc_limbs,
                        uint64_t *delta, uint64_t *const *outs, uint64_t *out_single, size_t ppo, size_t n_polys,
-                       size_t L1, const uint64_t *qinv, const uint64_t *qinv_s, hipStream_t s) {
+                       size_t L1, const uint64_t *qinv, const uint64_t *qinv_s, hipStream_t s, bool rounded) {
     LimbMap mp{(uint32_t)L1, 0, 0xffffffffu, 0, 0xffffffffu, (uint32_t)c.nQ, 0, (uint64_t)L1 * c.N};
-    NttIO io{last, acc, outs, out_single, qinv, qinv_s, (uint32_t)acc_limbs, (uint32_t)ppo};
+    NttIO io{last, acc, outs, out_single, qinv, qinv_s, (uint32_t)acc_limbs, (uint32_t)ppo,
+             rounded ? (c.q[L1] >> 1) : 0};
     launch_ntt<true, IO_REDUCE, IO_RESCALE>(c, delta, mp, n_polys * L1, s, io);
 }
 

@@ -57,7 +57,7 @@ void gen_kswitch(Context &c, const SecretKey &sk, const uint64_t *newkey_dev, ui
     SHM_HIP(hipDeviceSynchronize());
 }
 
-void apply_galois(Context &c, Ciphertext &ct, uint32_t elt, const GaloisKeys &gk, hipStream_t s) {
+void apply_galois(Context &c, Ciphertext &ct, uint32_t elt, const GaloisKeys &gk, hipStream_t s, bool rounded) {
     if (ct.size != 2) throw InvalidArgument("encrypted size must be 2");
     auto it = gk.keys.find(elt);
     if (it == gk.keys.end()) throw InvalidArgument("Galois elt not present");
@@ -67,15 +67,15 @@ void apply_galois(Context &c, Ciphertext &ct, uint32_t elt, const GaloisKeys &gk
     launch_permute(c, ct.poly(0), t0.as<uint64_t>(), table, Ql, s);
     launch_permute(c, ct.poly(1), t1.as<uint64_t>(), table, Ql, s);
     SHM_HIP(hipMemcpyAsync(ct.poly(0), t0.as<uint64_t>(), Ql * N * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
-    keyswitch(c, t1.as<uint64_t>(), it->second->k.as<uint64_t>(), Ql, ct.poly(0), ct.poly(1), false, s);
+    keyswitch(c, t1.as<uint64_t>(), it->second->k.as<uint64_t>(), Ql, ct.poly(0), ct.poly(1), false, s, rounded);
 }
 
 // evaluate.cu:1650-1706 rotate_internal (NAF split, numth.h:17-34).
-void rotate_internal(Context &c, Ciphertext &ct, int step, const GaloisKeys &gk, hipStream_t s) {
+void rotate_internal(Context &c, Ciphertext &ct, int step, const GaloisKeys &gk, hipStream_t s, bool rounded) {
     if (step == 0) return;
     const uint32_t elt = c.elt_from_step(step);
     if (gk.keys.count(elt)) {
-        apply_galois(c, ct, elt, gk, s);
+        apply_galois(c, ct, elt, gk, s, rounded);
         return;
     }
     std::vector<int> naf;
@@ -90,14 +90,14 @@ void rotate_internal(Context &c, Ciphertext &ct, int step, const GaloisKeys &gk,
     }
     if (naf.size() == 1) throw InvalidArgument("Galois key not present");
     for (int t : naf)
-        if ((size_t)std::abs(t) != (c.N >> 1)) rotate_internal(c, ct, t, gk, s);
+        if ((size_t)std::abs(t) != (c.N >> 1)) rotate_internal(c, ct, t, gk, s, rounded);
 }
 
-void rescale_inplace(Context &c, Ciphertext &ct, hipStream_t s) {
+void rescale_inplace(Context &c, Ciphertext &ct, hipStream_t s, bool rounded) {
     const size_t Ql = ct.Ql();
     if (Ql < 2) throw LogicError("end of modulus switching chain reached");
     if (!ct.ntt) throw InvalidArgument("CKKS encrypted must be in NTT form");
-    launch_rescale(c, ct.data(), ct.data(), Ql, ct.size, s);
+    launch_rescale(c, ct.data(), ct.data(), Ql, ct.size, s, rounded);
     ct.chain += 1;
     ct.scale = ct.scale / (double)c.q[Ql - 1];
 }
@@ -186,20 +186,20 @@ void ct_add_const(Context &c, Ciphertext &a, double v, hipStream_t s) {
 }
 
 // a <- rescale(a * [v encoded at scale sigma]); scale becomes a.scale * sigma / q_last.
-void ct_mul_const_rescale(Context &c, Ciphertext &a, double v, double sigma, hipStream_t s) {
+void ct_mul_const_rescale(Context &c, Ciphertext &a, double v, double sigma, hipStream_t s, bool rounded) {
     const size_t Ql = a.Ql();
     if (Ql < 2) throw LogicError("end of modulus switching chain reached");
     Scratch pt(Ql * c.N * sizeof(uint64_t), s);
     encode_scalar(c, v, Ql, sigma, pt.as<uint64_t>(), s);
     launch_mul_plain(c, a.data(), pt.as<uint64_t>(), Ql, a.size, s);
     a.scale *= sigma;
-    rescale_inplace(c, a, s);
+    rescale_inplace(c, a, s, rounded);
 }
 
 // Bring a to (chain, scale): drop to one level above the target, then one
 // constant multiply + rescale (the constant 1 encoded at scale * q_last / a.scale,
 // so the result carries `scale` up to the rounding of that constant).
-void ct_align(Context &c, Ciphertext &a, size_t chain, double scale, hipStream_t s) {
+void ct_align(Context &c, Ciphertext &a, size_t chain, double scale, hipStream_t s, bool rounded) {
     if (a.chain > chain) throw LogicError("ct_align: ciphertext is already below the target level");
     if (a.chain == chain) {
         if (std::fabs(a.scale / scale - 1.0) > 1e-9) throw LogicError("ct_align: scales differ at equal levels");
@@ -208,13 +208,13 @@ void ct_align(Context &c, Ciphertext &a, size_t chain, double scale, hipStream_t
     }
     if (a.chain + 1 < chain) mod_switch_drop(c, a, chain - 1, s);
     const double qlast = (double)c.q[a.Ql() - 1];
-    ct_mul_const_rescale(c, a, 1.0, scale * qlast / a.scale, s);
+    ct_mul_const_rescale(c, a, 1.0, scale * qlast / a.scale, s, rounded);
     a.scale = scale;
 }
 
 // out = rescale(relinearize(a * b)); operands are brought to the lower level first.
 void ct_mul_relin_rescale(Context &c, Ciphertext &out, const Ciphertext &a, const Ciphertext &b,
-                          const KSwitchKey &rk, hipStream_t s) {
+                          const KSwitchKey &rk, hipStream_t s, bool rounded) {
     if (a.size != 2 || b.size != 2) throw InvalidArgument("multiply supports size-2 ciphertexts");
     const size_t chain = std::max(a.chain, b.chain);
     Ciphertext ta, tb;
@@ -238,9 +238,9 @@ void ct_mul_relin_rescale(Context &c, Ciphertext &out, const Ciphertext &a, cons
         launch_tensor(c, pa->data(), pb->data(), out.data(), Ql, s);
     out.scale = pa->scale * pb->scale;
     out.ntt = true;
-    keyswitch(c, out.poly(2), rk.k.as<uint64_t>(), Ql, out.poly(0), out.poly(1), true, s);
+    keyswitch(c, out.poly(2), rk.k.as<uint64_t>(), Ql, out.poly(0), out.poly(1), true, s, rounded);
     out.size = 2;
-    rescale_inplace(c, out, s);
+    rescale_inplace(c, out, s, rounded);
 }
 
 void ct_mul_monomial_half(Context &c, Ciphertext &a, bool negate, hipStream_t s) {

@@ -239,7 +239,8 @@ void launch_square(const Context &c, const uint64_t *a, uint64_t *out3, size_t Q
     SHM_HIP(hipGetLastError());
 }
 
-void launch_rescale(const Context &c, const uint64_t *in, uint64_t *out, size_t Ql, size_t npolys, hipStream_t s) {
+void launch_rescale(const Context &c, const uint64_t *in, uint64_t *out, size_t Ql, size_t npolys, hipStream_t s,
+                    bool rounded) {
     if (Ql < 2) throw LogicError("end of modulus switching chain reached");
     const size_t N = c.N, L1 = Ql - 1;
     Scratch last(npolys * N * sizeof(uint64_t), s);
@@ -250,7 +251,7 @@ void launch_rescale(const Context &c, const uint64_t *in, uint64_t *out, size_t 
     // in-place: write into delta (each element is read then written by one thread), then copy back
     uint64_t *dst = (in == out) ? delta.as<uint64_t>() : out;
     rescale_ntt_fused(c, last.as<uint64_t>(), in, Ql, delta.as<uint64_t>(), nullptr, dst, npolys, npolys, L1,
-                      L.qlast_inv.as<uint64_t>(), L.qlast_inv_s.as<uint64_t>(), s);
+                      L.qlast_inv.as<uint64_t>(), L.qlast_inv_s.as<uint64_t>(), s, rounded);
     if (in == out)
         SHM_HIP(hipMemcpyAsync(out, delta.as<uint64_t>(), npolys * L1 * N * sizeof(uint64_t), hipMemcpyDeviceToDevice,
                                s));

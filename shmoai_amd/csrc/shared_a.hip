@@ -217,7 +217,7 @@ void add_shared_galois_key(SharedAKeys &K, int step, uint64_t seed, hipStream_t 
 // ---- format switching ---------------------------------------------------------------------
 
 void fmt_switch_forward(SharedAKeys &K, const std::vector<const Ciphertext *> &cts, SharedABatch &out,
-                        hipStream_t s) {
+                        hipStream_t s, bool rounded) {
     Context &c = *K.ctx;
     const size_t n = K.n, N = c.N, nP = c.nP;
     if (cts.size() != n) throw InvalidArgument("fmt_switch_forward: number of ciphertexts does not match the format key");
@@ -264,11 +264,11 @@ void fmt_switch_forward(SharedAKeys &K, const std::vector<const Ciphertext *> &c
         SHM_HIP(hipMemcpyAsync(out.ct.poly(i), cts[i]->poly(0), Ql * N * sizeof(uint64_t), hipMemcpyDeviceToDevice,
                                s));
     const uint64_t mask = (n >= 64) ? ~0ull : ((1ull << n) - 1);
-    ks_moddown(c, cx.as<uint64_t>(), n + 1, Ql, out.ct.data(), Ql * N, mask, s);
+    ks_moddown(c, cx.as<uint64_t>(), n + 1, Ql, out.ct.data(), Ql * N, mask, s, rounded);
 }
 
 void fmt_switch_backward(SharedAKeys &K, const SharedABatch &in, const std::vector<Ciphertext *> &out,
-                         hipStream_t s) {
+                         hipStream_t s, bool rounded) {
     Context &c = *K.ctx;
     const size_t n = in.n, N = c.N;
     if (n != K.n) throw InvalidArgument("fmt_switch_backward: batch size does not match the format key");
@@ -298,13 +298,13 @@ void fmt_switch_backward(SharedAKeys &K, const SharedABatch &in, const std::vect
         o.ntt = true;
         o.scale = in.ct.scale;
         SHM_HIP(hipMemcpyAsync(o.poly(0), in.ct.poly(i), Ql * N * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
-        ks_moddown(c, cx.as<uint64_t>() + 2 * i * QlP * N, 2, Ql, o.data(), Ql * N, 1, s);
+        ks_moddown(c, cx.as<uint64_t>() + 2 * i * QlP * N, 2, Ql, o.data(), Ql * N, 1, s, rounded);
     }
 }
 
 // ---- shared-a rotation ----------------------------------------------------------------------
 
-void shared_apply_galois(SharedAKeys &K, SharedABatch &x, uint32_t elt, hipStream_t s) {
+void shared_apply_galois(SharedAKeys &K, SharedABatch &x, uint32_t elt, hipStream_t s, bool rounded) {
     Context &c = *K.ctx;
     auto it = K.galois.find(elt);
     if (it == K.galois.end()) throw InvalidArgument("shared-a Galois key not present");
@@ -328,21 +328,21 @@ void shared_apply_galois(SharedAKeys &K, SharedABatch &x, uint32_t elt, hipStrea
     kr.base[n] = key.A.as<uint64_t>();
     kr.stride[n] = QP * N;
     ks_inner_multi(c, d.as<uint64_t>(), Ql, beta, kr, n + 1, false, cx.as<uint64_t>(), s);
-    ks_moddown(c, cx.as<uint64_t>(), n + 1, Ql, rot.data(), Ql * N, (1ull << n) - 1, s);
+    ks_moddown(c, cx.as<uint64_t>(), n + 1, Ql, rot.data(), Ql * N, (1ull << n) - 1, s, rounded);
     x.ct = std::move(rot);
 }
 
 // shared_a_rotate by a slot step (0 = conjugation); a missing step is split in
 // its NAF terms like rotate_internal.
-void shared_rotate(SharedAKeys &K, SharedABatch &x, int step, hipStream_t s) {
+void shared_rotate(SharedAKeys &K, SharedABatch &x, int step, hipStream_t s, bool rounded) {
     Context &c = *K.ctx;
     if (step == 0) {
-        shared_apply_galois(K, x, (uint32_t)(2 * c.N - 1), s);
+        shared_apply_galois(K, x, (uint32_t)(2 * c.N - 1), s, rounded);
         return;
     }
     const uint32_t elt = c.elt_from_step(step);
     if (K.galois.count(elt)) {
-        shared_apply_galois(K, x, elt, s);
+        shared_apply_galois(K, x, elt, s, rounded);
         return;
     }
     std::vector<int> naf;
@@ -355,7 +355,7 @@ void shared_rotate(SharedAKeys &K, SharedABatch &x, int step, hipStream_t s) {
     }
     if (naf.size() == 1) throw InvalidArgument("shared-a Galois key not present");
     for (int t : naf)
-        if ((size_t)std::abs(t) != (c.N >> 1)) shared_rotate(K, x, t, s);
+        if ((size_t)std::abs(t) != (c.N >> 1)) shared_rotate(K, x, t, s, rounded);
 }
 
 // ---- batched bootstrapping -------------------------------------------------------------------
@@ -379,11 +379,11 @@ void bootstrap_shared_a_s2c_first(Bootstrapper &b, SharedAKeys &K, const std::ve
         SharedABatch tmp;
         tmp.n = K.n;
         tmp.ct = std::move(x);
-        shared_rotate(K, tmp, step, s);
+        shared_rotate(K, tmp, step, s, true);
         x = std::move(tmp.ct);
     };
     SharedABatch batch;
-    fmt_switch_forward(K, in, batch, s);
+    fmt_switch_forward(K, in, batch, s, true);
     slot_to_coeff(b, batch.ct, rot, s);
     ct_modraise(c, batch.ct, s);
     const double q0 = (double)c.q[0];
@@ -398,8 +398,8 @@ void bootstrap_shared_a_s2c_first(Bootstrapper &b, SharedAKeys &K, const std::ve
         pr[i] = &res[i];
         pi[i] = &ims[i];
     }
-    fmt_switch_backward(K, re, pr, s);
-    fmt_switch_backward(K, im, pi, s);
+    fmt_switch_backward(K, re, pr, s, true);
+    fmt_switch_backward(K, im, pi, s, true);
     const double F = b.prm.layer3_factor;
     const double E = F * d_in * b.prm.input_bound / q0;
     const double pi_ = 3.14159265358979323846;
