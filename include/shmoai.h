@@ -315,6 +315,14 @@ int shm_shared_a_add_inplace(shm_shared_a_batch *a, const shm_shared_a_batch *b,
 int shm_shared_a_multiply_plain_inplace(shm_shared_a_batch *b, const shm_plaintext *pt, shm_stream_t stream);
 int shm_shared_a_rescale_to_next_inplace(shm_shared_a_batch *b, shm_stream_t stream);
 int shm_shared_a_modraise_inplace(shm_shared_a_batch *b, shm_stream_t stream);
+/* Bootstrapper::shared_a_sfl_full_3 (SharedACoeffToSlot.cu:311-358) on a batch at the
+ * 4-prime level (3 levels consumed) */
+int shm_bootstrap_shared_slot_to_coeff(shm_bootstrapper *b, shm_shared_a_keys *keys, shm_shared_a_batch *batch,
+                                       shm_stream_t stream);
+/* Bootstrapper::shared_a_coefftoslot_full_3 (SharedACoeffToSlot.cu:269-301): batch at the
+ * top level (after shm_shared_a_modraise_inplace) -> real / imaginary part batches */
+int shm_bootstrap_shared_coeff_to_slot(shm_bootstrapper *b, shm_shared_a_keys *keys, shm_shared_a_batch *batch,
+                                       shm_shared_a_batch *re, shm_shared_a_batch *im, shm_stream_t stream);
 /* BatchBootstrapper::bootstrap_shared_a_s2c_first (BatchBootstrap.cuh:719-820): exactly n
  * ciphertexts with 4 primes (n = the keys' batch size); out_scale 0 = the input scale */
 int shm_bootstrap_shared_a_s2c_first(shm_bootstrapper *b, shm_shared_a_keys *keys, const shm_ciphertext *const *in,

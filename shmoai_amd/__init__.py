@@ -473,6 +473,14 @@ class Bootstrapper:
     def eval_mod(self, ct: Ciphertext, relin_key: KSwitchKey, stream=None):
         check(_lib().shm_bootstrap_eval_mod(self.h, ct.h, relin_key.h, stream))
 
+    def shared_slot_to_coeff(self, keys: "SharedAKeys", batch: "SharedABatch", stream=None):
+        check(_lib().shm_bootstrap_shared_slot_to_coeff(self.h, keys.h, batch.h, stream))
+
+    def shared_coeff_to_slot(self, keys: "SharedAKeys", batch: "SharedABatch", stream=None):
+        re, im = SharedABatch(self.ctx, keys.n, 1), SharedABatch(self.ctx, keys.n, 1)
+        check(_lib().shm_bootstrap_shared_coeff_to_slot(self.h, keys.h, batch.h, re.h, im.h, stream))
+        return re, im
+
     def bootstrap_s2c_first(self, cts: Sequence[Ciphertext], gk: GaloisKeys, relin_key: KSwitchKey,
                             out_scale: float = 0.0, stream=None) -> list:
         out = [Ciphertext(self.ctx, 2, 1) for _ in cts]

@@ -139,6 +139,8 @@ _SIGS = {
     "shm_shared_a_rescale_to_next_inplace": (i32, [vp, vp]),
     "shm_shared_a_modraise_inplace": (i32, [vp, vp]),
     "shm_bootstrap_shared_a_s2c_first": (i32, [vp, vp, pvp, sz, pvp, vp, dbl, vp]),
+    "shm_bootstrap_shared_slot_to_coeff": (i32, [vp, vp, vp, vp]),
+    "shm_bootstrap_shared_coeff_to_slot": (i32, [vp, vp, vp, vp, vp, vp]),
 }
 
 _lib = None

@@ -140,6 +140,9 @@ void fmt_switch_backward(SharedAKeys &K, const SharedABatch &in, const std::vect
                          hipStream_t s, bool rounded = false);
 void shared_apply_galois(SharedAKeys &K, SharedABatch &x, uint32_t elt, hipStream_t s, bool rounded = false);
 void shared_rotate(SharedAKeys &K, SharedABatch &x, int step, hipStream_t s, bool rounded = false);
+void shared_slot_to_coeff(Bootstrapper &b, SharedAKeys &K, SharedABatch &x, hipStream_t s);
+void shared_coeff_to_slot(Bootstrapper &b, SharedAKeys &K, SharedABatch &x, SharedABatch &re, SharedABatch &im,
+                          hipStream_t s);
 void bootstrap_shared_a_s2c_first(Bootstrapper &b, SharedAKeys &K, const std::vector<const Ciphertext *> &in,
                                   const std::vector<Ciphertext *> &out, const KSwitchKey &rk, double out_scale,
                                   hipStream_t s);

@@ -1189,3 +1189,25 @@ int shm_bootstrap_shared_a_s2c_first(shm_bootstrapper *b, shm_shared_a_keys *key
         SHM_HIP(hipStreamSynchronize(S(st)));
     });
 }
+
+int shm_bootstrap_shared_slot_to_coeff(shm_bootstrapper *b, shm_shared_a_keys *keys, shm_shared_a_batch *batch,
+                                       shm_stream_t st) {
+    return guard([&] {
+        boot::Bootstrapper &bs = ready(b, S(st));
+        need(keys, "keys");
+        need(batch, "batch");
+        boot::shared_slot_to_coeff(bs, *keys->k, batch->b, S(st));
+    });
+}
+
+int shm_bootstrap_shared_coeff_to_slot(shm_bootstrapper *b, shm_shared_a_keys *keys, shm_shared_a_batch *batch,
+                                       shm_shared_a_batch *re, shm_shared_a_batch *im, shm_stream_t st) {
+    return guard([&] {
+        boot::Bootstrapper &bs = ready(b, S(st));
+        need(keys, "keys");
+        need(batch, "batch");
+        need(re, "re");
+        need(im, "im");
+        boot::shared_coeff_to_slot(bs, *keys->k, batch->b, re->b, im->b, S(st));
+    });
+}

@@ -262,8 +262,9 @@ struct KeyRefs {
     const uint64_t *base[kMaxKeys];  // output o reads digit g at base[o] + g * stride[o]
     size_t stride[kMaxKeys];
 };
-// ModUp of one Ql-limb NTT polynomial into [beta][QlP][N] (NTT form)
-void ks_modup(const Context &c, const uint64_t *c2, size_t Ql, uint64_t *d, hipStream_t s);
+// ModUp of one Ql-limb NTT polynomial into [beta][QlP][N] (NTT form); rounded =
+// exact centred digit lift instead of the reference's approximate conversion
+void ks_modup(const Context &c, const uint64_t *c2, size_t Ql, uint64_t *d, hipStream_t s, bool rounded = false);
 // cx[o] = sum_{g < ndig} d_g * key_o(g) over QlP limbs, o < nout
 void ks_inner_multi(const Context &c, const uint64_t *d, size_t Ql, size_t ndig, const KeyRefs &kr, size_t nout,
                     bool local, uint64_t *cx, hipStream_t s);

@@ -26,12 +26,16 @@ __device__ __forceinline__ uint32_t qlp_prime(size_t k, size_t Ql, size_t nQ) {
     return (uint32_t)(k < Ql ? k : nQ + (k - Ql));
 }
 
-// One launch covers every digit: out[g][k][j] for all g < beta, k < QlP.
+// One launch covers every digit: out[g][k][j] for all g < beta, k < QlP. With
+// `rounded`, the conversion is exact and centred (it subtracts
+// round(sum_i y_i / q_i) D_g), so every digit is lifted to (-D_g/2, D_g/2]:
+// zero-mean digits keep the key-error product free of the constant component
+// that otherwise lands on the slots nearest w = 1.
 __global__ void k_modup_bconv(const uint64_t *__restrict__ c2, const uint64_t *__restrict__ tcoef,
                               uint64_t *__restrict__ out, const DevMod *__restrict__ mods,
                               const uint64_t *__restrict__ hat_inv, const uint64_t *__restrict__ hat_inv_s,
                               const uint64_t *__restrict__ hat_mod, size_t Ql, size_t nQ, size_t alpha,
-                              size_t beta, size_t N) {
+                              size_t beta, size_t N, int rounded) {
     const size_t QlP = Ql + alpha;
     const size_t total = beta * QlP * N;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
@@ -42,17 +46,28 @@ __global__ void k_modup_bconv(const uint64_t *__restrict__ c2, const uint64_t *_
             continue;
         }
         const DevMod mp = mods[qlp_prime(k, Ql, nQ)];
-        if (alpha == 1) {
-            out[t] = reduce64(tcoef[start * N + j], mp);
+        if (size == 1) {
+            const uint64_t x = tcoef[start * N + j];
+            uint64_t v = reduce64(x, mp);
+            if (rounded && x > (mods[start].q >> 1)) v = sub_mod(v, reduce64(mods[start].q, mp), mp.q);
+            out[t] = v;
         } else {
             Acc128 acc{0, 0};
+            double vsum = 0.0;
             for (size_t i = 0; i < size; i++) {
                 const size_t li = start + i;
                 const uint64_t qi = mods[li].q;
                 const uint64_t x = mul_shoup(tcoef[li * N + j], hat_inv[li], hat_inv_s[li], qi);
                 mac128(acc, x, hat_mod[(g * alpha + i) * QlP + k]);
+                if (rounded) vsum = __dadd_rn(vsum, __ddiv_rn((double)x, (double)qi));
             }
-            out[t] = reduce128(acc, mp);
+            uint64_t v = reduce128(acc, mp);
+            if (rounded) {
+                // D_g mod p = (D_g / q_start) q_start mod p
+                const uint64_t dg = mul_mod(hat_mod[(g * alpha) * QlP + k], reduce64(mods[start].q, mp), mp);
+                v = sub_mod(v, mul_mod((uint64_t)__dadd_rn(vsum, 0.5), dg, mp), mp.q);
+            }
+            out[t] = v;
         }
     }
 }
@@ -161,7 +176,7 @@ __global__ void k_moddown_combine(const uint64_t *__restrict__ cx, const uint64_
 
 }  // namespace
 
-void ks_modup(const Context &c, const uint64_t *c2, size_t Ql, uint64_t *d, hipStream_t s) {
+void ks_modup(const Context &c, const uint64_t *c2, size_t Ql, uint64_t *d, hipStream_t s, bool rounded) {
     const size_t N = c.N, alpha = c.nP, QlP = Ql + alpha;
     const LevelConsts &L = c.level(Ql);
     const size_t beta = L.beta;
@@ -170,7 +185,7 @@ void ks_modup(const Context &c, const uint64_t *c2, size_t Ql, uint64_t *d, hipS
     ntt_inverse(c, tcoef.as<uint64_t>(), 1, Ql, 0, s);
     k_modup_bconv<<<grid_for(beta * QlP * N), kBlock, 0, s>>>(
         c2, tcoef.as<uint64_t>(), d, c.mods(), L.hat_inv.as<uint64_t>(), L.hat_inv_s.as<uint64_t>(),
-        L.hat_mod.as<uint64_t>(), Ql, c.nQ, alpha, beta, N);
+        L.hat_mod.as<uint64_t>(), Ql, c.nQ, alpha, beta, N, rounded ? 1 : 0);
     SHM_HIP(hipGetLastError());
     for (size_t g = 0; g < beta; g++) {
         const size_t start = g * alpha, size = std::min(alpha, Ql - start);
@@ -214,7 +229,7 @@ void keyswitch(const Context &c, const uint64_t *c2, const uint64_t *key, size_t
     if (beta > c.dnum) throw LogicError("key switching key has too few digits for this level");
     Scratch d(beta * QlP * N * sizeof(uint64_t), s);
     Scratch cx(2 * QlP * N * sizeof(uint64_t), s);
-    ks_modup(c, c2, Ql, d.as<uint64_t>(), s);
+    ks_modup(c, c2, Ql, d.as<uint64_t>(), s, rounded);
     k_ks_inner<<<grid_for(QlP * N), kBlock, 0, s>>>(d.as<uint64_t>(), key, cx.as<uint64_t>(), c.mods(), Ql, c.nQ,
                                                     c.nQP(), alpha, beta, N);
     SHM_HIP(hipGetLastError());

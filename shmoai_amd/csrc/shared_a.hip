@@ -276,7 +276,7 @@ void fmt_switch_backward(SharedAKeys &K, const SharedABatch &in, const std::vect
     if (!in.ct.ntt) throw InvalidArgument("fmt_switch_backward: only NTT form is supported");
     const size_t Ql = in.ct.Ql(), QlP = Ql + c.nP, beta = c.level(Ql).beta;
     Scratch d(beta * QlP * N * sizeof(uint64_t), s);
-    ks_modup(c, in.ct.poly(n), Ql, d.as<uint64_t>(), s);
+    ks_modup(c, in.ct.poly(n), Ql, d.as<uint64_t>(), s, rounded);
     Scratch cx(2 * n * QlP * N * sizeof(uint64_t), s);
     // at most kMaxKeys outputs per inner-product launch: members in groups of 16
     for (size_t i0 = 0; i0 < n; i0 += 16) {
@@ -319,7 +319,7 @@ void shared_apply_galois(SharedAKeys &K, SharedABatch &x, uint32_t elt, hipStrea
     rot.scale = x.ct.scale;
     launch_permute(c, x.ct.data(), rot.data(), table, (n + 1) * Ql, s);
     Scratch d(beta * QlP * N * sizeof(uint64_t), s), cx((n + 1) * QlP * N * sizeof(uint64_t), s);
-    ks_modup(c, rot.poly(n), Ql, d.as<uint64_t>(), s);
+    ks_modup(c, rot.poly(n), Ql, d.as<uint64_t>(), s, rounded);
     KeyRefs kr{};
     for (size_t i = 0; i < n; i++) {
         kr.base[i] = key.B.as<uint64_t>() + i * QP * N;
@@ -360,6 +360,34 @@ void shared_rotate(SharedAKeys &K, SharedABatch &x, int step, hipStream_t s, boo
 
 // ---- batched bootstrapping -------------------------------------------------------------------
 
+namespace {
+RotateFn shared_rotation(SharedAKeys &K, hipStream_t s) {
+    return [&K, s](Ciphertext &x, int step) {
+        SharedABatch tmp;
+        tmp.n = K.n;
+        tmp.ct = std::move(x);
+        shared_rotate(K, tmp, step, s, true);
+        x = std::move(tmp.ct);
+    };
+}
+}  // namespace
+
+// shared_a_sfl_full_3 (SharedACoeffToSlot.cu:311-358)
+void shared_slot_to_coeff(Bootstrapper &b, SharedAKeys &K, SharedABatch &x, hipStream_t s) {
+    if (x.n != K.n) throw InvalidArgument("batch size does not match the shared-a keys");
+    if (!b.prepared) b.prepare(s);
+    slot_to_coeff(b, x.ct, shared_rotation(K, s), s);
+}
+
+// shared_a_coefftoslot_full_3 (SharedACoeffToSlot.cu:269-301)
+void shared_coeff_to_slot(Bootstrapper &b, SharedAKeys &K, SharedABatch &x, SharedABatch &re, SharedABatch &im,
+                          hipStream_t s) {
+    if (x.n != K.n) throw InvalidArgument("batch size does not match the shared-a keys");
+    if (!b.prepared) b.prepare(s);
+    re.n = im.n = K.n;
+    coeff_to_slot(b, x.ct, re.ct, im.ct, shared_rotation(K, s), s);
+}
+
 // BatchBootstrap.cuh:719-820 bootstrap_shared_a_s2c_first: FmtSwitch -> Sh-S2C ->
 // shared-a modraise -> Sh-C2S -> backward FmtSwitch -> per-member EvalMod +
 // recombination (amplitude correction folded into EvalMod).
@@ -375,22 +403,14 @@ void bootstrap_shared_a_s2c_first(Bootstrapper &b, SharedAKeys &K, const std::ve
     if (!b.prepared) b.prepare(s);
     const double d_in = in[0]->scale;
     if (out_scale <= 0) out_scale = d_in;
-    RotateFn rot = [&](Ciphertext &x, int step) {
-        SharedABatch tmp;
-        tmp.n = K.n;
-        tmp.ct = std::move(x);
-        shared_rotate(K, tmp, step, s, true);
-        x = std::move(tmp.ct);
-    };
     SharedABatch batch;
     fmt_switch_forward(K, in, batch, s, true);
-    slot_to_coeff(b, batch.ct, rot, s);
+    shared_slot_to_coeff(b, K, batch, s);
     ct_modraise(c, batch.ct, s);
     const double q0 = (double)c.q[0];
     batch.ct.scale = q0;
     SharedABatch re, im;
-    re.n = im.n = K.n;
-    coeff_to_slot(b, batch.ct, re.ct, im.ct, rot, s);
+    shared_coeff_to_slot(b, K, batch, re, im, s);
     const size_t n = K.n;
     std::vector<Ciphertext> res(n), ims(n);
     std::vector<Ciphertext *> pr(n), pi(n);
