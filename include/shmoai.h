@@ -179,6 +179,10 @@ int shm_apply_galois_inplace(shm_context *ctx, shm_ciphertext *ct, uint32_t galo
 /* evaluate.cu:1650-1715 rotate_vector_inplace (NAF split when a key is missing) */
 int shm_rotate_vector_inplace(shm_context *ctx, shm_ciphertext *ct, int step, const shm_galois_keys *gk,
                               shm_stream_t stream);
+/* evaluate.cu:1727-1920 hoisting_inplace: ct <- sum of the rotations of ct by steps[],
+ * one ModUp of c1 and one ModDown of the summed inner products */
+int shm_hoisting_inplace(shm_context *ctx, shm_ciphertext *ct, const shm_galois_keys *gk, const int *steps,
+                         size_t n_steps, shm_stream_t stream);
 /* evaluate.cu:1717-1725 complex_conjugate_inplace */
 int shm_complex_conjugate_inplace(shm_context *ctx, shm_ciphertext *ct, const shm_galois_keys *gk,
                                   shm_stream_t stream);

@@ -390,6 +390,11 @@ class Evaluator:
     def complex_conjugate_inplace(self, ct, gk: GaloisKeys, stream=None):
         check(self.L.shm_complex_conjugate_inplace(self.ctx.h, ct.h, gk.h, stream))
 
+    def hoisting_inplace(self, ct, gk: GaloisKeys, steps: Sequence[int], stream=None):
+        """evaluate.cu:1727-1920: ct <- sum of rot(ct, step) over steps (one ModUp, one ModDown)."""
+        arr = (C.c_int * len(steps))(*steps)
+        check(self.L.shm_hoisting_inplace(self.ctx.h, ct.h, gk.h, arr, len(steps), stream))
+
     def multiply_by_i_inplace(self, ct, negate: bool = False, stream=None):
         """Multiply every slot by i (X^(N/2)), or by -i (X^(3N/2)) when negate."""
         check(self.L.shm_multiply_by_i_inplace(self.ctx.h, ct.h, 1 if negate else 0, stream))

@@ -101,6 +101,7 @@ _SIGS = {
     "shm_apply_galois_inplace": (i32, [vp, vp, u32, vp, vp]),
     "shm_rotate_vector_inplace": (i32, [vp, vp, i32, vp, vp]),
     "shm_complex_conjugate_inplace": (i32, [vp, vp, vp, vp]),
+    "shm_hoisting_inplace": (i32, [vp, vp, vp, C.POINTER(C.c_int), sz, vp]),
     "shm_ct_pt_matrix_mul": (i32, [vp, pvp, u32, C.POINTER(dbl), u32, pvp, i32, vp]),
     "shm_last_matmul_kernel_ms": (dbl, []),
     "shm_last_matmul_variant": (i32, []),

@@ -89,10 +89,16 @@ std::unique_ptr<Bootstrapper> make_plan(size_t N, size_t nQ, const BootParams &p
 // Device pipeline. A RotateFn rotates a ciphertext (or shared-a batch) in place
 // by a slot step; step 0 is complex conjugation.
 using RotateFn = std::function<void(Ciphertext &, int)>;
-void lt_apply(Context &c, const LtLayer &L, Ciphertext &ct, const RotateFn &rot, hipStream_t s);
-void slot_to_coeff(Bootstrapper &b, Ciphertext &ct, const RotateFn &rot, hipStream_t s);
+// Several rotations of one input sharing one ModUp: outs[i] = rot(in, steps[i]).
+using HoistFn = std::function<void(const Ciphertext &, const std::vector<int> &, const std::vector<Ciphertext *> &)>;
+struct Rotator {
+    RotateFn rot;
+    HoistFn hoist;  // optional
+};
+void lt_apply(Context &c, const LtLayer &L, Ciphertext &ct, const Rotator &r, hipStream_t s);
+void slot_to_coeff(Bootstrapper &b, Ciphertext &ct, const Rotator &r, hipStream_t s);
 void slot_to_coeff(Bootstrapper &b, Ciphertext &ct, const GaloisKeys &gk, hipStream_t s);
-void coeff_to_slot(Bootstrapper &b, Ciphertext &ct, Ciphertext &re, Ciphertext &im, const RotateFn &rot,
+void coeff_to_slot(Bootstrapper &b, Ciphertext &ct, Ciphertext &re, Ciphertext &im, const Rotator &r,
                    hipStream_t s);
 void coeff_to_slot(Bootstrapper &b, Ciphertext &ct, Ciphertext &re, Ciphertext &im, const GaloisKeys &gk,
                    hipStream_t s);
@@ -140,6 +146,9 @@ void fmt_switch_backward(SharedAKeys &K, const SharedABatch &in, const std::vect
                          hipStream_t s, bool rounded = false);
 void shared_apply_galois(SharedAKeys &K, SharedABatch &x, uint32_t elt, hipStream_t s, bool rounded = false);
 void shared_rotate(SharedAKeys &K, SharedABatch &x, int step, hipStream_t s, bool rounded = false);
+// hoisted Sh-Rot: one ModUp of a for all steps; outs are batch blocks of size n + 1
+void shared_rotate_hoisted(SharedAKeys &K, const Ciphertext &x, const std::vector<int> &steps,
+                           const std::vector<Ciphertext *> &outs, hipStream_t s, bool rounded);
 void shared_slot_to_coeff(Bootstrapper &b, SharedAKeys &K, SharedABatch &x, hipStream_t s);
 void shared_coeff_to_slot(Bootstrapper &b, SharedAKeys &K, SharedABatch &x, SharedABatch &re, SharedABatch &im,
                           hipStream_t s);

@@ -112,18 +112,28 @@ void const_lincomb(Context &c, Ciphertext &acc, const std::vector<const Cipherte
 // inner sums with pre-rotated diagonals, giant rotation of each inner sum. `ct`
 // is an ordinary ciphertext or a shared-a batch (every polynomial is multiplied
 // by the same diagonal); `rot` rotates it in place.
-void lt_apply(Context &c, const LtLayer &L, Ciphertext &ct, const RotateFn &rot, hipStream_t s) {
+void lt_apply(Context &c, const LtLayer &L, Ciphertext &ct, const Rotator &r, hipStream_t s) {
     if (ct.size < 2) throw InvalidArgument("linear transform takes ciphertexts of size >= 2");
     if (ct.chain != L.chain) throw InvalidArgument("linear transform applied at the wrong level");
     if (L.pts.empty()) throw LogicError("bootstrapper is not prepared");
     const size_t Ql = ct.Ql(), n = c.N / 2;
     std::vector<Ciphertext> baby(L.g);
+    std::vector<int> steps;
+    std::vector<Ciphertext *> outs;
     for (int j = 1; j < L.g; j++) {
         bool used = false;
         for (int i = 0; i < L.n_giants; i++) used |= !L.pre[i][j].empty();
         if (!used) continue;
-        ct_clone(baby[j], ct, s);
-        rot(baby[j], L.baby_step(j));
+        steps.push_back(L.baby_step(j));
+        outs.push_back(&baby[j]);
+    }
+    if (r.hoist) {
+        r.hoist(ct, steps, outs);
+    } else {
+        for (size_t t = 0; t < steps.size(); t++) {
+            ct_clone(*outs[t], ct, s);
+            r.rot(*outs[t], steps[t]);
+        }
     }
     Ciphertext out;
     bool have = false;
@@ -145,7 +155,7 @@ void lt_apply(Context &c, const LtLayer &L, Ciphertext &ct, const RotateFn &rot,
         k_lt_mac<<<grid_for(ct.size * Ql * c.N), kBlock, 0, s>>>(inner.data(), pk, nt, c.mods(), Ql, c.N, ct.size);
         SHM_HIP(hipGetLastError());
         const long G = ((L.giant_offset(i) % (long)n) + (long)n) % (long)n;
-        if (G) rot(inner, (int)G);
+        if (G) r.rot(inner, (int)G);
         if (!have) {
             out = std::move(inner);
             have = true;
@@ -161,24 +171,39 @@ void lt_apply(Context &c, const LtLayer &L, Ciphertext &ct, const RotateFn &rot,
     ct = std::move(out);
 }
 
-void slot_to_coeff(Bootstrapper &b, Ciphertext &ct, const RotateFn &rot, hipStream_t s) {
-    for (int l = 0; l < 3; l++) lt_apply(*b.ctx, b.s2c[l], ct, rot, s);
+void slot_to_coeff(Bootstrapper &b, Ciphertext &ct, const Rotator &r, hipStream_t s) {
+    for (int l = 0; l < 3; l++) lt_apply(*b.ctx, b.s2c[l], ct, r, s);
 }
 
+namespace {
+Rotator ordinary_rotator(Context &c, const GaloisKeys &gk, hipStream_t s) {
+    Rotator r;
+    r.rot = [&c, &gk, s](Ciphertext &x, int step) {
+        if (step == 0)
+            apply_galois(c, x, (uint32_t)(2 * c.N - 1), gk, s, true);
+        else
+            rotate_internal(c, x, step, gk, s, true);
+    };
+    r.hoist = [&c, &gk, s](const Ciphertext &x, const std::vector<int> &steps, const std::vector<Ciphertext *> &outs) {
+        rotate_hoisted(c, x, steps, gk, outs, s, true);
+    };
+    return r;
+}
+}  // namespace
+
 void slot_to_coeff(Bootstrapper &b, Ciphertext &ct, const GaloisKeys &gk, hipStream_t s) {
-    Context &c = *b.ctx;
-    slot_to_coeff(b, ct, [&](Ciphertext &x, int step) { rotate_internal(c, x, step, gk, s, true); }, s);
+    slot_to_coeff(b, ct, ordinary_rotator(*b.ctx, gk, s), s);
 }
 
 // coefftoslot_full_3 (Bootstrapper.cu:3048-3067): re = w + conj(w) and
 // im = -i (w - conj(w)) with one conjugation; -i is the exact monomial X^(3N/2).
-void coeff_to_slot(Bootstrapper &b, Ciphertext &ct, Ciphertext &re, Ciphertext &im, const RotateFn &rot,
+void coeff_to_slot(Bootstrapper &b, Ciphertext &ct, Ciphertext &re, Ciphertext &im, const Rotator &r,
                    hipStream_t s) {
     Context &c = *b.ctx;
-    for (int l = 0; l < 3; l++) lt_apply(c, b.c2s[l], ct, rot, s);
+    for (int l = 0; l < 3; l++) lt_apply(c, b.c2s[l], ct, r, s);
     Ciphertext conj;
     ct_clone(conj, ct, s);
-    rot(conj, 0);  // step 0 = complex conjugation
+    r.rot(conj, 0);  // step 0 = complex conjugation
     ct_clone(re, ct, s);
     ct_add(c, re, conj, s);  // w + conj(w)
     ct_clone(im, ct, s);
@@ -188,16 +213,7 @@ void coeff_to_slot(Bootstrapper &b, Ciphertext &ct, Ciphertext &re, Ciphertext &
 
 void coeff_to_slot(Bootstrapper &b, Ciphertext &ct, Ciphertext &re, Ciphertext &im, const GaloisKeys &gk,
                    hipStream_t s) {
-    Context &c = *b.ctx;
-    coeff_to_slot(
-        b, ct, re, im,
-        [&](Ciphertext &x, int step) {
-            if (step == 0)
-                apply_galois(c, x, (uint32_t)(2 * c.N - 1), gk, s, true);
-            else
-                rotate_internal(c, x, step, gk, s, true);
-        },
-        s);
+    coeff_to_slot(b, ct, re, im, ordinary_rotator(*b.ctx, gk, s), s);
 }
 
 namespace {

@@ -719,6 +719,17 @@ int shm_rotate_vector_inplace(shm_context *ctx, shm_ciphertext *ct, int step, co
     });
 }
 
+int shm_hoisting_inplace(shm_context *ctx, shm_ciphertext *ct, const shm_galois_keys *gk, const int *steps,
+                         size_t n_steps, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(ct, "ct");
+        need(gk, "galois_keys");
+        need(steps, "steps");
+        hoisting_inplace(*ctx->c, ct->ct, std::vector<int>(steps, steps + n_steps), gk->g, S(st));
+    });
+}
+
 int shm_complex_conjugate_inplace(shm_context *ctx, shm_ciphertext *ct, const shm_galois_keys *gk, shm_stream_t st) {
     return guard([&] {
         need(ctx, "ctx");

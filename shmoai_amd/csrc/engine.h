@@ -265,9 +265,10 @@ struct KeyRefs {
 // ModUp of one Ql-limb NTT polynomial into [beta][QlP][N] (NTT form); rounded =
 // exact centred digit lift instead of the reference's approximate conversion
 void ks_modup(const Context &c, const uint64_t *c2, size_t Ql, uint64_t *d, hipStream_t s, bool rounded = false);
-// cx[o] = sum_{g < ndig} d_g * key_o(g) over QlP limbs, o < nout
+// cx[o] = sum_{g < ndig} d_g * key_o(g) over QlP limbs, o < nout; with perm, the
+// digits are read through that NTT-domain permutation (hoisted rotations)
 void ks_inner_multi(const Context &c, const uint64_t *d, size_t Ql, size_t ndig, const KeyRefs &kr, size_t nout,
-                    bool local, uint64_t *cx, hipStream_t s);
+                    bool local, uint64_t *cx, hipStream_t s, const uint32_t *perm = nullptr, bool accumulate = false);
 // ModDown of npolys [QlP][N] polynomials into out + p * out_stride (bit p of acc_mask: accumulate)
 void ks_moddown(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, uint64_t *out, size_t out_stride,
                 uint64_t acc_mask, hipStream_t s, bool rounded = false);
@@ -293,6 +294,15 @@ void gen_kswitch(Context &c, const SecretKey &sk, const uint64_t *newkey_dev, ui
 void apply_galois(Context &c, Ciphertext &ct, uint32_t elt, const GaloisKeys &gk, hipStream_t s, bool rounded = false);
 void rotate_internal(Context &c, Ciphertext &ct, int step, const GaloisKeys &gk, hipStream_t s, bool rounded = false);
 void rescale_inplace(Context &c, Ciphertext &ct, hipStream_t s, bool rounded = false);
+// Rotations of one ciphertext by several steps sharing one ModUp of c1 (hoisting,
+// evaluate.cu:1727-1920 hoisting_inplace); outs[i] receives rot(ct, steps[i]).
+// Steps without their own key fall back to rotate_internal.
+void rotate_hoisted(Context &c, const Ciphertext &ct, const std::vector<int> &steps, const GaloisKeys &gk,
+                    const std::vector<Ciphertext *> &outs, hipStream_t s, bool rounded = false);
+// evaluate.cu:1727-1920 hoisting_inplace: ct <- sum_i rot(ct, steps[i]) with one
+// ModUp of c1 and one ModDown of the accumulated inner products
+void hoisting_inplace(Context &c, Ciphertext &ct, const std::vector<int> &steps, const GaloisKeys &gk,
+                      hipStream_t s);
 void mod_switch_drop(Context &c, Ciphertext &ct, size_t new_chain, hipStream_t s);
 void ct_clone(Ciphertext &dst, const Ciphertext &src, hipStream_t s);
 void ct_add(Context &c, Ciphertext &a, const Ciphertext &b, hipStream_t s);

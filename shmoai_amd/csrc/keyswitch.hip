@@ -26,48 +26,58 @@ __device__ __forceinline__ uint32_t qlp_prime(size_t k, size_t Ql, size_t nQ) {
     return (uint32_t)(k < Ql ? k : nQ + (k - Ql));
 }
 
-// One launch covers every digit: out[g][k][j] for all g < beta, k < QlP. With
+// One thread per (digit g, coefficient j): the digit's own limbs are copied (NTT
+// form) and its y_i = [x_i (D_g/q_i)^-1]_{q_i} are formed once, then every other
+// QlP limb gets sum_i y_i [D_g/q_i]_p (the approximate fast conversion). With
 // `rounded`, the conversion is exact and centred (it subtracts
 // round(sum_i y_i / q_i) D_g), so every digit is lifted to (-D_g/2, D_g/2]:
 // zero-mean digits keep the key-error product free of the constant component
 // that otherwise lands on the slots nearest w = 1.
+constexpr int kMaxAlpha = 16;
+
 __global__ void k_modup_bconv(const uint64_t *__restrict__ c2, const uint64_t *__restrict__ tcoef,
                               uint64_t *__restrict__ out, const DevMod *__restrict__ mods,
                               const uint64_t *__restrict__ hat_inv, const uint64_t *__restrict__ hat_inv_s,
                               const uint64_t *__restrict__ hat_mod, size_t Ql, size_t nQ, size_t alpha,
                               size_t beta, size_t N, int rounded) {
     const size_t QlP = Ql + alpha;
-    const size_t total = beta * QlP * N;
+    const size_t total = beta * N;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t g = t / (QlP * N), r = t % (QlP * N), k = r / N, j = r % N;
+        const size_t g = t / N, j = t % N;
         const size_t start = g * alpha, size = min(alpha, Ql - start);
-        if (k >= start && k < start + size) {
-            out[t] = c2[k * N + j];
-            continue;
-        }
-        const DevMod mp = mods[qlp_prime(k, Ql, nQ)];
-        if (size == 1) {
-            const uint64_t x = tcoef[start * N + j];
-            uint64_t v = reduce64(x, mp);
-            if (rounded && x > (mods[start].q >> 1)) v = sub_mod(v, reduce64(mods[start].q, mp), mp.q);
-            out[t] = v;
-        } else {
-            Acc128 acc{0, 0};
-            double vsum = 0.0;
-            for (size_t i = 0; i < size; i++) {
+        uint64_t *o = out + g * QlP * N + j;
+        uint64_t y[kMaxAlpha];
+        double v = 0.0;
+#pragma unroll
+        for (int i = 0; i < kMaxAlpha; i++) {
+            y[i] = 0;
+            if ((size_t)i < size) {
                 const size_t li = start + i;
                 const uint64_t qi = mods[li].q;
-                const uint64_t x = mul_shoup(tcoef[li * N + j], hat_inv[li], hat_inv_s[li], qi);
-                mac128(acc, x, hat_mod[(g * alpha + i) * QlP + k]);
-                if (rounded) vsum = __dadd_rn(vsum, __ddiv_rn((double)x, (double)qi));
+                y[i] = size == 1 ? tcoef[li * N + j] : mul_shoup(tcoef[li * N + j], hat_inv[li], hat_inv_s[li], qi);
+                if (rounded) v = __dadd_rn(v, __ddiv_rn((double)y[i], (double)qi));
+                o[li * N] = c2[li * N + j];
             }
-            uint64_t v = reduce128(acc, mp);
-            if (rounded) {
-                // D_g mod p = (D_g / q_start) q_start mod p
-                const uint64_t dg = mul_mod(hat_mod[(g * alpha) * QlP + k], reduce64(mods[start].q, mp), mp);
-                v = sub_mod(v, mul_mod((uint64_t)__dadd_rn(vsum, 0.5), dg, mp), mp.q);
+        }
+        const uint64_t r = rounded ? (uint64_t)__dadd_rn(v, 0.5) : 0;
+        const uint64_t qs = mods[start].q;
+        for (size_t k = 0; k < QlP; k++) {
+            if (k >= start && k < start + size) continue;
+            const DevMod mp = mods[qlp_prime(k, Ql, nQ)];
+            uint64_t val;
+            if (size == 1) {
+                val = reduce64(y[0], mp);
+                if (r) val = sub_mod(val, reduce64(qs, mp), mp.q);
+            } else {
+                Acc128 acc{0, 0};
+                const uint64_t *hm = hat_mod + (g * alpha) * QlP + k;
+#pragma unroll
+                for (int i = 0; i < kMaxAlpha; i++)
+                    if ((size_t)i < size) mac128(acc, y[i], hm[i * QlP]);
+                val = reduce128(acc, mp);
+                if (r) val = sub_mod(val, mul_mod(r, mul_mod(hm[0], reduce64(qs, mp), mp), mp), mp.q);
             }
-            out[t] = v;
+            o[k * N] = val;
         }
     }
 }
@@ -98,15 +108,17 @@ __global__ void k_ks_inner(const uint64_t *__restrict__ d, const uint64_t *__res
 // position (keys generated for one level).
 __global__ void k_ks_inner_multi(const uint64_t *__restrict__ d, KeyRefs kr, int nout, uint64_t *__restrict__ cx,
                                  const DevMod *__restrict__ mods, size_t Ql, size_t nQ, size_t alpha, size_t ndig,
-                                 size_t N, int local) {
+                                 size_t N, int local, const uint32_t *__restrict__ perm, int accumulate) {
     const size_t QlP = Ql + alpha, total = QlP * N;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
         const size_t k = t / N, j = t % N;
         const uint32_t pi = qlp_prime(k, Ql, nQ);
         const size_t kl = (local ? k : (size_t)pi) * N + j;
+        // hoisted rotation: the digits are read through the NTT-domain Galois permutation
+        const size_t src = perm ? k * N + perm[j] : t;
         uint64_t dv[kMaxDigits];  // fully unrolled: stays in registers
 #pragma unroll
-        for (int g = 0; g < kMaxDigits; g++) dv[g] = (size_t)g < ndig ? d[g * total + t] : 0;
+        for (int g = 0; g < kMaxDigits; g++) dv[g] = (size_t)g < ndig ? d[g * total + src] : 0;
         const DevMod m = mods[pi];
         for (int o = 0; o < nout; o++) {
             Acc128 a{0, 0};
@@ -115,44 +127,53 @@ __global__ void k_ks_inner_multi(const uint64_t *__restrict__ d, KeyRefs kr, int
 #pragma unroll
             for (int g = 0; g < kMaxDigits; g++)
                 if ((size_t)g < ndig) mac128(a, dv[g], kb[g * st + kl]);
-            cx[o * total + t] = reduce128(a, m);
+            const uint64_t v = reduce128(a, m);
+            cx[o * total + t] = accumulate ? add_mod(cx[o * total + t], v, m.q) : v;
         }
     }
 }
 
-// delta[i][j] = fastbconv of the coefficient-form P limbs (cxP) to q_i. With
-// `rounded`, the conversion is exact and centred: it subtracts
-// round(sum_j y_j / p_j) P, so delta is the representative of cx mod P in
-// (-P/2, P/2] and (cx - delta) / P rounds to nearest.
-__global__ void k_moddown_bconv(const uint64_t *__restrict__ cxP, uint64_t *__restrict__ delta,
+// delta[p][i][j] = fastbconv of the coefficient-form P limbs of polynomial p to
+// q_i, one thread per (p, j). With `rounded`, the conversion is exact and
+// centred: it subtracts round(sum_j y_j / p_j) P, so delta is the representative
+// of cx mod P in (-P/2, P/2] and (cx - delta) / P rounds to nearest.
+__global__ void k_moddown_bconv(const uint64_t *__restrict__ cx, uint64_t *__restrict__ delta,
                                 const DevMod *__restrict__ mods, const uint64_t *__restrict__ phat_inv,
                                 const uint64_t *__restrict__ phat_inv_s, const uint64_t *__restrict__ phat_mod,
                                 const uint64_t *__restrict__ pmod, size_t Ql, size_t nQ, size_t nP, size_t N,
-                                int rounded) {
-    const size_t total = Ql * N;
+                                size_t npolys, int rounded) {
+    const size_t QlP = Ql + nP, total = npolys * N;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t i = t / N, j = t % N;
-        const DevMod mq = mods[i];
-        if (nP == 1) {
-            const uint64_t x = cxP[j];
-            uint64_t d = reduce64(x, mq);
-            if (rounded && x > (mods[nQ].q >> 1)) d = sub_mod(d, reduce64(mods[nQ].q, mq), mq.q);
-            delta[t] = d;
-        } else {
-            Acc128 acc{0, 0};
-            double v = 0.0;
-            for (size_t jj = 0; jj < nP; jj++) {
-                const uint64_t p = mods[nQ + jj].q;
-                const uint64_t x = mul_shoup(cxP[jj * N + j], phat_inv[jj], phat_inv_s[jj], p);
-                mac128(acc, x, phat_mod[i * nP + jj]);
-                if (rounded) v = __dadd_rn(v, __ddiv_rn((double)x, (double)p));
+        const size_t p = t / N, j = t % N;
+        const uint64_t *cxP = cx + (p * QlP + Ql) * N + j;
+        uint64_t *d = delta + p * Ql * N + j;
+        uint64_t y[kMaxAlpha];
+        double v = 0.0;
+#pragma unroll
+        for (int jj = 0; jj < kMaxAlpha; jj++) {
+            y[jj] = 0;
+            if ((size_t)jj < nP) {
+                const uint64_t pj = mods[nQ + jj].q;
+                y[jj] = nP == 1 ? cxP[0] : mul_shoup(cxP[jj * N], phat_inv[jj], phat_inv_s[jj], pj);
+                if (rounded) v = __dadd_rn(v, __ddiv_rn((double)y[jj], (double)pj));
             }
-            uint64_t d = reduce128(acc, mq);
-            if (rounded) {
-                const uint64_t r = (uint64_t)__dadd_rn(v, 0.5);  // round(sum y_j / p_j)
-                d = sub_mod(d, mul_mod(r, pmod[i], mq), mq.q);
+        }
+        const uint64_t r = rounded ? (uint64_t)__dadd_rn(v, 0.5) : 0;
+        for (size_t i = 0; i < Ql; i++) {
+            const DevMod mq = mods[i];
+            uint64_t val;
+            if (nP == 1) {
+                val = reduce64(y[0], mq);
+                if (r) val = sub_mod(val, reduce64(mods[nQ].q, mq), mq.q);
+            } else {
+                Acc128 acc{0, 0};
+#pragma unroll
+                for (int jj = 0; jj < kMaxAlpha; jj++)
+                    if ((size_t)jj < nP) mac128(acc, y[jj], phat_mod[i * nP + jj]);
+                val = reduce128(acc, mq);
+                if (r) val = sub_mod(val, mul_mod(r, pmod[i], mq), mq.q);
             }
-            delta[t] = d;
+            d[i * N] = val;
         }
     }
 }
@@ -183,7 +204,8 @@ void ks_modup(const Context &c, const uint64_t *c2, size_t Ql, uint64_t *d, hipS
     Scratch tcoef(Ql * N * sizeof(uint64_t), s);
     SHM_HIP(hipMemcpyAsync(tcoef.as<uint64_t>(), c2, Ql * N * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
     ntt_inverse(c, tcoef.as<uint64_t>(), 1, Ql, 0, s);
-    k_modup_bconv<<<grid_for(beta * QlP * N), kBlock, 0, s>>>(
+    if (alpha > (size_t)kMaxAlpha) throw InvalidArgument("at most 16 special primes");
+    k_modup_bconv<<<grid_for(beta * N), kBlock, 0, s>>>(
         c2, tcoef.as<uint64_t>(), d, c.mods(), L.hat_inv.as<uint64_t>(), L.hat_inv_s.as<uint64_t>(),
         L.hat_mod.as<uint64_t>(), Ql, c.nQ, alpha, beta, N, rounded ? 1 : 0);
     SHM_HIP(hipGetLastError());
@@ -194,12 +216,12 @@ void ks_modup(const Context &c, const uint64_t *c2, size_t Ql, uint64_t *d, hipS
 }
 
 void ks_inner_multi(const Context &c, const uint64_t *d, size_t Ql, size_t ndig, const KeyRefs &kr, size_t nout,
-                    bool local, uint64_t *cx, hipStream_t s) {
+                    bool local, uint64_t *cx, hipStream_t s, const uint32_t *perm, bool accumulate) {
     if (nout == 0 || nout > (size_t)kMaxKeys) throw InvalidArgument("ks_inner_multi: 1..34 outputs");
     if (ndig == 0 || ndig > (size_t)kMaxDigits) throw InvalidArgument("ks_inner_multi: 1..16 digits");
     const size_t QlP = Ql + c.nP;
     k_ks_inner_multi<<<grid_for(QlP * c.N), kBlock, 0, s>>>(d, kr, (int)nout, cx, c.mods(), Ql, c.nQ, c.nP, ndig,
-                                                           c.N, local ? 1 : 0);
+                                                           c.N, local ? 1 : 0, perm, accumulate ? 1 : 0);
     SHM_HIP(hipGetLastError());
 }
 
@@ -209,11 +231,10 @@ void ks_moddown(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, uint64
     if (npolys == 0 || npolys > 64) throw InvalidArgument("ks_moddown: 1..64 polynomials");
     Scratch delta(npolys * Ql * N * sizeof(uint64_t), s);
     ntt_inverse_qlp(c, cx, Ql, Ql, QlP, npolys, QlP * N, s);
-    for (size_t p = 0; p < npolys; p++)
-        k_moddown_bconv<<<grid_for(Ql * N), kBlock, 0, s>>>(
-            cx + p * QlP * N + Ql * N, delta.as<uint64_t>() + p * Ql * N, c.mods(), c.phat_inv.as<uint64_t>(),
-            c.phat_inv_s.as<uint64_t>(), c.phat_mod.as<uint64_t>(), c.pmod.as<uint64_t>(), Ql, c.nQ, nP, N,
-            rounded ? 1 : 0);
+    if (nP > (size_t)kMaxAlpha) throw InvalidArgument("at most 16 special primes");
+    k_moddown_bconv<<<grid_for(npolys * N), kBlock, 0, s>>>(
+        cx, delta.as<uint64_t>(), c.mods(), c.phat_inv.as<uint64_t>(), c.phat_inv_s.as<uint64_t>(),
+        c.phat_mod.as<uint64_t>(), c.pmod.as<uint64_t>(), Ql, c.nQ, nP, N, npolys, rounded ? 1 : 0);
     SHM_HIP(hipGetLastError());
     ntt_forward(c, delta.as<uint64_t>(), npolys, Ql, 0, s);
     k_moddown_combine<<<grid_for(npolys * Ql * N), kBlock, 0, s>>>(cx, delta.as<uint64_t>(), out, c.mods(),

@@ -93,6 +93,68 @@ void rotate_internal(Context &c, Ciphertext &ct, int step, const GaloisKeys &gk,
         if ((size_t)std::abs(t) != (c.N >> 1)) rotate_internal(c, ct, t, gk, s, rounded);
 }
 
+void rotate_hoisted(Context &c, const Ciphertext &ct, const std::vector<int> &steps, const GaloisKeys &gk,
+                    const std::vector<Ciphertext *> &outs, hipStream_t s, bool rounded) {
+    if (ct.size != 2) throw InvalidArgument("encrypted size must be 2");
+    if (steps.size() != outs.size()) throw InvalidArgument("one output per step");
+    const size_t Ql = ct.Ql(), N = c.N, QlP = Ql + c.nP, QP = c.nQP(), beta = c.level(Ql).beta;
+    Scratch d(beta * QlP * N * sizeof(uint64_t), s), cx(2 * QlP * N * sizeof(uint64_t), s);
+    bool have_modup = false;
+    for (size_t i = 0; i < steps.size(); i++) {
+        Ciphertext &o = *outs[i];
+        const uint32_t elt = c.elt_from_step(steps[i]);
+        auto it = gk.keys.find(elt);
+        if (steps[i] == 0 || it == gk.keys.end()) {
+            ct_clone(o, ct, s);
+            rotate_internal(c, o, steps[i], gk, s, rounded);
+            continue;
+        }
+        if (!have_modup) {
+            ks_modup(c, ct.poly(1), Ql, d.as<uint64_t>(), s, rounded);
+            have_modup = true;
+        }
+        const uint32_t *table = c.galois_table(elt, s);
+        KeyRefs kr{};
+        kr.base[0] = it->second->k.as<uint64_t>();
+        kr.stride[0] = 2 * QP * N;
+        kr.base[1] = kr.base[0] + QP * N;
+        kr.stride[1] = 2 * QP * N;
+        ks_inner_multi(c, d.as<uint64_t>(), Ql, beta, kr, 2, false, cx.as<uint64_t>(), s, table);
+        o.ctx = &c;
+        o.resize(2, ct.chain, s);
+        o.scale = ct.scale;
+        o.ntt = true;
+        launch_permute(c, ct.poly(0), o.poly(0), table, Ql, s);
+        ks_moddown(c, cx.as<uint64_t>(), 2, Ql, o.data(), Ql * N, 1, s, rounded);
+    }
+}
+
+void hoisting_inplace(Context &c, Ciphertext &ct, const std::vector<int> &steps, const GaloisKeys &gk,
+                      hipStream_t s) {
+    if (ct.size != 2) throw InvalidArgument("ciphertext size must be 2");
+    if (steps.empty()) throw InvalidArgument("steps is empty");
+    const size_t Ql = ct.Ql(), N = c.N, QlP = Ql + c.nP, QP = c.nQP(), beta = c.level(Ql).beta;
+    Scratch d(beta * QlP * N * sizeof(uint64_t), s), cx(2 * QlP * N * sizeof(uint64_t), s);
+    Scratch c0(Ql * N * sizeof(uint64_t), s), t0(Ql * N * sizeof(uint64_t), s);
+    ks_modup(c, ct.poly(1), Ql, d.as<uint64_t>(), s);
+    for (size_t i = 0; i < steps.size(); i++) {
+        const uint32_t elt = c.elt_from_step(steps[i]);
+        auto it = gk.keys.find(elt);
+        if (it == gk.keys.end()) throw InvalidArgument("Galois elt not present");
+        const uint32_t *table = c.galois_table(elt, s);
+        KeyRefs kr{};
+        kr.base[0] = it->second->k.as<uint64_t>();
+        kr.stride[0] = 2 * QP * N;
+        kr.base[1] = kr.base[0] + QP * N;
+        kr.stride[1] = 2 * QP * N;
+        ks_inner_multi(c, d.as<uint64_t>(), Ql, beta, kr, 2, false, cx.as<uint64_t>(), s, table, i > 0);
+        launch_permute(c, ct.poly(0), i ? t0.as<uint64_t>() : c0.as<uint64_t>(), table, Ql, s);
+        if (i) launch_add(c, c0.as<uint64_t>(), t0.as<uint64_t>(), Ql, 1, s);
+    }
+    SHM_HIP(hipMemcpyAsync(ct.poly(0), c0.as<uint64_t>(), Ql * N * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    ks_moddown(c, cx.as<uint64_t>(), 2, Ql, ct.data(), Ql * N, 1, s);
+}
+
 void rescale_inplace(Context &c, Ciphertext &ct, hipStream_t s, bool rounded) {
     const size_t Ql = ct.Ql();
     if (Ql < 2) throw LogicError("end of modulus switching chain reached");

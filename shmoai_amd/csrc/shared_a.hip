@@ -332,6 +332,48 @@ void shared_apply_galois(SharedAKeys &K, SharedABatch &x, uint32_t elt, hipStrea
     x.ct = std::move(rot);
 }
 
+void shared_rotate_hoisted(SharedAKeys &K, const Ciphertext &x, const std::vector<int> &steps,
+                           const std::vector<Ciphertext *> &outs, hipStream_t s, bool rounded) {
+    Context &c = *K.ctx;
+    const size_t n = K.n, N = c.N, Ql = x.Ql(), QlP = Ql + c.nP, QP = c.nQP(), beta = c.level(Ql).beta;
+    if (x.size != n + 1) throw InvalidArgument("shared_a_rotate: batch size does not match the Galois key");
+    Scratch d(beta * QlP * N * sizeof(uint64_t), s), cx((n + 1) * QlP * N * sizeof(uint64_t), s);
+    bool have_modup = false;
+    for (size_t t = 0; t < steps.size(); t++) {
+        Ciphertext &o = *outs[t];
+        const uint32_t elt = steps[t] == 0 ? (uint32_t)(2 * N - 1) : c.elt_from_step(steps[t]);
+        auto it = K.galois.find(elt);
+        if (it == K.galois.end()) {
+            SharedABatch tmp;
+            tmp.n = n;
+            ct_clone(tmp.ct, x, s);
+            shared_rotate(K, tmp, steps[t], s, rounded);
+            o = std::move(tmp.ct);
+            continue;
+        }
+        if (!have_modup) {
+            ks_modup(c, x.poly(n), Ql, d.as<uint64_t>(), s, rounded);
+            have_modup = true;
+        }
+        const uint32_t *table = c.galois_table(elt, s);
+        const SharedGaloisKey &key = *it->second;
+        KeyRefs kr{};
+        for (size_t i = 0; i < n; i++) {
+            kr.base[i] = key.B.as<uint64_t>() + i * QP * N;
+            kr.stride[i] = n * QP * N;
+        }
+        kr.base[n] = key.A.as<uint64_t>();
+        kr.stride[n] = QP * N;
+        ks_inner_multi(c, d.as<uint64_t>(), Ql, beta, kr, n + 1, false, cx.as<uint64_t>(), s, table);
+        o.ctx = &c;
+        o.resize(n + 1, x.chain, s);
+        o.scale = x.scale;
+        o.ntt = true;
+        launch_permute(c, x.data(), o.data(), table, n * Ql, s);
+        ks_moddown(c, cx.as<uint64_t>(), n + 1, Ql, o.data(), Ql * N, (1ull << n) - 1, s, rounded);
+    }
+}
+
 // shared_a_rotate by a slot step (0 = conjugation); a missing step is split in
 // its NAF terms like rotate_internal.
 void shared_rotate(SharedAKeys &K, SharedABatch &x, int step, hipStream_t s, bool rounded) {
@@ -361,14 +403,19 @@ void shared_rotate(SharedAKeys &K, SharedABatch &x, int step, hipStream_t s, boo
 // ---- batched bootstrapping -------------------------------------------------------------------
 
 namespace {
-RotateFn shared_rotation(SharedAKeys &K, hipStream_t s) {
-    return [&K, s](Ciphertext &x, int step) {
+Rotator shared_rotation(SharedAKeys &K, hipStream_t s) {
+    Rotator r;
+    r.rot = [&K, s](Ciphertext &x, int step) {
         SharedABatch tmp;
         tmp.n = K.n;
         tmp.ct = std::move(x);
         shared_rotate(K, tmp, step, s, true);
         x = std::move(tmp.ct);
     };
+    r.hoist = [&K, s](const Ciphertext &x, const std::vector<int> &steps, const std::vector<Ciphertext *> &outs) {
+        shared_rotate_hoisted(K, x, steps, outs, s, true);
+    };
+    return r;
 }
 }  // namespace
 

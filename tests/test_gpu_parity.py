@@ -118,3 +118,39 @@ def test_encrypt_decrypt_encode_bitexact(c1):
     re_ref, _ = ref.decode(dec_ref, 8, 2.0 ** 40)
     assert np.max(np.abs(out - re_ref)) <= 1e-12
     assert np.max(np.abs(out - v)) < 1e-6
+
+
+@pytest.mark.parametrize("bits,n_special,Ql", [(C1_BITS, 1, 8), ([50] * 8 + [60] * 3, 3, 6)])
+def test_hoisting_bitexact(oracle_lib, bits, n_special, Ql):
+    """evaluate.cu:1727-1920 hoisting_inplace: sum of rotations, one ModUp of c1,
+    permuted digits per step, summed inner products, one ModDown."""
+    import shmoai_amd as shm
+    ctx = shm.Context(13, bits, n_special)
+    ref = oracle_lib.Ref(13, bits, n_special)
+    rng = np.random.default_rng(8)
+    N, steps = ref.N, [1, 2, -3]
+    x = _rand_limbs(ref, rng, 2, Ql)
+    sk = shm.SecretKey(ctx, seed=1, hamming_weight=64)
+    s_ref = ref.sk_ntt(ref.sample_ternary(1, 64))
+    gk = sk.create_galois_keys_from_steps(steps, seed=9)
+    ct = shm.Ciphertext(ctx, 2, ctx.chain(Ql)).load(x)
+    shm.Evaluator(ctx).hoisting_inplace(ct, gk, steps)
+    # oracle composition
+    QlP = Ql + ref.nP
+    qs = np.array([int(ref.primes[k]) for k in ref.pidx(Ql, True)], dtype=np.uint64)[:, None]
+    qQ = qs[:Ql]
+    c0, c1 = x[:Ql * N], x[Ql * N:]
+    d = ref.modup(c1, Ql)
+    acc = np.zeros((2, QlP, N), np.uint64)
+    c0sum = np.zeros((Ql, N), np.uint64)
+    for st in steps:
+        elt = ref.elt(st)
+        key = ref.gen_galois_key(s_ref, elt, ref.splitmix64(9, elt))
+        dp = ref.apply_galois(d, elt, ref.beta(Ql) * QlP)
+        cx = ref.ks_inner(dp, key, Ql).reshape(2, QlP, N)
+        acc = (acc + cx) % qs
+        c0sum = (c0sum + ref.apply_galois(c0, elt, Ql).reshape(Ql, N)) % qQ
+    k0 = ref.moddown(acc[0].ravel(), Ql).reshape(Ql, N)
+    k1 = ref.moddown(acc[1].ravel(), Ql)
+    exp = np.concatenate([((c0sum + k0) % qQ).ravel(), k1])
+    assert np.array_equal(ct.numpy(), exp)
