@@ -6,17 +6,26 @@
 //   evaluation at psi^(2 brv(j) + 1) in slot j, fully reduced to [0, q).
 //   inverse Gentleman-Sande with itw[brv(k)] = psi^-k, then x N^-1.
 //
-// Decomposition: N = N1 * N2. The first log N1 stages only pair elements whose
-// indices differ by multiples of N2, so each "column" {c + N2 r} is an
-// independent N1-point transform (phase A); the last log N2 stages act on
-// contiguous blocks of N2 elements (phase B). Each phase stages a 4096-element
-// tile in LDS (transposed with one word of padding so the 64 lanes hit distinct
-// banks), then runs rounds of up to 4 butterfly stages in registers (radix-16
+// Decomposition: N = N1 * N2 with N2 = min(4096, N / 16). The first log N1
+// stages (4 or 5) only pair elements whose indices differ by multiples of N2,
+// so each "column" {c + N2 r} is an independent N1-point transform: phase A
+// gives one column to each thread and runs it in registers. The last log N2
+// stages act on contiguous blocks of N2 elements (phase B): a 4096-word tile
+// is staged in LDS, then rounds of up to 4 stages run in registers (radix-16
 // units). A sub-transform whose first-stage group id is G uses twiddle
 // tw[(G << u) + i'] at its local stage u for local group i'.
 //
-// Lazy ranges (Harvey): forward keeps [0, 4q) between stages, inverse [0, 2q);
-// primes are < 2^61 so nothing wraps.
+// Two arithmetic paths, chosen per launch:
+//  - FP64 (every prime of the launch < 2^51): residues are signed doubles
+//    inside the transform and the butterflies use exact FMA product splitting (f_mul,
+//    f_red in arith.h). CDNA4 issues FP64 FMA at full rate, while the 64-bit
+//    integer products of the Shoup path lower to quarter-rate 32-bit multiply
+//    chains, so this path needs about a third of the VALU cycles.
+//  - integer Shoup (any prime < 2^61): lazy ranges (Harvey), forward keeps
+//    [0, 4q) between stages, inverse [0, 2q); nothing wraps.
+// Both produce canonical residues, bit-identical to each other.
+#include <algorithm>
+
 #include "engine.h"
 
 namespace shm {
@@ -46,10 +55,27 @@ namespace {
 constexpr int kTile = 4096;
 constexpr int kThreads = 256;
 
+// Per-prime constants as the butterflies see them.
+struct IntK {
+    uint64_t q;
+};
+struct FpK {
+    double q, qinv;
+};
+
+// Twiddle and constant tables of one launch (forward or inverse set).
+struct NttTables {
+    const DevMod *mods;
+    const uint64_t *tw, *tws;   // integer twiddles and their Shoup quotients
+    const uint64_t *ninv, *ninv_s;
+    const double *ftw, *ftwq;   // FP64 path: signed twiddles and twiddle / q
+    const FpMod *fmods;
+};
+
 template <int R>
 __device__ __forceinline__ void ct_unit(uint64_t *x, uint32_t G, const uint64_t *__restrict__ tw,
-                                        const uint64_t *__restrict__ tws, uint64_t q) {
-    const uint64_t q2 = q << 1;
+                                        const uint64_t *__restrict__ tws, IntK k) {
+    const uint64_t q = k.q, q2 = q << 1;
 #pragma unroll
     for (int u = 0; u < R; u++) {
         const int h = 1 << (R - 1 - u);
@@ -70,8 +96,8 @@ __device__ __forceinline__ void ct_unit(uint64_t *x, uint32_t G, const uint64_t 
 
 template <int R>
 __device__ __forceinline__ void gs_unit(uint64_t *x, uint32_t G, const uint64_t *__restrict__ tw,
-                                        const uint64_t *__restrict__ tws, uint64_t q) {
-    const uint64_t q2 = q << 1;
+                                        const uint64_t *__restrict__ tws, IntK k) {
+    const uint64_t q = k.q, q2 = q << 1;
 #pragma unroll
     for (int u = R - 1; u >= 0; u--) {
         const int h = 1 << (R - 1 - u);
@@ -90,6 +116,61 @@ __device__ __forceinline__ void gs_unit(uint64_t *x, uint32_t G, const uint64_t 
     }
 }
 
+// FP64 butterflies on signed residues (q < 2^51). Forward: a is brought to
+// |a| <= q/2 before a +- t, and |t| <= q/2 + q |b| 2^-53, so the bound B of
+// the values obeys B' <= q (1 + B 2^-53) + small: from inputs below 4q it
+// stays below 2q (< 2^52). Inverse: a + b is reduced, (a - b) w is a fresh
+// f_mul, so values stay below q.
+template <int R>
+__device__ __forceinline__ void ct_unit(double *x, uint32_t G, const double *__restrict__ tw,
+                                        const double *__restrict__ twq, FpK k) {
+#pragma unroll
+    for (int u = 0; u < R; u++) {
+        const int h = 1 << (R - 1 - u);
+#pragma unroll
+        for (int l = 0; l < (1 << R); l++) {
+            if ((l & h) == 0) {
+                const uint32_t ti = (G << u) + (uint32_t)(l >> (R - u));
+                const double a = f_red(x[l], k.q, k.qinv);
+                const double t = f_mul(x[l + h], tw[ti], twq[ti], k.q);
+                x[l] = a + t;
+                x[l + h] = a - t;
+            }
+        }
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void gs_unit(double *x, uint32_t G, const double *__restrict__ tw,
+                                        const double *__restrict__ twq, FpK k) {
+#pragma unroll
+    for (int u = R - 1; u >= 0; u--) {
+        const int h = 1 << (R - 1 - u);
+#pragma unroll
+        for (int l = 0; l < (1 << R); l++) {
+            if ((l & h) == 0) {
+                const uint32_t ti = (G << u) + (uint32_t)(l >> (R - u));
+                const double a = x[l], b = x[l + h];
+                x[l] = f_red(a + b, k.q, k.qinv);
+                x[l + h] = f_mul(a - b, tw[ti], twq[ti], k.q);
+            }
+        }
+    }
+}
+
+// LDS layout of a tile: rows of T words. Several rows (T < 4096) get one word
+// of padding each so lanes walking down a column hit distinct banks; within a
+// row of at least 256 words the low 4 index bits are XOR-ed with bits 4..7, so
+// the last round (lanes 16 words apart) is spread over the banks too.
+template <int T>
+__host__ __device__ constexpr int row_stride() {
+    return T >= kTile ? T : T + 1;
+}
+template <int T>
+__device__ __forceinline__ int swz(int r) {
+    return T >= 256 ? (r ^ ((r >> 4) & 15)) : r;
+}
+
 // Round split of LOGT stages into chunks of at most 4.
 __host__ __device__ constexpr int n_rounds(int logt) { return (logt + 3) / 4; }
 __host__ __device__ constexpr int round_len(int logt, int r) {
@@ -105,13 +186,12 @@ __host__ __device__ constexpr int round_start(int logt, int r) {
 // One round over the LDS tile: SUBS independent sub-transforms of size 2^LOGT
 // (row stride TP words). Sub-transform `sub` has first-stage group id
 // gbase(sub) at local stage 0.
-template <int LOGT, int R, bool FWD>
-__device__ __forceinline__ void lds_round(uint64_t *lds, int s0, uint32_t gmul, uint32_t gadd_step, uint32_t gadd0,
-                                          const uint64_t *__restrict__ tw, const uint64_t *__restrict__ tws,
-                                          uint64_t q) {
+template <int LOGT, int R, bool FWD, typename E, typename K>
+__device__ __forceinline__ void lds_round(E *lds, int s0, uint32_t gmul, uint32_t gadd_step, uint32_t gadd0,
+                                          const E *__restrict__ tw, const E *__restrict__ tws, K k) {
     constexpr int T = 1 << LOGT;
     constexpr int SUBS = kTile / T;
-    constexpr int TP = T + 1;
+    constexpr int TP = row_stride<T>();
     const int T0 = T >> s0;          // size of the sub-transforms at this round
     const int ups = T0 >> R;         // units per sub-transform
     const int units = (T >> R) * SUBS;
@@ -122,32 +202,33 @@ __device__ __forceinline__ void lds_round(uint64_t *lds, int s0, uint32_t gmul, 
         const int o = w2 - kk * ups;
         // group id of the whole phase for this sub-transform, shifted by s0
         const uint32_t G = ((gmul + gadd0 + gadd_step * (uint32_t)sub) << s0) + (uint32_t)kk;
-        uint64_t *p = lds + sub * TP + kk * T0 + o;
-        uint64_t x[1 << R];
+        E *row = lds + sub * TP;
+        const int i0 = kk * T0 + o;
+        E x[1 << R];
 #pragma unroll
-        for (int l = 0; l < (1 << R); l++) x[l] = p[l * ups];
+        for (int l = 0; l < (1 << R); l++) x[l] = row[swz<T>(i0 + l * ups)];
         if (FWD)
-            ct_unit<R>(x, G, tw, tws, q);
+            ct_unit<R>(x, G, tw, tws, k);
         else
-            gs_unit<R>(x, G, tw, tws, q);
+            gs_unit<R>(x, G, tw, tws, k);
 #pragma unroll
-        for (int l = 0; l < (1 << R); l++) p[l * ups] = x[l];
+        for (int l = 0; l < (1 << R); l++) row[swz<T>(i0 + l * ups)] = x[l];
     }
 }
 
-template <int LOGT, bool FWD>
-__device__ __forceinline__ void run_rounds(uint64_t *lds, uint32_t gmul, uint32_t gadd_step, uint32_t gadd0,
-                                           const uint64_t *tw, const uint64_t *tws, uint64_t q) {
+template <int LOGT, bool FWD, typename E, typename K>
+__device__ __forceinline__ void run_rounds(E *lds, uint32_t gmul, uint32_t gadd_step, uint32_t gadd0, const E *tw,
+                                           const E *tws, K k) {
     constexpr int NR = n_rounds(LOGT);
     if (FWD) {
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             const int R = round_len(LOGT, r), s0 = round_start(LOGT, r);
             switch (R) {
-                case 4: lds_round<LOGT, 4, true>(lds, s0, gmul, gadd_step, gadd0, tw, tws, q); break;
-                case 3: lds_round<LOGT, 3, true>(lds, s0, gmul, gadd_step, gadd0, tw, tws, q); break;
-                case 2: lds_round<LOGT, 2, true>(lds, s0, gmul, gadd_step, gadd0, tw, tws, q); break;
-                default: lds_round<LOGT, 1, true>(lds, s0, gmul, gadd_step, gadd0, tw, tws, q); break;
+                case 4: lds_round<LOGT, 4, true>(lds, s0, gmul, gadd_step, gadd0, tw, tws, k); break;
+                case 3: lds_round<LOGT, 3, true>(lds, s0, gmul, gadd_step, gadd0, tw, tws, k); break;
+                case 2: lds_round<LOGT, 2, true>(lds, s0, gmul, gadd_step, gadd0, tw, tws, k); break;
+                default: lds_round<LOGT, 1, true>(lds, s0, gmul, gadd_step, gadd0, tw, tws, k); break;
             }
             __syncthreads();
         }
@@ -156,10 +237,10 @@ __device__ __forceinline__ void run_rounds(uint64_t *lds, uint32_t gmul, uint32_
         for (int r = NR - 1; r >= 0; r--) {
             const int R = round_len(LOGT, r), s0 = round_start(LOGT, r);
             switch (R) {
-                case 4: lds_round<LOGT, 4, false>(lds, s0, gmul, gadd_step, gadd0, tw, tws, q); break;
-                case 3: lds_round<LOGT, 3, false>(lds, s0, gmul, gadd_step, gadd0, tw, tws, q); break;
-                case 2: lds_round<LOGT, 2, false>(lds, s0, gmul, gadd_step, gadd0, tw, tws, q); break;
-                default: lds_round<LOGT, 1, false>(lds, s0, gmul, gadd_step, gadd0, tw, tws, q); break;
+                case 4: lds_round<LOGT, 4, false>(lds, s0, gmul, gadd_step, gadd0, tw, tws, k); break;
+                case 3: lds_round<LOGT, 3, false>(lds, s0, gmul, gadd_step, gadd0, tw, tws, k); break;
+                case 2: lds_round<LOGT, 2, false>(lds, s0, gmul, gadd_step, gadd0, tw, tws, k); break;
+                default: lds_round<LOGT, 1, false>(lds, s0, gmul, gadd_step, gadd0, tw, tws, k); break;
             }
             __syncthreads();
         }
@@ -188,161 +269,248 @@ struct NttIO {
     uint64_t center;          // IO_REDUCE: 0, or h = q_last >> 1: values above h stand for v - q_last
 };
 
-// Phase A (columns, first log N1 stages). LOGT = log N1.
-// FWD: forward stages 0..logN1-1 (output stays lazy [0,4q)).
+// Between the two phases the FP64 path keeps its signed doubles in `data`
+// (bit patterns in the uint64 words); every phase that reads or writes the
+// caller's residues converts at that edge, so both paths present canonical
+// uint64 residues outside the transform.
+
+// Phase A (columns, first log N1 stages, log N1 <= 5). Each thread owns one
+// column {c + N2 r} and runs all its stages in registers: no LDS, no barrier;
+// the 256 threads of a block read and write 256 consecutive words per row.
+// FWD: forward stages 0..logN1-1 (output stays lazy).
 // !FWD: inverse stages logN1-1..0, then x N^-1, output canonical.
-template <int LOGT, bool FWD, int IO>
-__global__ __launch_bounds__(kThreads) void k_ntt_cols(uint64_t *__restrict__ data, LimbMap mp,
-                                                       const DevMod *__restrict__ mods,
-                                                       const uint64_t *__restrict__ tw_all,
-                                                       const uint64_t *__restrict__ tws_all,
-                                                       const uint64_t *__restrict__ ninv,
-                                                       const uint64_t *__restrict__ ninv_s, int logN, NttIO io) {
+template <int LOGT, bool FWD, int IO, bool FP>
+__global__ __launch_bounds__(kThreads) void k_ntt_cols(uint64_t *__restrict__ data, LimbMap mp, NttTables tb,
+                                                       int logN, NttIO io) {
     constexpr int T = 1 << LOGT;
-    constexpr int COLS = kTile / T;
-    constexpr int TP = T + 1;
-    __shared__ uint64_t lds[COLS * TP];
     const size_t N = size_t(1) << logN;
-    const int N2 = (int)(N >> LOGT);
-    const int groups = N2 / COLS;
+    const size_t N2 = N >> LOGT;
+    const uint32_t groups = (uint32_t)(N2 / kThreads);
     const uint32_t z = blockIdx.x / groups;
-    const int c0 = (blockIdx.x % groups) * COLS;
+    const size_t c = (size_t)(blockIdx.x % groups) * kThreads + threadIdx.x;
     uint32_t poly, k, prime;
     limb_of(mp, z, poly, k, prime);
-    uint64_t *base = data + poly * mp.poly_stride + (size_t)k * N + c0;
-    const uint64_t q = mods[prime].q;
-    const uint64_t *tw = tw_all + (size_t)prime * N;
-    const uint64_t *tws = tws_all + (size_t)prime * N;
-    if (IO == IO_REDUCE) {
-        const DevMod m = mods[prime];
-        const uint64_t *src = io.src + (size_t)poly * N + c0;
-        for (int i = threadIdx.x; i < kTile; i += kThreads) {
-            const int c = i % COLS, r = i / COLS;
-            const uint64_t v = src[(size_t)r * N2 + c];
-            uint64_t x = reduce64(v, m);
-            if (io.center && v > io.center) x = sub_mod(x, reduce64(2 * io.center + 1, m), m.q);
-            lds[c * TP + r] = x;
+    uint64_t *base = data + poly * mp.poly_stride + (size_t)k * N + c;
+    // every column's group id at stage 0 is 1 (phase A starts at s = 0)
+    if constexpr (FP) {
+        const FpMod fm = tb.fmods[prime];
+        const FpK K{fm.q, fm.qinv};
+        double x[T];
+        if (!FWD) {
+#pragma unroll
+            for (int r = 0; r < T; r++) x[r] = __longlong_as_double((long long)base[r * N2]);
+        } else if (IO == IO_REDUCE) {
+            const uint64_t *src = io.src + (size_t)poly * N + c;
+            const double qlast = (double)(2 * io.center + 1);
+#pragma unroll
+            for (int r = 0; r < T; r++) {
+                const uint64_t v = src[r * N2];
+                double y = f_of_u64(v);
+                if (io.center && v > io.center) y -= qlast;
+                x[r] = f_red(y, K.q, K.qinv);
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < T; r++) x[r] = f_of_u64(base[r * N2]);
+        }
+        const double *tw = tb.ftw + (size_t)prime * N, *twq = tb.ftwq + (size_t)prime * N;
+        if (FWD) {
+            ct_unit<LOGT>(x, 1u, tw, twq, K);
+#pragma unroll
+            for (int r = 0; r < T; r++) base[r * N2] = (uint64_t)__double_as_longlong(x[r]);
+        } else {
+            gs_unit<LOGT>(x, 1u, tw, twq, K);
+#pragma unroll
+            for (int r = 0; r < T; r++)
+                base[r * N2] = f_to_canonical(f_mul(x[r], fm.ninv, fm.ninv_q, fm.q), fm.q, fm.qinv);
         }
     } else {
-        for (int i = threadIdx.x; i < kTile; i += kThreads) {
-            const int c = i % COLS, r = i / COLS;
-            lds[c * TP + r] = base[(size_t)r * N2 + c];
+        const uint64_t q = tb.mods[prime].q;
+        uint64_t x[T];
+        if (IO == IO_REDUCE) {
+            const DevMod m = tb.mods[prime];
+            const uint64_t *src = io.src + (size_t)poly * N + c;
+#pragma unroll
+            for (int r = 0; r < T; r++) {
+                const uint64_t v = src[r * N2];
+                uint64_t y = reduce64(v, m);
+                if (io.center && v > io.center) y = sub_mod(y, reduce64(2 * io.center + 1, m), m.q);
+                x[r] = y;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < T; r++) x[r] = base[r * N2];
         }
-    }
-    __syncthreads();
-    // group id of every column at local stage 0 is 1 (phase A starts at s = 0)
-    run_rounds<LOGT, FWD>(lds, 1u, 0u, 0u, tw, tws, q);
-    if (FWD) {
-        for (int i = threadIdx.x; i < kTile; i += kThreads) {
-            const int c = i % COLS, r = i / COLS;
-            base[(size_t)r * N2 + c] = lds[c * TP + r];
-        }
-    } else {
-        const uint64_t ni = ninv[prime], nis = ninv_s[prime];
-        for (int i = threadIdx.x; i < kTile; i += kThreads) {
-            const int c = i % COLS, r = i / COLS;
-            base[(size_t)r * N2 + c] = mul_shoup(lds[c * TP + r], ni, nis, q);
+        const uint64_t *tw = tb.tw + (size_t)prime * N, *tws = tb.tws + (size_t)prime * N;
+        if (FWD) {
+            ct_unit<LOGT>(x, 1u, tw, tws, IntK{q});
+#pragma unroll
+            for (int r = 0; r < T; r++) base[r * N2] = x[r];
+        } else {
+            gs_unit<LOGT>(x, 1u, tw, tws, IntK{q});
+            const uint64_t ni = tb.ninv[prime], nis = tb.ninv_s[prime];
+#pragma unroll
+            for (int r = 0; r < T; r++) base[r * N2] = mul_shoup(x[r], ni, nis, q);
         }
     }
 }
 
 // Phase B (contiguous blocks of N2, last log N2 stages). LOGT = log N2.
-// FWD: input lazy [0,4q), output canonical. !FWD: input canonical (< 2q
-// accepted), output lazy [0,2q).
-template <int LOGT, bool FWD, int IO>
-__global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ data, LimbMap mp,
-                                                         const DevMod *__restrict__ mods,
-                                                         const uint64_t *__restrict__ tw_all,
-                                                         const uint64_t *__restrict__ tws_all, int logN, NttIO io) {
+// FWD: input lazy, output canonical. !FWD: input canonical (< 2q accepted),
+// output lazy.
+// Work order: workgroups are dispatched to the 8 XCDs round-robin, so block b
+// runs on XCD b % 8. Each XCD takes a contiguous range of the work index
+// O = (limb * tiles + tile) * n_polys + poly: the polynomials that share one
+// prime and tile (hence one twiddle slice) run back to back on the same L2.
+template <int LOGT, bool FWD, int IO, bool FP>
+__global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ data, LimbMap mp, NttTables tb,
+                                                         int logN, NttIO io, uint32_t n_polys, uint32_t per_xcd) {
     constexpr int T = 1 << LOGT;
     constexpr int BLKS = kTile / T;
-    constexpr int TP = T + 1;
+    constexpr int TP = row_stride<T>();
     __shared__ uint64_t lds[BLKS * TP];
     const size_t N = size_t(1) << logN;
-    const int tiles = (int)(N / kTile);
-    const uint32_t z = blockIdx.x / tiles;
-    const int t0 = (blockIdx.x % tiles) * kTile;
+    const uint32_t tiles = (uint32_t)(N / kTile);
+    const uint32_t O = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+    if (O >= n_polys * mp.n_active * tiles) return;  // whole block, before any barrier
+    const uint32_t pz = O % n_polys, rest = O / n_polys;
+    const uint32_t z = pz * mp.n_active + rest / tiles;
+    const int t0 = (int)(rest % tiles) * kTile;
     uint32_t poly, k, prime;
     limb_of(mp, z, poly, k, prime);
     uint64_t *base = data + poly * mp.poly_stride + (size_t)k * N + t0;
-    const uint64_t q = mods[prime].q;
-    const uint64_t *tw = tw_all + (size_t)prime * N;
-    const uint64_t *tws = tws_all + (size_t)prime * N;
-    for (int i = threadIdx.x; i < kTile; i += kThreads) {
-        const int b = i / T, r = i % T;
-        lds[b * TP + r] = base[i];
-    }
-    __syncthreads();
     // block b (global index t0/T + b) has group id N1 + b at its first stage
     const uint32_t N1 = (uint32_t)(N >> LOGT);
-    run_rounds<LOGT, FWD>(lds, N1, 1u, (uint32_t)(t0 / T), tw, tws, q);
-    if (FWD) {
-        const uint64_t q2 = q << 1;
-        if (IO == IO_RESCALE) {
+    if constexpr (FP) {
+        const FpMod fm = tb.fmods[prime];
+        const FpK K{fm.q, fm.qinv};
+        double *fl = reinterpret_cast<double *>(lds);
+        for (int i = threadIdx.x; i < kTile; i += kThreads) {
+            const int b = i / T, r = i % T;
+            fl[b * TP + swz<T>(r)] = FWD ? __longlong_as_double((long long)base[i]) : f_of_u64(base[i]);
+        }
+        // rescale: the accumulator words of this thread are loaded now and
+        // arrive while the butterflies run
+        constexpr int PER = kTile / kThreads;
+        uint64_t accv[IO == IO_RESCALE ? PER : 1];
+        if (FWD && IO == IO_RESCALE) {
             const uint64_t *acc = io.acc + ((size_t)poly * io.acc_limbs + k) * N + t0;
+#pragma unroll
+            for (int j = 0; j < PER; j++) accv[j] = acc[threadIdx.x + j * kThreads];
+        }
+        __syncthreads();
+        run_rounds<LOGT, FWD>(fl, N1, 1u, (uint32_t)(t0 / T), tb.ftw + (size_t)prime * N,
+                              tb.ftwq + (size_t)prime * N, K);
+        if (FWD && IO == IO_RESCALE) {
             uint64_t *ob = io.outs ? io.outs[poly / io.ppo] : io.out_single;
             uint64_t *out = ob + ((size_t)(poly % io.ppo) * mp.n_active + k) * N + t0;
-            const uint64_t qi = io.qinv[prime], qis = io.qinv_s[prime];
+            const uint64_t qi = io.qinv[prime], qh = (uint64_t)fm.q;
+            const double qiw = (double)(qi > qh / 2 ? (int64_t)qi - (int64_t)qh : (int64_t)qi);
+            const double qiq = qiw / fm.q;
+#pragma unroll
+            for (int j = 0; j < PER; j++) {
+                const int i = threadIdx.x + j * kThreads;
+                const int b = i / T, r = i % T;
+                const double d = f_of_u64(accv[j]) - fl[b * TP + swz<T>(r)];
+                out[i] = f_to_canonical(f_mul(d, qiw, qiq, fm.q), fm.q, fm.qinv);
+            }
+        } else if (FWD) {
             for (int i = threadIdx.x; i < kTile; i += kThreads) {
                 const int b = i / T, r = i % T;
-                uint64_t v = lds[b * TP + r];
-                v = v >= q2 ? v - q2 : v;
-                v = v >= q ? v - q : v;
-                out[i] = mul_shoup(sub_mod(acc[i], v, q), qi, qis, q);
+                base[i] = f_to_canonical(fl[b * TP + swz<T>(r)], fm.q, fm.qinv);
             }
         } else {
             for (int i = threadIdx.x; i < kTile; i += kThreads) {
                 const int b = i / T, r = i % T;
-                uint64_t v = lds[b * TP + r];
-                v = v >= q2 ? v - q2 : v;
-                base[i] = v >= q ? v - q : v;
+                base[i] = (uint64_t)__double_as_longlong(fl[b * TP + swz<T>(r)]);
             }
         }
     } else {
+        const uint64_t q = tb.mods[prime].q;
+        const uint64_t *tw = tb.tw + (size_t)prime * N;
+        const uint64_t *tws = tb.tws + (size_t)prime * N;
         for (int i = threadIdx.x; i < kTile; i += kThreads) {
             const int b = i / T, r = i % T;
-            base[i] = lds[b * TP + r];
+            lds[b * TP + swz<T>(r)] = base[i];
+        }
+        __syncthreads();
+        run_rounds<LOGT, FWD>(lds, N1, 1u, (uint32_t)(t0 / T), tw, tws, IntK{q});
+        if (FWD) {
+            const uint64_t q2 = q << 1;
+            if (IO == IO_RESCALE) {
+                const uint64_t *acc = io.acc + ((size_t)poly * io.acc_limbs + k) * N + t0;
+                uint64_t *ob = io.outs ? io.outs[poly / io.ppo] : io.out_single;
+                uint64_t *out = ob + ((size_t)(poly % io.ppo) * mp.n_active + k) * N + t0;
+                const uint64_t qi = io.qinv[prime], qis = io.qinv_s[prime];
+                for (int i = threadIdx.x; i < kTile; i += kThreads) {
+                    const int b = i / T, r = i % T;
+                    uint64_t v = lds[b * TP + swz<T>(r)];
+                    v = v >= q2 ? v - q2 : v;
+                    v = v >= q ? v - q : v;
+                    out[i] = mul_shoup(sub_mod(acc[i], v, q), qi, qis, q);
+                }
+            } else {
+                for (int i = threadIdx.x; i < kTile; i += kThreads) {
+                    const int b = i / T, r = i % T;
+                    uint64_t v = lds[b * TP + swz<T>(r)];
+                    v = v >= q2 ? v - q2 : v;
+                    base[i] = v >= q ? v - q : v;
+                }
+            }
+        } else {
+            for (int i = threadIdx.x; i < kTile; i += kThreads) {
+                const int b = i / T, r = i % T;
+                base[i] = lds[b * TP + swz<T>(r)];
+            }
         }
     }
 }
 
-template <bool FWD, int IOA, int IOB>
-void launch_ntt(const Context &c, uint64_t *data, const LimbMap &mp, size_t total_limbs, hipStream_t s,
-                const NttIO &io = NttIO{}) {
-    if (total_limbs == 0) return;
+template <bool FP, bool FWD, int IOA, int IOB>
+void launch_ntt_impl(const Context &c, uint64_t *data, const LimbMap &mp, size_t total_limbs, hipStream_t s,
+                     const NttIO &io) {
     const int logN = c.logN;
-    const int log2b = (logN + 1) / 2;  // blocks (contiguous) get the larger half
+    // Phase B takes whole 4096-word tiles (12 stages) and phase A the rest:
+    // with few column stages, phase A moves rows of >= 2 KB contiguous words
+    // (N >> log1 >= 4096 / 2^log1 columns of a tile) instead of short strided
+    // segments.
+    const int log2b = std::min(12, logN - 4);
     const int log1 = logN - log2b;
     const size_t N = c.N;
-    if (N < (size_t)kTile) throw InvalidArgument("NTT requires N >= 4096");
-    const unsigned grid_cols = (unsigned)(total_limbs * ((N >> log1) / (kTile >> log1)));
-    const unsigned grid_blk = (unsigned)(total_limbs * (N / kTile));
-    const DevMod *mods = c.mods();
-    const uint64_t *twa = FWD ? c.tw.as<uint64_t>() : c.itw.as<uint64_t>();
-    const uint64_t *twsa = FWD ? c.tw_s.as<uint64_t>() : c.itw_s.as<uint64_t>();
-#define SHM_COLS(L)                                                                                             \
-    case L:                                                                                                     \
-        k_ntt_cols<L, FWD, IOA><<<grid_cols, kThreads, 0, s>>>(data, mp, mods, twa, twsa, c.ninv.as<uint64_t>(), \
-                                                               c.ninv_s.as<uint64_t>(), logN, io);             \
+    const unsigned grid_cols = (unsigned)(total_limbs * ((N >> log1) / kThreads));
+    const uint32_t n_polys = (uint32_t)(total_limbs / mp.n_active);
+    const uint32_t per_xcd = (uint32_t)((total_limbs * (N / kTile) + 7) / 8);
+    const unsigned grid_blk = 8u * per_xcd;
+    NttTables tb{c.mods(),
+                 FWD ? c.tw.as<uint64_t>() : c.itw.as<uint64_t>(),
+                 FWD ? c.tw_s.as<uint64_t>() : c.itw_s.as<uint64_t>(),
+                 c.ninv.as<uint64_t>(),
+                 c.ninv_s.as<uint64_t>(),
+                 FP ? (FWD ? c.ftw.as<double>() : c.fitw.as<double>()) : nullptr,
+                 FP ? (FWD ? c.ftwq.as<double>() : c.fitwq.as<double>()) : nullptr,
+                 FP ? c.fmod.as<FpMod>() : nullptr};
+#define SHM_COLS(L)                                                                               \
+    case L:                                                                                       \
+        k_ntt_cols<L, FWD, IOA, FP><<<grid_cols, kThreads, 0, s>>>(data, mp, tb, logN, io);       \
         break;
-#define SHM_BLKS(L)                                                                                   \
-    case L:                                                                                           \
-        k_ntt_blocks<L, FWD, IOB><<<grid_blk, kThreads, 0, s>>>(data, mp, mods, twa, twsa, logN, io); \
+#define SHM_BLKS(L)                                                                                          \
+    case L:                                                                                                  \
+        k_ntt_blocks<L, FWD, IOB, FP><<<grid_blk, kThreads, 0, s>>>(data, mp, tb, logN, io, n_polys, per_xcd); \
         break;
     auto cols = [&]() {
         switch (log1) {
-            SHM_COLS(6)
-            SHM_COLS(7)
-            SHM_COLS(8)
+            SHM_COLS(4)
+            SHM_COLS(5)
             default: throw InvalidArgument("unsupported ring degree for NTT");
         }
     };
     auto blks = [&]() {
         switch (log2b) {
-            SHM_BLKS(6)
-            SHM_BLKS(7)
             SHM_BLKS(8)
             SHM_BLKS(9)
+            SHM_BLKS(10)
+            SHM_BLKS(11)
+            SHM_BLKS(12)
             default: throw InvalidArgument("unsupported ring degree for NTT");
         }
     };
@@ -356,6 +524,26 @@ void launch_ntt(const Context &c, uint64_t *data, const LimbMap &mp, size_t tota
         cols();
     }
     SHM_HIP(hipGetLastError());
+}
+
+template <bool FWD, int IOA, int IOB>
+void launch_ntt(const Context &c, uint64_t *data, const LimbMap &mp, size_t total_limbs, hipStream_t s,
+                const NttIO &io = NttIO{}) {
+    if (total_limbs == 0) return;
+    if (c.N < (size_t)kTile) throw InvalidArgument("NTT requires N >= 4096");
+    if (total_limbs % mp.n_active != 0) throw LogicError("NTT limb count is not a multiple of the polynomial size");
+    // FP64 path when every prime this launch touches is below 2^51
+    bool fp = true;
+    for (uint32_t j = 0; j < mp.n_active && fp; j++) {
+        uint32_t k = mp.lo + j;
+        if (k >= mp.skip_lo) k += mp.skip_width;
+        const uint32_t prime = k < mp.Ql ? mp.first_prime + k : mp.nQ + (k - mp.Ql);
+        fp = c.fp_prime.at(prime) != 0;
+    }
+    if (fp)
+        launch_ntt_impl<true, FWD, IOA, IOB>(c, data, mp, total_limbs, s, io);
+    else
+        launch_ntt_impl<false, FWD, IOA, IOB>(c, data, mp, total_limbs, s, io);
 }
 
 }  // namespace

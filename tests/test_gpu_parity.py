@@ -46,6 +46,39 @@ def test_ntt_roundtrip_bitexact(c1):
     assert np.array_equal(ct.numpy(), x)
 
 
+@pytest.mark.parametrize("logN,bits,n_special", [
+    (12, [51, 51, 30, 20, 61], 1),
+    (16, [51, 50, 46, 60], 1),
+    (17, [51, 40, 51], 1),
+])
+def test_ntt_paths_bitexact(oracle_lib, logN, bits, n_special):
+    """Primes below 2^51 take the FP64 butterflies, larger ones the integer
+    Shoup butterflies; both must equal the oracle's NTT bit for bit, including
+    the extreme residues 0 and q - 1 and the three tile splits of N."""
+    import shmoai_amd as shm
+    ctx = shm.Context(logN, bits, n_special)
+    ref = oracle_lib.Ref(logN, bits, n_special)
+    N, nQ, QP = ref.N, len(bits) - n_special, len(bits)
+    rng = np.random.default_rng(logN)
+    for first, n in ((0, nQ), (nQ, QP - nQ)):
+        x = np.empty((2, n, N), np.uint64)
+        for k in range(n):
+            q = int(ref.primes[first + k])
+            x[0, k] = rng.integers(0, q, size=N, dtype=np.uint64)
+            x[1, k] = np.where(np.arange(N) % 3 == 0, q - 1, np.where(np.arange(N) % 3 == 1, 0, q // 2))
+        x = x.ravel()
+        pidx = np.tile(np.arange(first, first + n), 2)
+        ct = shm.Ciphertext(ctx, 2, 1)
+        buf = np.zeros(2 * nQ * N, np.uint64)
+        buf[:x.size] = x
+        ct.load(buf)
+        ctx.ntt_forward(ct.data_ptr(), 2, n, first)
+        got = ct.numpy()[:x.size]
+        assert np.array_equal(got, ref.ntt(x, pidx)), f"forward, primes {first}..{first + n}"
+        ctx.ntt_inverse(ct.data_ptr(), 2, n, first)
+        assert np.array_equal(ct.numpy()[:x.size], x), f"inverse, primes {first}..{first + n}"
+
+
 def test_secret_and_keys_bitexact(c1):
     import shmoai_amd as shm
     ctx, ref = c1
