@@ -23,9 +23,8 @@ COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-result",
 ]
-# The FP64 encoder and the FP64 NTT butterflies must round every product and
-# sum separately (the exact product split relies on it).
-PER_FILE = {"encoder.hip": ["-ffp-contract=off"], "ntt.hip": ["-ffp-contract=off"]}
+# The FP64 encoder must round every product and sum separately.
+PER_FILE = {"encoder.hip": ["-ffp-contract=off"]}
 
 
 def hipcc() -> str:

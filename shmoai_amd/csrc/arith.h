@@ -20,42 +20,6 @@ struct DevMod {
     uint64_t mu_lo;
 };
 
-// FP64 residue arithmetic for primes q < 2^51 (the NTT's fast path).
-// Values are signed doubles holding integers, |v| < 2^53, so each is exact.
-// f_mul: b * w mod q for |b| < 2^53, |w| <= q/2 and wq = fl(w / q). The
-// product is split exactly as h + l (h = fl(b w), l = fma(b, w, -h)); with
-// c = rint(b wq), h - c q is an integer below 2^52 in magnitude, so the fma
-// that forms it is exact, and so is the final sum. |c - b w / q| <= 1/2 +
-// |b| 2^-53, hence |result| <= q/2 + q |b| 2^-53.
-// f_red: a - q rint(a / q), |result| <= q/2 (1 + 2^-50) for |a| <= 4q.
-struct FpMod {
-    double q, qinv;       // q, fl(1 / q)
-    double ninv, ninv_q;  // N^-1 mod q as a signed residue, fl(ninv / q)
-};
-
-__device__ __forceinline__ double f_mul(double b, double w, double wq, double q) {
-    const double h = b * w;
-    const double l = __fma_rn(b, w, -h);
-    const double c = rint(b * wq);
-    return __fma_rn(-c, q, h) + l;
-}
-
-__device__ __forceinline__ double f_red(double a, double q, double qinv) { return __fma_rn(-rint(a * qinv), q, a); }
-
-// u64 below 2^53 -> double (exact)
-__device__ __forceinline__ double f_of_u64(uint64_t x) {
-    return __fma_rn((double)(uint32_t)(x >> 32), 0x1p32, (double)(uint32_t)x);
-}
-
-// |v| <= 4q -> canonical residue in [0, q) as uint64
-__device__ __forceinline__ uint64_t f_to_canonical(double v, double q, double qinv) {
-    v = f_red(v, q, qinv);
-    v = v < 0.0 ? v + q : v;
-    const double hi = floor(v * 0x1p-32);
-    const uint32_t h = (uint32_t)hi, l = (uint32_t)__fma_rn(-hi, 0x1p32, v);
-    return ((uint64_t)h << 32) | l;
-}
-
 __device__ __forceinline__ uint64_t csub(uint64_t x, uint64_t q) { return x >= q ? x - q : x; }
 
 __device__ __forceinline__ uint64_t add_mod(uint64_t a, uint64_t b, uint64_t q) { return csub(a + b, q); }

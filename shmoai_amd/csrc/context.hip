@@ -276,31 +276,6 @@ std::unique_ptr<Context> make_context(int logN, const std::vector<int> &bits, si
     upload(c->itw_s, itws);
     upload(c->ninv, ninv);
     upload(c->ninv_s, ninvs);
-    c->fp_prime.resize(QP);
-    for (size_t p = 0; p < QP; p++) c->fp_prime[p] = c->q[p] < (uint64_t(1) << 51);
-    if (std::find(c->fp_prime.begin(), c->fp_prime.end(), 1) != c->fp_prime.end()) {
-        // residues as signed doubles in (-q/2, q/2], and their quotient by q
-        auto sgn = [](uint64_t v, uint64_t q) { return (double)(v > q / 2 ? (int64_t)v - (int64_t)q : (int64_t)v); };
-        std::vector<double> f(QP * N), fq(QP * N), fi(QP * N), fiq(QP * N);
-        std::vector<FpMod> fm(QP);
-        for (size_t p = 0; p < QP; p++) {
-            const uint64_t q = c->q[p];
-            const double qd = (double)q;
-            for (size_t i = 0; i < N; i++) {
-                f[p * N + i] = sgn(tw[p * N + i], q);
-                fq[p * N + i] = f[p * N + i] / qd;
-                fi[p * N + i] = sgn(itw[p * N + i], q);
-                fiq[p * N + i] = fi[p * N + i] / qd;
-            }
-            const double ni = sgn(ninv[p], q);
-            fm[p] = FpMod{qd, 1.0 / qd, ni, ni / qd};
-        }
-        upload(c->ftw, f);
-        upload(c->ftwq, fq);
-        upload(c->fitw, fi);
-        upload(c->fitwq, fiq);
-        upload(c->fmod, fm);
-    }
 
     // ModDown constants (rns.cu:95-128, 187-190): P mod q_i, P^-1 mod q_i,
     // (P/p_j)^-1 mod p_j, (P/p_j) mod q_i.

@@ -126,11 +126,6 @@ struct Context {
     DevBuf tw, tw_s;        // [QP][N]
     DevBuf itw, itw_s;      // [QP][N]
     DevBuf ninv, ninv_s;    // [QP]
-    // FP64 NTT for primes < 2^51 (fp_prime[p] != 0): signed twiddles and
-    // twiddle / q as double; entries of larger primes are unused
-    std::vector<uint8_t> fp_prime;
-    DevBuf ftw, ftwq, fitw, fitwq;  // [QP][N] double
-    DevBuf fmod;                    // FpMod[QP]
     DevBuf pinv, pinv_s;    // P^-1 mod q_i, i < nQ
     DevBuf pmod, pmod_s;    // P mod q_i, i < nQ
     DevBuf phat_inv, phat_inv_s;  // [nP] (P/p_j)^-1 mod p_j

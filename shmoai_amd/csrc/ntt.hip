@@ -15,15 +15,8 @@
 // units). A sub-transform whose first-stage group id is G uses twiddle
 // tw[(G << u) + i'] at its local stage u for local group i'.
 //
-// Two arithmetic paths, chosen per launch:
-//  - FP64 (every prime of the launch < 2^51): residues are signed doubles
-//    inside the transform and the butterflies use exact FMA product splitting (f_mul,
-//    f_red in arith.h). CDNA4 issues FP64 FMA at full rate, while the 64-bit
-//    integer products of the Shoup path lower to quarter-rate 32-bit multiply
-//    chains, so this path needs about a third of the VALU cycles.
-//  - integer Shoup (any prime < 2^61): lazy ranges (Harvey), forward keeps
-//    [0, 4q) between stages, inverse [0, 2q); nothing wraps.
-// Both produce canonical residues, bit-identical to each other.
+// Lazy ranges (Harvey): forward keeps [0, 4q) between stages, inverse [0, 2q);
+// primes are < 2^61 so nothing wraps. Outputs are canonical residues.
 #include <algorithm>
 
 #include "engine.h"
@@ -55,21 +48,16 @@ namespace {
 constexpr int kTile = 4096;
 constexpr int kThreads = 256;
 
-// Per-prime constants as the butterflies see them.
+// Per-prime constant as the butterflies see it.
 struct IntK {
     uint64_t q;
-};
-struct FpK {
-    double q, qinv;
 };
 
 // Twiddle and constant tables of one launch (forward or inverse set).
 struct NttTables {
     const DevMod *mods;
-    const uint64_t *tw, *tws;   // integer twiddles and their Shoup quotients
+    const uint64_t *tw, *tws;   // twiddles and their Shoup quotients
     const uint64_t *ninv, *ninv_s;
-    const double *ftw, *ftwq;   // FP64 path: signed twiddles and twiddle / q
-    const FpMod *fmods;
 };
 
 template <int R>
@@ -111,48 +99,6 @@ __device__ __forceinline__ void gs_unit(uint64_t *x, uint32_t G, const uint64_t 
                 s = s >= q2 ? s - q2 : s;
                 x[l] = s;
                 x[l + h] = mul_shoup_lazy(a + q2 - b, w, ws, q);
-            }
-        }
-    }
-}
-
-// FP64 butterflies on signed residues (q < 2^51). Forward: a is brought to
-// |a| <= q/2 before a +- t, and |t| <= q/2 + q |b| 2^-53, so the bound B of
-// the values obeys B' <= q (1 + B 2^-53) + small: from inputs below 4q it
-// stays below 2q (< 2^52). Inverse: a + b is reduced, (a - b) w is a fresh
-// f_mul, so values stay below q.
-template <int R>
-__device__ __forceinline__ void ct_unit(double *x, uint32_t G, const double *__restrict__ tw,
-                                        const double *__restrict__ twq, FpK k) {
-#pragma unroll
-    for (int u = 0; u < R; u++) {
-        const int h = 1 << (R - 1 - u);
-#pragma unroll
-        for (int l = 0; l < (1 << R); l++) {
-            if ((l & h) == 0) {
-                const uint32_t ti = (G << u) + (uint32_t)(l >> (R - u));
-                const double a = f_red(x[l], k.q, k.qinv);
-                const double t = f_mul(x[l + h], tw[ti], twq[ti], k.q);
-                x[l] = a + t;
-                x[l + h] = a - t;
-            }
-        }
-    }
-}
-
-template <int R>
-__device__ __forceinline__ void gs_unit(double *x, uint32_t G, const double *__restrict__ tw,
-                                        const double *__restrict__ twq, FpK k) {
-#pragma unroll
-    for (int u = R - 1; u >= 0; u--) {
-        const int h = 1 << (R - 1 - u);
-#pragma unroll
-        for (int l = 0; l < (1 << R); l++) {
-            if ((l & h) == 0) {
-                const uint32_t ti = (G << u) + (uint32_t)(l >> (R - u));
-                const double a = x[l], b = x[l + h];
-                x[l] = f_red(a + b, k.q, k.qinv);
-                x[l + h] = f_mul(a - b, tw[ti], twq[ti], k.q);
             }
         }
     }
@@ -269,17 +215,12 @@ struct NttIO {
     uint64_t center;          // IO_REDUCE: 0, or h = q_last >> 1: values above h stand for v - q_last
 };
 
-// Between the two phases the FP64 path keeps its signed doubles in `data`
-// (bit patterns in the uint64 words); every phase that reads or writes the
-// caller's residues converts at that edge, so both paths present canonical
-// uint64 residues outside the transform.
-
 // Phase A (columns, first log N1 stages, log N1 <= 5). Each thread owns one
 // column {c + N2 r} and runs all its stages in registers: no LDS, no barrier;
 // the 256 threads of a block read and write 256 consecutive words per row.
 // FWD: forward stages 0..logN1-1 (output stays lazy).
 // !FWD: inverse stages logN1-1..0, then x N^-1, output canonical.
-template <int LOGT, bool FWD, int IO, bool FP>
+template <int LOGT, bool FWD, int IO>
 __global__ __launch_bounds__(kThreads) void k_ntt_cols(uint64_t *__restrict__ data, LimbMap mp, NttTables tb,
                                                        int logN, NttIO io) {
     constexpr int T = 1 << LOGT;
@@ -292,39 +233,7 @@ __global__ __launch_bounds__(kThreads) void k_ntt_cols(uint64_t *__restrict__ da
     limb_of(mp, z, poly, k, prime);
     uint64_t *base = data + poly * mp.poly_stride + (size_t)k * N + c;
     // every column's group id at stage 0 is 1 (phase A starts at s = 0)
-    if constexpr (FP) {
-        const FpMod fm = tb.fmods[prime];
-        const FpK K{fm.q, fm.qinv};
-        double x[T];
-        if (!FWD) {
-#pragma unroll
-            for (int r = 0; r < T; r++) x[r] = __longlong_as_double((long long)base[r * N2]);
-        } else if (IO == IO_REDUCE) {
-            const uint64_t *src = io.src + (size_t)poly * N + c;
-            const double qlast = (double)(2 * io.center + 1);
-#pragma unroll
-            for (int r = 0; r < T; r++) {
-                const uint64_t v = src[r * N2];
-                double y = f_of_u64(v);
-                if (io.center && v > io.center) y -= qlast;
-                x[r] = f_red(y, K.q, K.qinv);
-            }
-        } else {
-#pragma unroll
-            for (int r = 0; r < T; r++) x[r] = f_of_u64(base[r * N2]);
-        }
-        const double *tw = tb.ftw + (size_t)prime * N, *twq = tb.ftwq + (size_t)prime * N;
-        if (FWD) {
-            ct_unit<LOGT>(x, 1u, tw, twq, K);
-#pragma unroll
-            for (int r = 0; r < T; r++) base[r * N2] = (uint64_t)__double_as_longlong(x[r]);
-        } else {
-            gs_unit<LOGT>(x, 1u, tw, twq, K);
-#pragma unroll
-            for (int r = 0; r < T; r++)
-                base[r * N2] = f_to_canonical(f_mul(x[r], fm.ninv, fm.ninv_q, fm.q), fm.q, fm.qinv);
-        }
-    } else {
+    {
         const uint64_t q = tb.mods[prime].q;
         uint64_t x[T];
         if (IO == IO_REDUCE) {
@@ -362,7 +271,7 @@ __global__ __launch_bounds__(kThreads) void k_ntt_cols(uint64_t *__restrict__ da
 // runs on XCD b % 8. Each XCD takes a contiguous range of the work index
 // O = (limb * tiles + tile) * n_polys + poly: the polynomials that share one
 // prime and tile (hence one twiddle slice) run back to back on the same L2.
-template <int LOGT, bool FWD, int IO, bool FP>
+template <int LOGT, bool FWD, int IO>
 __global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ data, LimbMap mp, NttTables tb,
                                                          int logN, NttIO io, uint32_t n_polys, uint32_t per_xcd) {
     constexpr int T = 1 << LOGT;
@@ -381,94 +290,57 @@ __global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ 
     uint64_t *base = data + poly * mp.poly_stride + (size_t)k * N + t0;
     // block b (global index t0/T + b) has group id N1 + b at its first stage
     const uint32_t N1 = (uint32_t)(N >> LOGT);
-    if constexpr (FP) {
-        const FpMod fm = tb.fmods[prime];
-        const FpK K{fm.q, fm.qinv};
-        double *fl = reinterpret_cast<double *>(lds);
+    const uint64_t q = tb.mods[prime].q, q2 = q << 1;
+    for (int i = threadIdx.x; i < kTile; i += kThreads) {
+        const int b = i / T, r = i % T;
+        lds[b * TP + swz<T>(r)] = base[i];
+    }
+    // rescale: this thread's accumulator words are loaded now and arrive
+    // while the butterflies run
+    constexpr int PER = kTile / kThreads;
+    uint64_t accv[IO == IO_RESCALE ? PER : 1];
+    if (FWD && IO == IO_RESCALE) {
+        const uint64_t *acc = io.acc + ((size_t)poly * io.acc_limbs + k) * N + t0;
+#pragma unroll
+        for (int j = 0; j < PER; j++) accv[j] = acc[threadIdx.x + j * kThreads];
+    }
+    __syncthreads();
+    run_rounds<LOGT, FWD>(lds, N1, 1u, (uint32_t)(t0 / T), tb.tw + (size_t)prime * N, tb.tws + (size_t)prime * N,
+                          IntK{q});
+    if (FWD && IO == IO_RESCALE) {
+        uint64_t *ob = io.outs ? io.outs[poly / io.ppo] : io.out_single;
+        uint64_t *out = ob + ((size_t)(poly % io.ppo) * mp.n_active + k) * N + t0;
+        const uint64_t qi = io.qinv[prime], qis = io.qinv_s[prime];
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const int i = threadIdx.x + j * kThreads;
+            const int b = i / T, r = i % T;
+            uint64_t v = lds[b * TP + swz<T>(r)];
+            v = v >= q2 ? v - q2 : v;
+            v = v >= q ? v - q : v;
+            out[i] = mul_shoup(sub_mod(accv[j], v, q), qi, qis, q);
+        }
+    } else if (FWD) {
         for (int i = threadIdx.x; i < kTile; i += kThreads) {
             const int b = i / T, r = i % T;
-            fl[b * TP + swz<T>(r)] = FWD ? __longlong_as_double((long long)base[i]) : f_of_u64(base[i]);
-        }
-        // rescale: the accumulator words of this thread are loaded now and
-        // arrive while the butterflies run
-        constexpr int PER = kTile / kThreads;
-        uint64_t accv[IO == IO_RESCALE ? PER : 1];
-        if (FWD && IO == IO_RESCALE) {
-            const uint64_t *acc = io.acc + ((size_t)poly * io.acc_limbs + k) * N + t0;
-#pragma unroll
-            for (int j = 0; j < PER; j++) accv[j] = acc[threadIdx.x + j * kThreads];
-        }
-        __syncthreads();
-        run_rounds<LOGT, FWD>(fl, N1, 1u, (uint32_t)(t0 / T), tb.ftw + (size_t)prime * N,
-                              tb.ftwq + (size_t)prime * N, K);
-        if (FWD && IO == IO_RESCALE) {
-            uint64_t *ob = io.outs ? io.outs[poly / io.ppo] : io.out_single;
-            uint64_t *out = ob + ((size_t)(poly % io.ppo) * mp.n_active + k) * N + t0;
-            const uint64_t qi = io.qinv[prime], qh = (uint64_t)fm.q;
-            const double qiw = (double)(qi > qh / 2 ? (int64_t)qi - (int64_t)qh : (int64_t)qi);
-            const double qiq = qiw / fm.q;
-#pragma unroll
-            for (int j = 0; j < PER; j++) {
-                const int i = threadIdx.x + j * kThreads;
-                const int b = i / T, r = i % T;
-                const double d = f_of_u64(accv[j]) - fl[b * TP + swz<T>(r)];
-                out[i] = f_to_canonical(f_mul(d, qiw, qiq, fm.q), fm.q, fm.qinv);
-            }
-        } else if (FWD) {
-            for (int i = threadIdx.x; i < kTile; i += kThreads) {
-                const int b = i / T, r = i % T;
-                base[i] = f_to_canonical(fl[b * TP + swz<T>(r)], fm.q, fm.qinv);
-            }
-        } else {
-            for (int i = threadIdx.x; i < kTile; i += kThreads) {
-                const int b = i / T, r = i % T;
-                base[i] = (uint64_t)__double_as_longlong(fl[b * TP + swz<T>(r)]);
-            }
+            uint64_t v = lds[b * TP + swz<T>(r)];
+            v = v >= q2 ? v - q2 : v;
+            base[i] = v >= q ? v - q : v;
         }
     } else {
-        const uint64_t q = tb.mods[prime].q;
-        const uint64_t *tw = tb.tw + (size_t)prime * N;
-        const uint64_t *tws = tb.tws + (size_t)prime * N;
         for (int i = threadIdx.x; i < kTile; i += kThreads) {
             const int b = i / T, r = i % T;
-            lds[b * TP + swz<T>(r)] = base[i];
-        }
-        __syncthreads();
-        run_rounds<LOGT, FWD>(lds, N1, 1u, (uint32_t)(t0 / T), tw, tws, IntK{q});
-        if (FWD) {
-            const uint64_t q2 = q << 1;
-            if (IO == IO_RESCALE) {
-                const uint64_t *acc = io.acc + ((size_t)poly * io.acc_limbs + k) * N + t0;
-                uint64_t *ob = io.outs ? io.outs[poly / io.ppo] : io.out_single;
-                uint64_t *out = ob + ((size_t)(poly % io.ppo) * mp.n_active + k) * N + t0;
-                const uint64_t qi = io.qinv[prime], qis = io.qinv_s[prime];
-                for (int i = threadIdx.x; i < kTile; i += kThreads) {
-                    const int b = i / T, r = i % T;
-                    uint64_t v = lds[b * TP + swz<T>(r)];
-                    v = v >= q2 ? v - q2 : v;
-                    v = v >= q ? v - q : v;
-                    out[i] = mul_shoup(sub_mod(acc[i], v, q), qi, qis, q);
-                }
-            } else {
-                for (int i = threadIdx.x; i < kTile; i += kThreads) {
-                    const int b = i / T, r = i % T;
-                    uint64_t v = lds[b * TP + swz<T>(r)];
-                    v = v >= q2 ? v - q2 : v;
-                    base[i] = v >= q ? v - q : v;
-                }
-            }
-        } else {
-            for (int i = threadIdx.x; i < kTile; i += kThreads) {
-                const int b = i / T, r = i % T;
-                base[i] = lds[b * TP + swz<T>(r)];
-            }
+            base[i] = lds[b * TP + swz<T>(r)];
         }
     }
 }
 
-template <bool FP, bool FWD, int IOA, int IOB>
-void launch_ntt_impl(const Context &c, uint64_t *data, const LimbMap &mp, size_t total_limbs, hipStream_t s,
-                     const NttIO &io) {
+template <bool FWD, int IOA, int IOB>
+void launch_ntt(const Context &c, uint64_t *data, const LimbMap &mp, size_t total_limbs, hipStream_t s,
+                const NttIO &io = NttIO{}) {
+    if (total_limbs == 0) return;
+    if (c.N < (size_t)kTile) throw InvalidArgument("NTT requires N >= 4096");
+    if (total_limbs % mp.n_active != 0) throw LogicError("NTT limb count is not a multiple of the polynomial size");
     const int logN = c.logN;
     // Phase B takes whole 4096-word tiles (12 stages) and phase A the rest:
     // with few column stages, phase A moves rows of >= 2 KB contiguous words
@@ -485,17 +357,14 @@ void launch_ntt_impl(const Context &c, uint64_t *data, const LimbMap &mp, size_t
                  FWD ? c.tw.as<uint64_t>() : c.itw.as<uint64_t>(),
                  FWD ? c.tw_s.as<uint64_t>() : c.itw_s.as<uint64_t>(),
                  c.ninv.as<uint64_t>(),
-                 c.ninv_s.as<uint64_t>(),
-                 FP ? (FWD ? c.ftw.as<double>() : c.fitw.as<double>()) : nullptr,
-                 FP ? (FWD ? c.ftwq.as<double>() : c.fitwq.as<double>()) : nullptr,
-                 FP ? c.fmod.as<FpMod>() : nullptr};
-#define SHM_COLS(L)                                                                               \
-    case L:                                                                                       \
-        k_ntt_cols<L, FWD, IOA, FP><<<grid_cols, kThreads, 0, s>>>(data, mp, tb, logN, io);       \
+                 c.ninv_s.as<uint64_t>()};
+#define SHM_COLS(L)                                                                           \
+    case L:                                                                                   \
+        k_ntt_cols<L, FWD, IOA><<<grid_cols, kThreads, 0, s>>>(data, mp, tb, logN, io);       \
         break;
-#define SHM_BLKS(L)                                                                                          \
-    case L:                                                                                                  \
-        k_ntt_blocks<L, FWD, IOB, FP><<<grid_blk, kThreads, 0, s>>>(data, mp, tb, logN, io, n_polys, per_xcd); \
+#define SHM_BLKS(L)                                                                                      \
+    case L:                                                                                              \
+        k_ntt_blocks<L, FWD, IOB><<<grid_blk, kThreads, 0, s>>>(data, mp, tb, logN, io, n_polys, per_xcd); \
         break;
     auto cols = [&]() {
         switch (log1) {
@@ -524,26 +393,6 @@ void launch_ntt_impl(const Context &c, uint64_t *data, const LimbMap &mp, size_t
         cols();
     }
     SHM_HIP(hipGetLastError());
-}
-
-template <bool FWD, int IOA, int IOB>
-void launch_ntt(const Context &c, uint64_t *data, const LimbMap &mp, size_t total_limbs, hipStream_t s,
-                const NttIO &io = NttIO{}) {
-    if (total_limbs == 0) return;
-    if (c.N < (size_t)kTile) throw InvalidArgument("NTT requires N >= 4096");
-    if (total_limbs % mp.n_active != 0) throw LogicError("NTT limb count is not a multiple of the polynomial size");
-    // FP64 path when every prime this launch touches is below 2^51
-    bool fp = true;
-    for (uint32_t j = 0; j < mp.n_active && fp; j++) {
-        uint32_t k = mp.lo + j;
-        if (k >= mp.skip_lo) k += mp.skip_width;
-        const uint32_t prime = k < mp.Ql ? mp.first_prime + k : mp.nQ + (k - mp.Ql);
-        fp = c.fp_prime.at(prime) != 0;
-    }
-    if (fp)
-        launch_ntt_impl<true, FWD, IOA, IOB>(c, data, mp, total_limbs, s, io);
-    else
-        launch_ntt_impl<false, FWD, IOA, IOB>(c, data, mp, total_limbs, s, io);
 }
 
 }  // namespace

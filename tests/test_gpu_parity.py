@@ -52,9 +52,9 @@ def test_ntt_roundtrip_bitexact(c1):
     (17, [51, 40, 51], 1),
 ])
 def test_ntt_paths_bitexact(oracle_lib, logN, bits, n_special):
-    """Primes below 2^51 take the FP64 butterflies, larger ones the integer
-    Shoup butterflies; both must equal the oracle's NTT bit for bit, including
-    the extreme residues 0 and q - 1 and the three tile splits of N."""
+    """Primes from 20 to 61 bits and the tile splits of N = 2^12, 2^16, 2^17
+    (column phase of 4 or 5 stages): the NTT must equal the oracle's bit for
+    bit, including the extreme residues 0 and q - 1."""
     import shmoai_amd as shm
     ctx = shm.Context(logN, bits, n_special)
     ref = oracle_lib.Ref(logN, bits, n_special)
