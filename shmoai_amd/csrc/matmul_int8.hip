@@ -30,6 +30,7 @@ typedef int8_t i8x16 __attribute__((ext_vector_type(16)));
 typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kBlock = 256;
+constexpr int kCorrCache = 256;  // output rows whose correction is cached in LDS
 
 inline unsigned grid_for(size_t work) {
     size_t g = (work + kBlock - 1) / kBlock;
@@ -125,8 +126,10 @@ __device__ __forceinline__ void transpose4(uint32_t w0, uint32_t w1, uint32_t w2
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
+// Two waves per SIMD: the register budget (X digit fragments, 11 planes, the
+// prefetched weight fragments of the next phase) is capped at 256 per lane.
 template <int DX, int DW, int KS, int RB>
-__global__ __launch_bounds__(256) void k_contract_int8(const uint64_t *const *__restrict__ X,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_contract_int8(const uint64_t *const *__restrict__ X,
                                                        const i8x16 *__restrict__ frag,
                                                        const uint64_t *__restrict__ corr,
                                                        const uint64_t *__restrict__ mu_arr,
@@ -139,6 +142,7 @@ __global__ __launch_bounds__(256) void k_contract_int8(const uint64_t *const *__
     static_assert(DX <= 8, "at most 8 digits per limb");
     constexpr int FR = DW * RB * KS * 64;  // fragments staged per row phase
     __shared__ i8x16 wl[FR];
+    __shared__ uint64_t cl[kCorrCache];  // row-sum corrections of this limb
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const size_t tiles = N / 64;
     const size_t tile = blockIdx.x % tiles, lp = blockIdx.x / tiles;  // lp = p * Ql + l
@@ -146,6 +150,7 @@ __global__ __launch_bounds__(256) void k_contract_int8(const uint64_t *const *__
     const size_t col = tile * 64 + 16 * wave + (lane & 15);
     const size_t off = lp * N + col;
     const uint64_t q = mods[l].q, mu = mu_arr[l];
+    for (int t = threadIdx.x; t < min(col_w, kCorrCache); t += 256) cl[t] = corr[l * col_w + t];
 
     // B fragments: lane holds X'_a[k = 64 ks + 16 (lane/16) + e][col] for
     // e = 0..15, built by byte transposes of the little-endian limb words.
@@ -170,12 +175,39 @@ __global__ __launch_bounds__(256) void k_contract_int8(const uint64_t *const *__
     }
     // rows k >= row_w carry digit -128 in B; their A entries are 0 so they add nothing.
 
+    // Block barrier that orders LDS only: a full __syncthreads would also wait
+    // for this wave's outstanding output stores before every phase.
+    auto lds_barrier = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    };
     const int nph = (MB + RB - 1) / RB;
+    // The weight fragments of the next phase are loaded into registers while
+    // the current phase runs, then copied to LDS between two barriers.
+    constexpr int PF = (FR + 255) / 256;
+    i8x16 pf[PF];
+#pragma unroll
+    for (int u = 0; u < PF; u++) {
+        const int t = threadIdx.x + 256 * u;
+        if (t < FR) pf[u] = frag[t];
+    }
     for (int ph = 0; ph < nph; ph++) {
-        __syncthreads();
-        const i8x16 *src = frag + (size_t)ph * FR;
-        for (int t = threadIdx.x; t < FR; t += 256) wl[t] = src[t];
-        __syncthreads();
+        lds_barrier();
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            const int t = threadIdx.x + 256 * u;
+            if (t < FR) wl[t] = pf[u];
+        }
+        lds_barrier();
+        if (ph + 1 < nph) {
+            const i8x16 *src = frag + (size_t)(ph + 1) * FR;
+#pragma unroll
+            for (int u = 0; u < PF; u++) {
+                const int t = threadIdx.x + 256 * u;
+                if (t < FR) pf[u] = src[t];
+            }
+        }
 #pragma unroll 1
         for (int ibl = 0; ibl < RB; ibl++) {
             const int ib = ph * RB + ibl;
@@ -220,8 +252,9 @@ __global__ __launch_bounds__(256) void k_contract_int8(const uint64_t *const *__
                 res = res >= 2 * q ? res - 2 * q : res;
                 res = res >= q ? res - q : res;
                 if (row < col_w) {
-                    res = add_mod(res, corr[l * col_w + row], q);
-                    acc_out[(size_t)row * 2 * Ql * N + off] = res;
+                    res = add_mod(res, row < kCorrCache ? cl[row] : corr[l * col_w + row], q);
+                    // streamed once by the rescale that follows: keep it out of the caches
+                    __builtin_nontemporal_store(res, &acc_out[(size_t)row * 2 * Ql * N + off]);
                 }
             }
         }
