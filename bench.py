@@ -17,6 +17,13 @@ parameters (N = 2^16, Q = 1 x 51 + 20 x 46 + 14 x 51 bits, P = 5 x 51 bits,
 HW 192, inputs with 4 primes, slots U(-1, 1)), timed on a bounded number of
 ciphertexts and extrapolated to 768: ordinary (one ciphertext at a time) and
 shared-a batches of n = 4, with the decryption error of the timed outputs.
+
+Two more objects give the HBM rate of the transform and key-switch kernels:
+`ntt` (forward + inverse NTT of 64 polynomials x 24 limbs at N = 2^16) and
+`bootstrap_768.keyswitch` (one rotation of a top-level ciphertext of the
+bootstrapping parameters), each as algorithmic GB/s and fraction of peak.
+`roofline.traffic` is the contraction's HBM bytes per launch from the
+committed rocprofv3 PMC summary in profiles/.
 """
 from __future__ import annotations
 
@@ -37,6 +44,23 @@ INT8_PEAK_TOPS = 5000.0        # MI355X dense INT8 MFMA (POP/s x 1e3)
 LOG_N, L, ROW_W, COL_W, INPUTS = 16, 24, 192, 192, 256
 BITS = [50] * L + [60]
 SCALE = 2.0 ** 40
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "contract_int8_192x192_N65536_L24_pmc.txt")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the contraction kernel from the committed
+    rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE runs scaled by a copy
+    calibration), or None when the summary is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return int(line.split(":")[1].split()[0])
+    except OSError:
+        pass
+    return None
 
 
 def log(msg: str) -> None:
@@ -76,6 +100,58 @@ def cpu_baseline(xs, W, threads: int):
 BOOT_BITS = [51] + [46] * 20 + [51] * 14 + [51] * 5
 
 
+def ntt_bench(shm, ctx, iters: int = 10):
+    """Forward + inverse NTT of one batch of 64 polynomials x 24 limbs (N = 2^16).
+
+    Algorithmic bytes of one transform: 16 B per word (read + write once, the
+    per-NTT figure 16 * limbs * N); the two-pass kernels move twice that."""
+    # four zero-filled buffers of 16 polynomials x 24 limbs (one launch each)
+    bufs = [shm.SharedABatch(ctx, 15, 1) for _ in range(4)]
+    ptrs, per, npoly = [b.data_ptr() for b in bufs], 16, 64
+
+    def once():
+        for p in ptrs:
+            ctx.ntt_forward(p, per, L, 0)
+        for p in ptrs:
+            ctx.ntt_inverse(p, per, L, 0)
+
+    once()
+    shm.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        once()
+    shm.synchronize()
+    ms = (time.perf_counter() - t0) / (2 * iters) * 1e3
+    words = npoly * L * ctx.N
+    alg_gbs = 16 * words / (ms * 1e-3) / 1e9
+    log(f"bench: NTT {npoly}x{L} limbs {ms:.3f} ms per transform, {alg_gbs:.0f} GB/s algorithmic")
+    return {"limbs": npoly * L, "ms_per_transform": round(ms, 4), "algorithmic_bytes": 16 * words,
+            "achieved_gbs": round(alg_gbs, 1), "frac": round(alg_gbs / HBM_PEAK_GBS, 4),
+            "moved_gbs_two_pass": round(2 * alg_gbs, 1)}
+
+
+def keyswitch_bench(shm, ctx, sk, gk, step: int, enc, iters: int = 10):
+    """Rotation (one hybrid key switch) of a top-level ciphertext of the
+    bootstrapping parameters. Algorithmic bytes: 8N [Ql + 2 beta (Ql + P)
+    + 2 Ql + 2 Ql] (input limbs, key, read c0/c1, write)."""
+    ev = shm.Evaluator(ctx)
+    ct = sk.encrypt_symmetric(enc.encode(np.zeros(ctx.N // 2), 2.0 ** 46, 1), seed=9)
+    ev.rotate_vector_inplace(ct, step, gk)
+    shm.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        ev.rotate_vector_inplace(ct, step, gk)
+    shm.synchronize()
+    ms = (time.perf_counter() - t0) / iters * 1e3
+    Ql, P = ctx.size_q, ctx.size_p
+    beta = -(-Ql // P)
+    alg = 8 * ctx.N * (Ql + 2 * beta * (Ql + P) + 2 * Ql + 2 * Ql)
+    gbs = alg / (ms * 1e-3) / 1e9
+    log(f"bench: key switch (rotate, Ql={Ql}, dnum={beta}) {ms:.3f} ms, {gbs:.0f} GB/s algorithmic")
+    return {"Ql": Ql, "dnum": beta, "ms": round(ms, 4), "algorithmic_bytes": alg, "achieved_gbs": round(gbs, 1),
+            "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
 def bootstrap_bench(shm, n_ord: int = 8, n_shared: int = 4, chunks: int = 2):
     """Seconds per ciphertext of ordinary and shared-a S2C-first bootstrapping."""
     ctx = shm.Context(16, BOOT_BITS, 5)
@@ -95,6 +171,7 @@ def bootstrap_bench(shm, n_ord: int = 8, n_shared: int = 4, chunks: int = 2):
     total = max(n_ord, n_shared * chunks)
     zs = [rng.uniform(-1, 1, n) for _ in range(total)]
     enc, dec = shm.Encoder(ctx), shm.Decryptor(ctx, sk)
+    ks = keyswitch_bench(shm, ctx, sk, gk, min(s for s in steps if s > 0), enc)
     cts = [sk.encrypt_symmetric(enc.encode(z, 2.0 ** 46, ctx.chain(4)), seed=100 + i) for i, z in enumerate(zs)]
     boot.bootstrap_s2c_first(cts[:1], gk, rk)  # warm-up
     t0 = time.perf_counter()
@@ -119,7 +196,7 @@ def bootstrap_bench(shm, n_ord: int = 8, n_shared: int = 4, chunks: int = 2):
             f"shared_a_n{n_shared}_768_s_extrapolated": round(sh_s * 768, 2),
             "batched_speedup_vs_sequential": round(ord_s / sh_s, 3),
             "max_abs_err_ordinary": err_ord, "max_abs_err_shared_a": err_sh,
-            "keys_resident_setup_s": round(setup_s, 2),
+            "keys_resident_setup_s": round(setup_s, 2), "keyswitch": ks,
             "sample": f"{n_ord} ordinary ciphertexts and {chunks} shared-a chunks of {n_shared}, "
                       "wall clock incl. host overhead, extrapolated to 768"}
 
@@ -177,7 +254,7 @@ def main() -> int:
                 "achieved": round(tops if variant == shm.MATMUL_INT8 else hbm_gbs, 2),
                 "peak": INT8_PEAK_TOPS if variant == shm.MATMUL_INT8 else HBM_PEAK_GBS,
                 "unit": "TOP/s" if variant == shm.MATMUL_INT8 else "GB/s",
-                "traffic": None,
+                "traffic": measured_traffic() if variant == shm.MATMUL_INT8 else None,
                 "kernel": "k_contract_int8" if variant == shm.MATMUL_INT8 else "k_contract_u64",
                 "kernel_ms": round(k_ms, 4),
                 "hbm_achieved_gbs": round(hbm_gbs, 1), "hbm_frac": round(hbm_gbs / HBM_PEAK_GBS, 4),
@@ -193,6 +270,7 @@ def main() -> int:
                  "batched_speedup_vs_plain": round(plain_ms / ms_per_step, 2)}
         log(f"bench: plain per-ciphertext form {plain_ms:.1f} ms")
 
+    ntt = ntt_bench(shm, ctx)
     boot = None
     if not args.no_bootstrap:
         boot = bootstrap_bench(shm)
@@ -221,6 +299,7 @@ def main() -> int:
                    "variant": args.variant, "inputs_per_step": INPUTS},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "ntt": ntt,
         "bootstrap_768": boot,
         **extra,
     }

@@ -66,6 +66,12 @@ typedef struct shm_context_info {
 const char *shm_last_error(void);
 const char *shm_version(void);
 
+/* Waits until all work queued on `stream` (NULL: the default stream) has
+ * finished. Most entry points return with their kernels still queued; the
+ * reference's callers use cudaStreamSynchronize on the stream they passed
+ * (e.g. ct_pt_matrix_mul_wo_pre, Ct_pt_matrix_mul.cuh:172). */
+int shm_synchronize(shm_stream_t stream);
+
 /* ---- context --------------------------------------------------------------
  * Replaces EncryptionParameters + CoeffModulus::Create (modulus.cu:79-108) +
  * PhantomContext(parms) (context.cu:125-238). coeff_bit_sizes lists the data
@@ -300,6 +306,9 @@ int shm_shared_a_batch_destroy(shm_shared_a_batch *b);
 int shm_shared_a_batch_info(const shm_shared_a_batch *b, uint32_t *n, uint32_t *chain_index,
                             uint32_t *coeff_mod_size, double *scale);
 int shm_shared_a_batch_set_scale(shm_shared_a_batch *b, double scale);
+/* device words [n+1][Ql][N]: b_0..b_{n-1}, then a (the public a_ / b_ buffers
+ * of SharedACiphertextBatch, SharedACiphertext.cuh:29-30, in one allocation) */
+uint64_t *shm_shared_a_batch_data(shm_shared_a_batch *b);
 /* words [n+1][Ql][N]: b_0..b_{n-1}, then a */
 int shm_shared_a_batch_import(shm_shared_a_batch *b, const uint64_t *host, size_t n_words, shm_stream_t stream);
 int shm_shared_a_batch_export(const shm_shared_a_batch *b, uint64_t *host, size_t n_words, shm_stream_t stream);

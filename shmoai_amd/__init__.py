@@ -518,6 +518,11 @@ def ct_pt_matrix_mul_wo_pre(enc_X: Sequence[Ciphertext], W: np.ndarray, col_X: i
     return out
 
 
+def synchronize(stream=None) -> None:
+    """Wait for all work queued on `stream` (None: the default stream)."""
+    check(_lib().shm_synchronize(stream))
+
+
 def last_matmul_kernel_ms() -> float:
     return float(_lib().shm_last_matmul_kernel_ms())
 
@@ -567,6 +572,10 @@ class SharedABatch:
     @scale.setter
     def scale(self, v: float):
         check(_lib().shm_shared_a_batch_set_scale(self.h, v))
+
+    def data_ptr(self) -> int:
+        """Device address of the words [n+1][Ql][N] (b_0..b_{n-1}, then a)."""
+        return _lib().shm_shared_a_batch_data(self.h)
 
     def words(self) -> int:
         return (self.n + 1) * self.coeff_modulus_size * self.ctx.N

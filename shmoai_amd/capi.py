@@ -51,6 +51,7 @@ _SIGS = {
     "shm_context_get_info": (i32, [vp, C.POINTER(ContextInfo)]),
     "shm_context_coeff_modulus": (i32, [vp, pu64, sz]),
     "shm_galois_elt_from_step": (i32, [vp, i32, C.POINTER(u32)]),
+    "shm_synchronize": (i32, [vp]),
     "shm_ntt_forward": (i32, [vp, vp, u32, u32, u32, vp]),
     "shm_ntt_inverse": (i32, [vp, vp, u32, u32, u32, vp]),
     "shm_ciphertext_create": (i32, [vp, u32, u32, pvp]),
@@ -58,6 +59,7 @@ _SIGS = {
     "shm_ciphertext_info": (i32, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(dbl)]),
     "shm_ciphertext_set_scale": (i32, [vp, dbl]),
     "shm_ciphertext_data": (vp, [vp]),
+    "shm_shared_a_batch_data": (vp, [vp]),
     "shm_ciphertext_import": (i32, [vp, pu64, sz, vp]),
     "shm_ciphertext_export": (i32, [vp, pu64, sz, vp]),
     "shm_ciphertext_copy": (i32, [vp, vp, vp]),

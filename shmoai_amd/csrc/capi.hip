@@ -150,6 +150,10 @@ int shm_ntt_forward(shm_context *ctx, uint64_t *data, uint32_t n_polys, uint32_t
     });
 }
 
+int shm_synchronize(shm_stream_t s) {
+    return guard([&] { SHM_HIP(hipStreamSynchronize(S(s))); });
+}
+
 int shm_ntt_inverse(shm_context *ctx, uint64_t *data, uint32_t n_polys, uint32_t n_limbs, uint32_t first,
                     shm_stream_t s) {
     return guard([&] {
@@ -1069,6 +1073,8 @@ int shm_shared_a_batch_set_scale(shm_shared_a_batch *b, double scale) {
         b->b.ct.scale = scale;
     });
 }
+
+uint64_t *shm_shared_a_batch_data(shm_shared_a_batch *b) { return b ? b->b.ct.data() : nullptr; }
 
 int shm_shared_a_batch_import(shm_shared_a_batch *b, const uint64_t *host, size_t nw, shm_stream_t st) {
     return guard([&] {
