@@ -325,9 +325,15 @@ std::unique_ptr<Context> make_context(int logN, const std::vector<int> &bits, si
             qi[i] = h_inv(c->q[Ql - 1] % c->q[i], c->q[i]);
             qis[i] = h_shoup(qi[i], c->q[i]);
         }
-        std::vector<uint64_t> hi(Ql), his(Ql), hm(L->beta * alpha * QlP, 0);
+        std::vector<uint64_t> hi(Ql), his(Ql), hm(L->beta * alpha * QlP, 0), dm(L->beta * QlP, 0);
         for (size_t g = 0; g < L->beta; g++) {
             const size_t start = g * alpha, size = std::min(alpha, Ql - start);
+            for (size_t k = 0; k < QlP; k++) {
+                const uint64_t p = c->q[k < Ql ? k : nQ + (k - Ql)];
+                uint64_t d = 1;
+                for (size_t i = 0; i < size; i++) d = h_mulmod(d, c->q[start + i] % p, p);
+                dm[g * QlP + k] = d;
+            }
             for (size_t i = 0; i < size; i++) {
                 const uint64_t qv = c->q[start + i];
                 uint64_t h = 1;
@@ -350,6 +356,7 @@ std::unique_ptr<Context> make_context(int logN, const std::vector<int> &bits, si
         upload(L->hat_inv, hi);
         upload(L->hat_inv_s, his);
         upload(L->hat_mod, hm);
+        upload(L->dig_mod, dm);
         c->levels[Ql] = std::move(L);
     }
 

@@ -112,6 +112,8 @@ struct LevelConsts {
     DevBuf hat_inv, hat_inv_s;
     // modup: [qhat_i]_{p_k} for digit g, local i, out limb k: [beta][alpha][QlP]
     DevBuf hat_mod;
+    // modup: [D_g]_{p_k}, D_g = product of digit g's primes: [beta][QlP]
+    DevBuf dig_mod;
 };
 
 struct Context {

@@ -26,8 +26,8 @@ __device__ __forceinline__ uint32_t qlp_prime(size_t k, size_t Ql, size_t nQ) {
     return (uint32_t)(k < Ql ? k : nQ + (k - Ql));
 }
 
-// One thread per (digit g, coefficient j): the digit's own limbs are copied (NTT
-// form) and its y_i = [x_i (D_g/q_i)^-1]_{q_i} are formed once, then every other
+// One thread per (digit g, coefficient j): the digit's own limbs are copied (in
+// coefficient form, transformed with the rest) and its y_i = [x_i (D_g/q_i)^-1]_{q_i} are formed once, then every other
 // QlP limb gets sum_i y_i [D_g/q_i]_p (the approximate fast conversion). With
 // `rounded`, the conversion is exact and centred (it subtracts
 // round(sum_i y_i / q_i) D_g), so every digit is lifted to (-D_g/2, D_g/2]:
@@ -35,15 +35,17 @@ __device__ __forceinline__ uint32_t qlp_prime(size_t k, size_t Ql, size_t nQ) {
 // that otherwise lands on the slots nearest w = 1.
 constexpr int kMaxAlpha = 16;
 
+// Blocks never straddle two digits (N is a multiple of the block size), so the
+// digit index and every table address derived from it are wave-uniform.
 __global__ void k_modup_bconv(const uint64_t *__restrict__ c2, const uint64_t *__restrict__ tcoef,
                               uint64_t *__restrict__ out, const DevMod *__restrict__ mods,
                               const uint64_t *__restrict__ hat_inv, const uint64_t *__restrict__ hat_inv_s,
-                              const uint64_t *__restrict__ hat_mod, size_t Ql, size_t nQ, size_t alpha,
-                              size_t beta, size_t N, int rounded) {
+                              const uint64_t *__restrict__ hat_mod, const uint64_t *__restrict__ dig_mod, size_t Ql,
+                              size_t nQ, size_t alpha, size_t beta, size_t N, int rounded) {
     const size_t QlP = Ql + alpha;
     const size_t total = beta * N;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t g = t / N, j = t % N;
+        const size_t g = __builtin_amdgcn_readfirstlane((uint32_t)(t / N)), j = t % N;
         const size_t start = g * alpha, size = min(alpha, Ql - start);
         uint64_t *o = out + g * QlP * N + j;
         uint64_t y[kMaxAlpha];
@@ -56,7 +58,8 @@ __global__ void k_modup_bconv(const uint64_t *__restrict__ c2, const uint64_t *_
                 const uint64_t qi = mods[li].q;
                 y[i] = size == 1 ? tcoef[li * N + j] : mul_shoup(tcoef[li * N + j], hat_inv[li], hat_inv_s[li], qi);
                 if (rounded) v = __dadd_rn(v, __ddiv_rn((double)y[i], (double)qi));
-                o[li * N] = c2[li * N + j];
+                // own limb in coefficient form: the batched NTT below restores c2 exactly
+                o[li * N] = tcoef[li * N + j];
             }
         }
         const uint64_t r = rounded ? (uint64_t)__dadd_rn(v, 0.5) : 0;
@@ -74,8 +77,9 @@ __global__ void k_modup_bconv(const uint64_t *__restrict__ c2, const uint64_t *_
 #pragma unroll
                 for (int i = 0; i < kMaxAlpha; i++)
                     if ((size_t)i < size) mac128(acc, y[i], hm[i * QlP]);
+                // - r D_g == (p - r) [D_g]_p (mod p), folded in before the one reduction
+                if (rounded) mac128(acc, mp.q - r, dig_mod[g * QlP + k]);
                 val = reduce128(acc, mp);
-                if (r) val = sub_mod(val, mul_mod(r, mul_mod(hm[0], reduce64(qs, mp), mp), mp), mp.q);
             }
             o[k * N] = val;
         }
@@ -170,8 +174,9 @@ __global__ void k_moddown_bconv(const uint64_t *__restrict__ cx, uint64_t *__res
 #pragma unroll
                 for (int jj = 0; jj < kMaxAlpha; jj++)
                     if ((size_t)jj < nP) mac128(acc, y[jj], phat_mod[i * nP + jj]);
+                // - r P == (q - r) [P]_q (mod q), folded in before the one reduction
+                if (rounded) mac128(acc, mq.q - r, pmod[i]);
                 val = reduce128(acc, mq);
-                if (r) val = sub_mod(val, mul_mod(r, pmod[i], mq), mq.q);
             }
             d[i * N] = val;
         }
@@ -207,12 +212,11 @@ void ks_modup(const Context &c, const uint64_t *c2, size_t Ql, uint64_t *d, hipS
     if (alpha > (size_t)kMaxAlpha) throw InvalidArgument("at most 16 special primes");
     k_modup_bconv<<<grid_for(beta * N), kBlock, 0, s>>>(
         c2, tcoef.as<uint64_t>(), d, c.mods(), L.hat_inv.as<uint64_t>(), L.hat_inv_s.as<uint64_t>(),
-        L.hat_mod.as<uint64_t>(), Ql, c.nQ, alpha, beta, N, rounded ? 1 : 0);
+        L.hat_mod.as<uint64_t>(), L.dig_mod.as<uint64_t>(), Ql, c.nQ, alpha, beta, N, rounded ? 1 : 0);
     SHM_HIP(hipGetLastError());
-    for (size_t g = 0; g < beta; g++) {
-        const size_t start = g * alpha, size = std::min(alpha, Ql - start);
-        ntt_forward_qlp(c, d + g * QlP * N, Ql, 0, QlP, start, start + size, 1, QlP * N, s);
-    }
+    // one launch over every digit's QlP limbs (the own limbs were written in
+    // coefficient form and transform back to their NTT-form input words)
+    ntt_forward_qlp(c, d, Ql, 0, QlP, 0, 0, beta, QlP * N, s);
 }
 
 void ks_inner_multi(const Context &c, const uint64_t *d, size_t Ql, size_t ndig, const KeyRefs &kr, size_t nout,
