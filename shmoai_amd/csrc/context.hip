@@ -6,6 +6,7 @@
 #include <complex>
 #include <cstring>
 #include <map>
+#include <mutex>
 
 #include "engine.h"
 
@@ -131,16 +132,106 @@ void DevBuf::alloc(size_t bytes) {
     bytes_ = bytes;
 }
 
-void DevBuf::alloc_async(size_t bytes, hipStream_t s) {
-    release();
-    if (bytes == 0) bytes = 8;
-    hipError_t e = hipMallocAsync(&ptr_, bytes, s);
+// Stream-ordered block cache for the default stream. A block released on the
+// default stream is handed out again only on the default stream, so stream
+// order alone puts every kernel of its previous user ahead of its next user;
+// the per-op hipMallocAsync / hipFreeAsync pairs (a few microseconds of host
+// time each, several per key switch) disappear from the hot loops. Blocks on
+// other streams go straight back to the pool. Sizes are rounded up to a
+// bucket (powers of two to 1 MiB, then whole MiB), so a cached block always
+// holds its bucket.
+namespace {
+
+struct BlockCache {
+    std::mutex mu;
+    std::map<std::pair<int, size_t>, std::vector<void *>> free;
+    size_t cached = 0;
+};
+
+BlockCache &block_cache() {
+    static BlockCache *c = new BlockCache();  // outlives static destruction order
+    return *c;
+}
+
+constexpr size_t kCacheCap = size_t(32) << 30;
+
+size_t bucket_of(size_t bytes) {
+    constexpr size_t MiB = size_t(1) << 20;
+    if (bytes <= MiB) {
+        size_t b = 256;
+        while (b < bytes) b <<= 1;
+        return b;
+    }
+    return (bytes + MiB - 1) / MiB * MiB;
+}
+
+void flush_block_cache() {
+    BlockCache &bc = block_cache();
+    std::lock_guard<std::mutex> lk(bc.mu);
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    for (auto &kv : bc.free) {
+        (void)hipSetDevice(kv.first.first);
+        for (void *p : kv.second) (void)hipFreeAsync(p, nullptr);
+        bc.cached -= kv.first.second * kv.second.size();
+        kv.second.clear();
+    }
+    (void)hipSetDevice(cur);
+}
+
+}  // namespace
+
+void *pool_alloc(size_t bytes, hipStream_t s, int *dev_out) {
+    const size_t b = bucket_of(bytes == 0 ? 8 : bytes);
+    int dev = 0;
+    SHM_HIP(hipGetDevice(&dev));
+    *dev_out = dev;
+    if (s == nullptr) {
+        BlockCache &bc = block_cache();
+        std::lock_guard<std::mutex> lk(bc.mu);
+        auto it = bc.free.find({dev, b});
+        if (it != bc.free.end() && !it->second.empty()) {
+            void *p = it->second.back();
+            it->second.pop_back();
+            bc.cached -= b;
+            return p;
+        }
+    }
+    void *p = nullptr;
+    hipError_t e = hipMallocAsync(&p, b, s);
     if (e == hipErrorOutOfMemory) {
         (void)hipGetLastError();
-        ptr_ = nullptr;
+        flush_block_cache();
+        (void)hipDeviceSynchronize();
+        e = hipMallocAsync(&p, b, s);
+    }
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
         throw std::bad_alloc();
     }
     hip_check(e, "hipMallocAsync");
+    return p;
+}
+
+void pool_free(void *p, size_t bytes, hipStream_t s, int dev) {
+    if (!p) return;
+    const size_t b = bucket_of(bytes == 0 ? 8 : bytes);
+    if (s == nullptr) {
+        BlockCache &bc = block_cache();
+        std::lock_guard<std::mutex> lk(bc.mu);
+        if (bc.cached + b <= kCacheCap) {
+            bc.free[{dev, b}].push_back(p);
+            bc.cached += b;
+            return;
+        }
+    }
+    (void)hipFreeAsync(p, s);
+}
+
+void DevBuf::alloc_async(size_t bytes, hipStream_t s) {
+    release();
+    if (bytes == 0) bytes = 8;
+    ptr_ = pool_alloc(bytes, s, &dev_);
     bytes_ = bytes;
     s_ = s;
     async_ = true;
@@ -159,7 +250,7 @@ void DevBuf::use_on(hipStream_t s) {
 void DevBuf::release() {
     if (ptr_) {
         if (async_) {
-            (void)hipFreeAsync(ptr_, s_);
+            pool_free(ptr_, bytes_, s_, dev_);
         } else {
             // queued work may still read the buffer: drain the device before freeing
             (void)hipDeviceSynchronize();
@@ -172,19 +263,9 @@ void DevBuf::release() {
     bytes_ = 0;
 }
 
-Scratch::Scratch(size_t bytes, hipStream_t s) : s_(s) {
-    if (bytes == 0) bytes = 8;
-    hipError_t e = hipMallocAsync(&ptr_, bytes, s);
-    if (e == hipErrorOutOfMemory) {
-        (void)hipGetLastError();
-        throw std::bad_alloc();
-    }
-    hip_check(e, "hipMallocAsync");
-}
+Scratch::Scratch(size_t bytes, hipStream_t s) : bytes_(bytes), s_(s) { ptr_ = pool_alloc(bytes, s, &dev_); }
 
-Scratch::~Scratch() {
-    if (ptr_) (void)hipFreeAsync(ptr_, s_);
-}
+Scratch::~Scratch() { pool_free(ptr_, bytes_, s_, dev_); }
 
 uint32_t Context::elt_from_step(int step) const {
     // galois.cu:47-90

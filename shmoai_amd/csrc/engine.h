@@ -41,10 +41,16 @@ struct DeviceError : std::runtime_error {
 void hip_check(hipError_t e, const char *what);
 #define SHM_HIP(call) ::shm::hip_check((call), #call)
 
+// Stream-ordered device blocks: hipMallocAsync on stream s, returned with
+// hipFreeAsync, except that blocks of the default stream are cached and reused
+// on the default stream (context.hip).
+void *pool_alloc(size_t bytes, hipStream_t s, int *dev_out);
+void pool_free(void *p, size_t bytes, hipStream_t s, int dev);
+
 // Device buffer owned by the host object (freed with the object). alloc() is a
 // plain hipMalloc (freed after draining the device); alloc_async() takes the
-// block from the stream-ordered pool and returns it with hipFreeAsync on the
-// last stream the buffer was used on, so no host synchronisation is needed.
+// block from the stream-ordered pool and returns it on the last stream the
+// buffer was used on, so no host synchronisation is needed.
 class DevBuf {
   public:
     DevBuf() = default;
@@ -52,7 +58,7 @@ class DevBuf {
     ~DevBuf() { release(); }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : ptr_(o.ptr_), bytes_(o.bytes_), s_(o.s_), async_(o.async_) {
+    DevBuf(DevBuf &&o) noexcept : ptr_(o.ptr_), bytes_(o.bytes_), s_(o.s_), async_(o.async_), dev_(o.dev_) {
         o.ptr_ = nullptr;
         o.bytes_ = 0;
     }
@@ -63,6 +69,7 @@ class DevBuf {
             bytes_ = o.bytes_;
             s_ = o.s_;
             async_ = o.async_;
+            dev_ = o.dev_;
             o.ptr_ = nullptr;
             o.bytes_ = 0;
         }
@@ -84,9 +91,10 @@ class DevBuf {
     size_t bytes_ = 0;
     hipStream_t s_ = nullptr;
     bool async_ = false;
+    int dev_ = 0;
 };
 
-// Stream-ordered scratch (hipMallocAsync / hipFreeAsync on the op's stream).
+// Stream-ordered scratch on the op's stream (pool_alloc / pool_free).
 class Scratch {
   public:
     Scratch(size_t bytes, hipStream_t s);
@@ -100,7 +108,9 @@ class Scratch {
 
   private:
     void *ptr_ = nullptr;
+    size_t bytes_ = 0;
     hipStream_t s_;
+    int dev_ = 0;
 };
 
 // Per-level constants (level = number of data limbs Ql).
