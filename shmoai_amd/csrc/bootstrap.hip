@@ -102,7 +102,7 @@ void const_lincomb(Context &c, Ciphertext &acc, const std::vector<const Cipherte
         if (!(std::fabs(sp.v[k]) < 3.4e38)) throw InvalidArgument("constant out of range");
     }
     acc.ctx = &c;
-    acc.resize(2, c.nQ - Ql + 1, s);
+    acc.resize(2, c.nQ - Ql + 1, s, false);  // k_const_lincomb writes every word
     acc.ntt = true;
     k_const_lincomb<<<grid_for(2 * Ql * c.N), kBlock, 0, s>>>(acc.data(), sp, (int)src.size(), c.mods(), Ql, c.N);
     SHM_HIP(hipGetLastError());
@@ -149,7 +149,7 @@ void lt_apply(Context &c, const LtLayer &L, Ciphertext &ct, const Rotator &r, hi
         if (!nt) continue;
         Ciphertext inner;
         inner.ctx = &c;
-        inner.resize(ct.size, ct.chain, s);
+        inner.resize(ct.size, ct.chain, s, false);  // k_lt_mac writes every word
         inner.ntt = true;
         inner.scale = ct.scale;
         k_lt_mac<<<grid_for(ct.size * Ql * c.N), kBlock, 0, s>>>(inner.data(), pk, nt, c.mods(), Ql, c.N, ct.size);

@@ -505,13 +505,15 @@ std::unique_ptr<Context> make_context(int logN, const std::vector<int> &bits, si
     return c;
 }
 
-void Ciphertext::resize(size_t new_size, size_t new_chain, hipStream_t s) {
+// A fresh block is zero-filled unless the caller overwrites every word
+// (zero = false).
+void Ciphertext::resize(size_t new_size, size_t new_chain, hipStream_t s, bool zero) {
     const size_t nl = ctx->ql_of_chain(new_chain);
     if (new_size > cap_size || nl > cap_limbs) {
         const size_t bytes = new_size * nl * ctx->N * sizeof(uint64_t);
         DevBuf nb;
         nb.alloc_async(bytes, s);
-        SHM_HIP(hipMemsetAsync(nb.as<void>(), 0, bytes, s));
+        if (zero) SHM_HIP(hipMemsetAsync(nb.as<void>(), 0, bytes, s));
         buf.use_on(s);
         buf = std::move(nb);  // the old block returns to the pool after the work queued on s
         cap_size = new_size;

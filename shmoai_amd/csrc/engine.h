@@ -183,7 +183,7 @@ struct Ciphertext {
     uint64_t *poly(size_t i) const { return data() + i * Ql() * ctx->N; }
     // (re)allocates from the stream-ordered pool on s when the capacity is too small;
     // contents are kept only when no reallocation happens
-    void resize(size_t new_size, size_t new_chain, hipStream_t s = nullptr);
+    void resize(size_t new_size, size_t new_chain, hipStream_t s = nullptr, bool zero = true);
 };
 
 struct Plaintext {

@@ -121,7 +121,7 @@ void rotate_hoisted(Context &c, const Ciphertext &ct, const std::vector<int> &st
         kr.stride[1] = 2 * QP * N;
         ks_inner_multi(c, d.as<uint64_t>(), Ql, beta, kr, 2, false, cx.as<uint64_t>(), s, table);
         o.ctx = &c;
-        o.resize(2, ct.chain, s);
+        o.resize(2, ct.chain, s, false);  // permute writes c0, ModDown writes c1
         o.scale = ct.scale;
         o.ntt = true;
         launch_permute(c, ct.poly(0), o.poly(0), table, Ql, s);
@@ -222,7 +222,7 @@ __global__ void k_mono_half(uint64_t *a, const DevMod *mods, const uint64_t *iot
 
 void ct_clone(Ciphertext &dst, const Ciphertext &src, hipStream_t s) {
     dst.ctx = src.ctx;
-    dst.resize(src.size, src.chain, s);
+    dst.resize(src.size, src.chain, s, false);  // the copy writes every word
     dst.scale = src.scale;
     dst.ntt = src.ntt;
     SHM_HIP(hipMemcpyAsync(dst.data(), src.data(), src.size * src.Ql() * src.ctx->N * sizeof(uint64_t),
@@ -293,7 +293,7 @@ void ct_mul_relin_rescale(Context &c, Ciphertext &out, const Ciphertext &a, cons
     }
     const size_t Ql = c.ql_of_chain(chain);
     out.ctx = &c;
-    out.resize(3, chain, s);
+    out.resize(3, chain, s, false);  // tensor / square write d0, d1, d2
     if (pa == pb || (pa->data() == pb->data()))
         launch_square(c, pa->data(), out.data(), Ql, s);
     else
@@ -325,7 +325,7 @@ void ct_modraise(Context &c, Ciphertext &a, hipStream_t s) {
     ntt_inverse(c, low.as<uint64_t>(), np, 1, 0, s);
     Ciphertext out;
     out.ctx = &c;
-    out.resize(np, 1, s);
+    out.resize(np, 1, s, false);  // k_modraise writes every limb
     for (size_t p = 0; p < np; p++)
         k_modraise<<<grid_ops(nl * N), kBlockOps, 0, s>>>(low.as<uint64_t>() + p * N, out.poly(p), c.mods(), nl, N);
     SHM_HIP(hipGetLastError());
