@@ -271,6 +271,12 @@ int shm_bootstrap_coeff_to_slot(shm_bootstrapper *b, shm_ciphertext *ct, shm_cip
  * |x| < K -> sin(2 pi x), same scale; 1 + ceil(log2(deg+1)) + 1 + double_angles levels */
 int shm_bootstrap_eval_mod(shm_bootstrapper *b, shm_ciphertext *ct, const shm_kswitch_key *relin,
                            shm_stream_t stream);
+/* ordinary_evalmod_combine_batch (BatchBootstrap.cuh:192-260) runs EvalMod over a
+ * chunk of ciphertexts: the same modular reduction on `count` (1..32) ciphertexts
+ * at one level and scale, in lockstep, with one key-switch and rescale pass per
+ * step for the group; each result equals shm_bootstrap_eval_mod's bit for bit. */
+int shm_bootstrap_eval_mod_batch(shm_bootstrapper *b, shm_ciphertext *const *cts, size_t count,
+                                 const shm_kswitch_key *relin, shm_stream_t stream);
 /* BatchBootstrapper::bootstrap_ordinary_s2c_first (BatchBootstrap.cuh:674-709) over
  * `count` ciphertexts with exactly 4 primes; out[i] receives the refreshed
  * ciphertext at scale out_scale (0 = the input scale). Inputs are not modified. */

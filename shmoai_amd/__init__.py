@@ -478,6 +478,11 @@ class Bootstrapper:
     def eval_mod(self, ct: Ciphertext, relin_key: KSwitchKey, stream=None):
         check(_lib().shm_bootstrap_eval_mod(self.h, ct.h, relin_key.h, stream))
 
+    def eval_mod_batch(self, cts: Sequence[Ciphertext], relin_key: KSwitchKey, stream=None):
+        """EvalMod of up to 32 ciphertexts at one level and scale, in lockstep."""
+        hs = (C.c_void_p * len(cts))(*[c.h.value for c in cts])
+        check(_lib().shm_bootstrap_eval_mod_batch(self.h, hs, len(cts), relin_key.h, stream))
+
     def shared_slot_to_coeff(self, keys: "SharedAKeys", batch: "SharedABatch", stream=None):
         check(_lib().shm_bootstrap_shared_slot_to_coeff(self.h, keys.h, batch.h, stream))
 

@@ -120,6 +120,7 @@ _SIGS = {
     "shm_bootstrap_slot_to_coeff": (i32, [vp, vp, vp, vp]),
     "shm_bootstrap_coeff_to_slot": (i32, [vp, vp, vp, vp, vp, vp]),
     "shm_bootstrap_eval_mod": (i32, [vp, vp, vp, vp]),
+    "shm_bootstrap_eval_mod_batch": (i32, [vp, vp, sz, vp, vp]),
     "shm_bootstrap_s2c_first": (i32, [vp, pvp, sz, pvp, vp, vp, dbl, vp]),
     "shm_shared_a_keys_create": (i32, [vp, vp, u32, u32, u64, u32, pvp]),
     "shm_shared_a_keys_destroy": (i32, [vp]),

@@ -104,8 +104,18 @@ void coeff_to_slot(Bootstrapper &b, Ciphertext &ct, Ciphertext &re, Ciphertext &
                    hipStream_t s);
 void eval_mod(Bootstrapper &b, Ciphertext &x, const KSwitchKey &rk, double out_const, double out_scale,
               hipStream_t s);
+// eval_mod of up to 32 ciphertexts at one level and scale, batched (bit-identical
+// to eval_mod per ciphertext)
+void eval_mod_batch(Bootstrapper &b, const std::vector<Ciphertext *> &xs, const KSwitchKey &rk, double out_const,
+                    double out_scale, hipStream_t s);
 void bootstrap_s2c_first(Bootstrapper &b, const Ciphertext &in, Ciphertext &out, const GaloisKeys &gk,
                          const KSwitchKey &rk, double out_scale, hipStream_t s);
+// bootstrap_s2c_first of several ciphertexts: groups of up to 16 inputs with one
+// scale share one batched EvalMod over their real and imaginary parts.
+// out_scale 0 keeps each input's scale.
+void bootstrap_s2c_first_batch(Bootstrapper &b, const std::vector<const Ciphertext *> &in,
+                               const std::vector<Ciphertext *> &out, const GaloisKeys &gk, const KSwitchKey &rk,
+                               double out_scale, hipStream_t s);
 // ---- shared-a batches (shared_a.hip) -------------------------------------------------
 // SharedACiphertextBatch (SharedACiphertext.cuh:22-99): ct has size n + 1;
 // polynomials 0..n-1 are the b_i, polynomial n the shared a (NTT form).

@@ -340,7 +340,213 @@ struct ChebEval {
     }
 };
 
+// ChebEval over m ciphertexts that go through the same polynomial in lockstep:
+// every product of the recursion is one mul_relin_rescale_batch for the group
+// and every rescale one rescale_batch, so each stage is a handful of launches
+// for all members and the relinearisation key is streamed once per group.
+// Member i sees exactly the operations ChebEval applies to it alone.
+struct ChebEvalBatch {
+    typedef std::vector<std::unique_ptr<Ciphertext>> Group;
+    Context &c;
+    const KSwitchKey &rk;
+    hipStream_t s;
+    size_t m;
+    std::vector<Group> T;
+
+    ChebEvalBatch(Context &c_, const KSwitchKey &rk_, hipStream_t s_, Group &&t1)
+        : c(c_), rk(rk_), s(s_), m(t1.size()), T(65) {
+        T[1] = std::move(t1);
+    }
+
+    static std::vector<Ciphertext *> ptrs(const Group &g) {
+        std::vector<Ciphertext *> v;
+        for (auto &p : g) v.push_back(p.get());
+        return v;
+    }
+    static std::vector<const Ciphertext *> cptrs(const Group &g) {
+        std::vector<const Ciphertext *> v;
+        for (auto &p : g) v.push_back(p.get());
+        return v;
+    }
+    Group fresh() const {
+        Group g(m);
+        for (auto &p : g) p = std::make_unique<Ciphertext>();
+        return g;
+    }
+    size_t tchain(int k) const { return T[1][0]->chain + ChebEval::depth(k); }
+
+    // ct_align of every member onto (chain, scale), one rescale pass for the group
+    void align(std::vector<Ciphertext *> xs, size_t chain, double scale) {
+        std::vector<Ciphertext *> todo;
+        for (Ciphertext *x : xs) {
+            if (x->chain > chain) throw LogicError("ct_align: ciphertext is already below the target level");
+            if (x->chain == chain) {
+                if (std::fabs(x->scale / scale - 1.0) > 1e-9) throw LogicError("ct_align: scales differ at equal levels");
+                x->scale = scale;
+                continue;
+            }
+            if (x->chain + 1 < chain) mod_switch_drop(c, *x, chain - 1, s);
+            const double qlast = (double)c.q[x->Ql() - 1];
+            ct_mul_const(c, *x, 1.0, scale * qlast / x->scale, s);
+            todo.push_back(x);
+        }
+        rescale_batch(c, todo, s, true);
+        for (Ciphertext *x : todo) x->scale = scale;
+    }
+
+    const Group &get(int k) {
+        if (!T[k].empty()) return T[k];
+        const int a = ChebEval::split_a(k), b = k - a;
+        const Group &ta = get(a);
+        const Group &tb = get(b);
+        Group out = fresh();
+        mul_relin_rescale_batch(c, ptrs(out), cptrs(ta), cptrs(tb), rk, s, true);
+        for (auto &o : out) ct_add(c, *o, *o, s);
+        if (a == b) {
+            for (auto &o : out) ct_add_const(c, *o, -1.0, s);
+        } else {
+            const Group &d = get(a - b);
+            Group tmp = fresh();
+            for (size_t i = 0; i < m; i++) ct_clone(*tmp[i], *d[i], s);
+            align(ptrs(tmp), out[0]->chain, out[0]->scale);
+            for (size_t i = 0; i < m; i++) ct_sub(c, *out[i], *tmp[i], s);
+        }
+        if (out[0]->chain != tchain(k)) throw LogicError("Chebyshev basis level bookkeeping");
+        T[k] = std::move(out);
+        return T[k];
+    }
+
+    size_t baby_chain(const std::vector<double> &p) const {
+        size_t ch = 0;
+        for (size_t k = 1; k < p.size(); k++)
+            if (p[k] != 0.0) ch = std::max(ch, tchain((int)k));
+        return ch + 1;
+    }
+    size_t eval_chain(const std::vector<double> &p) const {
+        const size_t deg = p.size() - 1;
+        if (deg < 16) return baby_chain(p);
+        const size_t mm = deg >= 32 ? 32 : 16;
+        std::vector<double> q, r;
+        cheb_split(p, mm, q, r);
+        return std::max(tchain((int)mm), eval_chain(r)) + 1;
+    }
+
+    void baby(const std::vector<double> &p, size_t chain, double scale, const Group &out) {
+        const size_t Qc = c.ql_of_chain(chain - 1);
+        const double ql = (double)c.q[Qc - 1];
+        for (size_t k = 1; k < p.size(); k++)
+            if (p[k] != 0.0) get((int)k);
+        for (size_t i = 0; i < m; i++) {
+            std::vector<const Ciphertext *> src;
+            std::vector<double> vals, sig;
+            for (size_t k = 1; k < p.size(); k++) {
+                if (p[k] == 0.0) continue;
+                const Ciphertext &t = *T[k][i];
+                if (t.chain > chain - 1) throw LogicError("Chebyshev basis element below the combination level");
+                src.push_back(&t);
+                vals.push_back(p[k]);
+                sig.push_back(scale * ql / t.scale);
+            }
+            if (src.empty()) throw LogicError("constant-only Chebyshev block");
+            const_lincomb(c, *out[i], src, vals, sig, Qc, s);
+            out[i]->scale = scale * ql;
+        }
+        rescale_batch(c, ptrs(out), s, true);
+        for (auto &o : out) {
+            o->scale = scale;
+            if (p[0] != 0.0) ct_add_const(c, *o, p[0], s);
+        }
+    }
+
+    void eval(const std::vector<double> &p, double target, const Group &out) {
+        const size_t deg = p.size() - 1;
+        if (deg < 16) {
+            baby(p, baby_chain(p), target, out);
+            return;
+        }
+        const size_t mm = deg >= 32 ? 32 : 16;
+        std::vector<double> q, r;
+        cheb_split(p, mm, q, r);
+        const size_t pc = std::max(tchain((int)mm), eval_chain(r));
+        const double qp = (double)c.q[c.ql_of_chain(pc) - 1];
+        const Group &Tm = get((int)mm);
+        Group R = fresh();
+        eval(r, target * qp / Tm[0]->scale, R);
+        mul_relin_rescale_batch(c, ptrs(out), cptrs(Tm), cptrs(R), rk, s, true);
+        for (auto &o : out) o->scale = target;
+        bool q_terms = false;
+        for (size_t k = 1; k < q.size(); k++) q_terms |= q[k] != 0.0;
+        if (!q_terms) {
+            if (q[0] != 0.0)
+                for (auto &o : out) ct_add_const(c, *o, q[0], s);
+            return;
+        }
+        Group Q = fresh();
+        if (q.size() <= 16) {
+            baby(q, out[0]->chain, target, Q);
+        } else {
+            eval(q, target, Q);
+            std::vector<Ciphertext *> low;
+            for (auto &x : Q)
+                if (x->chain < out[0]->chain) low.push_back(x.get());
+            align(low, out[0]->chain, target);
+        }
+        for (size_t i = 0; i < m; i++) add_checked(c, *out[i], *Q[i], s);
+    }
+};
+
 }  // namespace
+
+// modular_reduction of several ciphertexts at once (same level and scale, up to
+// 32): the batched form of eval_mod below, bit-identical to it member by member
+// (BatchBootstrap.cuh:192 ordinary_evalmod_combine_batch runs EvalMod per chunk).
+void eval_mod_batch(Bootstrapper &b, const std::vector<Ciphertext *> &xs, const KSwitchKey &rk, double out_const,
+                    double out_scale, hipStream_t s) {
+    Context &c = *b.ctx;
+    const size_t m = xs.size();
+    if (m == 0) return;
+    if (m > 32) throw InvalidArgument("eval_mod_batch: at most 32 ciphertexts");
+    const int r = b.prm.double_angles;
+    ChebEvalBatch::Group t1(m);
+    for (size_t i = 0; i < m; i++) {
+        if (xs[i]->chain != xs[0]->chain || xs[i]->scale != xs[0]->scale)
+            throw InvalidArgument("eval_mod_batch: inputs at different levels or scales");
+        t1[i] = std::make_unique<Ciphertext>();
+        ct_clone(*t1[i], *xs[i], s);
+    }
+    const double s1 = xs[0]->scale;
+    for (auto &t : t1)
+        ct_mul_const(c, *t, 1.0 / (double)b.prm.K, s1 * (double)c.q[t->Ql() - 1] / t->scale, s);
+    rescale_batch(c, ChebEvalBatch::ptrs(t1), s, true);
+    for (auto &t : t1) t->scale = s1;
+    ChebEvalBatch ev(c, rk, s, std::move(t1));
+    std::vector<double> coeffs = b.cheb;
+    if (r == 0)
+        for (auto &v : coeffs) v *= out_const;
+    const size_t cp = ev.eval_chain(coeffs);
+    if (c.ql_of_chain(cp) < (size_t)r + 1) throw LogicError("end of modulus switching chain reached");
+    std::vector<double> sig(r + 1);
+    sig[r] = out_scale;
+    for (int j = r - 1; j >= 0; j--) {
+        const double qj = (double)c.q[c.ql_of_chain(cp + j) - 1];
+        sig[j] = std::sqrt(2.0 * (j == r - 1 ? out_const : 1.0) * qj * sig[j + 1]);
+    }
+    ChebEvalBatch::Group out = ev.fresh();
+    ev.eval(coeffs, sig[0], out);
+    if (out[0]->chain != cp) throw LogicError("EvalMod level bookkeeping");
+    for (int j = 0; j < r; j++) {
+        ChebEvalBatch::Group sq = ev.fresh();
+        mul_relin_rescale_batch(c, ChebEvalBatch::ptrs(sq), ChebEvalBatch::cptrs(out), ChebEvalBatch::cptrs(out), rk,
+                                s, true);
+        const double cst = j == r - 1 ? out_const : 1.0;
+        for (auto &x : sq) {
+            x->scale = sig[j + 1];  // = sig[j]^2 / (2 cst q_j): the square read as 2 cst c^2
+            ct_add_const(c, *x, -cst, s);
+        }
+        out = std::move(sq);
+    }
+    for (size_t i = 0; i < m; i++) *xs[i] = std::move(*out[i]);
+}
 
 // ModularReducer::modular_reduction (ModularReducer.cu:64-84). x holds slots
 // x' = eps + integer with |x'| < K at any scale; the result holds
@@ -387,34 +593,57 @@ void eval_mod(Bootstrapper &b, Ciphertext &x, const KSwitchKey &rk, double out_c
 
 void bootstrap_s2c_first(Bootstrapper &b, const Ciphertext &in, Ciphertext &out, const GaloisKeys &gk,
                          const KSwitchKey &rk, double out_scale, hipStream_t s) {
+    bootstrap_s2c_first_batch(b, {&in}, {&out}, gk, rk, out_scale, s);
+}
+
+void bootstrap_s2c_first_batch(Bootstrapper &b, const std::vector<const Ciphertext *> &in,
+                               const std::vector<Ciphertext *> &out, const GaloisKeys &gk, const KSwitchKey &rk,
+                               double out_scale, hipStream_t s) {
     Context &c = *b.ctx;
-    if (in.size != 2) throw InvalidArgument("bootstrapping takes size-2 ciphertexts");
-    if (in.Ql() != b.prm.s2c_input_primes)
-        throw InvalidArgument("bootstrap_s2c_first: inputs must have exactly 4 primes (lS2C+1)");
+    if (in.size() != out.size()) throw InvalidArgument("bootstrap: input and output counts differ");
+    for (const Ciphertext *x : in) {
+        if (x->size != 2) throw InvalidArgument("bootstrapping takes size-2 ciphertexts");
+        if (x->Ql() != b.prm.s2c_input_primes)
+            throw InvalidArgument("bootstrap_s2c_first: inputs must have exactly 4 primes (lS2C+1)");
+    }
     if (!b.prepared) b.prepare(s);
-    const double d_in = in.scale;
-    Ciphertext x;
-    ct_clone(x, in, s);
-    slot_to_coeff(b, x, gk, s);
-    ct_modraise(c, x, s);
     const double q0 = (double)c.q[0];
-    x.scale = q0;
-    Ciphertext re, im;
-    coeff_to_slot(b, x, re, im, gk, s);
-    // amplitude correction (BatchBootstrap.cuh:444-460) folded into EvalMod:
-    // sin(2 pi eps)/(2 pi) ~ eps, with the linear minimax constant over
-    // |eps| <= E = F * scale_in * bound / q0 absorbing the cubic term (error
-    // (2 pi)^2 E^3 / 24), times q0 / (F * scale_in) to return to the message.
-    const double F = b.prm.layer3_factor;
-    const double E = F * d_in * b.prm.input_bound / q0;
-    const double pi = 3.14159265358979323846;
-    const double cst = (1.0 + 0.5 * pi * pi * E * E) / (2.0 * pi) * q0 / (F * d_in);
-    eval_mod(b, re, rk, cst, out_scale, s);
-    eval_mod(b, im, rk, cst, out_scale, s);
-    ct_mul_monomial_half(c, im, false, s);  // i * m_im
-    add_checked(c, re, im, s);
-    re.scale = out_scale;
-    out = std::move(re);
+    for (size_t i = 0; i < in.size();) {
+        size_t j = i + 1;
+        while (j < in.size() && j - i < 16 && in[j]->scale == in[i]->scale) j++;
+        const size_t g = j - i;
+        const double d_in = in[i]->scale, osc = out_scale > 0 ? out_scale : d_in;
+        std::vector<Ciphertext> re(g), im(g);
+        for (size_t k = 0; k < g; k++) {
+            Ciphertext x;
+            ct_clone(x, *in[i + k], s);
+            slot_to_coeff(b, x, gk, s);
+            ct_modraise(c, x, s);
+            x.scale = q0;
+            coeff_to_slot(b, x, re[k], im[k], gk, s);
+        }
+        // amplitude correction (BatchBootstrap.cuh:444-460) folded into EvalMod:
+        // sin(2 pi eps)/(2 pi) ~ eps, with the linear minimax constant over
+        // |eps| <= E = F * scale_in * bound / q0 absorbing the cubic term (error
+        // (2 pi)^2 E^3 / 24), times q0 / (F * scale_in) to return to the message.
+        const double F = b.prm.layer3_factor;
+        const double E = F * d_in * b.prm.input_bound / q0;
+        const double pi = 3.14159265358979323846;
+        const double cst = (1.0 + 0.5 * pi * pi * E * E) / (2.0 * pi) * q0 / (F * d_in);
+        std::vector<Ciphertext *> parts;
+        for (size_t k = 0; k < g; k++) {
+            parts.push_back(&re[k]);
+            parts.push_back(&im[k]);
+        }
+        eval_mod_batch(b, parts, rk, cst, osc, s);
+        for (size_t k = 0; k < g; k++) {
+            ct_mul_monomial_half(c, im[k], false, s);  // i * m_im
+            add_checked(c, re[k], im[k], s);
+            re[k].scale = osc;
+            *out[i + k] = std::move(re[k]);
+        }
+        i = j;
+    }
 }
 
 }  // namespace boot

@@ -942,6 +942,22 @@ int shm_bootstrap_eval_mod(shm_bootstrapper *b, shm_ciphertext *ct, const shm_ks
     });
 }
 
+int shm_bootstrap_eval_mod_batch(shm_bootstrapper *b, shm_ciphertext *const *cts, size_t count,
+                                 const shm_kswitch_key *relin, shm_stream_t st) {
+    return guard([&] {
+        need(relin, "relin_keys");
+        if (count == 0 || count > 32) throw InvalidArgument("eval_mod_batch: 1..32 ciphertexts");
+        if (!cts) throw InvalidArgument("cts is null");
+        std::vector<Ciphertext *> xs(count);
+        for (size_t i = 0; i < count; i++) {
+            need(cts[i], "cts[i]");
+            xs[i] = &cts[i]->ct;
+        }
+        const double sc = xs[0]->scale;
+        boot::eval_mod_batch(ready(b, S(st)), xs, relin->k, 1.0, sc, S(st));
+    });
+}
+
 int shm_bootstrap_s2c_first(shm_bootstrapper *b, const shm_ciphertext *const *in, size_t count,
                             shm_ciphertext *const *out, const shm_galois_keys *gk, const shm_kswitch_key *relin,
                             double out_scale, shm_stream_t st) {
@@ -954,9 +970,13 @@ int shm_bootstrap_s2c_first(shm_bootstrapper *b, const shm_ciphertext *const *in
             need(in[i], "in[i]");
             need(out[i], "out[i]");
         }
-        for (size_t i = 0; i < count; i++)
-            boot::bootstrap_s2c_first(bs, in[i]->ct, out[i]->ct, gk->g, relin->k,
-                                      out_scale > 0 ? out_scale : in[i]->ct.scale, S(st));
+        std::vector<const Ciphertext *> ins(count);
+        std::vector<Ciphertext *> outs(count);
+        for (size_t i = 0; i < count; i++) {
+            ins[i] = &in[i]->ct;
+            outs[i] = &out[i]->ct;
+        }
+        boot::bootstrap_s2c_first_batch(bs, ins, outs, gk->g, relin->k, out_scale, S(st));
         SHM_HIP(hipStreamSynchronize(S(st)));
     });
 }

@@ -233,6 +233,18 @@ void rescale_ntt_fused(const Context &c, const uint64_t *last, const uint64_t *a
                        uint64_t *delta, uint64_t *const *outs, uint64_t *out_single, size_t ppo, size_t n_polys,
                        size_t L1, const uint64_t *qinv, const uint64_t *qinv_s, hipStream_t s, bool rounded = false);
 
+// Per-polynomial base pointers for batched calls over polynomials that live in
+// separate buffers (passed to kernels by value).
+constexpr int kPolyPack = 64;
+struct PolyPack {
+    uint64_t *p[kPolyPack];
+};
+// The same rescale over n_polys polynomials in separate buffers: input p at
+// in.p[p] ([L1 + 1][N]), output p at out.p[p] ([L1][N]).
+void rescale_ntt_fused_pack(const Context &c, const uint64_t *last, const PolyPack &in, const PolyPack &out,
+                            uint64_t *delta, size_t n_polys, size_t L1, const uint64_t *qinv, const uint64_t *qinv_s,
+                            hipStream_t s, bool rounded);
+
 // elementwise (poly.hip)
 void launch_add(const Context &c, uint64_t *a, const uint64_t *b, size_t Ql, size_t npolys, hipStream_t s);
 void launch_sub(const Context &c, uint64_t *a, const uint64_t *b, size_t Ql, size_t npolys, hipStream_t s);
@@ -244,6 +256,9 @@ void launch_rescale(const Context &c, const uint64_t *in, uint64_t *out, size_t 
                     bool rounded = false);
 void launch_permute(const Context &c, const uint64_t *in, uint64_t *out, const uint32_t *table, size_t nlimbs,
                     hipStream_t s);
+// out[p][i] = in.p[p][i], i < words, p < npolys
+void launch_gather_polys(const Context &c, const PolyPack &in, uint64_t *out, size_t words, size_t npolys,
+                         hipStream_t s);
 void launch_copy_limbs(const Context &c, const uint64_t *in, size_t in_limbs, uint64_t *out, size_t out_limbs,
                        size_t npolys, hipStream_t s);
 // samplers
@@ -284,6 +299,14 @@ void ks_inner_multi(const Context &c, const uint64_t *d, size_t Ql, size_t ndig,
 // ModDown of npolys [QlP][N] polynomials into out + p * out_stride (bit p of acc_mask: accumulate)
 void ks_moddown(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, uint64_t *out, size_t out_stride,
                 uint64_t acc_mask, hipStream_t s, bool rounded = false);
+// The same with polynomial p written to out.p[p]
+void ks_moddown_pack(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, const PolyPack &out,
+                     uint64_t acc_mask, hipStream_t s, bool rounded = false);
+// Key switch of m polynomials c2.p[i] (Ql limbs, NTT) with one key; the
+// switched pair of member i is added to (ct01.p[2i], ct01.p[2i + 1]). One
+// launch per stage for the whole group, each key word read once per group.
+void keyswitch_batch(const Context &c, const PolyPack &c2, size_t m, const uint64_t *key, size_t Ql,
+                     const PolyPack &ct01, hipStream_t s, bool rounded = false);
 
 // encoder (encoder.hip)
 void encode(const Context &c, const double *re, const double *im, size_t n_values, size_t Ql, double scale,
@@ -306,6 +329,8 @@ void gen_kswitch(Context &c, const SecretKey &sk, const uint64_t *newkey_dev, ui
 void apply_galois(Context &c, Ciphertext &ct, uint32_t elt, const GaloisKeys &gk, hipStream_t s, bool rounded = false);
 void rotate_internal(Context &c, Ciphertext &ct, int step, const GaloisKeys &gk, hipStream_t s, bool rounded = false);
 void rescale_inplace(Context &c, Ciphertext &ct, hipStream_t s, bool rounded = false);
+// rescale_inplace of ciphertexts at one level in one pass (<= 64 polynomials)
+void rescale_batch(Context &c, const std::vector<Ciphertext *> &cts, hipStream_t s, bool rounded = false);
 // Rotations of one ciphertext by several steps sharing one ModUp of c1 (hoisting,
 // evaluate.cu:1727-1920 hoisting_inplace); outs[i] receives rot(ct, steps[i]).
 // Steps without their own key fall back to rotate_internal.
@@ -324,6 +349,12 @@ void ct_mul_const_rescale(Context &c, Ciphertext &a, double v, double sigma, hip
 void ct_align(Context &c, Ciphertext &a, size_t chain, double scale, hipStream_t s, bool rounded = false);
 void ct_mul_relin_rescale(Context &c, Ciphertext &out, const Ciphertext &a, const Ciphertext &b,
                           const KSwitchKey &rk, hipStream_t s, bool rounded = false);
+// ct_mul_relin_rescale of out[i] = a[i] * b[i], i < m <= 32, batched
+void mul_relin_rescale_batch(Context &c, const std::vector<Ciphertext *> &out,
+                             const std::vector<const Ciphertext *> &a, const std::vector<const Ciphertext *> &b,
+                             const KSwitchKey &rk, hipStream_t s, bool rounded = false);
+// a <- a * [v encoded at scale sigma] (no rescale)
+void ct_mul_const(Context &c, Ciphertext &a, double v, double sigma, hipStream_t s);
 void ct_mul_monomial_half(Context &c, Ciphertext &a, bool negate, hipStream_t s);
 void ct_modraise(Context &c, Ciphertext &a, hipStream_t s);
 

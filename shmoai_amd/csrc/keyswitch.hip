@@ -37,17 +37,21 @@ constexpr int kMaxAlpha = 16;
 
 // Blocks never straddle two digits (N is a multiple of the block size), so the
 // digit index and every table address derived from it are wave-uniform.
-__global__ void k_modup_bconv(const uint64_t *__restrict__ c2, const uint64_t *__restrict__ tcoef,
-                              uint64_t *__restrict__ out, const DevMod *__restrict__ mods,
-                              const uint64_t *__restrict__ hat_inv, const uint64_t *__restrict__ hat_inv_s,
-                              const uint64_t *__restrict__ hat_mod, const uint64_t *__restrict__ dig_mod, size_t Ql,
-                              size_t nQ, size_t alpha, size_t beta, size_t N, int rounded) {
+// `members` polynomials are converted at once: member m reads tcoef + m Ql N
+// and writes out + m beta QlP N.
+__global__ void k_modup_bconv(const uint64_t *__restrict__ tcoef_all, uint64_t *__restrict__ out_all,
+                              const DevMod *__restrict__ mods, const uint64_t *__restrict__ hat_inv,
+                              const uint64_t *__restrict__ hat_inv_s, const uint64_t *__restrict__ hat_mod,
+                              const uint64_t *__restrict__ dig_mod, size_t Ql, size_t nQ, size_t alpha, size_t beta,
+                              size_t N, size_t members, int rounded) {
     const size_t QlP = Ql + alpha;
-    const size_t total = beta * N;
+    const size_t total = members * beta * N;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t g = __builtin_amdgcn_readfirstlane((uint32_t)(t / N)), j = t % N;
+        const size_t gm = __builtin_amdgcn_readfirstlane((uint32_t)(t / N)), j = t % N;
+        const size_t mem = gm / beta, g = gm % beta;
+        const uint64_t *tcoef = tcoef_all + mem * Ql * N;
         const size_t start = g * alpha, size = min(alpha, Ql - start);
-        uint64_t *o = out + g * QlP * N + j;
+        uint64_t *o = out_all + (mem * beta + g) * QlP * N + j;
         uint64_t y[kMaxAlpha];
         double v = 0.0;
 #pragma unroll
@@ -183,19 +187,57 @@ __global__ void k_moddown_bconv(const uint64_t *__restrict__ cx, uint64_t *__res
     }
 }
 
-// out = (cx - delta) P^-1 mod q; then ct (+)= out. Over npolys polynomials:
-// cx [p][QlP][N], delta [p][Ql][N], ct [p] at ct + p * ct_stride; bit p of
+// cx[m][0/1][k][j] = sum_g d[m][g][k][j] * key_g.(b/a)[limb(k)][j] for `members`
+// digit sets: the 2 beta key words of (k, j) are read once and reused for every
+// member (eval_key_switch.cu:12-60 per member).
+__global__ void k_ks_inner_batch(const uint64_t *__restrict__ d, const uint64_t *__restrict__ key,
+                                 uint64_t *__restrict__ cx, const DevMod *__restrict__ mods, size_t Ql, size_t nQ,
+                                 size_t QP, size_t alpha, size_t beta, size_t N, size_t members) {
+    const size_t QlP = Ql + alpha, total = QlP * N;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t k = t / N, j = t % N;
+        const uint32_t pi = qlp_prime(k, Ql, nQ);
+        uint64_t kb[kMaxDigits], ka[kMaxDigits];
+#pragma unroll
+        for (int g = 0; g < kMaxDigits; g++) {
+            kb[g] = ka[g] = 0;
+            if ((size_t)g < beta) {
+                const uint64_t *kg = key + g * 2 * QP * N;
+                kb[g] = kg[(size_t)pi * N + j];
+                ka[g] = kg[(QP + pi) * N + j];
+            }
+        }
+        const DevMod m = mods[pi];
+        for (size_t mem = 0; mem < members; mem++) {
+            const uint64_t *dm = d + mem * beta * total + t;
+            Acc128 a0{0, 0}, a1{0, 0};
+#pragma unroll
+            for (int g = 0; g < kMaxDigits; g++) {
+                if ((size_t)g < beta) {
+                    const uint64_t dv = dm[g * total];
+                    mac128(a0, dv, kb[g]);
+                    mac128(a1, dv, ka[g]);
+                }
+            }
+            cx[(2 * mem) * total + t] = reduce128(a0, m);
+            cx[(2 * mem + 1) * total + t] = reduce128(a1, m);
+        }
+    }
+}
+
+// out = (cx - delta) P^-1 mod q, then ct (+)= out, over npolys polynomials:
+// cx [p][QlP][N], delta [p][Ql][N], output polynomial p at out.p[p]; bit p of
 // acc_mask selects accumulation.
-__global__ void k_moddown_combine(const uint64_t *__restrict__ cx, const uint64_t *__restrict__ delta,
-                                  uint64_t *__restrict__ ct, const DevMod *__restrict__ mods,
-                                  const uint64_t *__restrict__ pinv, const uint64_t *__restrict__ pinv_s, size_t Ql,
-                                  size_t nP, size_t N, size_t npolys, size_t ct_stride, uint64_t acc_mask) {
+__global__ void k_moddown_combine_pack(const uint64_t *__restrict__ cx, const uint64_t *__restrict__ delta,
+                                       PolyPack out, const DevMod *__restrict__ mods, const uint64_t *__restrict__ pinv,
+                                       const uint64_t *__restrict__ pinv_s, size_t Ql, size_t nP, size_t N,
+                                       size_t npolys, uint64_t acc_mask) {
     const size_t S = Ql * N, total = npolys * S;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
         const size_t p = t / S, r = t % S, i = r / N;
         const uint64_t q = mods[i].q;
         const uint64_t v = mul_shoup(sub_mod(cx[p * (S + nP * N) + r], delta[t], q), pinv[i], pinv_s[i], q);
-        uint64_t *o = ct + p * ct_stride + r;
+        uint64_t *o = out.p[p] + r;
         *o = ((acc_mask >> p) & 1) ? add_mod(*o, v, q) : v;
     }
 }
@@ -211,8 +253,8 @@ void ks_modup(const Context &c, const uint64_t *c2, size_t Ql, uint64_t *d, hipS
     ntt_inverse(c, tcoef.as<uint64_t>(), 1, Ql, 0, s);
     if (alpha > (size_t)kMaxAlpha) throw InvalidArgument("at most 16 special primes");
     k_modup_bconv<<<grid_for(beta * N), kBlock, 0, s>>>(
-        c2, tcoef.as<uint64_t>(), d, c.mods(), L.hat_inv.as<uint64_t>(), L.hat_inv_s.as<uint64_t>(),
-        L.hat_mod.as<uint64_t>(), L.dig_mod.as<uint64_t>(), Ql, c.nQ, alpha, beta, N, rounded ? 1 : 0);
+        tcoef.as<uint64_t>(), d, c.mods(), L.hat_inv.as<uint64_t>(), L.hat_inv_s.as<uint64_t>(),
+        L.hat_mod.as<uint64_t>(), L.dig_mod.as<uint64_t>(), Ql, c.nQ, alpha, beta, N, 1, rounded ? 1 : 0);
     SHM_HIP(hipGetLastError());
     // one launch over every digit's QlP limbs (the own limbs were written in
     // coefficient form and transform back to their NTT-form input words)
@@ -231,6 +273,14 @@ void ks_inner_multi(const Context &c, const uint64_t *d, size_t Ql, size_t ndig,
 
 void ks_moddown(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, uint64_t *out, size_t out_stride,
                 uint64_t acc_mask, hipStream_t s, bool rounded) {
+    if (npolys == 0 || npolys > 64) throw InvalidArgument("ks_moddown: 1..64 polynomials");
+    PolyPack pk{};
+    for (size_t p = 0; p < npolys; p++) pk.p[p] = out + p * out_stride;
+    ks_moddown_pack(c, cx, npolys, Ql, pk, acc_mask, s, rounded);
+}
+
+void ks_moddown_pack(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, const PolyPack &out,
+                     uint64_t acc_mask, hipStream_t s, bool rounded) {
     const size_t N = c.N, nP = c.nP, QlP = Ql + nP;
     if (npolys == 0 || npolys > 64) throw InvalidArgument("ks_moddown: 1..64 polynomials");
     Scratch delta(npolys * Ql * N * sizeof(uint64_t), s);
@@ -241,9 +291,10 @@ void ks_moddown(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, uint64
         c.phat_mod.as<uint64_t>(), c.pmod.as<uint64_t>(), Ql, c.nQ, nP, N, npolys, rounded ? 1 : 0);
     SHM_HIP(hipGetLastError());
     ntt_forward(c, delta.as<uint64_t>(), npolys, Ql, 0, s);
-    k_moddown_combine<<<grid_for(npolys * Ql * N), kBlock, 0, s>>>(cx, delta.as<uint64_t>(), out, c.mods(),
-                                                                   c.pinv.as<uint64_t>(), c.pinv_s.as<uint64_t>(),
-                                                                   Ql, nP, N, npolys, out_stride, acc_mask);
+    k_moddown_combine_pack<<<grid_for(npolys * Ql * N), kBlock, 0, s>>>(cx, delta.as<uint64_t>(), out, c.mods(),
+                                                                        c.pinv.as<uint64_t>(),
+                                                                        c.pinv_s.as<uint64_t>(), Ql, nP, N, npolys,
+                                                                        acc_mask);
     SHM_HIP(hipGetLastError());
 }
 
@@ -264,6 +315,34 @@ void keyswitch(const Context &c, const uint64_t *c2, const uint64_t *key, size_t
         ks_moddown(c, cx.as<uint64_t>(), 1, Ql, ct0, Ql * N, 1, s, rounded);
         ks_moddown(c, cx.as<uint64_t>() + QlP * N, 1, Ql, ct1, Ql * N, add_to_ct1 ? 1 : 0, s, rounded);
     }
+}
+
+void keyswitch_batch(const Context &c, const PolyPack &c2, size_t m, const uint64_t *key, size_t Ql,
+                     const PolyPack &ct01, hipStream_t s, bool rounded) {
+    const size_t N = c.N, alpha = c.nP, QlP = Ql + alpha;
+    const LevelConsts &L = c.level(Ql);
+    const size_t beta = L.beta;
+    if (m == 0 || 2 * m > (size_t)kPolyPack) throw InvalidArgument("keyswitch_batch: 1..32 members");
+    if (beta > c.dnum) throw LogicError("key switching key has too few digits for this level");
+    if (alpha > (size_t)kMaxAlpha) throw InvalidArgument("at most 16 special primes");
+    // ModUp of all members: gather + iNTT, conversion, one NTT over m beta QlP limbs
+    Scratch tcoef(m * Ql * N * sizeof(uint64_t), s);
+    launch_gather_polys(c, c2, tcoef.as<uint64_t>(), Ql * N, m, s);
+    ntt_inverse(c, tcoef.as<uint64_t>(), m, Ql, 0, s);
+    Scratch d(m * beta * QlP * N * sizeof(uint64_t), s);
+    k_modup_bconv<<<grid_for(m * beta * N), kBlock, 0, s>>>(
+        tcoef.as<uint64_t>(), d.as<uint64_t>(), c.mods(), L.hat_inv.as<uint64_t>(), L.hat_inv_s.as<uint64_t>(),
+        L.hat_mod.as<uint64_t>(), L.dig_mod.as<uint64_t>(), Ql, c.nQ, alpha, beta, N, m, rounded ? 1 : 0);
+    SHM_HIP(hipGetLastError());
+    ntt_forward_qlp(c, d.as<uint64_t>(), Ql, 0, QlP, 0, 0, m * beta, QlP * N, s);
+    // inner products: each key word read once for the group
+    Scratch cx(2 * m * QlP * N * sizeof(uint64_t), s);
+    k_ks_inner_batch<<<grid_for(QlP * N), kBlock, 0, s>>>(d.as<uint64_t>(), key, cx.as<uint64_t>(), c.mods(), Ql,
+                                                          c.nQ, c.nQP(), alpha, beta, N, m);
+    SHM_HIP(hipGetLastError());
+    // ModDown of the 2m polynomials, added into the members' (c0, c1)
+    const uint64_t mask = 2 * m == 64 ? ~uint64_t(0) : ((uint64_t(1) << (2 * m)) - 1);
+    ks_moddown_pack(c, cx.as<uint64_t>(), 2 * m, Ql, ct01, mask, s, rounded);
 }
 
 }  // namespace shm

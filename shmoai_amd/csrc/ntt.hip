@@ -213,6 +213,8 @@ struct NttIO {
     uint32_t acc_limbs;
     uint32_t ppo;
     uint64_t center;          // IO_REDUCE: 0, or h = q_last >> 1: values above h stand for v - q_last
+    uint32_t packed;          // IO_RESCALE: nonzero -> polynomial p reads acc_p[p], writes out_p[p]
+    PolyPack acc_p, out_p;
 };
 
 // Phase A (columns, first log N1 stages, log N1 <= 5). Each thread owns one
@@ -300,7 +302,8 @@ __global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ 
     constexpr int PER = kTile / kThreads;
     uint64_t accv[IO == IO_RESCALE ? PER : 1];
     if (FWD && IO == IO_RESCALE) {
-        const uint64_t *acc = io.acc + ((size_t)poly * io.acc_limbs + k) * N + t0;
+        const uint64_t *acc = io.packed ? io.acc_p.p[poly] + (size_t)k * N + t0
+                                        : io.acc + ((size_t)poly * io.acc_limbs + k) * N + t0;
 #pragma unroll
         for (int j = 0; j < PER; j++) accv[j] = acc[threadIdx.x + j * kThreads];
     }
@@ -308,8 +311,13 @@ __global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ 
     run_rounds<LOGT, FWD>(lds, N1, 1u, (uint32_t)(t0 / T), tb.tw + (size_t)prime * N, tb.tws + (size_t)prime * N,
                           IntK{q});
     if (FWD && IO == IO_RESCALE) {
-        uint64_t *ob = io.outs ? io.outs[poly / io.ppo] : io.out_single;
-        uint64_t *out = ob + ((size_t)(poly % io.ppo) * mp.n_active + k) * N + t0;
+        uint64_t *out;
+        if (io.packed) {
+            out = io.out_p.p[poly] + (size_t)k * N + t0;
+        } else {
+            uint64_t *ob = io.outs ? io.outs[poly / io.ppo] : io.out_single;
+            out = ob + ((size_t)(poly % io.ppo) * mp.n_active + k) * N + t0;
+        }
         const uint64_t qi = io.qinv[prime], qis = io.qinv_s[prime];
 #pragma unroll
         for (int j = 0; j < PER; j++) {
@@ -435,8 +443,32 @@ void rescale_ntt_fused(const Context &c, const uint64_t *last, const uint64_t *a
                        uint64_t *delta, uint64_t *const *outs, uint64_t *out_single, size_t ppo, size_t n_polys,
                        size_t L1, const uint64_t *qinv, const uint64_t *qinv_s, hipStream_t s, bool rounded) {
     LimbMap mp{(uint32_t)L1, 0, 0xffffffffu, 0, 0xffffffffu, (uint32_t)c.nQ, 0, (uint64_t)L1 * c.N};
-    NttIO io{last, acc, outs, out_single, qinv, qinv_s, (uint32_t)acc_limbs, (uint32_t)ppo,
-             rounded ? (c.q[L1] >> 1) : 0};
+    NttIO io{};
+    io.src = last;
+    io.acc = acc;
+    io.outs = outs;
+    io.out_single = out_single;
+    io.qinv = qinv;
+    io.qinv_s = qinv_s;
+    io.acc_limbs = (uint32_t)acc_limbs;
+    io.ppo = (uint32_t)ppo;
+    io.center = rounded ? (c.q[L1] >> 1) : 0;
+    launch_ntt<true, IO_REDUCE, IO_RESCALE>(c, delta, mp, n_polys * L1, s, io);
+}
+
+void rescale_ntt_fused_pack(const Context &c, const uint64_t *last, const PolyPack &in, const PolyPack &out,
+                            uint64_t *delta, size_t n_polys, size_t L1, const uint64_t *qinv, const uint64_t *qinv_s,
+                            hipStream_t s, bool rounded) {
+    if (n_polys == 0 || n_polys > (size_t)kPolyPack) throw InvalidArgument("rescale: 1..64 polynomials per batch");
+    LimbMap mp{(uint32_t)L1, 0, 0xffffffffu, 0, 0xffffffffu, (uint32_t)c.nQ, 0, (uint64_t)L1 * c.N};
+    NttIO io{};
+    io.src = last;
+    io.qinv = qinv;
+    io.qinv_s = qinv_s;
+    io.center = rounded ? (c.q[L1] >> 1) : 0;
+    io.packed = 1;
+    io.acc_p = in;
+    io.out_p = out;
     launch_ntt<true, IO_REDUCE, IO_RESCALE>(c, delta, mp, n_polys * L1, s, io);
 }
 

@@ -247,15 +247,61 @@ void ct_add_const(Context &c, Ciphertext &a, double v, hipStream_t s) {
     launch_add(c, a.data(), pt.as<uint64_t>(), Ql, 1, s);
 }
 
-// a <- rescale(a * [v encoded at scale sigma]); scale becomes a.scale * sigma / q_last.
-void ct_mul_const_rescale(Context &c, Ciphertext &a, double v, double sigma, hipStream_t s, bool rounded) {
+// a <- a * [v encoded at scale sigma]; scale becomes a.scale * sigma.
+void ct_mul_const(Context &c, Ciphertext &a, double v, double sigma, hipStream_t s) {
     const size_t Ql = a.Ql();
-    if (Ql < 2) throw LogicError("end of modulus switching chain reached");
     Scratch pt(Ql * c.N * sizeof(uint64_t), s);
     encode_scalar(c, v, Ql, sigma, pt.as<uint64_t>(), s);
     launch_mul_plain(c, a.data(), pt.as<uint64_t>(), Ql, a.size, s);
     a.scale *= sigma;
+}
+
+// a <- rescale(a * [v encoded at scale sigma]); scale becomes a.scale * sigma / q_last.
+void ct_mul_const_rescale(Context &c, Ciphertext &a, double v, double sigma, hipStream_t s, bool rounded) {
+    if (a.Ql() < 2) throw LogicError("end of modulus switching chain reached");
+    ct_mul_const(c, a, v, sigma, s);
     rescale_inplace(c, a, s, rounded);
+}
+
+// rescale_inplace of several ciphertexts at one level as one pass: one gather of
+// the last limbs, one iNTT launch and one fused NTT launch for all polynomials,
+// each output written to a fresh buffer.
+void rescale_batch(Context &c, const std::vector<Ciphertext *> &cts, hipStream_t s, bool rounded) {
+    if (cts.empty()) return;
+    const size_t Ql = cts[0]->Ql(), N = c.N;
+    if (Ql < 2) throw LogicError("end of modulus switching chain reached");
+    size_t np = 0;
+    for (const Ciphertext *ct : cts) {
+        if (ct->Ql() != Ql) throw InvalidArgument("rescale_batch: ciphertexts at different levels");
+        if (!ct->ntt) throw InvalidArgument("CKKS encrypted must be in NTT form");
+        np += ct->size;
+    }
+    if (np > (size_t)kPolyPack) throw InvalidArgument("rescale_batch: at most 64 polynomials");
+    const size_t L1 = Ql - 1;
+    PolyPack lastp{}, in{}, out{};
+    std::vector<Ciphertext> fresh(cts.size());
+    size_t p = 0;
+    for (size_t i = 0; i < cts.size(); i++) {
+        fresh[i].ctx = &c;
+        fresh[i].resize(cts[i]->size, cts[i]->chain + 1, s, false);  // the fused NTT writes every word
+        for (size_t k = 0; k < cts[i]->size; k++, p++) {
+            in.p[p] = cts[i]->poly(k);
+            lastp.p[p] = cts[i]->poly(k) + L1 * N;
+            out.p[p] = fresh[i].poly(k);
+        }
+    }
+    Scratch last(np * N * sizeof(uint64_t), s);
+    Scratch delta(np * L1 * N * sizeof(uint64_t), s);
+    launch_gather_polys(c, lastp, last.as<uint64_t>(), N, np, s);
+    ntt_inverse(c, last.as<uint64_t>(), np, 1, L1, s);
+    const LevelConsts &L = c.level(Ql);
+    rescale_ntt_fused_pack(c, last.as<uint64_t>(), in, out, delta.as<uint64_t>(), np, L1,
+                           L.qlast_inv.as<uint64_t>(), L.qlast_inv_s.as<uint64_t>(), s, rounded);
+    for (size_t i = 0; i < cts.size(); i++) {
+        fresh[i].scale = cts[i]->scale / (double)c.q[Ql - 1];
+        fresh[i].ntt = true;
+        *cts[i] = std::move(fresh[i]);
+    }
 }
 
 // Bring a to (chain, scale): drop to one level above the target, then one
@@ -303,6 +349,53 @@ void ct_mul_relin_rescale(Context &c, Ciphertext &out, const Ciphertext &a, cons
     keyswitch(c, out.poly(2), rk.k.as<uint64_t>(), Ql, out.poly(0), out.poly(1), true, s, rounded);
     out.size = 2;
     rescale_inplace(c, out, s, rounded);
+}
+
+// ct_mul_relin_rescale over m independent products, with one key switch pass
+// (keyswitch_batch) and one rescale pass (rescale_batch) for the group.
+void mul_relin_rescale_batch(Context &c, const std::vector<Ciphertext *> &out,
+                             const std::vector<const Ciphertext *> &a, const std::vector<const Ciphertext *> &b,
+                             const KSwitchKey &rk, hipStream_t s, bool rounded) {
+    const size_t m = out.size();
+    if (m == 0) return;
+    if (a.size() != m || b.size() != m || 2 * m > (size_t)kPolyPack)
+        throw InvalidArgument("mul_relin_rescale_batch: 1..32 products");
+    std::vector<Ciphertext> tmp(2 * m);
+    PolyPack c2{}, ct01{};
+    size_t Ql = 0;
+    for (size_t i = 0; i < m; i++) {
+        if (a[i]->size != 2 || b[i]->size != 2) throw InvalidArgument("multiply supports size-2 ciphertexts");
+        const size_t chain = std::max(a[i]->chain, b[i]->chain);
+        const Ciphertext *pa = a[i], *pb = b[i];
+        if (a[i]->chain != chain) {
+            ct_clone(tmp[2 * i], *a[i], s);
+            mod_switch_drop(c, tmp[2 * i], chain, s);
+            pa = &tmp[2 * i];
+        }
+        if (b[i]->chain != chain) {
+            ct_clone(tmp[2 * i + 1], *b[i], s);
+            mod_switch_drop(c, tmp[2 * i + 1], chain, s);
+            pb = &tmp[2 * i + 1];
+        }
+        const size_t ql = c.ql_of_chain(chain);
+        if (i == 0) Ql = ql;
+        if (ql != Ql) throw InvalidArgument("mul_relin_rescale_batch: products at different levels");
+        Ciphertext &o = *out[i];
+        o.ctx = &c;
+        o.resize(3, chain, s, false);  // tensor / square write d0, d1, d2
+        if (pa == pb || (pa->data() == pb->data()))
+            launch_square(c, pa->data(), o.data(), Ql, s);
+        else
+            launch_tensor(c, pa->data(), pb->data(), o.data(), Ql, s);
+        o.scale = pa->scale * pb->scale;
+        o.ntt = true;
+        c2.p[i] = o.poly(2);
+        ct01.p[2 * i] = o.poly(0);
+        ct01.p[2 * i + 1] = o.poly(1);
+    }
+    keyswitch_batch(c, c2, m, rk.k.as<uint64_t>(), Ql, ct01, s, rounded);
+    for (Ciphertext *o : out) o->size = 2;
+    rescale_batch(c, out, s, rounded);
 }
 
 void ct_mul_monomial_half(Context &c, Ciphertext &a, bool negate, hipStream_t s) {

@@ -88,6 +88,13 @@ __global__ void k_gather_limb(const uint64_t *in, uint64_t *out, size_t Ql, size
     }
 }
 
+// out[p][i] = in.p[p][i] for i < words: polynomials of separate buffers into one.
+__global__ void k_gather_polys(PolyPack in, uint64_t *__restrict__ out, size_t words, size_t npolys) {
+    const size_t total = npolys * words;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x)
+        out[t] = in.p[t / words][t % words];
+}
+
 // galois.cu:8-20 apply_galois_ntt_permutation.
 __global__ void k_permute(const uint64_t *in, uint64_t *out, const uint32_t *table, size_t N, size_t total) {
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
@@ -255,6 +262,13 @@ void launch_rescale(const Context &c, const uint64_t *in, uint64_t *out, size_t 
     if (in == out)
         SHM_HIP(hipMemcpyAsync(out, delta.as<uint64_t>(), npolys * L1 * N * sizeof(uint64_t), hipMemcpyDeviceToDevice,
                                s));
+    SHM_HIP(hipGetLastError());
+}
+
+void launch_gather_polys(const Context &c, const PolyPack &in, uint64_t *out, size_t words, size_t npolys,
+                         hipStream_t s) {
+    if (npolys == 0 || npolys > (size_t)kPolyPack) throw InvalidArgument("gather: 1..64 polynomials");
+    k_gather_polys<<<grid_for(npolys * words), kBlock, 0, s>>>(in, out, words, npolys);
     SHM_HIP(hipGetLastError());
 }
 

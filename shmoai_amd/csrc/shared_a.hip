@@ -471,9 +471,13 @@ void bootstrap_shared_a_s2c_first(Bootstrapper &b, SharedAKeys &K, const std::ve
     const double E = F * d_in * b.prm.input_bound / q0;
     const double pi_ = 3.14159265358979323846;
     const double cst = (1.0 + 0.5 * pi_ * pi_ * E * E) / (2.0 * pi_) * q0 / (F * d_in);
+    std::vector<Ciphertext *> parts;
     for (size_t i = 0; i < n; i++) {
-        eval_mod(b, res[i], rk, cst, out_scale, s);
-        eval_mod(b, ims[i], rk, cst, out_scale, s);
+        parts.push_back(&res[i]);
+        parts.push_back(&ims[i]);
+    }
+    eval_mod_batch(b, parts, rk, cst, out_scale, s);  // every member's two parts in one batch
+    for (size_t i = 0; i < n; i++) {
         ct_mul_monomial_half(c, ims[i], false, s);
         ct_add(c, res[i], ims[i], s);
         res[i].scale = out_scale;

@@ -103,6 +103,27 @@ def test_eval_mod_sine(small):
     assert err < 1e-6, err
 
 
+def test_eval_mod_batch_bitexact(small):
+    """Batched EvalMod (one key switch and rescale pass per step for the group)
+    equals EvalMod applied to each ciphertext alone, limb for limb."""
+    ctx, ref, sk, boot, gk, rk = small
+    n = ctx.N // 2
+    rng = np.random.default_rng(38)
+    q0 = float(ctx.coeff_modulus[0])
+    xs = [rng.integers(-20, 21, size=n) + rng.uniform(-1 / 512, 1 / 512, n) for _ in range(3)]
+    batch = [_encrypt(ctx, sk, x, q0, ctx.size_q - 3, 60 + i) for i, x in enumerate(xs)]
+    single = [_encrypt(ctx, sk, x, q0, ctx.size_q - 3, 60 + i) for i, x in enumerate(xs)]
+    boot.eval_mod_batch(batch, rk)
+    for b, t, x in zip(batch, single, xs):
+        boot.eval_mod(t, rk)
+        assert b.coeff_modulus_size == t.coeff_modulus_size and b.scale == t.scale
+        assert np.array_equal(b.numpy(), t.numpy())
+        got = shm.Encoder(ctx).decode(shm.Decryptor(ctx, sk).decrypt(b))
+        assert np.max(np.abs(got - np.sin(2 * np.pi * x))) < 1e-6
+    with pytest.raises(shm.capi.ShmInvalidArgument):
+        boot.eval_mod_batch([batch[0], _encrypt(ctx, sk, xs[0], q0, ctx.size_q - 4, 70)], rk)
+
+
 def test_bootstrap_s2c_first_small(small):
     ctx, ref, sk, boot, gk, rk = small
     n = ctx.N // 2
