@@ -189,6 +189,23 @@ Rotator ordinary_rotator(Context &c, const GaloisKeys &gk, hipStream_t s) {
     };
     return r;
 }
+
+// Rotations of a group container (b_0, a_0, b_1, a_1, ...) of ordinary
+// ciphertexts: the linear transforms then run once for the whole group (every
+// polynomial meets the same diagonal), with batched key switches.
+Rotator group_rotator(Context &c, const GaloisKeys &gk, hipStream_t s) {
+    Rotator r;
+    r.rot = [&c, &gk, s](Ciphertext &x, int step) {
+        if (step == 0)
+            apply_galois_group(c, x, (uint32_t)(2 * c.N - 1), gk, s, true);
+        else
+            rotate_internal_group(c, x, step, gk, s, true);
+    };
+    r.hoist = [&c, &gk, s](const Ciphertext &x, const std::vector<int> &steps, const std::vector<Ciphertext *> &outs) {
+        rotate_hoisted_group(c, x, steps, gk, outs, s, true);
+    };
+    return r;
+}
 }  // namespace
 
 void slot_to_coeff(Bootstrapper &b, Ciphertext &ct, const GaloisKeys &gk, hipStream_t s) {
@@ -613,14 +630,35 @@ void bootstrap_s2c_first_batch(Bootstrapper &b, const std::vector<const Cipherte
         while (j < in.size() && j - i < 16 && in[j]->scale == in[i]->scale) j++;
         const size_t g = j - i;
         const double d_in = in[i]->scale, osc = out_scale > 0 ? out_scale : d_in;
+        // the group's S2C, modraise and C2S run on one container of 2g polynomials
+        const size_t N = c.N, Qin = in[i]->Ql();
+        Ciphertext G;
+        G.ctx = &c;
+        G.resize(2 * g, in[i]->chain, s, false);  // every member's pair is copied in
+        G.scale = d_in;
+        G.ntt = true;
+        for (size_t k = 0; k < g; k++)
+            SHM_HIP(hipMemcpyAsync(G.poly(2 * k), in[i + k]->data(), 2 * Qin * N * sizeof(uint64_t),
+                                   hipMemcpyDeviceToDevice, s));
+        const Rotator rot = group_rotator(c, gk, s);
+        slot_to_coeff(b, G, rot, s);
+        ct_modraise(c, G, s);
+        G.scale = q0;
+        Ciphertext RE, IM;
+        coeff_to_slot(b, G, RE, IM, rot, s);
         std::vector<Ciphertext> re(g), im(g);
+        const size_t Qc = RE.Ql();
         for (size_t k = 0; k < g; k++) {
-            Ciphertext x;
-            ct_clone(x, *in[i + k], s);
-            slot_to_coeff(b, x, gk, s);
-            ct_modraise(c, x, s);
-            x.scale = q0;
-            coeff_to_slot(b, x, re[k], im[k], gk, s);
+            for (int part = 0; part < 2; part++) {
+                Ciphertext &dst = part ? im[k] : re[k];
+                const Ciphertext &src = part ? IM : RE;
+                dst.ctx = &c;
+                dst.resize(2, src.chain, s, false);  // the copy writes both polynomials
+                dst.scale = src.scale;
+                dst.ntt = true;
+                SHM_HIP(hipMemcpyAsync(dst.data(), src.poly(2 * k), 2 * Qc * N * sizeof(uint64_t),
+                                       hipMemcpyDeviceToDevice, s));
+            }
         }
         // amplitude correction (BatchBootstrap.cuh:444-460) folded into EvalMod:
         // sin(2 pi eps)/(2 pi) ~ eps, with the linear minimax constant over

@@ -302,11 +302,19 @@ void ks_moddown(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, uint64
 // The same with polynomial p written to out.p[p]
 void ks_moddown_pack(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, const PolyPack &out,
                      uint64_t acc_mask, hipStream_t s, bool rounded = false);
-// Key switch of m polynomials c2.p[i] (Ql limbs, NTT) with one key; the
-// switched pair of member i is added to (ct01.p[2i], ct01.p[2i + 1]). One
-// launch per stage for the whole group, each key word read once per group.
+// ModUp of m polynomials c2.p[i] (Ql limbs, NTT) into d [m][beta][QlP][N]
+void ks_modup_batch(const Context &c, const PolyPack &c2, size_t m, size_t Ql, uint64_t *d, hipStream_t s,
+                    bool rounded = false);
+// cx [m][2][QlP][N] = inner products of the m digit sets of d with one key
+// ([dnum][2][QP][N]); each key word is read once for the group; with perm the
+// digits are read through that Galois permutation
+void ks_inner_batch(const Context &c, const uint64_t *d, size_t m, size_t Ql, const uint64_t *key, uint64_t *cx,
+                    hipStream_t s, const uint32_t *perm = nullptr);
+// Key switch of m polynomials c2.p[i] with one key; the switched pair of member
+// i goes to (ct01.p[2i], ct01.p[2i + 1]), accumulated where acc_mask has the
+// polynomial's bit. One launch per stage for the whole group.
 void keyswitch_batch(const Context &c, const PolyPack &c2, size_t m, const uint64_t *key, size_t Ql,
-                     const PolyPack &ct01, hipStream_t s, bool rounded = false);
+                     const PolyPack &ct01, uint64_t acc_mask, hipStream_t s, bool rounded = false);
 
 // encoder (encoder.hip)
 void encode(const Context &c, const double *re, const double *im, size_t n_values, size_t Ql, double scale,
@@ -340,6 +348,14 @@ void rotate_hoisted(Context &c, const Ciphertext &ct, const std::vector<int> &st
 // ModUp of c1 and one ModDown of the accumulated inner products
 void hoisting_inplace(Context &c, Ciphertext &ct, const std::vector<int> &steps, const GaloisKeys &gk,
                       hipStream_t s);
+// The same operations on a group of ordinary ciphertexts held as one container
+// of 2m polynomials (b_0, a_0, b_1, a_1, ...), m <= 32, batched over the members
+void apply_galois_group(Context &c, Ciphertext &g, uint32_t elt, const GaloisKeys &gk, hipStream_t s,
+                        bool rounded = false);
+void rotate_internal_group(Context &c, Ciphertext &g, int step, const GaloisKeys &gk, hipStream_t s,
+                           bool rounded = false);
+void rotate_hoisted_group(Context &c, const Ciphertext &g, const std::vector<int> &steps, const GaloisKeys &gk,
+                          const std::vector<Ciphertext *> &outs, hipStream_t s, bool rounded = false);
 void mod_switch_drop(Context &c, Ciphertext &ct, size_t new_chain, hipStream_t s);
 void ct_clone(Ciphertext &dst, const Ciphertext &src, hipStream_t s);
 void ct_add(Context &c, Ciphertext &a, const Ciphertext &b, hipStream_t s);

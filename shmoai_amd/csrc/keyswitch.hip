@@ -190,12 +190,16 @@ __global__ void k_moddown_bconv(const uint64_t *__restrict__ cx, uint64_t *__res
 // cx[m][0/1][k][j] = sum_g d[m][g][k][j] * key_g.(b/a)[limb(k)][j] for `members`
 // digit sets: the 2 beta key words of (k, j) are read once and reused for every
 // member (eval_key_switch.cu:12-60 per member).
+// With perm, the digits are read through that NTT-domain Galois permutation
+// (hoisted rotations of every member by one step).
 __global__ void k_ks_inner_batch(const uint64_t *__restrict__ d, const uint64_t *__restrict__ key,
                                  uint64_t *__restrict__ cx, const DevMod *__restrict__ mods, size_t Ql, size_t nQ,
-                                 size_t QP, size_t alpha, size_t beta, size_t N, size_t members) {
+                                 size_t QP, size_t alpha, size_t beta, size_t N, size_t members,
+                                 const uint32_t *__restrict__ perm) {
     const size_t QlP = Ql + alpha, total = QlP * N;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
         const size_t k = t / N, j = t % N;
+        const size_t src = perm ? k * N + perm[j] : t;
         const uint32_t pi = qlp_prime(k, Ql, nQ);
         uint64_t kb[kMaxDigits], ka[kMaxDigits];
 #pragma unroll
@@ -209,7 +213,7 @@ __global__ void k_ks_inner_batch(const uint64_t *__restrict__ d, const uint64_t 
         }
         const DevMod m = mods[pi];
         for (size_t mem = 0; mem < members; mem++) {
-            const uint64_t *dm = d + mem * beta * total + t;
+            const uint64_t *dm = d + mem * beta * total + src;
             Acc128 a0{0, 0}, a1{0, 0};
 #pragma unroll
             for (int g = 0; g < kMaxDigits; g++) {
@@ -317,32 +321,45 @@ void keyswitch(const Context &c, const uint64_t *c2, const uint64_t *key, size_t
     }
 }
 
-void keyswitch_batch(const Context &c, const PolyPack &c2, size_t m, const uint64_t *key, size_t Ql,
-                     const PolyPack &ct01, hipStream_t s, bool rounded) {
+void ks_modup_batch(const Context &c, const PolyPack &c2, size_t m, size_t Ql, uint64_t *d, hipStream_t s,
+                    bool rounded) {
     const size_t N = c.N, alpha = c.nP, QlP = Ql + alpha;
     const LevelConsts &L = c.level(Ql);
     const size_t beta = L.beta;
-    if (m == 0 || 2 * m > (size_t)kPolyPack) throw InvalidArgument("keyswitch_batch: 1..32 members");
-    if (beta > c.dnum) throw LogicError("key switching key has too few digits for this level");
+    if (m == 0 || m > (size_t)kPolyPack) throw InvalidArgument("ks_modup_batch: 1..64 members");
     if (alpha > (size_t)kMaxAlpha) throw InvalidArgument("at most 16 special primes");
-    // ModUp of all members: gather + iNTT, conversion, one NTT over m beta QlP limbs
+    // gather + iNTT, conversion, one NTT over m beta QlP limbs
     Scratch tcoef(m * Ql * N * sizeof(uint64_t), s);
     launch_gather_polys(c, c2, tcoef.as<uint64_t>(), Ql * N, m, s);
     ntt_inverse(c, tcoef.as<uint64_t>(), m, Ql, 0, s);
-    Scratch d(m * beta * QlP * N * sizeof(uint64_t), s);
     k_modup_bconv<<<grid_for(m * beta * N), kBlock, 0, s>>>(
-        tcoef.as<uint64_t>(), d.as<uint64_t>(), c.mods(), L.hat_inv.as<uint64_t>(), L.hat_inv_s.as<uint64_t>(),
+        tcoef.as<uint64_t>(), d, c.mods(), L.hat_inv.as<uint64_t>(), L.hat_inv_s.as<uint64_t>(),
         L.hat_mod.as<uint64_t>(), L.dig_mod.as<uint64_t>(), Ql, c.nQ, alpha, beta, N, m, rounded ? 1 : 0);
     SHM_HIP(hipGetLastError());
-    ntt_forward_qlp(c, d.as<uint64_t>(), Ql, 0, QlP, 0, 0, m * beta, QlP * N, s);
+    ntt_forward_qlp(c, d, Ql, 0, QlP, 0, 0, m * beta, QlP * N, s);
+}
+
+void ks_inner_batch(const Context &c, const uint64_t *d, size_t m, size_t Ql, const uint64_t *key, uint64_t *cx,
+                    hipStream_t s, const uint32_t *perm) {
+    const size_t QlP = Ql + c.nP, beta = c.level(Ql).beta;
+    if (beta > c.dnum) throw LogicError("key switching key has too few digits for this level");
+    if (beta > (size_t)kMaxDigits) throw InvalidArgument("at most 16 key digits");
+    k_ks_inner_batch<<<grid_for(QlP * c.N), kBlock, 0, s>>>(d, key, cx, c.mods(), Ql, c.nQ, c.nQP(), c.nP, beta,
+                                                            c.N, m, perm);
+    SHM_HIP(hipGetLastError());
+}
+
+void keyswitch_batch(const Context &c, const PolyPack &c2, size_t m, const uint64_t *key, size_t Ql,
+                     const PolyPack &ct01, uint64_t acc_mask, hipStream_t s, bool rounded) {
+    const size_t N = c.N, QlP = Ql + c.nP, beta = c.level(Ql).beta;
+    if (m == 0 || 2 * m > (size_t)kPolyPack) throw InvalidArgument("keyswitch_batch: 1..32 members");
+    Scratch d(m * beta * QlP * N * sizeof(uint64_t), s);
+    ks_modup_batch(c, c2, m, Ql, d.as<uint64_t>(), s, rounded);
     // inner products: each key word read once for the group
     Scratch cx(2 * m * QlP * N * sizeof(uint64_t), s);
-    k_ks_inner_batch<<<grid_for(QlP * N), kBlock, 0, s>>>(d.as<uint64_t>(), key, cx.as<uint64_t>(), c.mods(), Ql,
-                                                          c.nQ, c.nQP(), alpha, beta, N, m);
-    SHM_HIP(hipGetLastError());
-    // ModDown of the 2m polynomials, added into the members' (c0, c1)
-    const uint64_t mask = 2 * m == 64 ? ~uint64_t(0) : ((uint64_t(1) << (2 * m)) - 1);
-    ks_moddown_pack(c, cx.as<uint64_t>(), 2 * m, Ql, ct01, mask, s, rounded);
+    ks_inner_batch(c, d.as<uint64_t>(), m, Ql, key, cx.as<uint64_t>(), s, nullptr);
+    // ModDown of the 2m polynomials into the members' (c0, c1)
+    ks_moddown_pack(c, cx.as<uint64_t>(), 2 * m, Ql, ct01, acc_mask, s, rounded);
 }
 
 }  // namespace shm

@@ -93,6 +93,97 @@ void rotate_internal(Context &c, Ciphertext &ct, int step, const GaloisKeys &gk,
         if ((size_t)std::abs(t) != (c.N >> 1)) rotate_internal(c, ct, t, gk, s, rounded);
 }
 
+// Groups of ordinary ciphertexts held as one container of 2m polynomials
+// (b_0, a_0, b_1, a_1, ...) at one level; every member is rotated by the same
+// element with the same key, so one permute launch covers all polynomials and
+// the key switch of all a_i is one keyswitch_batch (key words read once).
+namespace {
+uint64_t even_bits(size_t npolys) {
+    uint64_t m = 0;
+    for (size_t p = 0; p < npolys; p += 2) m |= uint64_t(1) << p;
+    return m;
+}
+}  // namespace
+
+void apply_galois_group(Context &c, Ciphertext &g, uint32_t elt, const GaloisKeys &gk, hipStream_t s, bool rounded) {
+    if (g.size == 0 || g.size % 2 || g.size > (size_t)kPolyPack)
+        throw InvalidArgument("a ciphertext group holds 1..32 pairs of polynomials");
+    auto it = gk.keys.find(elt);
+    if (it == gk.keys.end()) throw InvalidArgument("Galois elt not present");
+    const size_t m = g.size / 2, Ql = g.Ql(), N = c.N;
+    const uint32_t *table = c.galois_table(elt, s);
+    Scratch t(g.size * Ql * N * sizeof(uint64_t), s);
+    launch_permute(c, g.data(), t.as<uint64_t>(), table, g.size * Ql, s);
+    // b_i := sigma(b_i); a_i is overwritten by the switched part below
+    SHM_HIP(hipMemcpyAsync(g.data(), t.as<uint64_t>(), g.size * Ql * N * sizeof(uint64_t), hipMemcpyDeviceToDevice,
+                           s));
+    PolyPack c2{}, ct01{};
+    for (size_t i = 0; i < m; i++) {
+        c2.p[i] = t.as<uint64_t>() + (2 * i + 1) * Ql * N;
+        ct01.p[2 * i] = g.poly(2 * i);
+        ct01.p[2 * i + 1] = g.poly(2 * i + 1);
+    }
+    keyswitch_batch(c, c2, m, it->second->k.as<uint64_t>(), Ql, ct01, even_bits(g.size), s, rounded);
+}
+
+void rotate_internal_group(Context &c, Ciphertext &g, int step, const GaloisKeys &gk, hipStream_t s, bool rounded) {
+    if (step == 0) return;
+    const uint32_t elt = c.elt_from_step(step);
+    if (gk.keys.count(elt)) {
+        apply_galois_group(c, g, elt, gk, s, rounded);
+        return;
+    }
+    std::vector<int> naf;
+    {
+        const bool sign = step < 0;
+        int v = std::abs(step);
+        for (int i = 0; v; i++) {
+            const int zi = (v & 1) ? 2 - (v & 3) : 0;
+            v = (v - zi) >> 1;
+            if (zi) naf.push_back((sign ? -zi : zi) * (1 << i));
+        }
+    }
+    if (naf.size() == 1) throw InvalidArgument("Galois key not present");
+    for (int t : naf)
+        if ((size_t)std::abs(t) != (c.N >> 1)) rotate_internal_group(c, g, t, gk, s, rounded);
+}
+
+void rotate_hoisted_group(Context &c, const Ciphertext &g, const std::vector<int> &steps, const GaloisKeys &gk,
+                          const std::vector<Ciphertext *> &outs, hipStream_t s, bool rounded) {
+    if (g.size == 0 || g.size % 2 || g.size > (size_t)kPolyPack)
+        throw InvalidArgument("a ciphertext group holds 1..32 pairs of polynomials");
+    if (steps.size() != outs.size()) throw InvalidArgument("one output per step");
+    const size_t m = g.size / 2, Ql = g.Ql(), N = c.N, QlP = Ql + c.nP, beta = c.level(Ql).beta;
+    Scratch d(m * beta * QlP * N * sizeof(uint64_t), s), cx(2 * m * QlP * N * sizeof(uint64_t), s);
+    bool have_modup = false;
+    for (size_t i = 0; i < steps.size(); i++) {
+        Ciphertext &o = *outs[i];
+        const uint32_t elt = c.elt_from_step(steps[i]);
+        auto it = gk.keys.find(elt);
+        if (steps[i] == 0 || it == gk.keys.end()) {
+            ct_clone(o, g, s);
+            rotate_internal_group(c, o, steps[i], gk, s, rounded);
+            continue;
+        }
+        if (!have_modup) {
+            PolyPack a{};
+            for (size_t k = 0; k < m; k++) a.p[k] = g.poly(2 * k + 1);
+            ks_modup_batch(c, a, m, Ql, d.as<uint64_t>(), s, rounded);
+            have_modup = true;
+        }
+        const uint32_t *table = c.galois_table(elt, s);
+        ks_inner_batch(c, d.as<uint64_t>(), m, Ql, it->second->k.as<uint64_t>(), cx.as<uint64_t>(), s, table);
+        o.ctx = &c;
+        o.resize(g.size, g.chain, s, false);  // permute writes every b_i, ModDown every a_i
+        o.scale = g.scale;
+        o.ntt = true;
+        launch_permute(c, g.data(), o.data(), table, g.size * Ql, s);
+        PolyPack op{};
+        for (size_t k = 0; k < g.size; k++) op.p[k] = o.poly(k);
+        ks_moddown_pack(c, cx.as<uint64_t>(), g.size, Ql, op, even_bits(g.size), s, rounded);
+    }
+}
+
 void rotate_hoisted(Context &c, const Ciphertext &ct, const std::vector<int> &steps, const GaloisKeys &gk,
                     const std::vector<Ciphertext *> &outs, hipStream_t s, bool rounded) {
     if (ct.size != 2) throw InvalidArgument("encrypted size must be 2");
@@ -393,7 +484,8 @@ void mul_relin_rescale_batch(Context &c, const std::vector<Ciphertext *> &out,
         ct01.p[2 * i] = o.poly(0);
         ct01.p[2 * i + 1] = o.poly(1);
     }
-    keyswitch_batch(c, c2, m, rk.k.as<uint64_t>(), Ql, ct01, s, rounded);
+    const uint64_t all = 2 * m == 64 ? ~uint64_t(0) : ((uint64_t(1) << (2 * m)) - 1);
+    keyswitch_batch(c, c2, m, rk.k.as<uint64_t>(), Ql, ct01, all, s, rounded);  // relinearise: add to both
     for (Ciphertext *o : out) o->size = 2;
     rescale_batch(c, out, s, rounded);
 }

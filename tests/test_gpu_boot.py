@@ -147,6 +147,21 @@ def test_bootstrap_s2c_first_small(small):
         boot.bootstrap_s2c_first([bad], gk, rk)
 
 
+def test_bootstrap_group_equals_single(small):
+    """Bootstrapping three ciphertexts in one call (one group: batched linear
+    transforms and EvalMod) gives exactly the limbs of bootstrapping each alone."""
+    ctx, ref, sk, boot, gk, rk = small
+    n = ctx.N // 2
+    rng = np.random.default_rng(39)
+    zs = [rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n) for _ in range(3)]
+    cts = [_encrypt(ctx, sk, z, 2.0 ** 46, 4, 80 + i) for i, z in enumerate(zs)]
+    group = boot.bootstrap_s2c_first(cts, gk, rk)
+    for ct, g in zip(cts, group):
+        alone = boot.bootstrap_s2c_first([ct], gk, rk)[0]
+        assert g.coeff_modulus_size == alone.coeff_modulus_size and g.scale == alone.scale
+        assert np.array_equal(g.numpy(), alone.numpy())
+
+
 def test_bootstrap_s2c_first_reference_params():
     """N = 2^16, Q = 1 x 51 + 20 x 46 + 14 x 51 bits, P = 5 x 51 bits (dnum 7),
     Hamming weight 192, scale 2^46, inputs with 4 primes
