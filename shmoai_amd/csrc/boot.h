@@ -110,8 +110,9 @@ void eval_mod_batch(Bootstrapper &b, const std::vector<Ciphertext *> &xs, const 
                     double out_scale, hipStream_t s);
 void bootstrap_s2c_first(Bootstrapper &b, const Ciphertext &in, Ciphertext &out, const GaloisKeys &gk,
                          const KSwitchKey &rk, double out_scale, hipStream_t s);
-// bootstrap_s2c_first of several ciphertexts: groups of up to 16 inputs with one
-// scale share one batched EvalMod over their real and imaginary parts.
+// bootstrap_s2c_first of several ciphertexts: groups of up to 8 inputs with one
+// scale run their linear transforms on one container and share one batched
+// EvalMod over their real and imaginary parts.
 // out_scale 0 keeps each input's scale.
 void bootstrap_s2c_first_batch(Bootstrapper &b, const std::vector<const Ciphertext *> &in,
                                const std::vector<Ciphertext *> &out, const GaloisKeys &gk, const KSwitchKey &rk,

@@ -627,7 +627,9 @@ void bootstrap_s2c_first_batch(Bootstrapper &b, const std::vector<const Cipherte
     const double q0 = (double)c.q[0];
     for (size_t i = 0; i < in.size();) {
         size_t j = i + 1;
-        while (j < in.size() && j - i < 16 && in[j]->scale == in[i]->scale) j++;
+        // groups of up to 8: measured at C3, 1 / 2 / 4 / 8 / 16 per group give
+        // 57.5 / 48.2 / 43.8 / 42.7 / 55 ms per ciphertext
+        while (j < in.size() && j - i < 8 && in[j]->scale == in[i]->scale) j++;
         const size_t g = j - i;
         const double d_in = in[i]->scale, osc = out_scale > 0 ? out_scale : d_in;
         // the group's S2C, modraise and C2S run on one container of 2g polynomials
