@@ -255,8 +255,9 @@ __global__ __launch_bounds__(kThreads) void k_ntt_cols(uint64_t *__restrict__ da
         const uint64_t *tw = tb.tw + (size_t)prime * N, *tws = tb.tws + (size_t)prime * N;
         if (FWD) {
             ct_unit<LOGT>(x, 1u, tw, tws, IntK{q});
+            // streaming store: the tile pass reads these words back long after
 #pragma unroll
-            for (int r = 0; r < T; r++) base[r * N2] = x[r];
+            for (int r = 0; r < T; r++) __builtin_nontemporal_store(x[r], &base[r * N2]);
         } else {
             gs_unit<LOGT>(x, 1u, tw, tws, IntK{q});
             const uint64_t ni = tb.ninv[prime], nis = tb.ninv_s[prime];
@@ -326,7 +327,7 @@ __global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ 
             uint64_t v = lds[b * TP + swz<T>(r)];
             v = v >= q2 ? v - q2 : v;
             v = v >= q ? v - q : v;
-            out[i] = mul_shoup(sub_mod(accv[j], v, q), qi, qis, q);
+            __builtin_nontemporal_store(mul_shoup(sub_mod(accv[j], v, q), qi, qis, q), &out[i]);
         }
     } else if (FWD) {
         for (int i = threadIdx.x; i < kTile; i += kThreads) {
