@@ -126,10 +126,13 @@ __device__ __forceinline__ void transpose4(uint32_t w0, uint32_t w1, uint32_t w2
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-// Two waves per SIMD: the register budget (X digit fragments, 11 planes, the
-// prefetched weight fragments of the next phase) is capped at 256 per lane.
+// Registers: the X digit fragments (DX KS 4 VGPRs) and the 11 planes dominate.
+// Up to DX KS = 21 (e.g. 7 digits x 192 rows) the kernel is held to 3 waves per
+// SIMD (156 registers at 7 x 3, no spill), which measured faster than
+// prefetching the next phase's weight fragments at 2 waves per SIMD; larger
+// shapes get 2 waves per SIMD so that they do not spill.
 template <int DX, int DW, int KS, int RB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_contract_int8(const uint64_t *const *__restrict__ X,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DX * KS <= 21 ? 3 : 2))) void k_contract_int8(const uint64_t *const *__restrict__ X,
                                                        const i8x16 *__restrict__ frag,
                                                        const uint64_t *__restrict__ corr,
                                                        const uint64_t *__restrict__ mu_arr,
@@ -183,31 +186,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
     };
     const int nph = (MB + RB - 1) / RB;
-    // The weight fragments of the next phase are loaded into registers while
-    // the current phase runs, then copied to LDS between two barriers.
-    constexpr int PF = (FR + 255) / 256;
-    i8x16 pf[PF];
-#pragma unroll
-    for (int u = 0; u < PF; u++) {
-        const int t = threadIdx.x + 256 * u;
-        if (t < FR) pf[u] = frag[t];
-    }
     for (int ph = 0; ph < nph; ph++) {
         lds_barrier();
-#pragma unroll
-        for (int u = 0; u < PF; u++) {
-            const int t = threadIdx.x + 256 * u;
-            if (t < FR) wl[t] = pf[u];
-        }
+        const i8x16 *src = frag + (size_t)ph * FR;
+        for (int t = threadIdx.x; t < FR; t += 256) wl[t] = src[t];
         lds_barrier();
-        if (ph + 1 < nph) {
-            const i8x16 *src = frag + (size_t)(ph + 1) * FR;
-#pragma unroll
-            for (int u = 0; u < PF; u++) {
-                const int t = threadIdx.x + 256 * u;
-                if (t < FR) pf[u] = src[t];
-            }
-        }
 #pragma unroll 1
         for (int ibl = 0; ibl < RB; ibl++) {
             const int ib = ph * RB + ibl;
