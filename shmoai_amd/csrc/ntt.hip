@@ -16,7 +16,9 @@
 // tw[(G << u) + i'] at its local stage u for local group i'.
 //
 // Lazy ranges (Harvey): forward keeps [0, 4q) between stages, inverse [0, 2q);
-// primes are < 2^61 so nothing wraps. Outputs are canonical residues.
+// primes are < 2^61 so nothing wraps. When every prime of a launch is below
+// 2^58 the forward stages skip the per-butterfly reduction of a altogether
+// (values stay below 36q). Outputs are canonical residues.
 #include <algorithm>
 
 #include "engine.h"
@@ -80,6 +82,43 @@ __device__ __forceinline__ void ct_unit(uint64_t *x, uint32_t G, const uint64_t 
             }
         }
     }
+}
+
+// Forward butterflies that leave a unreduced: with t = x w mod q in [0, 2q),
+// both outputs stay below a + 2q, so 16 stages from inputs below 4q end below
+// 36q, which fits 64 bits for q < 2^58. Saves the compare-select-subtract of
+// every butterfly; the final pass reduces once.
+struct IntKLazy {
+    uint64_t q;
+};
+
+template <int R>
+__device__ __forceinline__ void ct_unit(uint64_t *x, uint32_t G, const uint64_t *__restrict__ tw,
+                                        const uint64_t *__restrict__ tws, IntKLazy k) {
+    const uint64_t q = k.q, q2 = q << 1;
+#pragma unroll
+    for (int u = 0; u < R; u++) {
+        const int h = 1 << (R - 1 - u);
+#pragma unroll
+        for (int l = 0; l < (1 << R); l++) {
+            if ((l & h) == 0) {
+                const uint32_t ti = (G << u) + (uint32_t)(l >> (R - u));
+                const uint64_t a = x[l];
+                const uint64_t t = mul_shoup_lazy(x[l + h], tw[ti], tws[ti], q);
+                x[l] = a + t;
+                x[l + h] = a + q2 - t;
+            }
+        }
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void gs_unit(uint64_t *x, uint32_t G, const uint64_t *__restrict__ tw,
+                                        const uint64_t *__restrict__ tws, IntK k);
+template <int R>
+__device__ __forceinline__ void gs_unit(uint64_t *x, uint32_t G, const uint64_t *__restrict__ tw,
+                                        const uint64_t *__restrict__ tws, IntKLazy k) {
+    gs_unit<R>(x, G, tw, tws, IntK{k.q});  // the inverse keeps its reductions
 }
 
 template <int R>
@@ -222,7 +261,7 @@ struct NttIO {
 // the 256 threads of a block read and write 256 consecutive words per row.
 // FWD: forward stages 0..logN1-1 (output stays lazy).
 // !FWD: inverse stages logN1-1..0, then x N^-1, output canonical.
-template <int LOGT, bool FWD, int IO>
+template <int LOGT, bool FWD, int IO, bool LAZY>
 __global__ __launch_bounds__(kThreads) void k_ntt_cols(uint64_t *__restrict__ data, LimbMap mp, NttTables tb,
                                                        int logN, NttIO io) {
     constexpr int T = 1 << LOGT;
@@ -254,7 +293,10 @@ __global__ __launch_bounds__(kThreads) void k_ntt_cols(uint64_t *__restrict__ da
         }
         const uint64_t *tw = tb.tw + (size_t)prime * N, *tws = tb.tws + (size_t)prime * N;
         if (FWD) {
-            ct_unit<LOGT>(x, 1u, tw, tws, IntK{q});
+            if (LAZY)
+                ct_unit<LOGT>(x, 1u, tw, tws, IntKLazy{q});
+            else
+                ct_unit<LOGT>(x, 1u, tw, tws, IntK{q});
             // streaming store: the tile pass reads these words back long after
 #pragma unroll
             for (int r = 0; r < T; r++) __builtin_nontemporal_store(x[r], &base[r * N2]);
@@ -274,7 +316,7 @@ __global__ __launch_bounds__(kThreads) void k_ntt_cols(uint64_t *__restrict__ da
 // runs on XCD b % 8. Each XCD takes a contiguous range of the work index
 // O = (limb * tiles + tile) * n_polys + poly: the polynomials that share one
 // prime and tile (hence one twiddle slice) run back to back on the same L2.
-template <int LOGT, bool FWD, int IO>
+template <int LOGT, bool FWD, int IO, bool LAZY>
 __global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ data, LimbMap mp, NttTables tb,
                                                          int logN, NttIO io, uint32_t n_polys, uint32_t per_xcd) {
     constexpr int T = 1 << LOGT;
@@ -309,8 +351,18 @@ __global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ 
         for (int j = 0; j < PER; j++) accv[j] = acc[threadIdx.x + j * kThreads];
     }
     __syncthreads();
-    run_rounds<LOGT, FWD>(lds, N1, 1u, (uint32_t)(t0 / T), tb.tw + (size_t)prime * N, tb.tws + (size_t)prime * N,
-                          IntK{q});
+    if (LAZY)
+        run_rounds<LOGT, FWD>(lds, N1, 1u, (uint32_t)(t0 / T), tb.tw + (size_t)prime * N,
+                              tb.tws + (size_t)prime * N, IntKLazy{q});
+    else
+        run_rounds<LOGT, FWD>(lds, N1, 1u, (uint32_t)(t0 / T), tb.tw + (size_t)prime * N,
+                              tb.tws + (size_t)prime * N, IntK{q});
+    // canonical residue of a forward output: below 4q, or below 36q when lazy
+    auto canon = [&](uint64_t v) -> uint64_t {
+        if (LAZY) return reduce64(v, tb.mods[prime]);
+        v = v >= q2 ? v - q2 : v;
+        return v >= q ? v - q : v;
+    };
     if (FWD && IO == IO_RESCALE) {
         uint64_t *out;
         if (io.packed) {
@@ -324,17 +376,13 @@ __global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ 
         for (int j = 0; j < PER; j++) {
             const int i = threadIdx.x + j * kThreads;
             const int b = i / T, r = i % T;
-            uint64_t v = lds[b * TP + swz<T>(r)];
-            v = v >= q2 ? v - q2 : v;
-            v = v >= q ? v - q : v;
+            const uint64_t v = canon(lds[b * TP + swz<T>(r)]);
             __builtin_nontemporal_store(mul_shoup(sub_mod(accv[j], v, q), qi, qis, q), &out[i]);
         }
     } else if (FWD) {
         for (int i = threadIdx.x; i < kTile; i += kThreads) {
             const int b = i / T, r = i % T;
-            uint64_t v = lds[b * TP + swz<T>(r)];
-            v = v >= q2 ? v - q2 : v;
-            base[i] = v >= q ? v - q : v;
+            base[i] = canon(lds[b * TP + swz<T>(r)]);
         }
     } else {
         for (int i = threadIdx.x; i < kTile; i += kThreads) {
@@ -367,13 +415,30 @@ void launch_ntt(const Context &c, uint64_t *data, const LimbMap &mp, size_t tota
                  FWD ? c.tw_s.as<uint64_t>() : c.itw_s.as<uint64_t>(),
                  c.ninv.as<uint64_t>(),
                  c.ninv_s.as<uint64_t>()};
-#define SHM_COLS(L)                                                                           \
-    case L:                                                                                   \
-        k_ntt_cols<L, FWD, IOA><<<grid_cols, kThreads, 0, s>>>(data, mp, tb, logN, io);       \
+    // unreduced forward butterflies when every prime of the launch is below 2^58
+    // (plain transforms only: the fused rescale's tile pass measured slower so)
+    bool lazy = FWD && IOB != IO_RESCALE;
+    for (uint32_t j = 0; j < mp.n_active && lazy; j++) {
+        uint32_t k = mp.lo + j;
+        if (k >= mp.skip_lo) k += mp.skip_width;
+        const uint32_t prime = k < mp.Ql ? mp.first_prime + k : mp.nQ + (k - mp.Ql);
+        lazy = c.q.at(prime) < (uint64_t(1) << 58);
+    }
+#define SHM_COLS(L)                                                                                    \
+    case L:                                                                                            \
+        if (lazy)                                                                                      \
+            k_ntt_cols<L, FWD, IOA, true><<<grid_cols, kThreads, 0, s>>>(data, mp, tb, logN, io);      \
+        else                                                                                           \
+            k_ntt_cols<L, FWD, IOA, false><<<grid_cols, kThreads, 0, s>>>(data, mp, tb, logN, io);     \
         break;
-#define SHM_BLKS(L)                                                                                      \
-    case L:                                                                                              \
-        k_ntt_blocks<L, FWD, IOB><<<grid_blk, kThreads, 0, s>>>(data, mp, tb, logN, io, n_polys, per_xcd); \
+#define SHM_BLKS(L)                                                                                        \
+    case L:                                                                                                \
+        if (lazy)                                                                                          \
+            k_ntt_blocks<L, FWD, IOB, true><<<grid_blk, kThreads, 0, s>>>(data, mp, tb, logN, io, n_polys,  \
+                                                                          per_xcd);                         \
+        else                                                                                               \
+            k_ntt_blocks<L, FWD, IOB, false><<<grid_blk, kThreads, 0, s>>>(data, mp, tb, logN, io, n_polys, \
+                                                                           per_xcd);                        \
         break;
     auto cols = [&]() {
         switch (log1) {
