@@ -205,8 +205,9 @@ int shm_ct_pt_matrix_mul(shm_context *ctx, const shm_ciphertext *const *enc_X, u
 /* duration of the dominant kernel of the last matmul on this process, ms (HIP events) */
 double shm_last_matmul_kernel_ms(void);
 /* variant the last matmul actually ran: SHM_MATMUL_INT8 falls back to SHM_MATMUL_U64
- * when a prime is outside (2^46, 2^61], row_W > 256 or a weight needs more than
- * 5 balanced digits (4 for 8-digit limbs) */
+ * when a prime is outside (2^46, 2^61] or a weight needs more than 5 balanced
+ * digits (4 for 8-digit limbs). Any row_W is accepted: rows beyond 256 are
+ * contracted in chunks (ct_pt_matrix_mul_wo_pre_large, Ct_pt_matrix_mul.cuh:584). */
 int shm_last_matmul_variant(void);
 
 /* ---- bootstrapping (slot-to-coefficient first) ------------------------------------

@@ -523,6 +523,19 @@ def ct_pt_matrix_mul_wo_pre(enc_X: Sequence[Ciphertext], W: np.ndarray, col_X: i
     return out
 
 
+def ct_pt_matrix_mul_wo_pre_large(enc_X: Sequence[Ciphertext], W: np.ndarray, col_X: int, col_W: int, row_W: int,
+                                  context: Context, variant: int = MATMUL_INT8, out=None, stream=None):
+    """Ct_pt_matrix_mul.cuh:584-700, the wide-column form (col_W = 128 * col_W_t).
+
+    The reference visits output c = i * col_W_t + k with weight column c, so the
+    sums are those of ct_pt_matrix_mul_wo_pre. It fills only 128 * (col_W // 128)
+    outputs; here a col_W that is not a multiple of 128 raises instead.
+    """
+    if col_W % 128:
+        raise capi.ShmInvalidArgument(capi.SHM_ERR_INVALID_ARGUMENT, "col_W must be a multiple of 128")
+    return ct_pt_matrix_mul_wo_pre(enc_X, W, col_X, col_W, row_W, context, variant, out, stream)
+
+
 def synchronize(stream=None) -> None:
     """Wait for all work queued on `stream` (None: the default stream)."""
     check(_lib().shm_synchronize(stream))

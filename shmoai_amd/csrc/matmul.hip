@@ -17,6 +17,10 @@
 //                    8-bit digits, weights into balanced signed 8-bit digits,
 //                    exact int32 accumulation, recombined with 128-bit integer
 //                    arithmetic and one Barrett reduction (bit-identical).
+// Both batched variants contract at most 256 rows of W per launch; longer
+// contractions (row_W = 768 or 3072, the BERT projection and FFN shapes of
+// ct_pt_matrix_mul_wo_pre_large, Ct_pt_matrix_mul.cuh:584-700, which computes
+// the same sums) run in row chunks whose reduced partial sums are added mod q.
 #include <cmath>
 #include <vector>
 
@@ -66,7 +70,8 @@ template <int OUT_PER_THREAD>
 __global__ __launch_bounds__(256) void k_contract_u64(const uint64_t *const *__restrict__ X,
                                                       const uint64_t *__restrict__ wres,
                                                       uint64_t *__restrict__ acc_out, const DevMod *__restrict__ mods,
-                                                      size_t row_w, size_t col_w, size_t Ql, size_t N) {
+                                                      size_t row_w, size_t col_w, size_t Ql, size_t N,
+                                                      int accumulate) {
     constexpr int TN = 32;
     extern __shared__ uint64_t xs[];  // [row_w][TN]
     const size_t tiles = N / TN;
@@ -94,8 +99,13 @@ __global__ __launch_bounds__(256) void k_contract_u64(const uint64_t *const *__r
                 if (i0 + k < col_w) mac128(acc[k], x, wl[j * col_w + i0 + k]);
         }
 #pragma unroll
-        for (int k = 0; k < OUT_PER_THREAD; k++)
-            if (i0 + k < col_w) acc_out[(i0 + k) * 2 * Ql * N + off + n] = reduce128(acc[k], m);
+        for (int k = 0; k < OUT_PER_THREAD; k++) {
+            if (i0 + k >= col_w) continue;
+            uint64_t *dst = &acc_out[(i0 + k) * 2 * Ql * N + off + n];
+            uint64_t v = reduce128(acc[k], m);
+            // a later row chunk adds to the sum of the earlier ones
+            *dst = accumulate ? add_mod(v, *dst, m.q) : v;
+        }
     }
 }
 
@@ -196,24 +206,34 @@ void ctpt_matmul(const Context &c, const std::vector<const Ciphertext *> &X, con
         if (variant == 2) {
             ctpt_matmul_int8(c, X, W, row_w, col_w, acc.as<uint64_t>(), s, e0, e1);
         } else {
+            // X rows staged in LDS per launch: at most 256 rows (64 KiB); longer
+            // contractions run in balanced row chunks added mod q
+            constexpr size_t kRows = 256;
+            const size_t nch = (row_w + kRows - 1) / kRows, kc_max = (row_w + nch - 1) / nch;
             Scratch dW(row_w * col_w * sizeof(double), s);
             Scratch dscale(row_w * sizeof(double), s);
-            Scratch wres(Ql * row_w * col_w * sizeof(uint64_t), s);
+            Scratch wres(Ql * kc_max * col_w * sizeof(uint64_t), s);
             Scratch xptr(row_w * sizeof(uint64_t *), s);
             std::vector<const uint64_t *> xp(row_w);
             for (size_t j = 0; j < row_w; j++) xp[j] = X[j]->data();
             SHM_HIP(hipMemcpyAsync(dW.as<void>(), W, row_w * col_w * sizeof(double), hipMemcpyHostToDevice, s));
             SHM_HIP(hipMemcpyAsync(dscale.as<void>(), scales.data(), row_w * sizeof(double), hipMemcpyHostToDevice, s));
             SHM_HIP(hipMemcpyAsync(xptr.as<void>(), xp.data(), row_w * sizeof(uint64_t *), hipMemcpyHostToDevice, s));
-            k_weight_residues<<<grid_for(Ql * row_w * col_w), kBlock, 0, s>>>(
-                dW.as<double>(), dscale.as<double>(), wres.as<uint64_t>(), c.mods(), row_w, col_w, Ql);
             const size_t blocks = 2 * Ql * (N / 32);
-            const size_t lds = row_w * 32 * sizeof(uint64_t);
-            if (lds > 64 * 1024) throw InvalidArgument("row_W too large for the 64-bit contraction");
-            SHM_HIP(hipEventRecord(e0, s));
-            k_contract_u64<8><<<(unsigned)blocks, 256, lds, s>>>(xptr.as<const uint64_t *>(), wres.as<uint64_t>(),
-                                                                 acc.as<uint64_t>(), c.mods(), row_w, col_w, Ql, N);
-            SHM_HIP(hipGetLastError());
+            size_t k0 = 0;
+            for (size_t ch = 0; ch < nch; ch++) {
+                const size_t kc = row_w / nch + (ch < row_w % nch ? 1 : 0);
+                k_weight_residues<<<grid_for(Ql * kc * col_w), kBlock, 0, s>>>(
+                    dW.as<double>() + k0 * col_w, dscale.as<double>() + k0, wres.as<uint64_t>(), c.mods(), kc, col_w,
+                    Ql);
+                const size_t lds = kc * 32 * sizeof(uint64_t);
+                if (ch == 0) SHM_HIP(hipEventRecord(e0, s));
+                k_contract_u64<8><<<(unsigned)blocks, 256, lds, s>>>(xptr.as<const uint64_t *>() + k0,
+                                                                     wres.as<uint64_t>(), acc.as<uint64_t>(),
+                                                                     c.mods(), kc, col_w, Ql, N, ch > 0 ? 1 : 0);
+                SHM_HIP(hipGetLastError());
+                k0 += kc;
+            }
             SHM_HIP(hipEventRecord(e1, s));
             SHM_HIP(hipStreamSynchronize(s));
         }

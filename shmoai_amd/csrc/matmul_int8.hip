@@ -138,7 +138,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DX * KS <= 
                                                        const uint64_t *__restrict__ mu_arr,
                                                        uint64_t *__restrict__ acc_out,
                                                        const DevMod *__restrict__ mods, int row_w, int col_w,
-                                                       int MB, size_t Ql, size_t N) {
+                                                       int MB, size_t Ql, size_t N, int accumulate) {
     constexpr int ND = DX + DW - 1;
     // |C_d| < 2^24.2, so |S| < 2^104.3 and 0 < S + 2^105 < 2^106 needs ND <= 11.
     static_assert(ND >= 10 && ND <= 11, "plane recombination assumes 10 or 11 planes");
@@ -236,8 +236,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DX * KS <= 
                 res = res >= q ? res - q : res;
                 if (row < col_w) {
                     res = add_mod(res, row < kCorrCache ? cl[row] : corr[l * col_w + row], q);
+                    uint64_t *dst = &acc_out[(size_t)row * 2 * Ql * N + off];
+                    // a later row chunk adds to the sum of the earlier ones
+                    if (accumulate) res = add_mod(res, __builtin_nontemporal_load(dst), q);
                     // streamed once by the rescale that follows: keep it out of the caches
-                    __builtin_nontemporal_store(res, &acc_out[(size_t)row * 2 * Ql * N + off]);
+                    __builtin_nontemporal_store(res, dst);
                 }
             }
         }
@@ -245,7 +248,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DX * KS <= 
 }
 
 typedef void (*ContractFn)(const uint64_t *const *, const i8x16 *, const uint64_t *, const uint64_t *, uint64_t *,
-                           const DevMod *, int, int, int, size_t, size_t);
+                           const DevMod *, int, int, int, size_t, size_t, int);
 
 constexpr int rb_for(int KS) { return KS == 1 ? 4 : KS == 2 ? 4 : KS == 3 ? 3 : 2; }
 
@@ -266,10 +269,15 @@ ContractFn pick(int DX, int DW, int KS) {
 
 }  // namespace
 
-// Returns false when the shape or the weights fall outside the INT8 path.
+// Rows of W contracted by one launch. A launch keeps the X digit fragments of
+// all its rows in registers (at most 4 x 64 rows); longer contractions run in
+// row chunks whose reduced sums are added mod q into the accumulators.
+constexpr size_t kChunkRows = 256;
+
+// Returns false when the primes or the weights fall outside the INT8 path.
 bool int8_supported(const Context &c, size_t Ql, const double *W, const double *scales, size_t row_w, size_t col_w,
                     int *dx_out, int *dw_out) {
-    if (row_w > 256 || c.N % 64) return false;
+    if (c.N % 64) return false;
     int qbits = 0, qmin = 64;
     for (size_t l = 0; l < Ql; l++) {
         int b = 64 - __builtin_clzll(c.q[l]);
@@ -286,7 +294,7 @@ bool int8_supported(const Context &c, size_t Ql, const double *W, const double *
     // balanced digits: D digits cover |w| <= 127 * (256^D - 1) / 255
     while (DW < 9 && mx > 127.0 * (std::pow(256.0, DW) - 1) / 255.0) DW++;
     if (DW < 4) DW = 4;
-    if (!pick(DX, DW, (int)((row_w + 63) / 64))) return false;
+    if (!pick(DX, DW, 1)) return false;
     *dx_out = DX;
     *dw_out = DW;
     return true;
@@ -299,13 +307,18 @@ void ctpt_matmul_int8(const Context &c, const std::vector<const Ciphertext *> &X
     for (size_t j = 0; j < row_w; j++) scales[j] = X[j]->scale;
     int DX = 0, DW = 0;
     if (!int8_supported(c, Ql, W, scales.data(), row_w, col_w, &DX, &DW))
-        throw InvalidArgument("weights or shape outside the INT8 MFMA contraction");
-    const int MB = (int)((col_w + 15) / 16), KS = (int)((row_w + 63) / 64), RB = rb_for(KS);
-    const int NPH = (MB + RB - 1) / RB;
-    const size_t nfrag = (size_t)NPH * DW * RB * KS * 64;
+        throw InvalidArgument("weights or primes outside the INT8 MFMA contraction");
+    // balanced row chunks of at most kChunkRows
+    const size_t nch = (row_w + kChunkRows - 1) / kChunkRows;
+    const size_t kc_max = (row_w + nch - 1) / nch;
+    const int MB = (int)((col_w + 15) / 16);
+    auto frag_count = [&](int KS) {
+        const int RB = rb_for(KS), NPH = (MB + RB - 1) / RB;
+        return (size_t)NPH * DW * RB * KS * 64;
+    };
     Scratch dW(row_w * col_w * sizeof(double), s);
     Scratch dscale(row_w * sizeof(double), s);
-    Scratch frag(nfrag * sizeof(i8x16), s);
+    Scratch frag(frag_count((int)((kc_max + 63) / 64)) * sizeof(i8x16), s);
     Scratch corr(Ql * col_w * sizeof(uint64_t), s);
     Scratch mu(Ql * sizeof(uint64_t), s);
     Scratch xptr(row_w * sizeof(uint64_t *), s);
@@ -314,17 +327,27 @@ void ctpt_matmul_int8(const Context &c, const std::vector<const Ciphertext *> &X
     SHM_HIP(hipMemcpyAsync(dW.as<void>(), W, row_w * col_w * sizeof(double), hipMemcpyHostToDevice, s));
     SHM_HIP(hipMemcpyAsync(dscale.as<void>(), scales.data(), row_w * sizeof(double), hipMemcpyHostToDevice, s));
     SHM_HIP(hipMemcpyAsync(xptr.as<void>(), xp.data(), row_w * sizeof(uint64_t *), hipMemcpyHostToDevice, s));
-    k_weight_frags<<<grid_for(nfrag), kBlock, 0, s>>>(dW.as<double>(), dscale.as<double>(), frag.as<i8x16>(), row_w,
-                                                     col_w, DW, MB, KS, RB);
-    k_weight_corr<<<grid_for(Ql * col_w), kBlock, 0, s>>>(dW.as<double>(), dscale.as<double>(), corr.as<uint64_t>(),
-                                                          mu.as<uint64_t>(), c.mods(), row_w, col_w, Ql, DX);
-    SHM_HIP(hipGetLastError());
     const unsigned blocks = (unsigned)(2 * Ql * (N / 64));
-    ContractFn fn = pick(DX, DW, KS);
-    SHM_HIP(hipEventRecord(ev0, s));
-    hipLaunchKernelGGL(fn, dim3(blocks), dim3(256), 0, s, xptr.as<const uint64_t *>(), frag.as<i8x16>(),
-                       corr.as<uint64_t>(), mu.as<uint64_t>(), acc, c.mods(), (int)row_w, (int)col_w, MB, Ql, N);
-    SHM_HIP(hipGetLastError());
+    size_t k0 = 0;
+    for (size_t ch = 0; ch < nch; ch++) {
+        // chunk rows [k0, k0 + kc): W rows are contiguous, so the chunk is a sub-matrix
+        const size_t kc = row_w / nch + (ch < row_w % nch ? 1 : 0);
+        const int KS = (int)((kc + 63) / 64), RB = rb_for(KS);
+        const size_t nfrag = frag_count(KS);
+        const double *Wc = dW.as<double>() + k0 * col_w;
+        const double *sc = dscale.as<double>() + k0;
+        k_weight_frags<<<grid_for(nfrag), kBlock, 0, s>>>(Wc, sc, frag.as<i8x16>(), kc, col_w, DW, MB, KS, RB);
+        k_weight_corr<<<grid_for(Ql * col_w), kBlock, 0, s>>>(Wc, sc, corr.as<uint64_t>(), mu.as<uint64_t>(),
+                                                              c.mods(), kc, col_w, Ql, DX);
+        SHM_HIP(hipGetLastError());
+        ContractFn fn = pick(DX, DW, KS);
+        if (ch == 0) SHM_HIP(hipEventRecord(ev0, s));
+        hipLaunchKernelGGL(fn, dim3(blocks), dim3(256), 0, s, xptr.as<const uint64_t *>() + k0, frag.as<i8x16>(),
+                           corr.as<uint64_t>(), mu.as<uint64_t>(), acc, c.mods(), (int)kc, (int)col_w, MB, Ql, N,
+                           ch > 0 ? 1 : 0);
+        SHM_HIP(hipGetLastError());
+        k0 += kc;
+    }
     SHM_HIP(hipEventRecord(ev1, s));
     SHM_HIP(hipStreamSynchronize(s));
 }

@@ -1,6 +1,9 @@
 """GPU parity of the MOAI ct-pt matmul (Ct_pt_matrix_mul.cuh:18-185).
 
-- small shape: every variant bit-exact against the oracle on all outputs;
+- small shape: every variant bit-exact against the oracle on all outputs,
+  at row_W = 20 (one launch) and row_W = 300 (two row chunks);
+- row_W = 768 (the BERT projection width, three row chunks): INT8-MFMA and
+  64-bit paths bit-exact against the oracle;
 - full quarter-width shape (N=2^16, L=24, 192 x 192): INT8-MFMA == 64-bit
   path on every limb of every output, and sampled output columns bit-exact
   against the oracle;
@@ -30,13 +33,14 @@ def _rand_cts(ctx, ref, rng, n, Ql):
     return xs, hs
 
 
+@pytest.mark.parametrize("row_w", [20, 300])
 @pytest.mark.parametrize("variant", [shm.MATMUL_U64, shm.MATMUL_INT8, shm.MATMUL_PER_CT])
-def test_matmul_small_bitexact(oracle_lib, variant):
+def test_matmul_small_bitexact(oracle_lib, variant, row_w):
     bits = [50] * 5 + [60]
     ctx = shm.Context(13, bits, 1)
     ref = oracle_lib.Ref(13, bits, 1)
     rng = np.random.default_rng(5)
-    row_w, col_w, Ql = 20, 12, 5
+    col_w, Ql = 12, 5
     xs, hs = _rand_cts(ctx, ref, rng, row_w, Ql)
     W = np.random.default_rng(6).normal(0, 0.02, (row_w, col_w))
     out = shm.ct_pt_matrix_mul_wo_pre(hs, W, row_w, col_w, row_w, ctx, variant=variant)
@@ -46,6 +50,23 @@ def test_matmul_small_bitexact(oracle_lib, variant):
         assert out[i].chain_index == ctx.chain(Ql - 1)
         assert out[i].scale == 2.0 ** 40
         assert np.array_equal(out[i].numpy(), exp[i]), f"output {i}"
+
+
+def test_matmul_row_chunks_768(oracle_lib):
+    bits = [50] * 4 + [60]
+    ctx = shm.Context(13, bits, 1)
+    ref = oracle_lib.Ref(13, bits, 1)
+    rng = np.random.default_rng(7)
+    row_w, col_w, Ql = 768, 8, 4
+    xs, hs = _rand_cts(ctx, ref, rng, row_w, Ql)
+    W = np.random.default_rng(8).normal(0, 0.02, (row_w, col_w))
+    W[5, 3], W[700, 0] = 0.45, -0.45  # near the 5-digit weight bound, first and last chunk
+    exp = ref.ctpt_matmul(xs, W, Ql, np.full(row_w, 2.0 ** 40))
+    for variant in (shm.MATMUL_INT8, shm.MATMUL_U64):
+        out = shm.ct_pt_matrix_mul_wo_pre(hs, W, row_w, col_w, row_w, ctx, variant=variant)
+        assert shm.last_matmul_variant() == variant
+        for i in range(col_w):
+            assert np.array_equal(out[i].numpy(), exp[i]), f"variant {variant} output {i}"
 
 
 def test_matmul_quarter_width_int8_equals_u64_and_oracle(oracle_lib):
