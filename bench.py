@@ -20,6 +20,9 @@ sequential form), ordinary in one group of 8 (batched linear transforms and
 EvalMod), and shared-a chunks of n = 4 and n = 8, with the decryption error of
 the timed outputs.
 
+`full_width_768` times one 768 x 768 matmul of the same kind (the BERT-base
+projection width, W rows contracted in chunks of 256).
+
 Two more objects give the HBM rate of the transform and key-switch kernels:
 `ntt` (forward + inverse NTT of 64 polynomials x 24 limbs at N = 2^16) and
 `bootstrap_768.keyswitch` (one rotation of a top-level ciphertext of the
@@ -227,6 +230,32 @@ def bootstrap_bench(shm, n_seq: int = 3, n_group: int = 8, shared=((4, 2), (8, 1
     return res
 
 
+def full_width_bench(shm, ctx, hs, steps: int = 2):
+    """One 768 x 768 matmul (the BERT-base projection width; rows contracted in
+    three chunks of 256) on 768 distinct device ciphertexts: the 192 inputs and
+    three device copies of each (limb values do not change the integer work)."""
+    R = 768
+    X = list(hs) + [h.copy() for _ in range(3) for h in hs]
+    W = np.random.default_rng(7).normal(0.0, 0.02, (R, R))
+    out = [shm.Ciphertext(ctx, 2, 2) for _ in range(R)]
+    shm.ct_pt_matrix_mul_wo_pre(X, W, R, R, R, ctx, out=out)
+    log("bench: 768x768 warmup done")
+    ks = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        shm.ct_pt_matrix_mul_wo_pre(X, W, R, R, R, ctx, out=out)
+        ks.append(shm.last_matmul_kernel_ms())
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    assert shm.last_matmul_variant() == shm.MATMUL_INT8
+    k_ms = float(np.mean(ks))
+    tops = 2 * 35 * R * R * 2 * L * ctx.N / (k_ms * 1e-3) / 1e12
+    log(f"bench: 768x768 matmul {ms:.2f} ms/step, contraction {k_ms:.2f} ms")
+    return {"workload": "768x768 ct-pt matmul, N=2^16, L=24, incl. rescale", "steps": steps,
+            "ms_per_step": round(ms, 3), "s_per_input": ms * 1e-3 / INPUTS,
+            "contraction_ms": round(k_ms, 3), "contraction_int8_tops": round(tops, 1),
+            "contraction_frac_of_int8_peak": round(tops / INT8_PEAK_TOPS, 4)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,6 +265,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-off per-ciphertext form timing")
     ap.add_argument("--no-bootstrap", action="store_true", help="skip the bootstrap timing")
+    ap.add_argument("--no-full-width", action="store_true", help="skip the 768x768 matmul timing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one device-to-device copy of a 24 MiB ciphertext (profiler byte-counter calibration)")
     args = ap.parse_args()
@@ -296,6 +326,10 @@ def main() -> int:
                  "batched_speedup_vs_plain": round(plain_ms / ms_per_step, 2)}
         log(f"bench: plain per-ciphertext form {plain_ms:.1f} ms")
 
+    full = None
+    if not args.no_full_width and variant == shm.MATMUL_INT8:
+        full = full_width_bench(shm, ctx, hs)
+
     ntt = ntt_bench(shm, ctx)
     boot = None
     if not args.no_bootstrap:
@@ -325,6 +359,7 @@ def main() -> int:
                    "variant": args.variant, "inputs_per_step": INPUTS},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "full_width_768": full,
         "ntt": ntt,
         "bootstrap_768": boot,
         **extra,

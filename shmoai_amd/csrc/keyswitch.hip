@@ -110,34 +110,43 @@ __global__ void k_ks_inner(const uint64_t *__restrict__ d, const uint64_t *__res
     }
 }
 
-// cx[o][k][j] = sum_g d[g][k][j] * key_o(g)[limb(k)][j] for every output o; the
-// digits of one coefficient are read once and reused for all outputs. Key limbs
-// are addressed by their global prime (QP layout) or, with `local`, by the QlP
-// position (keys generated for one level).
-__global__ void k_ks_inner_multi(const uint64_t *__restrict__ d, KeyRefs kr, int nout, uint64_t *__restrict__ cx,
-                                 const DevMod *__restrict__ mods, size_t Ql, size_t nQ, size_t alpha, size_t ndig,
-                                 size_t N, int local, const uint32_t *__restrict__ perm, int accumulate) {
+// cx[o][k][j] = sum_g d[g][k][j] * key_o(g)[limb(k)][j] for every output o. Key
+// limbs are addressed by their global prime (QP layout) or, with `local`, by the
+// QlP position (keys generated for one level).
+// Layout: one thread per (output, limb k, coefficient j); a block covers
+// J = 256 / OG consecutive coefficients and OG = 2^lgog output slots (OG <= 16).
+// The OG threads of one coefficient load the same digit words (L1 serves all
+// but the first), so the digits leave HBM once for all outputs, and every
+// thread waits only on its own 2 ndig loads.
+template <int MAXB>
+__global__ __launch_bounds__(256) void k_ks_inner_multi(const uint64_t *__restrict__ d, KeyRefs kr, int nout,
+                                                        uint64_t *__restrict__ cx, const DevMod *__restrict__ mods,
+                                                        size_t Ql, size_t nQ, size_t alpha, int ndig, size_t N,
+                                                        int local, const uint32_t *__restrict__ perm, int accumulate,
+                                                        int lgog) {
     const size_t QlP = Ql + alpha, total = QlP * N;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t k = t / N, j = t % N;
-        const uint32_t pi = qlp_prime(k, Ql, nQ);
-        const size_t kl = (local ? k : (size_t)pi) * N + j;
-        // hoisted rotation: the digits are read through the NTT-domain Galois permutation
-        const size_t src = perm ? k * N + perm[j] : t;
-        uint64_t dv[kMaxDigits];  // fully unrolled: stays in registers
+    const int lgj = 8 - lgog;
+    const size_t t = ((size_t)blockIdx.x << lgj) + (threadIdx.x & ((1u << lgj) - 1));
+    const int os = (int)(threadIdx.x >> lgj);
+    if (t >= total || os >= nout) return;
+    const size_t k = t / N, j = t % N;
+    const uint32_t pi = qlp_prime(k, Ql, nQ);
+    const size_t kl = (local ? k : (size_t)pi) * N + j;
+    // hoisted rotation: the digits are read through the NTT-domain Galois permutation
+    const size_t src = perm ? k * N + perm[j] : t;
+    uint64_t dv[MAXB];
 #pragma unroll
-        for (int g = 0; g < kMaxDigits; g++) dv[g] = (size_t)g < ndig ? d[g * total + src] : 0;
-        const DevMod m = mods[pi];
-        for (int o = 0; o < nout; o++) {
-            Acc128 a{0, 0};
-            const uint64_t *kb = kr.base[o];
-            const size_t st = kr.stride[o];
+    for (int g = 0; g < MAXB; g++) dv[g] = g < ndig ? d[(size_t)g * total + src] : 0;
+    const DevMod m = mods[pi];
+    for (int o = os; o < nout; o += 1 << lgog) {
+        Acc128 a{0, 0};
+        const uint64_t *kb = kr.base[o];
+        const size_t st = kr.stride[o];
 #pragma unroll
-            for (int g = 0; g < kMaxDigits; g++)
-                if ((size_t)g < ndig) mac128(a, dv[g], kb[g * st + kl]);
-            const uint64_t v = reduce128(a, m);
-            cx[o * total + t] = accumulate ? add_mod(cx[o * total + t], v, m.q) : v;
-        }
+        for (int g = 0; g < MAXB; g++)
+            if (g < ndig) mac128(a, dv[g], kb[(size_t)g * st + kl]);
+        const uint64_t v = reduce128(a, m);
+        cx[(size_t)o * total + t] = accumulate ? add_mod(cx[(size_t)o * total + t], v, m.q) : v;
     }
 }
 
@@ -188,44 +197,54 @@ __global__ void k_moddown_bconv(const uint64_t *__restrict__ cx, uint64_t *__res
 }
 
 // cx[m][0/1][k][j] = sum_g d[m][g][k][j] * key_g.(b/a)[limb(k)][j] for `members`
-// digit sets: the 2 beta key words of (k, j) are read once and reused for every
-// member (eval_key_switch.cu:12-60 per member).
+// digit sets (eval_key_switch.cu:12-60 per member).
 // With perm, the digits are read through that NTT-domain Galois permutation
 // (hoisted rotations of every member by one step).
-__global__ void k_ks_inner_batch(const uint64_t *__restrict__ d, const uint64_t *__restrict__ key,
-                                 uint64_t *__restrict__ cx, const DevMod *__restrict__ mods, size_t Ql, size_t nQ,
-                                 size_t QP, size_t alpha, size_t beta, size_t N, size_t members,
-                                 const uint32_t *__restrict__ perm) {
+// Layout: one thread per (member, limb k, coefficient j). A block covers
+// J = 256 / MG consecutive coefficients and MG = 2^lgmg member slots (MG <= 16,
+// so J >= 16 and the lanes of one member read whole 128-byte runs); the MG
+// threads of one coefficient load the same 2 beta key words, which L1 serves
+// after the first, so each key word leaves HBM once for the group. Every
+// thread has only its own beta digit loads to wait for, and the grid holds
+// members x QlP x N threads to hide their latency.
+template <int MAXB>
+__global__ __launch_bounds__(256) void k_ks_inner_batch(const uint64_t *__restrict__ d, const uint64_t *__restrict__ key,
+                                                        uint64_t *__restrict__ cx, const DevMod *__restrict__ mods,
+                                                        size_t Ql, size_t nQ, size_t QP, size_t alpha, int beta,
+                                                        size_t N, int members, int lgmg,
+                                                        const uint32_t *__restrict__ perm) {
     const size_t QlP = Ql + alpha, total = QlP * N;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t k = t / N, j = t % N;
-        const size_t src = perm ? k * N + perm[j] : t;
-        const uint32_t pi = qlp_prime(k, Ql, nQ);
-        uint64_t kb[kMaxDigits], ka[kMaxDigits];
+    const int lgj = 8 - lgmg;
+    const size_t t = ((size_t)blockIdx.x << lgj) + (threadIdx.x & ((1u << lgj) - 1));
+    const int ms = (int)(threadIdx.x >> lgj);
+    if (t >= total || ms >= members) return;
+    const size_t k = t / N, j = t % N;
+    const size_t src = perm ? k * N + perm[j] : t;
+    const uint32_t pi = qlp_prime(k, Ql, nQ);
+    uint64_t kb[MAXB], ka[MAXB];
 #pragma unroll
-        for (int g = 0; g < kMaxDigits; g++) {
-            kb[g] = ka[g] = 0;
-            if ((size_t)g < beta) {
-                const uint64_t *kg = key + g * 2 * QP * N;
-                kb[g] = kg[(size_t)pi * N + j];
-                ka[g] = kg[(QP + pi) * N + j];
+    for (int g = 0; g < MAXB; g++) {
+        kb[g] = ka[g] = 0;
+        if (g < beta) {
+            const uint64_t *kg = key + (size_t)g * 2 * QP * N;
+            kb[g] = kg[(size_t)pi * N + j];
+            ka[g] = kg[(QP + pi) * N + j];
+        }
+    }
+    const DevMod m = mods[pi];
+    for (int mem = ms; mem < members; mem += 1 << lgmg) {
+        const uint64_t *dm = d + (size_t)mem * beta * total + src;
+        Acc128 a0{0, 0}, a1{0, 0};
+#pragma unroll
+        for (int g = 0; g < MAXB; g++) {
+            if (g < beta) {
+                const uint64_t dv = dm[(size_t)g * total];
+                mac128(a0, dv, kb[g]);
+                mac128(a1, dv, ka[g]);
             }
         }
-        const DevMod m = mods[pi];
-        for (size_t mem = 0; mem < members; mem++) {
-            const uint64_t *dm = d + mem * beta * total + src;
-            Acc128 a0{0, 0}, a1{0, 0};
-#pragma unroll
-            for (int g = 0; g < kMaxDigits; g++) {
-                if ((size_t)g < beta) {
-                    const uint64_t dv = dm[g * total];
-                    mac128(a0, dv, kb[g]);
-                    mac128(a1, dv, ka[g]);
-                }
-            }
-            cx[(2 * mem) * total + t] = reduce128(a0, m);
-            cx[(2 * mem + 1) * total + t] = reduce128(a1, m);
-        }
+        cx[(size_t)(2 * mem) * total + t] = reduce128(a0, m);
+        cx[(size_t)(2 * mem + 1) * total + t] = reduce128(a1, m);
     }
 }
 
@@ -269,9 +288,13 @@ void ks_inner_multi(const Context &c, const uint64_t *d, size_t Ql, size_t ndig,
                     bool local, uint64_t *cx, hipStream_t s, const uint32_t *perm, bool accumulate) {
     if (nout == 0 || nout > (size_t)kMaxKeys) throw InvalidArgument("ks_inner_multi: 1..34 outputs");
     if (ndig == 0 || ndig > (size_t)kMaxDigits) throw InvalidArgument("ks_inner_multi: 1..16 digits");
-    const size_t QlP = Ql + c.nP;
-    k_ks_inner_multi<<<grid_for(QlP * c.N), kBlock, 0, s>>>(d, kr, (int)nout, cx, c.mods(), Ql, c.nQ, c.nP, ndig,
-                                                           c.N, local ? 1 : 0, perm, accumulate ? 1 : 0);
+    const size_t QlP = Ql + c.nP, total = QlP * c.N;
+    int lgog = 0;
+    while (lgog < 4 && (size_t(1) << lgog) < nout) lgog++;
+    const unsigned blocks = (unsigned)((total + (256u >> lgog) - 1) >> (8 - lgog));
+    auto fn = ndig <= 4 ? k_ks_inner_multi<4> : ndig <= 8 ? k_ks_inner_multi<8> : k_ks_inner_multi<16>;
+    fn<<<blocks, kBlock, 0, s>>>(d, kr, (int)nout, cx, c.mods(), Ql, c.nQ, c.nP, (int)ndig, c.N, local ? 1 : 0, perm,
+                                 accumulate ? 1 : 0, lgog);
     SHM_HIP(hipGetLastError());
 }
 
@@ -344,8 +367,13 @@ void ks_inner_batch(const Context &c, const uint64_t *d, size_t m, size_t Ql, co
     const size_t QlP = Ql + c.nP, beta = c.level(Ql).beta;
     if (beta > c.dnum) throw LogicError("key switching key has too few digits for this level");
     if (beta > (size_t)kMaxDigits) throw InvalidArgument("at most 16 key digits");
-    k_ks_inner_batch<<<grid_for(QlP * c.N), kBlock, 0, s>>>(d, key, cx, c.mods(), Ql, c.nQ, c.nQP(), c.nP, beta,
-                                                            c.N, m, perm);
+    if (m == 0 || m > 64) throw InvalidArgument("ks_inner_batch: 1..64 members");
+    int lgmg = 0;
+    while (lgmg < 4 && (size_t(1) << lgmg) < m) lgmg++;
+    const size_t total = QlP * c.N;
+    const unsigned blocks = (unsigned)((total + (256u >> lgmg) - 1) >> (8 - lgmg));
+    auto fn = beta <= 4 ? k_ks_inner_batch<4> : beta <= 8 ? k_ks_inner_batch<8> : k_ks_inner_batch<16>;
+    fn<<<blocks, kBlock, 0, s>>>(d, key, cx, c.mods(), Ql, c.nQ, c.nQP(), c.nP, (int)beta, c.N, (int)m, lgmg, perm);
     SHM_HIP(hipGetLastError());
 }
 
