@@ -278,6 +278,16 @@ int shm_bootstrap_eval_mod(shm_bootstrapper *b, shm_ciphertext *ct, const shm_ks
  * step for the group; each result equals shm_bootstrap_eval_mod's bit for bit. */
 int shm_bootstrap_eval_mod_batch(shm_bootstrapper *b, shm_ciphertext *const *cts, size_t count,
                                  const shm_kswitch_key *relin, shm_stream_t stream);
+/* Polynomial::homomorphic_poly_evaluation (bootstrapping/common/Polynomial.cu:253-476)
+ * in Chebyshev form, as the non-linear layers use it (gelu_cheb,
+ * non_linear_func/gelu_other.cuh:257-330: sum_k d_k T_k(u), u affine in x):
+ * each of `count` (1..32) ciphertexts at one level and scale becomes
+ * sum_{k < n_coeffs} coeffs[k] T_k((2x - lo - hi) / (hi - lo)) in place, at the input
+ * scale. Degree n_coeffs - 1 in 1..63; baby steps T_1..T_15, giant steps T_16 and
+ * T_32; depth 1 (affine map) + ceil(log2(n_coeffs)) + 1 levels at most. All
+ * ciphertexts go through each product together (one key switch and rescale pass). */
+int shm_evaluate_chebyshev(shm_context *ctx, shm_ciphertext *const *cts, size_t count, const double *coeffs,
+                           size_t n_coeffs, double lo, double hi, const shm_kswitch_key *relin, shm_stream_t stream);
 /* BatchBootstrapper::bootstrap_ordinary_s2c_first (BatchBootstrap.cuh:674-709) over
  * `count` ciphertexts with exactly 4 primes; out[i] receives the refreshed
  * ciphertext at scale out_scale (0 = the input scale). Inputs are not modified. */

@@ -369,6 +369,17 @@ class Evaluator:
     def apply_kswitch_key_inplace(self, ct, key: KSwitchKey, stream=None):
         check(self.L.shm_apply_kswitch_key_inplace(self.ctx.h, ct.h, key.h, stream))
 
+    def evaluate_chebyshev_inplace(self, cts, coeffs, lo: float, hi: float, relin_key: KSwitchKey, stream=None):
+        """Polynomial::homomorphic_poly_evaluation (bootstrapping/common/Polynomial.cu:253-476)
+        in Chebyshev form: every ciphertext of `cts` (one, or a list of up to 32 at
+        one level and scale) becomes sum_k coeffs[k] T_k((2x - lo - hi) / (hi - lo))
+        at its own scale, the list in lockstep."""
+        cts = [cts] if isinstance(cts, Ciphertext) else list(cts)
+        c = np.ascontiguousarray(coeffs, np.float64)
+        hs = (C.c_void_p * len(cts))(*[x.h.value for x in cts])
+        check(self.L.shm_evaluate_chebyshev(self.ctx.h, hs, len(cts), c.ctypes.data_as(C.POINTER(C.c_double)),
+                                            c.size, float(lo), float(hi), relin_key.h, stream))
+
     def rescale_to_next_inplace(self, ct, stream=None):
         check(self.L.shm_rescale_to_next_inplace(self.ctx.h, ct.h, stream))
 

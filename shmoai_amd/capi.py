@@ -122,6 +122,7 @@ _SIGS = {
     "shm_bootstrap_eval_mod": (i32, [vp, vp, vp, vp]),
     "shm_bootstrap_eval_mod_batch": (i32, [vp, vp, sz, vp, vp]),
     "shm_bootstrap_s2c_first": (i32, [vp, pvp, sz, pvp, vp, vp, dbl, vp]),
+    "shm_evaluate_chebyshev": (i32, [vp, pvp, sz, C.POINTER(dbl), sz, dbl, dbl, vp, vp]),
     "shm_shared_a_keys_create": (i32, [vp, vp, u32, u32, u64, u32, pvp]),
     "shm_shared_a_keys_destroy": (i32, [vp]),
     "shm_shared_a_keys_add_galois": (i32, [vp, C.POINTER(C.c_int), sz, u64]),

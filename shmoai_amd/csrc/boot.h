@@ -108,6 +108,11 @@ void eval_mod(Bootstrapper &b, Ciphertext &x, const KSwitchKey &rk, double out_c
 // to eval_mod per ciphertext)
 void eval_mod_batch(Bootstrapper &b, const std::vector<Ciphertext *> &xs, const KSwitchKey &rk, double out_const,
                     double out_scale, hipStream_t s);
+// sum_k coeffs[k] T_k((2 x - lo - hi) / (hi - lo)) on up to 32 ciphertexts at one
+// level and scale, in place, in lockstep (baby-step/giant-step Chebyshev, one
+// batched key switch and rescale per product); the result keeps the input scale.
+void chebyshev_eval_batch(Context &c, const std::vector<Ciphertext *> &xs, const std::vector<double> &coeffs,
+                          double lo, double hi, const KSwitchKey &rk, hipStream_t s);
 void bootstrap_s2c_first(Bootstrapper &b, const Ciphertext &in, Ciphertext &out, const GaloisKeys &gk,
                          const KSwitchKey &rk, double out_scale, hipStream_t s);
 // bootstrap_s2c_first of several ciphertexts: groups of up to 8 inputs with one

@@ -565,6 +565,43 @@ void eval_mod_batch(Bootstrapper &b, const std::vector<Ciphertext *> &xs, const 
     for (size_t i = 0; i < m; i++) *xs[i] = std::move(*out[i]);
 }
 
+void chebyshev_eval_batch(Context &c, const std::vector<Ciphertext *> &xs, const std::vector<double> &coeffs,
+                          double lo, double hi, const KSwitchKey &rk, hipStream_t s) {
+    const size_t m = xs.size();
+    if (m == 0 || m > 32) throw InvalidArgument("evaluate_chebyshev: 1..32 ciphertexts");
+    if (coeffs.size() < 2 || coeffs.size() > 64) throw InvalidArgument("evaluate_chebyshev: degree 1..63");
+    bool any = false;
+    for (size_t k = 1; k < coeffs.size(); k++) any |= coeffs[k] != 0.0;
+    if (!any) throw InvalidArgument("evaluate_chebyshev: every non-constant coefficient is zero");
+    if (!(hi > lo)) throw InvalidArgument("evaluate_chebyshev: empty interval");
+    ChebEvalBatch::Group t1(m);
+    for (size_t i = 0; i < m; i++) {
+        if (xs[i]->size != 2) throw InvalidArgument("evaluate_chebyshev takes size-2 ciphertexts");
+        if (xs[i]->chain != xs[0]->chain || xs[i]->scale != xs[0]->scale)
+            throw InvalidArgument("evaluate_chebyshev: inputs at different levels or scales");
+        t1[i] = std::make_unique<Ciphertext>();
+        ct_clone(*t1[i], *xs[i], s);
+    }
+    // u = (2 x - (lo + hi)) / (hi - lo): one exact constant multiply and rescale
+    // back onto the input scale, then the constant term
+    const double s1 = xs[0]->scale;
+    if (c.ql_of_chain(xs[0]->chain) < 2) throw LogicError("end of modulus switching chain reached");
+    for (auto &t : t1) ct_mul_const(c, *t, 2.0 / (hi - lo), s1 * (double)c.q[t->Ql() - 1] / t->scale, s);
+    rescale_batch(c, ChebEvalBatch::ptrs(t1), s, true);
+    for (auto &t : t1) {
+        t->scale = s1;
+        const double shift = -(lo + hi) / (hi - lo);
+        if (shift != 0.0) ct_add_const(c, *t, shift, s);
+    }
+    ChebEvalBatch ev(c, rk, s, std::move(t1));
+    const size_t cp = ev.eval_chain(coeffs);
+    if (cp > c.nQ) throw LogicError("end of modulus switching chain reached");
+    ChebEvalBatch::Group out = ev.fresh();
+    ev.eval(coeffs, s1, out);
+    if (out[0]->chain != cp) throw LogicError("Chebyshev level bookkeeping");
+    for (size_t i = 0; i < m; i++) *xs[i] = std::move(*out[i]);
+}
+
 // ModularReducer::modular_reduction (ModularReducer.cu:64-84). x holds slots
 // x' = eps + integer with |x'| < K at any scale; the result holds
 // out_const * sin(2 pi x') = out_const * cos(2 pi (x' - 1/4)) at scale out_scale.

@@ -958,6 +958,24 @@ int shm_bootstrap_eval_mod_batch(shm_bootstrapper *b, shm_ciphertext *const *cts
     });
 }
 
+int shm_evaluate_chebyshev(shm_context *ctx, shm_ciphertext *const *cts, size_t count, const double *coeffs,
+                           size_t n_coeffs, double lo, double hi, const shm_kswitch_key *relin, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(relin, "relin_keys");
+        need(coeffs, "coeffs");
+        if (!cts) throw InvalidArgument("cts is null");
+        if (count == 0 || count > 32) throw InvalidArgument("evaluate_chebyshev: 1..32 ciphertexts");
+        std::vector<Ciphertext *> xs(count);
+        for (size_t i = 0; i < count; i++) {
+            need(cts[i], "cts[i]");
+            xs[i] = &cts[i]->ct;
+        }
+        boot::chebyshev_eval_batch(*ctx->c, xs, std::vector<double>(coeffs, coeffs + n_coeffs), lo, hi, relin->k,
+                                   S(st));
+    });
+}
+
 int shm_bootstrap_s2c_first(shm_bootstrapper *b, const shm_ciphertext *const *in, size_t count,
                             shm_ciphertext *const *out, const shm_galois_keys *gk, const shm_kswitch_key *relin,
                             double out_scale, shm_stream_t st) {
