@@ -202,6 +202,15 @@ int shm_complex_conjugate_inplace(shm_context *ctx, shm_ciphertext *ct, const sh
  * Synchronises the stream. */
 int shm_ct_pt_matrix_mul(shm_context *ctx, const shm_ciphertext *const *enc_X, uint32_t row_W, const double *W,
                          uint32_t col_W, shm_ciphertext *const *out, int variant, shm_stream_t stream);
+/* Replaces ct_pt_matrix_mul_wo_pre_w_mask / _w_mask_single
+ * (Ct_pt_matrix_mul.cuh:846-985, 987-1100): as shm_ct_pt_matrix_mul, but the
+ * plaintext of term (j, i) encodes the slot vector {bias_vec[t] == 1 ? W[j][i] : 0}
+ * (t < slot_count = N/2) at X[j]'s scale, a full polynomial per term. The terms are
+ * encoded on the device in blocks (FP64 special FFT as in shm_encode, then NTT) and
+ * contracted with 128-bit sums. n_bias must equal N/2. Synchronises the stream. */
+int shm_ct_pt_matrix_mul_masked(shm_context *ctx, const shm_ciphertext *const *enc_X, uint32_t row_W,
+                                const double *W, uint32_t col_W, const int32_t *bias_vec, size_t n_bias,
+                                shm_ciphertext *const *out, shm_stream_t stream);
 /* duration of the dominant kernel of the last matmul on this process, ms (HIP events) */
 double shm_last_matmul_kernel_ms(void);
 /* variant the last matmul actually ran: SHM_MATMUL_INT8 falls back to SHM_MATMUL_U64

@@ -547,6 +547,28 @@ def ct_pt_matrix_mul_wo_pre_large(enc_X: Sequence[Ciphertext], W: np.ndarray, co
     return ct_pt_matrix_mul_wo_pre(enc_X, W, col_X, col_W, row_W, context, variant, out, stream)
 
 
+def ct_pt_matrix_mul_wo_pre_w_mask(enc_X: Sequence[Ciphertext], W: np.ndarray, bias_vec, col_X: int, col_W: int,
+                                   row_W: int, context: Context, out=None, stream=None):
+    """Ct_pt_matrix_mul.cuh:846-985 (and _w_mask_single, 987-1100): as
+    ct_pt_matrix_mul_wo_pre, with the plaintext of term (j, i) encoding
+    W[j][i] in the slots where bias_vec == 1 and 0 elsewhere (bias_vec holds
+    slot_count = N/2 entries)."""
+    if col_X != row_W:
+        raise capi.ShmInvalidArgument(capi.SHM_ERR_INVALID_ARGUMENT, "bad dimensions of X or W")
+    if len(enc_X) < row_W:
+        raise capi.ShmInvalidArgument(capi.SHM_ERR_INVALID_ARGUMENT, "enc_X size < row_W")
+    Wm = np.ascontiguousarray(np.asarray(W, np.float64)[:row_W, :col_W])
+    bv = np.ascontiguousarray(bias_vec, np.int32)
+    chain = enc_X[0].chain_index
+    if out is None:
+        out = [Ciphertext(context, 2, chain + 1) for _ in range(col_W)]
+    xs = (C.c_void_p * row_W)(*[x.h.value for x in enc_X[:row_W]])
+    os_ = (C.c_void_p * col_W)(*[o.h.value for o in out])
+    check(_lib().shm_ct_pt_matrix_mul_masked(context.h, xs, row_W, Wm.ctypes.data_as(C.POINTER(C.c_double)), col_W,
+                                             bv.ctypes.data_as(C.POINTER(C.c_int32)), bv.size, os_, stream))
+    return out
+
+
 def synchronize(stream=None) -> None:
     """Wait for all work queued on `stream` (None: the default stream)."""
     check(_lib().shm_synchronize(stream))

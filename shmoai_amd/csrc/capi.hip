@@ -767,6 +767,34 @@ int shm_ct_pt_matrix_mul(shm_context *ctx, const shm_ciphertext *const *X, uint3
     });
 }
 
+int shm_ct_pt_matrix_mul_masked(shm_context *ctx, const shm_ciphertext *const *X, uint32_t row_W, const double *W,
+                                uint32_t col_W, const int32_t *bias_vec, size_t n_bias, shm_ciphertext *const *out,
+                                shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(X, "enc_X");
+        need(W, "W");
+        need(bias_vec, "bias_vec");
+        need(out, "out");
+        const size_t n = ctx->c->N / 2;
+        if (n_bias != n) throw InvalidArgument("bias_vec must hold slot_count entries");
+        std::vector<uint8_t> mask(n);
+        for (size_t t = 0; t < n; t++) mask[t] = bias_vec[t] == 1 ? 1 : 0;
+        std::vector<const Ciphertext *> xs(row_W);
+        for (uint32_t j = 0; j < row_W; j++) {
+            need(X[j], "enc_X[j]");
+            xs[j] = &X[j]->ct;
+        }
+        std::vector<Ciphertext *> os(col_W);
+        for (uint32_t i = 0; i < col_W; i++) {
+            need(out[i], "out[i]");
+            out[i]->ct.ctx = ctx->c.get();
+            os[i] = &out[i]->ct;
+        }
+        ctpt_matmul_masked(*ctx->c, xs, W, row_W, col_W, mask.data(), os, S(st));
+    });
+}
+
 double shm_last_matmul_kernel_ms(void) { return (double)last_matmul_kernel_ms(); }
 int shm_last_matmul_variant(void) { return last_matmul_variant(); }
 

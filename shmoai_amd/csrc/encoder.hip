@@ -53,25 +53,48 @@ __device__ __forceinline__ uint32_t psi_index(const uint32_t *group, uint32_t k,
     return idx;
 }
 
+// Butterfly tid of one Gentleman-Sande stage of the inverse special FFT.
+__device__ __forceinline__ void ifft_butterfly(double2 *buf, const double2 *roots, const uint32_t *group,
+                                               uint32_t tid, int logn, int logPairs, int logRatio, uint32_t M,
+                                               double scalar, int last) {
+    const uint32_t pairs = 1u << logPairs;
+    const uint32_t k = tid >> logPairs, j = tid & (pairs - 1);
+    const uint32_t g = 2 * k * pairs + j;
+    const double2 tw = roots[M - psi_index(group, k, logPairs, logn, logRatio, M)];
+    const double2 a = buf[g], b = buf[g + pairs];
+    double2 s0 = make_double2(a.x + b.x, a.y + b.y);
+    double2 s1 = make_double2(a.x - b.x, a.y - b.y);
+    s1 = cmul(s1, tw);
+    if (last && scalar != 0.0) {
+        s0 = make_double2(s0.x * scalar, s0.y * scalar);
+        s1 = make_double2(s1.x * scalar, s1.y * scalar);
+    }
+    buf[g] = s0;
+    buf[g + pairs] = s1;
+}
+
 // One Gentleman-Sande stage of the inverse special FFT.
 __global__ void k_ifft_stage(double2 *buf, const double2 *roots, const uint32_t *group, uint32_t n, int logn,
                              int logPairs, int logRatio, uint32_t M, double scalar, int last) {
-    const uint32_t pairs = 1u << logPairs;
-    for (uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x; tid < n / 2; tid += gridDim.x * blockDim.x) {
-        const uint32_t k = tid >> logPairs, j = tid & (pairs - 1);
-        const uint32_t g = 2 * k * pairs + j;
-        const double2 tw = roots[M - psi_index(group, k, logPairs, logn, logRatio, M)];
-        const double2 a = buf[g], b = buf[g + pairs];
-        double2 s0 = make_double2(a.x + b.x, a.y + b.y);
-        double2 s1 = make_double2(a.x - b.x, a.y - b.y);
-        s1 = cmul(s1, tw);
-        if (last && scalar != 0.0) {
-            s0 = make_double2(s0.x * scalar, s0.y * scalar);
-            s1 = make_double2(s1.x * scalar, s1.y * scalar);
-        }
-        buf[g] = s0;
-        buf[g + pairs] = s1;
-    }
+    for (uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x; tid < n / 2; tid += gridDim.x * blockDim.x)
+        ifft_butterfly(buf, roots, group, tid, logn, logPairs, logRatio, M, scalar, last);
+}
+
+// The same stage over a batch of vectors: buf[b][n], last-stage scalar fix[b].
+__global__ void k_ifft_stage_batch(double2 *buf, const double2 *roots, const uint32_t *group, uint32_t n, int logn,
+                                   int logPairs, int logRatio, uint32_t M, const double *fix, int last) {
+    const uint32_t b = blockIdx.y;
+    for (uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x; tid < n / 2; tid += gridDim.x * blockDim.x)
+        ifft_butterfly(buf + (size_t)b * n, roots, group, tid, logn, logPairs, logRatio, M, fix[b], last);
+}
+
+// Slot vectors w[b] * mask, bit-reversed: exactly the values a host vector
+// {mask[t] == 1 ? w : 0.0} loads (Ct_pt_matrix_mul.cuh:931-938).
+__global__ void k_mask_load_batch(const double *w, const uint8_t *mask, double2 *buf, uint32_t n, int logn) {
+    const uint32_t b = blockIdx.y;
+    const double v = w[b];
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+        buf[(size_t)b * n + brv_bits(t, logn)] = make_double2(mask[t] ? v : 0.0, 0.0);
 }
 
 // One Cooley-Tukey stage of the forward special FFT.
@@ -97,6 +120,20 @@ __global__ void k_decompose(const double2 *buf, uint64_t *pt, const DevMod *mods
         const size_t k = t / (2 * n), ci = t % (2 * n);
         const double v = ci < n ? round(buf[ci].x) : round(buf[ci - n].y);
         pt[k * N + ci * ratio] = residue_of_rounded(v, mods[k]);
+    }
+}
+
+// k_decompose over a batch: plaintext b at pt + b * Ql * N (every coefficient
+// is written: n = N / 2 here).
+__global__ void k_decompose_batch(const double2 *buf, uint64_t *pt, const DevMod *mods, uint32_t n, size_t Ql,
+                                  size_t N) {
+    const size_t b = blockIdx.y, total = Ql * 2 * (size_t)n;
+    const double2 *bb = buf + b * n;
+    uint64_t *pb = pt + b * Ql * N;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t k = t / (2 * n), ci = t % (2 * n);
+        const double v = ci < n ? round(bb[ci].x) : round(bb[ci - n].y);
+        pb[k * N + ci] = residue_of_rounded(v, mods[k]);
     }
 }
 
@@ -233,6 +270,28 @@ void encode(const Context &c, const double *re, const double *im, size_t n_value
                                                         N);
     SHM_HIP(hipGetLastError());
     ntt_forward(c, out, 1, Ql, 0, s);
+}
+
+void encode_masked_batch(const Context &c, const double *w, const double *fix, const uint8_t *mask, size_t B,
+                         size_t Ql, uint64_t *out, double2 *fftbuf, hipStream_t s) {
+    const size_t N = c.N, n = N / 2, M = 2 * N;
+    if (B == 0) return;
+    if (B > 65535) throw InvalidArgument("encode_masked_batch: at most 65535 plaintexts");
+    const int logn = ilog2(n), logRatio = ilog2(M) - logn - 2;
+    const unsigned gx = std::min<unsigned>(grid_for(n / 2), 64u);
+    k_mask_load_batch<<<dim3(std::min<unsigned>(grid_for(n), 128u), (unsigned)B), kBlock, 0, s>>>(w, mask, fftbuf,
+                                                                                                 (uint32_t)n, logn);
+    for (int iter = logn - 1; iter >= 0; iter--) {
+        const int logPairs = logn - iter - 1;
+        const int last = (size_t(1) << logPairs) == n / 2;
+        k_ifft_stage_batch<<<dim3(gx, (unsigned)B), kBlock, 0, s>>>(fftbuf, c.enc_roots.as<double2>(),
+                                                                     c.enc_group.as<uint32_t>(), (uint32_t)n, logn,
+                                                                     logPairs, logRatio, (uint32_t)M, fix, last);
+    }
+    k_decompose_batch<<<dim3(std::min<unsigned>(grid_for(Ql * N), 256u), (unsigned)B), kBlock, 0, s>>>(
+        fftbuf, out, c.mods(), (uint32_t)n, Ql, N);
+    SHM_HIP(hipGetLastError());
+    ntt_forward(c, out, B, Ql, 0, s);
 }
 
 void encode_scalar(const Context &c, double value, size_t Ql, double scale, uint64_t *out, hipStream_t s) {

@@ -320,12 +320,20 @@ void keyswitch_batch(const Context &c, const PolyPack &c2, size_t m, const uint6
 void encode(const Context &c, const double *re, const double *im, size_t n_values, size_t Ql, double scale,
             uint64_t *out, hipStream_t s);
 void encode_scalar(const Context &c, double value, size_t Ql, double scale, uint64_t *out, hipStream_t s);
+// B plaintexts [B][Ql][N] in NTT form: plaintext b encodes the slot vector
+// {mask[t] ? w[b] : 0} (t < N/2) at scale fix[b] * N/2, exactly as encode() does
+// for that vector. w, fix, mask are device arrays; fftbuf holds B * N/2 double2.
+void encode_masked_batch(const Context &c, const double *w, const double *fix, const uint8_t *mask, size_t B,
+                         size_t Ql, uint64_t *out, double2 *fftbuf, hipStream_t s);
 void decode(const Context &c, const uint64_t *pt, size_t Ql, double scale, double *re, double *im, size_t n,
             hipStream_t s);
 
 // MOAI matmul (matmul.hip)
 void ctpt_matmul(const Context &c, const std::vector<const Ciphertext *> &X, const double *W, size_t row_w,
                  size_t col_w, const std::vector<Ciphertext *> &out, int variant, hipStream_t s);
+// masked form: the plaintext of term (j, i) encodes {mask[t] ? W[j][i] : 0} (t < N/2)
+void ctpt_matmul_masked(const Context &c, const std::vector<const Ciphertext *> &X, const double *W, size_t row_w,
+                        size_t col_w, const uint8_t *mask, const std::vector<Ciphertext *> &out, hipStream_t s);
 
 // evaluator-level operations (ops.hip)
 uint64_t host_splitmix64(uint64_t key, uint64_t ctr);

@@ -121,6 +121,59 @@ __global__ void k_mul_const_add(uint64_t *acc, const uint64_t *x, const uint64_t
     }
 }
 
+// Masked form: up to kMaskJ input ciphertexts of one row block, by value.
+constexpr int kMaskJ = 8;
+struct XBlock {
+    const uint64_t *p[kMaskJ];
+};
+
+// acc[cc][poly][t] (+)= sum_{jj < jb} X_jj[poly][t] * pt[jj * cb + cc][t] for
+// t < Ql N: the products of one block of inputs and one block of outputs, with
+// 128-bit sums and one Barrett reduction per output word; `first` starts the
+// sums, otherwise they are added mod q to the earlier row blocks'.
+__global__ void k_mask_mac(XBlock xb, int jb, const uint64_t *__restrict__ pt, int cb, uint64_t *__restrict__ acc,
+                           const DevMod *__restrict__ mods, size_t Ql, size_t N, int first) {
+    const size_t S = Ql * N;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < S; t += (size_t)gridDim.x * blockDim.x) {
+        const DevMod m = mods[t / N];
+        uint64_t x0[kMaskJ], x1[kMaskJ];
+#pragma unroll
+        for (int jj = 0; jj < kMaskJ; jj++) {
+            x0[jj] = jj < jb ? xb.p[jj][t] : 0;
+            x1[jj] = jj < jb ? xb.p[jj][S + t] : 0;
+        }
+        for (int cc = 0; cc < cb; cc++) {
+            Acc128 a0{0, 0}, a1{0, 0};
+#pragma unroll
+            for (int jj = 0; jj < kMaskJ; jj++) {
+                if (jj < jb) {
+                    const uint64_t w = pt[(size_t)(jj * cb + cc) * S + t];
+                    mac128(a0, x0[jj], w);
+                    mac128(a1, x1[jj], w);
+                }
+            }
+            uint64_t v0 = reduce128(a0, m), v1 = reduce128(a1, m);
+            uint64_t *o = acc + (size_t)cc * 2 * S + t;
+            if (!first) {
+                v0 = add_mod(v0, o[0], m.q);
+                v1 = add_mod(v1, o[S], m.q);
+            }
+            o[0] = v0;
+            o[S] = v1;
+        }
+    }
+}
+
+// w[b] = W[j0 + jj][c0 + cc], fix[b] = fixj[j0 + jj] for b = jj * cb + cc.
+__global__ void k_mask_weights(const double *W, const double *fixj, double *w, double *fix, size_t j0, size_t c0,
+                               int jb, int cb, size_t col_w) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= jb * cb) return;
+    const int jj = b / cb, cc = b % cb;
+    w[b] = W[(j0 + jj) * col_w + c0 + cc];
+    fix[b] = fixj[j0 + jj];
+}
+
 // Batched rescale helpers over many ciphertexts (each [2][Ql][N]).
 __global__ void k_gather_last(const uint64_t *acc, uint64_t *last, size_t Ql, size_t N, size_t npolys) {
     const size_t total = npolys * N;
@@ -157,6 +210,55 @@ bool int8_supported(const Context &c, size_t Ql, const double *W, const double *
                     int *dx_out, int *dw_out);
 
 int g_last_variant = -1;
+
+void ctpt_matmul_masked(const Context &c, const std::vector<const Ciphertext *> &X, const double *W, size_t row_w,
+                        size_t col_w, const uint8_t *mask, const std::vector<Ciphertext *> &out, hipStream_t s) {
+    if (X.size() < row_w || out.size() != col_w || row_w == 0 || col_w == 0)
+        throw InvalidArgument("bad dimensions of X or W");
+    const size_t Ql = X[0]->Ql(), N = c.N, S = Ql * N, n = N / 2;
+    if (Ql < 2) throw LogicError("end of modulus switching chain reached");
+    for (size_t j = 0; j < row_w; j++) {
+        if (X[j]->chain != X[0]->chain || X[j]->size != 2)
+            throw InvalidArgument("encrypted and plain parameter mismatch");
+        if (!(X[j]->scale > 0)) throw InvalidArgument("scale out of bounds");
+    }
+    const double out_scale = X[0]->scale;
+    for (Ciphertext *o : out) o->resize(2, X[0]->chain + 1, s);
+    // plaintexts of one (row block, output block) are encoded, transformed and
+    // contracted together
+    const size_t JB = std::min<size_t>(row_w, kMaskJ), CB = std::min<size_t>(col_w, 16);
+    std::vector<double> fixj(row_w);
+    for (size_t j = 0; j < row_w; j++) fixj[j] = X[j]->scale / (double)n;  // encode()'s scale / n
+    Scratch acc(col_w * 2 * S * sizeof(uint64_t), s);
+    Scratch pt(JB * CB * S * sizeof(uint64_t), s);
+    Scratch fft(JB * CB * n * sizeof(double2), s);
+    Scratch dmask(n, s);
+    Scratch dW(row_w * col_w * sizeof(double), s);
+    Scratch dfixj(row_w * sizeof(double), s);
+    Scratch dw(JB * CB * sizeof(double), s);
+    Scratch dfix(JB * CB * sizeof(double), s);
+    SHM_HIP(hipMemcpyAsync(dmask.as<void>(), mask, n, hipMemcpyHostToDevice, s));
+    SHM_HIP(hipMemcpyAsync(dW.as<void>(), W, row_w * col_w * sizeof(double), hipMemcpyHostToDevice, s));
+    SHM_HIP(hipMemcpyAsync(dfixj.as<void>(), fixj.data(), row_w * sizeof(double), hipMemcpyHostToDevice, s));
+    for (size_t c0 = 0; c0 < col_w; c0 += CB) {
+        const int cb = (int)std::min(CB, col_w - c0);
+        for (size_t j0 = 0; j0 < row_w; j0 += JB) {
+            const int jb = (int)std::min(JB, row_w - j0);
+            k_mask_weights<<<1, kBlock, 0, s>>>(dW.as<double>(), dfixj.as<double>(), dw.as<double>(),
+                                                 dfix.as<double>(), j0, c0, jb, cb, col_w);
+            encode_masked_batch(c, dw.as<double>(), dfix.as<double>(), dmask.as<uint8_t>(), (size_t)(jb * cb), Ql,
+                                pt.as<uint64_t>(), fft.as<double2>(), s);
+            XBlock xb{};
+            for (int jj = 0; jj < jb; jj++) xb.p[jj] = X[j0 + jj]->data();
+            k_mask_mac<<<grid_for(S), kBlock, 0, s>>>(xb, jb, pt.as<uint64_t>(), cb, acc.as<uint64_t>() + c0 * 2 * S,
+                                                      c.mods(), Ql, N, j0 == 0 ? 1 : 0);
+            SHM_HIP(hipGetLastError());
+        }
+    }
+    batched_rescale(c, acc.as<uint64_t>(), out, Ql, s);
+    SHM_HIP(hipStreamSynchronize(s));
+    for (Ciphertext *o : out) o->scale = out_scale;
+}
 
 float last_matmul_kernel_ms() { return g_last_ms; }
 int last_matmul_variant() { return g_last_variant; }

@@ -7,6 +7,11 @@
 - full quarter-width shape (N=2^16, L=24, 192 x 192): INT8-MFMA == 64-bit
   path on every limb of every output, and sampled output columns bit-exact
   against the oracle;
+- masked form (Ct_pt_matrix_mul.cuh:846-985): bit-exact against the oracle
+  composed from its encode (vector), multiply_plain, add and rescale, over two
+  row blocks and two output blocks; with an all-ones mask and a power-of-two
+  scale it equals the unmasked matmul limb for limb (the inverse special FFT of
+  a constant vector is exact);
 - closed-form known answer from the reference's own test
   (test_ct_pt_matrix_mul.cuh: X[i][j][k] = j+1, W = 1/128 -> slot of token j
   decodes to (row_W/128) * (j+1)).
@@ -111,3 +116,54 @@ def test_matmul_known_answer():
             v = enc.decode(dec.decrypt(o))
             expect = np.repeat(np.arange(1, num_row + 1) * (row_w / 128.0), num_X)
             assert np.max(np.abs(v - expect)) < 1e-3
+
+
+def _masked_oracle(ref, xs, W, mask, Ql, scale):
+    outs = []
+    for i in range(W.shape[1]):
+        acc = None
+        for j in range(W.shape[0]):
+            pt = ref.encode(np.where(mask == 1, W[j, i], 0.0), Ql, scale)
+            t = ref.multiply_plain(xs[j], pt, Ql)
+            acc = t if acc is None else ref.add(acc, t, Ql, 2)
+        outs.append(ref.rescale(acc, Ql))
+    return outs
+
+
+def test_matmul_masked_bitexact(oracle_lib):
+    bits = [50] * 4 + [60]
+    ctx = shm.Context(13, bits, 1)
+    ref = oracle_lib.Ref(13, bits, 1)
+    rng = np.random.default_rng(9)
+    row_w, col_w, Ql = 10, 20, 4
+    xs, hs = _rand_cts(ctx, ref, rng, row_w, Ql)
+    W = np.random.default_rng(10).normal(0, 0.02, (row_w, col_w))
+    mask = (rng.random(ctx.N // 2) < 0.6).astype(np.int32)
+    mask[:7] = [1, 0, 2, 1, -1, 1, 0]  # only entries equal to 1 select (bias_vec[j] == 1)
+    out = shm.ct_pt_matrix_mul_wo_pre_w_mask(hs, W, mask, row_w, col_w, row_w, ctx)
+    exp = _masked_oracle(ref, xs, W, mask, Ql, 2.0 ** 40)
+    for i in range(col_w):
+        assert out[i].chain_index == ctx.chain(Ql - 1)
+        assert out[i].scale == 2.0 ** 40
+        assert np.array_equal(out[i].numpy(), exp[i]), f"output {i}"
+
+
+def test_matmul_masked_all_ones_equals_unmasked(oracle_lib):
+    bits = [50] * 6 + [60]
+    ctx = shm.Context(13, bits, 1)
+    ref = oracle_lib.Ref(13, bits, 1)
+    rng = np.random.default_rng(11)
+    row_w, col_w, Ql = 24, 18, 6
+    xs, hs = _rand_cts(ctx, ref, rng, row_w, Ql)
+    W = np.random.default_rng(12).normal(0, 0.02, (row_w, col_w))
+    om = shm.ct_pt_matrix_mul_wo_pre_w_mask(hs, W, np.ones(ctx.N // 2, np.int32), row_w, col_w, row_w, ctx)
+    ou = shm.ct_pt_matrix_mul_wo_pre(hs, W, row_w, col_w, row_w, ctx, variant=shm.MATMUL_U64)
+    for i in range(col_w):
+        assert np.array_equal(om[i].numpy(), ou[i].numpy()), f"output {i}"
+
+
+def test_matmul_masked_rejects_short_mask():
+    ctx = shm.Context(13, [50] * 3 + [60], 1)
+    hs = [shm.Ciphertext(ctx, 2, 1) for _ in range(2)]
+    with pytest.raises(shm.capi.ShmInvalidArgument):
+        shm.ct_pt_matrix_mul_wo_pre_w_mask(hs, np.zeros((2, 2)), np.ones(10, np.int32), 2, 2, 2, ctx)
