@@ -21,7 +21,8 @@ EvalMod), and shared-a chunks of n = 4 and n = 8, with the decryption error of
 the timed outputs.
 
 `full_width_768` times one 768 x 768 matmul of the same kind (the BERT-base
-projection width, W rows contracted in chunks of 256).
+projection width, W rows contracted in chunks of 256), and in `masked` the
+masked form (a token mask in every weight plaintext) per term.
 
 Two more objects give the HBM rate of the transform and key-switch kernels:
 `ntt` (forward + inverse NTT of 64 polynomials x 24 limbs at N = 2^16) and
@@ -256,6 +257,27 @@ def full_width_bench(shm, ctx, hs, steps: int = 2):
             "contraction_frac_of_int8_peak": round(tops / INT8_PEAK_TOPS, 4)}
 
 
+def masked_bench(shm, ctx, hs, cols: int = 16):
+    """The masked form (ct_pt_matrix_mul_wo_pre_w_mask) on the 192 inputs and
+    `cols` outputs: every term encodes W[j][i] in the slots of a token mask
+    (a full polynomial per term), so the time per term is the figure; it is
+    extrapolated to the FFN output projection of the reference's layer
+    (3072 x 768 terms)."""
+    rng = np.random.default_rng(8)
+    mask = (rng.random(ctx.N // 2) < 0.7).astype(np.int32)
+    W = rng.normal(0.0, 0.02, (ROW_W, cols))
+    out = [shm.Ciphertext(ctx, 2, 2) for _ in range(cols)]
+    shm.ct_pt_matrix_mul_wo_pre_w_mask(hs[:8], W[:8], mask, 8, cols, 8, ctx, out=out)
+    t0 = time.perf_counter()
+    shm.ct_pt_matrix_mul_wo_pre_w_mask(hs, W, mask, ROW_W, cols, ROW_W, ctx, out=out)
+    dt = time.perf_counter() - t0
+    per = dt / (ROW_W * cols)
+    log(f"bench: masked matmul {ROW_W}x{cols}: {dt * 1e3:.1f} ms, {per * 1e6:.2f} us per term")
+    return {"workload": f"masked ct-pt matmul {ROW_W}x{cols}, N=2^16, L=24, 70% of slots selected, incl. rescale",
+            "ms": round(dt * 1e3, 2), "us_per_term": round(per * 1e6, 3),
+            "ffn_3072x768_s_extrapolated": round(per * 3072 * 768, 2)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -329,6 +351,7 @@ def main() -> int:
     full = None
     if not args.no_full_width and variant == shm.MATMUL_INT8:
         full = full_width_bench(shm, ctx, hs)
+        full["masked"] = masked_bench(shm, ctx, hs)
 
     ntt = ntt_bench(shm, ctx)
     boot = None

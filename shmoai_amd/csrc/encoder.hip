@@ -123,20 +123,6 @@ __global__ void k_decompose(const double2 *buf, uint64_t *pt, const DevMod *mods
     }
 }
 
-// k_decompose over a batch: plaintext b at pt + b * Ql * N (every coefficient
-// is written: n = N / 2 here).
-__global__ void k_decompose_batch(const double2 *buf, uint64_t *pt, const DevMod *mods, uint32_t n, size_t Ql,
-                                  size_t N) {
-    const size_t b = blockIdx.y, total = Ql * 2 * (size_t)n;
-    const double2 *bb = buf + b * n;
-    uint64_t *pb = pt + b * Ql * N;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t k = t / (2 * n), ci = t % (2 * n);
-        const double v = ci < n ? round(bb[ci].x) : round(bb[ci - n].y);
-        pb[k * N + ci] = residue_of_rounded(v, mods[k]);
-    }
-}
-
 // Constant plaintext in NTT form: every slot holds [round(fl(value*scale))]_q.
 __global__ void k_fill_scalar(uint64_t *pt, const DevMod *mods, double v, size_t N, size_t total) {
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
@@ -288,10 +274,9 @@ void encode_masked_batch(const Context &c, const double *w, const double *fix, c
                                                                      c.enc_group.as<uint32_t>(), (uint32_t)n, logn,
                                                                      logPairs, logRatio, (uint32_t)M, fix, last);
     }
-    k_decompose_batch<<<dim3(std::min<unsigned>(grid_for(Ql * N), 256u), (unsigned)B), kBlock, 0, s>>>(
-        fftbuf, out, c.mods(), (uint32_t)n, Ql, N);
     SHM_HIP(hipGetLastError());
-    ntt_forward(c, out, B, Ql, 0, s);
+    // decompose (round, residue per prime) fused into the NTT's column pass
+    ntt_forward_encoded(c, fftbuf, out, B, Ql, s);
 }
 
 void encode_scalar(const Context &c, double value, size_t Ql, double scale, uint64_t *out, hipStream_t s) {

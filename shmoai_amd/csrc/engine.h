@@ -215,6 +215,11 @@ struct GaloisKeys {
 // (device array) or by a contiguous range starting at first_prime.
 void ntt_forward(const Context &c, uint64_t *data, size_t n_polys, size_t n_limbs, size_t first_prime,
                  hipStream_t s);
+// Encoder decompose + forward NTT in one: out[p][k] = NTT of the rounded FFT
+// output fft[p] (N/2 complex entries: real parts are coefficients 0..N/2-1,
+// imaginary parts N/2..N-1) reduced mod prime k, for k < n_limbs.
+void ntt_forward_encoded(const Context &c, const double2 *fft, uint64_t *out, size_t n_polys, size_t n_limbs,
+                         hipStream_t s);
 void ntt_inverse(const Context &c, uint64_t *data, size_t n_polys, size_t n_limbs, size_t first_prime,
                  hipStream_t s);
 // Same, for a QlP-shaped polynomial: limbs [0, Ql) use primes 0..Ql-1 and the

@@ -240,7 +240,11 @@ __device__ __forceinline__ void run_rounds(E *lds, uint32_t gmul, uint32_t gadd_
 //   IO_RESCALE : (phase B, forward) instead of storing v = NTT(delta), store
 //                (acc - v) * qinv mod q to outs[p / ppo] + (p % ppo) L1 N + k N
 //                (rns.cu:1138-1155 fused with the final NTT pass).
-enum { IO_PLAIN = 0, IO_REDUCE = 1, IO_RESCALE = 2 };
+//   IO_ENCODE  : (phase A, forward) the input of limb k of polynomial p is the
+//                rounded coefficient of the encoder's FFT output fft[p] (index
+//                i < N/2: round(re), i >= N/2: round(im) of entry i - N/2) as a
+//                residue, i.e. the encoder's decompose step fused into the load.
+enum { IO_PLAIN = 0, IO_REDUCE = 1, IO_RESCALE = 2, IO_ENCODE = 3 };
 
 struct NttIO {
     const uint64_t *src;      // IO_REDUCE: [n_polys][N]
@@ -254,6 +258,7 @@ struct NttIO {
     uint64_t center;          // IO_REDUCE: 0, or h = q_last >> 1: values above h stand for v - q_last
     uint32_t packed;          // IO_RESCALE: nonzero -> polynomial p reads acc_p[p], writes out_p[p]
     PolyPack acc_p, out_p;
+    const double2 *fft;       // IO_ENCODE: [n_polys][N/2]
 };
 
 // Phase A (columns, first log N1 stages, log N1 <= 5). Each thread owns one
@@ -286,6 +291,17 @@ __global__ __launch_bounds__(kThreads) void k_ntt_cols(uint64_t *__restrict__ da
                 uint64_t y = reduce64(v, m);
                 if (io.center && v > io.center) y = sub_mod(y, reduce64(2 * io.center + 1, m), m.q);
                 x[r] = y;
+            }
+        } else if (IO == IO_ENCODE) {
+            const DevMod m = tb.mods[prime];
+            const double2 *fb = io.fft + (size_t)poly * (N / 2) + c;
+            // rows r < T/2 lie below N/2 (real parts), row r + T/2 is the
+            // imaginary part of the same entry
+#pragma unroll
+            for (int r = 0; r < T / 2; r++) {
+                const double2 v = fb[r * N2];
+                x[r] = residue_of_rounded(round(v.x), m);
+                x[r + T / 2] = residue_of_rounded(round(v.y), m);
             }
         } else {
 #pragma unroll
@@ -476,6 +492,14 @@ void ntt_forward(const Context &c, uint64_t *data, size_t n_polys, size_t n_limb
     LimbMap mp{(uint32_t)n_limbs, 0, 0xffffffffu, 0, 0xffffffffu, (uint32_t)c.nQ, (uint32_t)first_prime,
                (uint64_t)n_limbs * c.N};
     launch_ntt<true, IO_PLAIN, IO_PLAIN>(c, data, mp, n_polys * n_limbs, s);
+}
+
+void ntt_forward_encoded(const Context &c, const double2 *fft, uint64_t *out, size_t n_polys, size_t n_limbs,
+                         hipStream_t s) {
+    LimbMap mp{(uint32_t)n_limbs, 0, 0xffffffffu, 0, 0xffffffffu, (uint32_t)c.nQ, 0, (uint64_t)n_limbs * c.N};
+    NttIO io{};
+    io.fft = fft;
+    launch_ntt<true, IO_ENCODE, IO_PLAIN>(c, out, mp, n_polys * n_limbs, s, io);
 }
 
 void ntt_inverse(const Context &c, uint64_t *data, size_t n_polys, size_t n_limbs, size_t first_prime,
