@@ -356,16 +356,13 @@ __global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ 
         const int b = i / T, r = i % T;
         lds[b * TP + swz<T>(r)] = base[i];
     }
-    // rescale: this thread's accumulator words are loaded now and arrive
-    // while the butterflies run
+    // rescale: the accumulator words are read when the outputs are written
+    // (a prefetch into registers before the butterflies cost occupancy)
     constexpr int PER = kTile / kThreads;
-    uint64_t accv[IO == IO_RESCALE ? PER : 1];
-    if (FWD && IO == IO_RESCALE) {
-        const uint64_t *acc = io.packed ? io.acc_p.p[poly] + (size_t)k * N + t0
-                                        : io.acc + ((size_t)poly * io.acc_limbs + k) * N + t0;
-#pragma unroll
-        for (int j = 0; j < PER; j++) accv[j] = acc[threadIdx.x + j * kThreads];
-    }
+    const uint64_t *accp = nullptr;
+    if (FWD && IO == IO_RESCALE)
+        accp = io.packed ? io.acc_p.p[poly] + (size_t)k * N + t0
+                         : io.acc + ((size_t)poly * io.acc_limbs + k) * N + t0;
     __syncthreads();
     if (LAZY)
         run_rounds<LOGT, FWD>(lds, N1, 1u, (uint32_t)(t0 / T), tb.tw + (size_t)prime * N,
@@ -393,7 +390,7 @@ __global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ 
             const int i = threadIdx.x + j * kThreads;
             const int b = i / T, r = i % T;
             const uint64_t v = canon(lds[b * TP + swz<T>(r)]);
-            __builtin_nontemporal_store(mul_shoup(sub_mod(accv[j], v, q), qi, qis, q), &out[i]);
+            __builtin_nontemporal_store(mul_shoup(sub_mod(accp[i], v, q), qi, qis, q), &out[i]);
         }
     } else if (FWD) {
         for (int i = threadIdx.x; i < kTile; i += kThreads) {
@@ -432,8 +429,7 @@ void launch_ntt(const Context &c, uint64_t *data, const LimbMap &mp, size_t tota
                  c.ninv.as<uint64_t>(),
                  c.ninv_s.as<uint64_t>()};
     // unreduced forward butterflies when every prime of the launch is below 2^58
-    // (plain transforms only: the fused rescale's tile pass measured slower so)
-    bool lazy = FWD && IOB != IO_RESCALE;
+    bool lazy = FWD;
     for (uint32_t j = 0; j < mp.n_active && lazy; j++) {
         uint32_t k = mp.lo + j;
         if (k >= mp.skip_lo) k += mp.skip_width;
