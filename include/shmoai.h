@@ -287,6 +287,32 @@ int shm_bootstrap_eval_mod(shm_bootstrapper *b, shm_ciphertext *ct, const shm_ks
  * step for the group; each result equals shm_bootstrap_eval_mod's bit for bit. */
 int shm_bootstrap_eval_mod_batch(shm_bootstrapper *b, shm_ciphertext *const *cts, size_t count,
                                  const shm_kswitch_key *relin, shm_stream_t stream);
+/* ---- bootstrapping server (continuous batching) --------------------------------------
+ * No counterpart in the reference, whose BatchBootstrapper (BatchBootstrap.cuh:470-820)
+ * takes one fixed batch per call. A worker thread holds the keys resident and
+ * bootstraps (S2C-first, as shm_bootstrap_s2c_first) the ciphertexts of all admitted
+ * requests in groups of up to max_group (1..8) ciphertexts of one scale, slots of
+ * different requests side by side; requests submitted while a group runs join the next
+ * group. A request is admitted while the estimated device footprint of the admitted
+ * requests stays within mem_budget_bytes (one larger request enters alone). Each
+ * output equals, limb for limb, the bootstrap of that input alone. Device work runs on
+ * the default stream. Submit copies the inputs; out[] must not be touched until
+ * shm_server_wait returns for that request. Destroy finishes all submitted work. */
+typedef struct shm_server shm_server;
+typedef struct {
+    uint64_t submitted, completed, failed, ciphertexts, groups;
+    uint64_t in_flight_bytes;
+    double p50_ms, p99_ms; /* request latency, submit to completion */
+    double busy_s;         /* wall time inside grouped bootstraps */
+} shm_server_stats;
+int shm_server_create(shm_bootstrapper *b, const shm_galois_keys *gk, const shm_kswitch_key *relin,
+                      uint64_t mem_budget_bytes, uint32_t max_group, shm_server **out);
+int shm_server_destroy(shm_server *srv);
+int shm_server_submit(shm_server *srv, const shm_ciphertext *const *in, size_t count, shm_ciphertext *const *out,
+                      uint64_t *request_id);
+int shm_server_wait(shm_server *srv, uint64_t request_id, double *latency_ms);
+int shm_server_get_stats(shm_server *srv, shm_server_stats *stats);
+
 /* Polynomial::homomorphic_poly_evaluation (bootstrapping/common/Polynomial.cu:253-476)
  * in Chebyshev form, as the non-linear layers use it (gelu_cheb,
  * non_linear_func/gelu_other.cuh:257-330: sum_k d_k T_k(u), u affine in x):

@@ -511,6 +511,53 @@ class Bootstrapper:
         return out
 
 
+class BootstrapServer:
+    """Request server with continuous batching of S2C-first bootstraps (no counterpart
+    in the reference). submit() queues a request of a few 4-prime ciphertexts and
+    returns its id and output ciphertexts; wait(id) blocks until they are ready and
+    returns the request latency in ms. Requests from any thread share groups of up to
+    max_group ciphertexts; admission is bounded by mem_budget bytes."""
+
+    def __init__(self, boot: Bootstrapper, gk: GaloisKeys, relin_key: KSwitchKey, mem_budget: int = 16 << 30,
+                 max_group: int = 8):
+        self.ctx = boot.ctx
+        self._keep = (boot, gk, relin_key)
+        h = C.c_void_p()
+        check(_lib().shm_server_create(boot.h, gk.h, relin_key.h, int(mem_budget), int(max_group), C.byref(h)))
+        self.h = h
+        self._outs = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(_lib().shm_server_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, cts: Sequence[Ciphertext]) -> int:
+        outs = [Ciphertext(self.ctx, 2, 1) for _ in cts]
+        rid = C.c_uint64()
+        check(_lib().shm_server_submit(self.h, _ptrs(cts), len(cts), _ptrs(outs), C.byref(rid)))
+        self._outs[rid.value] = outs
+        return rid.value
+
+    def wait(self, request_id: int):
+        """Returns (outputs, latency_ms)."""
+        lat = C.c_double()
+        outs = self._outs.pop(request_id)
+        check(_lib().shm_server_wait(self.h, request_id, C.byref(lat)))
+        return outs, lat.value
+
+    def stats(self) -> dict:
+        st = capi.ServerStats()
+        check(_lib().shm_server_get_stats(self.h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in st._fields_}
+
+
 def ct_pt_matrix_mul_wo_pre(enc_X: Sequence[Ciphertext], W: np.ndarray, col_X: int, col_W: int, row_W: int,
                             context: Context, variant: int = MATMUL_INT8, out=None, stream=None):
     """Ct_pt_matrix_mul.cuh:18-185: out[i] = rescale(sum_j X[j] (.) encode(W[j][i])).

@@ -39,6 +39,11 @@ class BootParams(C.Structure):
                 ("double_angles", C.c_int32), ("input_bound", C.c_double)]
 
 
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", u64), ("completed", u64), ("failed", u64), ("ciphertexts", u64), ("groups", u64),
+                ("in_flight_bytes", u64), ("p50_ms", dbl), ("p99_ms", dbl), ("busy_s", dbl)]
+
+
 class ContextInfo(C.Structure):
     _fields_ = [("poly_degree", u32), ("size_q", u32), ("size_p", u32), ("dnum", u32)]
 
@@ -123,6 +128,11 @@ _SIGS = {
     "shm_bootstrap_eval_mod": (i32, [vp, vp, vp, vp]),
     "shm_bootstrap_eval_mod_batch": (i32, [vp, vp, sz, vp, vp]),
     "shm_bootstrap_s2c_first": (i32, [vp, pvp, sz, pvp, vp, vp, dbl, vp]),
+    "shm_server_create": (i32, [vp, vp, vp, u64, u32, pvp]),
+    "shm_server_destroy": (i32, [vp]),
+    "shm_server_submit": (i32, [vp, pvp, sz, pvp, C.POINTER(u64)]),
+    "shm_server_wait": (i32, [vp, u64, C.POINTER(dbl)]),
+    "shm_server_get_stats": (i32, [vp, vp]),
     "shm_evaluate_chebyshev": (i32, [vp, pvp, sz, C.POINTER(dbl), sz, dbl, dbl, vp, vp]),
     "shm_shared_a_keys_create": (i32, [vp, vp, u32, u32, u64, u32, pvp]),
     "shm_shared_a_keys_destroy": (i32, [vp]),

@@ -14,9 +14,16 @@
 // split), evaluated with baby-step / giant-step rotations.
 #pragma once
 
+#include <chrono>
 #include <complex>
+#include <condition_variable>
+#include <deque>
+#include <exception>
 #include <functional>
 #include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "engine.h"
@@ -171,6 +178,55 @@ void shared_coeff_to_slot(Bootstrapper &b, SharedAKeys &K, SharedABatch &x, Shar
 void bootstrap_shared_a_s2c_first(Bootstrapper &b, SharedAKeys &K, const std::vector<const Ciphertext *> &in,
                                   const std::vector<Ciphertext *> &out, const KSwitchKey &rk, double out_scale,
                                   hipStream_t s);
+
+// Request server with continuous batching of S2C-first bootstraps (server.hip).
+struct ServerStats {
+    uint64_t submitted, completed, failed, ciphertexts, groups;
+    size_t in_flight_bytes;
+    double p50_ms, p99_ms;  // request latency, submit to completion (completed requests)
+    double busy_s;          // wall time spent in grouped bootstraps
+};
+
+class BootServer {
+   public:
+    BootServer(Bootstrapper &b, const GaloisKeys &gk, const KSwitchKey &rk, size_t mem_budget, size_t max_group);
+    ~BootServer();  // finishes every submitted request, then stops the worker
+    BootServer(const BootServer &) = delete;
+    BootServer &operator=(const BootServer &) = delete;
+    // queue a request; out[i] receives the bootstrapped in[i] (4 primes in)
+    uint64_t submit(const std::vector<const Ciphertext *> &in, const std::vector<Ciphertext *> &out);
+    // block until the request has finished; rethrows its error
+    void wait(uint64_t id, double *latency_ms);
+    ServerStats stats();
+
+   private:
+    struct Request {
+        uint64_t id = 0;
+        std::vector<std::unique_ptr<Ciphertext>> in;
+        std::vector<Ciphertext *> out;
+        size_t next = 0, done = 0, bytes = 0;
+        bool finished = false;
+        double latency_ms = 0.0;
+        std::exception_ptr error;
+        std::chrono::steady_clock::time_point t_submit;
+    };
+    void loop();
+    void admit_locked();
+
+    Bootstrapper &b_;
+    const GaloisKeys &gk_;
+    const KSwitchKey &rk_;
+    size_t budget_, max_group_, ct_bytes_ = 0, in_flight_ = 0;
+    std::mutex mu_;
+    std::condition_variable cv_work_, cv_done_;
+    std::deque<std::shared_ptr<Request>> waiting_, active_;
+    std::map<uint64_t, std::shared_ptr<Request>> all_;
+    std::vector<double> lat_;
+    uint64_t next_id_ = 0, submitted_ = 0, completed_ = 0, failed_ = 0, cts_done_ = 0, groups_ = 0;
+    double busy_s_ = 0.0;
+    bool stop_ = false;
+    std::thread th_;
+};
 
 // Multiply by the constant `value` encoded at scale `sigma` and accumulate:
 // acc (+)= src * [round(value * sigma)]_q per limb over Ql limbs of src.

@@ -1027,6 +1027,70 @@ int shm_bootstrap_s2c_first(shm_bootstrapper *b, const shm_ciphertext *const *in
     });
 }
 
+// ---- bootstrapping server --------------------------------------------------------------------
+struct shm_server {
+    std::unique_ptr<boot::BootServer> s;
+};
+
+int shm_server_create(shm_bootstrapper *b, const shm_galois_keys *gk, const shm_kswitch_key *relin,
+                      uint64_t mem_budget_bytes, uint32_t max_group, shm_server **out) {
+    return guard([&] {
+        need(gk, "galois_keys");
+        need(relin, "relin_keys");
+        need(out, "out");
+        boot::Bootstrapper &bs = ready(b, nullptr);
+        auto h = std::make_unique<shm_server>();
+        h->s = std::make_unique<boot::BootServer>(bs, gk->g, relin->k, (size_t)mem_budget_bytes, max_group);
+        *out = h.release();
+    });
+}
+
+int shm_server_destroy(shm_server *srv) {
+    return guard([&] { delete srv; });
+}
+
+int shm_server_submit(shm_server *srv, const shm_ciphertext *const *in, size_t count, shm_ciphertext *const *out,
+                      uint64_t *request_id) {
+    return guard([&] {
+        need(srv, "server");
+        need(request_id, "request_id");
+        if (count == 0 || !in || !out) throw InvalidArgument("server: empty request");
+        std::vector<const Ciphertext *> ins(count);
+        std::vector<Ciphertext *> outs(count);
+        for (size_t i = 0; i < count; i++) {
+            need(in[i], "in[i]");
+            need(out[i], "out[i]");
+            ins[i] = &in[i]->ct;
+            outs[i] = &out[i]->ct;
+        }
+        *request_id = srv->s->submit(ins, outs);
+    });
+}
+
+int shm_server_wait(shm_server *srv, uint64_t request_id, double *latency_ms) {
+    return guard([&] {
+        need(srv, "server");
+        srv->s->wait(request_id, latency_ms);
+    });
+}
+
+int shm_server_get_stats(shm_server *srv, shm_server_stats *st) {
+    return guard([&] {
+        need(srv, "server");
+        need(st, "stats");
+        const boot::ServerStats v = srv->s->stats();
+        st->submitted = v.submitted;
+        st->completed = v.completed;
+        st->failed = v.failed;
+        st->ciphertexts = v.ciphertexts;
+        st->groups = v.groups;
+        st->in_flight_bytes = v.in_flight_bytes;
+        st->p50_ms = v.p50_ms;
+        st->p99_ms = v.p99_ms;
+        st->busy_s = v.busy_s;
+    });
+}
+
 // ---- shared-a batches ------------------------------------------------------------------------
 struct shm_shared_a_keys {
     std::unique_ptr<boot::SharedAKeys> k;
