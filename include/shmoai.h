@@ -72,6 +72,28 @@ const char *shm_version(void);
  * (e.g. ct_pt_matrix_mul_wo_pre, Ct_pt_matrix_mul.cuh:172). */
 int shm_synchronize(shm_stream_t stream);
 
+/* Streams and graph capture. The reference runs on phantom::util::cuda_stream_wrapper
+ * pool streams (Ct_pt_matrix_mul.cuh:66-106); shm_stream_create makes such a stream
+ * (default priority, non-blocking). A sequence of enqueue-only calls on a created
+ * stream can be recorded into a graph and replayed: run it once eagerly on that stream
+ * first (first uses build device tables with a synchronous upload and move buffers
+ * onto the stream), keep the ciphertexts it touches at their sizes (an output that
+ * would grow allocates), then shm_graph_capture_begin, the calls,
+ * shm_graph_capture_end, and shm_graph_launch as often as needed. Temporaries are
+ * allocated and freed inside the graph. Entry points documented as synchronising
+ * (matmul, bootstrap, encode/decode with host buffers, import/export) cannot be
+ * captured. A ciphertext or plaintext last used on a caller-owned stream returns its
+ * memory on that stream when destroyed: destroy it before that stream. Streams from
+ * shm_stream_create are kept alive by the library after shm_stream_destroy (drained and
+ * reused), so they are always safe. No counterpart in the reference for the graph calls. */
+int shm_stream_create(shm_stream_t *stream);
+int shm_stream_destroy(shm_stream_t stream);
+typedef struct shm_graph shm_graph;
+int shm_graph_capture_begin(shm_stream_t stream);
+int shm_graph_capture_end(shm_stream_t stream, shm_graph **graph);
+int shm_graph_launch(shm_graph *graph, shm_stream_t stream);
+int shm_graph_destroy(shm_graph *graph);
+
 /* ---- context --------------------------------------------------------------
  * Replaces EncryptionParameters + CoeffModulus::Create (modulus.cu:79-108) +
  * PhantomContext(parms) (context.cu:125-238). coeff_bit_sizes lists the data

@@ -616,6 +616,56 @@ def ct_pt_matrix_mul_wo_pre_w_mask(enc_X: Sequence[Ciphertext], W: np.ndarray, b
     return out
 
 
+class Stream:
+    """A created HIP stream (default priority, non-blocking); pass `.h` as `stream=`."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(_lib().shm_stream_create(C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(_lib().shm_stream_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """Calls recorded on a Stream between begin() and end(), replayed by launch()."""
+
+    def __init__(self, stream: Stream):
+        self.stream = stream
+        self.h = None
+
+    def begin(self):
+        check(_lib().shm_graph_capture_begin(self.stream.h))
+
+    def end(self):
+        h = C.c_void_p()
+        check(_lib().shm_graph_capture_end(self.stream.h, C.byref(h)))
+        self.h = h
+
+    def launch(self):
+        check(_lib().shm_graph_launch(self.h, self.stream.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(_lib().shm_graph_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def synchronize(stream=None) -> None:
     """Wait for all work queued on `stream` (None: the default stream)."""
     check(_lib().shm_synchronize(stream))

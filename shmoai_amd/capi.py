@@ -128,6 +128,12 @@ _SIGS = {
     "shm_bootstrap_eval_mod": (i32, [vp, vp, vp, vp]),
     "shm_bootstrap_eval_mod_batch": (i32, [vp, vp, sz, vp, vp]),
     "shm_bootstrap_s2c_first": (i32, [vp, pvp, sz, pvp, vp, vp, dbl, vp]),
+    "shm_stream_create": (i32, [pvp]),
+    "shm_stream_destroy": (i32, [vp]),
+    "shm_graph_capture_begin": (i32, [vp]),
+    "shm_graph_capture_end": (i32, [vp, pvp]),
+    "shm_graph_launch": (i32, [vp, vp]),
+    "shm_graph_destroy": (i32, [vp]),
     "shm_server_create": (i32, [vp, vp, vp, u64, u32, pvp]),
     "shm_server_destroy": (i32, [vp]),
     "shm_server_submit": (i32, [vp, pvp, sz, pvp, C.POINTER(u64)]),

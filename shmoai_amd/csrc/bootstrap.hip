@@ -677,8 +677,7 @@ void bootstrap_s2c_first_batch(Bootstrapper &b, const std::vector<const Cipherte
         G.scale = d_in;
         G.ntt = true;
         for (size_t k = 0; k < g; k++)
-            SHM_HIP(hipMemcpyAsync(G.poly(2 * k), in[i + k]->data(), 2 * Qin * N * sizeof(uint64_t),
-                                   hipMemcpyDeviceToDevice, s));
+            launch_copy_words(G.poly(2 * k), in[i + k]->data(), 2 * Qin * N, s);
         const Rotator rot = group_rotator(c, gk, s);
         slot_to_coeff(b, G, rot, s);
         ct_modraise(c, G, s);
@@ -695,8 +694,7 @@ void bootstrap_s2c_first_batch(Bootstrapper &b, const std::vector<const Cipherte
                 dst.resize(2, src.chain, s, false);  // the copy writes both polynomials
                 dst.scale = src.scale;
                 dst.ntt = true;
-                SHM_HIP(hipMemcpyAsync(dst.data(), src.poly(2 * k), 2 * Qc * N * sizeof(uint64_t),
-                                       hipMemcpyDeviceToDevice, s));
+                launch_copy_words(dst.data(), src.poly(2 * k), 2 * Qc * N, s);
             }
         }
         // amplitude correction (BatchBootstrap.cuh:444-460) folded into EvalMod:

@@ -1,6 +1,7 @@
 // C ABI (include/shmoai.h): thin, exception-safe wrappers over the engine.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -154,6 +155,81 @@ int shm_synchronize(shm_stream_t s) {
     return guard([&] { SHM_HIP(hipStreamSynchronize(S(s))); });
 }
 
+// Streams made here are never destroyed: a buffer that was last used on a stream
+// returns to the pool on that stream when it is freed, possibly after the caller
+// has released the stream, so released streams are kept (drained) and handed out
+// again by the next shm_stream_create.
+namespace {
+std::mutex g_stream_mu;
+std::vector<hipStream_t> g_stream_free;
+}  // namespace
+
+int shm_stream_create(shm_stream_t *out) {
+    return guard([&] {
+        need(out, "out");
+        {
+            std::lock_guard<std::mutex> lk(g_stream_mu);
+            if (!g_stream_free.empty()) {
+                *out = (shm_stream_t)g_stream_free.back();
+                g_stream_free.pop_back();
+                return;
+            }
+        }
+        hipStream_t st = nullptr;
+        // default priority; non-blocking, so it never waits on the legacy stream implicitly
+        SHM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        *out = (shm_stream_t)st;
+    });
+}
+
+int shm_stream_destroy(shm_stream_t st) {
+    return guard([&] {
+        if (!st) return;
+        SHM_HIP(hipStreamSynchronize(S(st)));
+        std::lock_guard<std::mutex> lk(g_stream_mu);
+        g_stream_free.push_back(S(st));
+    });
+}
+
+struct shm_graph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+};
+
+int shm_graph_capture_begin(shm_stream_t st) {
+    return guard([&] {
+        if (!st) throw InvalidArgument("graph capture needs a created stream (not the default stream)");
+        SHM_HIP(hipStreamBeginCapture(S(st), hipStreamCaptureModeThreadLocal));
+    });
+}
+
+int shm_graph_capture_end(shm_stream_t st, shm_graph **out) {
+    return guard([&] {
+        need(out, "out");
+        if (!st) throw InvalidArgument("graph capture needs a created stream (not the default stream)");
+        auto h = std::make_unique<shm_graph>();
+        SHM_HIP(hipStreamEndCapture(S(st), &h->g));
+        SHM_HIP(hipGraphInstantiate(&h->x, h->g, nullptr, nullptr, 0));
+        *out = h.release();
+    });
+}
+
+int shm_graph_launch(shm_graph *g, shm_stream_t st) {
+    return guard([&] {
+        need(g, "graph");
+        SHM_HIP(hipGraphLaunch(g->x, S(st)));
+    });
+}
+
+int shm_graph_destroy(shm_graph *g) {
+    return guard([&] {
+        if (!g) return;
+        if (g->x) SHM_HIP(hipGraphExecDestroy(g->x));
+        if (g->g) SHM_HIP(hipGraphDestroy(g->g));
+        delete g;
+    });
+}
+
 int shm_ntt_inverse(shm_context *ctx, uint64_t *data, uint32_t n_polys, uint32_t n_limbs, uint32_t first,
                     shm_stream_t s) {
     return guard([&] {
@@ -238,11 +314,10 @@ int shm_ciphertext_copy(const shm_ciphertext *src, shm_ciphertext *dst, shm_stre
         need(src, "src");
         need(dst, "dst");
         dst->ct.ctx = src->ct.ctx;
-        dst->ct.resize(src->ct.size, src->ct.chain, S(s));
+        dst->ct.resize(src->ct.size, src->ct.chain, S(s), false);  // the copy writes every word
         dst->ct.scale = src->ct.scale;
         dst->ct.ntt = src->ct.ntt;
-        SHM_HIP(hipMemcpyAsync(dst->ct.data(), src->ct.data(), src->ct.size * src->ct.Ql() * src->ct.ctx->N * 8,
-                               hipMemcpyDeviceToDevice, S(s)));
+        launch_copy_words(dst->ct.data(), src->ct.data(), src->ct.size * src->ct.Ql() * src->ct.ctx->N, S(s));
     });
 }
 
@@ -629,7 +704,7 @@ int shm_multiply_inplace(shm_context *ctx, shm_ciphertext *a, const shm_cipherte
         else
             launch_tensor(c, a->ct.data(), b->ct.data(), out.as<uint64_t>(), Ql, s);
         a->ct.resize(3, a->ct.chain, s);
-        SHM_HIP(hipMemcpyAsync(a->ct.data(), out.as<uint64_t>(), S3 * 8, hipMemcpyDeviceToDevice, s));
+        launch_copy_words(a->ct.data(), out.as<uint64_t>(), S3, s);
         a->ct.scale = a->ct.scale * b->ct.scale;
     });
 }
@@ -661,7 +736,7 @@ int shm_apply_kswitch_key_inplace(shm_context *ctx, shm_ciphertext *ct, const sh
         const size_t Ql = ct->ct.Ql();
         const hipStream_t s = S(st);
         Scratch t(Ql * c.N * sizeof(uint64_t), s);
-        SHM_HIP(hipMemcpyAsync(t.as<uint64_t>(), ct->ct.poly(1), Ql * c.N * 8, hipMemcpyDeviceToDevice, s));
+        launch_copy_words(t.as<uint64_t>(), ct->ct.poly(1), Ql * c.N, s);
         keyswitch(c, t.as<uint64_t>(), key->k.k.as<uint64_t>(), Ql, ct->ct.poly(0), ct->ct.poly(1), false, s);
     });
 }
@@ -1239,8 +1314,8 @@ int shm_shared_a_batch_member(const shm_shared_a_batch *b, uint32_t i, shm_ciphe
         out->ct.resize(2, ct.chain, S(st));
         out->ct.scale = ct.scale;
         out->ct.ntt = true;
-        SHM_HIP(hipMemcpyAsync(out->ct.poly(0), ct.poly(i), w * 8, hipMemcpyDeviceToDevice, S(st)));
-        SHM_HIP(hipMemcpyAsync(out->ct.poly(1), ct.poly(b->b.n), w * 8, hipMemcpyDeviceToDevice, S(st)));
+        launch_copy_words(out->ct.poly(0), ct.poly(i), w, S(st));
+        launch_copy_words(out->ct.poly(1), ct.poly(b->b.n), w, S(st));
     });
 }
 

@@ -513,7 +513,7 @@ void Ciphertext::resize(size_t new_size, size_t new_chain, hipStream_t s, bool z
         const size_t bytes = new_size * nl * ctx->N * sizeof(uint64_t);
         DevBuf nb;
         nb.alloc_async(bytes, s);
-        if (zero) SHM_HIP(hipMemsetAsync(nb.as<void>(), 0, bytes, s));
+        if (zero) launch_fill_zero(nb.as<uint64_t>(), bytes / sizeof(uint64_t), s);
         buf.use_on(s);
         buf = std::move(nb);  // the old block returns to the pool after the work queued on s
         cap_size = new_size;

@@ -251,7 +251,7 @@ void encode(const Context &c, const double *re, const double *im, size_t n_value
                                                         c.enc_group.as<uint32_t>(), (uint32_t)n, logn, logPairs,
                                                         logRatio, (uint32_t)M, fix, last);
     }
-    SHM_HIP(hipMemsetAsync(out, 0, Ql * N * sizeof(uint64_t), s));
+    launch_fill_zero(out, Ql * N, s);
     k_decompose<<<grid_for(Ql * 2 * n), kBlock, 0, s>>>(buf.as<double2>(), out, c.mods(), (uint32_t)n, N / (2 * n), Ql,
                                                         N);
     SHM_HIP(hipGetLastError());
@@ -345,10 +345,10 @@ void decode(const Context &c, const uint64_t *pt, size_t Ql, double scale, doubl
     SHM_HIP(hipMemcpyAsync(dhalf, half.data(), W * 8, hipMemcpyHostToDevice, s));
 
     Scratch x(Ql * N * sizeof(uint64_t), s);
-    SHM_HIP(hipMemcpyAsync(x.as<uint64_t>(), pt, Ql * N * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    launch_copy_words(x.as<uint64_t>(), pt, Ql * N, s);
     ntt_inverse(c, x.as<uint64_t>(), 1, Ql, 0, s);
     Scratch buf(n * sizeof(double2), s);
-    SHM_HIP(hipMemsetAsync(buf.as<void>(), 0, n * sizeof(double2), s));
+    launch_fill_zero(buf.as<uint64_t>(), 2 * n, s);  // all-zero bits are +0.0
     k_compose<<<grid_for(2 * n), 64, 0, s>>>(x.as<uint64_t>(), buf.as<double2>(), c.mods(), dqhat, dqhinv, dqhinv_s,
                                             dQ, dhalf, Ql, N, (uint32_t)n, N / (2 * n), 1.0 / scale);
     for (int iter = 0; iter < logn; iter++) {

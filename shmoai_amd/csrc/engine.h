@@ -264,6 +264,10 @@ void launch_permute(const Context &c, const uint64_t *in, uint64_t *out, const u
 // out[p][i] = in.p[p][i], i < words, p < npolys
 void launch_gather_polys(const Context &c, const PolyPack &in, uint64_t *out, size_t words, size_t npolys,
                          hipStream_t s);
+// zero fill by a kernel (a memset node recorded into a captured graph is avoided)
+void launch_fill_zero(uint64_t *out, size_t words, hipStream_t s);
+// device-to-device copy by a kernel (non-overlapping ranges)
+void launch_copy_words(uint64_t *dst, const uint64_t *src, size_t words, hipStream_t s);
 void launch_copy_limbs(const Context &c, const uint64_t *in, size_t in_limbs, uint64_t *out, size_t out_limbs,
                        size_t npolys, hipStream_t s);
 // samplers

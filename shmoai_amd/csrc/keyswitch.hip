@@ -272,7 +272,7 @@ void ks_modup(const Context &c, const uint64_t *c2, size_t Ql, uint64_t *d, hipS
     const LevelConsts &L = c.level(Ql);
     const size_t beta = L.beta;
     Scratch tcoef(Ql * N * sizeof(uint64_t), s);
-    SHM_HIP(hipMemcpyAsync(tcoef.as<uint64_t>(), c2, Ql * N * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    launch_copy_words(tcoef.as<uint64_t>(), c2, Ql * N, s);
     ntt_inverse(c, tcoef.as<uint64_t>(), 1, Ql, 0, s);
     if (alpha > (size_t)kMaxAlpha) throw InvalidArgument("at most 16 special primes");
     k_modup_bconv<<<grid_for(beta * N), kBlock, 0, s>>>(

@@ -66,7 +66,7 @@ void apply_galois(Context &c, Ciphertext &ct, uint32_t elt, const GaloisKeys &gk
     Scratch t0(Ql * N * sizeof(uint64_t), s), t1(Ql * N * sizeof(uint64_t), s);
     launch_permute(c, ct.poly(0), t0.as<uint64_t>(), table, Ql, s);
     launch_permute(c, ct.poly(1), t1.as<uint64_t>(), table, Ql, s);
-    SHM_HIP(hipMemcpyAsync(ct.poly(0), t0.as<uint64_t>(), Ql * N * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    launch_copy_words(ct.poly(0), t0.as<uint64_t>(), Ql * N, s);
     keyswitch(c, t1.as<uint64_t>(), it->second->k.as<uint64_t>(), Ql, ct.poly(0), ct.poly(1), false, s, rounded);
 }
 
@@ -115,8 +115,7 @@ void apply_galois_group(Context &c, Ciphertext &g, uint32_t elt, const GaloisKey
     Scratch t(g.size * Ql * N * sizeof(uint64_t), s);
     launch_permute(c, g.data(), t.as<uint64_t>(), table, g.size * Ql, s);
     // b_i := sigma(b_i); a_i is overwritten by the switched part below
-    SHM_HIP(hipMemcpyAsync(g.data(), t.as<uint64_t>(), g.size * Ql * N * sizeof(uint64_t), hipMemcpyDeviceToDevice,
-                           s));
+    launch_copy_words(g.data(), t.as<uint64_t>(), g.size * Ql * N, s);
     PolyPack c2{}, ct01{};
     for (size_t i = 0; i < m; i++) {
         c2.p[i] = t.as<uint64_t>() + (2 * i + 1) * Ql * N;
@@ -242,7 +241,7 @@ void hoisting_inplace(Context &c, Ciphertext &ct, const std::vector<int> &steps,
         launch_permute(c, ct.poly(0), i ? t0.as<uint64_t>() : c0.as<uint64_t>(), table, Ql, s);
         if (i) launch_add(c, c0.as<uint64_t>(), t0.as<uint64_t>(), Ql, 1, s);
     }
-    SHM_HIP(hipMemcpyAsync(ct.poly(0), c0.as<uint64_t>(), Ql * N * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    launch_copy_words(ct.poly(0), c0.as<uint64_t>(), Ql * N, s);
     ks_moddown(c, cx.as<uint64_t>(), 2, Ql, ct.data(), Ql * N, 1, s);
 }
 
@@ -263,8 +262,7 @@ void mod_switch_drop(Context &c, Ciphertext &ct, size_t new_chain, hipStream_t s
     if (ct.size > 1) {
         Scratch tmp(ct.size * nl * c.N * sizeof(uint64_t), s);
         launch_copy_limbs(c, ct.data(), Ql, tmp.as<uint64_t>(), nl, ct.size, s);
-        SHM_HIP(hipMemcpyAsync(ct.data(), tmp.as<uint64_t>(), ct.size * nl * c.N * sizeof(uint64_t),
-                               hipMemcpyDeviceToDevice, s));
+        launch_copy_words(ct.data(), tmp.as<uint64_t>(), ct.size * nl * c.N, s);
     }
     ct.chain = new_chain;
 }
@@ -316,8 +314,7 @@ void ct_clone(Ciphertext &dst, const Ciphertext &src, hipStream_t s) {
     dst.resize(src.size, src.chain, s, false);  // the copy writes every word
     dst.scale = src.scale;
     dst.ntt = src.ntt;
-    SHM_HIP(hipMemcpyAsync(dst.data(), src.data(), src.size * src.Ql() * src.ctx->N * sizeof(uint64_t),
-                           hipMemcpyDeviceToDevice, s));
+    launch_copy_words(dst.data(), src.data(), src.size * src.Ql() * src.ctx->N, s);
 }
 
 void ct_add(Context &c, Ciphertext &a, const Ciphertext &b, hipStream_t s) {
@@ -506,7 +503,7 @@ void ct_modraise(Context &c, Ciphertext &a, hipStream_t s) {
     if (a.Ql() != 1) throw InvalidArgument("Ciphertexts in the lowest level are supported only!");
     const size_t N = c.N, nl = c.nQ, np = a.size;
     Scratch low(np * N * sizeof(uint64_t), s);
-    SHM_HIP(hipMemcpyAsync(low.as<uint64_t>(), a.data(), np * N * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    launch_copy_words(low.as<uint64_t>(), a.data(), np * N, s);
     ntt_inverse(c, low.as<uint64_t>(), np, 1, 0, s);
     Ciphertext out;
     out.ctx = &c;

@@ -112,6 +112,11 @@ __global__ void k_copy_limbs(const uint64_t *in, size_t in_limbs, uint64_t *out,
     }
 }
 
+__global__ void k_fill_zero(uint64_t *out, size_t words) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < words; t += (size_t)gridDim.x * blockDim.x)
+        out[t] = 0;
+}
+
 // ---- samplers (SplitMix64 counters; identical streams on host and device) --
 __device__ __forceinline__ uint64_t splitmix64(uint64_t key, uint64_t ctr) {
     uint64_t z = key + (ctr + 1) * 0x9E3779B97F4A7C15ULL;
@@ -260,8 +265,7 @@ void launch_rescale(const Context &c, const uint64_t *in, uint64_t *out, size_t 
     rescale_ntt_fused(c, last.as<uint64_t>(), in, Ql, delta.as<uint64_t>(), nullptr, dst, npolys, npolys, L1,
                       L.qlast_inv.as<uint64_t>(), L.qlast_inv_s.as<uint64_t>(), s, rounded);
     if (in == out)
-        SHM_HIP(hipMemcpyAsync(out, delta.as<uint64_t>(), npolys * L1 * N * sizeof(uint64_t), hipMemcpyDeviceToDevice,
-                               s));
+        launch_copy_words(out, delta.as<uint64_t>(), npolys * L1 * N, s);
     SHM_HIP(hipGetLastError());
 }
 
@@ -282,6 +286,34 @@ void launch_permute(const Context &c, const uint64_t *in, uint64_t *out, const u
 void launch_copy_limbs(const Context &c, const uint64_t *in, size_t in_limbs, uint64_t *out, size_t out_limbs,
                        size_t npolys, hipStream_t s) {
     k_copy_limbs<<<grid_for(npolys * out_limbs * c.N), kBlock, 0, s>>>(in, in_limbs, out, out_limbs, c.N, npolys);
+    SHM_HIP(hipGetLastError());
+}
+
+// Device-to-device copy as a kernel (16-byte accesses when both ends allow):
+// it orders like every other launch on the stream, also inside a captured graph.
+__global__ void k_copy_words(uint64_t *__restrict__ dst, const uint64_t *__restrict__ src, size_t words, int vec) {
+    typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (vec) {
+        const size_t pairs = words / 2;
+        for (size_t i = t; i < pairs; i += stride) ((u64x2 *)dst)[i] = ((const u64x2 *)src)[i];
+        if (t == 0 && (words & 1)) dst[words - 1] = src[words - 1];
+    } else {
+        for (size_t i = t; i < words; i += stride) dst[i] = src[i];
+    }
+}
+
+void launch_copy_words(uint64_t *dst, const uint64_t *src, size_t words, hipStream_t s) {
+    if (words == 0 || dst == src) return;
+    const int vec = (((uintptr_t)dst | (uintptr_t)src) & 15) == 0 ? 1 : 0;
+    k_copy_words<<<grid_for(vec ? (words + 1) / 2 : words), kBlock, 0, s>>>(dst, src, words, vec);
+    SHM_HIP(hipGetLastError());
+}
+
+void launch_fill_zero(uint64_t *out, size_t words, hipStream_t s) {
+    if (words == 0) return;
+    k_fill_zero<<<grid_for(words), kBlock, 0, s>>>(out, words);
     SHM_HIP(hipGetLastError());
 }
 

@@ -232,10 +232,8 @@ void fmt_switch_forward(SharedAKeys &K, const std::vector<const Ciphertext *> &c
     // 1. exact extension of every a_j to QlP (own limbs stay in NTT form)
     Scratch ext(n * QlP * N * sizeof(uint64_t), s), coef(n * Ql * N * sizeof(uint64_t), s);
     for (size_t j = 0; j < n; j++) {
-        SHM_HIP(hipMemcpyAsync(ext.as<uint64_t>() + j * QlP * N, cts[j]->poly(1), Ql * N * sizeof(uint64_t),
-                               hipMemcpyDeviceToDevice, s));
-        SHM_HIP(hipMemcpyAsync(coef.as<uint64_t>() + j * Ql * N, cts[j]->poly(1), Ql * N * sizeof(uint64_t),
-                               hipMemcpyDeviceToDevice, s));
+        launch_copy_words(ext.as<uint64_t>() + j * QlP * N, cts[j]->poly(1), Ql * N, s);
+        launch_copy_words(coef.as<uint64_t>() + j * Ql * N, cts[j]->poly(1), Ql * N, s);
     }
     ntt_inverse(c, coef.as<uint64_t>(), n, Ql, 0, s);
     k_extend_exact<<<grid_for(n * N), kBlock, 0, s>>>(
@@ -261,8 +259,7 @@ void fmt_switch_forward(SharedAKeys &K, const std::vector<const Ciphertext *> &c
     out.ct.ntt = true;
     out.ct.scale = cts[0]->scale;
     for (size_t i = 0; i < n; i++)
-        SHM_HIP(hipMemcpyAsync(out.ct.poly(i), cts[i]->poly(0), Ql * N * sizeof(uint64_t), hipMemcpyDeviceToDevice,
-                               s));
+        launch_copy_words(out.ct.poly(i), cts[i]->poly(0), Ql * N, s);
     const uint64_t mask = (n >= 64) ? ~0ull : ((1ull << n) - 1);
     ks_moddown(c, cx.as<uint64_t>(), n + 1, Ql, out.ct.data(), Ql * N, mask, s, rounded);
 }
@@ -297,7 +294,7 @@ void fmt_switch_backward(SharedAKeys &K, const SharedABatch &in, const std::vect
         o.resize(2, in.ct.chain, s);
         o.ntt = true;
         o.scale = in.ct.scale;
-        SHM_HIP(hipMemcpyAsync(o.poly(0), in.ct.poly(i), Ql * N * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+        launch_copy_words(o.poly(0), in.ct.poly(i), Ql * N, s);
         ks_moddown(c, cx.as<uint64_t>() + 2 * i * QlP * N, 2, Ql, o.data(), Ql * N, 1, s, rounded);
     }
 }
