@@ -332,6 +332,21 @@ class Ref:
                                np.ascontiguousarray(scales, np.float64), col_begin, col_end, op)
         return outs
 
+    def ctpt_matmul_masked(self, X, W, mask, Ql, scales):
+        """ct_pt_matrix_mul_wo_pre_w_mask (Ct_pt_matrix_mul.cuh:846-985): for each output
+        i, sum over j of X[j] (.) encode({mask[t] == 1 ? W[j][i] : 0}, scale_j), each
+        product and sum reduced as multiply_plain and add_inplace do, then rescale."""
+        mask = np.asarray(mask)
+        outs = []
+        for i in range(W.shape[1]):
+            acc = None
+            for j in range(W.shape[0]):
+                pt = self.encode(np.where(mask == 1, W[j, i], 0.0), Ql, scales[j])
+                t = self.multiply_plain(X[j], pt, Ql)
+                acc = t if acc is None else self.add(acc, t, Ql, 2)
+            outs.append(self.rescale(acc, Ql))
+        return outs
+
     # bootstrapping pieces -------------------------------------------------------------
     def modraise(self, ct_low):
         """[2][N] at prime 0 (NTT form) -> [2][nQ][N] (Bootstrapper.cu:3287-3356)."""

@@ -118,18 +118,6 @@ def test_matmul_known_answer():
             assert np.max(np.abs(v - expect)) < 1e-3
 
 
-def _masked_oracle(ref, xs, W, mask, Ql, scale):
-    outs = []
-    for i in range(W.shape[1]):
-        acc = None
-        for j in range(W.shape[0]):
-            pt = ref.encode(np.where(mask == 1, W[j, i], 0.0), Ql, scale)
-            t = ref.multiply_plain(xs[j], pt, Ql)
-            acc = t if acc is None else ref.add(acc, t, Ql, 2)
-        outs.append(ref.rescale(acc, Ql))
-    return outs
-
-
 def test_matmul_masked_bitexact(oracle_lib):
     bits = [50] * 4 + [60]
     ctx = shm.Context(13, bits, 1)
@@ -141,7 +129,7 @@ def test_matmul_masked_bitexact(oracle_lib):
     mask = (rng.random(ctx.N // 2) < 0.6).astype(np.int32)
     mask[:7] = [1, 0, 2, 1, -1, 1, 0]  # only entries equal to 1 select (bias_vec[j] == 1)
     out = shm.ct_pt_matrix_mul_wo_pre_w_mask(hs, W, mask, row_w, col_w, row_w, ctx)
-    exp = _masked_oracle(ref, xs, W, mask, Ql, 2.0 ** 40)
+    exp = ref.ctpt_matmul_masked(xs, W, mask, Ql, np.full(row_w, 2.0 ** 40))
     for i in range(col_w):
         assert out[i].chain_index == ctx.chain(Ql - 1)
         assert out[i].scale == 2.0 ** 40

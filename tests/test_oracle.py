@@ -283,6 +283,25 @@ def test_matmul_known_answer_oracle(oracle_lib):
         assert np.max(np.abs(v - slot * row_w / 128.0)) < 1e-3
 
 
+def test_masked_matmul_oracle_decrypts_to_masked_sums(oracle_lib):
+    """The masked form (Ct_pt_matrix_mul.cuh:846-985): output i decrypts to
+    sum_j W[j][i] x_j in the slots where bias_vec == 1 and to 0 elsewhere."""
+    r = oracle_lib.Ref(11, [60, 40, 40, 60], 1)
+    s = r.sk_ntt(r.sample_ternary(41, 64))
+    rng = np.random.default_rng(42)
+    n, row_w, col_w, Ql = r.N // 2, 5, 3, 3
+    xs = [rng.uniform(-1, 1, n) for _ in range(row_w)]
+    cts = [r.encrypt_symmetric(s, r.encode(x, Ql, 2.0 ** 40), Ql, 50 + i) for i, x in enumerate(xs)]
+    W = rng.normal(0, 0.3, (row_w, col_w))
+    mask = (rng.random(n) < 0.5).astype(np.int32)
+    mask[:4] = [1, 0, 2, -1]  # only entries equal to 1 select
+    out = r.ctpt_matmul_masked(cts, W, mask, Ql, np.full(row_w, 2.0 ** 40))
+    for i, o in enumerate(out):
+        v = r.decode(r.decrypt(s, o, 2, Ql - 1), Ql - 1, 2.0 ** 40)[0]
+        want = np.where(mask == 1, sum(W[j, i] * xs[j] for j in range(row_w)), 0.0)
+        assert np.max(np.abs(v - want)) < 1e-4
+
+
 def test_golden_fixtures(oracle_lib):
     """Committed vectors written by tests/golden/make_fixtures.py (pure Python)."""
     with open(os.path.join(GOLDEN, "fixtures.json")) as f:
