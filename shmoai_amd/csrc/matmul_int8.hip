@@ -153,7 +153,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DX * KS <= 
     const size_t col = tile * 64 + 16 * wave + (lane & 15);
     const size_t off = lp * N + col;
     const uint64_t q = mods[l].q, mu = mu_arr[l];
+    // row pointers staged in LDS once per block; rows k >= row_w point at row 0
+    // so that every load is unconditional (their words are zeroed below)
+    __shared__ const uint64_t *xl[KS * 64];
     for (int t = threadIdx.x; t < min(col_w, kCorrCache); t += 256) cl[t] = corr[l * col_w + t];
+    for (int t = threadIdx.x; t < KS * 64; t += 256) xl[t] = X[t < row_w ? t : 0];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 
     // B fragments: lane holds X'_a[k = 64 ks + 16 (lane/16) + e][col] for
     // e = 0..15, built by byte transposes of the little-endian limb words.
@@ -164,7 +171,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DX * KS <= 
 #pragma unroll
         for (int e = 0; e < 16; e++) {
             const int k = 64 * ks + 16 * (lane >> 4) + e;
-            x[e] = k < row_w ? X[k][off] : 0;
+            const uint64_t v = xl[k][off];
+            x[e] = k < row_w ? v : 0;
         }
 #pragma unroll
         for (int g = 0; g < 4; g++) {
