@@ -30,7 +30,6 @@ typedef int8_t i8x16 __attribute__((ext_vector_type(16)));
 typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kBlock = 256;
-constexpr int kCorrCache = 256;  // output rows whose correction is cached in LDS
 
 inline unsigned grid_for(size_t work) {
     size_t g = (work + kBlock - 1) / kBlock;
@@ -138,14 +137,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DX * KS <= 
                                                        const uint64_t *__restrict__ mu_arr,
                                                        uint64_t *__restrict__ acc_out,
                                                        const DevMod *__restrict__ mods, int row_w, int col_w,
-                                                       int MB, size_t Ql, size_t N, int accumulate) {
+                                                       int MB, size_t Ql, size_t N, int accumulate, int radix) {
     constexpr int ND = DX + DW - 1;
     // |C_d| < 2^24.2, so |S| < 2^104.3 and 0 < S + 2^105 < 2^106 needs ND <= 11.
     static_assert(ND >= 10 && ND <= 11, "plane recombination assumes 10 or 11 planes");
     static_assert(DX <= 8, "at most 8 digits per limb");
     constexpr int FR = DW * RB * KS * 64;  // fragments staged per row phase
     __shared__ i8x16 wl[FR];
-    __shared__ uint64_t cl[kCorrCache];  // row-sum corrections of this limb
+    __shared__ uint64_t cl[RB * 16];  // row-sum corrections of this limb for the phase's rows
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const size_t tiles = N / 64;
     const size_t tile = blockIdx.x % tiles, lp = blockIdx.x / tiles;  // lp = p * Ql + l
@@ -156,7 +155,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DX * KS <= 
     // row pointers staged in LDS once per block; rows k >= row_w point at row 0
     // so that every load is unconditional (their words are zeroed below)
     __shared__ const uint64_t *xl[KS * 64];
-    for (int t = threadIdx.x; t < min(col_w, kCorrCache); t += 256) cl[t] = corr[l * col_w + t];
     for (int t = threadIdx.x; t < KS * 64; t += 256) xl[t] = X[t < row_w ? t : 0];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
     __builtin_amdgcn_s_barrier();
@@ -193,11 +191,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DX * KS <= 
         __builtin_amdgcn_s_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
     };
+    // 2^{8e} from the runtime radix (256): kept out of constant folding so that each
+    // term is one signed 32 x 32 + 64 multiply-add instead of shifts and moves
+    int32_t pw[4];
+    pw[0] = 1;
+#pragma unroll
+    for (int e = 1; e < 4; e++) pw[e] = pw[e - 1] * radix;
     const int nph = (MB + RB - 1) / RB;
     for (int ph = 0; ph < nph; ph++) {
         lds_barrier();
         const i8x16 *src = frag + (size_t)ph * FR;
         for (int t = threadIdx.x; t < FR; t += 256) wl[t] = src[t];
+        for (int t = threadIdx.x; t < RB * 16; t += 256) {
+            const int row = 16 * RB * ph + t;
+            cl[t] = row < col_w ? corr[l * col_w + row] : 0;
+        }
         lds_barrier();
 #pragma unroll 1
         for (int ibl = 0; ibl < RB; ibl++) {
@@ -221,21 +229,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DX * KS <= 
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = 16 * ib + 4 * (lane >> 4) + r;
-                // S = sum_d C_d 256^d as two int64 Horner halves plus the top planes
-                int64_t lv = C[4][r];
+                // S = sum_d C_d 256^d as three signed 64-bit sums of 32 x 32 products,
+                // g_k = sum_{e<4} C_{4k+e} 2^{8e}, S = g0 + g1 2^32 + g2 2^64
+                int64_t g0 = 0, g1 = 0, g2 = 0;
 #pragma unroll
-                for (int d = 3; d >= 0; d--) lv = lv * 256 + C[d][r];
-                int64_t h1 = C[9][r];
-#pragma unroll
-                for (int d = 8; d >= 5; d--) h1 = h1 * 256 + C[d][r];
-                int64_t h2 = 0;
-#pragma unroll
-                for (int d = ND - 1; d >= 10; d--) h2 = h2 * 256 + C[d][r];
-                // T = S + 2^105 = lv + h1 2^40 + h2 2^80 + 2^105 (0 < T < 2^106)
-                const uint64_t a_lo = (uint64_t)h1 << 40;
-                const uint64_t lo = (uint64_t)lv + a_lo;
-                const uint64_t hi = (uint64_t)(lv >> 63) + (uint64_t)(h1 >> 24) + (lo < a_lo ? 1 : 0) +
-                                    ((uint64_t)h2 << 16) + (uint64_t(1) << 41);
+                for (int d = 0; d < ND; d++) {
+                    const int64_t t = (int64_t)C[d][r] * (int64_t)pw[d & 3];
+                    if (d < 4) g0 += t; else if (d < 8) g1 += t; else g2 += t;
+                }
+                // T = S + 2^105 (0 < T < 2^106)
+                const uint64_t a_lo = (uint64_t)g1 << 32;
+                const uint64_t lo = (uint64_t)g0 + a_lo;
+                const uint64_t hi = (uint64_t)(g0 >> 63) + (uint64_t)(g1 >> 32) + (lo < a_lo ? 1 : 0) + (uint64_t)g2 +
+                                    (uint64_t(1) << 41);
                 // quotient from the top 62 bits: T' = T >> 44, qe = floor(T' mu / 2^64)
                 const uint64_t tp = (hi << 20) | (lo >> 44);
                 const uint64_t qe = __umul64hi(tp, mu);
@@ -243,7 +249,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DX * KS <= 
                 res = res >= 2 * q ? res - 2 * q : res;
                 res = res >= q ? res - q : res;
                 if (row < col_w) {
-                    res = add_mod(res, row < kCorrCache ? cl[row] : corr[l * col_w + row], q);
+                    res = add_mod(res, cl[row - 16 * RB * ph], q);
                     uint64_t *dst = &acc_out[(size_t)row * 2 * Ql * N + off];
                     // a later row chunk adds to the sum of the earlier ones
                     if (accumulate) res = add_mod(res, __builtin_nontemporal_load(dst), q);
@@ -256,7 +262,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DX * KS <= 
 }
 
 typedef void (*ContractFn)(const uint64_t *const *, const i8x16 *, const uint64_t *, const uint64_t *, uint64_t *,
-                           const DevMod *, int, int, int, size_t, size_t, int);
+                           const DevMod *, int, int, int, size_t, size_t, int, int);
 
 constexpr int rb_for(int KS) { return KS == 1 ? 4 : KS == 2 ? 4 : KS == 3 ? 3 : 2; }
 
@@ -352,7 +358,7 @@ void ctpt_matmul_int8(const Context &c, const std::vector<const Ciphertext *> &X
         if (ch == 0) SHM_HIP(hipEventRecord(ev0, s));
         hipLaunchKernelGGL(fn, dim3(blocks), dim3(256), 0, s, xptr.as<const uint64_t *>() + k0, frag.as<i8x16>(),
                            corr.as<uint64_t>(), mu.as<uint64_t>(), acc, c.mods(), (int)kc, (int)col_w, MB, Ql, N,
-                           ch > 0 ? 1 : 0);
+                           ch > 0 ? 1 : 0, 256);
         SHM_HIP(hipGetLastError());
         k0 += kc;
     }
