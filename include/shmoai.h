@@ -235,6 +235,8 @@ int shm_ct_pt_matrix_mul_masked(shm_context *ctx, const shm_ciphertext *const *e
                                 shm_ciphertext *const *out, shm_stream_t stream);
 /* duration of the dominant kernel of the last matmul on this process, ms (HIP events) */
 double shm_last_matmul_kernel_ms(void);
+/* duration of the tile pass of the last batched matmul's fused rescale NTT, ms (HIP events) */
+double shm_last_matmul_rescale_tile_ms(void);
 /* variant the last matmul actually ran: SHM_MATMUL_INT8 falls back to SHM_MATMUL_U64
  * when a prime is outside (2^46, 2^61] or a weight needs more than 5 balanced
  * digits (4 for 8-digit limbs). Any row_W is accepted: rows beyond 256 are

@@ -675,6 +675,10 @@ def last_matmul_kernel_ms() -> float:
     return float(_lib().shm_last_matmul_kernel_ms())
 
 
+def last_matmul_rescale_tile_ms() -> float:
+    return float(_lib().shm_last_matmul_rescale_tile_ms())
+
+
 def last_matmul_variant() -> int:
     return int(_lib().shm_last_matmul_variant())
 

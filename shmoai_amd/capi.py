@@ -112,6 +112,7 @@ _SIGS = {
     "shm_ct_pt_matrix_mul": (i32, [vp, pvp, u32, C.POINTER(dbl), u32, pvp, i32, vp]),
     "shm_ct_pt_matrix_mul_masked": (i32, [vp, pvp, u32, C.POINTER(dbl), u32, C.POINTER(C.c_int32), sz, pvp, vp]),
     "shm_last_matmul_kernel_ms": (dbl, []),
+    "shm_last_matmul_rescale_tile_ms": (dbl, []),
     "shm_last_matmul_variant": (i32, []),
     "shm_bootstrapper_create": (i32, [vp, C.POINTER(BootParams), pvp]),
     "shm_bootstrapper_create_plan": (i32, [u32, u32, C.POINTER(BootParams), pvp]),

@@ -871,6 +871,7 @@ int shm_ct_pt_matrix_mul_masked(shm_context *ctx, const shm_ciphertext *const *X
 }
 
 double shm_last_matmul_kernel_ms(void) { return (double)last_matmul_kernel_ms(); }
+double shm_last_matmul_rescale_tile_ms(void) { return (double)last_matmul_rescale_tile_ms(); }
 int shm_last_matmul_variant(void) { return last_matmul_variant(); }
 
 }  // extern "C"

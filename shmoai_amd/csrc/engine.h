@@ -215,6 +215,9 @@ struct GaloisKeys {
 // (device array) or by a contiguous range starting at first_prime.
 void ntt_forward(const Context &c, uint64_t *data, size_t n_polys, size_t n_limbs, size_t first_prime,
                  hipStream_t s);
+// Events recorded around the tile pass of every fused-rescale NTT launched while
+// set (nullptr to stop); used to time the matmul's rescale.
+void set_rescale_tile_events(hipEvent_t start, hipEvent_t stop);
 // Encoder decompose + forward NTT in one: out[p][k] = NTT of the rounded FFT
 // output fft[p] (N/2 complex entries: real parts are coefficients 0..N/2-1,
 // imaginary parts N/2..N-1) reduced mod prime k, for k < n_limbs.
@@ -340,6 +343,7 @@ void decode(const Context &c, const uint64_t *pt, size_t Ql, double scale, doubl
 // MOAI matmul (matmul.hip)
 void ctpt_matmul(const Context &c, const std::vector<const Ciphertext *> &X, const double *W, size_t row_w,
                  size_t col_w, const std::vector<Ciphertext *> &out, int variant, hipStream_t s);
+float last_matmul_rescale_tile_ms(); // tile pass of the last batched matmul's fused rescale NTT
 // masked form: the plaintext of term (j, i) encodes {mask[t] ? W[j][i] : 0} (t < N/2)
 void ctpt_matmul_masked(const Context &c, const std::vector<const Ciphertext *> &X, const double *W, size_t row_w,
                         size_t col_w, const uint8_t *mask, const std::vector<Ciphertext *> &out, hipStream_t s);
@@ -391,7 +395,7 @@ void ct_mul_const(Context &c, Ciphertext &a, double v, double sigma, hipStream_t
 void ct_mul_monomial_half(Context &c, Ciphertext &a, bool negate, hipStream_t s);
 void ct_modraise(Context &c, Ciphertext &a, hipStream_t s);
 
-// timing hook for the dominant kernel of the last matmul (HIP events)
+// timing hook for the contraction kernel(s) of the last batched matmul (HIP events)
 float last_matmul_kernel_ms();
 int last_matmul_variant();
 

@@ -39,6 +39,7 @@ inline unsigned grid_for(size_t work) {
 }
 
 float g_last_ms = 0.f;
+float g_last_tile_ms = 0.f;
 
 // Weight residues: wres[l][j][i] = [round(fl(W[j][i] * scale_j))]_{q_l}.
 __global__ void k_weight_residues(const double *W, const double *scales, uint64_t *wres, const DevMod *mods,
@@ -261,6 +262,7 @@ void ctpt_matmul_masked(const Context &c, const std::vector<const Ciphertext *> 
 }
 
 float last_matmul_kernel_ms() { return g_last_ms; }
+float last_matmul_rescale_tile_ms() { return g_last_tile_ms; }
 int last_matmul_variant() { return g_last_variant; }
 
 void ctpt_matmul(const Context &c, const std::vector<const Ciphertext *> &X, const double *W, size_t row_w,
@@ -339,7 +341,21 @@ void ctpt_matmul(const Context &c, const std::vector<const Ciphertext *> &X, con
             SHM_HIP(hipEventRecord(e1, s));
             SHM_HIP(hipStreamSynchronize(s));
         }
-        batched_rescale(c, acc.as<uint64_t>(), out, Ql, s);
+        hipEvent_t t0, t1;
+        SHM_HIP(hipEventCreate(&t0));
+        SHM_HIP(hipEventCreate(&t1));
+        set_rescale_tile_events(t0, t1);
+        try {
+            batched_rescale(c, acc.as<uint64_t>(), out, Ql, s);
+        } catch (...) {
+            set_rescale_tile_events(nullptr, nullptr);
+            throw;
+        }
+        set_rescale_tile_events(nullptr, nullptr);
+        SHM_HIP(hipEventSynchronize(t1));
+        SHM_HIP(hipEventElapsedTime(&g_last_tile_ms, t0, t1));
+        SHM_HIP(hipEventDestroy(t0));
+        SHM_HIP(hipEventDestroy(t1));
     }
     SHM_HIP(hipEventSynchronize(e1));
     SHM_HIP(hipEventElapsedTime(&g_last_ms, e0, e1));

@@ -405,6 +405,9 @@ __global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ 
     }
 }
 
+// optional events around the tile pass of the fused rescale (timing of one launch)
+hipEvent_t g_rescale_tile_ev[2] = {nullptr, nullptr};
+
 template <bool FWD, int IOA, int IOB>
 void launch_ntt(const Context &c, uint64_t *data, const LimbMap &mp, size_t total_limbs, hipStream_t s,
                 const NttIO &io = NttIO{}) {
@@ -473,7 +476,10 @@ void launch_ntt(const Context &c, uint64_t *data, const LimbMap &mp, size_t tota
 #undef SHM_BLKS
     if (FWD) {
         cols();
+        const bool timed = IOB == IO_RESCALE && g_rescale_tile_ev[0] && g_rescale_tile_ev[1];
+        if (timed) SHM_HIP(hipEventRecord(g_rescale_tile_ev[0], s));
         blks();
+        if (timed) SHM_HIP(hipEventRecord(g_rescale_tile_ev[1], s));
     } else {
         blks();
         cols();
@@ -488,6 +494,11 @@ void ntt_forward(const Context &c, uint64_t *data, size_t n_polys, size_t n_limb
     LimbMap mp{(uint32_t)n_limbs, 0, 0xffffffffu, 0, 0xffffffffu, (uint32_t)c.nQ, (uint32_t)first_prime,
                (uint64_t)n_limbs * c.N};
     launch_ntt<true, IO_PLAIN, IO_PLAIN>(c, data, mp, n_polys * n_limbs, s);
+}
+
+void set_rescale_tile_events(hipEvent_t start, hipEvent_t stop) {
+    g_rescale_tile_ev[0] = start;
+    g_rescale_tile_ev[1] = stop;
 }
 
 void ntt_forward_encoded(const Context &c, const double2 *fft, uint64_t *out, size_t n_polys, size_t n_limbs,
