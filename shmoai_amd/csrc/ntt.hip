@@ -201,14 +201,20 @@ __device__ __forceinline__ void lds_round(E *lds, int s0, uint32_t gmul, uint32_
     }
 }
 
-template <int LOGT, bool FWD, typename E, typename K>
+struct NoHook {
+    __device__ void operator()() const {}
+};
+
+// before_last runs (forward only) just before the last round
+template <int LOGT, bool FWD, typename E, typename K, typename H = NoHook>
 __device__ __forceinline__ void run_rounds(E *lds, uint32_t gmul, uint32_t gadd_step, uint32_t gadd0, const E *tw,
-                                           const E *tws, K k) {
+                                           const E *tws, K k, H before_last = H{}) {
     constexpr int NR = n_rounds(LOGT);
     if (FWD) {
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             const int R = round_len(LOGT, r), s0 = round_start(LOGT, r);
+            if (r == NR - 1) before_last();
             switch (R) {
                 case 4: lds_round<LOGT, 4, true>(lds, s0, gmul, gadd_step, gadd0, tw, tws, k); break;
                 case 3: lds_round<LOGT, 3, true>(lds, s0, gmul, gadd_step, gadd0, tw, tws, k); break;
@@ -356,20 +362,27 @@ __global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ 
         const int b = i / T, r = i % T;
         lds[b * TP + swz<T>(r)] = base[i];
     }
-    // rescale: the accumulator words are read when the outputs are written
-    // (a prefetch into registers before the butterflies cost occupancy)
+    // rescale: the accumulator words are loaded before the last round, so they
+    // arrive during it (loading them before all rounds cost occupancy)
     constexpr int PER = kTile / kThreads;
     const uint64_t *accp = nullptr;
     if (FWD && IO == IO_RESCALE)
         accp = io.packed ? io.acc_p.p[poly] + (size_t)k * N + t0
                          : io.acc + ((size_t)poly * io.acc_limbs + k) * N + t0;
+    uint64_t accv[IO == IO_RESCALE ? PER : 1];
+    auto load_acc = [&] {
+        if (FWD && IO == IO_RESCALE) {
+#pragma unroll
+            for (int j = 0; j < PER; j++) accv[j] = accp[threadIdx.x + j * kThreads];
+        }
+    };
     __syncthreads();
     if (LAZY)
         run_rounds<LOGT, FWD>(lds, N1, 1u, (uint32_t)(t0 / T), tb.tw + (size_t)prime * N,
-                              tb.tws + (size_t)prime * N, IntKLazy{q});
+                              tb.tws + (size_t)prime * N, IntKLazy{q}, load_acc);
     else
         run_rounds<LOGT, FWD>(lds, N1, 1u, (uint32_t)(t0 / T), tb.tw + (size_t)prime * N,
-                              tb.tws + (size_t)prime * N, IntK{q});
+                              tb.tws + (size_t)prime * N, IntK{q}, load_acc);
     // canonical residue of a forward output: below 4q, or below 36q when lazy
     auto canon = [&](uint64_t v) -> uint64_t {
         if (LAZY) return reduce64(v, tb.mods[prime]);
@@ -390,7 +403,7 @@ __global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ 
             const int i = threadIdx.x + j * kThreads;
             const int b = i / T, r = i % T;
             const uint64_t v = canon(lds[b * TP + swz<T>(r)]);
-            __builtin_nontemporal_store(mul_shoup(sub_mod(accp[i], v, q), qi, qis, q), &out[i]);
+            __builtin_nontemporal_store(mul_shoup(sub_mod(accv[j], v, q), qi, qis, q), &out[i]);
         }
     } else if (FWD) {
         for (int i = threadIdx.x; i < kTile; i += kThreads) {
