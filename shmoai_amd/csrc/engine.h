@@ -314,14 +314,16 @@ void ks_moddown(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, uint64
 // The same with polynomial p written to out.p[p]
 void ks_moddown_pack(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, const PolyPack &out,
                      uint64_t acc_mask, hipStream_t s, bool rounded = false);
-// ModUp of m polynomials c2.p[i] (Ql limbs, NTT) into d [m][beta][QlP][N]
+// ModUp of m polynomials c2.p[i] (Ql limbs, NTT) into d [m][beta][QlP][N]; from 4
+// members on the digits' own limbs are not written (they are c2's limbs: pass c2
+// as `own` to ks_inner_batch, which reads them from there in every case)
 void ks_modup_batch(const Context &c, const PolyPack &c2, size_t m, size_t Ql, uint64_t *d, hipStream_t s,
                     bool rounded = false);
 // cx [m][2][QlP][N] = inner products of the m digit sets of d with one key
 // ([dnum][2][QP][N]); each key word is read once for the group; with perm the
 // digits are read through that Galois permutation
 void ks_inner_batch(const Context &c, const uint64_t *d, size_t m, size_t Ql, const uint64_t *key, uint64_t *cx,
-                    hipStream_t s, const uint32_t *perm = nullptr);
+                    hipStream_t s, const uint32_t *perm = nullptr, const PolyPack *own = nullptr);
 // Key switch of m polynomials c2.p[i] with one key; the switched pair of member
 // i goes to (ct01.p[2i], ct01.p[2i + 1]), accumulated where acc_mask has the
 // polynomial's bit. One launch per stage for the whole group.

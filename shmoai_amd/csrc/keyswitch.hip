@@ -8,6 +8,8 @@
 // cx0 = sum_g d_g key_g.b, cx1 = sum_g d_g key_g.a with 128-bit accumulation.
 // ModDown (rns_bconv.cu:776-828, ntt_moddown.cu:106-261): iNTT of the P limbs,
 // delta = fastbconv(P -> Ql), NTT(delta), out = (cx - delta) P^-1 mod q.
+#include <algorithm>
+
 #include "engine.h"
 
 namespace shm {
@@ -43,7 +45,7 @@ __global__ void k_modup_bconv(const uint64_t *__restrict__ tcoef_all, uint64_t *
                               const DevMod *__restrict__ mods, const uint64_t *__restrict__ hat_inv,
                               const uint64_t *__restrict__ hat_inv_s, const uint64_t *__restrict__ hat_mod,
                               const uint64_t *__restrict__ dig_mod, size_t Ql, size_t nQ, size_t alpha, size_t beta,
-                              size_t N, size_t members, int rounded) {
+                              size_t N, size_t members, int rounded, int write_own) {
     const size_t QlP = Ql + alpha;
     const size_t total = members * beta * N;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
@@ -62,8 +64,9 @@ __global__ void k_modup_bconv(const uint64_t *__restrict__ tcoef_all, uint64_t *
                 const uint64_t qi = mods[li].q;
                 y[i] = size == 1 ? tcoef[li * N + j] : mul_shoup(tcoef[li * N + j], hat_inv[li], hat_inv_s[li], qi);
                 if (rounded) v = __dadd_rn(v, __ddiv_rn((double)y[i], (double)qi));
-                // own limb in coefficient form: the batched NTT below restores c2 exactly
-                o[li * N] = tcoef[li * N + j];
+                // own limb in coefficient form: the NTT below restores c2 exactly
+                // (without write_own the caller reads the own limbs from c2 itself)
+                if (write_own) o[li * N] = tcoef[li * N + j];
             }
         }
         const uint64_t r = rounded ? (uint64_t)__dadd_rn(v, 0.5) : 0;
@@ -212,7 +215,7 @@ __global__ __launch_bounds__(256) void k_ks_inner_batch(const uint64_t *__restri
                                                         uint64_t *__restrict__ cx, const DevMod *__restrict__ mods,
                                                         size_t Ql, size_t nQ, size_t QP, size_t alpha, int beta,
                                                         size_t N, int members, int lgmg,
-                                                        const uint32_t *__restrict__ perm) {
+                                                        const uint32_t *__restrict__ perm, PolyPack c2, int own_c2) {
     const size_t QlP = Ql + alpha, total = QlP * N;
     const int lgj = 8 - lgmg;
     const size_t t = ((size_t)blockIdx.x << lgj) + (threadIdx.x & ((1u << lgj) - 1));
@@ -238,7 +241,9 @@ __global__ __launch_bounds__(256) void k_ks_inner_batch(const uint64_t *__restri
 #pragma unroll
         for (int g = 0; g < MAXB; g++) {
             if (g < beta) {
-                const uint64_t dv = dm[(size_t)g * total];
+                // digit g's own limbs k in [g alpha, g alpha + alpha) are c2's limbs
+                const bool own = own_c2 && k < Ql && k >= (size_t)g * alpha && k < (size_t)(g + 1) * alpha;
+                const uint64_t dv = own ? c2.p[mem][src] : dm[(size_t)g * total];
                 mac128(a0, dv, kb[g]);
                 mac128(a1, dv, ka[g]);
             }
@@ -277,7 +282,7 @@ void ks_modup(const Context &c, const uint64_t *c2, size_t Ql, uint64_t *d, hipS
     if (alpha > (size_t)kMaxAlpha) throw InvalidArgument("at most 16 special primes");
     k_modup_bconv<<<grid_for(beta * N), kBlock, 0, s>>>(
         tcoef.as<uint64_t>(), d, c.mods(), L.hat_inv.as<uint64_t>(), L.hat_inv_s.as<uint64_t>(),
-        L.hat_mod.as<uint64_t>(), L.dig_mod.as<uint64_t>(), Ql, c.nQ, alpha, beta, N, 1, rounded ? 1 : 0);
+        L.hat_mod.as<uint64_t>(), L.dig_mod.as<uint64_t>(), Ql, c.nQ, alpha, beta, N, 1, rounded ? 1 : 0, 1);
     SHM_HIP(hipGetLastError());
     // one launch over every digit's QlP limbs (the own limbs were written in
     // coefficient form and transform back to their NTT-form input words)
@@ -355,15 +360,29 @@ void ks_modup_batch(const Context &c, const PolyPack &c2, size_t m, size_t Ql, u
     Scratch tcoef(m * Ql * N * sizeof(uint64_t), s);
     launch_gather_polys(c, c2, tcoef.as<uint64_t>(), Ql * N, m, s);
     ntt_inverse(c, tcoef.as<uint64_t>(), m, Ql, 0, s);
+    // From 4 members on, each digit gets its own NTT launch over the m members that
+    // skips the digit's own limbs: they are c2's NTT-form words, which the inner
+    // product reads from c2. With fewer members such launches are too small to
+    // fill the GPU, so the own limbs are written in coefficient form and one launch
+    // transforms every limb (back to c2's words for the own ones).
+    const bool per_digit = m >= 4;
     k_modup_bconv<<<grid_for(m * beta * N), kBlock, 0, s>>>(
         tcoef.as<uint64_t>(), d, c.mods(), L.hat_inv.as<uint64_t>(), L.hat_inv_s.as<uint64_t>(),
-        L.hat_mod.as<uint64_t>(), L.dig_mod.as<uint64_t>(), Ql, c.nQ, alpha, beta, N, m, rounded ? 1 : 0);
+        L.hat_mod.as<uint64_t>(), L.dig_mod.as<uint64_t>(), Ql, c.nQ, alpha, beta, N, m, rounded ? 1 : 0,
+        per_digit ? 0 : 1);
     SHM_HIP(hipGetLastError());
-    ntt_forward_qlp(c, d, Ql, 0, QlP, 0, 0, m * beta, QlP * N, s);
+    if (per_digit) {
+        for (size_t g = 0; g < beta; g++) {
+            const size_t own_lo = g * alpha, own_hi = std::min(own_lo + alpha, Ql);
+            ntt_forward_qlp(c, d + g * QlP * N, Ql, 0, QlP, own_lo, own_hi, m, beta * QlP * N, s);
+        }
+    } else {
+        ntt_forward_qlp(c, d, Ql, 0, QlP, 0, 0, m * beta, QlP * N, s);
+    }
 }
 
 void ks_inner_batch(const Context &c, const uint64_t *d, size_t m, size_t Ql, const uint64_t *key, uint64_t *cx,
-                    hipStream_t s, const uint32_t *perm) {
+                    hipStream_t s, const uint32_t *perm, const PolyPack *own) {
     const size_t QlP = Ql + c.nP, beta = c.level(Ql).beta;
     if (beta > c.dnum) throw LogicError("key switching key has too few digits for this level");
     if (beta > (size_t)kMaxDigits) throw InvalidArgument("at most 16 key digits");
@@ -373,7 +392,8 @@ void ks_inner_batch(const Context &c, const uint64_t *d, size_t m, size_t Ql, co
     const size_t total = QlP * c.N;
     const unsigned blocks = (unsigned)((total + (256u >> lgmg) - 1) >> (8 - lgmg));
     auto fn = beta <= 4 ? k_ks_inner_batch<4> : beta <= 8 ? k_ks_inner_batch<8> : k_ks_inner_batch<16>;
-    fn<<<blocks, kBlock, 0, s>>>(d, key, cx, c.mods(), Ql, c.nQ, c.nQP(), c.nP, (int)beta, c.N, (int)m, lgmg, perm);
+    fn<<<blocks, kBlock, 0, s>>>(d, key, cx, c.mods(), Ql, c.nQ, c.nQP(), c.nP, (int)beta, c.N, (int)m, lgmg, perm,
+                                 own ? *own : PolyPack{}, own ? 1 : 0);
     SHM_HIP(hipGetLastError());
 }
 
@@ -385,7 +405,7 @@ void keyswitch_batch(const Context &c, const PolyPack &c2, size_t m, const uint6
     ks_modup_batch(c, c2, m, Ql, d.as<uint64_t>(), s, rounded);
     // inner products: each key word read once for the group
     Scratch cx(2 * m * QlP * N * sizeof(uint64_t), s);
-    ks_inner_batch(c, d.as<uint64_t>(), m, Ql, key, cx.as<uint64_t>(), s, nullptr);
+    ks_inner_batch(c, d.as<uint64_t>(), m, Ql, key, cx.as<uint64_t>(), s, nullptr, &c2);
     // ModDown of the 2m polynomials into the members' (c0, c1)
     ks_moddown_pack(c, cx.as<uint64_t>(), 2 * m, Ql, ct01, acc_mask, s, rounded);
 }

@@ -155,6 +155,8 @@ void rotate_hoisted_group(Context &c, const Ciphertext &g, const std::vector<int
     const size_t m = g.size / 2, Ql = g.Ql(), N = c.N, QlP = Ql + c.nP, beta = c.level(Ql).beta;
     Scratch d(m * beta * QlP * N * sizeof(uint64_t), s), cx(2 * m * QlP * N * sizeof(uint64_t), s);
     bool have_modup = false;
+    PolyPack a{};
+    for (size_t k = 0; k < m; k++) a.p[k] = g.poly(2 * k + 1);
     for (size_t i = 0; i < steps.size(); i++) {
         Ciphertext &o = *outs[i];
         const uint32_t elt = c.elt_from_step(steps[i]);
@@ -165,13 +167,11 @@ void rotate_hoisted_group(Context &c, const Ciphertext &g, const std::vector<int
             continue;
         }
         if (!have_modup) {
-            PolyPack a{};
-            for (size_t k = 0; k < m; k++) a.p[k] = g.poly(2 * k + 1);
             ks_modup_batch(c, a, m, Ql, d.as<uint64_t>(), s, rounded);
             have_modup = true;
         }
         const uint32_t *table = c.galois_table(elt, s);
-        ks_inner_batch(c, d.as<uint64_t>(), m, Ql, it->second->k.as<uint64_t>(), cx.as<uint64_t>(), s, table);
+        ks_inner_batch(c, d.as<uint64_t>(), m, Ql, it->second->k.as<uint64_t>(), cx.as<uint64_t>(), s, table, &a);
         o.ctx = &c;
         o.resize(g.size, g.chain, s, false);  // permute writes every b_i, ModDown every a_i
         o.scale = g.scale;
