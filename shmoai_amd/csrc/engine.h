@@ -258,8 +258,12 @@ void launch_add(const Context &c, uint64_t *a, const uint64_t *b, size_t Ql, siz
 void launch_sub(const Context &c, uint64_t *a, const uint64_t *b, size_t Ql, size_t npolys, hipStream_t s);
 void launch_negate(const Context &c, uint64_t *a, size_t Ql, size_t npolys, hipStream_t s);
 void launch_mul_plain(const Context &c, uint64_t *ct, const uint64_t *pt, size_t Ql, size_t npolys, hipStream_t s);
-void launch_tensor(const Context &c, const uint64_t *a, const uint64_t *b, uint64_t *out3, size_t Ql, hipStream_t s);
-void launch_square(const Context &c, const uint64_t *a, uint64_t *out3, size_t Ql, hipStream_t s);
+// operands with more limbs than Ql (a_limbs / b_limbs per polynomial, 0 = Ql) are
+// used through their first Ql limbs
+void launch_tensor(const Context &c, const uint64_t *a, const uint64_t *b, uint64_t *out3, size_t Ql, hipStream_t s,
+                   size_t a_limbs = 0, size_t b_limbs = 0);
+void launch_square(const Context &c, const uint64_t *a, uint64_t *out3, size_t Ql, hipStream_t s,
+                   size_t a_limbs = 0);
 void launch_rescale(const Context &c, const uint64_t *in, uint64_t *out, size_t Ql, size_t npolys, hipStream_t s,
                     bool rounded = false);
 void launch_permute(const Context &c, const uint64_t *in, uint64_t *out, const uint32_t *table, size_t nlimbs,

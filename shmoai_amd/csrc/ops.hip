@@ -408,30 +408,21 @@ void ct_align(Context &c, Ciphertext &a, size_t chain, double scale, hipStream_t
     a.scale = scale;
 }
 
-// out = rescale(relinearize(a * b)); operands are brought to the lower level first.
+// out = rescale(relinearize(a * b)) at the lower of the two operands' levels.
 void ct_mul_relin_rescale(Context &c, Ciphertext &out, const Ciphertext &a, const Ciphertext &b,
                           const KSwitchKey &rk, hipStream_t s, bool rounded) {
     if (a.size != 2 || b.size != 2) throw InvalidArgument("multiply supports size-2 ciphertexts");
+    // the operand at the higher level enters through its first Ql limbs (the
+    // limbs a mod switch to the lower level keeps), with no copy
     const size_t chain = std::max(a.chain, b.chain);
-    Ciphertext ta, tb;
     const Ciphertext *pa = &a, *pb = &b;
-    if (a.chain != chain) {
-        ct_clone(ta, a, s);
-        mod_switch_drop(c, ta, chain, s);
-        pa = &ta;
-    }
-    if (b.chain != chain) {
-        ct_clone(tb, b, s);
-        mod_switch_drop(c, tb, chain, s);
-        pb = &tb;
-    }
     const size_t Ql = c.ql_of_chain(chain);
     out.ctx = &c;
     out.resize(3, chain, s, false);  // tensor / square write d0, d1, d2
     if (pa == pb || (pa->data() == pb->data()))
-        launch_square(c, pa->data(), out.data(), Ql, s);
+        launch_square(c, pa->data(), out.data(), Ql, s, pa->Ql());
     else
-        launch_tensor(c, pa->data(), pb->data(), out.data(), Ql, s);
+        launch_tensor(c, pa->data(), pb->data(), out.data(), Ql, s, pa->Ql(), pb->Ql());
     out.scale = pa->scale * pb->scale;
     out.ntt = true;
     keyswitch(c, out.poly(2), rk.k.as<uint64_t>(), Ql, out.poly(0), out.poly(1), true, s, rounded);
@@ -448,23 +439,13 @@ void mul_relin_rescale_batch(Context &c, const std::vector<Ciphertext *> &out,
     if (m == 0) return;
     if (a.size() != m || b.size() != m || 2 * m > (size_t)kPolyPack)
         throw InvalidArgument("mul_relin_rescale_batch: 1..32 products");
-    std::vector<Ciphertext> tmp(2 * m);
     PolyPack c2{}, ct01{};
     size_t Ql = 0;
     for (size_t i = 0; i < m; i++) {
         if (a[i]->size != 2 || b[i]->size != 2) throw InvalidArgument("multiply supports size-2 ciphertexts");
+        // an operand at a higher level enters through its first Ql limbs, uncopied
         const size_t chain = std::max(a[i]->chain, b[i]->chain);
         const Ciphertext *pa = a[i], *pb = b[i];
-        if (a[i]->chain != chain) {
-            ct_clone(tmp[2 * i], *a[i], s);
-            mod_switch_drop(c, tmp[2 * i], chain, s);
-            pa = &tmp[2 * i];
-        }
-        if (b[i]->chain != chain) {
-            ct_clone(tmp[2 * i + 1], *b[i], s);
-            mod_switch_drop(c, tmp[2 * i + 1], chain, s);
-            pb = &tmp[2 * i + 1];
-        }
         const size_t ql = c.ql_of_chain(chain);
         if (i == 0) Ql = ql;
         if (ql != Ql) throw InvalidArgument("mul_relin_rescale_batch: products at different levels");
@@ -472,9 +453,9 @@ void mul_relin_rescale_batch(Context &c, const std::vector<Ciphertext *> &out,
         o.ctx = &c;
         o.resize(3, chain, s, false);  // tensor / square write d0, d1, d2
         if (pa == pb || (pa->data() == pb->data()))
-            launch_square(c, pa->data(), o.data(), Ql, s);
+            launch_square(c, pa->data(), o.data(), Ql, s, pa->Ql());
         else
-            launch_tensor(c, pa->data(), pb->data(), o.data(), Ql, s);
+            launch_tensor(c, pa->data(), pb->data(), o.data(), Ql, s, pa->Ql(), pb->Ql());
         o.scale = pa->scale * pb->scale;
         o.ntt = true;
         c2.p[i] = o.poly(2);

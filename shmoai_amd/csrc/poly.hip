@@ -47,12 +47,15 @@ __global__ void k_mul_plain(uint64_t *ct, const uint64_t *pt, const DevMod *mods
 }
 
 // polymath.cu:468-503 tensor_prod_2x2_rns_poly (Karatsuba middle term).
+// a and b hold a_limbs and b_limbs limbs per polynomial (>= Ql): an operand at a
+// higher level is used through its first Ql limbs, which is what dropping its
+// top limbs (mod_switch_to) would leave.
 __global__ void k_tensor(const uint64_t *a, const uint64_t *b, uint64_t *out, const DevMod *mods, size_t Ql,
-                         size_t N) {
-    const size_t S = Ql * N;
+                         size_t N, size_t a_limbs, size_t b_limbs) {
+    const size_t S = Ql * N, Sa = a_limbs * N, Sb = b_limbs * N;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < S; t += (size_t)gridDim.x * blockDim.x) {
         const DevMod m = mods[t / N];
-        const uint64_t a0 = a[t], a1 = a[t + S], b0 = b[t], b1 = b[t + S];
+        const uint64_t a0 = a[t], a1 = a[t + Sa], b0 = b[t], b1 = b[t + Sb];
         const uint64_t d0 = mul_mod(a0, b0, m), d2 = mul_mod(a1, b1, m);
         uint64_t d1 = mul_mod(a0 + a1, b0 + b1, m);
         d1 = d1 + 2 * m.q - d0 - d2;
@@ -65,11 +68,12 @@ __global__ void k_tensor(const uint64_t *a, const uint64_t *b, uint64_t *out, co
 }
 
 // polymath.cu:505-550 tensor_square_2x2_rns_poly.
-__global__ void k_square(const uint64_t *a, uint64_t *out, const DevMod *mods, size_t Ql, size_t N) {
-    const size_t S = Ql * N;
+__global__ void k_square(const uint64_t *a, uint64_t *out, const DevMod *mods, size_t Ql, size_t N,
+                         size_t a_limbs) {
+    const size_t S = Ql * N, Sa = a_limbs * N;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < S; t += (size_t)gridDim.x * blockDim.x) {
         const DevMod m = mods[t / N];
-        const uint64_t c0 = a[t], c1 = a[t + S];
+        const uint64_t c0 = a[t], c1 = a[t + Sa];
         const uint64_t d0 = mul_mod(c0, c0, m), d2 = mul_mod(c1, c1, m);
         const uint64_t lo = c0 * c1, hi = __umul64hi(c0, c1);
         const uint64_t d1 = barrett128((hi << 1) | (lo >> 63), lo << 1, m);
@@ -241,13 +245,19 @@ void launch_mul_plain(const Context &c, uint64_t *ct, const uint64_t *pt, size_t
     SHM_HIP(hipGetLastError());
 }
 
-void launch_tensor(const Context &c, const uint64_t *a, const uint64_t *b, uint64_t *out3, size_t Ql, hipStream_t s) {
-    k_tensor<<<grid_for(Ql * c.N), kBlock, 0, s>>>(a, b, out3, c.mods(), Ql, c.N);
+void launch_tensor(const Context &c, const uint64_t *a, const uint64_t *b, uint64_t *out3, size_t Ql, hipStream_t s,
+                   size_t a_limbs, size_t b_limbs) {
+    if (!a_limbs) a_limbs = Ql;
+    if (!b_limbs) b_limbs = Ql;
+    if (a_limbs < Ql || b_limbs < Ql) throw LogicError("tensor operand below the product level");
+    k_tensor<<<grid_for(Ql * c.N), kBlock, 0, s>>>(a, b, out3, c.mods(), Ql, c.N, a_limbs, b_limbs);
     SHM_HIP(hipGetLastError());
 }
 
-void launch_square(const Context &c, const uint64_t *a, uint64_t *out3, size_t Ql, hipStream_t s) {
-    k_square<<<grid_for(Ql * c.N), kBlock, 0, s>>>(a, out3, c.mods(), Ql, c.N);
+void launch_square(const Context &c, const uint64_t *a, uint64_t *out3, size_t Ql, hipStream_t s, size_t a_limbs) {
+    if (!a_limbs) a_limbs = Ql;
+    if (a_limbs < Ql) throw LogicError("square operand below the product level");
+    k_square<<<grid_for(Ql * c.N), kBlock, 0, s>>>(a, out3, c.mods(), Ql, c.N, a_limbs);
     SHM_HIP(hipGetLastError());
 }
 
