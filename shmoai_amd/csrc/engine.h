@@ -271,6 +271,10 @@ void launch_permute(const Context &c, const uint64_t *in, uint64_t *out, const u
 // out[p][i] = in.p[p][i], i < words, p < npolys
 void launch_gather_polys(const Context &c, const PolyPack &in, uint64_t *out, size_t words, size_t npolys,
                          hipStream_t s);
+// poly 0 (+)= [rounded]_q per limb; npolys polynomials (*)= [rounded]_q per limb
+// (rounded = round(value * scale), as encode_scalar computes it)
+void launch_add_scalar(const Context &c, uint64_t *a, double rounded, size_t Ql, hipStream_t s);
+void launch_mul_scalar(const Context &c, uint64_t *a, double rounded, size_t Ql, size_t npolys, hipStream_t s);
 // zero fill by a kernel (a memset node recorded into a captured graph is avoided)
 void launch_fill_zero(uint64_t *out, size_t words, hipStream_t s);
 // device-to-device copy by a kernel (non-overlapping ranges)

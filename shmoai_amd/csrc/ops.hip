@@ -329,18 +329,16 @@ void ct_sub(Context &c, Ciphertext &a, const Ciphertext &b, hipStream_t s) {
 
 // a += constant v (encoded at a's exact scale).
 void ct_add_const(Context &c, Ciphertext &a, double v, hipStream_t s) {
-    const size_t Ql = a.Ql();
-    Scratch pt(Ql * c.N * sizeof(uint64_t), s);
-    encode_scalar(c, v, Ql, a.scale, pt.as<uint64_t>(), s);
-    launch_add(c, a.data(), pt.as<uint64_t>(), Ql, 1, s);
+    // add_plain of encode_scalar(v, a.scale) without materialising the plaintext
+    if (!(a.scale > 0)) throw InvalidArgument("scale out of bounds");
+    launch_add_scalar(c, a.data(), round(v * a.scale), a.Ql(), s);
 }
 
 // a <- a * [v encoded at scale sigma]; scale becomes a.scale * sigma.
 void ct_mul_const(Context &c, Ciphertext &a, double v, double sigma, hipStream_t s) {
-    const size_t Ql = a.Ql();
-    Scratch pt(Ql * c.N * sizeof(uint64_t), s);
-    encode_scalar(c, v, Ql, sigma, pt.as<uint64_t>(), s);
-    launch_mul_plain(c, a.data(), pt.as<uint64_t>(), Ql, a.size, s);
+    // multiply_plain by encode_scalar(v, sigma) without materialising the plaintext
+    if (!(sigma > 0)) throw InvalidArgument("scale out of bounds");
+    launch_mul_scalar(c, a.data(), round(v * sigma), a.Ql(), a.size, s);
     a.scale *= sigma;
 }
 

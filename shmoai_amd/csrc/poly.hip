@@ -116,6 +116,28 @@ __global__ void k_copy_limbs(const uint64_t *in, size_t in_limbs, uint64_t *out,
     }
 }
 
+// a (+)= / (*)= the constant [round(value * scale)]_q of every limb, the residue
+// being what encode_scalar writes into each slot of the constant plaintext (so
+// the result equals add_plain / multiply_plain with that plaintext).
+__global__ void k_add_scalar(uint64_t *a, const DevMod *mods, double v, size_t N, size_t total) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const DevMod m = mods[t / N];
+        uint64_t r = residue_of_rounded(v, m);
+        r = r >= m.q ? r - m.q : r;
+        a[t] = add_mod(a[t], r, m.q);
+    }
+}
+
+__global__ void k_mul_scalar(uint64_t *a, const DevMod *mods, double v, size_t Ql, size_t N, size_t total) {
+    const size_t S = Ql * N;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const DevMod m = mods[(t % S) / N];
+        uint64_t r = residue_of_rounded(v, m);
+        r = r >= m.q ? r - m.q : r;
+        a[t] = mul_mod(a[t], r, m);
+    }
+}
+
 __global__ void k_fill_zero(uint64_t *out, size_t words) {
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < words; t += (size_t)gridDim.x * blockDim.x)
         out[t] = 0;
@@ -318,6 +340,17 @@ void launch_copy_words(uint64_t *dst, const uint64_t *src, size_t words, hipStre
     if (words == 0 || dst == src) return;
     const int vec = (((uintptr_t)dst | (uintptr_t)src) & 15) == 0 ? 1 : 0;
     k_copy_words<<<grid_for(vec ? (words + 1) / 2 : words), kBlock, 0, s>>>(dst, src, words, vec);
+    SHM_HIP(hipGetLastError());
+}
+
+void launch_add_scalar(const Context &c, uint64_t *a, double rounded, size_t Ql, hipStream_t s) {
+    k_add_scalar<<<grid_for(Ql * c.N), kBlock, 0, s>>>(a, c.mods(), rounded, c.N, Ql * c.N);
+    SHM_HIP(hipGetLastError());
+}
+
+void launch_mul_scalar(const Context &c, uint64_t *a, double rounded, size_t Ql, size_t npolys, hipStream_t s) {
+    const size_t total = npolys * Ql * c.N;
+    k_mul_scalar<<<grid_for(total), kBlock, 0, s>>>(a, c.mods(), rounded, Ql, c.N, total);
     SHM_HIP(hipGetLastError());
 }
 
