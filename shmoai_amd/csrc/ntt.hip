@@ -398,12 +398,17 @@ __global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ 
             out = ob + ((size_t)(poly % io.ppo) * mp.n_active + k) * N + t0;
         }
         const uint64_t qi = io.qinv[prime], qis = io.qinv_s[prime];
+        // unreduced outputs stay below (4 + 2 logN) q (< 2^64 for q < 2^58): acc - v is
+        // taken as acc + (4 + 2 logN) q - v, which the Shoup product accepts as it is
+        const uint64_t kq = (uint64_t)(4 + 2 * logN) * q;
 #pragma unroll
         for (int j = 0; j < PER; j++) {
             const int i = threadIdx.x + j * kThreads;
             const int b = i / T, r = i % T;
-            const uint64_t v = canon(lds[b * TP + swz<T>(r)]);
-            __builtin_nontemporal_store(mul_shoup(sub_mod(accv[j], v, q), qi, qis, q), &out[i]);
+            const uint64_t raw = lds[b * TP + swz<T>(r)];
+            const uint64_t y = LAZY ? mul_shoup(accv[j] + kq - raw, qi, qis, q)
+                                    : mul_shoup(sub_mod(accv[j], canon(raw), q), qi, qis, q);
+            __builtin_nontemporal_store(y, &out[i]);
         }
     } else if (FWD) {
         for (int i = threadIdx.x; i < kTile; i += kThreads) {
