@@ -38,6 +38,15 @@ __device__ __forceinline__ uint64_t mul_shoup(uint64_t x, uint64_t w, uint64_t w
     return csub(mul_shoup_lazy(x, w, ws, q), q);
 }
 
+// Shoup companion floor(w * 2^64 / q) of w < q from the Barrett constant:
+// floor(w * mu / 2^64) is at most one short, and the remainder w * 2^64 - e * q
+// (exact in 64 bits, below 2q) says when.
+__device__ __forceinline__ uint64_t shoup_companion(uint64_t w, const DevMod &m) {
+    uint64_t e = w * m.mu_hi + __umul64hi(w, m.mu_lo);
+    const uint64_t rem = 0 - e * m.q;
+    return rem >= m.q ? e + 1 : e;
+}
+
 // Barrett reduction of a 128-bit value (hi, lo) with mu = floor(2^128/q).
 // The quotient estimate is at most 2 short, so the remainder is in [0, 3q)
 // before the final corrections.

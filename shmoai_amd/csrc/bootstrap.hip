@@ -24,10 +24,13 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kMaxTerms = 16;
 
-inline unsigned grid_for(size_t work) {
-    size_t g = (work + kBlock - 1) / kBlock;
-    if (g > (1u << 20)) g = 1u << 20;
-    return (unsigned)(g ? g : 1);
+// [rows][N] launches: blockIdx.y is the row (one limb of one polynomial), so a
+// row's limb is uniform per block; 4 words per thread when the launch is large.
+inline dim3 rows_grid(size_t rows, size_t N) {
+    size_t bx = (N + kBlock - 1) / kBlock;
+    if (rows * bx >= 8192) bx = (bx + 3) / 4;
+    const size_t by = rows == 0 ? 1 : (rows < 65535 ? rows : 65535);
+    return dim3((unsigned)bx, (unsigned)by);
 }
 
 struct PtPack {
@@ -39,12 +42,14 @@ struct PtPack {
 // products accumulate in 128 bits (at most 16 terms below 2^122).
 __global__ void k_lt_mac(uint64_t *acc, PtPack pk, int nterms, const DevMod *mods, size_t Ql, size_t N,
                          size_t npolys) {
-    const size_t S = Ql * N, total = npolys * S;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t r = t % S;
-        Acc128 a{0, 0};
-        for (int k = 0; k < nterms; k++) mac128(a, pk.x[k][t], pk.pt[k][r]);
-        acc[t] = reduce128(a, mods[r / N]);
+    for (size_t row = blockIdx.y; row < npolys * Ql; row += gridDim.y) {
+        const size_t limb = row % Ql;
+        const DevMod m = mods[limb];
+        for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
+            Acc128 a{0, 0};
+            for (int k = 0; k < nterms; k++) mac128(a, pk.x[k][row * N + i], pk.pt[k][limb * N + i]);
+            acc[row * N + i] = reduce128(a, m);
+        }
     }
 }
 
@@ -55,20 +60,25 @@ struct SrcPack {
 };
 
 // acc[p][l][i] = sum_k x_k[p][l][i] * [v_k]_{q_l}: a linear combination of
-// ciphertexts at different levels, each read over the first Ql limbs; the
-// constant residues are formed in-thread from the integral doubles.
+// ciphertexts at different levels, each read over the first Ql limbs. One row
+// (p, l) per blockIdx.y; the constant residues of the row are formed once, from
+// the integral doubles, into LDS.
 __global__ void k_const_lincomb(uint64_t *acc, SrcPack sp, int nterms, const DevMod *mods, size_t Ql, size_t N) {
-    const size_t S = Ql * N, total = 2 * S;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t p = t / S, r = t % S, l = r / N, i = r % N;
+    __shared__ uint64_t cres[kMaxTerms];
+    for (size_t row = blockIdx.y; row < 2 * Ql; row += gridDim.y) {
+        const size_t p = row / Ql, l = row % Ql;
         const DevMod m = mods[l];
-        Acc128 a{0, 0};
-        for (int k = 0; k < nterms; k++) {
-            uint64_t c = residue_of_rounded(sp.v[k], m);
-            c = c >= m.q ? c - m.q : c;
-            mac128(a, sp.x[k][(p * sp.ql[k] + l) * N + i], c);
+        __syncthreads();  // the previous row's readers are done with cres
+        if (threadIdx.x < (unsigned)nterms) {
+            const uint64_t c = residue_of_rounded(sp.v[threadIdx.x], m);
+            cres[threadIdx.x] = c >= m.q ? c - m.q : c;
         }
-        acc[t] = reduce128(a, m);
+        __syncthreads();
+        for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
+            Acc128 a{0, 0};
+            for (int k = 0; k < nterms; k++) mac128(a, sp.x[k][(p * sp.ql[k] + l) * N + i], cres[k]);
+            acc[row * N + i] = reduce128(a, m);
+        }
     }
 }
 
@@ -104,7 +114,7 @@ void const_lincomb(Context &c, Ciphertext &acc, const std::vector<const Cipherte
     acc.ctx = &c;
     acc.resize(2, c.nQ - Ql + 1, s, false);  // k_const_lincomb writes every word
     acc.ntt = true;
-    k_const_lincomb<<<grid_for(2 * Ql * c.N), kBlock, 0, s>>>(acc.data(), sp, (int)src.size(), c.mods(), Ql, c.N);
+    k_const_lincomb<<<rows_grid(2 * Ql, c.N), kBlock, 0, s>>>(acc.data(), sp, (int)src.size(), c.mods(), Ql, c.N);
     SHM_HIP(hipGetLastError());
 }
 
@@ -152,7 +162,7 @@ void lt_apply(Context &c, const LtLayer &L, Ciphertext &ct, const Rotator &r, hi
         inner.resize(ct.size, ct.chain, s, false);  // k_lt_mac writes every word
         inner.ntt = true;
         inner.scale = ct.scale;
-        k_lt_mac<<<grid_for(ct.size * Ql * c.N), kBlock, 0, s>>>(inner.data(), pk, nt, c.mods(), Ql, c.N, ct.size);
+        k_lt_mac<<<rows_grid(ct.size * Ql, c.N), kBlock, 0, s>>>(inner.data(), pk, nt, c.mods(), Ql, c.N, ct.size);
         SHM_HIP(hipGetLastError());
         const long G = ((L.giant_offset(i) % (long)n) + (long)n) % (long)n;
         if (G) r.rot(inner, (int)G);

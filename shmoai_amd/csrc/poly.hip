@@ -15,71 +15,97 @@ inline unsigned grid_for(size_t work) {
     return (unsigned)(g ? g : 1);
 }
 
-__global__ void k_add(uint64_t *a, const uint64_t *b, const DevMod *mods, size_t Ql, size_t N, size_t total) {
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t limb = (t / N) % Ql;
-        a[t] = add_mod(a[t], b[t], mods[limb].q);
+// Row grid for [rows][N] work: blockIdx.y walks the rows (one limb of one
+// polynomial each), blockIdx.x and the thread walk the N words of a row, so
+// the limb of a row is one uniform value per block rather than a 64-bit
+// division per word. Threads take 4 words of a row when the launch is large.
+inline dim3 rows_grid(size_t rows, size_t N) {
+    size_t bx = (N + kBlock - 1) / kBlock;
+    if (rows * bx >= 8192) bx = (bx + 3) / 4;
+    const size_t by = rows == 0 ? 1 : (rows < 65535 ? rows : 65535);
+    return dim3((unsigned)bx, (unsigned)by);
+}
+
+#define SHM_ROWS_LOOP(row, rows) for (size_t row = blockIdx.y; row < (rows); row += gridDim.y)
+#define SHM_COLS_LOOP(i, N) for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (N); i += (size_t)gridDim.x * blockDim.x)
+
+__global__ void k_add(uint64_t *a, const uint64_t *b, const DevMod *mods, size_t Ql, size_t N, size_t rows) {
+    SHM_ROWS_LOOP(row, rows) {
+        const uint64_t q = mods[row % Ql].q;
+        uint64_t *ar = a + row * N;
+        const uint64_t *br = b + row * N;
+        SHM_COLS_LOOP(i, N) ar[i] = add_mod(ar[i], br[i], q);
     }
 }
 
-__global__ void k_sub(uint64_t *a, const uint64_t *b, const DevMod *mods, size_t Ql, size_t N, size_t total) {
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t limb = (t / N) % Ql;
-        a[t] = sub_mod(a[t], b[t], mods[limb].q);
+__global__ void k_sub(uint64_t *a, const uint64_t *b, const DevMod *mods, size_t Ql, size_t N, size_t rows) {
+    SHM_ROWS_LOOP(row, rows) {
+        const uint64_t q = mods[row % Ql].q;
+        uint64_t *ar = a + row * N;
+        const uint64_t *br = b + row * N;
+        SHM_COLS_LOOP(i, N) ar[i] = sub_mod(ar[i], br[i], q);
     }
 }
 
-__global__ void k_negate(uint64_t *a, const DevMod *mods, size_t Ql, size_t N, size_t total) {
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t limb = (t / N) % Ql;
-        a[t] = neg_mod(a[t], mods[limb].q);
+__global__ void k_negate(uint64_t *a, const DevMod *mods, size_t Ql, size_t N, size_t rows) {
+    SHM_ROWS_LOOP(row, rows) {
+        const uint64_t q = mods[row % Ql].q;
+        uint64_t *ar = a + row * N;
+        SHM_COLS_LOOP(i, N) ar[i] = neg_mod(ar[i], q);
     }
 }
 
 // polymath.cu:161 multiply_rns_poly: ct (.) pt per limb, pt broadcast over polys.
-__global__ void k_mul_plain(uint64_t *ct, const uint64_t *pt, const DevMod *mods, size_t Ql, size_t N, size_t total) {
-    const size_t S = Ql * N;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t r = t % S;
-        const DevMod m = mods[r / N];
-        ct[t] = mul_mod(ct[t], pt[r], m);
+__global__ void k_mul_plain(uint64_t *ct, const uint64_t *pt, const DevMod *mods, size_t Ql, size_t N, size_t rows) {
+    SHM_ROWS_LOOP(row, rows) {
+        const size_t limb = row % Ql;
+        const DevMod m = mods[limb];
+        uint64_t *cr = ct + row * N;
+        const uint64_t *pr = pt + limb * N;
+        SHM_COLS_LOOP(i, N) cr[i] = mul_mod(cr[i], pr[i], m);
     }
 }
 
 // polymath.cu:468-503 tensor_prod_2x2_rns_poly (Karatsuba middle term).
 // a and b hold a_limbs and b_limbs limbs per polynomial (>= Ql): an operand at a
 // higher level is used through its first Ql limbs, which is what dropping its
-// top limbs (mod_switch_to) would leave.
+// top limbs (mod_switch_to) would leave. One row per limb.
 __global__ void k_tensor(const uint64_t *a, const uint64_t *b, uint64_t *out, const DevMod *mods, size_t Ql,
                          size_t N, size_t a_limbs, size_t b_limbs) {
     const size_t S = Ql * N, Sa = a_limbs * N, Sb = b_limbs * N;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < S; t += (size_t)gridDim.x * blockDim.x) {
-        const DevMod m = mods[t / N];
-        const uint64_t a0 = a[t], a1 = a[t + Sa], b0 = b[t], b1 = b[t + Sb];
-        const uint64_t d0 = mul_mod(a0, b0, m), d2 = mul_mod(a1, b1, m);
-        uint64_t d1 = mul_mod(a0 + a1, b0 + b1, m);
-        d1 = d1 + 2 * m.q - d0 - d2;
-        d1 = csub(d1, m.q);
-        d1 = csub(d1, m.q);
-        out[t] = d0;
-        out[t + S] = d1;
-        out[t + 2 * S] = d2;
+    SHM_ROWS_LOOP(row, Ql) {
+        const DevMod m = mods[row];
+        SHM_COLS_LOOP(i, N) {
+            const size_t t = row * N + i;
+            const uint64_t a0 = a[t], a1 = a[t + Sa], b0 = b[t], b1 = b[t + Sb];
+            const uint64_t d0 = mul_mod(a0, b0, m), d2 = mul_mod(a1, b1, m);
+            uint64_t d1 = mul_mod(a0 + a1, b0 + b1, m);
+            d1 = d1 + 2 * m.q - d0 - d2;
+            d1 = csub(d1, m.q);
+            d1 = csub(d1, m.q);
+            out[t] = d0;
+            out[t + S] = d1;
+            out[t + 2 * S] = d2;
+        }
     }
 }
 
-// polymath.cu:505-550 tensor_square_2x2_rns_poly.
+// polymath.cu:505-550 tensor_square_2x2_rns_poly. One row per limb.
 __global__ void k_square(const uint64_t *a, uint64_t *out, const DevMod *mods, size_t Ql, size_t N,
                          size_t a_limbs) {
     const size_t S = Ql * N, Sa = a_limbs * N;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < S; t += (size_t)gridDim.x * blockDim.x) {
-        const DevMod m = mods[t / N];
-        const uint64_t c0 = a[t], c1 = a[t + Sa];
-        const uint64_t d0 = mul_mod(c0, c0, m), d2 = mul_mod(c1, c1, m);
-        const uint64_t lo = c0 * c1, hi = __umul64hi(c0, c1);
-        const uint64_t d1 = barrett128((hi << 1) | (lo >> 63), lo << 1, m);
-        out[t] = d0;
-        out[t + S] = d1;
-        out[t + 2 * S] = d2;
+    SHM_ROWS_LOOP(row, Ql) {
+        const DevMod m = mods[row];
+        SHM_COLS_LOOP(i, N) {
+            const size_t t = row * N + i;
+            const uint64_t c0 = a[t], c1 = a[t + Sa];
+            const uint64_t d0 = mul_mod(c0, c0, m), d2 = mul_mod(c1, c1, m);
+            const uint64_t lo = c0 * c1, hi = __umul64hi(c0, c1);
+            const uint64_t d1 = barrett128((hi << 1) | (lo >> 63), lo << 1, m);
+            out[t] = d0;
+            out[t + S] = d1;
+            out[t + 2 * S] = d2;
+        }
     }
 }
 
@@ -92,49 +118,59 @@ __global__ void k_gather_limb(const uint64_t *in, uint64_t *out, size_t Ql, size
     }
 }
 
-// out[p][i] = in.p[p][i] for i < words: polynomials of separate buffers into one.
+// out[p][i] = in.p[p][i] for i < words: polynomials of separate buffers into one
+// (one row per polynomial).
 __global__ void k_gather_polys(PolyPack in, uint64_t *__restrict__ out, size_t words, size_t npolys) {
-    const size_t total = npolys * words;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x)
-        out[t] = in.p[t / words][t % words];
-}
-
-// galois.cu:8-20 apply_galois_ntt_permutation.
-__global__ void k_permute(const uint64_t *in, uint64_t *out, const uint32_t *table, size_t N, size_t total) {
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t limb = t / N;
-        out[t] = in[limb * N + table[t % N]];
+    SHM_ROWS_LOOP(p, npolys) {
+        const uint64_t *src = in.p[p];
+        uint64_t *dst = out + p * words;
+        SHM_COLS_LOOP(i, words) dst[i] = src[i];
     }
 }
 
+// galois.cu:8-20 apply_galois_ntt_permutation (one row per limb).
+__global__ void k_permute(const uint64_t *in, uint64_t *out, const uint32_t *table, size_t N, size_t nlimbs) {
+    SHM_ROWS_LOOP(limb, nlimbs) {
+        const uint64_t *src = in + limb * N;
+        uint64_t *dst = out + limb * N;
+        SHM_COLS_LOOP(i, N) dst[i] = src[table[i]];
+    }
+}
+
+// The first out_limbs limbs of each polynomial (one row per output limb).
 __global__ void k_copy_limbs(const uint64_t *in, size_t in_limbs, uint64_t *out, size_t out_limbs, size_t N,
                              size_t npolys) {
-    const size_t total = npolys * out_limbs * N;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t p = t / (out_limbs * N), r = t % (out_limbs * N);
-        out[t] = in[p * in_limbs * N + r];
+    SHM_ROWS_LOOP(row, npolys * out_limbs) {
+        const size_t p = row / out_limbs, l = row % out_limbs;
+        const uint64_t *src = in + (p * in_limbs + l) * N;
+        uint64_t *dst = out + row * N;
+        SHM_COLS_LOOP(i, N) dst[i] = src[i];
     }
 }
 
 // a (+)= / (*)= the constant [round(value * scale)]_q of every limb, the residue
 // being what encode_scalar writes into each slot of the constant plaintext (so
 // the result equals add_plain / multiply_plain with that plaintext).
-__global__ void k_add_scalar(uint64_t *a, const DevMod *mods, double v, size_t N, size_t total) {
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const DevMod m = mods[t / N];
+// One row per limb; the residue (and for the product its Shoup companion) is
+// formed once per row.
+__global__ void k_add_scalar(uint64_t *a, const DevMod *mods, double v, size_t N, size_t Ql) {
+    SHM_ROWS_LOOP(row, Ql) {
+        const DevMod m = mods[row];
         uint64_t r = residue_of_rounded(v, m);
         r = r >= m.q ? r - m.q : r;
-        a[t] = add_mod(a[t], r, m.q);
+        uint64_t *ar = a + row * N;
+        SHM_COLS_LOOP(i, N) ar[i] = add_mod(ar[i], r, m.q);
     }
 }
 
-__global__ void k_mul_scalar(uint64_t *a, const DevMod *mods, double v, size_t Ql, size_t N, size_t total) {
-    const size_t S = Ql * N;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const DevMod m = mods[(t % S) / N];
+__global__ void k_mul_scalar(uint64_t *a, const DevMod *mods, double v, size_t Ql, size_t N, size_t rows) {
+    SHM_ROWS_LOOP(row, rows) {
+        const DevMod m = mods[row % Ql];
         uint64_t r = residue_of_rounded(v, m);
         r = r >= m.q ? r - m.q : r;
-        a[t] = mul_mod(a[t], r, m);
+        const uint64_t rs = shoup_companion(r, m);
+        uint64_t *ar = a + row * N;
+        SHM_COLS_LOOP(i, N) ar[i] = mul_shoup(ar[i], r, rs, m.q);
     }
 }
 
@@ -244,26 +280,22 @@ __global__ void k_mul_elementwise(const uint64_t *a, const uint64_t *b, uint64_t
 }  // namespace
 
 void launch_add(const Context &c, uint64_t *a, const uint64_t *b, size_t Ql, size_t npolys, hipStream_t s) {
-    const size_t total = npolys * Ql * c.N;
-    k_add<<<grid_for(total), kBlock, 0, s>>>(a, b, c.mods(), Ql, c.N, total);
+    k_add<<<rows_grid(npolys * Ql, c.N), kBlock, 0, s>>>(a, b, c.mods(), Ql, c.N, npolys * Ql);
     SHM_HIP(hipGetLastError());
 }
 
 void launch_sub(const Context &c, uint64_t *a, const uint64_t *b, size_t Ql, size_t npolys, hipStream_t s) {
-    const size_t total = npolys * Ql * c.N;
-    k_sub<<<grid_for(total), kBlock, 0, s>>>(a, b, c.mods(), Ql, c.N, total);
+    k_sub<<<rows_grid(npolys * Ql, c.N), kBlock, 0, s>>>(a, b, c.mods(), Ql, c.N, npolys * Ql);
     SHM_HIP(hipGetLastError());
 }
 
 void launch_negate(const Context &c, uint64_t *a, size_t Ql, size_t npolys, hipStream_t s) {
-    const size_t total = npolys * Ql * c.N;
-    k_negate<<<grid_for(total), kBlock, 0, s>>>(a, c.mods(), Ql, c.N, total);
+    k_negate<<<rows_grid(npolys * Ql, c.N), kBlock, 0, s>>>(a, c.mods(), Ql, c.N, npolys * Ql);
     SHM_HIP(hipGetLastError());
 }
 
 void launch_mul_plain(const Context &c, uint64_t *ct, const uint64_t *pt, size_t Ql, size_t npolys, hipStream_t s) {
-    const size_t total = npolys * Ql * c.N;
-    k_mul_plain<<<grid_for(total), kBlock, 0, s>>>(ct, pt, c.mods(), Ql, c.N, total);
+    k_mul_plain<<<rows_grid(npolys * Ql, c.N), kBlock, 0, s>>>(ct, pt, c.mods(), Ql, c.N, npolys * Ql);
     SHM_HIP(hipGetLastError());
 }
 
@@ -272,14 +304,14 @@ void launch_tensor(const Context &c, const uint64_t *a, const uint64_t *b, uint6
     if (!a_limbs) a_limbs = Ql;
     if (!b_limbs) b_limbs = Ql;
     if (a_limbs < Ql || b_limbs < Ql) throw LogicError("tensor operand below the product level");
-    k_tensor<<<grid_for(Ql * c.N), kBlock, 0, s>>>(a, b, out3, c.mods(), Ql, c.N, a_limbs, b_limbs);
+    k_tensor<<<rows_grid(Ql, c.N), kBlock, 0, s>>>(a, b, out3, c.mods(), Ql, c.N, a_limbs, b_limbs);
     SHM_HIP(hipGetLastError());
 }
 
 void launch_square(const Context &c, const uint64_t *a, uint64_t *out3, size_t Ql, hipStream_t s, size_t a_limbs) {
     if (!a_limbs) a_limbs = Ql;
     if (a_limbs < Ql) throw LogicError("square operand below the product level");
-    k_square<<<grid_for(Ql * c.N), kBlock, 0, s>>>(a, out3, c.mods(), Ql, c.N, a_limbs);
+    k_square<<<rows_grid(Ql, c.N), kBlock, 0, s>>>(a, out3, c.mods(), Ql, c.N, a_limbs);
     SHM_HIP(hipGetLastError());
 }
 
@@ -304,20 +336,19 @@ void launch_rescale(const Context &c, const uint64_t *in, uint64_t *out, size_t 
 void launch_gather_polys(const Context &c, const PolyPack &in, uint64_t *out, size_t words, size_t npolys,
                          hipStream_t s) {
     if (npolys == 0 || npolys > (size_t)kPolyPack) throw InvalidArgument("gather: 1..64 polynomials");
-    k_gather_polys<<<grid_for(npolys * words), kBlock, 0, s>>>(in, out, words, npolys);
+    k_gather_polys<<<rows_grid(npolys, words), kBlock, 0, s>>>(in, out, words, npolys);
     SHM_HIP(hipGetLastError());
 }
 
 void launch_permute(const Context &c, const uint64_t *in, uint64_t *out, const uint32_t *table, size_t nlimbs,
                     hipStream_t s) {
-    const size_t total = nlimbs * c.N;
-    k_permute<<<grid_for(total), kBlock, 0, s>>>(in, out, table, c.N, total);
+    k_permute<<<rows_grid(nlimbs, c.N), kBlock, 0, s>>>(in, out, table, c.N, nlimbs);
     SHM_HIP(hipGetLastError());
 }
 
 void launch_copy_limbs(const Context &c, const uint64_t *in, size_t in_limbs, uint64_t *out, size_t out_limbs,
                        size_t npolys, hipStream_t s) {
-    k_copy_limbs<<<grid_for(npolys * out_limbs * c.N), kBlock, 0, s>>>(in, in_limbs, out, out_limbs, c.N, npolys);
+    k_copy_limbs<<<rows_grid(npolys * out_limbs, c.N), kBlock, 0, s>>>(in, in_limbs, out, out_limbs, c.N, npolys);
     SHM_HIP(hipGetLastError());
 }
 
@@ -344,13 +375,12 @@ void launch_copy_words(uint64_t *dst, const uint64_t *src, size_t words, hipStre
 }
 
 void launch_add_scalar(const Context &c, uint64_t *a, double rounded, size_t Ql, hipStream_t s) {
-    k_add_scalar<<<grid_for(Ql * c.N), kBlock, 0, s>>>(a, c.mods(), rounded, c.N, Ql * c.N);
+    k_add_scalar<<<rows_grid(Ql, c.N), kBlock, 0, s>>>(a, c.mods(), rounded, c.N, Ql);
     SHM_HIP(hipGetLastError());
 }
 
 void launch_mul_scalar(const Context &c, uint64_t *a, double rounded, size_t Ql, size_t npolys, hipStream_t s) {
-    const size_t total = npolys * Ql * c.N;
-    k_mul_scalar<<<grid_for(total), kBlock, 0, s>>>(a, c.mods(), rounded, Ql, c.N, total);
+    k_mul_scalar<<<rows_grid(npolys * Ql, c.N), kBlock, 0, s>>>(a, c.mods(), rounded, Ql, c.N, npolys * Ql);
     SHM_HIP(hipGetLastError());
 }
 
