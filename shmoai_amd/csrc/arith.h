@@ -4,6 +4,9 @@
 // products lower to 32-bit v_mul_hi/v_mad_u64_u32 chains. Every helper keeps
 // operands below 2^62 so that lazy [0, 4q) butterflies never wrap (all primes
 // are at most 61 bits).
+//
+// The ring degree N is a power of two (2^10 .. 2^17), so kernels split a flat
+// index over [..][N] with a shift and a mask rather than a 64-bit division.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -126,7 +126,7 @@ __global__ void k_decompose(const double2 *buf, uint64_t *pt, const DevMod *mods
 // Constant plaintext in NTT form: every slot holds [round(fl(value*scale))]_q.
 __global__ void k_fill_scalar(uint64_t *pt, const DevMod *mods, double v, size_t N, size_t total) {
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const DevMod m = mods[t / N];
+        const DevMod m = mods[t >> __builtin_ctzll(N)];
         uint64_t r = residue_of_rounded(v, m);
         pt[t] = r >= m.q ? r - m.q : r;
     }

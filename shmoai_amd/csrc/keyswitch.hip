@@ -24,6 +24,15 @@ inline unsigned grid_for(size_t work) {
     return (unsigned)(g ? g : 1);
 }
 
+// [rows][N] launches: blockIdx.y is the row (one limb of one polynomial), so a
+// row's limb is uniform per block; 4 words per thread when the launch is large.
+inline dim3 rows_grid(size_t rows, size_t N) {
+    size_t bx = (N + kBlock - 1) / kBlock;
+    if (rows * bx >= 8192) bx = (bx + 3) / 4;
+    const size_t by = rows == 0 ? 1 : (rows < 65535 ? rows : 65535);
+    return dim3((unsigned)bx, (unsigned)by);
+}
+
 __device__ __forceinline__ uint32_t qlp_prime(size_t k, size_t Ql, size_t nQ) {
     return (uint32_t)(k < Ql ? k : nQ + (k - Ql));
 }
@@ -49,7 +58,7 @@ __global__ void k_modup_bconv(const uint64_t *__restrict__ tcoef_all, uint64_t *
     const size_t QlP = Ql + alpha;
     const size_t total = members * beta * N;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t gm = __builtin_amdgcn_readfirstlane((uint32_t)(t / N)), j = t % N;
+        const size_t gm = __builtin_amdgcn_readfirstlane((uint32_t)(t >> __builtin_ctzll(N))), j = t & (N - 1);
         const size_t mem = gm / beta, g = gm % beta;
         const uint64_t *tcoef = tcoef_all + mem * Ql * N;
         const size_t start = g * alpha, size = min(alpha, Ql - start);
@@ -98,7 +107,7 @@ __global__ void k_ks_inner(const uint64_t *__restrict__ d, const uint64_t *__res
                            size_t QP, size_t alpha, size_t beta, size_t N) {
     const size_t QlP = Ql + alpha, total = QlP * N;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t k = t / N, j = t % N;
+        const size_t k = t >> __builtin_ctzll(N), j = t & (N - 1);
         const uint32_t pi = qlp_prime(k, Ql, nQ);
         Acc128 a0{0, 0}, a1{0, 0};
         for (size_t g = 0; g < beta; g++) {
@@ -132,7 +141,7 @@ __global__ __launch_bounds__(256) void k_ks_inner_multi(const uint64_t *__restri
     const size_t t = ((size_t)blockIdx.x << lgj) + (threadIdx.x & ((1u << lgj) - 1));
     const int os = (int)(threadIdx.x >> lgj);
     if (t >= total || os >= nout) return;
-    const size_t k = t / N, j = t % N;
+    const size_t k = t >> __builtin_ctzll(N), j = t & (N - 1);
     const uint32_t pi = qlp_prime(k, Ql, nQ);
     const size_t kl = (local ? k : (size_t)pi) * N + j;
     // hoisted rotation: the digits are read through the NTT-domain Galois permutation
@@ -164,7 +173,7 @@ __global__ void k_moddown_bconv(const uint64_t *__restrict__ cx, uint64_t *__res
                                 size_t npolys, int rounded) {
     const size_t QlP = Ql + nP, total = npolys * N;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t p = t / N, j = t % N;
+        const size_t p = t >> __builtin_ctzll(N), j = t & (N - 1);
         const uint64_t *cxP = cx + (p * QlP + Ql) * N + j;
         uint64_t *d = delta + p * Ql * N + j;
         uint64_t y[kMaxAlpha];
@@ -221,7 +230,7 @@ __global__ __launch_bounds__(256) void k_ks_inner_batch(const uint64_t *__restri
     const size_t t = ((size_t)blockIdx.x << lgj) + (threadIdx.x & ((1u << lgj) - 1));
     const int ms = (int)(threadIdx.x >> lgj);
     if (t >= total || ms >= members) return;
-    const size_t k = t / N, j = t % N;
+    const size_t k = t >> __builtin_ctzll(N), j = t & (N - 1);
     const size_t src = perm ? k * N + perm[j] : t;
     const uint32_t pi = qlp_prime(k, Ql, nQ);
     uint64_t kb[MAXB], ka[MAXB];
@@ -260,13 +269,18 @@ __global__ void k_moddown_combine_pack(const uint64_t *__restrict__ cx, const ui
                                        PolyPack out, const DevMod *__restrict__ mods, const uint64_t *__restrict__ pinv,
                                        const uint64_t *__restrict__ pinv_s, size_t Ql, size_t nP, size_t N,
                                        size_t npolys, uint64_t acc_mask) {
-    const size_t S = Ql * N, total = npolys * S;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t p = t / S, r = t % S, i = r / N;
-        const uint64_t q = mods[i].q;
-        const uint64_t v = mul_shoup(sub_mod(cx[p * (S + nP * N) + r], delta[t], q), pinv[i], pinv_s[i], q);
-        uint64_t *o = out.p[p] + r;
-        *o = ((acc_mask >> p) & 1) ? add_mod(*o, v, q) : v;
+    const size_t S = Ql * N;
+    for (size_t row = blockIdx.y; row < npolys * Ql; row += gridDim.y) {
+        const size_t p = row / Ql, i = row % Ql;
+        const uint64_t q = mods[i].q, w = pinv[i], ws = pinv_s[i];
+        const uint64_t *cr = cx + p * (S + nP * N) + i * N;
+        const uint64_t *dr = delta + row * N;
+        uint64_t *o = out.p[p] + i * N;
+        const bool acc = (acc_mask >> p) & 1;
+        for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < N; j += (size_t)gridDim.x * blockDim.x) {
+            const uint64_t v = mul_shoup(sub_mod(cr[j], dr[j], q), w, ws, q);
+            o[j] = acc ? add_mod(o[j], v, q) : v;
+        }
     }
 }
 
@@ -323,7 +337,7 @@ void ks_moddown_pack(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, c
         c.phat_mod.as<uint64_t>(), c.pmod.as<uint64_t>(), Ql, c.nQ, nP, N, npolys, rounded ? 1 : 0);
     SHM_HIP(hipGetLastError());
     ntt_forward(c, delta.as<uint64_t>(), npolys, Ql, 0, s);
-    k_moddown_combine_pack<<<grid_for(npolys * Ql * N), kBlock, 0, s>>>(cx, delta.as<uint64_t>(), out, c.mods(),
+    k_moddown_combine_pack<<<rows_grid(npolys * Ql, N), kBlock, 0, s>>>(cx, delta.as<uint64_t>(), out, c.mods(),
                                                                         c.pinv.as<uint64_t>(),
                                                                         c.pinv_s.as<uint64_t>(), Ql, nP, N, npolys,
                                                                         acc_mask);

@@ -136,7 +136,7 @@ __global__ void k_mask_mac(XBlock xb, int jb, const uint64_t *__restrict__ pt, i
                            const DevMod *__restrict__ mods, size_t Ql, size_t N, int first) {
     const size_t S = Ql * N;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < S; t += (size_t)gridDim.x * blockDim.x) {
-        const DevMod m = mods[t / N];
+        const DevMod m = mods[t >> __builtin_ctzll(N)];
         uint64_t x0[kMaskJ], x1[kMaskJ];
 #pragma unroll
         for (int jj = 0; jj < kMaskJ; jj++) {
@@ -179,7 +179,7 @@ __global__ void k_mask_weights(const double *W, const double *fixj, double *w, d
 __global__ void k_gather_last(const uint64_t *acc, uint64_t *last, size_t Ql, size_t N, size_t npolys) {
     const size_t total = npolys * N;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t p = t / N, j = t % N;
+        const size_t p = t >> __builtin_ctzll(N), j = t & (N - 1);
         last[t] = acc[(p * Ql + Ql - 1) * N + j];
     }
 }

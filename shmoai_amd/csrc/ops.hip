@@ -284,7 +284,7 @@ __global__ void k_modraise(const uint64_t *src0, uint64_t *dst, const DevMod *mo
     const uint64_t q0 = mods[0].q;
     const size_t total = nl * N;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t j = t / N, i = t % N;
+        const size_t j = t >> __builtin_ctzll(N), i = t & (N - 1);
         const DevMod m = mods[j];
         const uint64_t x = src0[i];
         uint64_t v = reduce64(x, m);
@@ -299,7 +299,7 @@ __global__ void k_modraise(const uint64_t *src0, uint64_t *dst, const DevMod *mo
 __global__ void k_mono_half(uint64_t *a, const DevMod *mods, const uint64_t *iota, const uint64_t *iota_s, size_t Ql,
                             size_t N, size_t total, int negate) {
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t limb = (t / N) % Ql, j = t % N;
+        const size_t limb = (t >> __builtin_ctzll(N)) % Ql, j = t & (N - 1);
         const uint64_t q = mods[limb].q;
         uint64_t v = mul_shoup(a[t], iota[limb], iota_s[limb], q);
         const bool neg = (j >= N / 2) != (negate != 0);

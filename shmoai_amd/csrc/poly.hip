@@ -113,7 +113,7 @@ __global__ void k_square(const uint64_t *a, uint64_t *out, const DevMod *mods, s
 __global__ void k_gather_limb(const uint64_t *in, uint64_t *out, size_t Ql, size_t limb, size_t N, size_t npolys) {
     const size_t total = npolys * N;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t p = t / N, j = t % N;
+        const size_t p = t >> __builtin_ctzll(N), j = t & (N - 1);
         out[t] = in[(p * Ql + limb) * N + j];
     }
 }
@@ -195,7 +195,7 @@ __device__ __forceinline__ uint32_t limb_prime(size_t limb, size_t Ql, size_t nQ
 __global__ void k_sample_uniform(uint64_t seed, uint64_t *out, const DevMod *mods, size_t Ql, size_t nQ, size_t N,
                                  size_t total) {
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t limb = t / N, j = t % N;
+        const size_t limb = t >> __builtin_ctzll(N), j = t & (N - 1);
         const uint32_t pi = limb_prime(limb, Ql, nQ);
         const uint64_t ctr = 2 * ((uint64_t)pi * N + j);
         out[t] = barrett128(splitmix64(seed, ctr), splitmix64(seed, ctr + 1), mods[pi]);
@@ -214,7 +214,7 @@ __constant__ uint64_t kCDT[19] = {
 __global__ void k_sample_gaussian(uint64_t seed, uint64_t *out, const DevMod *mods, size_t Ql, size_t nQ, size_t N,
                                   size_t total) {
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t limb = t / N, j = t % N;
+        const size_t limb = t >> __builtin_ctzll(N), j = t & (N - 1);
         const uint64_t u = splitmix64(seed, j), v = u >> 1;
         uint32_t mag = 0;
 #pragma unroll
@@ -227,7 +227,7 @@ __global__ void k_sample_gaussian(uint64_t seed, uint64_t *out, const DevMod *mo
 __global__ void k_signed_to_rns(const int8_t *coeffs, uint64_t *out, const DevMod *mods, size_t Ql, size_t nQ,
                                 size_t N, size_t total) {
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t limb = t / N, j = t % N;
+        const size_t limb = t >> __builtin_ctzll(N), j = t & (N - 1);
         const int64_t v = coeffs[j];
         const uint64_t q = mods[limb_prime(limb, Ql, nQ)].q;
         out[t] = v >= 0 ? (uint64_t)v : q - (uint64_t)(-v);
@@ -239,7 +239,7 @@ __global__ void k_ks_key_digit(const uint64_t *sk, const uint64_t *newkey, uint6
                                const uint64_t *e, const DevMod *mods, const uint64_t *pmod, const uint64_t *pmod_s,
                                size_t dlo, size_t dhi, size_t N, size_t total) {
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t limb = t / N;
+        const size_t limb = t >> __builtin_ctzll(N);
         const DevMod m = mods[limb];
         uint64_t v = neg_mod(add_mod(mul_mod(a[t], sk[t], m), e[t], m.q), m.q);
         if (limb >= dlo && limb < dhi) v = add_mod(v, mul_shoup(newkey[t], pmod[limb], pmod_s[limb], m.q), m.q);
@@ -251,7 +251,7 @@ __global__ void k_ks_key_digit(const uint64_t *sk, const uint64_t *newkey, uint6
 __global__ void k_encrypt_combine(const uint64_t *sk, const uint64_t *a, const uint64_t *e, const uint64_t *pt,
                                   uint64_t *c0, const DevMod *mods, size_t N, size_t total) {
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const DevMod m = mods[t / N];
+        const DevMod m = mods[t >> __builtin_ctzll(N)];
         const uint64_t v = neg_mod(add_mod(mul_mod(a[t], sk[t], m), e[t], m.q), m.q);
         c0[t] = add_mod(v, pt[t], m.q);
     }
@@ -261,7 +261,7 @@ __global__ void k_encrypt_combine(const uint64_t *sk, const uint64_t *a, const u
 __global__ void k_decrypt(const uint64_t *ct, size_t size, const uint64_t *sk, uint64_t *pt, const DevMod *mods,
                           size_t N, size_t S) {
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < S; t += (size_t)gridDim.x * blockDim.x) {
-        const DevMod m = mods[t / N];
+        const DevMod m = mods[t >> __builtin_ctzll(N)];
         const uint64_t s = sk[t];
         uint64_t v = add_mod(ct[t], mul_mod(ct[S + t], s, m), m.q);
         if (size > 2) v = add_mod(v, mul_mod(ct[2 * S + t], mul_mod(s, s, m), m), m.q);
@@ -272,7 +272,7 @@ __global__ void k_decrypt(const uint64_t *ct, size_t size, const uint64_t *sk, u
 __global__ void k_mul_elementwise(const uint64_t *a, const uint64_t *b, uint64_t *out, const DevMod *mods, size_t Ql,
                                   size_t nQ, size_t N, size_t total) {
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const DevMod m = mods[limb_prime(t / N, Ql, nQ)];
+        const DevMod m = mods[limb_prime((t >> __builtin_ctzll(N)), Ql, nQ)];
         out[t] = mul_mod(a[t], b[t], m);
     }
 }

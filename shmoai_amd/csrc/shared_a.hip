@@ -52,7 +52,7 @@ __global__ void k_extend_exact(const uint64_t *__restrict__ coef, uint64_t *__re
                                size_t npolys, size_t in_stride, size_t out_stride) {
     const size_t total = npolys * N;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t p = t / N, j = t % N;
+        const size_t p = t >> __builtin_ctzll(N), j = t & (N - 1);
         const uint64_t *x = coef + p * in_stride;
         double v = 0.0;
         for (size_t g = 0; g < Ql; g++) {
@@ -86,7 +86,7 @@ __global__ void k_fmt_key_entry(const uint64_t *__restrict__ a, const uint64_t *
                                 size_t nQ, size_t nP, size_t N, int diag) {
     const size_t total = (Ql + nP) * N;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t k = t / N, j = t % N;
+        const size_t k = t >> __builtin_ctzll(N), j = t & (N - 1);
         const size_t pi = k < Ql ? k : nQ + (k - Ql);
         const DevMod m = mods[pi];
         const uint64_t as = mul_mod(a[t], s_aux[pi * N + j], m);
