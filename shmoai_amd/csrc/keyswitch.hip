@@ -7,7 +7,8 @@
 // Inner product (eval_key_switch.cu:12-60): over QlP limbs,
 // cx0 = sum_g d_g key_g.b, cx1 = sum_g d_g key_g.a with 128-bit accumulation.
 // ModDown (rns_bconv.cu:776-828, ntt_moddown.cu:106-261): iNTT of the P limbs,
-// delta = fastbconv(P -> Ql), NTT(delta), out = (cx - delta) P^-1 mod q.
+// delta = fastbconv(P -> Ql), NTT(delta) whose last pass writes
+// out = (cx - delta) P^-1 mod q (adding onto the output where asked).
 #include <algorithm>
 
 #include "engine.h"
@@ -22,15 +23,6 @@ inline unsigned grid_for(size_t work) {
     size_t g = (work + kBlock - 1) / kBlock;
     if (g > (1u << 20)) g = 1u << 20;
     return (unsigned)(g ? g : 1);
-}
-
-// [rows][N] launches: blockIdx.y is the row (one limb of one polynomial), so a
-// row's limb is uniform per block; 4 words per thread when the launch is large.
-inline dim3 rows_grid(size_t rows, size_t N) {
-    size_t bx = (N + kBlock - 1) / kBlock;
-    if (rows * bx >= 8192) bx = (bx + 3) / 4;
-    const size_t by = rows == 0 ? 1 : (rows < 65535 ? rows : 65535);
-    return dim3((unsigned)bx, (unsigned)by);
 }
 
 __device__ __forceinline__ uint32_t qlp_prime(size_t k, size_t Ql, size_t nQ) {
@@ -262,27 +254,6 @@ __global__ __launch_bounds__(256) void k_ks_inner_batch(const uint64_t *__restri
     }
 }
 
-// out = (cx - delta) P^-1 mod q, then ct (+)= out, over npolys polynomials:
-// cx [p][QlP][N], delta [p][Ql][N], output polynomial p at out.p[p]; bit p of
-// acc_mask selects accumulation.
-__global__ void k_moddown_combine_pack(const uint64_t *__restrict__ cx, const uint64_t *__restrict__ delta,
-                                       PolyPack out, const DevMod *__restrict__ mods, const uint64_t *__restrict__ pinv,
-                                       const uint64_t *__restrict__ pinv_s, size_t Ql, size_t nP, size_t N,
-                                       size_t npolys, uint64_t acc_mask) {
-    const size_t S = Ql * N;
-    for (size_t row = blockIdx.y; row < npolys * Ql; row += gridDim.y) {
-        const size_t p = row / Ql, i = row % Ql;
-        const uint64_t q = mods[i].q, w = pinv[i], ws = pinv_s[i];
-        const uint64_t *cr = cx + p * (S + nP * N) + i * N;
-        const uint64_t *dr = delta + row * N;
-        uint64_t *o = out.p[p] + i * N;
-        const bool acc = (acc_mask >> p) & 1;
-        for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < N; j += (size_t)gridDim.x * blockDim.x) {
-            const uint64_t v = mul_shoup(sub_mod(cr[j], dr[j], q), w, ws, q);
-            o[j] = acc ? add_mod(o[j], v, q) : v;
-        }
-    }
-}
 
 }  // namespace
 
@@ -336,12 +307,9 @@ void ks_moddown_pack(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, c
         cx, delta.as<uint64_t>(), c.mods(), c.phat_inv.as<uint64_t>(), c.phat_inv_s.as<uint64_t>(),
         c.phat_mod.as<uint64_t>(), c.pmod.as<uint64_t>(), Ql, c.nQ, nP, N, npolys, rounded ? 1 : 0);
     SHM_HIP(hipGetLastError());
-    ntt_forward(c, delta.as<uint64_t>(), npolys, Ql, 0, s);
-    k_moddown_combine_pack<<<rows_grid(npolys * Ql, N), kBlock, 0, s>>>(cx, delta.as<uint64_t>(), out, c.mods(),
-                                                                        c.pinv.as<uint64_t>(),
-                                                                        c.pinv_s.as<uint64_t>(), Ql, nP, N, npolys,
-                                                                        acc_mask);
-    SHM_HIP(hipGetLastError());
+    // the last NTT pass of delta writes (cx - delta) * P^-1 into the outputs
+    moddown_ntt_fused(c, delta.as<uint64_t>(), cx, Ql, QlP, out, npolys, acc_mask, c.pinv.as<uint64_t>(),
+                      c.pinv_s.as<uint64_t>(), s);
 }
 
 void keyswitch(const Context &c, const uint64_t *c2, const uint64_t *key, size_t Ql, uint64_t *ct0, uint64_t *ct1,

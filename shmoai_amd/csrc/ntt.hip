@@ -245,7 +245,10 @@ __device__ __forceinline__ void run_rounds(E *lds, uint32_t gmul, uint32_t gadd_
 //                one coefficient-form limb to other primes, rns.cu:1125-1137).
 //   IO_RESCALE : (phase B, forward) instead of storing v = NTT(delta), store
 //                (acc - v) * qinv mod q to outs[p / ppo] + (p % ppo) L1 N + k N
-//                (rns.cu:1138-1155 fused with the final NTT pass).
+//                (rns.cu:1138-1155 fused with the final NTT pass). With
+//                out_packed, polynomial p reads acc at its acc_limbs stride and
+//                writes out_p[p], adding onto it when bit p of acc_mask is set
+//                (the ModDown combine, rns_bconv.cu:680-690 and 776-828, fused the same way).
 //   IO_ENCODE  : (phase A, forward) the input of limb k of polynomial p is the
 //                rounded coefficient of the encoder's FFT output fft[p] (index
 //                i < N/2: round(re), i >= N/2: round(im) of entry i - N/2) as a
@@ -263,6 +266,8 @@ struct NttIO {
     uint32_t ppo;
     uint64_t center;          // IO_REDUCE: 0, or h = q_last >> 1: values above h stand for v - q_last
     uint32_t packed;          // IO_RESCALE: nonzero -> polynomial p reads acc_p[p], writes out_p[p]
+    uint32_t out_packed;      // IO_RESCALE: nonzero -> polynomial p reads acc, writes out_p[p]
+    uint64_t acc_mask;        // IO_RESCALE with out_packed: bit p set -> out_p[p] += result
     PolyPack acc_p, out_p;
     const double2 *fft;       // IO_ENCODE: [n_polys][N/2]
 };
@@ -391,7 +396,7 @@ __global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ 
     };
     if (FWD && IO == IO_RESCALE) {
         uint64_t *out;
-        if (io.packed) {
+        if (io.packed || io.out_packed) {
             out = io.out_p.p[poly] + (size_t)k * N + t0;
         } else {
             uint64_t *ob = io.outs ? io.outs[poly / io.ppo] : io.out_single;
@@ -401,13 +406,15 @@ __global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ 
         // unreduced outputs stay below (4 + 2 logN) q (< 2^64 for q < 2^58): acc - v is
         // taken as acc + (4 + 2 logN) q - v, which the Shoup product accepts as it is
         const uint64_t kq = (uint64_t)(4 + 2 * logN) * q;
+        const bool addto = io.out_packed && poly < 64 && ((io.acc_mask >> poly) & 1);
 #pragma unroll
         for (int j = 0; j < PER; j++) {
             const int i = threadIdx.x + j * kThreads;
             const int b = i / T, r = i % T;
             const uint64_t raw = lds[b * TP + swz<T>(r)];
-            const uint64_t y = LAZY ? mul_shoup(accv[j] + kq - raw, qi, qis, q)
-                                    : mul_shoup(sub_mod(accv[j], canon(raw), q), qi, qis, q);
+            uint64_t y = LAZY ? mul_shoup(accv[j] + kq - raw, qi, qis, q)
+                              : mul_shoup(sub_mod(accv[j], canon(raw), q), qi, qis, q);
+            if (addto) y = add_mod(y, out[i], q);
             __builtin_nontemporal_store(y, &out[i]);
         }
     } else if (FWD) {
@@ -569,6 +576,22 @@ void rescale_ntt_fused(const Context &c, const uint64_t *last, const uint64_t *a
     io.ppo = (uint32_t)ppo;
     io.center = rounded ? (c.q[L1] >> 1) : 0;
     launch_ntt<true, IO_REDUCE, IO_RESCALE>(c, delta, mp, n_polys * L1, s, io);
+}
+
+void moddown_ntt_fused(const Context &c, uint64_t *delta, const uint64_t *cx, size_t Ql, size_t cx_limbs,
+                       const PolyPack &out, size_t n_polys, uint64_t acc_mask, const uint64_t *pinv,
+                       const uint64_t *pinv_s, hipStream_t s) {
+    if (n_polys == 0 || n_polys > (size_t)kPolyPack) throw InvalidArgument("ModDown: 1..64 polynomials per batch");
+    LimbMap mp{(uint32_t)Ql, 0, 0xffffffffu, 0, 0xffffffffu, (uint32_t)c.nQ, 0, (uint64_t)Ql * c.N};
+    NttIO io{};
+    io.acc = cx;
+    io.acc_limbs = (uint32_t)cx_limbs;
+    io.qinv = pinv;
+    io.qinv_s = pinv_s;
+    io.out_packed = 1;
+    io.acc_mask = acc_mask;
+    io.out_p = out;
+    launch_ntt<true, IO_PLAIN, IO_RESCALE>(c, delta, mp, n_polys * Ql, s, io);
 }
 
 void rescale_ntt_fused_pack(const Context &c, const uint64_t *last, const PolyPack &in, const PolyPack &out,
