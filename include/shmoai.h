@@ -347,6 +347,15 @@ int shm_server_get_stats(shm_server *srv, shm_server_stats *stats);
  * ciphertexts go through each product together (one key switch and rescale pass). */
 int shm_evaluate_chebyshev(shm_context *ctx, shm_ciphertext *const *cts, size_t count, const double *coeffs,
                            size_t n_coeffs, double lo, double hi, const shm_kswitch_key *relin, shm_stream_t stream);
+/* Ct_ct_matrix_mul.cuh:10-181 ct_ct_matrix_mul_colpacking: out[i] (i < row_X) =
+ * rescale(relinearize(sum_{j < col_X} enc_X[j] * rotate(enc_W[j], i * num_batch))), the
+ * diagonal-packed product of two column-packed encrypted matrices. col_X must equal
+ * col_W and row_X row_W (the reference's check); all operands share one level, the X
+ * operands one scale and the W operands one scale; outputs must not alias inputs. */
+int shm_ct_ct_matrix_mul_colpacking(shm_context *ctx, const shm_ciphertext *const *enc_X,
+                                    const shm_ciphertext *const *enc_W, int col_X, int row_X, int col_W, int row_W,
+                                    int num_batch, const shm_galois_keys *gk, const shm_kswitch_key *relin,
+                                    shm_ciphertext *const *out, shm_stream_t stream);
 /* BatchBootstrapper::bootstrap_ordinary_s2c_first (BatchBootstrap.cuh:674-709) over
  * `count` ciphertexts with exactly 4 primes; out[i] receives the refreshed
  * ciphertext at scale out_scale (0 = the input scale). Inputs are not modified. */

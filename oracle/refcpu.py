@@ -348,6 +348,25 @@ class Ref:
         return outs
 
     # bootstrapping pieces -------------------------------------------------------------
+    def ctct_matmul_colpacking(self, X, W, Ql, row_x, num_batch, gal_keys, relin_key):
+        """Ct_ct_matrix_mul.cuh:10-181 ct_ct_matrix_mul_colpacking: output i is
+        rescale(relinearize(sum_j X[j] * rotate(W[j], i * num_batch))), the products
+        formed by multiply (tensor, no relinearisation, Ct_ct_matrix_mul.cuh:125-143)
+        and summed by add_inplace before one relinearize and rescale (147-148).
+        gal_keys maps each rotation step (mod the slot count) to its Galois key; a
+        step of 0 is the identity."""
+        n = self.N // 2
+        out = []
+        for i in range(row_x):
+            step = (i * num_batch) % n
+            acc = None
+            for j in range(len(X)):
+                w = W[j] if step == 0 else self.rotate(W[j], Ql, self.elt(step), gal_keys[step])
+                t = self.tensor(np.ascontiguousarray(X[j], np.uint64), np.ascontiguousarray(w, np.uint64), Ql)
+                acc = t if acc is None else self.add(acc, t, Ql, 3)
+            out.append(self.rescale(self.relinearize(acc, Ql, relin_key), Ql))
+        return out
+
     def modraise(self, ct_low):
         """[2][N] at prime 0 (NTT form) -> [2][nQ][N] (Bootstrapper.cu:3287-3356)."""
         out = np.zeros(2 * self.nQ * self.N, np.uint64)

@@ -616,6 +616,28 @@ def ct_pt_matrix_mul_wo_pre_w_mask(enc_X: Sequence[Ciphertext], W: np.ndarray, b
     return out
 
 
+def ct_ct_matrix_mul_colpacking(enc_X: Sequence[Ciphertext], enc_W: Sequence[Ciphertext], RotK: GaloisKeys,
+                                relin_keys: KSwitchKey, context: Context, col_X: int, row_X: int, col_W: int,
+                                row_W: int, num_batch: int, out=None, stream=None):
+    """Ct_ct_matrix_mul.cuh:10-181: output[i] (i < row_X) =
+    rescale(relinearize(sum_j enc_X[j] * rotate(enc_W[j], i * num_batch))) for the
+    col_X column-packed ciphertexts of X and W. Raises ShmInvalidArgument when
+    col_X != col_W or row_X != row_W (where the reference prints an error)."""
+    if col_X != col_W or row_X != row_W:
+        raise capi.ShmInvalidArgument(capi.SHM_ERR_INVALID_ARGUMENT, "bad dimensions of X or W")
+    if len(enc_X) < col_X or len(enc_W) < col_X:
+        raise capi.ShmInvalidArgument(capi.SHM_ERR_INVALID_ARGUMENT, "enc_X / enc_W hold fewer than col_X ciphertexts")
+    chain = enc_X[0].chain_index
+    if out is None:
+        out = [Ciphertext(context, 2, chain + 1) for _ in range(row_X)]
+    xs = (C.c_void_p * col_X)(*[x.h.value for x in enc_X[:col_X]])
+    ws = (C.c_void_p * col_X)(*[w.h.value for w in enc_W[:col_X]])
+    os_ = (C.c_void_p * row_X)(*[o.h.value for o in out])
+    check(_lib().shm_ct_ct_matrix_mul_colpacking(context.h, xs, ws, col_X, row_X, col_W, row_W, num_batch, RotK.h,
+                                                  relin_keys.h, os_, stream))
+    return out
+
+
 class Stream:
     """A created HIP stream (default priority, non-blocking); pass `.h` as `stream=`."""
 

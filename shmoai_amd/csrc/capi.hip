@@ -1080,6 +1080,33 @@ int shm_evaluate_chebyshev(shm_context *ctx, shm_ciphertext *const *cts, size_t 
     });
 }
 
+int shm_ct_ct_matrix_mul_colpacking(shm_context *ctx, const shm_ciphertext *const *enc_X,
+                                    const shm_ciphertext *const *enc_W, int col_X, int row_X, int col_W, int row_W,
+                                    int num_batch, const shm_galois_keys *gk, const shm_kswitch_key *relin,
+                                    shm_ciphertext *const *out, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(gk, "galois_keys");
+        need(relin, "relin_keys");
+        if (col_X != col_W || row_X != row_W) throw InvalidArgument("ct_ct_matrix_mul: bad dimensions of X or W");
+        if (col_X <= 0 || row_X <= 0) throw InvalidArgument("ct_ct_matrix_mul: dimensions must be positive");
+        if (!enc_X || !enc_W || !out) throw InvalidArgument("enc_X/enc_W/out is null");
+        std::vector<const Ciphertext *> xs(col_X), ws(col_X);
+        for (int j = 0; j < col_X; j++) {
+            need(enc_X[j], "enc_X[j]");
+            need(enc_W[j], "enc_W[j]");
+            xs[j] = &enc_X[j]->ct;
+            ws[j] = &enc_W[j]->ct;
+        }
+        std::vector<Ciphertext *> os(row_X);
+        for (int i = 0; i < row_X; i++) {
+            need(out[i], "out[i]");
+            os[i] = &out[i]->ct;
+        }
+        ctct_matmul_colpacking(*ctx->c, xs, ws, num_batch, gk->g, relin->k, os, S(st));
+    });
+}
+
 int shm_bootstrap_s2c_first(shm_bootstrapper *b, const shm_ciphertext *const *in, size_t count,
                             shm_ciphertext *const *out, const shm_galois_keys *gk, const shm_kswitch_key *relin,
                             double out_scale, shm_stream_t st) {

@@ -267,7 +267,7 @@ void launch_mul_plain(const Context &c, uint64_t *ct, const uint64_t *pt, size_t
 // operands with more limbs than Ql (a_limbs / b_limbs per polynomial, 0 = Ql) are
 // used through their first Ql limbs
 void launch_tensor(const Context &c, const uint64_t *a, const uint64_t *b, uint64_t *out3, size_t Ql, hipStream_t s,
-                   size_t a_limbs = 0, size_t b_limbs = 0);
+                   size_t a_limbs = 0, size_t b_limbs = 0, bool accumulate = false);
 void launch_square(const Context &c, const uint64_t *a, uint64_t *out3, size_t Ql, hipStream_t s,
                    size_t a_limbs = 0);
 void launch_rescale(const Context &c, const uint64_t *in, uint64_t *out, size_t Ql, size_t npolys, hipStream_t s,
@@ -406,6 +406,11 @@ void ct_mul_relin_rescale(Context &c, Ciphertext &out, const Ciphertext &a, cons
 void mul_relin_rescale_batch(Context &c, const std::vector<Ciphertext *> &out,
                              const std::vector<const Ciphertext *> &a, const std::vector<const Ciphertext *> &b,
                              const KSwitchKey &rk, hipStream_t s, bool rounded = false);
+// Ct_ct_matrix_mul.cuh:10-181 ct_ct_matrix_mul_colpacking: out[i] =
+// rescale(relinearize(sum_j X[j] * rot(W[j], i * num_batch))) for i < out.size().
+void ctct_matmul_colpacking(Context &c, const std::vector<const Ciphertext *> &X,
+                            const std::vector<const Ciphertext *> &W, int num_batch, const GaloisKeys &gk,
+                            const KSwitchKey &rk, const std::vector<Ciphertext *> &out, hipStream_t s);
 // a <- a * [v encoded at scale sigma] (no rescale)
 void ct_mul_const(Context &c, Ciphertext &a, double v, double sigma, hipStream_t s);
 void ct_mul_monomial_half(Context &c, Ciphertext &a, bool negate, hipStream_t s);

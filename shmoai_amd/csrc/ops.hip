@@ -466,6 +466,77 @@ void mul_relin_rescale_batch(Context &c, const std::vector<Ciphertext *> &out,
     rescale_batch(c, out, s, rounded);
 }
 
+// Ct_ct_matrix_mul.cuh:10-181 ct_ct_matrix_mul_colpacking. Output i is
+// rescale(relinearize(sum_j X[j] * rot(W[j], i * num_batch))): the reference
+// multiplies (tensor, no relinearisation), adds the col_X products and then
+// relinearises and rescales once, and so does this. Each rotation is the
+// reference's rotate_vector (permutation, then key switch of the permuted c1),
+// so the result is the same limb for limb. Every product of an output
+// accumulates into its 3-polynomial ciphertext in the tensor kernel, and the
+// outputs are relinearised and rescaled in batched passes of up to 16. A step
+// that is a multiple of the slot count is the identity.
+void ctct_matmul_colpacking(Context &c, const std::vector<const Ciphertext *> &X,
+                            const std::vector<const Ciphertext *> &W, int num_batch, const GaloisKeys &gk,
+                            const KSwitchKey &rk, const std::vector<Ciphertext *> &out, hipStream_t s) {
+    const size_t colX = X.size(), rowX = out.size();
+    if (colX == 0 || W.size() != colX) throw InvalidArgument("ct_ct_matrix_mul: X and W need col_X ciphertexts each");
+    if (rowX == 0) throw InvalidArgument("ct_ct_matrix_mul: row_X must be positive");
+    if (num_batch < 0) throw InvalidArgument("ct_ct_matrix_mul: num_batch must be non-negative");
+    const size_t chain = X[0]->chain;
+    for (size_t j = 0; j < colX; j++) {
+        for (const Ciphertext *x : {X[j], W[j]}) {
+            if (!x || x->size != 2 || !x->ntt) throw InvalidArgument("ct_ct_matrix_mul: size-2 NTT-form ciphertexts");
+            if (x->chain != chain) throw InvalidArgument("ct_ct_matrix_mul: operands at different levels");
+        }
+        if (X[j]->scale != X[0]->scale || W[j]->scale != W[0]->scale)
+            throw InvalidArgument("ct_ct_matrix_mul: operands at different scales");
+    }
+    for (size_t i = 0; i < rowX; i++) {
+        if (!out[i]) throw InvalidArgument("ct_ct_matrix_mul: null output");
+        for (size_t j = 0; j < colX; j++)
+            if (out[i] == X[j] || out[i] == W[j]) throw InvalidArgument("ct_ct_matrix_mul: output aliases an input");
+        for (size_t k = 0; k < i; k++)
+            if (out[k] == out[i]) throw InvalidArgument("ct_ct_matrix_mul: an output appears twice");
+    }
+    const size_t Ql = c.ql_of_chain(chain), slots = c.N / 2;
+    if (Ql < 2) throw LogicError("end of modulus switching chain reached");
+    constexpr size_t kChunk = 16;
+    Ciphertext rot;
+    for (size_t i0 = 0; i0 < rowX; i0 += kChunk) {
+        const size_t i1 = std::min(rowX, i0 + kChunk), m = i1 - i0;
+        for (size_t i = i0; i < i1; i++) {
+            Ciphertext &o = *out[i];
+            o.ctx = &c;
+            o.resize(3, chain, s, false);  // the first product writes every word
+            o.ntt = true;
+            o.scale = X[0]->scale * W[0]->scale;
+        }
+        for (size_t i = i0; i < i1; i++) {
+            const size_t step = (i * (size_t)num_batch) % slots;
+            for (size_t j = 0; j < colX; j++) {
+                const Ciphertext *w = W[j];
+                if (step != 0) {
+                    ct_clone(rot, *W[j], s);
+                    rotate_internal(c, rot, (int)step, gk, s);
+                    w = &rot;
+                }
+                launch_tensor(c, X[j]->data(), w->data(), out[i]->data(), Ql, s, X[j]->Ql(), w->Ql(), j > 0);
+            }
+        }
+        PolyPack c2{}, ct01{};
+        std::vector<Ciphertext *> chunk(out.begin() + i0, out.begin() + i1);
+        for (size_t i = 0; i < m; i++) {
+            c2.p[i] = chunk[i]->poly(2);
+            ct01.p[2 * i] = chunk[i]->poly(0);
+            ct01.p[2 * i + 1] = chunk[i]->poly(1);
+        }
+        const uint64_t all = (uint64_t(1) << (2 * m)) - 1;
+        keyswitch_batch(c, c2, m, rk.k.as<uint64_t>(), Ql, ct01, all, s, false);  // relinearise: add to both
+        for (Ciphertext *o : chunk) o->size = 2;
+        rescale_batch(c, chunk, s, false);
+    }
+}
+
 void ct_mul_monomial_half(Context &c, Ciphertext &a, bool negate, hipStream_t s) {
     const size_t total = a.size * a.Ql() * c.N;
     k_mono_half<<<grid_ops(total), kBlockOps, 0, s>>>(a.data(), c.mods(), c.iota.as<uint64_t>(),

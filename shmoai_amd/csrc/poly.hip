@@ -69,9 +69,11 @@ __global__ void k_mul_plain(uint64_t *ct, const uint64_t *pt, const DevMod *mods
 // polymath.cu:468-503 tensor_prod_2x2_rns_poly (Karatsuba middle term).
 // a and b hold a_limbs and b_limbs limbs per polynomial (>= Ql): an operand at a
 // higher level is used through its first Ql limbs, which is what dropping its
-// top limbs (mod_switch_to) would leave. One row per limb.
+// top limbs (mod_switch_to) would leave. One row per limb. With `accumulate`
+// the three products are added onto out (a sum of tensor products, as
+// multiply + add_inplace of 3-polynomial ciphertexts).
 __global__ void k_tensor(const uint64_t *a, const uint64_t *b, uint64_t *out, const DevMod *mods, size_t Ql,
-                         size_t N, size_t a_limbs, size_t b_limbs) {
+                         size_t N, size_t a_limbs, size_t b_limbs, int accumulate) {
     const size_t S = Ql * N, Sa = a_limbs * N, Sb = b_limbs * N;
     SHM_ROWS_LOOP(row, Ql) {
         const DevMod m = mods[row];
@@ -83,9 +85,15 @@ __global__ void k_tensor(const uint64_t *a, const uint64_t *b, uint64_t *out, co
             d1 = d1 + 2 * m.q - d0 - d2;
             d1 = csub(d1, m.q);
             d1 = csub(d1, m.q);
-            out[t] = d0;
-            out[t + S] = d1;
-            out[t + 2 * S] = d2;
+            if (accumulate) {
+                out[t] = add_mod(out[t], d0, m.q);
+                out[t + S] = add_mod(out[t + S], d1, m.q);
+                out[t + 2 * S] = add_mod(out[t + 2 * S], d2, m.q);
+            } else {
+                out[t] = d0;
+                out[t + S] = d1;
+                out[t + 2 * S] = d2;
+            }
         }
     }
 }
@@ -300,11 +308,12 @@ void launch_mul_plain(const Context &c, uint64_t *ct, const uint64_t *pt, size_t
 }
 
 void launch_tensor(const Context &c, const uint64_t *a, const uint64_t *b, uint64_t *out3, size_t Ql, hipStream_t s,
-                   size_t a_limbs, size_t b_limbs) {
+                   size_t a_limbs, size_t b_limbs, bool accumulate) {
     if (!a_limbs) a_limbs = Ql;
     if (!b_limbs) b_limbs = Ql;
     if (a_limbs < Ql || b_limbs < Ql) throw LogicError("tensor operand below the product level");
-    k_tensor<<<rows_grid(Ql, c.N), kBlock, 0, s>>>(a, b, out3, c.mods(), Ql, c.N, a_limbs, b_limbs);
+    k_tensor<<<rows_grid(Ql, c.N), kBlock, 0, s>>>(a, b, out3, c.mods(), Ql, c.N, a_limbs, b_limbs,
+                                                   accumulate ? 1 : 0);
     SHM_HIP(hipGetLastError());
 }
 

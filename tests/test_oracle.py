@@ -302,6 +302,28 @@ def test_masked_matmul_oracle_decrypts_to_masked_sums(oracle_lib):
         assert np.max(np.abs(v - want)) < 1e-4
 
 
+def test_ctct_matmul_oracle_decrypts_to_rotated_products(oracle_lib):
+    """The column-packing ct-ct product (Ct_ct_matrix_mul.cuh:10-181): output i
+    decrypts to sum_j x_j * roll(w_j, -i * num_batch) (slot s holds
+    sum_j x_j[s] w_j[s + i num_batch]), at scale Delta^2 / q_last."""
+    r = oracle_lib.Ref(11, [60, 40, 40, 60], 1)
+    s = r.sk_ntt(r.sample_ternary(61, 64))
+    rng = np.random.default_rng(62)
+    n, col_x, row_x, nb, Ql, D = r.N // 2, 3, 4, 3, 3, 2.0 ** 40
+    xs = [rng.uniform(-1, 1, n) for _ in range(col_x)]
+    ws = [rng.uniform(-1, 1, n) for _ in range(col_x)]
+    X = [r.encrypt_symmetric(s, r.encode(x, Ql, D), Ql, 70 + j) for j, x in enumerate(xs)]
+    W = [r.encrypt_symmetric(s, r.encode(w, Ql, D), Ql, 80 + j) for j, w in enumerate(ws)]
+    gk = {i * nb: r.gen_galois_key(s, r.elt(i * nb), r.splitmix64(9, r.elt(i * nb))) for i in range(1, row_x)}
+    rk = r.gen_relin_key(s, 5)
+    out = r.ctct_matmul_colpacking(X, W, Ql, row_x, nb, gk, rk)
+    scale = D * D / float(r.primes[Ql - 1])
+    for i, o in enumerate(out):
+        v = r.decode(r.decrypt(s, o, 2, Ql - 1), Ql - 1, scale)[0]
+        want = sum(xs[j] * np.roll(ws[j], -i * nb) for j in range(col_x))
+        assert np.max(np.abs(v - want)) < 1e-4, i
+
+
 def test_golden_fixtures(oracle_lib):
     """Committed vectors written by tests/golden/make_fixtures.py (pure Python)."""
     with open(os.path.join(GOLDEN, "fixtures.json")) as f:
