@@ -356,6 +356,16 @@ int shm_ct_ct_matrix_mul_colpacking(shm_context *ctx, const shm_ciphertext *cons
                                     const shm_ciphertext *const *enc_W, int col_X, int row_X, int col_W, int row_W,
                                     int num_batch, const shm_galois_keys *gk, const shm_kswitch_key *relin,
                                     shm_ciphertext *const *out, shm_stream_t stream);
+/* Ct_ct_matrix_mul.cuh:182-609 ct_ct_matrix_mul_diagpacking (baby-step giant-step,
+ * g = ceil(sqrt(col_X)) babies): out[i] (i < col_W) = sum_j rotate(giant_j, j g num_batch)
+ * with giant_j = rescale(relinearize(sum_k rotate(enc_W[i], k num_batch) *
+ * rotate(enc_X[j g + k], (col_X - j g) num_batch))), labelled with enc_X's scale as in
+ * the reference. Reads enc_X[0 .. col_X); row_X must be at least col_X (the reference
+ * indexes its pre-rotated X by col_X); row_W is not used, as in the reference. */
+int shm_ct_ct_matrix_mul_diagpacking(shm_context *ctx, const shm_ciphertext *const *enc_X,
+                                     const shm_ciphertext *const *enc_W, int col_X, int row_X, int col_W, int row_W,
+                                     int num_batch, const shm_galois_keys *gk, const shm_kswitch_key *relin,
+                                     shm_ciphertext *const *out, shm_stream_t stream);
 /* BatchBootstrapper::bootstrap_ordinary_s2c_first (BatchBootstrap.cuh:674-709) over
  * `count` ciphertexts with exactly 4 primes; out[i] receives the refreshed
  * ciphertext at scale out_scale (0 = the input scale). Inputs are not modified. */

@@ -367,6 +367,46 @@ class Ref:
             out.append(self.rescale(self.relinearize(acc, Ql, relin_key), Ql))
         return out
 
+    def ctct_matmul_diagpacking(self, X, W, Ql, num_batch, gal_keys, relin_key):
+        """Ct_ct_matrix_mul.cuh:182-609 ct_ct_matrix_mul_diagpacking: g = ceil(sqrt(col_X)),
+        b = ceil(col_X / g); X[j g + k] pre-rotated by (col_X - j g) num_batch for j > 0
+        (241-258); babies rot(W[i], k num_batch) (342-349); giant j = rescale(relinearize(
+        sum_k baby_k * X'[j g + k])) (351-372); output i = sum_j rot(giant_j, j g num_batch)
+        (374-385). gal_keys maps each step (mod the slot count) to its key; 0 is the
+        identity. Returns the output limbs (the scale label is the caller's)."""
+        n, col_x = self.N // 2, len(X)
+        g = int(np.sqrt(col_x))
+        if g * g < col_x:
+            g += 1
+        b = col_x // g
+        if b * g < col_x:
+            b += 1
+
+        def rot(ct, step, ql=Ql):
+            step %= n
+            return ct if step == 0 else self.rotate(ct, ql, self.elt(step), gal_keys[step])
+
+        xr = [X[i] if i // g == 0 else rot(X[i], (col_x - (i // g) * g) * num_batch) for i in range(col_x)]
+        out = []
+        for w in W:
+            baby = [w] + [rot(w, k * num_batch) for k in range(1, g)]
+            acc_out = None
+            for j in range(b):
+                acc = None
+                for k in range(g):
+                    if j * g + k >= col_x:
+                        break
+                    t = self.tensor(np.ascontiguousarray(baby[k], np.uint64),
+                                    np.ascontiguousarray(xr[j * g + k], np.uint64), Ql)
+                    acc = t if acc is None else self.add(acc, t, Ql, 3)
+                giant = self.rescale(self.relinearize(acc, Ql, relin_key), Ql)
+                if j == 0:
+                    acc_out = giant
+                else:
+                    acc_out = self.add(acc_out, rot(giant, j * g * num_batch, Ql - 1), Ql - 1, 2)
+            out.append(acc_out)
+        return out
+
     def modraise(self, ct_low):
         """[2][N] at prime 0 (NTT form) -> [2][nQ][N] (Bootstrapper.cu:3287-3356)."""
         out = np.zeros(2 * self.nQ * self.N, np.uint64)

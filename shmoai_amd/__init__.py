@@ -638,6 +638,26 @@ def ct_ct_matrix_mul_colpacking(enc_X: Sequence[Ciphertext], enc_W: Sequence[Cip
     return out
 
 
+def ct_ct_matrix_mul_diagpacking(enc_X: Sequence[Ciphertext], enc_W: Sequence[Ciphertext], RotK: GaloisKeys,
+                                 relin_keys: KSwitchKey, context: Context, col_X: int, row_X: int, col_W: int,
+                                 row_W: int, num_batch: int, out=None, stream=None):
+    """Ct_ct_matrix_mul.cuh:182-609 (baby-step giant-step): output[i] (i < col_W) =
+    sum_j rotate(giant_j, j g num_batch), giant_j = rescale(relinearize(sum_k
+    rotate(enc_W[i], k num_batch) * rotate(enc_X[j g + k], (col_X - j g) num_batch))),
+    g = ceil(sqrt(col_X)); outputs carry enc_X's scale, as in the reference."""
+    if len(enc_X) < col_X or len(enc_W) < col_W:
+        raise capi.ShmInvalidArgument(capi.SHM_ERR_INVALID_ARGUMENT, "enc_X / enc_W hold too few ciphertexts")
+    chain = enc_X[0].chain_index
+    if out is None:
+        out = [Ciphertext(context, 2, chain + 1) for _ in range(col_W)]
+    xs = (C.c_void_p * col_X)(*[x.h.value for x in enc_X[:col_X]])
+    ws = (C.c_void_p * col_W)(*[w.h.value for w in enc_W[:col_W]])
+    os_ = (C.c_void_p * col_W)(*[o.h.value for o in out])
+    check(_lib().shm_ct_ct_matrix_mul_diagpacking(context.h, xs, ws, col_X, row_X, col_W, row_W, num_batch, RotK.h,
+                                                   relin_keys.h, os_, stream))
+    return out
+
+
 class Stream:
     """A created HIP stream (default priority, non-blocking); pass `.h` as `stream=`."""
 

@@ -142,6 +142,7 @@ _SIGS = {
     "shm_server_get_stats": (i32, [vp, vp]),
     "shm_evaluate_chebyshev": (i32, [vp, pvp, sz, C.POINTER(dbl), sz, dbl, dbl, vp, vp]),
     "shm_ct_ct_matrix_mul_colpacking": (i32, [vp, pvp, pvp, i32, i32, i32, i32, i32, vp, vp, pvp, vp]),
+    "shm_ct_ct_matrix_mul_diagpacking": (i32, [vp, pvp, pvp, i32, i32, i32, i32, i32, vp, vp, pvp, vp]),
     "shm_shared_a_keys_create": (i32, [vp, vp, u32, u32, u64, u32, pvp]),
     "shm_shared_a_keys_destroy": (i32, [vp]),
     "shm_shared_a_keys_add_galois": (i32, [vp, C.POINTER(C.c_int), sz, u64]),

@@ -1107,6 +1107,34 @@ int shm_ct_ct_matrix_mul_colpacking(shm_context *ctx, const shm_ciphertext *cons
     });
 }
 
+int shm_ct_ct_matrix_mul_diagpacking(shm_context *ctx, const shm_ciphertext *const *enc_X,
+                                     const shm_ciphertext *const *enc_W, int col_X, int row_X, int col_W, int row_W,
+                                     int num_batch, const shm_galois_keys *gk, const shm_kswitch_key *relin,
+                                     shm_ciphertext *const *out, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(gk, "galois_keys");
+        need(relin, "relin_keys");
+        (void)row_W;
+        if (col_X <= 0 || col_W <= 0) throw InvalidArgument("ct_ct_matrix_mul_diag: dimensions must be positive");
+        if (row_X < col_X) throw InvalidArgument("ct_ct_matrix_mul_diag: row_X must be at least col_X");
+        if (!enc_X || !enc_W || !out) throw InvalidArgument("enc_X/enc_W/out is null");
+        std::vector<const Ciphertext *> xs(col_X), ws(col_W);
+        for (int j = 0; j < col_X; j++) {
+            need(enc_X[j], "enc_X[j]");
+            xs[j] = &enc_X[j]->ct;
+        }
+        std::vector<Ciphertext *> os(col_W);
+        for (int i = 0; i < col_W; i++) {
+            need(enc_W[i], "enc_W[i]");
+            need(out[i], "out[i]");
+            ws[i] = &enc_W[i]->ct;
+            os[i] = &out[i]->ct;
+        }
+        ctct_matmul_diagpacking(*ctx->c, xs, ws, num_batch, gk->g, relin->k, os, S(st));
+    });
+}
+
 int shm_bootstrap_s2c_first(shm_bootstrapper *b, const shm_ciphertext *const *in, size_t count,
                             shm_ciphertext *const *out, const shm_galois_keys *gk, const shm_kswitch_key *relin,
                             double out_scale, shm_stream_t st) {

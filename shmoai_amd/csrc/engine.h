@@ -411,6 +411,11 @@ void mul_relin_rescale_batch(Context &c, const std::vector<Ciphertext *> &out,
 void ctct_matmul_colpacking(Context &c, const std::vector<const Ciphertext *> &X,
                             const std::vector<const Ciphertext *> &W, int num_batch, const GaloisKeys &gk,
                             const KSwitchKey &rk, const std::vector<Ciphertext *> &out, hipStream_t s);
+// Ct_ct_matrix_mul.cuh:182-609 ct_ct_matrix_mul_diagpacking (baby-step giant-step);
+// X holds the col_X ciphertexts the product reads, out one per W ciphertext.
+void ctct_matmul_diagpacking(Context &c, const std::vector<const Ciphertext *> &X,
+                             const std::vector<const Ciphertext *> &W, int num_batch, const GaloisKeys &gk,
+                             const KSwitchKey &rk, const std::vector<Ciphertext *> &out, hipStream_t s);
 // a <- a * [v encoded at scale sigma] (no rescale)
 void ct_mul_const(Context &c, Ciphertext &a, double v, double sigma, hipStream_t s);
 void ct_mul_monomial_half(Context &c, Ciphertext &a, bool negate, hipStream_t s);

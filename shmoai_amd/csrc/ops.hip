@@ -537,6 +537,108 @@ void ctct_matmul_colpacking(Context &c, const std::vector<const Ciphertext *> &X
     }
 }
 
+// Ct_ct_matrix_mul.cuh:182-609 ct_ct_matrix_mul_diagpacking (baby-step giant-step):
+// g = ceil(sqrt(col_X)), b = ceil(col_X / g). X[j g + k] is pre-rotated by
+// (col_X - j g) num_batch (j > 0); for each output i the babies are rot(W[i], k
+// num_batch), k < g; giant j sums the products of baby k with X[j g + k] (tensor,
+// accumulated), is relinearised and rescaled and labelled with the scale of X, as
+// the reference does; output i = sum_j rot(giant_j, j g num_batch). Rotations are
+// rotate_vector's, so the result is the reference's limb for limb. The giants of
+// one output are relinearised and rescaled in one batched pass.
+void ctct_matmul_diagpacking(Context &c, const std::vector<const Ciphertext *> &X,
+                             const std::vector<const Ciphertext *> &W, int num_batch, const GaloisKeys &gk,
+                             const KSwitchKey &rk, const std::vector<Ciphertext *> &out, hipStream_t s) {
+    const size_t colX = X.size(), colW = W.size();
+    if (colX == 0 || colW == 0) throw InvalidArgument("ct_ct_matrix_mul_diag: empty operands");
+    if (out.size() != colW) throw InvalidArgument("ct_ct_matrix_mul_diag: one output per W ciphertext");
+    if (num_batch < 0) throw InvalidArgument("ct_ct_matrix_mul: num_batch must be non-negative");
+    const size_t chain = X[0]->chain;
+    auto check_ct = [&](const Ciphertext *x) {
+        if (!x || x->size != 2 || !x->ntt) throw InvalidArgument("ct_ct_matrix_mul: size-2 NTT-form ciphertexts");
+        if (x->chain != chain) throw InvalidArgument("ct_ct_matrix_mul: operands at different levels");
+    };
+    for (const Ciphertext *x : X) {
+        check_ct(x);
+        if (x->scale != X[0]->scale) throw InvalidArgument("ct_ct_matrix_mul: operands at different scales");
+    }
+    for (const Ciphertext *w : W) {
+        check_ct(w);
+        if (w->scale != W[0]->scale) throw InvalidArgument("ct_ct_matrix_mul: operands at different scales");
+    }
+    for (size_t i = 0; i < colW; i++) {
+        if (!out[i]) throw InvalidArgument("ct_ct_matrix_mul: null output");
+        for (const Ciphertext *x : X)
+            if (out[i] == x) throw InvalidArgument("ct_ct_matrix_mul: output aliases an input");
+        for (const Ciphertext *w : W)
+            if (out[i] == w) throw InvalidArgument("ct_ct_matrix_mul: output aliases an input");
+        for (size_t k = 0; k < i; k++)
+            if (out[k] == out[i]) throw InvalidArgument("ct_ct_matrix_mul: an output appears twice");
+    }
+    const size_t Ql = c.ql_of_chain(chain), slots = c.N / 2, nb = (size_t)num_batch;
+    if (Ql < 2) throw LogicError("end of modulus switching chain reached");
+    size_t g = (size_t)std::sqrt((double)colX);
+    if (g * g < colX) g++;
+    size_t b = colX / g;
+    if (b * g < colX) b++;
+    if (b > 32) throw InvalidArgument("ct_ct_matrix_mul_diag: col_X above 1024");
+    auto rot_step = [&](size_t step) { return (int)(step % slots); };
+    // pre-rotated X (giant 0 uses X as it is)
+    std::vector<Ciphertext> rx(colX);
+    std::vector<const Ciphertext *> xr(colX);
+    for (size_t idx = 0; idx < colX; idx++) {
+        const size_t j = idx / g;
+        const int st = rot_step((colX - j * g) * nb);
+        if (j == 0 || st == 0) {
+            xr[idx] = X[idx];
+        } else {
+            ct_clone(rx[idx], *X[idx], s);
+            rotate_internal(c, rx[idx], st, gk, s);
+            xr[idx] = &rx[idx];
+        }
+    }
+    std::vector<Ciphertext> baby(g), giant(b);
+    std::vector<const Ciphertext *> bp(g);
+    for (size_t i = 0; i < colW; i++) {
+        bp[0] = W[i];
+        for (size_t k = 1; k < g; k++) {
+            const int st = rot_step(k * nb);
+            if (st == 0) {
+                bp[k] = W[i];
+                continue;
+            }
+            ct_clone(baby[k], *W[i], s);
+            rotate_internal(c, baby[k], st, gk, s);
+            bp[k] = &baby[k];
+        }
+        PolyPack c2{}, ct01{};
+        std::vector<Ciphertext *> gs(b);
+        for (size_t j = 0; j < b; j++) {
+            Ciphertext &o = giant[j];
+            o.ctx = &c;
+            o.resize(3, chain, s, false);  // the first product writes every word
+            o.ntt = true;
+            o.scale = W[0]->scale * X[0]->scale;
+            for (size_t k = 0; k < g && j * g + k < colX; k++)
+                launch_tensor(c, bp[k]->data(), xr[j * g + k]->data(), o.data(), Ql, s, bp[k]->Ql(),
+                              xr[j * g + k]->Ql(), k > 0);
+            c2.p[j] = o.poly(2);
+            ct01.p[2 * j] = o.poly(0);
+            ct01.p[2 * j + 1] = o.poly(1);
+            gs[j] = &o;
+        }
+        const uint64_t all = 2 * b == 64 ? ~uint64_t(0) : (uint64_t(1) << (2 * b)) - 1;
+        keyswitch_batch(c, c2, b, rk.k.as<uint64_t>(), Ql, ct01, all, s, false);  // relinearise: add to both
+        for (Ciphertext *o : gs) o->size = 2;
+        rescale_batch(c, gs, s, false);
+        for (Ciphertext *o : gs) o->scale = X[0]->scale;
+        ct_clone(*out[i], giant[0], s);
+        for (size_t j = 1; j < b; j++) {
+            rotate_internal(c, giant[j], rot_step(j * g * nb), gk, s);
+            ct_add(c, *out[i], giant[j], s);
+        }
+    }
+}
+
 void ct_mul_monomial_half(Context &c, Ciphertext &a, bool negate, hipStream_t s) {
     const size_t total = a.size * a.Ql() * c.N;
     k_mono_half<<<grid_ops(total), kBlockOps, 0, s>>>(a.data(), c.mods(), c.iota.as<uint64_t>(),

@@ -1,10 +1,15 @@
-"""Column-packing ct-ct matrix product (Ct_ct_matrix_mul.cuh:10-181) on the GPU.
+"""ct-ct matrix products (Ct_ct_matrix_mul.cuh) on the GPU.
 
+Column packing (10-181):
 - bit-exact against the oracle composed of rotate, multiply (tensor), add,
   relinearize and rescale, for 3 x 5 operands with num_batch 3 (steps 3..12)
   and for num_batch 0 (every output is the unrotated sum);
 - the outputs decrypt to sum_j x_j * roll(w_j, -i num_batch) within 1e-4;
 - mismatched dimensions (the reference's check) and aliasing are refused.
+Diagonal packing, baby-step giant-step (182-609):
+- col_X = 5 (3 babies, 2 giants), 3 W ciphertexts, num_batch 2: bit-exact against
+  the oracle, outputs labelled with X's scale, decrypting to the same baby/giant
+  rotations and sums taken on the plaintext vectors.
 """
 import numpy as np
 import pytest
@@ -69,3 +74,46 @@ def test_ctct_colpacking_rejects_bad_arguments(env):
         shm.ct_ct_matrix_mul_colpacking(X, W, gk, rk, ctx, 2, 3, 2, 4, 1)   # row_X != row_W
     with pytest.raises(shm.capi.ShmInvalidArgument):
         shm.ct_ct_matrix_mul_colpacking(X, W, gk, rk, ctx, 2, 2, 2, 2, 1, out=[X[0], W[1]])  # aliasing
+
+
+def _diag_expected(xs, ws_i, nb):
+    """Slot values of the diagonal-packing product for one W ciphertext, from the
+    plaintext vectors (rotate(v, s) = roll(v, -s)); test_oracle.py pins the oracle
+    against the same sums."""
+    col_x = len(xs)
+    g = int(np.sqrt(col_x))
+    g += g * g < col_x
+    b = -(-col_x // g)
+    out = 0.0
+    for j in range(b):
+        giant = 0.0
+        for k in range(g):
+            if j * g + k >= col_x:
+                break
+            xr = xs[j * g + k] if j == 0 else np.roll(xs[j * g + k], -(col_x - j * g) * nb)
+            giant = giant + np.roll(ws_i, -k * nb) * xr
+        out = out + np.roll(giant, -j * g * nb)
+    return out
+
+
+def test_ctct_diagpacking_bitexact(env):
+    ctx, ref, sk, s_ref = env
+    col_x, col_w, nb = 5, 3, 2
+    xs, _, X, _ = _operands(env, col_x, 21)
+    _, ws, _, W = _operands(env, col_w, 22)
+    Ql = X[0].coeff_modulus_size
+    steps = [2, 4, 6]  # babies 1..2 x nb, pre-rotation (5 - 3) nb, giant 3 nb
+    gk = sk.create_galois_keys_from_steps(steps, seed=9)
+    rk = sk.gen_relinkey(seed=5)
+    out = shm.ct_ct_matrix_mul_diagpacking(X, W, gk, rk, ctx, col_x, col_x, col_w, col_x, nb)
+    gk_ref = {st: ref.gen_galois_key(s_ref, ref.elt(st), ref.splitmix64(9, ref.elt(st))) for st in steps}
+    exp = ref.ctct_matmul_diagpacking([x.numpy() for x in X], [w.numpy() for w in W], Ql, nb, gk_ref,
+                                      ref.gen_relin_key(s_ref, 5))
+    dec, enc = shm.Decryptor(ctx, sk), shm.Encoder(ctx)
+    q_last = float(ctx.coeff_modulus[Ql - 1])
+    for i in range(col_w):
+        assert out[i].coeff_modulus_size == Ql - 1 and out[i].scale == D
+        assert np.array_equal(out[i].numpy(), exp[i]), f"output {i}"
+        got = enc.decode(dec.decrypt(out[i]))
+        want = _diag_expected(xs, ws[i], nb) * D / q_last
+        assert float(np.max(np.abs(got - want))) < 1e-4, i

@@ -324,6 +324,47 @@ def test_ctct_matmul_oracle_decrypts_to_rotated_products(oracle_lib):
         assert np.max(np.abs(v - want)) < 1e-4, i
 
 
+def _diag_expected(xs, ws_i, nb):
+    """Slot values of Ct_ct_matrix_mul.cuh:182-609 for one W ciphertext, from the
+    plaintext vectors (rotate(v, s) = roll(v, -s))."""
+    col_x = len(xs)
+    g = int(np.sqrt(col_x))
+    g += g * g < col_x
+    b = -(-col_x // g)
+    out = 0.0
+    for j in range(b):
+        giant = 0.0
+        for k in range(g):
+            if j * g + k >= col_x:
+                break
+            xr = xs[j * g + k] if j == 0 else np.roll(xs[j * g + k], -(col_x - j * g) * nb)
+            giant = giant + np.roll(ws_i, -k * nb) * xr
+        out = out + np.roll(giant, -j * g * nb)
+    return out
+
+
+def test_ctct_diag_oracle_decrypts_to_bsgs_sums(oracle_lib):
+    """The diagonal-packing ct-ct product (baby-step giant-step,
+    Ct_ct_matrix_mul.cuh:182-609), col_X = 5 (g = 3, b = 2): each output decrypts to
+    the same baby/giant rotations and sums taken on the plaintext vectors."""
+    r = oracle_lib.Ref(11, [60, 40, 40, 60], 1)
+    s = r.sk_ntt(r.sample_ternary(63, 64))
+    rng = np.random.default_rng(64)
+    n, col_x, col_w, nb, Ql, D = r.N // 2, 5, 2, 2, 3, 2.0 ** 40
+    xs = [rng.uniform(-1, 1, n) for _ in range(col_x)]
+    ws = [rng.uniform(-1, 1, n) for _ in range(col_w)]
+    X = [r.encrypt_symmetric(s, r.encode(x, Ql, D), Ql, 90 + j) for j, x in enumerate(xs)]
+    W = [r.encrypt_symmetric(s, r.encode(w, Ql, D), Ql, 95 + j) for j, w in enumerate(ws)]
+    steps = {st % n for st in (2, 4, 6, 4 * 2, 3 * 2)} - {0}   # babies k nb, pre-rotation (5-3) nb, giant 3 nb
+    gk = {st: r.gen_galois_key(s, r.elt(st), r.splitmix64(9, r.elt(st))) for st in steps}
+    rk = r.gen_relin_key(s, 5)
+    out = r.ctct_matmul_diagpacking(X, W, Ql, nb, gk, rk)
+    scale = D * D / float(r.primes[Ql - 1])
+    for i, o in enumerate(out):
+        v = r.decode(r.decrypt(s, o, 2, Ql - 1), Ql - 1, scale)[0]
+        assert np.max(np.abs(v - _diag_expected(xs, ws[i], nb))) < 1e-4, i
+
+
 def test_golden_fixtures(oracle_lib):
     """Committed vectors written by tests/golden/make_fixtures.py (pure Python)."""
     with open(os.path.join(GOLDEN, "fixtures.json")) as f:
