@@ -1,6 +1,7 @@
 // Evaluator-level operations shared by the C ABI and the bootstrapping
 // pipeline: key generation, Galois automorphisms with key switching, rescale,
 // level dropping, and the helpers that keep levels and scales aligned.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -466,12 +467,56 @@ void mul_relin_rescale_batch(Context &c, const std::vector<Ciphertext *> &out,
     rescale_batch(c, out, s, rounded);
 }
 
+namespace {
+// out[k] = rotate_vector(in[k], step) for every k: the permutations, then one
+// batched key switch per 32 ciphertexts (the key read once per batch). Each
+// result equals rotate_internal's limb for limb; a step without its own key
+// takes rotate_internal's NAF split one ciphertext at a time.
+void rotate_many(Context &c, const std::vector<const Ciphertext *> &in, const std::vector<Ciphertext *> &out, int step,
+                 const GaloisKeys &gk, hipStream_t s) {
+    const uint32_t elt = c.elt_from_step(step);
+    auto it = gk.keys.find(elt);
+    if (step == 0 || it == gk.keys.end()) {
+        for (size_t k = 0; k < in.size(); k++) {
+            ct_clone(*out[k], *in[k], s);
+            rotate_internal(c, *out[k], step, gk, s);
+        }
+        return;
+    }
+    const uint32_t *table = c.galois_table(elt, s);
+    for (size_t k0 = 0; k0 < in.size(); k0 += 32) {
+        const size_t m = std::min<size_t>(32, in.size() - k0);
+        const size_t Ql = in[k0]->Ql();
+        PolyPack c2{}, ct01{};
+        for (size_t k = 0; k < m; k++) {
+            const Ciphertext &x = *in[k0 + k];
+            Ciphertext &o = *out[k0 + k];
+            if (x.size != 2 || x.Ql() != Ql) throw InvalidArgument("rotate_many: size-2 ciphertexts at one level");
+            o.ctx = &c;
+            o.resize(2, x.chain, s, false);  // both polynomials are written by the permutation
+            o.scale = x.scale;
+            o.ntt = true;
+            launch_permute(c, x.poly(0), o.poly(0), table, Ql, s);
+            launch_permute(c, x.poly(1), o.poly(1), table, Ql, s);
+            // the permuted c1 is the key switch input; ModDown overwrites it with
+            // the new c1 after the inner products have read it
+            c2.p[k] = o.poly(1);
+            ct01.p[2 * k] = o.poly(0);
+            ct01.p[2 * k + 1] = o.poly(1);
+        }
+        const uint64_t add_c0 = 0x5555555555555555ull & ((m == 32) ? ~uint64_t(0) : (uint64_t(1) << (2 * m)) - 1);
+        keyswitch_batch(c, c2, m, it->second->k.as<uint64_t>(), Ql, ct01, add_c0, s, false);
+    }
+}
+}  // namespace
+
 // Ct_ct_matrix_mul.cuh:10-181 ct_ct_matrix_mul_colpacking. Output i is
 // rescale(relinearize(sum_j X[j] * rot(W[j], i * num_batch))): the reference
 // multiplies (tensor, no relinearisation), adds the col_X products and then
 // relinearises and rescales once, and so does this. Each rotation is the
 // reference's rotate_vector (permutation, then key switch of the permuted c1),
-// so the result is the same limb for limb. Every product of an output
+// so the result is the same limb for limb; the col_X rotations of one output
+// share a step and go through batched key switches. Every product of an output
 // accumulates into its 3-polynomial ciphertext in the tensor kernel, and the
 // outputs are relinearised and rescaled in batched passes of up to 16. A step
 // that is a multiple of the slot count is the identity.
@@ -501,7 +546,9 @@ void ctct_matmul_colpacking(Context &c, const std::vector<const Ciphertext *> &X
     const size_t Ql = c.ql_of_chain(chain), slots = c.N / 2;
     if (Ql < 2) throw LogicError("end of modulus switching chain reached");
     constexpr size_t kChunk = 16;
-    Ciphertext rot;
+    std::vector<Ciphertext> rot(colX);
+    std::vector<Ciphertext *> rotp(colX);
+    for (size_t j = 0; j < colX; j++) rotp[j] = &rot[j];
     for (size_t i0 = 0; i0 < rowX; i0 += kChunk) {
         const size_t i1 = std::min(rowX, i0 + kChunk), m = i1 - i0;
         for (size_t i = i0; i < i1; i++) {
@@ -513,13 +560,9 @@ void ctct_matmul_colpacking(Context &c, const std::vector<const Ciphertext *> &X
         }
         for (size_t i = i0; i < i1; i++) {
             const size_t step = (i * (size_t)num_batch) % slots;
+            if (step != 0) rotate_many(c, W, rotp, (int)step, gk, s);  // every W[j] by the same step
             for (size_t j = 0; j < colX; j++) {
-                const Ciphertext *w = W[j];
-                if (step != 0) {
-                    ct_clone(rot, *W[j], s);
-                    rotate_internal(c, rot, (int)step, gk, s);
-                    w = &rot;
-                }
+                const Ciphertext *w = step != 0 ? &rot[j] : W[j];
                 launch_tensor(c, X[j]->data(), w->data(), out[i]->data(), Ql, s, X[j]->Ql(), w->Ql(), j > 0);
             }
         }
@@ -585,16 +628,21 @@ void ctct_matmul_diagpacking(Context &c, const std::vector<const Ciphertext *> &
     // pre-rotated X (giant 0 uses X as it is)
     std::vector<Ciphertext> rx(colX);
     std::vector<const Ciphertext *> xr(colX);
-    for (size_t idx = 0; idx < colX; idx++) {
-        const size_t j = idx / g;
+    for (size_t j = 0; j < b; j++) {
+        const size_t lo = j * g, hi = std::min(colX, lo + g);
         const int st = rot_step((colX - j * g) * nb);
         if (j == 0 || st == 0) {
-            xr[idx] = X[idx];
-        } else {
-            ct_clone(rx[idx], *X[idx], s);
-            rotate_internal(c, rx[idx], st, gk, s);
+            for (size_t idx = lo; idx < hi; idx++) xr[idx] = X[idx];
+            continue;
+        }
+        // the group of giant j shares one step: one batched rotation
+        std::vector<const Ciphertext *> src(X.begin() + lo, X.begin() + hi);
+        std::vector<Ciphertext *> dst;
+        for (size_t idx = lo; idx < hi; idx++) {
+            dst.push_back(&rx[idx]);
             xr[idx] = &rx[idx];
         }
+        rotate_many(c, src, dst, st, gk, s);
     }
     std::vector<Ciphertext> baby(g), giant(b);
     std::vector<const Ciphertext *> bp(g);
