@@ -321,6 +321,30 @@ def masked_bench(shm, ctx, hs, cols: int = 16):
             "ffn_3072x768_s_extrapolated": round(per * 3072 * 768, 2)}
 
 
+def ctct_bench(shm, ctx, hs, rows: int = 4):
+    """The attention's Q K^T product in column packing (Ct_ct_matrix_mul.cuh:10-181,
+    called with col_X = 64 and row_X = 128 in single_att_block.cuh:210) on 64 + 64 of
+    the resident ciphertexts (N=2^16, L=24), num_batch 256: `rows` of the 128 outputs
+    are timed (each is 64 rotations, 64 tensor products, one relinearisation and
+    rescale) and extrapolated to 128."""
+    sk = shm.SecretKey(ctx, seed=3, hamming_weight=192)
+    nb = 256
+    gk = sk.create_galois_keys_from_steps([i * nb for i in range(1, rows)], seed=4)
+    rk = sk.gen_relinkey(seed=5)
+    X, W = hs[:64], hs[64:128]
+    out = [shm.Ciphertext(ctx, 2, 2) for _ in range(rows)]
+    shm.ct_ct_matrix_mul_colpacking(X, W, gk, rk, ctx, 64, 2, 64, 2, nb, out=out[:2])
+    shm.synchronize()
+    t0 = time.perf_counter()
+    shm.ct_ct_matrix_mul_colpacking(X, W, gk, rk, ctx, 64, rows, 64, rows, nb, out=out)
+    shm.synchronize()
+    dt = time.perf_counter() - t0
+    log(f"bench: ct-ct QK^T colpacking, {rows} of 128 outputs: {dt * 1e3:.1f} ms")
+    return {"workload": "ct-ct column-packing product, col_X=64, N=2^16, L=24, num_batch=256",
+            "outputs_timed": rows, "ms": round(dt * 1e3, 2), "ms_per_output": round(dt * 1e3 / rows, 2),
+            "qk_128_outputs_s_extrapolated": round(dt / rows * 128, 2)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -410,6 +434,7 @@ def main() -> int:
     if not args.no_full_width and variant == shm.MATMUL_INT8:
         full = full_width_bench(shm, ctx, hs)
         full["masked"] = masked_bench(shm, ctx, hs)
+        full["ctct_qk"] = ctct_bench(shm, ctx, hs)
 
     ntt = ntt_bench(shm, ctx)
     boot = None

@@ -476,7 +476,8 @@ void rotate_many(Context &c, const std::vector<const Ciphertext *> &in, const st
                  const GaloisKeys &gk, hipStream_t s) {
     const uint32_t elt = c.elt_from_step(step);
     auto it = gk.keys.find(elt);
-    if (step == 0 || it == gk.keys.end()) {
+    const bool batched = !in.empty() && c.level(in[0]->Ql()).beta <= (size_t)kMaxDigits;
+    if (step == 0 || it == gk.keys.end() || !batched) {
         for (size_t k = 0; k < in.size(); k++) {
             ct_clone(*out[k], *in[k], s);
             rotate_internal(c, *out[k], step, gk, s);
@@ -507,6 +508,28 @@ void rotate_many(Context &c, const std::vector<const Ciphertext *> &in, const st
         const uint64_t add_c0 = 0x5555555555555555ull & ((m == 32) ? ~uint64_t(0) : (uint64_t(1) << (2 * m)) - 1);
         keyswitch_batch(c, c2, m, it->second->k.as<uint64_t>(), Ql, ct01, add_c0, s, false);
     }
+}
+
+// relinearize of 3-polynomial ciphertexts at one level (Ql limbs), batched when
+// the level's digit count allows it (same result either way)
+void relin_many(Context &c, const std::vector<Ciphertext *> &cts, const KSwitchKey &rk, size_t Ql, hipStream_t s) {
+    if (c.level(Ql).beta > (size_t)kMaxDigits) {
+        for (Ciphertext *o : cts) {
+            keyswitch(c, o->poly(2), rk.k.as<uint64_t>(), Ql, o->poly(0), o->poly(1), true, s, false);
+            o->size = 2;
+        }
+        return;
+    }
+    const size_t m = cts.size();
+    PolyPack c2{}, ct01{};
+    for (size_t i = 0; i < m; i++) {
+        c2.p[i] = cts[i]->poly(2);
+        ct01.p[2 * i] = cts[i]->poly(0);
+        ct01.p[2 * i + 1] = cts[i]->poly(1);
+    }
+    const uint64_t all = 2 * m == 64 ? ~uint64_t(0) : (uint64_t(1) << (2 * m)) - 1;
+    keyswitch_batch(c, c2, m, rk.k.as<uint64_t>(), Ql, ct01, all, s, false);  // relinearise: add to both
+    for (Ciphertext *o : cts) o->size = 2;
 }
 }  // namespace
 
@@ -550,7 +573,7 @@ void ctct_matmul_colpacking(Context &c, const std::vector<const Ciphertext *> &X
     std::vector<Ciphertext *> rotp(colX);
     for (size_t j = 0; j < colX; j++) rotp[j] = &rot[j];
     for (size_t i0 = 0; i0 < rowX; i0 += kChunk) {
-        const size_t i1 = std::min(rowX, i0 + kChunk), m = i1 - i0;
+        const size_t i1 = std::min(rowX, i0 + kChunk);
         for (size_t i = i0; i < i1; i++) {
             Ciphertext &o = *out[i];
             o.ctx = &c;
@@ -566,16 +589,8 @@ void ctct_matmul_colpacking(Context &c, const std::vector<const Ciphertext *> &X
                 launch_tensor(c, X[j]->data(), w->data(), out[i]->data(), Ql, s, X[j]->Ql(), w->Ql(), j > 0);
             }
         }
-        PolyPack c2{}, ct01{};
         std::vector<Ciphertext *> chunk(out.begin() + i0, out.begin() + i1);
-        for (size_t i = 0; i < m; i++) {
-            c2.p[i] = chunk[i]->poly(2);
-            ct01.p[2 * i] = chunk[i]->poly(0);
-            ct01.p[2 * i + 1] = chunk[i]->poly(1);
-        }
-        const uint64_t all = (uint64_t(1) << (2 * m)) - 1;
-        keyswitch_batch(c, c2, m, rk.k.as<uint64_t>(), Ql, ct01, all, s, false);  // relinearise: add to both
-        for (Ciphertext *o : chunk) o->size = 2;
+        relin_many(c, chunk, rk, Ql, s);
         rescale_batch(c, chunk, s, false);
     }
 }
@@ -658,7 +673,6 @@ void ctct_matmul_diagpacking(Context &c, const std::vector<const Ciphertext *> &
             rotate_internal(c, baby[k], st, gk, s);
             bp[k] = &baby[k];
         }
-        PolyPack c2{}, ct01{};
         std::vector<Ciphertext *> gs(b);
         for (size_t j = 0; j < b; j++) {
             Ciphertext &o = giant[j];
@@ -669,14 +683,9 @@ void ctct_matmul_diagpacking(Context &c, const std::vector<const Ciphertext *> &
             for (size_t k = 0; k < g && j * g + k < colX; k++)
                 launch_tensor(c, bp[k]->data(), xr[j * g + k]->data(), o.data(), Ql, s, bp[k]->Ql(),
                               xr[j * g + k]->Ql(), k > 0);
-            c2.p[j] = o.poly(2);
-            ct01.p[2 * j] = o.poly(0);
-            ct01.p[2 * j + 1] = o.poly(1);
             gs[j] = &o;
         }
-        const uint64_t all = 2 * b == 64 ? ~uint64_t(0) : (uint64_t(1) << (2 * b)) - 1;
-        keyswitch_batch(c, c2, b, rk.k.as<uint64_t>(), Ql, ct01, all, s, false);  // relinearise: add to both
-        for (Ciphertext *o : gs) o->size = 2;
+        relin_many(c, gs, rk, Ql, s);
         rescale_batch(c, gs, s, false);
         for (Ciphertext *o : gs) o->scale = X[0]->scale;
         ct_clone(*out[i], giant[0], s);
