@@ -10,6 +10,8 @@ Diagonal packing, baby-step giant-step (182-609):
 - col_X = 5 (3 babies, 2 giants), 3 W ciphertexts, num_batch 2: bit-exact against
   the oracle, outputs labelled with X's scale, decrypting to the same baby/giant
   rotations and sums taken on the plaintext vectors.
+Above 16 key digits (18 here) the rotations and relinearisations take single key
+switches instead of the batched one: still bit-exact.
 """
 import numpy as np
 import pytest
@@ -117,3 +119,29 @@ def test_ctct_diagpacking_bitexact(env):
         got = enc.decode(dec.decrypt(out[i]))
         want = _diag_expected(xs, ws[i], nb) * D / q_last
         assert float(np.max(np.abs(got - want))) < 1e-4, i
+
+
+def test_ctct_colpacking_many_digits_bitexact(oracle_lib):
+    """18 key digits (one special prime, 18 data primes): above the batched key
+    switch's 16 digits the rotations and the relinearisation take single key
+    switches; the result is still the oracle's limb for limb."""
+    bits = [50] + [30] * 17 + [50]
+    ctx = shm.Context(12, bits, 1)
+    ref = oracle_lib.Ref(12, bits, 1)
+    sk = shm.SecretKey(ctx, seed=1, hamming_weight=32)
+    s_ref = ref.sk_ntt(ref.sample_ternary(1, 32))
+    rng = np.random.default_rng(31)
+    enc = shm.Encoder(ctx)
+    n = ctx.N // 2
+    X = [sk.encrypt_symmetric(enc.encode(rng.uniform(-1, 1, n), 2.0 ** 30, 1), seed=300 + j) for j in range(2)]
+    W = [sk.encrypt_symmetric(enc.encode(rng.uniform(-1, 1, n), 2.0 ** 30, 1), seed=310 + j) for j in range(2)]
+    Ql = X[0].coeff_modulus_size
+    assert Ql == 18
+    gk = sk.create_galois_keys_from_steps([1, 2], seed=9)
+    rk = sk.gen_relinkey(seed=5)
+    out = shm.ct_ct_matrix_mul_colpacking(X, W, gk, rk, ctx, 2, 3, 2, 3, 1)
+    gk_ref = {st: ref.gen_galois_key(s_ref, ref.elt(st), ref.splitmix64(9, ref.elt(st))) for st in (1, 2)}
+    exp = ref.ctct_matmul_colpacking([x.numpy() for x in X], [w.numpy() for w in W], Ql, 3, 1, gk_ref,
+                                     ref.gen_relin_key(s_ref, 5))
+    for i in range(3):
+        assert np.array_equal(out[i].numpy(), exp[i]), f"output {i}"
