@@ -168,6 +168,37 @@ int shm_encrypt_symmetric(shm_context *ctx, const shm_secret_key *sk, const shm_
 int shm_decrypt(shm_context *ctx, const shm_secret_key *sk, const shm_ciphertext *ct, shm_plaintext *out,
                 shm_stream_t stream);
 
+/* ---- public-key encryption (PhantomPublicKey, secretkey.h; moai::Encryptor
+ * ckks_evaluator_parallel.cuh:107-127) ----------------------------------------------
+ * Randomness is injected: the key or ciphertext draws from splitmix64(seed, 0)
+ * (uniform a / ternary u) and splitmix64(seed, 1) (the error). */
+typedef struct shm_public_key shm_public_key;
+/* PhantomSecretKey::gen_publickey (secretkey.cu:524-535): encrypt_zero_symmetric at the
+ * key level, pk = (-(a s + e), a) over all (size_q + size_p) primes, NTT form.
+ * Synchronises the default stream. */
+int shm_public_key_generate(shm_context *ctx, const shm_secret_key *sk, uint64_t seed, shm_public_key **out);
+int shm_public_key_destroy(shm_public_key *pk);
+/* words [2][size_q + size_p][N] */
+int shm_public_key_export(const shm_public_key *pk, uint64_t *host, size_t n_words);
+/* PhantomPublicKey::encrypt_asymmetric (secretkey.cu:145-217, CKKS branch) with
+ * encrypt_zero_asymmetric_internal(_internal) (secretkey.cu:15-143): u uniform ternary
+ * and one error e over the key level, (u pk_0 + e, u pk_1 + e) brought to Q by ModDown
+ * (rns_bconv.cu:712-761), plaintext added to c_0; scale = the plaintext's. The
+ * plaintext must be at chain_index 1 (the reference's lower-level path applies the
+ * level-1 ModDown to a ciphertext of another level). */
+int shm_encrypt_asymmetric(shm_context *ctx, const shm_public_key *pk, const shm_plaintext *pt, uint64_t seed,
+                           shm_ciphertext *out, shm_stream_t stream);
+/* batch_input (Batch_encode_encrypt.cuh:12-43): X is [num_X][num_row][num_col] doubles
+ * (row-major, host memory); out[i] (i < num_col) = encrypt_asymmetric(encode(v_i, scale))
+ * with v_i[num_X * k + j] = X[j][k][i] and zeros up to the slot count; column i draws
+ * its randomness from splitmix64(seed, i). num_X * num_row must not exceed N/2.
+ * Synchronises the stream (X is read from host memory). */
+int shm_batch_input(shm_context *ctx, const shm_public_key *pk, const double *X, int num_X, int num_row, int num_col,
+                    double scale, uint64_t seed, shm_ciphertext *const *out, shm_stream_t stream);
+/* bias_vec (Batch_encode_encrypt.cuh:45-57), host only: out[j * num_X + i] = 1 for
+ * j < lengths[i], else 0; n = num_X * num_row */
+int shm_bias_vec(const int *lengths, int num_X, int num_row, int *out, size_t n);
+
 /* ---- encoder (PhantomCKKSEncoder ckks.cu:89-246; moai::Encoder
  * ckks_evaluator_parallel.cuh:34-103) --------------------------------------- */
 int shm_encode(shm_context *ctx, const double *re, const double *im, size_t n_values, uint32_t chain_index,
@@ -177,6 +208,31 @@ int shm_encode_scalar(shm_context *ctx, double value, uint32_t chain_index, doub
                       shm_stream_t stream);
 /* synchronises the stream; n <= N/2 slots written to re/im (im may be NULL) */
 int shm_decode(shm_context *ctx, const shm_plaintext *pt, double *re, double *im, size_t n, shm_stream_t stream);
+
+/* ---- moai::Evaluator facade operations (ckks_evaluator_parallel.cuh:444-561;
+ * ckks_evaluator_parallel.cu:466-590). A constant or vector is encoded over all
+ * N/2 slots at the ciphertext's scale (moai::Encoder), mod-switched to its level. */
+/* multiply_const_inplace: ct *= encode(value); scale becomes scale^2 */
+int shm_multiply_const_inplace(shm_context *ctx, shm_ciphertext *ct, double value, shm_stream_t stream);
+/* the same with a complex constant (encode(complex<double>)) */
+int shm_multiply_const_complex_inplace(shm_context *ctx, shm_ciphertext *ct, double re, double im,
+                                       shm_stream_t stream);
+/* add_const_inplace: ct += encode(value) */
+int shm_add_const_inplace(shm_context *ctx, shm_ciphertext *ct, double value, shm_stream_t stream);
+/* multiply_vector_inplace_reduced_error: ct *= encode(values padded with zeros) */
+int shm_multiply_vector_inplace_reduced_error(shm_context *ctx, shm_ciphertext *ct, const double *re,
+                                              const double *im, size_t n, shm_stream_t stream);
+/* add/sub/multiply_inplace_reduced_error: operands at different levels are aligned by
+ * multiplying the higher one by s_low q_last / s_high^2, rescaling and mod switching;
+ * multiply relinearises. */
+int shm_add_inplace_reduced_error(shm_context *ctx, shm_ciphertext *ct1, const shm_ciphertext *ct2,
+                                  shm_stream_t stream);
+int shm_sub_inplace_reduced_error(shm_context *ctx, shm_ciphertext *ct1, const shm_ciphertext *ct2,
+                                  shm_stream_t stream);
+int shm_multiply_inplace_reduced_error(shm_context *ctx, shm_ciphertext *ct1, const shm_ciphertext *ct2,
+                                       const shm_kswitch_key *relin_keys, shm_stream_t stream);
+/* double_inplace: ct += ct */
+int shm_double_inplace(shm_context *ctx, shm_ciphertext *ct, shm_stream_t stream);
 
 /* ---- evaluator (evaluate.cuh:32-304) ---------------------------------------- */
 int shm_add_inplace(shm_context *ctx, shm_ciphertext *a, const shm_ciphertext *b, shm_stream_t stream);

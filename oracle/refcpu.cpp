@@ -766,6 +766,73 @@ void ref_encrypt_symmetric(const RefCtx *c, const uint64_t *sk_qp, const uint64_
     }
 }
 
+/* Uniform ternary polynomial (the role of sample_ternary_poly, prng.cu): coefficient
+ * j = (splitmix(seed, j) mod 3) - 1. */
+void ref_sample_ternary_uniform(const RefCtx *c, uint64_t seed, int64_t *out) {
+    for (size_t j = 0; j < c->N; j++) out[j] = (int64_t)(sm64(seed, j) % 3) - 1;
+}
+
+/* secretkey.cu:524-535 gen_publickey = encrypt_zero_symmetric (secretkey.cu:260-326)
+ * at chain index 0 (all of QP), NTT form: pk = (-(a s + e), a) with a from
+ * splitmix(seed, 0) and e from splitmix(seed, 1). pk: [2][QP][N]. */
+void ref_gen_public_key(const RefCtx *c, const uint64_t *sk_qp, uint64_t seed, uint64_t *pk) {
+    const size_t N = c->N, QP = c->nQP(), S = QP * N;
+    auto idx = idx_Ql(*c, c->nQ, true);
+    uint64_t *p0 = pk, *p1 = pk + S;
+    ref_sample_uniform(c, sm64(seed, 0), QP, idx.data(), p1);
+    std::vector<int64_t> e(N);
+    ref_sample_gaussian(c, sm64(seed, 1), e.data());
+    std::vector<uint64_t> er(S);
+    for (size_t k = 0; k < QP; k++)
+        for (size_t j = 0; j < N; j++) er[k * N + j] = from_signed(e[j], c->q[k]);
+    ntt_limbs(*c, er.data(), QP, idx.data());
+    for (size_t k = 0; k < QP; k++) {
+        const uint64_t q = c->q[k];
+        for (size_t j = 0; j < N; j++) {
+            const size_t i = k * N + j;
+            p0[i] = negmod(addmod(mulmod(p1[i], sk_qp[i], q), er[i], q), q);
+        }
+    }
+}
+
+/* secretkey.cu:145-217 encrypt_asymmetric (CKKS) with 87-143
+ * encrypt_zero_asymmetric_internal and 15-85 ..._internal_internal, for a plaintext
+ * at chain index 1: the zero encryption is formed at the key level QP,
+ *   c'_i = u pk_i + e   (NTT form, u uniform ternary, i = 0, 1),
+ * with ONE error polynomial for both components (the reference draws e for c'_0 and
+ * c'_1 from the same seed bytes, secretkey.cu:48-68), then each c'_i is brought to Q
+ * by moddown (rns_bconv.cu:712-761: iNTT of the P limbs, fast conversion P -> Q, NTT,
+ * (c' - delta) P^-1), and c_0 += m. u from splitmix(seed, 0), e from splitmix(seed, 1).
+ * ct: [2][nQ][N]. */
+void ref_encrypt_asymmetric(const RefCtx *c, const uint64_t *pk, const uint64_t *pt, uint64_t seed, uint64_t *ct) {
+    const size_t N = c->N, QP = c->nQP(), nQ = c->nQ, S = QP * N;
+    auto idx = idx_Ql(*c, nQ, true);
+    std::vector<int64_t> u(N), e(N);
+    ref_sample_ternary_uniform(c, sm64(seed, 0), u.data());
+    ref_sample_gaussian(c, sm64(seed, 1), e.data());
+    std::vector<uint64_t> ur(S), er(S), cx(S);
+    for (size_t k = 0; k < QP; k++)
+        for (size_t j = 0; j < N; j++) {
+            ur[k * N + j] = from_signed(u[j], c->q[k]);
+            er[k * N + j] = from_signed(e[j], c->q[k]);
+        }
+    ntt_limbs(*c, ur.data(), QP, idx.data());
+    ntt_limbs(*c, er.data(), QP, idx.data());
+    for (size_t p = 0; p < 2; p++) {
+        const uint64_t *pkp = pk + p * S;
+        for (size_t k = 0; k < QP; k++) {
+            const uint64_t q = c->q[k];
+            for (size_t j = 0; j < N; j++) {
+                const size_t i = k * N + j;
+                cx[i] = addmod(mulmod(ur[i], pkp[i], q), er[i], q);
+            }
+        }
+        ref_moddown(c, cx.data(), nQ, ct + p * nQ * N);
+    }
+    for (size_t k = 0; k < nQ; k++)
+        for (size_t j = 0; j < N; j++) ct[k * N + j] = addmod(ct[k * N + j], pt[k * N + j], c->q[k]);
+}
+
 /* secretkey.cu:725-766 ckks_decrypt: m = c0 + c1 s (+ c2 s^2). */
 void ref_decrypt(const RefCtx *c, const uint64_t *sk_qp, const uint64_t *ct, size_t size, size_t Ql, uint64_t *pt) {
     const size_t N = c->N, S = Ql * N;

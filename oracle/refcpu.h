@@ -82,6 +82,9 @@ void ref_gen_relin_key(const RefCtx *c, const uint64_t *sk_qp, uint64_t seed, ui
 void ref_gen_galois_key(const RefCtx *c, const uint64_t *sk_qp, uint32_t elt, uint64_t seed, uint64_t *key);
 void ref_encrypt_symmetric(const RefCtx *c, const uint64_t *sk_qp, const uint64_t *pt, size_t Ql, uint64_t seed, uint64_t *ct);
 void ref_decrypt(const RefCtx *c, const uint64_t *sk_qp, const uint64_t *ct, size_t size, size_t Ql, uint64_t *pt);
+void ref_sample_ternary_uniform(const RefCtx *c, uint64_t seed, int64_t *out);
+void ref_gen_public_key(const RefCtx *c, const uint64_t *sk_qp, uint64_t seed, uint64_t *pk);
+void ref_encrypt_asymmetric(const RefCtx *c, const uint64_t *pk, const uint64_t *pt, uint64_t seed, uint64_t *ct);
 
 /* CKKS encoder (FP64) ------------------------------------------------------- */
 void ref_encode(const RefCtx *c, const double *re, const double *im, size_t n_values, size_t sparse_slots,

@@ -77,6 +77,9 @@ def lib():
             "ref_gen_galois_key": (None, [vp, u64p, C.c_uint32, C.c_uint64, u64p]),
             "ref_encrypt_symmetric": (None, [vp, u64p, u64p, sz, C.c_uint64, u64p]),
             "ref_decrypt": (None, [vp, u64p, u64p, sz, sz, u64p]),
+            "ref_sample_ternary_uniform": (None, [vp, C.c_uint64, i64p]),
+            "ref_gen_public_key": (None, [vp, u64p, C.c_uint64, u64p]),
+            "ref_encrypt_asymmetric": (None, [vp, u64p, u64p, C.c_uint64, u64p]),
             "ref_encode": (None, [vp, f64p, C.c_void_p, sz, sz, sz, C.c_double, u64p]),
             "ref_encode_scalar": (None, [vp, C.c_double, sz, C.c_double, u64p]),
             "ref_decode": (None, [vp, u64p, sz, sz, C.c_double, f64p, f64p]),
@@ -294,6 +297,24 @@ class Ref:
     def decrypt(self, sk_qp, ct, size, Ql):
         out = np.zeros(Ql * self.N, np.uint64)
         self.L.ref_decrypt(self.h, sk_qp, np.ascontiguousarray(ct, np.uint64), size, Ql, out)
+        return out
+
+    def sample_ternary_uniform(self, seed):
+        out = np.zeros(self.N, np.int64)
+        self.L.ref_sample_ternary_uniform(self.h, seed, out)
+        return out
+
+    def gen_public_key(self, sk_qp, seed):
+        """secretkey.cu:524-535 gen_publickey: [2][QP][N] NTT form."""
+        out = np.zeros(2 * self.nQP * self.N, np.uint64)
+        self.L.ref_gen_public_key(self.h, np.ascontiguousarray(sk_qp, np.uint64), seed, out)
+        return out
+
+    def encrypt_asymmetric(self, pk, pt, seed):
+        """secretkey.cu:145-217 encrypt_asymmetric of a plaintext at chain index 1."""
+        out = np.zeros(2 * self.nQ * self.N, np.uint64)
+        self.L.ref_encrypt_asymmetric(self.h, np.ascontiguousarray(pk, np.uint64),
+                                      np.ascontiguousarray(pt, np.uint64), seed, out)
         return out
 
     # encoder -------------------------------------------------------------------------

@@ -19,7 +19,7 @@ from .capi import check, MATMUL_U64, MATMUL_PER_CT, MATMUL_INT8  # noqa: F401
 __all__ = [
     "Context", "Ciphertext", "Plaintext", "SecretKey", "KSwitchKey", "GaloisKeys", "Encoder", "Evaluator",
     "Decryptor", "ct_pt_matrix_mul_wo_pre", "MATMUL_U64", "MATMUL_PER_CT", "MATMUL_INT8", "Bootstrapper",
-    "SharedAKeys", "SharedABatch",
+    "SharedAKeys", "SharedABatch", "PublicKey", "Encryptor", "batch_input", "bias_vec",
 ]
 
 
@@ -283,10 +283,68 @@ class SecretKey:
         check(_lib().shm_galois_keys_generate(self.ctx.h, self.h, arr, len(steps), seed, C.byref(h)))
         return GaloisKeys(self.ctx, h, steps)
 
+    def gen_publickey(self, seed: int) -> "PublicKey":
+        h = C.c_void_p()
+        check(_lib().shm_public_key_generate(self.ctx.h, self.h, seed, C.byref(h)))
+        return PublicKey(self.ctx, h)
+
     def encrypt_symmetric(self, pt: Plaintext, seed: int, stream=None) -> Ciphertext:
         ct = Ciphertext(self.ctx, 2, pt.chain_index)
         check(_lib().shm_encrypt_symmetric(self.ctx.h, self.h, pt.h, seed, ct.h, stream))
         return ct
+
+
+class PublicKey:
+    """PhantomPublicKey (secretkey.h): (-(a s + e), a) over QP, NTT form."""
+
+    def __init__(self, ctx: Context, h):
+        self.ctx, self.h = ctx, h
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                _lib().shm_public_key_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def numpy(self) -> np.ndarray:
+        c = self.ctx
+        out = np.zeros(2 * (c.size_q + c.size_p) * c.N, np.uint64)
+        check(_lib().shm_public_key_export(self.h, _u64p(out), out.size))
+        return out
+
+
+class Encryptor:
+    """moai::Encryptor (ckks_evaluator_parallel.cuh:107-127): public-key encryption."""
+
+    def __init__(self, ctx: Context, pk: PublicKey):
+        self.ctx, self.pk = ctx, pk
+
+    def encrypt(self, pt: "Plaintext", seed: int, stream=None) -> "Ciphertext":
+        ct = Ciphertext(self.ctx, 2, 1)
+        check(_lib().shm_encrypt_asymmetric(self.ctx.h, self.pk.h, pt.h, seed, ct.h, stream))
+        return ct
+
+
+def batch_input(X: np.ndarray, scale: float, ctx: Context, pk: PublicKey, seed: int, stream=None) -> list:
+    """batch_input (Batch_encode_encrypt.cuh:12-43): X[num_X][num_row][num_col] ->
+    num_col ciphertexts, column i packed as slot num_X * k + j = X[j][k][i]."""
+    X = np.ascontiguousarray(X, np.float64)
+    num_x, num_row, num_col = X.shape
+    out = [Ciphertext(ctx, 2, 1) for _ in range(num_col)]
+    hs = (C.c_void_p * max(num_col, 1))(*[o.h.value for o in out])
+    check(_lib().shm_batch_input(ctx.h, pk.h, X.ctypes.data_as(C.POINTER(C.c_double)), num_x, num_row, num_col,
+                                 float(scale), seed, hs, stream))
+    return out
+
+
+def bias_vec(lengths: Sequence[int], num_x: int, num_row: int) -> np.ndarray:
+    """bias_vec (Batch_encode_encrypt.cuh:45-57): token mask, out[j * num_X + i] = j < lengths[i]."""
+    arr = (C.c_int * len(lengths))(*lengths)
+    out = np.zeros(num_x * num_row, np.int32)
+    check(_lib().shm_bias_vec(arr, num_x, num_row, out.ctypes.data_as(C.POINTER(C.c_int)), out.size))
+    return out
 
 
 class Decryptor:
@@ -405,6 +463,36 @@ class Evaluator:
         """evaluate.cu:1727-1920: ct <- sum of rot(ct, step) over steps (one ModUp, one ModDown)."""
         arr = (C.c_int * len(steps))(*steps)
         check(self.L.shm_hoisting_inplace(self.ctx.h, ct.h, gk.h, arr, len(steps), stream))
+
+    # moai::Evaluator facade operations (ckks_evaluator_parallel.cuh:444-561)
+    def multiply_const_inplace(self, ct, value, stream=None):
+        if isinstance(value, complex):
+            check(self.L.shm_multiply_const_complex_inplace(self.ctx.h, ct.h, value.real, value.imag, stream))
+        else:
+            check(self.L.shm_multiply_const_inplace(self.ctx.h, ct.h, float(value), stream))
+
+    def add_const_inplace(self, ct, value: float, stream=None):
+        check(self.L.shm_add_const_inplace(self.ctx.h, ct.h, float(value), stream))
+
+    def multiply_vector_inplace_reduced_error(self, ct, values, stream=None):
+        v = np.asarray(values)
+        re = np.ascontiguousarray(v.real, np.float64)
+        im = np.ascontiguousarray(v.imag, np.float64) if np.iscomplexobj(v) else None
+        check(self.L.shm_multiply_vector_inplace_reduced_error(
+            self.ctx.h, ct.h, re.ctypes.data_as(C.POINTER(C.c_double)),
+            im.ctypes.data_as(C.POINTER(C.c_double)) if im is not None else None, re.size, stream))
+
+    def add_inplace_reduced_error(self, ct1, ct2, stream=None):
+        check(self.L.shm_add_inplace_reduced_error(self.ctx.h, ct1.h, ct2.h, stream))
+
+    def sub_inplace_reduced_error(self, ct1, ct2, stream=None):
+        check(self.L.shm_sub_inplace_reduced_error(self.ctx.h, ct1.h, ct2.h, stream))
+
+    def multiply_inplace_reduced_error(self, ct1, ct2, relin_key: KSwitchKey, stream=None):
+        check(self.L.shm_multiply_inplace_reduced_error(self.ctx.h, ct1.h, ct2.h, relin_key.h, stream))
+
+    def double_inplace(self, ct, stream=None):
+        check(self.L.shm_double_inplace(self.ctx.h, ct.h, stream))
 
     def multiply_by_i_inplace(self, ct, negate: bool = False, stream=None):
         """Multiply every slot by i (X^(N/2)), or by -i (X^(3N/2)) when negate."""

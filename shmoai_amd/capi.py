@@ -166,6 +166,20 @@ _SIGS = {
     "shm_bootstrap_shared_a_s2c_first": (i32, [vp, vp, pvp, sz, pvp, vp, dbl, vp]),
     "shm_bootstrap_shared_slot_to_coeff": (i32, [vp, vp, vp, vp]),
     "shm_bootstrap_shared_coeff_to_slot": (i32, [vp, vp, vp, vp, vp, vp]),
+    "shm_public_key_generate": (i32, [vp, vp, u64, pvp]),
+    "shm_public_key_destroy": (i32, [vp]),
+    "shm_public_key_export": (i32, [vp, pu64, sz]),
+    "shm_encrypt_asymmetric": (i32, [vp, vp, vp, u64, vp, vp]),
+    "shm_batch_input": (i32, [vp, vp, C.POINTER(dbl), i32, i32, i32, dbl, u64, pvp, vp]),
+    "shm_bias_vec": (i32, [C.POINTER(C.c_int), i32, i32, C.POINTER(C.c_int), sz]),
+    "shm_multiply_const_inplace": (i32, [vp, vp, dbl, vp]),
+    "shm_multiply_const_complex_inplace": (i32, [vp, vp, dbl, dbl, vp]),
+    "shm_add_const_inplace": (i32, [vp, vp, dbl, vp]),
+    "shm_multiply_vector_inplace_reduced_error": (i32, [vp, vp, C.POINTER(dbl), C.POINTER(dbl), sz, vp]),
+    "shm_add_inplace_reduced_error": (i32, [vp, vp, vp, vp]),
+    "shm_sub_inplace_reduced_error": (i32, [vp, vp, vp, vp]),
+    "shm_multiply_inplace_reduced_error": (i32, [vp, vp, vp, vp, vp]),
+    "shm_double_inplace": (i32, [vp, vp, vp]),
 }
 
 _lib = None

@@ -1489,3 +1489,160 @@ int shm_bootstrap_shared_coeff_to_slot(shm_bootstrapper *b, shm_shared_a_keys *k
         boot::shared_coeff_to_slot(bs, *keys->k, batch->b, re->b, im->b, S(st));
     });
 }
+
+// ---- public-key encryption, input packing, facade operations ------------------------------
+struct shm_public_key {
+    PublicKey pk;
+};
+
+extern "C" {
+
+int shm_public_key_generate(shm_context *ctx, const shm_secret_key *sk, uint64_t seed, shm_public_key **out) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(sk, "sk");
+        need(out, "out");
+        auto h = std::make_unique<shm_public_key>();
+        gen_public_key(*ctx->c, sk->sk, seed, h->pk, nullptr);
+        SHM_HIP(hipStreamSynchronize(nullptr));
+        *out = h.release();
+    });
+}
+
+int shm_public_key_destroy(shm_public_key *pk) {
+    return guard([&] {
+        if (pk) SHM_HIP(hipDeviceSynchronize());
+        delete pk;
+    });
+}
+
+int shm_public_key_export(const shm_public_key *pk, uint64_t *host, size_t n_words) {
+    return guard([&] {
+        need(pk, "pk");
+        need(host, "host");
+        const Context &c = *pk->pk.ctx;
+        const size_t words = 2 * c.nQP() * c.N;
+        if (n_words != words) throw InvalidArgument("public key export: n_words must be 2 * (size_q + size_p) * N");
+        SHM_HIP(hipMemcpy(host, pk->pk.pk.as<void>(), words * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+int shm_encrypt_asymmetric(shm_context *ctx, const shm_public_key *pk, const shm_plaintext *pt, uint64_t seed,
+                           shm_ciphertext *out, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(pk, "pk");
+        need(pt, "pt");
+        need(out, "out");
+        encrypt_asymmetric(*ctx->c, pk->pk, pt->pt.data(), pt->pt.chain, pt->pt.scale, seed, out->ct, S(st));
+    });
+}
+
+int shm_batch_input(shm_context *ctx, const shm_public_key *pk, const double *X, int num_X, int num_row, int num_col,
+                    double scale, uint64_t seed, shm_ciphertext *const *out, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(pk, "pk");
+        if (num_X <= 0 || num_row <= 0 || num_col < 0) throw InvalidArgument("batch_input: bad dimensions");
+        if (num_col > 0) {
+            need(X, "X");
+            need(out, "out");
+        }
+        std::vector<Ciphertext *> os((size_t)num_col);
+        for (int i = 0; i < num_col; i++) {
+            need(out[i], "out[i]");
+            os[i] = &out[i]->ct;
+        }
+        batch_input(*ctx->c, pk->pk, X, (size_t)num_X, (size_t)num_row, (size_t)num_col, scale, seed, os, S(st));
+    });
+}
+
+int shm_bias_vec(const int *lengths, int num_X, int num_row, int *out, size_t n) {
+    return guard([&] {
+        if (num_X <= 0 || num_row <= 0) throw InvalidArgument("bias_vec: bad dimensions");
+        need(lengths, "lengths");
+        need(out, "out");
+        if (n != (size_t)num_X * (size_t)num_row) throw InvalidArgument("bias_vec: n must be num_X * num_row");
+        std::fill(out, out + n, 0);
+        for (int i = 0; i < num_X; i++) {
+            if (lengths[i] < 0 || lengths[i] > num_row) throw InvalidArgument("bias_vec: length out of range");
+            for (int j = 0; j < lengths[i]; j++) out[(size_t)j * num_X + i] = 1;
+        }
+    });
+}
+
+int shm_multiply_const_inplace(shm_context *ctx, shm_ciphertext *ct, double value, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(ct, "ct");
+        facade_multiply_const(*ctx->c, ct->ct, value, 0.0, S(st));
+    });
+}
+
+int shm_multiply_const_complex_inplace(shm_context *ctx, shm_ciphertext *ct, double re, double im,
+                                       shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(ct, "ct");
+        facade_multiply_const(*ctx->c, ct->ct, re, im, S(st));
+    });
+}
+
+int shm_add_const_inplace(shm_context *ctx, shm_ciphertext *ct, double value, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(ct, "ct");
+        facade_add_const(*ctx->c, ct->ct, value, 0.0, S(st));
+    });
+}
+
+int shm_multiply_vector_inplace_reduced_error(shm_context *ctx, shm_ciphertext *ct, const double *re,
+                                              const double *im, size_t n, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(ct, "ct");
+        if (n) need(re, "re");
+        facade_multiply_vector(*ctx->c, ct->ct, re, im, n, S(st));
+    });
+}
+
+int shm_add_inplace_reduced_error(shm_context *ctx, shm_ciphertext *ct1, const shm_ciphertext *ct2,
+                                  shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(ct1, "ct1");
+        need(ct2, "ct2");
+        facade_reduced_error(*ctx->c, ct1->ct, ct2->ct, 0, nullptr, S(st));
+    });
+}
+
+int shm_sub_inplace_reduced_error(shm_context *ctx, shm_ciphertext *ct1, const shm_ciphertext *ct2,
+                                  shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(ct1, "ct1");
+        need(ct2, "ct2");
+        facade_reduced_error(*ctx->c, ct1->ct, ct2->ct, 1, nullptr, S(st));
+    });
+}
+
+int shm_multiply_inplace_reduced_error(shm_context *ctx, shm_ciphertext *ct1, const shm_ciphertext *ct2,
+                                       const shm_kswitch_key *relin, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(ct1, "ct1");
+        need(ct2, "ct2");
+        need(relin, "relin_keys");
+        facade_reduced_error(*ctx->c, ct1->ct, ct2->ct, 2, &relin->k, S(st));
+    });
+}
+
+int shm_double_inplace(shm_context *ctx, shm_ciphertext *ct, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(ct, "ct");
+        launch_add(*ctx->c, ct->ct.data(), ct->ct.data(), ct->ct.Ql(), ct->ct.size, S(st));
+    });
+}
+
+}  // extern "C"

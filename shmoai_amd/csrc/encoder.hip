@@ -226,6 +226,47 @@ int ilog2(size_t v) {
 
 typedef unsigned __int128 hu128;
 
+// Every slot holds one complex constant (the vector(slot_count, value) the
+// facade's encode(double) builds, ckks_evaluator_parallel.cuh:73-78); a constant
+// vector is its own bit reversal.
+__global__ void k_fill_const(double2 *buf, double2 v, uint32_t n) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) buf[t] = v;
+}
+
+// batch_input's interleaved column packing (Batch_encode_encrypt.cuh:26-33): plaintext
+// b holds column i = col0 + b with slot num_x * k + j = X[j][k][i] (X is
+// [num_x][num_row][num_col]) and zeros beyond num_x * num_row, bit-reversed.
+__global__ void k_columns_load_batch(const double *X, uint32_t num_x, uint32_t num_row, uint32_t num_col,
+                                     uint32_t col0, double2 *buf, uint32_t n, int logn) {
+    const uint32_t b = blockIdx.y, i = col0 + b, used = num_x * num_row;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        double v = 0.0;
+        if (t < used) {
+            const uint32_t j = t % num_x, k = t / num_x;
+            v = X[((size_t)j * num_row + k) * num_col + i];
+        }
+        buf[(size_t)b * n + brv_bits(t, logn)] = make_double2(v, 0.0);
+    }
+}
+
+// The inverse special FFT of a loaded, bit-reversed slot vector (last stage scaled
+// by scale / n), rounding and reduction per prime, and the forward NTT.
+void encode_finish(const Context &c, double2 *buf, size_t n, int logn, int logRatio, size_t M, size_t Ql,
+                   double scale, uint64_t *out, hipStream_t s) {
+    const size_t N = c.N;
+    const double fix = scale / (double)n;
+    for (int iter = logn - 1; iter >= 0; iter--) {
+        const int logPairs = logn - iter - 1;
+        const int last = (size_t(1) << logPairs) == n / 2;
+        k_ifft_stage<<<grid_for(n / 2), kBlock, 0, s>>>(buf, c.enc_roots.as<double2>(), c.enc_group.as<uint32_t>(),
+                                                        (uint32_t)n, logn, logPairs, logRatio, (uint32_t)M, fix, last);
+    }
+    launch_fill_zero(out, Ql * N, s);
+    k_decompose<<<grid_for(Ql * 2 * n), kBlock, 0, s>>>(buf, out, c.mods(), (uint32_t)n, N / (2 * n), Ql, N);
+    SHM_HIP(hipGetLastError());
+    ntt_forward(c, out, 1, Ql, 0, s);
+}
+
 }  // namespace
 
 void encode(const Context &c, const double *re, const double *im, size_t n_values, size_t Ql, double scale,
@@ -243,19 +284,38 @@ void encode(const Context &c, const double *re, const double *im, size_t n_value
     }
     k_bitrev_load<<<grid_for(n), kBlock, 0, s>>>(dre, im ? dim : nullptr, n_values, buf.as<double2>(), (uint32_t)n,
                                                  logn);
-    const double fix = scale / (double)n;
+    encode_finish(c, buf.as<double2>(), n, logn, logRatio, M, Ql, scale, out, s);
+}
+
+void encode_const(const Context &c, double re, double im, size_t Ql, double scale, uint64_t *out, hipStream_t s) {
+    const size_t N = c.N, n = N / 2, M = 2 * N;
+    if (!(scale > 0)) throw InvalidArgument("scale out of bounds");
+    const int logn = ilog2(n), logRatio = ilog2(M) - logn - 2;
+    Scratch buf(n * sizeof(double2), s);
+    k_fill_const<<<grid_for(n), kBlock, 0, s>>>(buf.as<double2>(), make_double2(re, im), (uint32_t)n);
+    encode_finish(c, buf.as<double2>(), n, logn, logRatio, M, Ql, scale, out, s);
+}
+
+void encode_columns_batch(const Context &c, const double *X, size_t num_x, size_t num_row, size_t num_col,
+                          size_t col0, size_t B, size_t Ql, const double *fix, uint64_t *out, double2 *fftbuf,
+                          hipStream_t s) {
+    const size_t N = c.N, n = N / 2, M = 2 * N;
+    if (B == 0) return;
+    if (B > 65535) throw InvalidArgument("encode_columns_batch: at most 65535 plaintexts");
+    if (num_x * num_row > n) throw InvalidArgument("batch_input: num_X * num_row exceeds the slot count");
+    const int logn = ilog2(n), logRatio = ilog2(M) - logn - 2;
+    const unsigned gx = std::min<unsigned>(grid_for(n / 2), 64u);
+    k_columns_load_batch<<<dim3(std::min<unsigned>(grid_for(n), 128u), (unsigned)B), kBlock, 0, s>>>(
+        X, (uint32_t)num_x, (uint32_t)num_row, (uint32_t)num_col, (uint32_t)col0, fftbuf, (uint32_t)n, logn);
     for (int iter = logn - 1; iter >= 0; iter--) {
         const int logPairs = logn - iter - 1;
         const int last = (size_t(1) << logPairs) == n / 2;
-        k_ifft_stage<<<grid_for(n / 2), kBlock, 0, s>>>(buf.as<double2>(), c.enc_roots.as<double2>(),
-                                                        c.enc_group.as<uint32_t>(), (uint32_t)n, logn, logPairs,
-                                                        logRatio, (uint32_t)M, fix, last);
+        k_ifft_stage_batch<<<dim3(gx, (unsigned)B), kBlock, 0, s>>>(fftbuf, c.enc_roots.as<double2>(),
+                                                                     c.enc_group.as<uint32_t>(), (uint32_t)n, logn,
+                                                                     logPairs, logRatio, (uint32_t)M, fix, last);
     }
-    launch_fill_zero(out, Ql * N, s);
-    k_decompose<<<grid_for(Ql * 2 * n), kBlock, 0, s>>>(buf.as<double2>(), out, c.mods(), (uint32_t)n, N / (2 * n), Ql,
-                                                        N);
     SHM_HIP(hipGetLastError());
-    ntt_forward(c, out, 1, Ql, 0, s);
+    ntt_forward_encoded(c, fftbuf, out, B, Ql, s);
 }
 
 void encode_masked_batch(const Context &c, const double *w, const double *fix, const uint8_t *mask, size_t B,

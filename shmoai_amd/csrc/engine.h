@@ -200,6 +200,12 @@ struct SecretKey {
     DevBuf s;  // NTT form over QP: [QP][N]
 };
 
+// secretkey.h PhantomPublicKey: (-(a s + e), a) over QP in NTT form, [2][QP][N]
+struct PublicKey {
+    Context *ctx = nullptr;
+    DevBuf pk;
+};
+
 struct KSwitchKey {
     Context *ctx = nullptr;
     DevBuf k;  // [dnum][2][QP][N]
@@ -301,6 +307,15 @@ void launch_decrypt(const Context &c, const uint64_t *ct, size_t size, const uin
                     size_t Ql, hipStream_t s);
 void launch_mul_elementwise(const Context &c, const uint64_t *a, const uint64_t *b, uint64_t *out, size_t nlimbs,
                             bool withP, size_t Ql, hipStream_t s);
+// uniform ternary coefficients (splitmix64(seed, j) mod 3) - 1 as residues over Ql (+P) limbs
+void launch_sample_ternary_uniform(const Context &c, uint64_t seed, uint64_t *out, size_t Ql, bool withP,
+                                   hipStream_t s);
+// over all QP limbs: b = -(a s + e)
+void launch_neg_as_plus_e(const Context &c, const uint64_t *sk, const uint64_t *a, const uint64_t *e, uint64_t *b,
+                          hipStream_t s);
+// over all QP limbs, npolys polynomials: out[p] = u * pk[p] + e
+void launch_pk_mac(const Context &c, const uint64_t *u, const uint64_t *pk, const uint64_t *e, uint64_t *out,
+                   size_t npolys, hipStream_t s);
 
 // key switching (keyswitch.hip)
 // rounded: ModDown rounds to nearest with an exact base conversion (unbiased) instead
@@ -348,6 +363,14 @@ void keyswitch_batch(const Context &c, const PolyPack &c2, size_t m, const uint6
 void encode(const Context &c, const double *re, const double *im, size_t n_values, size_t Ql, double scale,
             uint64_t *out, hipStream_t s);
 void encode_scalar(const Context &c, double value, size_t Ql, double scale, uint64_t *out, hipStream_t s);
+// encode() of the full slot vector {re + i im, ..., re + i im} (device-filled, no host data)
+void encode_const(const Context &c, double re, double im, size_t Ql, double scale, uint64_t *out, hipStream_t s);
+// B plaintexts [B][Ql][N]: plaintext b encodes column col0 + b of the host-layout
+// array X [num_x][num_row][num_col] (device memory) in batch_input's interleaved
+// packing, slot num_x * k + j = X[j][k][col0 + b]; fix[b] = scale / (N/2) (device)
+void encode_columns_batch(const Context &c, const double *X, size_t num_x, size_t num_row, size_t num_col,
+                          size_t col0, size_t B, size_t Ql, const double *fix, uint64_t *out, double2 *fftbuf,
+                          hipStream_t s);
 // B plaintexts [B][Ql][N] in NTT form: plaintext b encodes the slot vector
 // {mask[t] ? w[b] : 0} (t < N/2) at scale fix[b] * N/2, exactly as encode() does
 // for that vector. w, fix, mask are device arrays; fftbuf holds B * N/2 double2.
@@ -394,6 +417,21 @@ void rotate_internal_group(Context &c, Ciphertext &g, int step, const GaloisKeys
 void rotate_hoisted_group(Context &c, const Ciphertext &g, const std::vector<int> &steps, const GaloisKeys &gk,
                           const std::vector<Ciphertext *> &outs, hipStream_t s, bool rounded = false);
 void mod_switch_drop(Context &c, Ciphertext &ct, size_t new_chain, hipStream_t s);
+// public-key encryption (encrypt.hip)
+void gen_public_key(Context &c, const SecretKey &sk, uint64_t seed, PublicKey &out, hipStream_t s);
+void encrypt_asymmetric(Context &c, const PublicKey &pk, const uint64_t *pt, size_t pt_chain, double pt_scale,
+                        uint64_t seed, Ciphertext &out, hipStream_t s);
+// batch_input: X host [num_x][num_row][num_col]; out[i] = Enc_pk(encode(column i))
+void batch_input(Context &c, const PublicKey &pk, const double *X, size_t num_x, size_t num_row, size_t num_col,
+                 double scale, uint64_t seed, const std::vector<Ciphertext *> &out, hipStream_t s);
+// facade operations of moai::Evaluator (facade.hip)
+void facade_multiply_const(Context &c, Ciphertext &ct, double re, double im, hipStream_t s);
+void facade_add_const(Context &c, Ciphertext &ct, double re, double im, hipStream_t s);
+void facade_multiply_vector(Context &c, Ciphertext &ct, const double *re, const double *im, size_t n,
+                            hipStream_t s);
+// op: 0 add, 1 sub, 2 multiply (+ relinearize with rk)
+void facade_reduced_error(Context &c, Ciphertext &ct1, const Ciphertext &ct2, int op, const KSwitchKey *rk,
+                          hipStream_t s);
 void ct_clone(Ciphertext &dst, const Ciphertext &src, hipStream_t s);
 void ct_add(Context &c, Ciphertext &a, const Ciphertext &b, hipStream_t s);
 void ct_sub(Context &c, Ciphertext &a, const Ciphertext &b, hipStream_t s);

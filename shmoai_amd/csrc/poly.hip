@@ -285,7 +285,50 @@ __global__ void k_mul_elementwise(const uint64_t *a, const uint64_t *b, uint64_t
     }
 }
 
+// (splitmix64(seed, j) mod 3) - 1 over every limb (the role of sample_ternary_poly).
+__global__ void k_sample_ternary_uniform(uint64_t seed, uint64_t *out, const DevMod *mods, size_t Ql, size_t nQ,
+                                         size_t N, size_t total) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t limb = t >> __builtin_ctzll(N), j = t & (N - 1);
+        const uint64_t r = splitmix64(seed, j) % 3;
+        const uint64_t q = mods[limb_prime(limb, Ql, nQ)].q;
+        out[t] = r == 0 ? q - 1 : r - 1;
+    }
+}
+
+// secretkey.cu:15-85 (NTT branch): c'_p = u * pk_p + e over QP; npolys outputs.
+__global__ void k_pk_mac(const uint64_t *u, const uint64_t *pk, const uint64_t *e, uint64_t *out, const DevMod *mods,
+                         size_t N, size_t S, size_t total) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t w = t % S;
+        const DevMod m = mods[w >> __builtin_ctzll(N)];
+        out[t] = add_mod(mul_mod(u[w], pk[t], m), e[w], m.q);
+    }
+}
+
 }  // namespace
+
+void launch_sample_ternary_uniform(const Context &c, uint64_t seed, uint64_t *out, size_t Ql, bool withP,
+                                   hipStream_t s) {
+    const size_t limbs = Ql + (withP ? c.nP : 0), total = limbs * c.N;
+    k_sample_ternary_uniform<<<grid_for(total), kBlock, 0, s>>>(seed, out, c.mods(), Ql, c.nQ, c.N, total);
+    SHM_HIP(hipGetLastError());
+}
+
+void launch_neg_as_plus_e(const Context &c, const uint64_t *sk, const uint64_t *a, const uint64_t *e, uint64_t *b,
+                          hipStream_t s) {
+    const size_t total = c.nQP() * c.N;
+    k_ks_key_digit<<<grid_for(total), kBlock, 0, s>>>(sk, nullptr, b, a, e, c.mods(), c.pmod.as<uint64_t>(),
+                                                      c.pmod_s.as<uint64_t>(), 0, 0, c.N, total);
+    SHM_HIP(hipGetLastError());
+}
+
+void launch_pk_mac(const Context &c, const uint64_t *u, const uint64_t *pk, const uint64_t *e, uint64_t *out,
+                   size_t npolys, hipStream_t s) {
+    const size_t S = c.nQP() * c.N, total = npolys * S;
+    k_pk_mac<<<grid_for(total), kBlock, 0, s>>>(u, pk, e, out, c.mods(), c.N, S, total);
+    SHM_HIP(hipGetLastError());
+}
 
 void launch_add(const Context &c, uint64_t *a, const uint64_t *b, size_t Ql, size_t npolys, hipStream_t s) {
     k_add<<<rows_grid(npolys * Ql, c.N), kBlock, 0, s>>>(a, b, c.mods(), Ql, c.N, npolys * Ql);

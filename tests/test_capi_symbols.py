@@ -18,3 +18,18 @@ def test_bindings_cover_header():
 
 def test_version_string():
     assert b"gfx950" in capi.load().shm_version()
+
+
+def test_bias_vec_known_answer():
+    """bias_vec (Batch_encode_encrypt.cuh:45-57): out[j * num_X + i] = 1 for j < lengths[i]."""
+    import shmoai_amd as shm
+    got = shm.bias_vec([2, 0, 3], 3, 4)
+    want = [0] * 12
+    for i, ln in enumerate([2, 0, 3]):
+        for j in range(ln):
+            want[j * 3 + i] = 1
+    assert list(got) == want
+    assert [k for k, v in enumerate(got) if v] == [0, 2, 3, 5, 8]
+    import pytest
+    with pytest.raises(shm.capi.ShmInvalidArgument):
+        shm.bias_vec([5, 0, 3], 3, 4)  # a length beyond num_row

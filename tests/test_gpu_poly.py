@@ -40,12 +40,26 @@ def _encrypt(env, xs, seed):
     return sk.encrypt_symmetric(enc.encode(xs, SCALE, 1), seed=seed)
 
 
-@pytest.mark.parametrize("case", ["gelu24", "random63"])
+# gelu_cheb's own coefficients d_0..d_24 (non_linear_func/gelu_other.cuh:286-291), for
+# sum_k d_k T_k(u), u = (x + 3.5) / 16 on [-19.5, 12.5]
+GELU_CHEB_D = [
+    3.4550752799e+00, 5.7852067632e+00, 3.1727714170e+00, 7.0234932500e-01, -4.6200569546e-01,
+    -3.7817535966e-01, 8.4136420694e-02, 2.2427456987e-01, 2.9451916291e-02, -1.2928072669e-01,
+    -6.6659430283e-02, 6.4349502364e-02, 7.1733295078e-02, -1.9882581810e-02, -6.1145844240e-02,
+    -8.4827295421e-03, 4.3597254356e-02, 2.3373027950e-02, -2.4790187963e-02, -2.7940100474e-02,
+    8.8404026343e-03, 2.3776567058e-02, 8.8171887083e-03, -4.6290425955e-02, -9.7997267900e-03]
+
+
+@pytest.mark.parametrize("case", ["gelu24", "gelu_cheb_reference", "random63"])
 def test_chebyshev_matches_fp64(env, case):
     ctx, sk, rk, enc, dec, ev = env
     n = ctx.N // 2
     rng = np.random.default_rng(21)
-    if case == "gelu24":
+    if case == "gelu_cheb_reference":
+        lo, hi = -19.5, 12.5
+        coeffs = np.array(GELU_CHEB_D)
+        depth = 6
+    elif case == "gelu24":
         lo, hi = -19.5, 12.5
         coeffs = npc.chebinterpolate(lambda u: _gelu((u * (hi - lo) + lo + hi) / 2.0), 24)
         depth = 6
@@ -63,6 +77,9 @@ def test_chebyshev_matches_fp64(env, case):
     want = npc.chebval((2.0 * x - lo - hi) / (hi - lo), coeffs)
     err = float(np.max(np.abs(got - want)))
     assert err < 2e-5, err
+    if case == "gelu_cheb_reference":
+        # the fit itself: gelu_cheb's stated maximum error 0.037 on the interval
+        assert float(np.max(np.abs(want - _gelu(x)))) < 0.04
 
 
 def test_chebyshev_batch_equals_single(env):

@@ -210,6 +210,46 @@ def test_keyswitch_decrypts_within_noise(oracle_lib, bits, nP):
     assert 0 < noise < 2 ** 20
 
 
+def _decrypt_noise(r, s_ntt, ct, pt, Ql, limbs_idx=None):
+    """max |c0 + c1 s - m| over the coefficients (centred CRT value)."""
+    N = r.N
+    idx = r.pidx(Ql) if limbs_idx is None else limbs_idx
+    primes = [int(r.primes[k]) for k in idx]
+    L = len(primes)
+    diff = np.zeros(L * N, np.uint64)
+    for i, q in enumerate(primes):
+        sl = slice(i * N, (i + 1) * N)
+        a = (ct[sl].astype(object) + ct[L * N:][sl].astype(object) * s_ntt[idx[i] * N:(idx[i] + 1) * N].astype(object)
+             - pt[sl].astype(object)) % q
+        diff[sl] = np.array(a, np.uint64)
+    dc = r.intt(diff, np.array(idx, np.uint32)).reshape(L, N)
+    Q = int(np.prod(primes, dtype=object))
+    return max(abs(_centered(P.crt([int(dc[i, j]) for i in range(L)], primes), Q)) for j in range(N))
+
+
+@pytest.mark.parametrize("bits,nP", [([50] * 4 + [60], 1), ([50] * 4 + [60, 60], 2)])
+def test_public_key_and_asymmetric_encryption_decrypt(oracle_lib, bits, nP):
+    """gen_publickey is an encryption of zero at the key level: pk0 + pk1 s = -e
+    with the Gaussian error (|e| <= 19); encrypt_asymmetric decrypts to the
+    plaintext plus the noise (u e_pk + e + e s) / P + ModDown rounding, far below
+    2^20 (a dropped ModDown, a wrong sign or a missing u would leave noise of the
+    size of the modulus)."""
+    r = oracle_lib.Ref(10, bits, nP)
+    s = r.sk_ntt(r.sample_ternary(61, 64))
+    pk = r.gen_public_key(s, 62)
+    S = r.nQP * r.N
+    zero = np.zeros(S, np.uint64)
+    assert 0 < _decrypt_noise(r, s, pk, zero, r.nQP, list(range(r.nQP))) <= 19
+    u = r.sample_ternary_uniform(5)
+    assert set(np.unique(u)) == {-1, 0, 1} and abs(np.count_nonzero(u) / r.N - 2 / 3) < 0.05
+    rng = np.random.default_rng(3)
+    Ql = r.nQ
+    pt = np.concatenate([rng.integers(0, int(r.primes[i]), r.N, dtype=np.uint64) for i in range(Ql)])
+    ct = r.encrypt_asymmetric(pk, pt, 63)
+    noise = _decrypt_noise(r, s, ct, pt, Ql)
+    assert 0 < noise < 2 ** 20
+
+
 def test_samplers(oracle_lib):
     r = oracle_lib.Ref(14, [50, 50, 60], 1)
     t = r.sample_ternary(5, 192)
