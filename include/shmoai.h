@@ -253,6 +253,14 @@ int shm_apply_kswitch_key_inplace(shm_context *ctx, shm_ciphertext *ct, const sh
                                   shm_stream_t stream);
 /* evaluate.cu:1563-1584 rescale_to_next (floor division by the last prime) */
 int shm_rescale_to_next_inplace(shm_context *ctx, shm_ciphertext *ct, shm_stream_t stream);
+/* No counterpart in the reference. rescale_to_next_inplace for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops): the same
+ * limbs as shm_rescale_to_next_inplace, with the kernels' scratch (size x N words for
+ * the last limbs plus size x (Ql - 1) x N for the correction) in a static 64 MiB device
+ * array per GPU instead of the memory pool; larger ciphertexts are refused with
+ * SHM_ERR_INVALID_ARGUMENT. Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (other streams or threads) race. */
+int shm_rescale_to_next_inplace_noalloc(shm_context *ctx, shm_ciphertext *ct, shm_stream_t stream);
 /* evaluate.cu:1445-1561 mod_switch_to_next / mod_switch_to (drop limbs) */
 int shm_mod_switch_to_next_inplace(shm_context *ctx, shm_ciphertext *ct, shm_stream_t stream);
 int shm_mod_switch_to_inplace(shm_context *ctx, shm_ciphertext *ct, uint32_t chain_index, shm_stream_t stream);

@@ -441,6 +441,10 @@ class Evaluator:
     def rescale_to_next_inplace(self, ct, stream=None):
         check(self.L.shm_rescale_to_next_inplace(self.ctx.h, ct.h, stream))
 
+    def rescale_to_next_inplace_noalloc(self, ct, stream=None):
+        """rescale_to_next_inplace without device allocation (scratch in a static device array)."""
+        check(self.L.shm_rescale_to_next_inplace_noalloc(self.ctx.h, ct.h, stream))
+
     def mod_switch_to_next_inplace(self, ct, stream=None):
         check(self.L.shm_mod_switch_to_next_inplace(self.ctx.h, ct.h, stream))
 

@@ -180,6 +180,7 @@ _SIGS = {
     "shm_sub_inplace_reduced_error": (i32, [vp, vp, vp, vp]),
     "shm_multiply_inplace_reduced_error": (i32, [vp, vp, vp, vp, vp]),
     "shm_double_inplace": (i32, [vp, vp, vp]),
+    "shm_rescale_to_next_inplace_noalloc": (i32, [vp, vp, vp]),
 }
 
 _lib = None

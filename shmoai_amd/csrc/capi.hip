@@ -749,6 +749,14 @@ int shm_rescale_to_next_inplace(shm_context *ctx, shm_ciphertext *ct, shm_stream
     });
 }
 
+int shm_rescale_to_next_inplace_noalloc(shm_context *ctx, shm_ciphertext *ct, shm_stream_t st) {
+    return guard([&] {
+        need(ctx, "ctx");
+        need(ct, "ct");
+        rescale_inplace_noalloc(*ctx->c, ct->ct, S(st));
+    });
+}
+
 int shm_mod_switch_to_next_inplace(shm_context *ctx, shm_ciphertext *ct, shm_stream_t st) {
     return guard([&] {
         need(ctx, "ctx");

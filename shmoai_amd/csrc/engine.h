@@ -278,6 +278,9 @@ void launch_square(const Context &c, const uint64_t *a, uint64_t *out3, size_t Q
                    size_t a_limbs = 0);
 void launch_rescale(const Context &c, const uint64_t *in, uint64_t *out, size_t Ql, size_t npolys, hipStream_t s,
                     bool rounded = false);
+// the same with caller-provided scratch: last [npolys][N], delta [npolys][Ql - 1][N]
+void launch_rescale_scratch(const Context &c, const uint64_t *in, uint64_t *out, size_t Ql, size_t npolys,
+                            hipStream_t s, bool rounded, uint64_t *last, uint64_t *delta);
 void launch_permute(const Context &c, const uint64_t *in, uint64_t *out, const uint32_t *table, size_t nlimbs,
                     hipStream_t s);
 // out[p][i] = in.p[p][i], i < words, p < npolys
@@ -397,6 +400,10 @@ void gen_kswitch(Context &c, const SecretKey &sk, const uint64_t *newkey_dev, ui
 void apply_galois(Context &c, Ciphertext &ct, uint32_t elt, const GaloisKeys &gk, hipStream_t s, bool rounded = false);
 void rotate_internal(Context &c, Ciphertext &ct, int step, const GaloisKeys &gk, hipStream_t s, bool rounded = false);
 void rescale_inplace(Context &c, Ciphertext &ct, hipStream_t s, bool rounded = false);
+// rescale_inplace with its scratch in the library's static device array (noalloc.hip):
+// no device allocation; calls that overlap on one GPU race on that array
+void rescale_inplace_noalloc(Context &c, Ciphertext &ct, hipStream_t s);
+size_t noalloc_scratch_words();
 // rescale_inplace of ciphertexts at one level in one pass (<= 64 polynomials)
 void rescale_batch(Context &c, const std::vector<Ciphertext *> &cts, hipStream_t s, bool rounded = false);
 // Rotations of one ciphertext by several steps sharing one ModUp of c1 (hoisting,

@@ -373,15 +373,21 @@ void launch_rescale(const Context &c, const uint64_t *in, uint64_t *out, size_t 
     const size_t N = c.N, L1 = Ql - 1;
     Scratch last(npolys * N * sizeof(uint64_t), s);
     Scratch delta(npolys * L1 * N * sizeof(uint64_t), s);
-    k_gather_limb<<<grid_for(npolys * N), kBlock, 0, s>>>(in, last.as<uint64_t>(), Ql, L1, N, npolys);
-    ntt_inverse(c, last.as<uint64_t>(), npolys, 1, L1, s);
+    launch_rescale_scratch(c, in, out, Ql, npolys, s, rounded, last.as<uint64_t>(), delta.as<uint64_t>());
+}
+
+void launch_rescale_scratch(const Context &c, const uint64_t *in, uint64_t *out, size_t Ql, size_t npolys,
+                            hipStream_t s, bool rounded, uint64_t *last, uint64_t *delta) {
+    if (Ql < 2) throw LogicError("end of modulus switching chain reached");
+    const size_t N = c.N, L1 = Ql - 1;
+    k_gather_limb<<<grid_for(npolys * N), kBlock, 0, s>>>(in, last, Ql, L1, N, npolys);
+    ntt_inverse(c, last, npolys, 1, L1, s);
     const LevelConsts &L = c.level(Ql);
     // in-place: write into delta (each element is read then written by one thread), then copy back
-    uint64_t *dst = (in == out) ? delta.as<uint64_t>() : out;
-    rescale_ntt_fused(c, last.as<uint64_t>(), in, Ql, delta.as<uint64_t>(), nullptr, dst, npolys, npolys, L1,
-                      L.qlast_inv.as<uint64_t>(), L.qlast_inv_s.as<uint64_t>(), s, rounded);
-    if (in == out)
-        launch_copy_words(out, delta.as<uint64_t>(), npolys * L1 * N, s);
+    uint64_t *dst = (in == out) ? delta : out;
+    rescale_ntt_fused(c, last, in, Ql, delta, nullptr, dst, npolys, npolys, L1, L.qlast_inv.as<uint64_t>(),
+                      L.qlast_inv_s.as<uint64_t>(), s, rounded);
+    if (in == out) launch_copy_words(out, delta, npolys * L1 * N, s);
     SHM_HIP(hipGetLastError());
 }
 
