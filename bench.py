@@ -1,43 +1,43 @@
 #!/usr/bin/env python3
-"""Benchmark: the quarter-width MOAI ct-pt matmul on one MI355X.
+"""Benchmark: the 768-ciphertext batched bootstrap of one BERT-base layer on one MI355X.
 
-Workload (one step): 192 column-packed input ciphertexts (N = 2^16, 24 x 50-bit
-primes, 256 inputs x 128 tokens = 32768 slots each) times a 192 x 192 weight
-matrix W ~ N(0, 0.02) encoded at scale 2^40, then rescale
-(Ct_pt_matrix_mul.cuh:18-185 semantics). Inputs are resident in HBM before
-the timed region; limbs are uniform random residues (synthetic data).
+Workload: the bootstrapping segment of an encrypted BERT-base layer, 768 ciphertexts
+(hidden 768, column packing) at the reference parameters of test_shared_a.cuh:1207-1274:
+N = 2^16, logn = 15, Q = 1 x 51 + 20 x 46 + 14 x 51 bits, P = 5 x 51 bits (dnum 7), secret
+Hamming weight 192, scale 2^46, inputs with 4 primes, slots U(-1, 1); S2C-first, three
+CoeffToSlot/SlotToCoeff layers, EvalMod by a degree-63 Chebyshev cosine + 2 double angles.
+The batched form is the shared-a bootstrap (n = 8 members per chunk, 96 chunks,
+BatchBootstrap.cuh:719-820, each chunk's linear transforms on one shared-a batch and its
+EvalMod batched over the 16 real and imaginary parts); inputs are resident in HBM before
+any timed region.
 
-Prints one JSON line: value = amortised seconds per input (step time / 256
-inputs), plus the roofline of the step's dominant kernel and of the other large
-one (the INT8-MFMA contraction and the tile pass of the fused rescale NTT, each
-timed with HIP events on the stream it runs on), and a CPU baseline (the oracle,
-on a bounded sample of the same workload, extrapolated).
+One step = one chunk of 8 of the 768 ciphertexts; `steps` chunks are timed back to back
+(cycling through the 768 inputs). `value` is the measured wall time of one complete pass
+over all 768 ciphertexts (96 chunks in one call), end to end (no extrapolation);
+`value_from_steps_s` is ms_per_step x 96 for comparison.
 
-The line also carries `bootstrap_768`: S2C-first bootstrapping at the reference
-parameters (N = 2^16, Q = 1 x 51 + 20 x 46 + 14 x 51 bits, P = 5 x 51 bits,
-HW 192, inputs with 4 primes, slots U(-1, 1)), timed on a bounded number of
-ciphertexts and extrapolated to 768: ordinary one ciphertext per call (the
-sequential form), ordinary in one group of 8 (batched linear transforms and
-EvalMod), and shared-a chunks of n = 4 and n = 8, with the decryption error of
-the timed outputs, and `serving`: requests of uneven sizes arriving at random
-through the bootstrapping server (continuous batching), with p50/p99 latency.
+`bootstrap_768` carries every form over all 768 ciphertexts, each measured end to end:
+the plain per-ciphertext form (768 sequential single-ciphertext calls, each waited for, the
+reference's loop), ordinary groups of 8, shared-a n = 4 and shared-a n = 8, with decryption
+errors on a sample of outputs. `roofline` is the dominant kernel of a
+shared-a chunk, the fused last NTT pass (ModDown combine and rescale), from HIP events
+around each of its launches in one eager chunk. `cpu_baseline` is the oracle's key-switch
+primitives at the same parameters, weighted by the key-switch work the GPU path counts per
+chunk, extrapolated to 768 (a lower bound: transforms outside key switching and the
+elementwise work are not counted).
 
-`full_width_768` times one 768 x 768 matmul of the same kind (the BERT-base
-projection width, W rows contracted in chunks of 256), and in `masked` the
-masked form (a token mask in every weight plaintext) per term.
-
-Two more objects give the HBM rate of the transform and key-switch kernels:
-`ntt` (forward + inverse NTT of 64 polynomials x 24 limbs at N = 2^16) and
-`bootstrap_768.keyswitch` (one rotation of a top-level ciphertext of the
-bootstrapping parameters), each as algorithmic GB/s and fraction of peak.
-`traffic` is each kernel's HBM bytes per launch from the committed rocprofv3
-PMC summary in profiles/.
+`matmul_192` and `full_width_768` keep the MOAI ct-pt matmul (192 x 192 and 768 x 768 at
+N = 2^16, L = 24 x 50-bit primes, incl. rescale) with the INT8-MFMA contraction's roofline;
+`ntt` and `keyswitch` give the HBM rate of the transform and of one top-level rotation;
+`serving` drives the bootstrapping server with requests of uneven sizes.
 """
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
+import platform
 import sys
 import time
 
@@ -52,23 +52,43 @@ INT8_PEAK_TOPS = 5000.0        # MI355X dense INT8 MFMA (POP/s x 1e3)
 LOG_N, L, ROW_W, COL_W, INPUTS = 16, 24, 192, 192, 256
 BITS = [50] * L + [60]
 SCALE = 2.0 ** 40
-
+BOOT_BITS = [51] + [46] * 20 + [51] * 14 + [51] * 5
+BOOT_SCALE = 2.0 ** 46
+N_CTS = 768
+REF_SHARED_A_N8_S = 193.3      # reference, 768 cts shared-a n = 8, RTX PRO 6000 (README.md:32-34)
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "contract_int8_192x192_N65536_L24_pmc.txt")
+BOOT_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "fused_ntt_shared_a_n8_c3_pmc.txt")
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
-    (FETCH_SIZE and WRITE_SIZE runs scaled by a copy calibration), or None when
-    the summary is absent."""
+def measured_traffic(kernel: str, path: str = TRAFFIC_PROFILE):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE runs scaled by a copy calibration), or None."""
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(path) as f:
             for line in f:
                 if line.startswith(f"traffic_bytes_per_launch ({kernel})"):
                     return int(line.split(":")[1].split()[0])
     except OSError:
         pass
     return None
+
+
+def host_cpu():
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    return {"model": model, "nproc": affinity, "os_cpu_count": os.cpu_count()}
 
 
 def log(msg: str) -> None:
@@ -90,7 +110,7 @@ def make_inputs(ctx, shm, rng):
     return xs, hs
 
 
-def cpu_baseline(xs, W, threads: int):
+def cpu_baseline_matmul(xs, W, threads: int):
     from oracle import refcpu
     refcpu.build()
     refcpu.set_threads(threads)
@@ -164,80 +184,6 @@ def keyswitch_bench(shm, ctx, sk, gk, step: int, enc, iters: int = 10):
     log(f"bench: key switch (rotate, Ql={Ql}, dnum={beta}) {ms:.3f} ms, {gbs:.0f} GB/s algorithmic")
     return {"Ql": Ql, "dnum": beta, "ms": round(ms, 4), "algorithmic_bytes": alg, "achieved_gbs": round(gbs, 1),
             "frac": round(gbs / HBM_PEAK_GBS, 4)}
-
-
-def bootstrap_bench(shm, n_seq: int = 3, n_group: int = 8, shared=((4, 2), (8, 1))):
-    """Seconds per ciphertext of S2C-first bootstrapping at the reference
-    parameters: ordinary one ciphertext per call (the reference's sequential
-    form), ordinary in one group of n_group ciphertexts (batched linear
-    transforms and EvalMod), and shared-a chunks of n members."""
-    ctx = shm.Context(16, BOOT_BITS, 5)
-    sk = shm.SecretKey(ctx, seed=1, hamming_weight=192)
-    boot = shm.Bootstrapper(ctx)
-    t0 = time.perf_counter()
-    steps = boot.rotation_steps()
-    gk = sk.create_galois_keys_from_steps(steps, seed=2)
-    rk = sk.gen_relinkey(seed=3)
-    boot.prepare()
-    keysets = {}
-    for n_sh, _ in shared:
-        keysets[n_sh] = shm.SharedAKeys(ctx, sk, n_sh, 192, 4 + n_sh, ctx.chain(4))
-        keysets[n_sh].add_galois(steps, seed=5 + n_sh)
-    setup_s = time.perf_counter() - t0
-    log(f"bench: bootstrap keys + diagonals ready in {setup_s:.1f} s")
-    n = ctx.N // 2
-    rng = np.random.default_rng(7)
-    total = max([n_group, n_seq] + [n_sh * ch for n_sh, ch in shared])
-    zs = [rng.uniform(-1, 1, n) for _ in range(total)]
-    enc, dec = shm.Encoder(ctx), shm.Decryptor(ctx, sk)
-    ks = keyswitch_bench(shm, ctx, sk, gk, min(s for s in steps if s > 0), enc)
-    cts = [sk.encrypt_symmetric(enc.encode(z, 2.0 ** 46, ctx.chain(4)), seed=100 + i) for i, z in enumerate(zs)]
-
-    def max_err(outs):
-        return max(float(np.max(np.abs(enc.decode(dec.decrypt(o), complex_out=True) - z)))
-                   for o, z in zip(outs[:2], zs[:2]))
-
-    boot.bootstrap_s2c_first(cts[:1], gk, rk)  # warm-up
-    t0 = time.perf_counter()
-    outs = [boot.bootstrap_s2c_first([c], gk, rk)[0] for c in cts[:n_seq]]  # each call synchronises
-    seq_s = (time.perf_counter() - t0) / n_seq
-    err_seq = max_err(outs)
-    log(f"bench: ordinary bootstrap, one ciphertext per call: {seq_s * 1e3:.1f} ms/ct, max err {err_seq:.2e}")
-    t0 = time.perf_counter()
-    outs = boot.bootstrap_s2c_first(cts[:n_group], gk, rk)
-    grp_s = (time.perf_counter() - t0) / n_group
-    err_grp = max_err(outs)
-    log(f"bench: ordinary bootstrap, group of {n_group}: {grp_s * 1e3:.1f} ms/ct, max err {err_grp:.2e}")
-    res = {"params": "N=2^16, logn=15, Q=1x51+20x46+14x51, P=5x51 (dnum 7), HW 192, scale 2^46, "
-                     "S2C-first, EvalMod deg-63 Chebyshev + 2 double angles, output 22 primes",
-           "ordinary_sequential_s_per_ct": round(seq_s, 5),
-           "ordinary_sequential_768_s_extrapolated": round(seq_s * 768, 2),
-           f"ordinary_group{n_group}_s_per_ct": round(grp_s, 5),
-           f"ordinary_group{n_group}_768_s_extrapolated": round(grp_s * 768, 2),
-           "max_abs_err_ordinary": max(err_seq, err_grp)}
-    best = grp_s
-    for n_sh, chunks in shared:
-        keys = keysets[n_sh]
-        shm.bootstrap_shared_a_s2c_first(boot, keys, cts[:n_sh], rk)  # warm-up
-        t0 = time.perf_counter()
-        outs = []
-        for c in range(chunks):
-            outs += shm.bootstrap_shared_a_s2c_first(boot, keys, cts[c * n_sh:(c + 1) * n_sh], rk)
-        sh_s = (time.perf_counter() - t0) / (n_sh * chunks)
-        err_sh = max_err(outs)
-        log(f"bench: shared-a n={n_sh} bootstrap {sh_s * 1e3:.1f} ms/ct, max err {err_sh:.2e}")
-        res[f"shared_a_n{n_sh}_s_per_ct"] = round(sh_s, 5)
-        res[f"shared_a_n{n_sh}_768_s_extrapolated"] = round(sh_s * 768, 2)
-        res[f"max_abs_err_shared_a_n{n_sh}"] = err_sh
-        best = min(best, sh_s)
-    res.update({"best_768_s_extrapolated": round(best * 768, 2),
-                "batched_speedup_vs_sequential": round(seq_s / best, 3),
-                "keys_resident_setup_s": round(setup_s, 2), "keyswitch": ks,
-                "sample": f"{n_seq} single-ciphertext calls, one group of {n_group}, shared-a chunks "
-                          + ", ".join(f"{ch} x n={n_sh}" for n_sh, ch in shared)
-                          + "; wall clock incl. host overhead, extrapolated to 768"})
-    res["serving"] = serving_bench(shm, boot, gk, rk, cts)
-    return res
 
 
 def serving_bench(shm, boot, gk, rk, cts, n_req: int = 12, seed: int = 9):
@@ -321,28 +267,236 @@ def masked_bench(shm, ctx, hs, cols: int = 16):
             "ffn_3072x768_s_extrapolated": round(per * 3072 * 768, 2)}
 
 
-def ctct_bench(shm, ctx, hs, rows: int = 4):
-    """The attention's Q K^T product in column packing (Ct_ct_matrix_mul.cuh:10-181,
-    called with col_X = 64 and row_X = 128 in single_att_block.cuh:210) on 64 + 64 of
-    the resident ciphertexts (N=2^16, L=24), num_batch 256: `rows` of the 128 outputs
-    are timed (each is 64 rotations, 64 tensor products, one relinearisation and
-    rescale) and extrapolated to 128."""
+LAYER_BITS = [51] + [46] * 23 + [51] * 11 + [51] * 5   # test_single_layer.cuh:473-535 (shared-a S2C-first)
+
+
+def ctct_bench(shm, rows: int = 4):
+    """The attention's Q K^T product in column packing (Ct_ct_matrix_mul.cuh:10-181, called
+    with col_X = 64 and row_X = 128 in single_att_block.cuh:210) on the reference layer's
+    prime chain (35 data primes, 5 special primes, dnum 7) at 24 primes, num_batch 256:
+    output 0 has no rotation and every other output takes 64; one output alone and `rows`
+    outputs are timed, giving t(output 0) + 127 t(rotated output) for the 128 outputs."""
+    ctx = shm.Context(16, LAYER_BITS, 5)
     sk = shm.SecretKey(ctx, seed=3, hamming_weight=192)
-    nb = 256
+    nb, Ql = 256, 24
     gk = sk.create_galois_keys_from_steps([i * nb for i in range(1, rows)], seed=4)
     rk = sk.gen_relinkey(seed=5)
-    X, W = hs[:64], hs[64:128]
-    out = [shm.Ciphertext(ctx, 2, 2) for _ in range(rows)]
+    rng = np.random.default_rng(9)
+    chain = ctx.chain(Ql)
+
+    def rand_ct():
+        a = np.concatenate([rng.integers(0, int(ctx.coeff_modulus[k]), size=ctx.N, dtype=np.uint64)
+                            for k in range(Ql)] * 2)
+        c = shm.Ciphertext(ctx, 2, chain).load(a)
+        c.scale = 2.0 ** 46
+        return c
+
+    base = [rand_ct() for _ in range(8)]
+    X = [base[i % 8].copy() for i in range(64)]
+    W = [base[(i + 3) % 8].copy() for i in range(64)]
+    out = [shm.Ciphertext(ctx, 2, chain) for _ in range(rows)]
     shm.ct_ct_matrix_mul_colpacking(X, W, gk, rk, ctx, 64, 2, 64, 2, nb, out=out[:2])
+    t1, _ = timed(shm, lambda: shm.ct_ct_matrix_mul_colpacking(X, W, gk, rk, ctx, 64, 1, 64, 1, nb, out=out[:1]))
+    tr, _ = timed(shm, lambda: shm.ct_ct_matrix_mul_colpacking(X, W, gk, rk, ctx, 64, rows, 64, rows, nb, out=out))
+    t_rot = (tr - t1) / (rows - 1)
+    log(f"bench: ct-ct QK^T colpacking: output 0 {t1 * 1e3:.1f} ms, rotated output {t_rot * 1e3:.1f} ms")
+    return {"workload": "ct-ct column-packing product, col_X=64, N=2^16, 24 of the layer's primes "
+                        "(P=5, dnum 7), num_batch=256",
+            "output0_ms": round(t1 * 1e3, 2), "rotated_output_ms": round(t_rot * 1e3, 2),
+            "outputs_timed": rows,
+            "qk_128_outputs_s_estimated": round(t1 + 127 * t_rot, 2)}
+
+
+def boot_setup(shm, log_t0):
+    ctx = shm.Context(16, BOOT_BITS, 5)
+    sk = shm.SecretKey(ctx, seed=1, hamming_weight=192)
+    boot = shm.Bootstrapper(ctx)
+    steps = boot.rotation_steps()
+    gk = sk.create_galois_keys_from_steps(steps, seed=2)
+    rk = sk.gen_relinkey(seed=3)
+    boot.prepare()
+    log(f"bench: bootstrap keys + diagonals ready ({time.perf_counter() - log_t0:.1f} s)")
+    return ctx, sk, boot, steps, gk, rk
+
+
+def make_boot_inputs(shm, ctx, sk, count):
+    enc = shm.Encoder(ctx)
+    rng = np.random.default_rng(7)
+    zs = [rng.uniform(-1, 1, ctx.N // 2) for _ in range(count)]
+    cts = [sk.encrypt_symmetric(enc.encode(z, BOOT_SCALE, ctx.chain(4)), seed=100 + i) for i, z in enumerate(zs)]
+    shm.synchronize()
+    return zs, cts
+
+
+def max_err(shm, ctx, sk, outs, zs, sample):
+    enc, dec = shm.Encoder(ctx), shm.Decryptor(ctx, sk)
+    return max(float(np.max(np.abs(enc.decode(dec.decrypt(outs[i]), complex_out=True) - zs[i]))) for i in sample)
+
+
+def timed(shm, fn):
     shm.synchronize()
     t0 = time.perf_counter()
-    shm.ct_ct_matrix_mul_colpacking(X, W, gk, rk, ctx, 64, rows, 64, rows, nb, out=out)
+    r = fn()
     shm.synchronize()
-    dt = time.perf_counter() - t0
-    log(f"bench: ct-ct QK^T colpacking, {rows} of 128 outputs: {dt * 1e3:.1f} ms")
-    return {"workload": "ct-ct column-packing product, col_X=64, N=2^16, L=24, num_batch=256",
-            "outputs_timed": rows, "ms": round(dt * 1e3, 2), "ms_per_output": round(dt * 1e3 / rows, 2),
-            "qk_128_outputs_s_extrapolated": round(dt / rows * 128, 2)}
+    return time.perf_counter() - t0, r
+
+
+def cpu_baseline_c3(shm, counters, chunk_cts, threads):
+    """The oracle's key-switch primitives at the bootstrap parameters (ModUp of one
+    polynomial, an inner product with two outputs, ModDown of one polynomial), timed at
+    three levels and interpolated in Ql, weighted by the key-switch work the GPU path
+    counted over one chunk, extrapolated to 768 ciphertexts."""
+    from oracle import refcpu
+    refcpu.build()
+    refcpu.set_threads(threads)
+    ref = refcpu.Ref(16, BOOT_BITS, 5)
+    rng = np.random.default_rng(11)
+    N, nP = ref.N, ref.nP
+
+    def limbs(idx):
+        return np.concatenate([rng.integers(0, int(ref.primes[k]), N, dtype=np.uint64) for k in idx])
+
+    key = limbs(list(range(ref.nQP)) * (2 * ref.dnum()))
+    meas = {}
+    t_all = time.perf_counter()
+    for Ql in (4, 22, 35):
+        c2 = limbs(range(Ql))
+        t0 = time.perf_counter()
+        d = ref.modup(c2, Ql)
+        t_up = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        cx = ref.ks_inner(d, key, Ql)
+        t_in = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ref.moddown(cx[:(Ql + nP) * N], Ql)
+        t_dn = time.perf_counter() - t0
+        meas[Ql] = (t_up, t_in / 2, t_dn)
+    sample_s = time.perf_counter() - t_all
+    pts = sorted(meas)
+
+    def interp(Ql, i):
+        xs = np.array(pts, float)
+        ys = np.array([meas[p][i] for p in pts])
+        return float(np.interp(Ql, xs, ys))
+
+    per_chunk = sum(up * interp(q, 0) + inner * interp(q, 1) + dn * interp(q, 2)
+                    for q, (up, inner, dn) in counters.items())
+    est = per_chunk * N_CTS / chunk_cts
+    return {"value": round(est, 1), "unit": "s per 768-ct batched bootstrap (extrapolated)", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle ModUp / inner product / ModDown at Ql = 4, 22, 35 (N=2^16, P=5) on {threads} "
+                      f"threads, {sample_s:.1f} s; weighted by the GPU path's key-switch counts over one "
+                      f"shared-a chunk of {chunk_cts} and extrapolated x{N_CTS // chunk_cts}; key switching "
+                      "only (a lower bound)",
+            "per_level_s": {str(q): [round(v, 4) for v in meas[q]] for q in pts}}
+
+
+def c1_pipeline(shm, threads, iters: int = 20):
+    """NTT -> rotate(1) -> multiply_plain -> rescale -> iNTT at N = 2^13, on the
+    GPU (through the C ABI) and on the oracle (all threads), end to end."""
+    from oracle import refcpu
+    refcpu.set_threads(threads)
+    bits = [50] * 8 + [60]
+    ctx = shm.Context(13, bits, 1)
+    ref = refcpu.Ref(13, bits, 1)
+    rng = np.random.default_rng(3)
+    N, Ql = ctx.N, 8
+    x = np.concatenate([rng.integers(0, int(ref.primes[k]), size=N, dtype=np.uint64) for k in range(Ql)] * 2)
+    sk = shm.SecretKey(ctx, seed=1, hamming_weight=64)
+    gk = sk.create_galois_keys_from_steps([1], seed=9)
+    s_ref = ref.sk_ntt(ref.sample_ternary(1, 64))
+    elt = ref.elt(1)
+    key = ref.gen_galois_key(s_ref, elt, ref.splitmix64(9, elt))
+    ev = shm.Evaluator(ctx)
+    pt = shm.Encoder(ctx).encode(0.37, 2.0 ** 50, 1)
+    cts = [shm.Ciphertext(ctx, 2, 1).load(x) for _ in range(iters + 1)]
+
+    def gpu(ct):
+        ctx.ntt_forward(ct.data_ptr(), 2, Ql, 0)
+        ev.rotate_vector_inplace(ct, 1, gk)
+        ev.multiply_plain_inplace(ct, pt)
+        ev.rescale_to_next_inplace(ct)
+        ctx.ntt_inverse(ct.data_ptr(), 2, Ql - 1, 0)
+
+    gpu(cts[0])
+    dt, _ = timed(shm, lambda: [gpu(c) for c in cts[1:]])
+    gpu_ms = dt / iters * 1e3
+    pt_ref = ref.encode_scalar(0.37, Ql, 2.0 ** 50)
+    t0 = time.perf_counter()
+    y = ref.ntt(x, np.tile(ref.pidx(Ql), 2))
+    y = ref.rotate(y, Ql, elt, key)
+    y = ref.multiply_plain(y, pt_ref, Ql)
+    y = ref.rescale(y, Ql)
+    y = ref.intt(y, np.tile(ref.pidx(Ql - 1), 2))
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    assert np.array_equal(cts[1].numpy(), y), "C1 pipeline: GPU differs from the oracle"
+    log(f"bench: C1 pipeline GPU {gpu_ms:.3f} ms, oracle {cpu_ms:.1f} ms on {threads} threads")
+    return {"workload": "NTT -> rotate(1) -> multiply_plain -> rescale -> iNTT, N=2^13, 8 x 50-bit + 60-bit",
+            "gpu_ms": round(gpu_ms, 4), "cpu_ms": round(cpu_ms, 2), "cpu_threads": threads, "bitexact": True}
+
+
+def matmul_bench(shm, args, threads):
+    """The MOAI ct-pt matmul at C2 (192 x 192) with the contraction and rescale rooflines."""
+    variant = shm.MATMUL_INT8 if args.variant == "int8" else shm.MATMUL_U64
+    ctx = shm.Context(LOG_N, BITS, 1)
+    rng = np.random.default_rng(5)
+    log("bench: generating 192 input ciphertexts (N=2^16, L=24) and uploading to HBM")
+    xs, hs = make_inputs(ctx, shm, rng)
+    W = np.random.default_rng(6).normal(0.0, 0.02, (ROW_W, COL_W))
+    out = [shm.Ciphertext(ctx, 2, 2) for _ in range(COL_W)]
+    for w in range(2):
+        shm.ct_pt_matrix_mul_wo_pre(hs, W, ROW_W, COL_W, ROW_W, ctx, variant=variant, out=out)
+    if args.copy_calibration:
+        hs[0].copy()
+        log(f"bench: calibration copy of {hs[0].words() * 8} bytes done")
+    kernel_ms, tile_ms = [], []
+    steps = 10
+    t0 = time.perf_counter()
+    for s in range(steps):
+        shm.ct_pt_matrix_mul_wo_pre(hs, W, ROW_W, COL_W, ROW_W, ctx, variant=variant, out=out)
+        kernel_ms.append(shm.last_matmul_kernel_ms())
+        tile_ms.append(shm.last_matmul_rescale_tile_ms())
+    dt = time.perf_counter() - t0  # every matmul call synchronises its stream before returning
+    ms_per_step = dt / steps * 1e3
+    k_ms, t_ms = float(np.mean(kernel_ms)), float(np.mean(tile_ms))
+    N = ctx.N
+    alg_step = ROW_W * 2 * L * N * 8 + COL_W * 2 * (L - 1) * N * 8   # read X, write Y (SURVEY 9.46 GB)
+    contraction_bytes = ROW_W * 2 * L * N * 8 + COL_W * 2 * L * N * 8  # what the contraction kernel moves
+    macs = ROW_W * COL_W * 2 * L * N
+    tops = 2 * 35 * macs / (k_ms * 1e-3) / 1e12
+    traffic_c = measured_traffic("k_contract_int8")
+    traffic_t = measured_traffic("k_ntt_blocks<12,true,2,true>")
+    res = {"workload": "quarter-width ct-pt matmul 192x192 (MOAI column packing), N=2^16, L=24 x 50-bit primes, "
+                       "256 inputs x 128 tokens per ciphertext, incl. rescale",
+           "variant": args.variant, "last_variant": shm.last_matmul_variant(), "steps": steps,
+           "ms_per_step": round(ms_per_step, 4), "s_per_input": ms_per_step * 1e-3 / INPUTS,
+           "algorithmic_bytes_per_step": alg_step,
+           "step_hbm_gbs_algorithmic": round(alg_step / (ms_per_step * 1e-3) / 1e9, 1),
+           "step_frac_of_hbm_peak": round(alg_step / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "contraction": {"bound": "mfma", "kernel": "k_contract_int8", "kernel_ms": round(k_ms, 4),
+                           "achieved": round(tops, 1), "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
+                           "frac": round(tops / INT8_PEAK_TOPS, 4), "traffic": traffic_c,
+                           "kernel_bytes_per_launch": contraction_bytes},
+           "rescale_tile_pass": {"kernel": "k_ntt_blocks<12,true,2,true>", "kernel_ms": round(t_ms, 4),
+                                 "traffic": traffic_t}}
+    if traffic_c and traffic_t:
+        res["traffic_over_algorithmic_step"] = round((traffic_c + traffic_t) / alg_step, 3)
+    log(f"bench: matmul 192x192 {ms_per_step:.3f} ms/step, contraction {k_ms:.3f} ms, rescale tile {t_ms:.3f} ms")
+    if not args.quick:
+        t1 = time.perf_counter()
+        shm.ct_pt_matrix_mul_wo_pre(hs, W, ROW_W, COL_W, ROW_W, ctx, variant=shm.MATMUL_PER_CT, out=out)
+        plain_ms = (time.perf_counter() - t1) * 1e3
+        res["plain_per_ct_ms_per_step"] = round(plain_ms, 2)
+        res["batched_speedup_vs_plain"] = round(plain_ms / ms_per_step, 2)
+        log(f"bench: matmul plain per-ciphertext form {plain_ms:.1f} ms")
+        full = full_width_bench(shm, ctx, hs)
+        full["masked"] = masked_bench(shm, ctx, hs)
+        full["ctct_qk"] = ctct_bench(shm)
+        res["full_width_768"] = full
+        res["ntt"] = ntt_bench(shm, ctx)
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline_matmul(xs, W, threads)
+    return res
 
 
 def main() -> int:
@@ -351,124 +505,149 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--variant", choices=["int8", "u64"], default="int8")
+    ap.add_argument("--quick", action="store_true",
+                    help="headline only: the shared-a n=8 form (steps and one 768 pass)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-plain", action="store_true", help="skip the one-off per-ciphertext form timing")
-    ap.add_argument("--no-bootstrap", action="store_true", help="skip the bootstrap timing")
-    ap.add_argument("--no-full-width", action="store_true", help="skip the 768x768 matmul timing")
+    ap.add_argument("--no-matmul", action="store_true", help="skip the ct-pt matmul objects")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run one device-to-device copy of a 24 MiB ciphertext (profiler byte-counter calibration)")
+                    help="also run one device-to-device copy of a ciphertext (profiler byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
-              "(the matmul shards by coefficient columns, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported: the 768 ciphertexts "
+              "shard by chunk across GPUs with keys replicated (DESIGN.md), and only one GPU is measured here",
+              file=sys.stderr)
         return 2
 
     import shmoai_amd as shm
-    variant = shm.MATMUL_INT8 if args.variant == "int8" else shm.MATMUL_U64
-    ctx = shm.Context(LOG_N, BITS, 1)
-    rng = np.random.default_rng(5)
-    log("bench: generating 192 input ciphertexts (N=2^16, L=24) and uploading to HBM")
-    xs, hs = make_inputs(ctx, shm, rng)
-    W = np.random.default_rng(6).normal(0.0, 0.02, (ROW_W, COL_W))
-    out = [shm.Ciphertext(ctx, 2, 2) for _ in range(COL_W)]
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    cpu_info = host_cpu()
+    t_setup = time.perf_counter()
+    ctx, sk, boot, steps, gk, rk = boot_setup(shm, t_setup)
+    keys8 = shm.SharedAKeys(ctx, sk, 8, 192, 12, ctx.chain(4))
+    keys8.add_galois(steps, seed=13)
+    log(f"bench: shared-a n=8 keys resident ({time.perf_counter() - t_setup:.1f} s)")
+    zs, cts = make_boot_inputs(shm, ctx, sk, N_CTS)
+    log(f"bench: {N_CTS} input ciphertexts encrypted and resident in HBM")
+    outs = [shm.Ciphertext(ctx, 2, 1) for _ in range(N_CTS)]
+    sample = [0, 7, 100, 383, 384, 500, 760, 767]
+
+    # headline form: shared-a n = 8, 96 chunks through the chunked entry point
+    nch = N_CTS // 8
+
+    def step(k):
+        c = k % nch
+        shm.bootstrap_shared_a_s2c_first(boot, keys8, cts[8 * c:8 * (c + 1)], rk, out=outs[8 * c:8 * (c + 1)])
 
     for w in range(args.warmup):
-        shm.ct_pt_matrix_mul_wo_pre(hs, W, ROW_W, COL_W, ROW_W, ctx, variant=variant, out=out)
+        step(w)
         log(f"bench: warmup step {w + 1}/{args.warmup} done")
-    if args.copy_calibration:
-        hs[0].copy()
-        log(f"bench: calibration copy of {hs[0].words() * 8} bytes done")
-    kernel_ms, tile_ms = [], []
+    shm.synchronize()
     t0 = time.perf_counter()
-    for s in range(args.steps):
-        shm.ct_pt_matrix_mul_wo_pre(hs, W, ROW_W, COL_W, ROW_W, ctx, variant=variant, out=out)
-        kernel_ms.append(shm.last_matmul_kernel_ms())
-        tile_ms.append(shm.last_matmul_rescale_tile_ms())
-        assert shm.last_matmul_variant() == variant, "matmul fell back to another variant"
-    dt = time.perf_counter() - t0  # every call synchronises its stream before returning
+    for k in range(args.steps):
+        step(k)  # each call waits for its chunk
+    shm.synchronize()
+    dt = time.perf_counter() - t0
     ms_per_step = dt / args.steps * 1e3
-    k_ms = float(np.mean(kernel_ms))
-    t_ms = float(np.mean(tile_ms))
-    log(f"bench: {args.steps} steps, {ms_per_step:.3f} ms/step, contraction kernel {k_ms:.3f} ms, "
-        f"rescale tile pass {t_ms:.3f} ms")
+    log(f"bench: {args.steps} steps (chunks of 8), {ms_per_step:.2f} ms/step")
+    full_s, _ = timed(shm, lambda: shm.bootstrap_shared_a_s2c_first_chunks(boot, keys8, cts, rk, out=outs))
+    err8 = max_err(shm, ctx, sk, outs, zs, sample)
+    log(f"bench: shared-a n=8, all {N_CTS} in one chunked call: {full_s:.2f} s, max err {err8:.2e}")
+    boot768 = {"params": "N=2^16, logn=15, Q=1x51+20x46+14x51, P=5x51 (dnum 7), HW 192, scale 2^46, S2C-first, "
+                         "EvalMod deg-63 Chebyshev + 2 double angles, output 22 primes",
+               "shared_a_n8_s": round(full_s, 3), "max_abs_err_shared_a_n8": err8}
 
-    N = ctx.N
-    bytes_alg = ROW_W * 2 * L * N * 8 + COL_W * 2 * L * N * 8        # read X limbs + write accumulators
-    macs = ROW_W * COL_W * 2 * L * N                                  # modular MACs
-    hbm_gbs = bytes_alg / (k_ms * 1e-3) / 1e9
-    digit_ops = 2 * 35 * macs                                         # 7 x 5 int8 digit products
-    tops = digit_ops / (k_ms * 1e-3) / 1e12
-    contraction = {"bound": "mfma" if variant == shm.MATMUL_INT8 else "hbm",
-                   "achieved": round(tops if variant == shm.MATMUL_INT8 else hbm_gbs, 2),
-                   "peak": INT8_PEAK_TOPS if variant == shm.MATMUL_INT8 else HBM_PEAK_GBS,
-                   "unit": "TOP/s" if variant == shm.MATMUL_INT8 else "GB/s",
-                   "traffic": measured_traffic("k_contract_int8") if variant == shm.MATMUL_INT8 else None,
-                   "kernel": "k_contract_int8" if variant == shm.MATMUL_INT8 else "k_contract_u64",
-                   "kernel_ms": round(k_ms, 4),
-                   "hbm_achieved_gbs": round(hbm_gbs, 1), "hbm_frac": round(hbm_gbs / HBM_PEAK_GBS, 4),
-                   "algorithmic_bytes_per_launch": bytes_alg, "modular_macs_per_launch": macs}
-    contraction["frac"] = round(contraction["achieved"] / contraction["peak"], 4)
-    # the fused rescale's tile pass: per output limb it reads the delta limb and the
-    # accumulator limb and writes the output limb (24 N bytes), 2 x 23 limbs per output
-    tile_bytes = 24 * N * COL_W * 2 * (L - 1)
-    tile_gbs = tile_bytes / (t_ms * 1e-3) / 1e9
-    tile = {"bound": "hbm", "achieved": round(tile_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(tile_gbs / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic("k_ntt_blocks<12,true,2,true>"),
-            "kernel": "k_ntt_blocks<12,true,2,true> (fused rescale NTT, tile pass)", "kernel_ms": round(t_ms, 4),
-            "algorithmic_bytes_per_launch": tile_bytes,
-            "note": "VALU-bound on 64-bit modular butterflies; HBM is the roofline it is quoted against"}
-    # the roofline object describes the step's dominant kernel; the other one rides along
-    roofline, other = (tile, contraction) if t_ms > k_ms else (contraction, tile)
+    # roofline of the dominant kernel and the key-switch work of one eager chunk
+    shm.keyswitch_counters_reset()
+    shm.probe_fused_ntt_begin()
+    shm.bootstrap_shared_a_s2c_first(boot, keys8, cts[:8], rk, out=outs[:8])
+    pr = shm.probe_fused_ntt_end()
+    counters = shm.keyswitch_counters()
+    avg_ms = pr["ms"] / max(pr["launches"], 1)
+    alg = pr["bytes"] / max(pr["launches"], 1)
+    gbs = alg / (avg_ms * 1e-3) / 1e9
+    traffic = measured_traffic("k_ntt_blocks<12,true,2,true>", BOOT_TRAFFIC_PROFILE)
+    roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "k_ntt_blocks<12,true,2,true> (fused last NTT pass: ModDown combine and rescale)",
+                "launches_per_chunk": pr["launches"], "kernel_ms_avg": round(avg_ms, 5),
+                "algorithmic_bytes_per_launch": int(alg),
+                "kernel_share_of_chunk_ms": round(pr["ms"], 2),
+                "note": "VALU-bound on 64-bit modular butterflies; quoted against HBM"}
+    if traffic:
+        roofline["traffic_over_algorithmic"] = round(traffic / alg, 3)
+    log(f"bench: fused NTT pass {pr['launches']} launches, {avg_ms * 1e3:.1f} us avg, {gbs:.0f} GB/s algorithmic")
 
-    extra = {}
-    if not args.no_plain:
-        t1 = time.perf_counter()
-        shm.ct_pt_matrix_mul_wo_pre(hs, W, ROW_W, COL_W, ROW_W, ctx, variant=shm.MATMUL_PER_CT, out=out)
-        plain_ms = (time.perf_counter() - t1) * 1e3
-        extra = {"plain_per_ct_ms_per_step": round(plain_ms, 2),
-                 "batched_speedup_vs_plain": round(plain_ms / ms_per_step, 2)}
-        log(f"bench: plain per-ciphertext form {plain_ms:.1f} ms")
+    if not args.quick:
+        # shared-a n = 4
+        keys4 = shm.SharedAKeys(ctx, sk, 4, 192, 8, ctx.chain(4))
+        keys4.add_galois(steps, seed=9)
+        shm.bootstrap_shared_a_s2c_first(boot, keys4, cts[:4], rk, out=outs[:4])  # warm-up
+        s4, _ = timed(shm, lambda: shm.bootstrap_shared_a_s2c_first_chunks(boot, keys4, cts, rk, out=outs))
+        e4 = max_err(shm, ctx, sk, outs, zs, sample)
+        boot768.update({"shared_a_n4_s": round(s4, 3), "max_abs_err_shared_a_n4": e4})
+        log(f"bench: shared-a n=4, all {N_CTS}: {s4:.2f} s, max err {e4:.2e}")
+        del keys4
+        gc.collect()
+        # ordinary S2C-first, groups of 8 (one call over all 768)
+        so, _ = timed(shm, lambda: boot.bootstrap_s2c_first(cts, gk, rk, out=outs))
+        eo = max_err(shm, ctx, sk, outs, zs, sample)
+        boot768.update({"ordinary_group8_s": round(so, 3), "max_abs_err_ordinary_group8": eo})
+        log(f"bench: ordinary groups of 8, all {N_CTS}: {so:.2f} s, max err {eo:.2e}")
 
-    full = None
-    if not args.no_full_width and variant == shm.MATMUL_INT8:
-        full = full_width_bench(shm, ctx, hs)
-        full["masked"] = masked_bench(shm, ctx, hs)
-        full["ctct_qk"] = ctct_bench(shm, ctx, hs)
-
-    ntt = ntt_bench(shm, ctx)
-    boot = None
-    if not args.no_bootstrap:
-        boot = bootstrap_bench(shm)
+        # the plain per-ciphertext form: one call per ciphertext, each waited for
+        def sequential():
+            for i in range(N_CTS):
+                boot.bootstrap_s2c_first([cts[i]], gk, rk, out=[outs[i]])
+        ss, _ = timed(shm, sequential)
+        es = max_err(shm, ctx, sk, outs, zs, sample)
+        best = min(full_s, s4, so)
+        boot768.update({"plain_sequential_s": round(ss, 3), "max_abs_err_plain_sequential": es,
+                        "best_batched_s": round(best, 3),
+                        "batched_speedup_vs_plain": round(ss / best, 3)})
+        log(f"bench: plain per-ciphertext form, {N_CTS} sequential calls: {ss:.2f} s, max err {es:.2e}")
+        enc = shm.Encoder(ctx)
+        boot768["keyswitch"] = keyswitch_bench(shm, ctx, sk, gk, min(s for s in steps if s > 0), enc)
+        boot768["serving"] = serving_bench(shm, boot, gk, rk, cts[:64])
+    boot768["keyswitch_counts_per_chunk_n8"] = {str(q): list(v) for q, v in counters.items()}
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"bench: CPU baseline (oracle) on {threads} threads")
-        cpu = cpu_baseline(xs, W, threads)
+        log(f"bench: CPU baseline (oracle key-switch primitives) on {threads} threads")
+        cpu = cpu_baseline_c3(shm, counters, 8, threads)
+        cpu.update(cpu_info)
+        cpu["threads_note"] = ("OMP_NUM_THREADS of the GPU box (its CPU share for one GPU)"
+                               if os.environ.get("OMP_NUM_THREADS") else "min(16, cpu_count)")
+
+    del outs, cts
+    gc.collect()
+    extra = {}
+    if not args.no_matmul and not args.quick:
+        extra["c1_pipeline"] = c1_pipeline(shm, threads)
+        extra["matmul_192"] = matmul_bench(shm, args, threads)
 
     line = {
         "metric": METRIC,
-        "value": ms_per_step * 1e-3 / INPUTS,
-        "unit": "s/input",
+        "value": round(full_s, 3),
+        "unit": "s per 768-ct batched bootstrap",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": False,
         "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u64 (int8 MFMA digits, exact)" if variant == shm.MATMUL_INT8 else "u64",
-        "data": "synthetic (uniform random limbs, W ~ N(0,0.02))",
-        "config": {"workload": "quarter-width ct-pt matmul 192x192 (MOAI column packing), N=2^16, "
-                               "L=24 x 50-bit primes, 256 inputs x 128 tokens per ciphertext, incl. rescale",
-                   "variant": args.variant, "inputs_per_step": INPUTS},
+        "vs_baseline": round(full_s / REF_SHARED_A_N8_S, 4),
+        "vs_baseline_note": "value / 193.3 s, the reference's 768-ct shared-a n=8 figure on an RTX PRO 6000 "
+                            "(README.md:32-34); below 1 is faster",
+        "dtype": "u64 (exact RNS limbs; FP64 only in the encoder)",
+        "data": "synthetic (slots U(-1,1), random-init keys)",
+        "config": {"workload": "768-ciphertext shared-a S2C-first bootstrap (one BERT-base layer's bootstrapping "
+                               "segment), C3: N=2^16, Q=1x51+20x46+14x51, P=5x51, HW 192, scale 2^46; "
+                               "96 chunks of n=8",
+                   "step": "one chunk of 8 of the 768 ciphertexts", "ciphertexts": N_CTS,
+                   "value_from_steps_s": round(ms_per_step * 1e-3 * N_CTS / 8, 3)},
         "roofline": roofline,
-        "roofline_other_kernel": other,
         "cpu_baseline": cpu,
-        "full_width_768": full,
-        "ntt": ntt,
-        "bootstrap_768": boot,
+        "bootstrap_768": boot768,
         **extra,
     }
     print(json.dumps(line), flush=True)

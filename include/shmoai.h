@@ -81,7 +81,7 @@ int shm_synchronize(shm_stream_t stream);
  * would grow allocates), then shm_graph_capture_begin, the calls,
  * shm_graph_capture_end, and shm_graph_launch as often as needed. Temporaries are
  * allocated and freed inside the graph. Entry points documented as synchronising
- * (matmul, bootstrap, encode/decode with host buffers, import/export) cannot be
+ * (matmul, encode/decode with host buffers, import/export, key generation) cannot be
  * captured. A ciphertext or plaintext last used on a caller-owned stream returns its
  * memory on that stream when destroyed: destroy it before that stream. Streams from
  * shm_stream_create are kept alive by the library after shm_stream_destroy (drained and
@@ -432,7 +432,9 @@ int shm_ct_ct_matrix_mul_diagpacking(shm_context *ctx, const shm_ciphertext *con
                                      shm_ciphertext *const *out, shm_stream_t stream);
 /* BatchBootstrapper::bootstrap_ordinary_s2c_first (BatchBootstrap.cuh:674-709) over
  * `count` ciphertexts with exactly 4 primes; out[i] receives the refreshed
- * ciphertext at scale out_scale (0 = the input scale). Inputs are not modified. */
+ * ciphertext at scale out_scale (0 = the input scale). Inputs are not modified.
+ * Synchronises the stream. Outputs keep their buffers across calls (the result is
+ * copied into them). */
 int shm_bootstrap_s2c_first(shm_bootstrapper *b, const shm_ciphertext *const *in, size_t count,
                             shm_ciphertext *const *out, const shm_galois_keys *gk, const shm_kswitch_key *relin,
                             double out_scale, shm_stream_t stream);
@@ -496,10 +498,56 @@ int shm_bootstrap_shared_slot_to_coeff(shm_bootstrapper *b, shm_shared_a_keys *k
 int shm_bootstrap_shared_coeff_to_slot(shm_bootstrapper *b, shm_shared_a_keys *keys, shm_shared_a_batch *batch,
                                        shm_shared_a_batch *re, shm_shared_a_batch *im, shm_stream_t stream);
 /* BatchBootstrapper::bootstrap_shared_a_s2c_first (BatchBootstrap.cuh:719-820): exactly n
- * ciphertexts with 4 primes (n = the keys' batch size); out_scale 0 = the input scale */
+ * ciphertexts with 4 primes (n = the keys' batch size); out_scale 0 = the input scale.
+ * Synchronises the stream. */
 int shm_bootstrap_shared_a_s2c_first(shm_bootstrapper *b, shm_shared_a_keys *keys, const shm_ciphertext *const *in,
                                      size_t n, shm_ciphertext *const *out, const shm_kswitch_key *relin,
                                      double out_scale, shm_stream_t stream);
+
+/* BatchBootstrapper::bootstrap_shared_a_s2c_first over count = k n ciphertexts, k chunks of
+ * the keys' batch size n in order (the driver loop of test_shared_a.cuh:1336-1392).
+ * Synchronises the stream once, after the last chunk. */
+int shm_bootstrap_shared_a_s2c_first_chunks(shm_bootstrapper *b, shm_shared_a_keys *keys,
+                                            const shm_ciphertext *const *in, size_t count,
+                                            shm_ciphertext *const *out, const shm_kswitch_key *relin,
+                                            double out_scale, shm_stream_t stream);
+
+/* ---- graph-captured bootstrapping (no counterpart in the reference, whose driver loop
+ * launches every kernel of every chunk from the host) ------------------------------------
+ * One chunk is recorded into a HIP graph on `stream` (a stream from shm_stream_create): a
+ * shared-a batch of n = the keys' batch size, or an ordinary group of `group` (1..8)
+ * ciphertexts. Creation runs the chunk once eagerly on zero inputs, then records it.
+ * shm_boot_graph_run bootstraps count (a multiple of the chunk) ciphertexts with the
+ * recorded level and scale: per chunk it copies the inputs into the graph's input
+ * buffers, replays the graph and copies the outputs out, all on the recording stream,
+ * ordered after the work queued on the default stream and before later default-stream
+ * work; it only enqueues. The bootstrapper and keys must outlive the graph.
+ * Currently switched off: creation returns SHM_ERR_LOGIC (the first replay of a
+ * recorded chunk on an MI355X faulted; the cause is not established). */
+typedef struct shm_boot_graph shm_boot_graph;
+int shm_boot_graph_create_shared_a(shm_bootstrapper *b, shm_shared_a_keys *keys, const shm_kswitch_key *relin,
+                                   double in_scale, double out_scale, shm_stream_t stream, shm_boot_graph **out);
+int shm_boot_graph_create_ordinary(shm_bootstrapper *b, const shm_galois_keys *gk, const shm_kswitch_key *relin,
+                                   uint32_t group, double in_scale, double out_scale, shm_stream_t stream,
+                                   shm_boot_graph **out);
+int shm_boot_graph_run(shm_boot_graph *g, const shm_ciphertext *const *in, size_t count, shm_ciphertext *const *out);
+/* chunk size and the number of recorded graph nodes */
+int shm_boot_graph_info(const shm_boot_graph *g, uint32_t *chunk, uint64_t *nodes);
+int shm_boot_graph_destroy(shm_boot_graph *g);
+
+/* ---- measurement hooks (no counterpart in the reference, which times no kernel) -------
+ * Probe of the fused last NTT pass (the tile pass that writes the ModDown combine
+ * (cx - NTT(delta)) P^-1 and the rescale (acc - NTT(last)) q_last^-1): between begin and
+ * end every eager launch is bracketed by timing events on its stream; end waits for
+ * the device and returns the launch count, the summed kernel milliseconds and the
+ * algorithmic bytes of those launches (per output limb: the result written and the
+ * combined operand read, plus the output read where it accumulates). */
+int shm_probe_fused_ntt_begin(void);
+int shm_probe_fused_ntt_end(uint64_t *launches, double *ms, double *bytes);
+/* Key-switching work since the last reset, per level Ql (entry Ql, cap entries): polynomials
+ * ModUp'd, inner-product output polynomials and polynomials ModDown'd. */
+int shm_keyswitch_counters(uint64_t *modup, uint64_t *inner, uint64_t *moddown, size_t cap);
+int shm_keyswitch_counters_reset(void);
 
 #ifdef __cplusplus
 }

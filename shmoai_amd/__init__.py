@@ -595,8 +595,9 @@ class Bootstrapper:
         return re, im
 
     def bootstrap_s2c_first(self, cts: Sequence[Ciphertext], gk: GaloisKeys, relin_key: KSwitchKey,
-                            out_scale: float = 0.0, stream=None) -> list:
-        out = [Ciphertext(self.ctx, 2, 1) for _ in cts]
+                            out_scale: float = 0.0, stream=None, out=None) -> list:
+        """Enqueue-only; `out` reuses existing output ciphertexts."""
+        out = [Ciphertext(self.ctx, 2, 1) for _ in cts] if out is None else out
         ins = (C.c_void_p * len(cts))(*[c.h.value for c in cts])
         outs = (C.c_void_p * len(cts))(*[o.h.value for o in out])
         check(_lib().shm_bootstrap_s2c_first(self.h, ins, len(cts), outs, gk.h, relin_key.h, out_scale, stream))
@@ -800,6 +801,29 @@ class Graph:
             pass
 
 
+def probe_fused_ntt_begin() -> None:
+    """Arm the timing probe of the fused last NTT pass (ModDown combine, rescale)."""
+    check(_lib().shm_probe_fused_ntt_begin())
+
+
+def probe_fused_ntt_end() -> dict:
+    """Launch count, summed kernel ms and algorithmic bytes of the probed launches."""
+    n, ms, by = C.c_uint64(), C.c_double(), C.c_double()
+    check(_lib().shm_probe_fused_ntt_end(C.byref(n), C.byref(ms), C.byref(by)))
+    return {"launches": n.value, "ms": ms.value, "bytes": by.value}
+
+
+def keyswitch_counters(cap: int = 128) -> dict:
+    """Key-switching work per level since the last reset: {Ql: (modup, inner, moddown)}."""
+    a, b, c = (np.zeros(cap, np.uint64) for _ in range(3))
+    check(_lib().shm_keyswitch_counters(_u64p(a), _u64p(b), _u64p(c), cap))
+    return {q: (int(a[q]), int(b[q]), int(c[q])) for q in range(cap) if a[q] or b[q] or c[q]}
+
+
+def keyswitch_counters_reset() -> None:
+    check(_lib().shm_keyswitch_counters_reset())
+
+
 def synchronize(stream=None) -> None:
     """Wait for all work queued on `stream` (None: the default stream)."""
     check(_lib().shm_synchronize(stream))
@@ -954,9 +978,63 @@ class SharedAKeys:
 
 
 def bootstrap_shared_a_s2c_first(boot: "Bootstrapper", keys: SharedAKeys, cts: Sequence[Ciphertext],
-                                 relin_key: KSwitchKey, out_scale: float = 0.0, stream=None) -> list:
-    """BatchBootstrap.cuh:719-820: shared-a S2C-first bootstrap of exactly keys.n ciphertexts."""
-    out = [Ciphertext(keys.ctx, 2, 1) for _ in cts]
+                                 relin_key: KSwitchKey, out_scale: float = 0.0, stream=None, out=None) -> list:
+    """BatchBootstrap.cuh:719-820: shared-a S2C-first bootstrap of exactly keys.n ciphertexts
+    (enqueue-only; `out` reuses existing output ciphertexts)."""
+    out = [Ciphertext(keys.ctx, 2, 1) for _ in cts] if out is None else out
     check(_lib().shm_bootstrap_shared_a_s2c_first(boot.h, keys.h, _ptrs(cts), len(cts), _ptrs(out), relin_key.h,
                                                   out_scale, stream))
     return out
+
+
+def bootstrap_shared_a_s2c_first_chunks(boot: "Bootstrapper", keys: SharedAKeys, cts: Sequence[Ciphertext],
+                                        relin_key: KSwitchKey, out_scale: float = 0.0, stream=None,
+                                        out=None) -> list:
+    """The shared-a bootstrap over k * keys.n ciphertexts, k chunks in order (enqueue-only)."""
+    out = [Ciphertext(keys.ctx, 2, 1) for _ in cts] if out is None else out
+    check(_lib().shm_bootstrap_shared_a_s2c_first_chunks(boot.h, keys.h, _ptrs(cts), len(cts), _ptrs(out),
+                                                         relin_key.h, out_scale, stream))
+    return out
+
+
+class BootGraph:
+    """One bootstrap chunk recorded into a HIP graph on a created stream and replayed
+    per chunk of a longer input list (shm_boot_graph_*). Build with shared_a() or
+    ordinary(); run() only enqueues (ordered against the default stream)."""
+
+    def __init__(self, h, ctx, keepalive):
+        self.h, self.ctx, self._keep = h, ctx, keepalive
+        c, nodes = C.c_uint32(), C.c_uint64()
+        check(_lib().shm_boot_graph_info(self.h, C.byref(c), C.byref(nodes)))
+        self.chunk, self.nodes = c.value, nodes.value
+
+    @classmethod
+    def shared_a(cls, boot: "Bootstrapper", keys: SharedAKeys, relin_key: KSwitchKey, in_scale: float,
+                 out_scale: float = 0.0, stream=None):
+        stream = stream or Stream()
+        h = C.c_void_p()
+        check(_lib().shm_boot_graph_create_shared_a(boot.h, keys.h, relin_key.h, float(in_scale), float(out_scale),
+                                                    stream.h, C.byref(h)))
+        return cls(h, boot.ctx, (boot, keys, relin_key, stream))
+
+    @classmethod
+    def ordinary(cls, boot: "Bootstrapper", gk: GaloisKeys, relin_key: KSwitchKey, group: int, in_scale: float,
+                 out_scale: float = 0.0, stream=None):
+        stream = stream or Stream()
+        h = C.c_void_p()
+        check(_lib().shm_boot_graph_create_ordinary(boot.h, gk.h, relin_key.h, group, float(in_scale),
+                                                    float(out_scale), stream.h, C.byref(h)))
+        return cls(h, boot.ctx, (boot, gk, relin_key, stream))
+
+    def run(self, cts: Sequence[Ciphertext], out=None) -> list:
+        out = [Ciphertext(self.ctx, 2, 1) for _ in cts] if out is None else out
+        check(_lib().shm_boot_graph_run(self.h, _ptrs(cts), len(cts), _ptrs(out)))
+        return out
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                _lib().shm_boot_graph_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass

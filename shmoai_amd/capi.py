@@ -181,6 +181,16 @@ _SIGS = {
     "shm_multiply_inplace_reduced_error": (i32, [vp, vp, vp, vp, vp]),
     "shm_double_inplace": (i32, [vp, vp, vp]),
     "shm_rescale_to_next_inplace_noalloc": (i32, [vp, vp, vp]),
+    "shm_bootstrap_shared_a_s2c_first_chunks": (i32, [vp, vp, pvp, sz, pvp, vp, dbl, vp]),
+    "shm_boot_graph_create_shared_a": (i32, [vp, vp, vp, dbl, dbl, vp, pvp]),
+    "shm_boot_graph_create_ordinary": (i32, [vp, vp, vp, u32, dbl, dbl, vp, pvp]),
+    "shm_boot_graph_run": (i32, [vp, pvp, sz, pvp]),
+    "shm_boot_graph_info": (i32, [vp, C.POINTER(u32), C.POINTER(u64)]),
+    "shm_boot_graph_destroy": (i32, [vp]),
+    "shm_probe_fused_ntt_begin": (i32, []),
+    "shm_probe_fused_ntt_end": (i32, [C.POINTER(u64), C.POINTER(dbl), C.POINTER(dbl)]),
+    "shm_keyswitch_counters": (i32, [pu64, pu64, pu64, sz]),
+    "shm_keyswitch_counters_reset": (i32, []),
 }
 
 _lib = None

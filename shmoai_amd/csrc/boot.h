@@ -179,6 +179,27 @@ void bootstrap_shared_a_s2c_first(Bootstrapper &b, SharedAKeys &K, const std::ve
                                   const std::vector<Ciphertext *> &out, const KSwitchKey &rk, double out_scale,
                                   hipStream_t s);
 
+// Graph-captured chunks (boot_graph.hip): `body` bootstraps one chunk of fixed
+// inputs into fixed outputs on a stream; it is run once eagerly, then recorded into
+// a graph on that (created, non-default) stream. run_boot_graph replays the graph for
+// every chunk of a larger input list (inputs copied into the staging buffers, outputs
+// copied out), on the recording stream, enqueue-only.
+using ChunkFn = std::function<void(const std::vector<const Ciphertext *> &, const std::vector<Ciphertext *> &,
+                                   hipStream_t)>;
+struct BootGraph {
+    Context *ctx = nullptr;
+    hipStream_t stream = nullptr;
+    size_t chunk = 0, in_chain = 0, nodes = 0;
+    double in_scale = 0.0;
+    std::vector<std::unique_ptr<Ciphertext>> in, out;  // staging buffers the graph reads / writes
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~BootGraph();
+};
+std::unique_ptr<BootGraph> make_boot_graph(Context &c, size_t chunk, size_t in_chain, double in_scale,
+                                           const ChunkFn &body, hipStream_t s);
+void run_boot_graph(BootGraph &g, const std::vector<const Ciphertext *> &in, const std::vector<Ciphertext *> &out);
+
 // Request server with continuous batching of S2C-first bootstraps (server.hip).
 struct ServerStats {
     uint64_t submitted, completed, failed, ciphertexts, groups;

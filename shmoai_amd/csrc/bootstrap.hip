@@ -725,7 +725,9 @@ void bootstrap_s2c_first_batch(Bootstrapper &b, const std::vector<const Cipherte
             ct_mul_monomial_half(c, im[k], false, s);  // i * m_im
             add_checked(c, re[k], im[k], s);
             re[k].scale = osc;
-            *out[i + k] = std::move(re[k]);
+            // copied into the caller's buffer (kept across calls, so a captured graph
+            // writes the same memory on every replay)
+            ct_clone(*out[i + k], re[k], s);
         }
         i = j;
     }

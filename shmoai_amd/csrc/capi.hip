@@ -1476,6 +1476,34 @@ int shm_bootstrap_shared_a_s2c_first(shm_bootstrapper *b, shm_shared_a_keys *key
     });
 }
 
+int shm_bootstrap_shared_a_s2c_first_chunks(shm_bootstrapper *b, shm_shared_a_keys *keys,
+                                            const shm_ciphertext *const *in, size_t count,
+                                            shm_ciphertext *const *out, const shm_kswitch_key *relin,
+                                            double out_scale, shm_stream_t st) {
+    return guard([&] {
+        boot::Bootstrapper &bs = ready(b, S(st));
+        need(keys, "keys");
+        need(relin, "relin_keys");
+        const size_t n = keys->k->n;
+        if (count % n) throw InvalidArgument("the ciphertext count must be a multiple of the batch size n");
+        if (count && (!in || !out)) throw InvalidArgument("in/out is null");
+        for (size_t i = 0; i < count; i++) {
+            need(in[i], "in[i]");
+            need(out[i], "out[i]");
+        }
+        std::vector<const Ciphertext *> vi(n);
+        std::vector<Ciphertext *> vo(n);
+        for (size_t c = 0; c < count; c += n) {
+            for (size_t i = 0; i < n; i++) {
+                vi[i] = &in[c + i]->ct;
+                vo[i] = &out[c + i]->ct;
+            }
+            boot::bootstrap_shared_a_s2c_first(bs, *keys->k, vi, vo, relin->k, out_scale, S(st));
+        }
+        SHM_HIP(hipStreamSynchronize(S(st)));
+    });
+}
+
 int shm_bootstrap_shared_slot_to_coeff(shm_bootstrapper *b, shm_shared_a_keys *keys, shm_shared_a_batch *batch,
                                        shm_stream_t st) {
     return guard([&] {
@@ -1651,6 +1679,111 @@ int shm_double_inplace(shm_context *ctx, shm_ciphertext *ct, shm_stream_t st) {
         need(ct, "ct");
         launch_add(*ctx->c, ct->ct.data(), ct->ct.data(), ct->ct.Ql(), ct->ct.size, S(st));
     });
+}
+
+}  // extern "C"
+
+// ---- graph-captured bootstrapping ------------------------------------------------------------
+struct shm_boot_graph {
+    std::unique_ptr<boot::BootGraph> g;
+};
+
+extern "C" {
+
+int shm_boot_graph_create_shared_a(shm_bootstrapper *b, shm_shared_a_keys *keys, const shm_kswitch_key *relin,
+                                   double in_scale, double out_scale, shm_stream_t st, shm_boot_graph **out) {
+    return guard([&] {
+        boot::Bootstrapper &bs = ready(b, S(st));
+        need(keys, "keys");
+        need(relin, "relin_keys");
+        need(out, "out");
+        if (!(in_scale > 0)) throw InvalidArgument("in_scale must be positive");
+        boot::SharedAKeys *K = keys->k.get();
+        const KSwitchKey *rk = &relin->k;
+        boot::ChunkFn body = [&bs, K, rk, out_scale](const std::vector<const Ciphertext *> &in,
+                                                      const std::vector<Ciphertext *> &o, hipStream_t s) {
+            boot::bootstrap_shared_a_s2c_first(bs, *K, in, o, *rk, out_scale, s);
+        };
+        auto h = std::make_unique<shm_boot_graph>();
+        h->g = boot::make_boot_graph(*bs.ctx, K->n, K->fmt_chain, in_scale, body, S(st));
+        *out = h.release();
+    });
+}
+
+int shm_boot_graph_create_ordinary(shm_bootstrapper *b, const shm_galois_keys *gk, const shm_kswitch_key *relin,
+                                   uint32_t group, double in_scale, double out_scale, shm_stream_t st,
+                                   shm_boot_graph **out) {
+    return guard([&] {
+        boot::Bootstrapper &bs = ready(b, S(st));
+        need(gk, "galois_keys");
+        need(relin, "relin_keys");
+        need(out, "out");
+        if (group < 1 || group > 8) throw InvalidArgument("group: 1..8 ciphertexts");
+        if (!(in_scale > 0)) throw InvalidArgument("in_scale must be positive");
+        const GaloisKeys *g = &gk->g;
+        const KSwitchKey *rk = &relin->k;
+        boot::ChunkFn body = [&bs, g, rk, out_scale](const std::vector<const Ciphertext *> &in,
+                                                      const std::vector<Ciphertext *> &o, hipStream_t s) {
+            boot::bootstrap_s2c_first_batch(bs, in, o, *g, *rk, out_scale, s);
+        };
+        const size_t chain = bs.ctx->chain_of_ql(bs.prm.s2c_input_primes);
+        auto h = std::make_unique<shm_boot_graph>();
+        h->g = boot::make_boot_graph(*bs.ctx, group, chain, in_scale, body, S(st));
+        *out = h.release();
+    });
+}
+
+int shm_boot_graph_run(shm_boot_graph *g, const shm_ciphertext *const *in, size_t count, shm_ciphertext *const *out) {
+    return guard([&] {
+        need(g, "graph");
+        if (count && (!in || !out)) throw InvalidArgument("in/out is null");
+        std::vector<const Ciphertext *> vi(count);
+        std::vector<Ciphertext *> vo(count);
+        for (size_t i = 0; i < count; i++) {
+            need(in[i], "in[i]");
+            need(out[i], "out[i]");
+            vi[i] = &in[i]->ct;
+            vo[i] = &out[i]->ct;
+        }
+        boot::run_boot_graph(*g->g, vi, vo);
+    });
+}
+
+int shm_boot_graph_info(const shm_boot_graph *g, uint32_t *chunk, uint64_t *nodes) {
+    return guard([&] {
+        need(g, "graph");
+        if (chunk) *chunk = (uint32_t)g->g->chunk;
+        if (nodes) *nodes = (uint64_t)g->g->nodes;
+    });
+}
+
+int shm_boot_graph_destroy(shm_boot_graph *g) {
+    return guard([&] { delete g; });
+}
+
+}  // extern "C"
+
+// ---- measurement hooks -------------------------------------------------------------------------
+extern "C" {
+
+int shm_probe_fused_ntt_begin(void) {
+    return guard([&] { probe_fused_ntt_begin(); });
+}
+
+int shm_probe_fused_ntt_end(uint64_t *launches, double *ms, double *bytes) {
+    return guard([&] {
+        size_t n = 0;
+        probe_fused_ntt_end(&n, ms, bytes);
+        if (launches) *launches = n;
+    });
+}
+
+int shm_keyswitch_counters(uint64_t *modup, uint64_t *inner, uint64_t *moddown, size_t cap) {
+    return guard([&] { ks_counters(modup, inner, moddown, cap); });
+}
+
+int shm_keyswitch_counters_reset(void) {
+    return guard([&] { ks_counters_reset(); });
 }
 
 }  // extern "C"

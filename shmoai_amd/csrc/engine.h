@@ -224,6 +224,15 @@ void ntt_forward(const Context &c, uint64_t *data, size_t n_polys, size_t n_limb
 // Events recorded around the tile pass of every fused-rescale NTT launched while
 // set (nullptr to stop); used to time the matmul's rescale.
 void set_rescale_tile_events(hipEvent_t start, hipEvent_t stop);
+// Timing probe of the fused last NTT pass (ModDown combine and rescale) over eager
+// launches: begin clears and arms it; end waits for the device and returns the
+// launch count, the summed kernel time and the algorithmic bytes of those launches.
+void probe_fused_ntt_begin();
+void probe_fused_ntt_end(size_t *launches, double *ms, double *bytes);
+// Key-switch work counters, per level Ql (index < 128): polynomials ModUp'd,
+// inner-product output polynomials, polynomials ModDown'd since the last reset.
+void ks_counters(uint64_t *modup, uint64_t *inner, uint64_t *moddown, size_t cap);
+void ks_counters_reset();
 // Encoder decompose + forward NTT in one: out[p][k] = NTT of the rounded FFT
 // output fft[p] (N/2 complex entries: real parts are coefficients 0..N/2-1,
 // imaginary parts N/2..N-1) reduced mod prime k, for k < n_limbs.

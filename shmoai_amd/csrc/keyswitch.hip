@@ -11,6 +11,8 @@
 // out = (cx - delta) P^-1 mod q (adding onto the output where asked).
 #include <algorithm>
 
+#include <atomic>
+
 #include "engine.h"
 
 namespace shm {
@@ -255,9 +257,35 @@ __global__ __launch_bounds__(256) void k_ks_inner_batch(const uint64_t *__restri
 }
 
 
+// work counters per level (host side, relaxed): ModUp'd polynomials, inner-product
+// output polynomials and ModDown'd polynomials
+constexpr size_t kCounterLevels = 128;
+std::atomic<uint64_t> g_cnt_modup[kCounterLevels], g_cnt_inner[kCounterLevels], g_cnt_moddown[kCounterLevels];
+
+inline void bump(std::atomic<uint64_t> *a, size_t Ql, size_t by) {
+    if (Ql < kCounterLevels) a[Ql].fetch_add(by, std::memory_order_relaxed);
+}
+
 }  // namespace
 
+void ks_counters(uint64_t *modup, uint64_t *inner, uint64_t *moddown, size_t cap) {
+    for (size_t i = 0; i < cap && i < kCounterLevels; i++) {
+        if (modup) modup[i] = g_cnt_modup[i].load(std::memory_order_relaxed);
+        if (inner) inner[i] = g_cnt_inner[i].load(std::memory_order_relaxed);
+        if (moddown) moddown[i] = g_cnt_moddown[i].load(std::memory_order_relaxed);
+    }
+}
+
+void ks_counters_reset() {
+    for (size_t i = 0; i < kCounterLevels; i++) {
+        g_cnt_modup[i].store(0, std::memory_order_relaxed);
+        g_cnt_inner[i].store(0, std::memory_order_relaxed);
+        g_cnt_moddown[i].store(0, std::memory_order_relaxed);
+    }
+}
+
 void ks_modup(const Context &c, const uint64_t *c2, size_t Ql, uint64_t *d, hipStream_t s, bool rounded) {
+    bump(g_cnt_modup, Ql, 1);
     const size_t N = c.N, alpha = c.nP, QlP = Ql + alpha;
     const LevelConsts &L = c.level(Ql);
     const size_t beta = L.beta;
@@ -278,6 +306,7 @@ void ks_inner_multi(const Context &c, const uint64_t *d, size_t Ql, size_t ndig,
                     bool local, uint64_t *cx, hipStream_t s, const uint32_t *perm, bool accumulate) {
     if (nout == 0 || nout > (size_t)kMaxKeys) throw InvalidArgument("ks_inner_multi: 1..34 outputs");
     if (ndig == 0 || ndig > (size_t)kMaxDigits) throw InvalidArgument("ks_inner_multi: 1..16 digits");
+    bump(g_cnt_inner, Ql, nout);
     const size_t QlP = Ql + c.nP, total = QlP * c.N;
     int lgog = 0;
     while (lgog < 4 && (size_t(1) << lgog) < nout) lgog++;
@@ -300,6 +329,7 @@ void ks_moddown_pack(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, c
                      uint64_t acc_mask, hipStream_t s, bool rounded) {
     const size_t N = c.N, nP = c.nP, QlP = Ql + nP;
     if (npolys == 0 || npolys > 64) throw InvalidArgument("ks_moddown: 1..64 polynomials");
+    bump(g_cnt_moddown, Ql, npolys);
     Scratch delta(npolys * Ql * N * sizeof(uint64_t), s);
     ntt_inverse_qlp(c, cx, Ql, Ql, QlP, npolys, QlP * N, s);
     if (nP > (size_t)kMaxAlpha) throw InvalidArgument("at most 16 special primes");
@@ -320,6 +350,7 @@ void keyswitch(const Context &c, const uint64_t *c2, const uint64_t *key, size_t
     Scratch d(beta * QlP * N * sizeof(uint64_t), s);
     Scratch cx(2 * QlP * N * sizeof(uint64_t), s);
     ks_modup(c, c2, Ql, d.as<uint64_t>(), s, rounded);
+    bump(g_cnt_inner, Ql, 2);
     k_ks_inner<<<grid_for(QlP * N), kBlock, 0, s>>>(d.as<uint64_t>(), key, cx.as<uint64_t>(), c.mods(), Ql, c.nQ,
                                                     c.nQP(), alpha, beta, N);
     SHM_HIP(hipGetLastError());
@@ -338,6 +369,7 @@ void ks_modup_batch(const Context &c, const PolyPack &c2, size_t m, size_t Ql, u
     const size_t beta = L.beta;
     if (m == 0 || m > (size_t)kPolyPack) throw InvalidArgument("ks_modup_batch: 1..64 members");
     if (alpha > (size_t)kMaxAlpha) throw InvalidArgument("at most 16 special primes");
+    bump(g_cnt_modup, Ql, m);
     // gather + iNTT, conversion, one NTT over m beta QlP limbs
     Scratch tcoef(m * Ql * N * sizeof(uint64_t), s);
     launch_gather_polys(c, c2, tcoef.as<uint64_t>(), Ql * N, m, s);
@@ -369,6 +401,7 @@ void ks_inner_batch(const Context &c, const uint64_t *d, size_t m, size_t Ql, co
     if (beta > c.dnum) throw LogicError("key switching key has too few digits for this level");
     if (beta > (size_t)kMaxDigits) throw InvalidArgument("at most 16 key digits");
     if (m == 0 || m > 64) throw InvalidArgument("ks_inner_batch: 1..64 members");
+    bump(g_cnt_inner, Ql, 2 * m);
     int lgmg = 0;
     while (lgmg < 4 && (size_t(1) << lgmg) < m) lgmg++;
     const size_t total = QlP * c.N;

@@ -21,6 +21,9 @@
 // (values stay below 36q). Outputs are canonical residues.
 #include <algorithm>
 
+#include <mutex>
+#include <vector>
+
 #include "engine.h"
 
 namespace shm {
@@ -433,6 +436,16 @@ __global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ 
 // optional events around the tile pass of the fused rescale (timing of one launch)
 hipEvent_t g_rescale_tile_ev[2] = {nullptr, nullptr};
 
+// Probe of the fused last pass (IO_RESCALE tile pass of forward NTTs: the ModDown
+// combine and the rescale): while on, every eager launch is bracketed by a pair of
+// timing events and its algorithmic bytes are recorded.
+struct FusedProbe {
+    std::mutex mu;
+    bool on = false;
+    std::vector<hipEvent_t> ev;  // start, stop per launch
+    double bytes = 0.0;
+} g_probe;
+
 template <bool FWD, int IOA, int IOB>
 void launch_ntt(const Context &c, uint64_t *data, const LimbMap &mp, size_t total_limbs, hipStream_t s,
                 const NttIO &io = NttIO{}) {
@@ -503,7 +516,27 @@ void launch_ntt(const Context &c, uint64_t *data, const LimbMap &mp, size_t tota
         cols();
         const bool timed = IOB == IO_RESCALE && g_rescale_tile_ev[0] && g_rescale_tile_ev[1];
         if (timed) SHM_HIP(hipEventRecord(g_rescale_tile_ev[0], s));
+        hipEvent_t pe[2] = {nullptr, nullptr};
+        if (IOB == IO_RESCALE) {
+            std::lock_guard<std::mutex> lk(g_probe.mu);
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            if (g_probe.on && hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) {
+                SHM_HIP(hipEventCreate(&pe[0]));
+                SHM_HIP(hipEventCreate(&pe[1]));
+                g_probe.ev.push_back(pe[0]);
+                g_probe.ev.push_back(pe[1]);
+                // per output limb: the result written and the combined operand read
+                // (plus the output read where the pass accumulates onto it); the
+                // transform's intermediate written by the column pass is not counted
+                size_t acc_polys = 0;
+                if (io.out_packed)
+                    for (uint32_t p = 0; p < n_polys && p < 64; p++) acc_polys += (io.acc_mask >> p) & 1;
+                g_probe.bytes += 16.0 * N * total_limbs + 8.0 * N * mp.n_active * acc_polys;
+            }
+        }
+        if (pe[0]) SHM_HIP(hipEventRecord(pe[0], s));
         blks();
+        if (pe[1]) SHM_HIP(hipEventRecord(pe[1], s));
         if (timed) SHM_HIP(hipEventRecord(g_rescale_tile_ev[1], s));
     } else {
         blks();
@@ -519,6 +552,31 @@ void ntt_forward(const Context &c, uint64_t *data, size_t n_polys, size_t n_limb
     LimbMap mp{(uint32_t)n_limbs, 0, 0xffffffffu, 0, 0xffffffffu, (uint32_t)c.nQ, (uint32_t)first_prime,
                (uint64_t)n_limbs * c.N};
     launch_ntt<true, IO_PLAIN, IO_PLAIN>(c, data, mp, n_polys * n_limbs, s);
+}
+
+void probe_fused_ntt_begin() {
+    std::lock_guard<std::mutex> lk(g_probe.mu);
+    for (hipEvent_t e : g_probe.ev) (void)hipEventDestroy(e);
+    g_probe.ev.clear();
+    g_probe.bytes = 0.0;
+    g_probe.on = true;
+}
+
+void probe_fused_ntt_end(size_t *launches, double *ms, double *bytes) {
+    SHM_HIP(hipDeviceSynchronize());
+    std::lock_guard<std::mutex> lk(g_probe.mu);
+    g_probe.on = false;
+    double total = 0.0;
+    for (size_t i = 0; i + 1 < g_probe.ev.size(); i += 2) {
+        float t = 0.0f;
+        SHM_HIP(hipEventElapsedTime(&t, g_probe.ev[i], g_probe.ev[i + 1]));
+        total += t;
+    }
+    if (launches) *launches = g_probe.ev.size() / 2;
+    if (ms) *ms = total;
+    if (bytes) *bytes = g_probe.bytes;
+    for (hipEvent_t e : g_probe.ev) (void)hipEventDestroy(e);
+    g_probe.ev.clear();
 }
 
 void set_rescale_tile_events(hipEvent_t start, hipEvent_t stop) {

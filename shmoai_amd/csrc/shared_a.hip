@@ -478,7 +478,7 @@ void bootstrap_shared_a_s2c_first(Bootstrapper &b, SharedAKeys &K, const std::ve
         ct_mul_monomial_half(c, ims[i], false, s);
         ct_add(c, res[i], ims[i], s);
         res[i].scale = out_scale;
-        *out[i] = std::move(res[i]);
+        ct_clone(*out[i], res[i], s);  // the caller's buffer is kept (graph replays write it)
     }
 }
 
