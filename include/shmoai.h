@@ -302,8 +302,9 @@ double shm_last_matmul_kernel_ms(void);
 /* duration of the tile pass of the last batched matmul's fused rescale NTT, ms (HIP events) */
 double shm_last_matmul_rescale_tile_ms(void);
 /* variant the last matmul actually ran: SHM_MATMUL_INT8 falls back to SHM_MATMUL_U64
- * when a prime is outside (2^46, 2^61] or a weight needs more than 5 balanced
- * digits (4 for 8-digit limbs). Any row_W is accepted: rows beyond 256 are
+ * when a prime is outside (2^45, 2^62), a limb/weight digit pair has no compiled
+ * contraction (6 or 7 limb digits with 4..6 weight digits, 8 with 4), or a row chunk's
+ * sum could reach 2^106 (256 rows x 2^(8 digits - 1) x the largest |round(W scale)|). Any row_W is accepted: rows beyond 256 are
  * contracted in chunks (ct_pt_matrix_mul_wo_pre_large, Ct_pt_matrix_mul.cuh:584). */
 int shm_last_matmul_variant(void);
 

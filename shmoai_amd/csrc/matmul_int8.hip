@@ -82,8 +82,8 @@ __global__ void k_weight_frags(const double *W, const double *scales, i8x16 *fra
     }
 }
 
-// corr[l][i] = [128 R rowsum_i - 2^105]_q (the 2^105 offset keeps the
-// recombined plane sum non-negative); mu[l] = floor(2^108 / q_l).
+// corr[l][i] = [128 R rowsum_i - 2^106]_q (the 2^106 offset keeps the
+// recombined plane sum non-negative); mu[l] = floor(2^109 / q_l).
 __global__ void k_weight_corr(const double *W, const double *scales, uint64_t *corr, uint64_t *mu,
                               const DevMod *mods, size_t row_w, size_t col_w, size_t Ql, int DX) {
     const size_t total = Ql * col_w;
@@ -97,12 +97,12 @@ __global__ void k_weight_corr(const double *W, const double *scales, uint64_t *c
         uint64_t R = 0;
         for (int a = 0; a < DX; a++) R = add_mod(mul_mod(R, 256, m), 1, m.q);
         const uint64_t c = mul_mod(128 % m.q, R, m);
-        uint64_t p105 = 1;
-        for (int e = 0; e < 105; e++) p105 = add_mod(p105, p105, m.q);
-        corr[t] = sub_mod(mul_mod(c, rs, m), p105, m.q);
+        uint64_t p106 = 1;
+        for (int e = 0; e < 106; e++) p106 = add_mod(p106, p106, m.q);
+        corr[t] = sub_mod(mul_mod(c, rs, m), p106, m.q);
         if (i == 0) {
-            // floor(2^108 / q) from floor(2^128 / q): shift by 20 bits
-            mu[l] = (m.mu_hi << 44) | (m.mu_lo >> 20);
+            // floor(2^109 / q) from floor(2^128 / q): shift by 19 bits (fits 64 bits for q > 2^45)
+            mu[l] = (m.mu_hi << 45) | (m.mu_lo >> 19);
         }
     }
 }
@@ -139,8 +139,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DX * KS <= 
                                                        const DevMod *__restrict__ mods, int row_w, int col_w,
                                                        int MB, size_t Ql, size_t N, int accumulate, int radix) {
     constexpr int ND = DX + DW - 1;
-    // |C_d| < 2^24.2, so |S| < 2^104.3 and 0 < S + 2^105 < 2^106 needs ND <= 11.
-    static_assert(ND >= 10 && ND <= 11, "plane recombination assumes 10 or 11 planes");
+    // The planes are recombined modulo 2^128, so S = sum_k x'_k w_k comes out exactly as
+    // long as its true value satisfies |S| < 2^106 (x' the centred digits, |x'| < 2^(8 DX - 1)),
+    // which the host checks from the weights before launching.
+    static_assert(ND >= 9 && ND <= 12, "plane recombination covers 9 to 12 planes");
     static_assert(DX <= 8, "at most 8 digits per limb");
     constexpr int FR = DW * RB * KS * 64;  // fragments staged per row phase
     __shared__ i8x16 wl[FR];
@@ -237,13 +239,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DX * KS <= 
                     const int64_t t = (int64_t)C[d][r] * (int64_t)pw[d & 3];
                     if (d < 4) g0 += t; else if (d < 8) g1 += t; else g2 += t;
                 }
-                // T = S + 2^105 (0 < T < 2^106)
+                // T = S + 2^106 (0 < T < 2^107)
                 const uint64_t a_lo = (uint64_t)g1 << 32;
                 const uint64_t lo = (uint64_t)g0 + a_lo;
                 const uint64_t hi = (uint64_t)(g0 >> 63) + (uint64_t)(g1 >> 32) + (lo < a_lo ? 1 : 0) + (uint64_t)g2 +
-                                    (uint64_t(1) << 41);
-                // quotient from the top 62 bits: T' = T >> 44, qe = floor(T' mu / 2^64)
-                const uint64_t tp = (hi << 20) | (lo >> 44);
+                                    (uint64_t(1) << 42);
+                // quotient from the top 62 bits: T' = T >> 45, qe = floor(T' mu / 2^64)
+                const uint64_t tp = (hi << 19) | (lo >> 45);
                 const uint64_t qe = __umul64hi(tp, mu);
                 uint64_t res = lo - qe * q;  // T - qe q < 4q
                 res = res >= 2 * q ? res - 2 * q : res;
@@ -274,8 +276,12 @@ ContractFn pick(int DX, int DW, int KS) {
         if (KS == 3) return k_contract_int8<dx, dw, 3, rb_for(3)>;                 \
         if (KS == 4) return k_contract_int8<dx, dw, 4, rb_for(4)>;                 \
     }
+    SHM_PICK(6, 4)
+    SHM_PICK(6, 5)
+    SHM_PICK(6, 6)
     SHM_PICK(7, 4)
     SHM_PICK(7, 5)
+    SHM_PICK(7, 6)
     SHM_PICK(8, 4)
 #undef SHM_PICK
     return nullptr;
@@ -298,8 +304,11 @@ bool int8_supported(const Context &c, size_t Ql, const double *W, const double *
         qbits = std::max(qbits, b);
         qmin = std::min(qmin, b);
     }
-    // the single-word quotient estimate needs 2^46 < q < 2^62
-    if (qmin < 47 || qbits > 61) return false;
+    // The single-word quotient estimate qe = floor(floor(T / 2^45) mu / 2^64), with
+    // mu = floor(2^109 / q) and 0 < T < 2^107, satisfies 0 <= T - qe q < q (1 + T / 2^109 + 2^45 / q)
+    // < 2.25 q for 2^45 < q (bit length >= 46), which two conditional subtractions reduce,
+    // and mu fits 64 bits for q > 2^45; q < 2^62 keeps the byte split within 8 digits.
+    if (qmin < 46 || qbits > 61) return false;
     const int DX = (qbits + 7) / 8;
     double mx = 0;
     for (size_t j = 0; j < row_w; j++)
@@ -308,6 +317,10 @@ bool int8_supported(const Context &c, size_t Ql, const double *W, const double *
     // balanced digits: D digits cover |w| <= 127 * (256^D - 1) / 255
     while (DW < 9 && mx > 127.0 * (std::pow(256.0, DW) - 1) / 255.0) DW++;
     if (DW < 4) DW = 4;
+    // the recombined sum S = sum_k x'_k w_k of a row chunk (x' the centred digits of a
+    // limb, |x'| < 2^(8 DX - 1)) must stay below 2^106 in magnitude
+    const size_t nch = (row_w + kChunkRows - 1) / kChunkRows, kc_max = (row_w + nch - 1) / nch;
+    if ((double)kc_max * std::ldexp(1.0, 8 * DX - 1) * mx >= std::ldexp(1.0, 106)) return false;
     if (!pick(DX, DW, 1)) return false;
     *dx_out = DX;
     *dw_out = DW;

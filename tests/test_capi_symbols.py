@@ -33,3 +33,18 @@ def test_bias_vec_known_answer():
     import pytest
     with pytest.raises(shm.capi.ShmInvalidArgument):
         shm.bias_vec([5, 0, 3], 3, 4)  # a length beyond num_row
+
+
+def test_cpp_facade_header_compiles(tmp_path):
+    """include/shmoai.hpp compiles as C++17 with every facade method instantiated by
+    the facade test program (syntax and types only; no GPU)."""
+    import os
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    assert gxx, "g++ is needed for the facade check"
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([gxx, "-std=c++17", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
+                        "-I", os.path.join(root, "include"), os.path.join(root, "tests", "cpp", "facade_c1.cpp")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

@@ -94,6 +94,33 @@ def test_matmul_quarter_width_int8_equals_u64_and_oracle(oracle_lib):
         assert np.array_equal(o8[i].numpy(), exp), f"output {i}"
 
 
+LAYER_BITS = [51] + [46] * 23 + [51] * 11 + [51] * 5   # test_single_layer.cuh:473-535
+
+
+@pytest.mark.parametrize("logN,Ql,row_w,col_w,oracle_cols", [(13, 6, 40, 12, None), (16, 24, 192, 192, (0, 191))])
+def test_matmul_layer_primes_int8_equals_u64_and_oracle(oracle_lib, logN, Ql, row_w, col_w, oracle_cols):
+    """The reference layer's prime chain (a 51-bit prime, then 46-bit primes) at scale 2^46:
+    the INT8-MFMA contraction takes 46-bit primes (7 limb digits, 6 weight digits) and
+    equals the 64-bit path and the oracle."""
+    ctx = shm.Context(logN, LAYER_BITS, 5)
+    ref = oracle_lib.Ref(logN, LAYER_BITS, 5)
+    rng = np.random.default_rng(15)
+    xs, hs = _rand_cts(ctx, ref, rng, row_w, Ql)
+    for h in hs:
+        h.scale = 2.0 ** 46
+    W = np.random.default_rng(16).normal(0, 0.02, (row_w, col_w))
+    o8 = shm.ct_pt_matrix_mul_wo_pre(hs, W, row_w, col_w, row_w, ctx, variant=shm.MATMUL_INT8)
+    assert shm.last_matmul_variant() == shm.MATMUL_INT8
+    o64 = shm.ct_pt_matrix_mul_wo_pre(hs, W, row_w, col_w, row_w, ctx, variant=shm.MATMUL_U64)
+    for i in range(col_w):
+        assert np.array_equal(o8[i].numpy(), o64[i].numpy()), f"output {i}"
+    oracle_lib.set_threads(16)
+    cols = range(col_w) if oracle_cols is None else oracle_cols
+    for i in cols:
+        exp = ref.ctpt_matmul(xs, W, Ql, np.full(row_w, 2.0 ** 46), i, i + 1)[0]
+        assert np.array_equal(o8[i].numpy(), exp), f"output {i}"
+
+
 def test_matmul_known_answer():
     """X[i][j][k] = j+1 (token j), W = 1/128: every slot of token j decodes to (row_W/128)(j+1)."""
     bits = [60] + [40] * 4 + [60]
