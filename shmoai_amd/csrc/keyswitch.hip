@@ -218,7 +218,8 @@ __global__ __launch_bounds__(256) void k_ks_inner_batch(const uint64_t *__restri
                                                         uint64_t *__restrict__ cx, const DevMod *__restrict__ mods,
                                                         size_t Ql, size_t nQ, size_t QP, size_t alpha, int beta,
                                                         size_t N, int members, int lgmg,
-                                                        const uint32_t *__restrict__ perm, PolyPack c2, int own_c2) {
+                                                        const uint32_t *__restrict__ perm, PolyPack c2, int own_c2,
+                                                        int g0, int nb, int accumulate) {
     const size_t QlP = Ql + alpha, total = QlP * N;
     const int lgj = 8 - lgmg;
     const size_t t = ((size_t)blockIdx.x << lgj) + (threadIdx.x & ((1u << lgj) - 1));
@@ -227,32 +228,40 @@ __global__ __launch_bounds__(256) void k_ks_inner_batch(const uint64_t *__restri
     const size_t k = t >> __builtin_ctzll(N), j = t & (N - 1);
     const size_t src = perm ? k * N + perm[j] : t;
     const uint32_t pi = qlp_prime(k, Ql, nQ);
+    // digits g0 .. g0 + nb - 1 of the beta of each member (nb <= MAXB)
     uint64_t kb[MAXB], ka[MAXB];
 #pragma unroll
     for (int g = 0; g < MAXB; g++) {
         kb[g] = ka[g] = 0;
-        if (g < beta) {
-            const uint64_t *kg = key + (size_t)g * 2 * QP * N;
+        if (g < nb) {
+            const uint64_t *kg = key + (size_t)(g0 + g) * 2 * QP * N;
             kb[g] = kg[(size_t)pi * N + j];
             ka[g] = kg[(QP + pi) * N + j];
         }
     }
     const DevMod m = mods[pi];
     for (int mem = ms; mem < members; mem += 1 << lgmg) {
-        const uint64_t *dm = d + (size_t)mem * beta * total + src;
+        const uint64_t *dm = d + ((size_t)mem * beta + g0) * total + src;
         Acc128 a0{0, 0}, a1{0, 0};
 #pragma unroll
         for (int g = 0; g < MAXB; g++) {
-            if (g < beta) {
-                // digit g's own limbs k in [g alpha, g alpha + alpha) are c2's limbs
-                const bool own = own_c2 && k < Ql && k >= (size_t)g * alpha && k < (size_t)(g + 1) * alpha;
+            if (g < nb) {
+                // digit g0 + g's own limbs k in [(g0 + g) alpha, (g0 + g + 1) alpha) are c2's limbs
+                const size_t dg = (size_t)(g0 + g);
+                const bool own = own_c2 && k < Ql && k >= dg * alpha && k < (dg + 1) * alpha;
                 const uint64_t dv = own ? c2.p[mem][src] : dm[(size_t)g * total];
                 mac128(a0, dv, kb[g]);
                 mac128(a1, dv, ka[g]);
             }
         }
-        cx[(size_t)(2 * mem) * total + t] = reduce128(a0, m);
-        cx[(size_t)(2 * mem + 1) * total + t] = reduce128(a1, m);
+        uint64_t v0 = reduce128(a0, m), v1 = reduce128(a1, m);
+        uint64_t *o0 = cx + (size_t)(2 * mem) * total + t, *o1 = cx + (size_t)(2 * mem + 1) * total + t;
+        if (accumulate) {
+            v0 = add_mod(v0, *o0, m.q);
+            v1 = add_mod(v1, *o1, m.q);
+        }
+        *o0 = v0;
+        *o1 = v1;
     }
 }
 
@@ -305,9 +314,19 @@ void ks_modup(const Context &c, const uint64_t *c2, size_t Ql, uint64_t *d, hipS
 void ks_inner_multi(const Context &c, const uint64_t *d, size_t Ql, size_t ndig, const KeyRefs &kr, size_t nout,
                     bool local, uint64_t *cx, hipStream_t s, const uint32_t *perm, bool accumulate) {
     if (nout == 0 || nout > (size_t)kMaxKeys) throw InvalidArgument("ks_inner_multi: 1..34 outputs");
-    if (ndig == 0 || ndig > (size_t)kMaxDigits) throw InvalidArgument("ks_inner_multi: 1..16 digits");
-    bump(g_cnt_inner, Ql, nout);
+    if (ndig == 0) throw InvalidArgument("ks_inner_multi: no digits");
     const size_t QlP = Ql + c.nP, total = QlP * c.N;
+    if (ndig > (size_t)kMaxDigits) {
+        // digits in chunks of kMaxDigits: later chunks add their partial sums mod q
+        for (size_t g0 = 0; g0 < ndig; g0 += kMaxDigits) {
+            KeyRefs k2 = kr;
+            for (size_t o = 0; o < nout; o++) k2.base[o] = kr.base[o] + g0 * kr.stride[o];
+            ks_inner_multi(c, d + g0 * total, Ql, std::min<size_t>(kMaxDigits, ndig - g0), k2, nout, local, cx, s,
+                           perm, accumulate || g0 > 0);
+        }
+        return;
+    }
+    bump(g_cnt_inner, Ql, nout);
     int lgog = 0;
     while (lgog < 4 && (size_t(1) << lgog) < nout) lgog++;
     const unsigned blocks = (unsigned)((total + (256u >> lgog) - 1) >> (8 - lgog));
@@ -399,17 +418,20 @@ void ks_inner_batch(const Context &c, const uint64_t *d, size_t m, size_t Ql, co
                     hipStream_t s, const uint32_t *perm, const PolyPack *own) {
     const size_t QlP = Ql + c.nP, beta = c.level(Ql).beta;
     if (beta > c.dnum) throw LogicError("key switching key has too few digits for this level");
-    if (beta > (size_t)kMaxDigits) throw InvalidArgument("at most 16 key digits");
     if (m == 0 || m > 64) throw InvalidArgument("ks_inner_batch: 1..64 members");
     bump(g_cnt_inner, Ql, 2 * m);
     int lgmg = 0;
     while (lgmg < 4 && (size_t(1) << lgmg) < m) lgmg++;
     const size_t total = QlP * c.N;
     const unsigned blocks = (unsigned)((total + (256u >> lgmg) - 1) >> (8 - lgmg));
-    auto fn = beta <= 4 ? k_ks_inner_batch<4> : beta <= 8 ? k_ks_inner_batch<8> : k_ks_inner_batch<16>;
-    fn<<<blocks, kBlock, 0, s>>>(d, key, cx, c.mods(), Ql, c.nQ, c.nQP(), c.nP, (int)beta, c.N, (int)m, lgmg, perm,
-                                 own ? *own : PolyPack{}, own ? 1 : 0);
-    SHM_HIP(hipGetLastError());
+    // digits in chunks of kMaxDigits; a later chunk adds its partial sums mod q
+    for (size_t g0 = 0; g0 < beta; g0 += kMaxDigits) {
+        const size_t nb = std::min<size_t>(kMaxDigits, beta - g0);
+        auto fn = nb <= 4 ? k_ks_inner_batch<4> : nb <= 8 ? k_ks_inner_batch<8> : k_ks_inner_batch<16>;
+        fn<<<blocks, kBlock, 0, s>>>(d, key, cx, c.mods(), Ql, c.nQ, c.nQP(), c.nP, (int)beta, c.N, (int)m, lgmg,
+                                     perm, own ? *own : PolyPack{}, own ? 1 : 0, (int)g0, (int)nb, g0 > 0 ? 1 : 0);
+        SHM_HIP(hipGetLastError());
+    }
 }
 
 void keyswitch_batch(const Context &c, const PolyPack &c2, size_t m, const uint64_t *key, size_t Ql,

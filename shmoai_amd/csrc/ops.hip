@@ -476,8 +476,7 @@ void rotate_many(Context &c, const std::vector<const Ciphertext *> &in, const st
                  const GaloisKeys &gk, hipStream_t s) {
     const uint32_t elt = c.elt_from_step(step);
     auto it = gk.keys.find(elt);
-    const bool batched = !in.empty() && c.level(in[0]->Ql()).beta <= (size_t)kMaxDigits;
-    if (step == 0 || it == gk.keys.end() || !batched) {
+    if (step == 0 || it == gk.keys.end()) {
         for (size_t k = 0; k < in.size(); k++) {
             ct_clone(*out[k], *in[k], s);
             rotate_internal(c, *out[k], step, gk, s);
@@ -513,13 +512,6 @@ void rotate_many(Context &c, const std::vector<const Ciphertext *> &in, const st
 // relinearize of 3-polynomial ciphertexts at one level (Ql limbs), batched when
 // the level's digit count allows it (same result either way)
 void relin_many(Context &c, const std::vector<Ciphertext *> &cts, const KSwitchKey &rk, size_t Ql, hipStream_t s) {
-    if (c.level(Ql).beta > (size_t)kMaxDigits) {
-        for (Ciphertext *o : cts) {
-            keyswitch(c, o->poly(2), rk.k.as<uint64_t>(), Ql, o->poly(0), o->poly(1), true, s, false);
-            o->size = 2;
-        }
-        return;
-    }
     const size_t m = cts.size();
     PolyPack c2{}, ct01{};
     for (size_t i = 0; i < m; i++) {
@@ -541,8 +533,8 @@ void relin_many(Context &c, const std::vector<Ciphertext *> &cts, const KSwitchK
 // so the result is the same limb for limb; the col_X rotations of one output
 // share a step and go through batched key switches. Every product of an output
 // accumulates into its 3-polynomial ciphertext in the tensor kernel, and the
-// outputs are relinearised and rescaled in batched passes of up to 16. A step
-// that is a multiple of the slot count is the identity.
+// outputs are relinearised and rescaled in batched passes of up to 16. Steps of
+// the slot count or more are refused, as get_elt_from_step does (galois.cu:64-69).
 void ctct_matmul_colpacking(Context &c, const std::vector<const Ciphertext *> &X,
                             const std::vector<const Ciphertext *> &W, int num_batch, const GaloisKeys &gk,
                             const KSwitchKey &rk, const std::vector<Ciphertext *> &out, hipStream_t s) {
@@ -568,6 +560,7 @@ void ctct_matmul_colpacking(Context &c, const std::vector<const Ciphertext *> &X
     }
     const size_t Ql = c.ql_of_chain(chain), slots = c.N / 2;
     if (Ql < 2) throw LogicError("end of modulus switching chain reached");
+    if ((rowX - 1) * (size_t)num_batch >= slots) throw InvalidArgument("step count too large");
     constexpr size_t kChunk = 16;
     std::vector<Ciphertext> rot(colX);
     std::vector<Ciphertext *> rotp(colX);
@@ -582,7 +575,7 @@ void ctct_matmul_colpacking(Context &c, const std::vector<const Ciphertext *> &X
             o.scale = X[0]->scale * W[0]->scale;
         }
         for (size_t i = i0; i < i1; i++) {
-            const size_t step = (i * (size_t)num_batch) % slots;
+            const size_t step = i * (size_t)num_batch;
             if (step != 0) rotate_many(c, W, rotp, (int)step, gk, s);  // every W[j] by the same step
             for (size_t j = 0; j < colX; j++) {
                 const Ciphertext *w = step != 0 ? &rot[j] : W[j];
@@ -639,7 +632,10 @@ void ctct_matmul_diagpacking(Context &c, const std::vector<const Ciphertext *> &
     size_t b = colX / g;
     if (b * g < colX) b++;
     if (b > 32) throw InvalidArgument("ct_ct_matrix_mul_diag: col_X above 1024");
-    auto rot_step = [&](size_t step) { return (int)(step % slots); };
+    // every rotation step stays below the slot count (get_elt_from_step, galois.cu:64-69)
+    if ((b > 1 && (colX - g) * nb >= slots) || (g - 1) * nb >= slots || (b - 1) * g * nb >= slots)
+        throw InvalidArgument("step count too large");
+    auto rot_step = [&](size_t step) { return (int)step; };
     // pre-rotated X (giant 0 uses X as it is)
     std::vector<Ciphertext> rx(colX);
     std::vector<const Ciphertext *> xr(colX);

@@ -122,9 +122,9 @@ def test_ctct_diagpacking_bitexact(env):
 
 
 def test_ctct_colpacking_many_digits_bitexact(oracle_lib):
-    """18 key digits (one special prime, 18 data primes): above the batched key
-    switch's 16 digits the rotations and the relinearisation take single key
-    switches; the result is still the oracle's limb for limb."""
+    """18 key digits (one special prime, 18 data primes): the batched key switch takes
+    its digits in chunks of 16 (the second chunk adds onto the first); the result is
+    still the oracle's limb for limb."""
     bits = [50] + [30] * 17 + [50]
     ctx = shm.Context(12, bits, 1)
     ref = oracle_lib.Ref(12, bits, 1)
@@ -145,3 +145,72 @@ def test_ctct_colpacking_many_digits_bitexact(oracle_lib):
                                      ref.gen_relin_key(s_ref, 5))
     for i in range(3):
         assert np.array_equal(out[i].numpy(), exp[i]), f"output {i}"
+
+
+@pytest.fixture(scope="module")
+def many(oracle_lib):
+    bits = [50] + [30] * 17 + [50]
+    ctx = shm.Context(12, bits, 1)
+    ref = oracle_lib.Ref(12, bits, 1)
+    sk = shm.SecretKey(ctx, seed=1, hamming_weight=32)
+    s_ref = ref.sk_ntt(ref.sample_ternary(1, 32))
+    return ctx, ref, sk, s_ref
+
+
+def test_ctct_diagpacking_many_digits_bitexact(many):
+    """Diagonal packing (baby-step giant-step) with 18 key digits, bit-exact."""
+    ctx, ref, sk, s_ref = many
+    rng = np.random.default_rng(41)
+    enc = shm.Encoder(ctx)
+    n = ctx.N // 2
+    col_x, col_w, nb = 5, 2, 2
+    X = [sk.encrypt_symmetric(enc.encode(rng.uniform(-1, 1, n), 2.0 ** 30, 1), seed=400 + j) for j in range(col_x)]
+    W = [sk.encrypt_symmetric(enc.encode(rng.uniform(-1, 1, n), 2.0 ** 30, 1), seed=410 + j) for j in range(col_w)]
+    Ql = X[0].coeff_modulus_size
+    steps = sorted({k * nb for k in range(1, 3)} | {(col_x - 3) * nb, 3 * nb})
+    gk = sk.create_galois_keys_from_steps(steps, seed=9)
+    rk = sk.gen_relinkey(seed=5)
+    out = shm.ct_ct_matrix_mul_diagpacking(X, W, gk, rk, ctx, col_x, col_x, col_w, col_x, nb)
+    gk_ref = {st: ref.gen_galois_key(s_ref, ref.elt(st), ref.splitmix64(9, ref.elt(st))) for st in steps}
+    exp = ref.ctct_matmul_diagpacking([x.numpy() for x in X], [w.numpy() for w in W], Ql, nb, gk_ref,
+                                      ref.gen_relin_key(s_ref, 5))
+    for i in range(col_w):
+        assert np.array_equal(out[i].numpy(), exp[i]), f"output {i}"
+
+
+def test_chebyshev_many_digits_batch_equals_single(many):
+    """The batched products of the Chebyshev evaluation (one key switch pass for the
+    group) at 18 key digits equal single calls limb for limb and decrypt to the series."""
+    from numpy.polynomial import chebyshev as npc
+    ctx, ref, sk, s_ref = many
+    rng = np.random.default_rng(42)
+    enc, dec = shm.Encoder(ctx), shm.Decryptor(ctx, sk)
+    n = ctx.N // 2
+    coeffs = np.array([0.1, 0.5, -0.2, 0.05, 0.02, -0.01])
+    xs = [rng.uniform(-1, 1, n) for _ in range(3)]
+    batch = [sk.encrypt_symmetric(enc.encode(x, 2.0 ** 30, 1), seed=500 + i) for i, x in enumerate(xs)]
+    single = [sk.encrypt_symmetric(enc.encode(x, 2.0 ** 30, 1), seed=500 + i) for i, x in enumerate(xs)]
+    rk = sk.gen_relinkey(seed=5)
+    ev = shm.Evaluator(ctx)
+    ev.evaluate_chebyshev_inplace(batch, coeffs, -1.0, 1.0, rk)
+    for c in single:
+        ev.evaluate_chebyshev_inplace(c, coeffs, -1.0, 1.0, rk)
+    for i in range(3):
+        assert np.array_equal(batch[i].numpy(), single[i].numpy()), f"member {i}"
+        got = enc.decode(dec.decrypt(batch[i]))
+        assert float(np.max(np.abs(got - npc.chebval(xs[i], coeffs)))) < 1e-4
+
+
+def test_ctct_steps_beyond_slot_count_refused(many):
+    """get_elt_from_step refuses steps of the slot count or more (galois.cu:64-69): so do
+    both products, before any work."""
+    ctx, ref, sk, s_ref = many
+    enc = shm.Encoder(ctx)
+    n = ctx.N // 2
+    X = [sk.encrypt_symmetric(enc.encode(np.zeros(n), 2.0 ** 30, 1), seed=600 + j) for j in range(2)]
+    gk = sk.create_galois_keys_from_steps([1], seed=9)
+    rk = sk.gen_relinkey(seed=5)
+    with pytest.raises(shm.capi.ShmInvalidArgument):
+        shm.ct_ct_matrix_mul_colpacking(X, X, gk, rk, ctx, 2, 3, 2, 3, n // 2)  # step 2 * n/2 = n
+    with pytest.raises(shm.capi.ShmInvalidArgument):
+        shm.ct_ct_matrix_mul_diagpacking(X, X, gk, rk, ctx, 2, 2, 2, 2, n)
