@@ -273,6 +273,27 @@ def test_scalar_encode_equals_full_fft_path(oracle_lib):
             assert np.all(got[k * r.N:(k + 1) * r.N] == v % int(r.primes[k]))
 
 
+def test_encode_matches_vandermonde_solution(oracle_lib):
+    """The encoder's coefficients against the canonical embedding solved directly in
+    numpy (no FFT): m(w_j) = scale z_j and m(conj w_j) = scale conj(z_j) with
+    w_j = exp(2 pi i 5^j / 2N), j < N/2, a linear system for the N real coefficients;
+    the oracle's plaintext, brought back to coefficients, equals round(m) per prime."""
+    r = oracle_lib.Ref(6, [40, 41, 50], 1)  # N = 64, two data primes
+    N, n = r.N, r.N // 2
+    rng = np.random.default_rng(10)
+    z = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+    scale = 2.0 ** 30
+    roots = np.exp(2j * np.pi * np.array([pow(5, j, 2 * N) for j in range(n)]) / (2 * N))
+    V = np.vstack([roots[:, None] ** np.arange(N), np.conj(roots)[:, None] ** np.arange(N)])
+    m = np.linalg.solve(V, scale * np.concatenate([z, np.conj(z)])).real
+    pt = r.encode(z.real, 2, scale, im=z.imag)
+    coef = r.intt(pt, r.pidx(2)).reshape(2, N)
+    for k in range(2):
+        q = int(r.primes[k])
+        want = np.array([int(v) % q for v in np.round(m)], np.uint64)
+        assert np.array_equal(coef[k], want)
+
+
 def test_encode_decode_roundtrip(oracle_lib):
     r = oracle_lib.Ref(12, [60, 40, 40, 60], 1)
     rng = np.random.default_rng(9)
