@@ -407,8 +407,8 @@ int shm_server_get_stats(shm_server *srv, shm_server_stats *stats);
  * non_linear_func/gelu_other.cuh:257-330: sum_k d_k T_k(u), u affine in x):
  * each of `count` (1..32) ciphertexts at one level and scale becomes
  * sum_{k < n_coeffs} coeffs[k] T_k((2x - lo - hi) / (hi - lo)) in place, at the input
- * scale. Degree n_coeffs - 1 in 1..63; baby steps T_1..T_15, giant steps T_16 and
- * T_32; depth 1 (affine map) + ceil(log2(n_coeffs)) + 1 levels at most. All
+ * scale. Degree n_coeffs - 1 in 1..63; baby steps T_1..T_8, giant steps T_8, T_16
+ * and T_32; depth 1 (affine map) + ceil(log2(n_coeffs)) + 1 levels at most. All
  * ciphertexts go through each product together (one key switch and rescale pass). */
 int shm_evaluate_chebyshev(shm_context *ctx, shm_ciphertext *const *cts, size_t count, const double *coeffs,
                            size_t n_coeffs, double lo, double hi, const shm_kswitch_key *relin, shm_stream_t stream);

@@ -196,7 +196,8 @@ std::vector<double> evalmod_chebyshev(int K, int r, int deg) {
 
 void cheb_split(const std::vector<double> &p, size_t m, std::vector<double> &q, std::vector<double> &r) {
     const size_t deg = p.size() - 1;
-    if (deg < m || deg >= 2 * m) throw InvalidArgument("cheb_split: degree must be in [m, 2m)");
+    // T_m T_m = (T_2m + T_0) / 2 covers deg = 2m
+    if (deg < m || deg > 2 * m) throw InvalidArgument("cheb_split: degree must be in [m, 2m]");
     q.assign(p.begin(), p.begin() + m);
     r.assign(deg - m + 1, 0.0);
     r[0] = p[m];

@@ -255,6 +255,10 @@ struct ChebEval {
         T[1] = std::make_unique<Ciphertext>(std::move(t1));
     }
 
+    // leaves are combinations of T_1..T_8; a longer polynomial of degree d splits
+    // as q + T_m r with m = split_a(d), the largest power of two below d
+    static constexpr size_t kLeaf = 8;
+
     static int split_a(int k) {
         int a = 1;
         while (2 * a < k) a *= 2;
@@ -300,8 +304,8 @@ struct ChebEval {
     // chain index eval() lands on
     size_t eval_chain(const std::vector<double> &p) const {
         const size_t deg = p.size() - 1;
-        if (deg < 16) return baby_chain(p);
-        const size_t m = deg >= 32 ? 32 : 16;
+        if (deg <= kLeaf) return baby_chain(p);
+        const size_t m = (size_t)split_a((int)deg);
         std::vector<double> q, r;
         cheb_split(p, m, q, r);
         return std::max(tchain((int)m), eval_chain(r)) + 1;
@@ -331,16 +335,18 @@ struct ChebEval {
     }
 
     // Homomorphic_poly_evaluation (common/Polynomial.cu:253-476) in Chebyshev
-    // form: p = q + T_m r with m in {32, 16}, babies T_1..T_15. The result
-    // carries exactly `target` as its scale: r is evaluated at the scale that
-    // the product with T_m maps onto `target`.
+    // form: p = q + T_m r with m = split_a(deg) in {8, 16, 32}, recursively, down
+    // to leaves over the babies T_1..T_8 (degree 63: 7 baby and 2 giant products,
+    // 7 combination products, depth 7). The result carries exactly `target` as
+    // its scale: r is evaluated at the scale that the product with T_m maps onto
+    // `target`.
     void eval(const std::vector<double> &p, double target, Ciphertext &out) {
         const size_t deg = p.size() - 1;
-        if (deg < 16) {
+        if (deg <= kLeaf) {
             baby(p, baby_chain(p), target, out);
             return;
         }
-        const size_t m = deg >= 32 ? 32 : 16;
+        const size_t m = (size_t)split_a((int)deg);
         std::vector<double> q, r;
         cheb_split(p, m, q, r);
         const size_t pc = std::max(tchain((int)m), eval_chain(r));
@@ -357,7 +363,7 @@ struct ChebEval {
             return;
         }
         Ciphertext Q;
-        if (q.size() <= 16) {
+        if (q.size() <= kLeaf + 1) {
             baby(q, out.chain, target, Q);
         } else {
             eval(q, target, Q);
@@ -451,8 +457,8 @@ struct ChebEvalBatch {
     }
     size_t eval_chain(const std::vector<double> &p) const {
         const size_t deg = p.size() - 1;
-        if (deg < 16) return baby_chain(p);
-        const size_t mm = deg >= 32 ? 32 : 16;
+        if (deg <= ChebEval::kLeaf) return baby_chain(p);
+        const size_t mm = (size_t)ChebEval::split_a((int)deg);
         std::vector<double> q, r;
         cheb_split(p, mm, q, r);
         return std::max(tchain((int)mm), eval_chain(r)) + 1;
@@ -487,11 +493,11 @@ struct ChebEvalBatch {
 
     void eval(const std::vector<double> &p, double target, const Group &out) {
         const size_t deg = p.size() - 1;
-        if (deg < 16) {
+        if (deg <= ChebEval::kLeaf) {
             baby(p, baby_chain(p), target, out);
             return;
         }
-        const size_t mm = deg >= 32 ? 32 : 16;
+        const size_t mm = (size_t)ChebEval::split_a((int)deg);
         std::vector<double> q, r;
         cheb_split(p, mm, q, r);
         const size_t pc = std::max(tchain((int)mm), eval_chain(r));
@@ -509,7 +515,7 @@ struct ChebEvalBatch {
             return;
         }
         Group Q = fresh();
-        if (q.size() <= 16) {
+        if (q.size() <= ChebEval::kLeaf + 1) {
             baby(q, out[0]->chain, target, Q);
         } else {
             eval(q, target, Q);

@@ -7,7 +7,9 @@ gelu_cheb, non_linear_func/gelu_other.cuh:257-330).
   T_32) decrypt within 2e-5 of the FP64 evaluation of the same series;
 - the output keeps the input scale and lands on the level the depth predicts;
 - a batch of three equals three single calls limb for limb (members do not
-  interact).
+  interact);
+- degrees 9, 16, 32 and 48, where the split meets a power of two, evaluate at the
+  predicted depth.
 """
 import math
 
@@ -80,6 +82,25 @@ def test_chebyshev_matches_fp64(env, case):
     if case == "gelu_cheb_reference":
         # the fit itself: gelu_cheb's stated maximum error 0.037 on the interval
         assert float(np.max(np.abs(want - _gelu(x)))) < 0.04
+
+
+@pytest.mark.parametrize("deg,depth", [(9, 5), (16, 6), (32, 7), (48, 7)])
+def test_chebyshev_split_degrees(env, deg, depth):
+    """Degrees where a split meets a power of two (deg = 2m: T_m T_m = (T_2m + T_0) / 2)
+    or leaves a quotient of degree at most 8; depth = 1 (affine map) + the recursion's."""
+    ctx, sk, rk, enc, dec, ev = env
+    rng = np.random.default_rng(100 + deg)
+    lo, hi = -1.5, 2.5
+    coeffs = rng.normal(0, 1, deg + 1) / (1.0 + np.arange(deg + 1)) ** 1.5
+    x = rng.uniform(lo, hi, ctx.N // 2)
+    ct = _encrypt(env, x, 60 + deg)
+    chain0 = ct.chain_index
+    ev.evaluate_chebyshev_inplace(ct, coeffs, lo, hi, rk)
+    assert ct.scale == SCALE
+    assert ct.chain_index == chain0 + depth
+    want = npc.chebval((2.0 * x - lo - hi) / (hi - lo), coeffs)
+    err = float(np.max(np.abs(enc.decode(dec.decrypt(ct)) - want)))
+    assert err < 2e-5, err
 
 
 def test_chebyshev_batch_equals_single(env):
