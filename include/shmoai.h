@@ -549,6 +549,13 @@ int shm_probe_fused_ntt_end(uint64_t *launches, double *ms, double *bytes);
  * ModUp'd, inner-product output polynomials and polynomials ModDown'd. */
 int shm_keyswitch_counters(uint64_t *modup, uint64_t *inner, uint64_t *moddown, size_t cap);
 int shm_keyswitch_counters_reset(void);
+/* Launch-shape knobs (no counterpart in the reference; results never depend on them):
+ * "ks_batch_slots_log2" and "ks_multi_slots_log2" set log2 of the member / output slots
+ * per block of the batched and multi-output key inner products (0..4; 0 or -1 = one slot,
+ * the default: each thread loops over all members / outputs); "ks_unroll" (1 or 2) is how
+ * many members / outputs a thread loads for before it multiplies. Process-wide. */
+int shm_set_tuning(const char *name, int32_t value);
+int shm_get_tuning(const char *name, int32_t *value);
 
 #ifdef __cplusplus
 }

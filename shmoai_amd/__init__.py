@@ -824,6 +824,17 @@ def keyswitch_counters_reset() -> None:
     check(_lib().shm_keyswitch_counters_reset())
 
 
+def set_tuning(name: str, value: int) -> None:
+    """Set a launch-shape knob ("ks_batch_slots_log2", "ks_multi_slots_log2"; -1 = default)."""
+    check(_lib().shm_set_tuning(name.encode(), int(value)))
+
+
+def get_tuning(name: str) -> int:
+    v = C.c_int()
+    check(_lib().shm_get_tuning(name.encode(), C.byref(v)))
+    return v.value
+
+
 def synchronize(stream=None) -> None:
     """Wait for all work queued on `stream` (None: the default stream)."""
     check(_lib().shm_synchronize(stream))

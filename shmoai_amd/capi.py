@@ -191,6 +191,8 @@ _SIGS = {
     "shm_probe_fused_ntt_end": (i32, [C.POINTER(u64), C.POINTER(dbl), C.POINTER(dbl)]),
     "shm_keyswitch_counters": (i32, [pu64, pu64, pu64, sz]),
     "shm_keyswitch_counters_reset": (i32, []),
+    "shm_set_tuning": (i32, [C.c_char_p, i32]),
+    "shm_get_tuning": (i32, [C.c_char_p, C.POINTER(i32)]),
 }
 
 _lib = None

@@ -124,7 +124,7 @@ __global__ void k_ks_inner(const uint64_t *__restrict__ d, const uint64_t *__res
 // The OG threads of one coefficient load the same digit words (L1 serves all
 // but the first), so the digits leave HBM once for all outputs, and every
 // thread waits only on its own 2 ndig loads.
-template <int MAXB>
+template <int MAXB, int UNR>
 __global__ __launch_bounds__(256) void k_ks_inner_multi(const uint64_t *__restrict__ d, KeyRefs kr, int nout,
                                                         uint64_t *__restrict__ cx, const DevMod *__restrict__ mods,
                                                         size_t Ql, size_t nQ, size_t alpha, int ndig, size_t N,
@@ -144,15 +144,29 @@ __global__ __launch_bounds__(256) void k_ks_inner_multi(const uint64_t *__restri
 #pragma unroll
     for (int g = 0; g < MAXB; g++) dv[g] = g < ndig ? d[(size_t)g * total + src] : 0;
     const DevMod m = mods[pi];
-    for (int o = os; o < nout; o += 1 << lgog) {
-        Acc128 a{0, 0};
-        const uint64_t *kb = kr.base[o];
-        const size_t st = kr.stride[o];
+    const int step = 1 << lgog;
+    // UNR outputs per iteration: all their key loads are issued before the MACs
+    for (int o0 = os; o0 < nout; o0 += UNR * step) {
+        uint64_t kv[UNR][MAXB];
 #pragma unroll
-        for (int g = 0; g < MAXB; g++)
-            if (g < ndig) mac128(a, dv[g], kb[(size_t)g * st + kl]);
-        const uint64_t v = reduce128(a, m);
-        cx[(size_t)o * total + t] = accumulate ? add_mod(cx[(size_t)o * total + t], v, m.q) : v;
+        for (int u = 0; u < UNR; u++) {
+            const int o = o0 + u * step < nout ? o0 + u * step : o0;
+            const uint64_t *kb = kr.base[o];
+            const size_t st = kr.stride[o];
+#pragma unroll
+            for (int g = 0; g < MAXB; g++) kv[u][g] = g < ndig ? kb[(size_t)g * st + kl] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const int o = o0 + u * step;
+            if (o >= nout) break;
+            Acc128 a{0, 0};
+#pragma unroll
+            for (int g = 0; g < MAXB; g++)
+                if (g < ndig) mac128(a, dv[g], kv[u][g]);
+            const uint64_t v = reduce128(a, m);
+            cx[(size_t)o * total + t] = accumulate ? add_mod(cx[(size_t)o * total + t], v, m.q) : v;
+        }
     }
 }
 
@@ -213,7 +227,7 @@ __global__ void k_moddown_bconv(const uint64_t *__restrict__ cx, uint64_t *__res
 // after the first, so each key word leaves HBM once for the group. Every
 // thread has only its own beta digit loads to wait for, and the grid holds
 // members x QlP x N threads to hide their latency.
-template <int MAXB>
+template <int MAXB, int UNR>
 __global__ __launch_bounds__(256) void k_ks_inner_batch(const uint64_t *__restrict__ d, const uint64_t *__restrict__ key,
                                                         uint64_t *__restrict__ cx, const DevMod *__restrict__ mods,
                                                         size_t Ql, size_t nQ, size_t QP, size_t alpha, int beta,
@@ -240,28 +254,42 @@ __global__ __launch_bounds__(256) void k_ks_inner_batch(const uint64_t *__restri
         }
     }
     const DevMod m = mods[pi];
-    for (int mem = ms; mem < members; mem += 1 << lgmg) {
-        const uint64_t *dm = d + ((size_t)mem * beta + g0) * total + src;
-        Acc128 a0{0, 0}, a1{0, 0};
+    // the digit among g0 .. g0 + nb - 1 whose own limbs hold k (digit dg's own limbs
+    // k in [dg alpha, (dg + 1) alpha) are c2's limbs), or -1
+    const int own_g = own_c2 && k < Ql ? (int)(k / alpha) - g0 : -1;
+    const int step = 1 << lgmg;
+    // UNR members per iteration: all their digit loads are issued before the MACs
+    for (int mem0 = ms; mem0 < members; mem0 += UNR * step) {
+        uint64_t dv[UNR][MAXB];
 #pragma unroll
-        for (int g = 0; g < MAXB; g++) {
-            if (g < nb) {
-                // digit g0 + g's own limbs k in [(g0 + g) alpha, (g0 + g + 1) alpha) are c2's limbs
-                const size_t dg = (size_t)(g0 + g);
-                const bool own = own_c2 && k < Ql && k >= dg * alpha && k < (dg + 1) * alpha;
-                const uint64_t dv = own ? c2.p[mem][src] : dm[(size_t)g * total];
-                mac128(a0, dv, kb[g]);
-                mac128(a1, dv, ka[g]);
+        for (int u = 0; u < UNR; u++) {
+            const int mem = mem0 + u * step;
+            const uint64_t *dm = d + ((size_t)(mem < members ? mem : mem0) * beta + g0) * total + src;
+#pragma unroll
+            for (int g = 0; g < MAXB; g++)
+                dv[u][g] = g < nb && mem < members ? (g == own_g ? c2.p[mem][src] : dm[(size_t)g * total]) : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const int mem = mem0 + u * step;
+            if (mem >= members) break;
+            Acc128 a0{0, 0}, a1{0, 0};
+#pragma unroll
+            for (int g = 0; g < MAXB; g++) {
+                if (g < nb) {
+                    mac128(a0, dv[u][g], kb[g]);
+                    mac128(a1, dv[u][g], ka[g]);
+                }
             }
+            uint64_t v0 = reduce128(a0, m), v1 = reduce128(a1, m);
+            uint64_t *o0 = cx + (size_t)(2 * mem) * total + t, *o1 = cx + (size_t)(2 * mem + 1) * total + t;
+            if (accumulate) {
+                v0 = add_mod(v0, *o0, m.q);
+                v1 = add_mod(v1, *o1, m.q);
+            }
+            *o0 = v0;
+            *o1 = v1;
         }
-        uint64_t v0 = reduce128(a0, m), v1 = reduce128(a1, m);
-        uint64_t *o0 = cx + (size_t)(2 * mem) * total + t, *o1 = cx + (size_t)(2 * mem + 1) * total + t;
-        if (accumulate) {
-            v0 = add_mod(v0, *o0, m.q);
-            v1 = add_mod(v1, *o1, m.q);
-        }
-        *o0 = v0;
-        *o1 = v1;
     }
 }
 
@@ -283,6 +311,11 @@ void ks_counters(uint64_t *modup, uint64_t *inner, uint64_t *moddown, size_t cap
         if (inner) inner[i] = g_cnt_inner[i].load(std::memory_order_relaxed);
         if (moddown) moddown[i] = g_cnt_moddown[i].load(std::memory_order_relaxed);
     }
+}
+
+KsTuning &ks_tuning() {
+    static KsTuning t;
+    return t;
 }
 
 void ks_counters_reset() {
@@ -327,10 +360,15 @@ void ks_inner_multi(const Context &c, const uint64_t *d, size_t Ql, size_t ndig,
         return;
     }
     bump(g_cnt_inner, Ql, nout);
+    // one output slot by default: each thread loads its digits once and loops over the
+    // outputs (measured fastest on the shared-a bootstrap chunk)
     int lgog = 0;
-    while (lgog < 4 && (size_t(1) << lgog) < nout) lgog++;
+    if (ks_tuning().multi_slots_log2 > 0)
+        while (lgog < ks_tuning().multi_slots_log2 && (size_t(1) << lgog) < nout) lgog++;
+    const int unr = ks_tuning().unroll;
     const unsigned blocks = (unsigned)((total + (256u >> lgog) - 1) >> (8 - lgog));
-    auto fn = ndig <= 4 ? k_ks_inner_multi<4> : ndig <= 8 ? k_ks_inner_multi<8> : k_ks_inner_multi<16>;
+    auto fn = unr == 2 ? (ndig <= 4 ? k_ks_inner_multi<4, 2> : ndig <= 8 ? k_ks_inner_multi<8, 2> : k_ks_inner_multi<16, 2>)
+                       : (ndig <= 4 ? k_ks_inner_multi<4, 1> : ndig <= 8 ? k_ks_inner_multi<8, 1> : k_ks_inner_multi<16, 1>);
     fn<<<blocks, kBlock, 0, s>>>(d, kr, (int)nout, cx, c.mods(), Ql, c.nQ, c.nP, (int)ndig, c.N, local ? 1 : 0, perm,
                                  accumulate ? 1 : 0, lgog);
     SHM_HIP(hipGetLastError());
@@ -420,14 +458,19 @@ void ks_inner_batch(const Context &c, const uint64_t *d, size_t m, size_t Ql, co
     if (beta > c.dnum) throw LogicError("key switching key has too few digits for this level");
     if (m == 0 || m > 64) throw InvalidArgument("ks_inner_batch: 1..64 members");
     bump(g_cnt_inner, Ql, 2 * m);
+    // one member slot by default: each thread loads its key words once and loops over
+    // the members (measured fastest on the shared-a bootstrap chunk)
     int lgmg = 0;
-    while (lgmg < 4 && (size_t(1) << lgmg) < m) lgmg++;
+    if (ks_tuning().batch_slots_log2 > 0)
+        while (lgmg < ks_tuning().batch_slots_log2 && (size_t(1) << lgmg) < m) lgmg++;
+    const int unr = ks_tuning().unroll;
     const size_t total = QlP * c.N;
     const unsigned blocks = (unsigned)((total + (256u >> lgmg) - 1) >> (8 - lgmg));
     // digits in chunks of kMaxDigits; a later chunk adds its partial sums mod q
     for (size_t g0 = 0; g0 < beta; g0 += kMaxDigits) {
         const size_t nb = std::min<size_t>(kMaxDigits, beta - g0);
-        auto fn = nb <= 4 ? k_ks_inner_batch<4> : nb <= 8 ? k_ks_inner_batch<8> : k_ks_inner_batch<16>;
+        auto fn = unr == 2 ? (nb <= 4 ? k_ks_inner_batch<4, 2> : nb <= 8 ? k_ks_inner_batch<8, 2> : k_ks_inner_batch<16, 2>)
+                           : (nb <= 4 ? k_ks_inner_batch<4, 1> : nb <= 8 ? k_ks_inner_batch<8, 1> : k_ks_inner_batch<16, 1>);
         fn<<<blocks, kBlock, 0, s>>>(d, key, cx, c.mods(), Ql, c.nQ, c.nQP(), c.nP, (int)beta, c.N, (int)m, lgmg,
                                      perm, own ? *own : PolyPack{}, own ? 1 : 0, (int)g0, (int)nb, g0 > 0 ? 1 : 0);
         SHM_HIP(hipGetLastError());
