@@ -432,6 +432,42 @@ std::unique_ptr<Context> make_context(int logN, const std::vector<int> &bits, si
                 }
             }
         }
+        if (Ql >= 2) {
+            // basis B = {q_last, p_0 .. p_(nP-1)} of the merged ModDown + rescale
+            const size_t nb = nP + 1, L1 = Ql - 1;
+            std::vector<uint64_t> bprime(nb);
+            bprime[0] = c->q[Ql - 1];
+            for (size_t j = 0; j < nP; j++) bprime[j + 1] = c->q[nQ + j];
+            std::vector<uint64_t> bhi(nb), bhis(nb), bhm(std::max<size_t>(L1, 1) * nb), bm(std::max<size_t>(L1, 1)),
+                bi(std::max<size_t>(L1, 1)), bis(std::max<size_t>(L1, 1));
+            for (size_t j = 0; j < nb; j++) {
+                uint64_t h = 1;
+                for (size_t k = 0; k < nb; k++)
+                    if (k != j) h = h_mulmod(h, bprime[k] % bprime[j], bprime[j]);
+                bhi[j] = h_inv(h, bprime[j]);
+                bhis[j] = h_shoup(bhi[j], bprime[j]);
+            }
+            for (size_t i = 0; i < L1; i++) {
+                const uint64_t qv = c->q[i];
+                uint64_t b = 1;
+                for (size_t k = 0; k < nb; k++) b = h_mulmod(b, bprime[k] % qv, qv);
+                bm[i] = b;
+                bi[i] = h_inv(b, qv);
+                bis[i] = h_shoup(bi[i], qv);
+                for (size_t j = 0; j < nb; j++) {
+                    uint64_t h = 1;
+                    for (size_t k = 0; k < nb; k++)
+                        if (k != j) h = h_mulmod(h, bprime[k] % qv, qv);
+                    bhm[i * nb + j] = h;
+                }
+            }
+            upload(L->bhat_inv, bhi);
+            upload(L->bhat_inv_s, bhis);
+            upload(L->bhat_mod, bhm);
+            upload(L->bmod, bm);
+            upload(L->binv, bi);
+            upload(L->binv_s, bis);
+        }
         upload(L->qlast_inv, qi);
         upload(L->qlast_inv_s, qis);
         upload(L->hat_inv, hi);

@@ -124,6 +124,11 @@ struct LevelConsts {
     DevBuf hat_mod;
     // modup: [D_g]_{p_k}, D_g = product of digit g's primes: [beta][QlP]
     DevBuf dig_mod;
+    // ModDown merged with the rescale: basis B = {q_last, p_0 .. p_(nP-1)} (the
+    // limbs Ql-1 .. Ql+nP-1 of a QlP polynomial). bhat_inv[j] = (B/b_j)^-1 mod b_j
+    // (value, shoup); bhat_mod[i][j] = (B/b_j) mod q_i, i < Ql-1; bmod[i] = B mod
+    // q_i; binv[i] = B^-1 mod q_i (value, shoup)
+    DevBuf bhat_inv, bhat_inv_s, bhat_mod, bmod, binv, binv_s;
 };
 
 struct Context {
@@ -372,9 +377,17 @@ void ks_modup_batch(const Context &c, const PolyPack &c2, size_t m, size_t Ql, u
                     bool rounded = false);
 // cx [m][2][QlP][N] = inner products of the m digit sets of d with one key
 // ([dnum][2][QP][N]); each key word is read once for the group; with perm the
-// digits are read through that Galois permutation
+// digits are read through that Galois permutation; with `add` (2m polynomials of
+// Ql limbs, NTT form), P * add.p[2i + o] is added onto the data limbs of cx[i][o]
 void ks_inner_batch(const Context &c, const uint64_t *d, size_t m, size_t Ql, const uint64_t *key, uint64_t *cx,
-                    hipStream_t s, const uint32_t *perm = nullptr, const PolyPack *own = nullptr);
+                    hipStream_t s, const uint32_t *perm = nullptr, const PolyPack *own = nullptr,
+                    const PolyPack *add = nullptr);
+// Relinearisation merged with the following rescale (inside bootstrapping): for m
+// products with tensor parts d01.p[2i], d01.p[2i + 1] and c2.p[i] (Ql limbs, NTT
+// form), out.p[2i + o] (Ql - 1 limbs) = round((P d_o + ks_o(c2)) / (P q_last)), one
+// exact centred conversion over {q_last, P} in place of ModDown then rescale.
+void keyswitch_rescale_batch(const Context &c, const PolyPack &c2, const PolyPack &d01, size_t m, const uint64_t *key,
+                             size_t Ql, const PolyPack &out, hipStream_t s);
 // Key switch of m polynomials c2.p[i] with one key; the switched pair of member
 // i goes to (ct01.p[2i], ct01.p[2i + 1]), accumulated where acc_mask has the
 // polynomial's bit. One launch per stage for the whole group.

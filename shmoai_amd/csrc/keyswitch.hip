@@ -216,6 +216,46 @@ __global__ void k_moddown_bconv(const uint64_t *__restrict__ cx, uint64_t *__res
     }
 }
 
+// delta[p][i][j], i < Ql-1: the exact centred conversion of polynomial p's
+// coefficient-form limbs over B = {q_last, p_0 .. p_(nP-1)} (positions Ql-1 ..
+// Ql+nP-1 of its QlP limbs) to q_i: sum_b y_b (B/b) - round(sum_b y_b / b) B with
+// y_b = [x_b (B/b)^-1]_b, the representative of the value mod B in (-B/2, B/2],
+// so that (x - delta) / B rounds x / (P q_last) to nearest. One thread per (p, j).
+__global__ void k_moddown_rescale_bconv(const uint64_t *__restrict__ cx, uint64_t *__restrict__ delta,
+                                        const DevMod *__restrict__ mods, const uint64_t *__restrict__ bhat_inv,
+                                        const uint64_t *__restrict__ bhat_inv_s,
+                                        const uint64_t *__restrict__ bhat_mod, const uint64_t *__restrict__ bmod,
+                                        size_t Ql, size_t nQ, size_t nP, size_t N, size_t npolys) {
+    const size_t QlP = Ql + nP, L1 = Ql - 1, nb = nP + 1, total = npolys * N;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t p = t >> __builtin_ctzll(N), j = t & (N - 1);
+        const uint64_t *xb = cx + (p * QlP + L1) * N + j;
+        uint64_t *d = delta + p * L1 * N + j;
+        uint64_t y[kMaxAlpha + 1];
+        double v = 0.0;
+#pragma unroll
+        for (int b = 0; b < kMaxAlpha + 1; b++) {
+            y[b] = 0;
+            if ((size_t)b < nb) {
+                const uint64_t qb = mods[b == 0 ? L1 : nQ + (b - 1)].q;
+                y[b] = mul_shoup(xb[b * N], bhat_inv[b], bhat_inv_s[b], qb);
+                v = __dadd_rn(v, __ddiv_rn((double)y[b], (double)qb));
+            }
+        }
+        const uint64_t r = (uint64_t)__dadd_rn(v, 0.5);
+        for (size_t i = 0; i < L1; i++) {
+            const DevMod mq = mods[i];
+            Acc128 acc{0, 0};
+#pragma unroll
+            for (int b = 0; b < kMaxAlpha + 1; b++)
+                if ((size_t)b < nb) mac128(acc, y[b], bhat_mod[i * nb + b]);
+            // - r B == (q - r) [B]_q (mod q), folded in before the one reduction
+            mac128(acc, mq.q - r, bmod[i]);
+            d[i * N] = reduce128(acc, mq);
+        }
+    }
+}
+
 // cx[m][0/1][k][j] = sum_g d[m][g][k][j] * key_g.(b/a)[limb(k)][j] for `members`
 // digit sets (eval_key_switch.cu:12-60 per member).
 // With perm, the digits are read through that NTT-domain Galois permutation
@@ -233,7 +273,9 @@ __global__ __launch_bounds__(256) void k_ks_inner_batch(const uint64_t *__restri
                                                         size_t Ql, size_t nQ, size_t QP, size_t alpha, int beta,
                                                         size_t N, int members, int lgmg,
                                                         const uint32_t *__restrict__ perm, PolyPack c2, int own_c2,
-                                                        int g0, int nb, int accumulate) {
+                                                        int g0, int nb, int accumulate, PolyPack addp, int add_on,
+                                                        const uint64_t *__restrict__ pmod,
+                                                        const uint64_t *__restrict__ pmod_s) {
     const size_t QlP = Ql + alpha, total = QlP * N;
     const int lgj = 8 - lgmg;
     const size_t t = ((size_t)blockIdx.x << lgj) + (threadIdx.x & ((1u << lgj) - 1));
@@ -286,6 +328,12 @@ __global__ __launch_bounds__(256) void k_ks_inner_batch(const uint64_t *__restri
             if (accumulate) {
                 v0 = add_mod(v0, *o0, m.q);
                 v1 = add_mod(v1, *o1, m.q);
+            } else if (add_on && k < Ql) {
+                // + P (d0, d1) on the data limbs: the merged ModDown + rescale
+                // divides P d + cx by P q_last in one step
+                const uint64_t pm = pmod[pi], pms = pmod_s[pi];
+                v0 = add_mod(v0, mul_shoup(addp.p[2 * mem][k * N + j], pm, pms, m.q), m.q);
+                v1 = add_mod(v1, mul_shoup(addp.p[2 * mem + 1][k * N + j], pm, pms, m.q), m.q);
             }
             *o0 = v0;
             *o1 = v1;
@@ -453,7 +501,7 @@ void ks_modup_batch(const Context &c, const PolyPack &c2, size_t m, size_t Ql, u
 }
 
 void ks_inner_batch(const Context &c, const uint64_t *d, size_t m, size_t Ql, const uint64_t *key, uint64_t *cx,
-                    hipStream_t s, const uint32_t *perm, const PolyPack *own) {
+                    hipStream_t s, const uint32_t *perm, const PolyPack *own, const PolyPack *add) {
     const size_t QlP = Ql + c.nP, beta = c.level(Ql).beta;
     if (beta > c.dnum) throw LogicError("key switching key has too few digits for this level");
     if (m == 0 || m > 64) throw InvalidArgument("ks_inner_batch: 1..64 members");
@@ -472,7 +520,9 @@ void ks_inner_batch(const Context &c, const uint64_t *d, size_t m, size_t Ql, co
         auto fn = unr == 2 ? (nb <= 4 ? k_ks_inner_batch<4, 2> : nb <= 8 ? k_ks_inner_batch<8, 2> : k_ks_inner_batch<16, 2>)
                            : (nb <= 4 ? k_ks_inner_batch<4, 1> : nb <= 8 ? k_ks_inner_batch<8, 1> : k_ks_inner_batch<16, 1>);
         fn<<<blocks, kBlock, 0, s>>>(d, key, cx, c.mods(), Ql, c.nQ, c.nQP(), c.nP, (int)beta, c.N, (int)m, lgmg,
-                                     perm, own ? *own : PolyPack{}, own ? 1 : 0, (int)g0, (int)nb, g0 > 0 ? 1 : 0);
+                                     perm, own ? *own : PolyPack{}, own ? 1 : 0, (int)g0, (int)nb, g0 > 0 ? 1 : 0,
+                                     add ? *add : PolyPack{}, add ? 1 : 0, c.pmod.as<uint64_t>(),
+                                     c.pmod_s.as<uint64_t>());
         SHM_HIP(hipGetLastError());
     }
 }
@@ -488,6 +538,32 @@ void keyswitch_batch(const Context &c, const PolyPack &c2, size_t m, const uint6
     ks_inner_batch(c, d.as<uint64_t>(), m, Ql, key, cx.as<uint64_t>(), s, nullptr, &c2);
     // ModDown of the 2m polynomials into the members' (c0, c1)
     ks_moddown_pack(c, cx.as<uint64_t>(), 2 * m, Ql, ct01, acc_mask, s, rounded);
+}
+
+void keyswitch_rescale_batch(const Context &c, const PolyPack &c2, const PolyPack &d01, size_t m, const uint64_t *key,
+                             size_t Ql, const PolyPack &out, hipStream_t s) {
+    const size_t N = c.N, nP = c.nP, QlP = Ql + nP, beta = c.level(Ql).beta;
+    if (m == 0 || 2 * m > (size_t)kPolyPack) throw InvalidArgument("keyswitch_rescale_batch: 1..32 members");
+    if (Ql < 2) throw LogicError("end of modulus switching chain reached");
+    if (nP > (size_t)kMaxAlpha) throw InvalidArgument("at most 16 special primes");
+    const LevelConsts &L = c.level(Ql);
+    Scratch d(m * beta * QlP * N * sizeof(uint64_t), s);
+    ks_modup_batch(c, c2, m, Ql, d.as<uint64_t>(), s, true);
+    // cx = P (d0, d1) + the inner products: the value the merged step divides
+    Scratch cx(2 * m * QlP * N * sizeof(uint64_t), s);
+    ks_inner_batch(c, d.as<uint64_t>(), m, Ql, key, cx.as<uint64_t>(), s, nullptr, &c2, &d01);
+    // B = {q_last, P}: inverse NTT of those limbs, exact centred conversion to the
+    // other Ql - 1 primes, and the NTT of the correction whose last pass writes
+    // (cx - delta) B^-1 into the outputs
+    bump(g_cnt_moddown, Ql, 2 * m);
+    ntt_inverse_qlp(c, cx.as<uint64_t>(), Ql, Ql - 1, QlP, 2 * m, QlP * N, s);
+    Scratch delta(2 * m * (Ql - 1) * N * sizeof(uint64_t), s);
+    k_moddown_rescale_bconv<<<grid_for(2 * m * N), kBlock, 0, s>>>(
+        cx.as<uint64_t>(), delta.as<uint64_t>(), c.mods(), L.bhat_inv.as<uint64_t>(), L.bhat_inv_s.as<uint64_t>(),
+        L.bhat_mod.as<uint64_t>(), L.bmod.as<uint64_t>(), Ql, c.nQ, nP, N, 2 * m);
+    SHM_HIP(hipGetLastError());
+    moddown_ntt_fused(c, delta.as<uint64_t>(), cx.as<uint64_t>(), Ql - 1, QlP, out, 2 * m, 0,
+                      L.binv.as<uint64_t>(), L.binv_s.as<uint64_t>(), s);
 }
 
 }  // namespace shm

@@ -411,6 +411,12 @@ void ct_align(Context &c, Ciphertext &a, size_t chain, double scale, hipStream_t
 void ct_mul_relin_rescale(Context &c, Ciphertext &out, const Ciphertext &a, const Ciphertext &b,
                           const KSwitchKey &rk, hipStream_t s, bool rounded) {
     if (a.size != 2 || b.size != 2) throw InvalidArgument("multiply supports size-2 ciphertexts");
+    // the rounded form merges the ModDown and the rescale; a group of one takes
+    // exactly the operations every member of mul_relin_rescale_batch takes
+    if (rounded) {
+        mul_relin_rescale_batch(c, {&out}, {&a}, {&b}, rk, s, true);
+        return;
+    }
     // the operand at the higher level enters through its first Ql limbs (the
     // limbs a mod switch to the lower level keeps), with no copy
     const size_t chain = std::max(a.chain, b.chain);
@@ -430,7 +436,10 @@ void ct_mul_relin_rescale(Context &c, Ciphertext &out, const Ciphertext &a, cons
 }
 
 // ct_mul_relin_rescale over m independent products, with one key switch pass
-// (keyswitch_batch) and one rescale pass (rescale_batch) for the group.
+// (keyswitch_batch) and one rescale pass (rescale_batch) for the group. The
+// rounded form (inside bootstrapping) divides P d + ks(d2) by P q_last in one
+// exact centred step (keyswitch_rescale_batch) instead of ModDown then rescale:
+// one conversion and one forward NTT of Ql - 1 limbs per polynomial fewer.
 void mul_relin_rescale_batch(Context &c, const std::vector<Ciphertext *> &out,
                              const std::vector<const Ciphertext *> &a, const std::vector<const Ciphertext *> &b,
                              const KSwitchKey &rk, hipStream_t s, bool rounded) {
@@ -460,6 +469,25 @@ void mul_relin_rescale_batch(Context &c, const std::vector<Ciphertext *> &out,
         c2.p[i] = o.poly(2);
         ct01.p[2 * i] = o.poly(0);
         ct01.p[2 * i + 1] = o.poly(1);
+    }
+    if (rounded) {
+        if (Ql < 2) throw LogicError("end of modulus switching chain reached");
+        std::vector<Ciphertext> fresh(m);
+        PolyPack outp{};
+        for (size_t i = 0; i < m; i++) {
+            fresh[i].ctx = &c;
+            fresh[i].resize(2, out[i]->chain + 1, s, false);  // the fused NTT writes every word
+            outp.p[2 * i] = fresh[i].poly(0);
+            outp.p[2 * i + 1] = fresh[i].poly(1);
+        }
+        keyswitch_rescale_batch(c, c2, ct01, m, rk.k.as<uint64_t>(), Ql, outp, s);
+        const double qlast = (double)c.q[Ql - 1];
+        for (size_t i = 0; i < m; i++) {
+            fresh[i].scale = out[i]->scale / qlast;
+            fresh[i].ntt = true;
+            *out[i] = std::move(fresh[i]);
+        }
+        return;
     }
     const uint64_t all = 2 * m == 64 ? ~uint64_t(0) : ((uint64_t(1) << (2 * m)) - 1);
     keyswitch_batch(c, c2, m, rk.k.as<uint64_t>(), Ql, ct01, all, s, rounded);  // relinearise: add to both
