@@ -221,6 +221,8 @@ class Ref:
         return out
 
     def ks_inner(self, modup, key, Ql):
+        if self.beta(Ql) > self.dnum():
+            raise ValueError("key switching key has too few digits for this level")
         out = np.zeros(2 * (Ql + self.nP) * self.N, np.uint64)
         self.L.ref_ks_inner(self.h, modup, key, Ql, out)
         return out

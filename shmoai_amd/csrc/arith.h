@@ -91,6 +91,40 @@ __device__ __forceinline__ void mac128(Acc128 &acc, uint64_t a, uint64_t b) {
 
 __device__ __forceinline__ uint64_t reduce128(const Acc128 &acc, const DevMod &m) { return barrett128(acc.hi, acc.lo, m); }
 
+// Sums of products of residues below 2^52 (every prime in (2^45, 2^52)): with
+// a = a1 2^26 + a0 and b = b1 2^26 + b0, a b = a1 b1 2^52 + (a1 b0 + a0 b1) 2^26 + a0 b0,
+// each partial product a 32 x 32 -> 64-bit multiply-add into one of three 64-bit
+// sums (no carries: up to 64 terms stay below 2^64).
+struct Acc52 {
+    uint64_t s0, s1, s2;
+};
+
+struct Split26 {
+    uint32_t hi, lo;
+};
+
+__device__ __forceinline__ Split26 split26(uint64_t x) { return {(uint32_t)(x >> 26), (uint32_t)x & 0x3ffffffu}; }
+
+__device__ __forceinline__ void mac52(Acc52 &acc, Split26 a, uint64_t b) {
+    const Split26 bs = split26(b);
+    acc.s0 += (uint64_t)a.lo * bs.lo;
+    acc.s1 += (uint64_t)a.hi * bs.lo;
+    acc.s1 += (uint64_t)a.lo * bs.hi;
+    acc.s2 += (uint64_t)a.hi * bs.hi;
+}
+
+// [S]_q of S = s2 2^52 + s1 2^26 + s0 < 2^109 for 2^45 < q < 2^52: T = floor(S / 2^45)
+// exactly, the quotient estimate floor(T mu / 2^64) with mu = floor(2^109 / q) (from
+// the Barrett constant floor(2^128 / q)) is at most 3 short, so S - qe q in [0, 4q)
+// is taken in 64 bits.
+__device__ __forceinline__ uint64_t reduce52(const Acc52 &a, const DevMod &m) {
+    const uint64_t T = (a.s2 << 7) + (a.s1 >> 19) + ((((a.s1 & 0x7ffffull) << 26) + a.s0) >> 45);
+    const uint64_t lo = a.s0 + (a.s1 << 26) + (a.s2 << 52);
+    const uint64_t mu = (m.mu_hi << 45) | (m.mu_lo >> 19);
+    const uint64_t r = lo - __umul64hi(T, mu) * m.q;
+    return csub(csub(r, m.q << 1), m.q);
+}
+
 // [v]_q of an integral double v with |v| < 2^128; negative v maps to q - (|v| mod q),
 // which is q itself when |v| is a multiple of q (callers reduce when they need [0, q)).
 __device__ __forceinline__ uint64_t residue_of_rounded(double v, const DevMod &m) {

@@ -317,6 +317,8 @@ std::unique_ptr<Context> make_context(int logN, const std::vector<int> &bits, si
     c->nQ = bits.size() - nP;
     c->dnum = c->nQ / c->nP;
     c->q = make_primes(c->N, bits);
+    c->primes52 = true;
+    for (uint64_t qv : c->q) c->primes52 = c->primes52 && qv > (uint64_t(1) << 45) && qv < (uint64_t(1) << 52);
     const size_t N = c->N, QP = c->q.size();
 
     std::vector<DevMod> mods(QP);

@@ -161,6 +161,8 @@ struct Context {
     std::map<uint32_t, std::unique_ptr<DevBuf>> galois_tables;
 
     size_t nQP() const { return nQ + nP; }
+    // every prime in (2^45, 2^52): base conversions take the 26-bit split products
+    bool primes52 = false;
     size_t ql_of_chain(size_t chain) const { return nQ - chain + 1; }
     size_t chain_of_ql(size_t Ql) const { return nQ - Ql + 1; }
     const DevMod *mods() const { return d_mod.as<DevMod>(); }

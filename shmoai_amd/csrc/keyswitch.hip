@@ -44,6 +44,7 @@ constexpr int kMaxAlpha = 16;
 // digit index and every table address derived from it are wave-uniform.
 // `members` polynomials are converted at once: member m reads tcoef + m Ql N
 // and writes out + m beta QlP N.
+template <bool P52>
 __global__ void k_modup_bconv(const uint64_t *__restrict__ tcoef_all, uint64_t *__restrict__ out_all,
                               const DevMod *__restrict__ mods, const uint64_t *__restrict__ hat_inv,
                               const uint64_t *__restrict__ hat_inv_s, const uint64_t *__restrict__ hat_mod,
@@ -74,6 +75,9 @@ __global__ void k_modup_bconv(const uint64_t *__restrict__ tcoef_all, uint64_t *
         }
         const uint64_t r = rounded ? (uint64_t)__dadd_rn(v, 0.5) : 0;
         const uint64_t qs = mods[start].q;
+        Split26 ys[kMaxAlpha];
+#pragma unroll
+        for (int i = 0; i < kMaxAlpha; i++) ys[i] = split26(P52 ? y[i] : 0);
         for (size_t k = 0; k < QlP; k++) {
             if (k >= start && k < start + size) continue;
             const DevMod mp = mods[qlp_prime(k, Ql, nQ)];
@@ -81,6 +85,14 @@ __global__ void k_modup_bconv(const uint64_t *__restrict__ tcoef_all, uint64_t *
             if (size == 1) {
                 val = reduce64(y[0], mp);
                 if (r) val = sub_mod(val, reduce64(qs, mp), mp.q);
+            } else if (P52) {
+                Acc52 acc{0, 0, 0};
+                const uint64_t *hm = hat_mod + (g * alpha) * QlP + k;
+#pragma unroll
+                for (int i = 0; i < kMaxAlpha; i++)
+                    if ((size_t)i < size) mac52(acc, ys[i], hm[i * QlP]);
+                if (rounded) mac52(acc, split26(mp.q - r), dig_mod[g * QlP + k]);
+                val = reduce52(acc, mp);
             } else {
                 Acc128 acc{0, 0};
                 const uint64_t *hm = hat_mod + (g * alpha) * QlP + k;
@@ -174,6 +186,7 @@ __global__ __launch_bounds__(256) void k_ks_inner_multi(const uint64_t *__restri
 // q_i, one thread per (p, j). With `rounded`, the conversion is exact and
 // centred: it subtracts round(sum_j y_j / p_j) P, so delta is the representative
 // of cx mod P in (-P/2, P/2] and (cx - delta) / P rounds to nearest.
+template <bool P52>
 __global__ void k_moddown_bconv(const uint64_t *__restrict__ cx, uint64_t *__restrict__ delta,
                                 const DevMod *__restrict__ mods, const uint64_t *__restrict__ phat_inv,
                                 const uint64_t *__restrict__ phat_inv_s, const uint64_t *__restrict__ phat_mod,
@@ -196,10 +209,20 @@ __global__ void k_moddown_bconv(const uint64_t *__restrict__ cx, uint64_t *__res
             }
         }
         const uint64_t r = rounded ? (uint64_t)__dadd_rn(v, 0.5) : 0;
+        Split26 ys[kMaxAlpha];
+#pragma unroll
+        for (int jj = 0; jj < kMaxAlpha; jj++) ys[jj] = split26(P52 ? y[jj] : 0);
         for (size_t i = 0; i < Ql; i++) {
             const DevMod mq = mods[i];
             uint64_t val;
-            if (nP == 1) {
+            if (P52 && nP > 1) {
+                Acc52 acc{0, 0, 0};
+#pragma unroll
+                for (int jj = 0; jj < kMaxAlpha; jj++)
+                    if ((size_t)jj < nP) mac52(acc, ys[jj], phat_mod[i * nP + jj]);
+                if (rounded) mac52(acc, split26(mq.q - r), pmod[i]);
+                val = reduce52(acc, mq);
+            } else if (nP == 1) {
                 val = reduce64(y[0], mq);
                 if (r) val = sub_mod(val, reduce64(mods[nQ].q, mq), mq.q);
             } else {
@@ -221,6 +244,7 @@ __global__ void k_moddown_bconv(const uint64_t *__restrict__ cx, uint64_t *__res
 // Ql+nP-1 of its QlP limbs) to q_i: sum_b y_b (B/b) - round(sum_b y_b / b) B with
 // y_b = [x_b (B/b)^-1]_b, the representative of the value mod B in (-B/2, B/2],
 // so that (x - delta) / B rounds x / (P q_last) to nearest. One thread per (p, j).
+template <bool P52>
 __global__ void k_moddown_rescale_bconv(const uint64_t *__restrict__ cx, uint64_t *__restrict__ delta,
                                         const DevMod *__restrict__ mods, const uint64_t *__restrict__ bhat_inv,
                                         const uint64_t *__restrict__ bhat_inv_s,
@@ -243,8 +267,20 @@ __global__ void k_moddown_rescale_bconv(const uint64_t *__restrict__ cx, uint64_
             }
         }
         const uint64_t r = (uint64_t)__dadd_rn(v, 0.5);
+        Split26 ys[kMaxAlpha + 1];
+#pragma unroll
+        for (int b = 0; b < kMaxAlpha + 1; b++) ys[b] = split26(P52 ? y[b] : 0);
         for (size_t i = 0; i < L1; i++) {
             const DevMod mq = mods[i];
+            if (P52) {
+                Acc52 acc{0, 0, 0};
+#pragma unroll
+                for (int b = 0; b < kMaxAlpha + 1; b++)
+                    if ((size_t)b < nb) mac52(acc, ys[b], bhat_mod[i * nb + b]);
+                mac52(acc, split26(mq.q - r), bmod[i]);
+                d[i * N] = reduce52(acc, mq);
+                continue;
+            }
             Acc128 acc{0, 0};
 #pragma unroll
             for (int b = 0; b < kMaxAlpha + 1; b++)
@@ -375,15 +411,19 @@ void ks_counters_reset() {
 }
 
 void ks_modup(const Context &c, const uint64_t *c2, size_t Ql, uint64_t *d, hipStream_t s, bool rounded) {
-    bump(g_cnt_modup, Ql, 1);
     const size_t N = c.N, alpha = c.nP, QlP = Ql + alpha;
     const LevelConsts &L = c.level(Ql);
     const size_t beta = L.beta;
+    // the digits are multiplied by key digits 0 .. beta-1, and keys hold nQ / nP of
+    // them: refuse before anything is launched (an inner product would read past
+    // the key)
+    if (beta > c.dnum) throw LogicError("key switching key has too few digits for this level");
+    bump(g_cnt_modup, Ql, 1);
     Scratch tcoef(Ql * N * sizeof(uint64_t), s);
     launch_copy_words(tcoef.as<uint64_t>(), c2, Ql * N, s);
     ntt_inverse(c, tcoef.as<uint64_t>(), 1, Ql, 0, s);
     if (alpha > (size_t)kMaxAlpha) throw InvalidArgument("at most 16 special primes");
-    k_modup_bconv<<<grid_for(beta * N), kBlock, 0, s>>>(
+    (c.primes52 ? k_modup_bconv<true> : k_modup_bconv<false>)<<<grid_for(beta * N), kBlock, 0, s>>>(
         tcoef.as<uint64_t>(), d, c.mods(), L.hat_inv.as<uint64_t>(), L.hat_inv_s.as<uint64_t>(),
         L.hat_mod.as<uint64_t>(), L.dig_mod.as<uint64_t>(), Ql, c.nQ, alpha, beta, N, 1, rounded ? 1 : 0, 1);
     SHM_HIP(hipGetLastError());
@@ -438,7 +478,7 @@ void ks_moddown_pack(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, c
     Scratch delta(npolys * Ql * N * sizeof(uint64_t), s);
     ntt_inverse_qlp(c, cx, Ql, Ql, QlP, npolys, QlP * N, s);
     if (nP > (size_t)kMaxAlpha) throw InvalidArgument("at most 16 special primes");
-    k_moddown_bconv<<<grid_for(npolys * N), kBlock, 0, s>>>(
+    (c.primes52 ? k_moddown_bconv<true> : k_moddown_bconv<false>)<<<grid_for(npolys * N), kBlock, 0, s>>>(
         cx, delta.as<uint64_t>(), c.mods(), c.phat_inv.as<uint64_t>(), c.phat_inv_s.as<uint64_t>(),
         c.phat_mod.as<uint64_t>(), c.pmod.as<uint64_t>(), Ql, c.nQ, nP, N, npolys, rounded ? 1 : 0);
     SHM_HIP(hipGetLastError());
@@ -473,6 +513,7 @@ void ks_modup_batch(const Context &c, const PolyPack &c2, size_t m, size_t Ql, u
     const LevelConsts &L = c.level(Ql);
     const size_t beta = L.beta;
     if (m == 0 || m > (size_t)kPolyPack) throw InvalidArgument("ks_modup_batch: 1..64 members");
+    if (beta > c.dnum) throw LogicError("key switching key has too few digits for this level");
     if (alpha > (size_t)kMaxAlpha) throw InvalidArgument("at most 16 special primes");
     bump(g_cnt_modup, Ql, m);
     // gather + iNTT, conversion, one NTT over m beta QlP limbs
@@ -485,7 +526,7 @@ void ks_modup_batch(const Context &c, const PolyPack &c2, size_t m, size_t Ql, u
     // fill the GPU, so the own limbs are written in coefficient form and one launch
     // transforms every limb (back to c2's words for the own ones).
     const bool per_digit = m >= 4;
-    k_modup_bconv<<<grid_for(m * beta * N), kBlock, 0, s>>>(
+    (c.primes52 ? k_modup_bconv<true> : k_modup_bconv<false>)<<<grid_for(m * beta * N), kBlock, 0, s>>>(
         tcoef.as<uint64_t>(), d, c.mods(), L.hat_inv.as<uint64_t>(), L.hat_inv_s.as<uint64_t>(),
         L.hat_mod.as<uint64_t>(), L.dig_mod.as<uint64_t>(), Ql, c.nQ, alpha, beta, N, m, rounded ? 1 : 0,
         per_digit ? 0 : 1);
@@ -558,7 +599,7 @@ void keyswitch_rescale_batch(const Context &c, const PolyPack &c2, const PolyPac
     bump(g_cnt_moddown, Ql, 2 * m);
     ntt_inverse_qlp(c, cx.as<uint64_t>(), Ql, Ql - 1, QlP, 2 * m, QlP * N, s);
     Scratch delta(2 * m * (Ql - 1) * N * sizeof(uint64_t), s);
-    k_moddown_rescale_bconv<<<grid_for(2 * m * N), kBlock, 0, s>>>(
+    (c.primes52 ? k_moddown_rescale_bconv<true> : k_moddown_rescale_bconv<false>)<<<grid_for(2 * m * N), kBlock, 0, s>>>(
         cx.as<uint64_t>(), delta.as<uint64_t>(), c.mods(), L.bhat_inv.as<uint64_t>(), L.bhat_inv_s.as<uint64_t>(),
         L.bhat_mod.as<uint64_t>(), L.bmod.as<uint64_t>(), Ql, c.nQ, nP, N, 2 * m);
     SHM_HIP(hipGetLastError());

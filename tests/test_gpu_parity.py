@@ -153,10 +153,13 @@ def test_encrypt_decrypt_encode_bitexact(c1):
     assert np.max(np.abs(out - v)) < 1e-6
 
 
-@pytest.mark.parametrize("bits,n_special,Ql", [(C1_BITS, 1, 8), ([50] * 8 + [60] * 3, 3, 6)])
+@pytest.mark.parametrize("bits,n_special,Ql", [(C1_BITS, 1, 8), ([50] * 8 + [60] * 3, 3, 6),
+                                               ([51] + [46] * 8 + [51] * 3, 3, 7)])
 def test_hoisting_bitexact(oracle_lib, bits, n_special, Ql):
     """evaluate.cu:1727-1920 hoisting_inplace: sum of rotations, one ModUp of c1,
-    permuted digits per step, summed inner products, one ModDown."""
+    permuted digits per step, summed inner products, one ModDown. The third case has
+    every prime below 2^52 (the reference layer's 46/51-bit chain), where the base
+    conversions take 26-bit split products, and a last digit of one prime (7 = 3 + 3 + 1)."""
     import shmoai_amd as shm
     ctx = shm.Context(13, bits, n_special)
     ref = oracle_lib.Ref(13, bits, n_special)
@@ -187,3 +190,27 @@ def test_hoisting_bitexact(oracle_lib, bits, n_special, Ql):
     k1 = ref.moddown(acc[1].ravel(), Ql)
     exp = np.concatenate([((c0sum + k0) % qQ).ravel(), k1])
     assert np.array_equal(ct.numpy(), exp)
+
+
+def test_keyswitch_refuses_missing_key_digits():
+    """Keys hold nQ / nP digits (secretkey.cu:328-373); with nQ = 8, nP = 3 that is 2, and
+    a key switch at Ql = 7 needs 3: hoisting, rotation and relinearisation refuse it
+    before any kernel is launched."""
+    import shmoai_amd as shm
+    ctx = shm.Context(13, [51] + [46] * 7 + [51] * 3, 3)
+    sk = shm.SecretKey(ctx, seed=1, hamming_weight=64)
+    gk = sk.create_galois_keys_from_steps([1], seed=9)
+    rk = sk.gen_relinkey(seed=10)
+    ev = shm.Evaluator(ctx)
+    ct = shm.Ciphertext(ctx, 2, ctx.chain(7))
+    with pytest.raises(shm.capi.ShmLogicError):
+        ev.hoisting_inplace(ct, gk, [1])
+    with pytest.raises(shm.capi.ShmLogicError):
+        ev.rotate_vector_inplace(ct, 1, gk)
+    ct3 = shm.Ciphertext(ctx, 3, ctx.chain(7))
+    with pytest.raises(shm.capi.ShmLogicError):
+        ev.relinearize_inplace(ct3, rk)
+    # one level lower (Ql = 6, 2 digits) is fine
+    ok = shm.Ciphertext(ctx, 2, ctx.chain(6))
+    ev.rotate_vector_inplace(ok, 1, gk)
+    shm.synchronize()

@@ -447,3 +447,13 @@ def test_golden_fixtures(oracle_lib):
         ntt_in = r.ntt(np.array(case["coeff_in"], np.uint64), r.pidx(Ql))
         out = r.intt(r.rescale(ntt_in, Ql, npolys=1), r.pidx(Ql - 1))
         assert [int(v) for v in out] == case["coeff_out"]
+
+
+def test_oracle_refuses_missing_key_digits(oracle_lib):
+    """Keys hold nQ / nP digits; a key switch needing more is refused rather than read
+    past the key (nQ = 8, nP = 3: 2 digits, Ql = 7 needs 3)."""
+    ref = oracle_lib.Ref(6, [51] + [46] * 7 + [51] * 3, 3)
+    assert ref.dnum() == 2 and ref.beta(7) == 3 and ref.beta(6) == 2
+    d = np.zeros(ref.beta(7) * 10 * ref.N, np.uint64)
+    with pytest.raises(ValueError):
+        ref.ks_inner(d, np.zeros(ref.key_size(), np.uint64), 7)
