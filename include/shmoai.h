@@ -552,8 +552,7 @@ int shm_keyswitch_counters_reset(void);
 /* Launch-shape knobs (no counterpart in the reference; results never depend on them):
  * "ks_batch_slots_log2" and "ks_multi_slots_log2" set log2 of the member / output slots
  * per block of the batched and multi-output key inner products (0..4; 0 or -1 = one slot,
- * the default: each thread loops over all members / outputs); "ks_unroll" (1 or 2) is how
- * many members / outputs a thread loads for before it multiplies. Process-wide. */
+ * the default: each thread loops over all members / outputs). Process-wide. */
 int shm_set_tuning(const char *name, int32_t value);
 int shm_get_tuning(const char *name, int32_t *value);
 

@@ -1790,11 +1790,6 @@ int shm_set_tuning(const char *name, int32_t value) {
     return guard([&] {
         if (!name) throw InvalidArgument("tuning knob name is null");
         const std::string n(name);
-        if (n == "ks_unroll") {
-            if (value != 1 && value != 2) throw InvalidArgument("ks_unroll: 1 or 2");
-            ks_tuning().unroll = value;
-            return;
-        }
         if (value < -1 || value > 4) throw InvalidArgument("tuning knob value: -1 (default) or 0..4");
         if (n == "ks_batch_slots_log2")
             ks_tuning().batch_slots_log2 = value;
@@ -1813,8 +1808,6 @@ int shm_get_tuning(const char *name, int32_t *value) {
             *value = ks_tuning().batch_slots_log2;
         else if (n == "ks_multi_slots_log2")
             *value = ks_tuning().multi_slots_log2;
-        else if (n == "ks_unroll")
-            *value = ks_tuning().unroll;
         else
             throw InvalidArgument("unknown tuning knob: " + n);
     });

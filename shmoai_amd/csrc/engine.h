@@ -242,12 +242,10 @@ void ks_counters(uint64_t *modup, uint64_t *inner, uint64_t *moddown, size_t cap
 void ks_counters_reset();
 // Launch-shape knobs of the key-switch inner products (results do not depend on
 // them): log2 of the member slots per block of the batched inner product and of
-// the output slots of the multi-output one (0 or -1: one slot, the default), and
-// how many members / outputs a thread loads for before it multiplies (1 or 2).
+// the output slots of the multi-output one (0 or -1: one slot, the default).
 struct KsTuning {
     int batch_slots_log2 = -1;
     int multi_slots_log2 = -1;
-    int unroll = 1;
 };
 KsTuning &ks_tuning();
 // Encoder decompose + forward NTT in one: out[p][k] = NTT of the rounded FFT

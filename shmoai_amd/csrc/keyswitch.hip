@@ -136,7 +136,7 @@ __global__ void k_ks_inner(const uint64_t *__restrict__ d, const uint64_t *__res
 // The OG threads of one coefficient load the same digit words (L1 serves all
 // but the first), so the digits leave HBM once for all outputs, and every
 // thread waits only on its own 2 ndig loads.
-template <int MAXB, int UNR>
+template <int MAXB, bool P52>
 __global__ __launch_bounds__(256) void k_ks_inner_multi(const uint64_t *__restrict__ d, KeyRefs kr, int nout,
                                                         uint64_t *__restrict__ cx, const DevMod *__restrict__ mods,
                                                         size_t Ql, size_t nQ, size_t alpha, int ndig, size_t N,
@@ -155,30 +155,31 @@ __global__ __launch_bounds__(256) void k_ks_inner_multi(const uint64_t *__restri
     uint64_t dv[MAXB];
 #pragma unroll
     for (int g = 0; g < MAXB; g++) dv[g] = g < ndig ? d[(size_t)g * total + src] : 0;
+    Split26 ds[MAXB];
+#pragma unroll
+    for (int g = 0; g < MAXB; g++) ds[g] = split26(P52 ? dv[g] : 0);
     const DevMod m = mods[pi];
-    const int step = 1 << lgog;
-    // UNR outputs per iteration: all their key loads are issued before the MACs
-    for (int o0 = os; o0 < nout; o0 += UNR * step) {
-        uint64_t kv[UNR][MAXB];
+    for (int o = os; o < nout; o += 1 << lgog) {
+        const uint64_t *kb = kr.base[o];
+        const size_t st = kr.stride[o];
+        uint64_t kv[MAXB];
 #pragma unroll
-        for (int u = 0; u < UNR; u++) {
-            const int o = o0 + u * step < nout ? o0 + u * step : o0;
-            const uint64_t *kb = kr.base[o];
-            const size_t st = kr.stride[o];
+        for (int g = 0; g < MAXB; g++) kv[g] = g < ndig ? kb[(size_t)g * st + kl] : 0;
+        uint64_t v;
+        if (P52) {
+            Acc52 a{0, 0, 0};
 #pragma unroll
-            for (int g = 0; g < MAXB; g++) kv[u][g] = g < ndig ? kb[(size_t)g * st + kl] : 0;
-        }
-#pragma unroll
-        for (int u = 0; u < UNR; u++) {
-            const int o = o0 + u * step;
-            if (o >= nout) break;
+            for (int g = 0; g < MAXB; g++)
+                if (g < ndig) mac52(a, ds[g], kv[g]);
+            v = reduce52(a, m);
+        } else {
             Acc128 a{0, 0};
 #pragma unroll
             for (int g = 0; g < MAXB; g++)
-                if (g < ndig) mac128(a, dv[g], kv[u][g]);
-            const uint64_t v = reduce128(a, m);
-            cx[(size_t)o * total + t] = accumulate ? add_mod(cx[(size_t)o * total + t], v, m.q) : v;
+                if (g < ndig) mac128(a, dv[g], kv[g]);
+            v = reduce128(a, m);
         }
+        cx[(size_t)o * total + t] = accumulate ? add_mod(cx[(size_t)o * total + t], v, m.q) : v;
     }
 }
 
@@ -303,7 +304,7 @@ __global__ void k_moddown_rescale_bconv(const uint64_t *__restrict__ cx, uint64_
 // after the first, so each key word leaves HBM once for the group. Every
 // thread has only its own beta digit loads to wait for, and the grid holds
 // members x QlP x N threads to hide their latency.
-template <int MAXB, int UNR>
+template <int MAXB, bool P52>
 __global__ __launch_bounds__(256) void k_ks_inner_batch(const uint64_t *__restrict__ d, const uint64_t *__restrict__ key,
                                                         uint64_t *__restrict__ cx, const DevMod *__restrict__ mods,
                                                         size_t Ql, size_t nQ, size_t QP, size_t alpha, int beta,
@@ -331,49 +332,66 @@ __global__ __launch_bounds__(256) void k_ks_inner_batch(const uint64_t *__restri
             ka[g] = kg[(QP + pi) * N + j];
         }
     }
+    Split26 kbs[MAXB], kas[MAXB];
+#pragma unroll
+    for (int g = 0; g < MAXB; g++) {
+        kbs[g] = split26(P52 ? kb[g] : 0);
+        kas[g] = split26(P52 ? ka[g] : 0);
+    }
     const DevMod m = mods[pi];
     // the digit among g0 .. g0 + nb - 1 whose own limbs hold k (digit dg's own limbs
     // k in [dg alpha, (dg + 1) alpha) are c2's limbs), or -1
     const int own_g = own_c2 && k < Ql ? (int)(k / alpha) - g0 : -1;
-    const int step = 1 << lgmg;
-    // UNR members per iteration: all their digit loads are issued before the MACs
-    for (int mem0 = ms; mem0 < members; mem0 += UNR * step) {
-        uint64_t dv[UNR][MAXB];
+    for (int mem = ms; mem < members; mem += 1 << lgmg) {
+        const uint64_t *dm = d + ((size_t)mem * beta + g0) * total + src;
+        uint64_t dv[MAXB];
 #pragma unroll
-        for (int u = 0; u < UNR; u++) {
-            const int mem = mem0 + u * step;
-            const uint64_t *dm = d + ((size_t)(mem < members ? mem : mem0) * beta + g0) * total + src;
+        for (int g = 0; g < MAXB; g++) dv[g] = g < nb ? (g == own_g ? c2.p[mem][src] : dm[(size_t)g * total]) : 0;
+        uint64_t v0, v1;
+        if (P52) {
+            Acc52 a0{0, 0, 0}, a1{0, 0, 0};
 #pragma unroll
-            for (int g = 0; g < MAXB; g++)
-                dv[u][g] = g < nb && mem < members ? (g == own_g ? c2.p[mem][src] : dm[(size_t)g * total]) : 0;
-        }
-#pragma unroll
-        for (int u = 0; u < UNR; u++) {
-            const int mem = mem0 + u * step;
-            if (mem >= members) break;
+            for (int g = 0; g < MAXB; g++) {
+                if (g < nb) {
+                    // the key words are the pre-split operand, the digit is split once
+                    const Split26 ds = split26(dv[g]);
+                    a0.s0 += (uint64_t)ds.lo * kbs[g].lo;
+                    a0.s1 += (uint64_t)ds.hi * kbs[g].lo;
+                    a0.s1 += (uint64_t)ds.lo * kbs[g].hi;
+                    a0.s2 += (uint64_t)ds.hi * kbs[g].hi;
+                    a1.s0 += (uint64_t)ds.lo * kas[g].lo;
+                    a1.s1 += (uint64_t)ds.hi * kas[g].lo;
+                    a1.s1 += (uint64_t)ds.lo * kas[g].hi;
+                    a1.s2 += (uint64_t)ds.hi * kas[g].hi;
+                }
+            }
+            v0 = reduce52(a0, m);
+            v1 = reduce52(a1, m);
+        } else {
             Acc128 a0{0, 0}, a1{0, 0};
 #pragma unroll
             for (int g = 0; g < MAXB; g++) {
                 if (g < nb) {
-                    mac128(a0, dv[u][g], kb[g]);
-                    mac128(a1, dv[u][g], ka[g]);
+                    mac128(a0, dv[g], kb[g]);
+                    mac128(a1, dv[g], ka[g]);
                 }
             }
-            uint64_t v0 = reduce128(a0, m), v1 = reduce128(a1, m);
-            uint64_t *o0 = cx + (size_t)(2 * mem) * total + t, *o1 = cx + (size_t)(2 * mem + 1) * total + t;
-            if (accumulate) {
-                v0 = add_mod(v0, *o0, m.q);
-                v1 = add_mod(v1, *o1, m.q);
-            } else if (add_on && k < Ql) {
-                // + P (d0, d1) on the data limbs: the merged ModDown + rescale
-                // divides P d + cx by P q_last in one step
-                const uint64_t pm = pmod[pi], pms = pmod_s[pi];
-                v0 = add_mod(v0, mul_shoup(addp.p[2 * mem][k * N + j], pm, pms, m.q), m.q);
-                v1 = add_mod(v1, mul_shoup(addp.p[2 * mem + 1][k * N + j], pm, pms, m.q), m.q);
-            }
-            *o0 = v0;
-            *o1 = v1;
+            v0 = reduce128(a0, m);
+            v1 = reduce128(a1, m);
         }
+        uint64_t *o0 = cx + (size_t)(2 * mem) * total + t, *o1 = cx + (size_t)(2 * mem + 1) * total + t;
+        if (accumulate) {
+            v0 = add_mod(v0, *o0, m.q);
+            v1 = add_mod(v1, *o1, m.q);
+        } else if (add_on && k < Ql) {
+            // + P (d0, d1) on the data limbs: the merged ModDown + rescale
+            // divides P d + cx by P q_last in one step
+            const uint64_t pm = pmod[pi], pms = pmod_s[pi];
+            v0 = add_mod(v0, mul_shoup(addp.p[2 * mem][k * N + j], pm, pms, m.q), m.q);
+            v1 = add_mod(v1, mul_shoup(addp.p[2 * mem + 1][k * N + j], pm, pms, m.q), m.q);
+        }
+        *o0 = v0;
+        *o1 = v1;
     }
 }
 
@@ -453,10 +471,10 @@ void ks_inner_multi(const Context &c, const uint64_t *d, size_t Ql, size_t ndig,
     int lgog = 0;
     if (ks_tuning().multi_slots_log2 > 0)
         while (lgog < ks_tuning().multi_slots_log2 && (size_t(1) << lgog) < nout) lgog++;
-    const int unr = ks_tuning().unroll;
     const unsigned blocks = (unsigned)((total + (256u >> lgog) - 1) >> (8 - lgog));
-    auto fn = unr == 2 ? (ndig <= 4 ? k_ks_inner_multi<4, 2> : ndig <= 8 ? k_ks_inner_multi<8, 2> : k_ks_inner_multi<16, 2>)
-                       : (ndig <= 4 ? k_ks_inner_multi<4, 1> : ndig <= 8 ? k_ks_inner_multi<8, 1> : k_ks_inner_multi<16, 1>);
+    auto fn = c.primes52
+                  ? (ndig <= 4 ? k_ks_inner_multi<4, true> : ndig <= 8 ? k_ks_inner_multi<8, true> : k_ks_inner_multi<16, true>)
+                  : (ndig <= 4 ? k_ks_inner_multi<4, false> : ndig <= 8 ? k_ks_inner_multi<8, false> : k_ks_inner_multi<16, false>);
     fn<<<blocks, kBlock, 0, s>>>(d, kr, (int)nout, cx, c.mods(), Ql, c.nQ, c.nP, (int)ndig, c.N, local ? 1 : 0, perm,
                                  accumulate ? 1 : 0, lgog);
     SHM_HIP(hipGetLastError());
@@ -552,14 +570,14 @@ void ks_inner_batch(const Context &c, const uint64_t *d, size_t m, size_t Ql, co
     int lgmg = 0;
     if (ks_tuning().batch_slots_log2 > 0)
         while (lgmg < ks_tuning().batch_slots_log2 && (size_t(1) << lgmg) < m) lgmg++;
-    const int unr = ks_tuning().unroll;
     const size_t total = QlP * c.N;
     const unsigned blocks = (unsigned)((total + (256u >> lgmg) - 1) >> (8 - lgmg));
     // digits in chunks of kMaxDigits; a later chunk adds its partial sums mod q
     for (size_t g0 = 0; g0 < beta; g0 += kMaxDigits) {
         const size_t nb = std::min<size_t>(kMaxDigits, beta - g0);
-        auto fn = unr == 2 ? (nb <= 4 ? k_ks_inner_batch<4, 2> : nb <= 8 ? k_ks_inner_batch<8, 2> : k_ks_inner_batch<16, 2>)
-                           : (nb <= 4 ? k_ks_inner_batch<4, 1> : nb <= 8 ? k_ks_inner_batch<8, 1> : k_ks_inner_batch<16, 1>);
+        auto fn = c.primes52
+                      ? (nb <= 4 ? k_ks_inner_batch<4, true> : nb <= 8 ? k_ks_inner_batch<8, true> : k_ks_inner_batch<16, true>)
+                      : (nb <= 4 ? k_ks_inner_batch<4, false> : nb <= 8 ? k_ks_inner_batch<8, false> : k_ks_inner_batch<16, false>);
         fn<<<blocks, kBlock, 0, s>>>(d, key, cx, c.mods(), Ql, c.nQ, c.nQP(), c.nP, (int)beta, c.N, (int)m, lgmg,
                                      perm, own ? *own : PolyPack{}, own ? 1 : 0, (int)g0, (int)nb, g0 > 0 ? 1 : 0,
                                      add ? *add : PolyPack{}, add ? 1 : 0, c.pmod.as<uint64_t>(),
