@@ -588,6 +588,21 @@ def main() -> int:
         log(f"bench: shared-a n=4, all {N_CTS}: {s4:.2f} s, max err {e4:.2e}")
         del keys4
         gc.collect()
+        # shared-a n = 8 with EvalMod as a degree-40 Chebyshev cosine + 3 double angles: the
+        # same depth (output level) with 13 + 3 instead of 16 + 2 relinearisations per part
+        boot40 = shm.Bootstrapper(ctx, deg=40, double_angles=3)
+        boot40.prepare()
+        shm.bootstrap_shared_a_s2c_first(boot40, keys8, cts[:8], rk, out=outs[:8])  # warm-up
+        s40, _ = timed(shm, lambda: shm.bootstrap_shared_a_s2c_first_chunks(boot40, keys8, cts, rk, out=outs))
+        e40 = max_err(shm, ctx, sk, outs, zs, sample)
+        boot768.update({"shared_a_n8_evalmod_deg40_r3_s": round(s40, 3),
+                        "max_abs_err_shared_a_n8_evalmod_deg40_r3": e40,
+                        "evalmod_deg40_r3_note": "EvalMod variant (not the headline configuration): "
+                                                 "degree-40 Chebyshev cosine + 3 double angles, same output "
+                                                 "level; the headline keeps degree 63 + 2"})
+        log(f"bench: shared-a n=8, EvalMod deg 40 + 3 double angles, all {N_CTS}: {s40:.2f} s, max err {e40:.2e}")
+        del boot40
+        gc.collect()
         # ordinary S2C-first, groups of 8 (one call over all 768)
         so, _ = timed(shm, lambda: boot.bootstrap_s2c_first(cts, gk, rk, out=outs))
         eo = max_err(shm, ctx, sk, outs, zs, sample)

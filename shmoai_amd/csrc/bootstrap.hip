@@ -273,22 +273,17 @@ struct ChebEval {
     size_t tchain(int k) const { return T[1]->chain + depth(k); }
 
     // T_k = 2 T_a T_b - T_{a-b}, a the largest power of two below k (T_{2a} = 2 T_a^2 - 1).
+    // The doubling and T_{a-b} (at the product's scale through an integer constant)
+    // enter the tensor before the relinearisation, so T_{a-b} needs no rescale of its own.
     Ciphertext &get(int k) {
         if (T[k]) return *T[k];
         const int a = split_a(k), b = k - a;
         Ciphertext &ta = get(a);
         Ciphertext &tb = get(b);
+        const std::vector<const Ciphertext *> term{a == b ? nullptr : &get(a - b)};
         auto out = std::make_unique<Ciphertext>();
-        ct_mul_relin_rescale(c, *out, ta, tb, rk, s, true);
-        ct_add(c, *out, *out, s);
-        if (a == b) {
-            ct_add_const(c, *out, -1.0, s);
-        } else {
-            Ciphertext tmp;
-            ct_clone(tmp, get(a - b), s);
-            ct_align(c, tmp, out->chain, out->scale, s, true);
-            ct_sub(c, *out, tmp, s);
-        }
+        mul_relin_rescale_batch(c, {out.get()}, {&ta}, {&tb}, rk, s, true, &term);
+        if (a == b) ct_add_const(c, *out, -1.0, s);
         if (out->chain != tchain(k)) throw LogicError("Chebyshev basis level bookkeeping");
         T[k] = std::move(out);
         return *T[k];
@@ -432,18 +427,12 @@ struct ChebEvalBatch {
         const int a = ChebEval::split_a(k), b = k - a;
         const Group &ta = get(a);
         const Group &tb = get(b);
+        std::vector<const Ciphertext *> term(m, nullptr);
+        if (a != b) term = cptrs(get(a - b));
         Group out = fresh();
-        mul_relin_rescale_batch(c, ptrs(out), cptrs(ta), cptrs(tb), rk, s, true);
-        for (auto &o : out) ct_add(c, *o, *o, s);
-        if (a == b) {
+        mul_relin_rescale_batch(c, ptrs(out), cptrs(ta), cptrs(tb), rk, s, true, &term);
+        if (a == b)
             for (auto &o : out) ct_add_const(c, *o, -1.0, s);
-        } else {
-            const Group &d = get(a - b);
-            Group tmp = fresh();
-            for (size_t i = 0; i < m; i++) ct_clone(*tmp[i], *d[i], s);
-            align(ptrs(tmp), out[0]->chain, out[0]->scale);
-            for (size_t i = 0; i < m; i++) ct_sub(c, *out[i], *tmp[i], s);
-        }
         if (out[0]->chain != tchain(k)) throw LogicError("Chebyshev basis level bookkeeping");
         T[k] = std::move(out);
         return T[k];

@@ -300,6 +300,10 @@ void launch_tensor(const Context &c, const uint64_t *a, const uint64_t *b, uint6
                    size_t a_limbs = 0, size_t b_limbs = 0, bool accumulate = false);
 void launch_square(const Context &c, const uint64_t *a, uint64_t *out3, size_t Ql, hipStream_t s,
                    size_t a_limbs = 0);
+// (d0, d1, d2) = 2 (a (x) b) - cval (t0, t1, 0), the Chebyshev step T_k = 2 T_a T_b - T_(a-b)
+// before relinearisation (b == a squares; t may be null); operands through their first Ql limbs
+void launch_tensor_cheb(const Context &c, const uint64_t *a, const uint64_t *b, const uint64_t *t, uint64_t cval,
+                        uint64_t *out3, size_t Ql, hipStream_t s, size_t a_limbs, size_t b_limbs, size_t t_limbs);
 void launch_rescale(const Context &c, const uint64_t *in, uint64_t *out, size_t Ql, size_t npolys, hipStream_t s,
                     bool rounded = false);
 // the same with caller-provided scratch: last [npolys][N], delta [npolys][Ql - 1][N]
@@ -480,9 +484,13 @@ void ct_align(Context &c, Ciphertext &a, size_t chain, double scale, hipStream_t
 void ct_mul_relin_rescale(Context &c, Ciphertext &out, const Ciphertext &a, const Ciphertext &b,
                           const KSwitchKey &rk, hipStream_t s, bool rounded = false);
 // ct_mul_relin_rescale of out[i] = a[i] * b[i], i < m <= 32, batched
+// With `cheb` (rounded only), each product is the Chebyshev step 2 a b - cheb[i]
+// (cheb[i] may be null: 2 a b), cheb[i] entering at the product's level and scale
+// through an integer constant folded into the tensor before the relinearisation.
 void mul_relin_rescale_batch(Context &c, const std::vector<Ciphertext *> &out,
                              const std::vector<const Ciphertext *> &a, const std::vector<const Ciphertext *> &b,
-                             const KSwitchKey &rk, hipStream_t s, bool rounded = false);
+                             const KSwitchKey &rk, hipStream_t s, bool rounded = false,
+                             const std::vector<const Ciphertext *> *cheb = nullptr);
 // Ct_ct_matrix_mul.cuh:10-181 ct_ct_matrix_mul_colpacking: out[i] =
 // rescale(relinearize(sum_j X[j] * rot(W[j], i * num_batch))) for i < out.size().
 void ctct_matmul_colpacking(Context &c, const std::vector<const Ciphertext *> &X,

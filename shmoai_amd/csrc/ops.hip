@@ -442,11 +442,13 @@ void ct_mul_relin_rescale(Context &c, Ciphertext &out, const Ciphertext &a, cons
 // one conversion and one forward NTT of Ql - 1 limbs per polynomial fewer.
 void mul_relin_rescale_batch(Context &c, const std::vector<Ciphertext *> &out,
                              const std::vector<const Ciphertext *> &a, const std::vector<const Ciphertext *> &b,
-                             const KSwitchKey &rk, hipStream_t s, bool rounded) {
+                             const KSwitchKey &rk, hipStream_t s, bool rounded,
+                             const std::vector<const Ciphertext *> *cheb) {
     const size_t m = out.size();
     if (m == 0) return;
     if (a.size() != m || b.size() != m || 2 * m > (size_t)kPolyPack)
         throw InvalidArgument("mul_relin_rescale_batch: 1..32 products");
+    if (cheb && (!rounded || cheb->size() != m)) throw LogicError("Chebyshev step: rounded products, one term each");
     PolyPack c2{}, ct01{};
     size_t Ql = 0;
     for (size_t i = 0; i < m; i++) {
@@ -460,10 +462,25 @@ void mul_relin_rescale_batch(Context &c, const std::vector<Ciphertext *> &out,
         Ciphertext &o = *out[i];
         o.ctx = &c;
         o.resize(3, chain, s, false);  // tensor / square write d0, d1, d2
-        if (pa == pb || (pa->data() == pb->data()))
+        const bool sq = pa == pb || (pa->data() == pb->data());
+        if (cheb) {
+            // 2 a b - T: T times the integer round(s_a s_b / s_T) carries the product's
+            // scale (relative rounding below 2^-52 of T's own scale)
+            const Ciphertext *t = (*cheb)[i];
+            uint64_t cval = 0;
+            if (t) {
+                if (t->size != 2 || t->Ql() < Ql) throw LogicError("Chebyshev step: term below the product level");
+                const double cv = std::nearbyint(pa->scale * pb->scale / t->scale);
+                if (!(cv >= 1.0 && cv < 9.2e18)) throw LogicError("Chebyshev step: scale ratio out of range");
+                cval = (uint64_t)cv;
+            }
+            launch_tensor_cheb(c, pa->data(), sq ? pa->data() : pb->data(), t ? t->data() : nullptr, cval, o.data(),
+                               Ql, s, pa->Ql(), sq ? pa->Ql() : pb->Ql(), t ? t->Ql() : 0);
+        } else if (sq) {
             launch_square(c, pa->data(), o.data(), Ql, s, pa->Ql());
-        else
+        } else {
             launch_tensor(c, pa->data(), pb->data(), o.data(), Ql, s, pa->Ql(), pb->Ql());
+        }
         o.scale = pa->scale * pb->scale;
         o.ntt = true;
         c2.p[i] = o.poly(2);

@@ -98,6 +98,49 @@ __global__ void k_tensor(const uint64_t *a, const uint64_t *b, uint64_t *out, co
     }
 }
 
+// One step of the Chebyshev basis recursion T_k = 2 T_a T_b - T_(a-b), before the
+// relinearisation: (d0, d1, d2) = 2 (a (x) b) - cval (t0, t1, 0) with cval an integer
+// (reduced per prime here), so that after the rescale the T_(a-b) term carries the
+// product's scale; t == nullptr leaves the doubled product; b == a squares. One row per
+// limb; every operand enters through its first Ql limbs.
+__global__ void k_tensor_cheb(const uint64_t *a, const uint64_t *b, const uint64_t *t, uint64_t *out,
+                              const DevMod *mods, size_t Ql, size_t N, size_t a_limbs, size_t b_limbs, size_t t_limbs,
+                              uint64_t cval) {
+    const size_t S = Ql * N, Sa = a_limbs * N, Sb = b_limbs * N, St = t_limbs * N;
+    SHM_ROWS_LOOP(row, Ql) {
+        const DevMod m = mods[row];
+        const uint64_t q = m.q;
+        const uint64_t cm = reduce64(cval, m), cms = shoup_companion(cm, m);
+        SHM_COLS_LOOP(i, N) {
+            const size_t x = row * N + i;
+            const uint64_t a0 = a[x], a1 = a[x + Sa];
+            uint64_t d0, d1, d2;
+            if (a == b) {
+                d0 = mul_mod(a0, a0, m);
+                d2 = mul_mod(a1, a1, m);
+                const uint64_t lo = a0 * a1, hi = __umul64hi(a0, a1);
+                d1 = barrett128((hi << 1) | (lo >> 63), lo << 1, m);
+            } else {
+                const uint64_t b0 = b[x], b1 = b[x + Sb];
+                d0 = mul_mod(a0, b0, m);
+                d2 = mul_mod(a1, b1, m);
+                d1 = mul_mod(a0 + a1, b0 + b1, m);
+                d1 = csub(csub(d1 + 2 * q - d0 - d2, q), q);
+            }
+            d0 = add_mod(d0, d0, q);
+            d1 = add_mod(d1, d1, q);
+            d2 = add_mod(d2, d2, q);
+            if (t) {
+                d0 = sub_mod(d0, mul_shoup(t[x], cm, cms, q), q);
+                d1 = sub_mod(d1, mul_shoup(t[x + St], cm, cms, q), q);
+            }
+            out[x] = d0;
+            out[x + S] = d1;
+            out[x + 2 * S] = d2;
+        }
+    }
+}
+
 // polymath.cu:505-550 tensor_square_2x2_rns_poly. One row per limb.
 __global__ void k_square(const uint64_t *a, uint64_t *out, const DevMod *mods, size_t Ql, size_t N,
                          size_t a_limbs) {
@@ -357,6 +400,14 @@ void launch_tensor(const Context &c, const uint64_t *a, const uint64_t *b, uint6
     if (a_limbs < Ql || b_limbs < Ql) throw LogicError("tensor operand below the product level");
     k_tensor<<<rows_grid(Ql, c.N), kBlock, 0, s>>>(a, b, out3, c.mods(), Ql, c.N, a_limbs, b_limbs,
                                                    accumulate ? 1 : 0);
+    SHM_HIP(hipGetLastError());
+}
+
+void launch_tensor_cheb(const Context &c, const uint64_t *a, const uint64_t *b, const uint64_t *t, uint64_t cval,
+                        uint64_t *out3, size_t Ql, hipStream_t s, size_t a_limbs, size_t b_limbs, size_t t_limbs) {
+    if (a_limbs < Ql || b_limbs < Ql || (t && t_limbs < Ql)) throw LogicError("Chebyshev operand below the product level");
+    k_tensor_cheb<<<rows_grid(Ql, c.N), kBlock, 0, s>>>(a, b, t, out3, c.mods(), Ql, c.N, a_limbs, b_limbs, t_limbs,
+                                                        cval);
     SHM_HIP(hipGetLastError());
 }
 
