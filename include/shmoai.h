@@ -378,24 +378,44 @@ int shm_bootstrap_eval_mod_batch(shm_bootstrapper *b, shm_ciphertext *const *cts
                                  const shm_kswitch_key *relin, shm_stream_t stream);
 /* ---- bootstrapping server (continuous batching) --------------------------------------
  * No counterpart in the reference, whose BatchBootstrapper (BatchBootstrap.cuh:470-820)
- * takes one fixed batch per call. A worker thread holds the keys resident and
- * bootstraps (S2C-first, as shm_bootstrap_s2c_first) the ciphertexts of all admitted
- * requests in groups of up to max_group (1..8) ciphertexts of one scale, slots of
- * different requests side by side; requests submitted while a group runs join the next
- * group. A request is admitted while the estimated device footprint of the admitted
- * requests stays within mem_budget_bytes (one larger request enters alone). Each
- * output equals, limb for limb, the bootstrap of that input alone. Device work runs on
- * the default stream. Submit copies the inputs; out[] must not be touched until
- * shm_server_wait returns for that request. Destroy finishes all submitted work. */
+ * takes one fixed batch per call. Worker threads hold the keys resident and bootstrap
+ * (S2C-first, as shm_bootstrap_s2c_first) the ciphertexts of all admitted requests in
+ * groups of up to max_group (1..8) ciphertexts of one scale, slots of different requests
+ * side by side; requests submitted while a group runs join the next group. Each output
+ * equals, limb for limb, the bootstrap of that input alone.
+ * Admission is by memory: a request enters while the estimated device footprint of the
+ * admitted requests stays within the cap, the smaller of mem_budget_bytes (0: none) and
+ * the device's free memory (hipMemGetInfo) less a reserve (one larger request enters
+ * alone). With several workers (shm_server_create_ex, 1..4) each has its own stream
+ * (default priority) and groups run concurrently. Inputs are copied at submit and
+ * outputs delivered on the default stream. out[] must not be touched until the wait
+ * returns for that request. Destroy finishes all submitted work. */
 typedef struct shm_server shm_server;
 typedef struct {
     uint64_t submitted, completed, failed, ciphertexts, groups;
     uint64_t in_flight_bytes;
     double p50_ms, p99_ms; /* request latency, submit to completion */
-    double busy_s;         /* wall time inside grouped bootstraps */
+    double busy_s;         /* wall time inside grouped bootstraps, summed over workers */
+    uint64_t packed_requests, shared_cts; /* packed requests done, shared ciphertexts bootstrapped */
+    uint64_t free_bytes, admit_cap_bytes; /* device free memory and admission cap at the last admission */
+    uint32_t workers;
 } shm_server_stats;
 int shm_server_create(shm_bootstrapper *b, const shm_galois_keys *gk, const shm_kswitch_key *relin,
                       uint64_t mem_budget_bytes, uint32_t max_group, shm_server **out);
+int shm_server_create_ex(shm_bootstrapper *b, const shm_galois_keys *gk, const shm_kswitch_key *relin,
+                         uint64_t mem_budget_bytes, uint32_t max_group, uint32_t workers, shm_server **out);
+/* Packed request (the shared slot packing of uneven token counts): in[i] (count column
+ * ciphertexts of one input in the batch_input layout, Batch_encode_encrypt.cuh:12-43)
+ * holds its data in slots [0, slots) and zeros elsewhere. Requests of the same count and
+ * scale are rotated to power-of-two-aligned slot blocks (the next power of two at or
+ * above slots) and summed into shared ciphertexts, bootstrapped once for all of them;
+ * out[i] receives the bootstrapped shared ciphertext i, with this request's slots at
+ * [offset, offset + slots) (shm_server_wait_packed) and the other requests' data in the
+ * other blocks (all requests share the secret key). The Galois keys must hold the
+ * steps +-2^k for every power of two from the block up to N/4. */
+int shm_server_submit_packed(shm_server *srv, const shm_ciphertext *const *in, size_t count, uint32_t slots,
+                             shm_ciphertext *const *out, uint64_t *request_id);
+int shm_server_wait_packed(shm_server *srv, uint64_t request_id, double *latency_ms, uint32_t *slot_offset);
 int shm_server_destroy(shm_server *srv);
 int shm_server_submit(shm_server *srv, const shm_ciphertext *const *in, size_t count, shm_ciphertext *const *out,
                       uint64_t *request_id);

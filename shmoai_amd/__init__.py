@@ -607,16 +607,21 @@ class Bootstrapper:
 class BootstrapServer:
     """Request server with continuous batching of S2C-first bootstraps (no counterpart
     in the reference). submit() queues a request of a few 4-prime ciphertexts and
-    returns its id and output ciphertexts; wait(id) blocks until they are ready and
-    returns the request latency in ms. Requests from any thread share groups of up to
-    max_group ciphertexts; admission is bounded by mem_budget bytes."""
+    returns its id; wait(id) blocks until its outputs are ready and returns them with
+    the request latency in ms. Requests from any thread share groups of up to
+    max_group ciphertexts. Admission is bounded by the device's free memory and by
+    mem_budget bytes (0: free memory only); `workers` (1..4) run groups concurrently.
+    submit_packed() queues one input's column ciphertexts with data in slots
+    [0, slots); requests of one column count and scale share ciphertexts, and wait()
+    also returns the request's slot offset in the shared outputs."""
 
     def __init__(self, boot: Bootstrapper, gk: GaloisKeys, relin_key: KSwitchKey, mem_budget: int = 16 << 30,
-                 max_group: int = 8):
+                 max_group: int = 8, workers: int = 1):
         self.ctx = boot.ctx
         self._keep = (boot, gk, relin_key)
         h = C.c_void_p()
-        check(_lib().shm_server_create(boot.h, gk.h, relin_key.h, int(mem_budget), int(max_group), C.byref(h)))
+        check(_lib().shm_server_create_ex(boot.h, gk.h, relin_key.h, int(mem_budget), int(max_group), int(workers),
+                                          C.byref(h)))
         self.h = h
         self._outs = {}
 
@@ -635,13 +640,24 @@ class BootstrapServer:
         outs = [Ciphertext(self.ctx, 2, 1) for _ in cts]
         rid = C.c_uint64()
         check(_lib().shm_server_submit(self.h, _ptrs(cts), len(cts), _ptrs(outs), C.byref(rid)))
-        self._outs[rid.value] = outs
+        self._outs[rid.value] = (outs, False)
+        return rid.value
+
+    def submit_packed(self, cts: Sequence[Ciphertext], slots: int) -> int:
+        outs = [Ciphertext(self.ctx, 2, 1) for _ in cts]
+        rid = C.c_uint64()
+        check(_lib().shm_server_submit_packed(self.h, _ptrs(cts), len(cts), int(slots), _ptrs(outs), C.byref(rid)))
+        self._outs[rid.value] = (outs, True)
         return rid.value
 
     def wait(self, request_id: int):
-        """Returns (outputs, latency_ms)."""
+        """Returns (outputs, latency_ms), or (outputs, latency_ms, slot_offset) for a packed request."""
         lat = C.c_double()
-        outs = self._outs.pop(request_id)
+        outs, packed = self._outs.pop(request_id)
+        if packed:
+            off = C.c_uint32()
+            check(_lib().shm_server_wait_packed(self.h, request_id, C.byref(lat), C.byref(off)))
+            return outs, lat.value, off.value
         check(_lib().shm_server_wait(self.h, request_id, C.byref(lat)))
         return outs, lat.value
 

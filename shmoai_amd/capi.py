@@ -41,7 +41,9 @@ class BootParams(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [("submitted", u64), ("completed", u64), ("failed", u64), ("ciphertexts", u64), ("groups", u64),
-                ("in_flight_bytes", u64), ("p50_ms", dbl), ("p99_ms", dbl), ("busy_s", dbl)]
+                ("in_flight_bytes", u64), ("p50_ms", dbl), ("p99_ms", dbl), ("busy_s", dbl),
+                ("packed_requests", u64), ("shared_cts", u64), ("free_bytes", u64), ("admit_cap_bytes", u64),
+                ("workers", u32)]
 
 
 class ContextInfo(C.Structure):
@@ -140,6 +142,9 @@ _SIGS = {
     "shm_server_submit": (i32, [vp, pvp, sz, pvp, C.POINTER(u64)]),
     "shm_server_wait": (i32, [vp, u64, C.POINTER(dbl)]),
     "shm_server_get_stats": (i32, [vp, vp]),
+    "shm_server_create_ex": (i32, [vp, vp, vp, u64, u32, u32, pvp]),
+    "shm_server_submit_packed": (i32, [vp, pvp, sz, u32, pvp, C.POINTER(u64)]),
+    "shm_server_wait_packed": (i32, [vp, u64, C.POINTER(dbl), C.POINTER(u32)]),
     "shm_evaluate_chebyshev": (i32, [vp, pvp, sz, C.POINTER(dbl), sz, dbl, dbl, vp, vp]),
     "shm_ct_ct_matrix_mul_colpacking": (i32, [vp, pvp, pvp, i32, i32, i32, i32, i32, vp, vp, pvp, vp]),
     "shm_ct_ct_matrix_mul_diagpacking": (i32, [vp, pvp, pvp, i32, i32, i32, i32, i32, vp, vp, pvp, vp]),

@@ -205,19 +205,31 @@ struct ServerStats {
     uint64_t submitted, completed, failed, ciphertexts, groups;
     size_t in_flight_bytes;
     double p50_ms, p99_ms;  // request latency, submit to completion (completed requests)
-    double busy_s;          // wall time spent in grouped bootstraps
+    double busy_s;          // wall time spent in grouped bootstraps (summed over workers)
+    uint64_t packed_requests, shared_cts;  // packed requests completed, shared ciphertexts bootstrapped
+    size_t free_bytes, admit_cap_bytes;    // device memory free and the admission cap at the last admission
+    uint32_t workers;
 };
 
 class BootServer {
    public:
-    BootServer(Bootstrapper &b, const GaloisKeys &gk, const KSwitchKey &rk, size_t mem_budget, size_t max_group);
-    ~BootServer();  // finishes every submitted request, then stops the worker
+    // mem_budget 0: admission by the device's free memory alone; workers 1..4, each
+    // with its own stream, run groups concurrently
+    BootServer(Bootstrapper &b, const GaloisKeys &gk, const KSwitchKey &rk, size_t mem_budget, size_t max_group,
+               size_t workers = 1);
+    ~BootServer();  // finishes every submitted request, then stops the workers
     BootServer(const BootServer &) = delete;
     BootServer &operator=(const BootServer &) = delete;
     // queue a request; out[i] receives the bootstrapped in[i] (4 primes in)
     uint64_t submit(const std::vector<const Ciphertext *> &in, const std::vector<Ciphertext *> &out);
+    // queue a packed request: in[i] holds its data in slots [0, slots) and zeros
+    // elsewhere; it is packed with other requests of the same count and scale into
+    // shared ciphertexts, and out[i] receives the bootstrapped shared ciphertext i with
+    // this request's slots at [offset, offset + slots) (offset from wait)
+    uint64_t submit_packed(const std::vector<const Ciphertext *> &in, size_t slots,
+                           const std::vector<Ciphertext *> &out);
     // block until the request has finished; rethrows its error
-    void wait(uint64_t id, double *latency_ms);
+    void wait(uint64_t id, double *latency_ms, size_t *slot_offset = nullptr);
     ServerStats stats();
 
    private:
@@ -226,27 +238,35 @@ class BootServer {
         std::vector<std::unique_ptr<Ciphertext>> in;
         std::vector<Ciphertext *> out;
         size_t next = 0, done = 0, bytes = 0;
+        size_t slots = 0, offset = 0;  // packed requests: slots > 0
         bool finished = false;
         double latency_ms = 0.0;
         std::exception_ptr error;
         std::chrono::steady_clock::time_point t_submit;
     };
-    void loop();
+    typedef std::vector<std::pair<std::shared_ptr<Request>, size_t>> Group;
+    typedef std::vector<std::shared_ptr<Request>> Pack;
+    void loop(size_t w);
     void admit_locked();
+    void finish_locked(const std::chrono::steady_clock::time_point &t1);
+    void run_group(Group &group, hipStream_t s);
+    void run_pack(Pack &pack, hipStream_t s);
 
     Bootstrapper &b_;
     const GaloisKeys &gk_;
     const KSwitchKey &rk_;
-    size_t budget_, max_group_, ct_bytes_ = 0, in_flight_ = 0;
+    size_t budget_, max_group_, ct_bytes_ = 0, in_flight_ = 0, free_bytes_ = 0, cap_bytes_ = 0;
     std::mutex mu_;
     std::condition_variable cv_work_, cv_done_;
     std::deque<std::shared_ptr<Request>> waiting_, active_;
     std::map<uint64_t, std::shared_ptr<Request>> all_;
     std::vector<double> lat_;
     uint64_t next_id_ = 0, submitted_ = 0, completed_ = 0, failed_ = 0, cts_done_ = 0, groups_ = 0;
+    uint64_t packed_done_ = 0, shared_cts_ = 0;
     double busy_s_ = 0.0;
     bool stop_ = false;
-    std::thread th_;
+    std::vector<hipStream_t> streams_;
+    std::vector<std::thread> th_;
 };
 
 // Multiply by the constant `value` encoded at scale `sigma` and accumulate:

@@ -1184,6 +1184,47 @@ int shm_server_create(shm_bootstrapper *b, const shm_galois_keys *gk, const shm_
     });
 }
 
+int shm_server_create_ex(shm_bootstrapper *b, const shm_galois_keys *gk, const shm_kswitch_key *relin,
+                         uint64_t mem_budget_bytes, uint32_t max_group, uint32_t workers, shm_server **out) {
+    return guard([&] {
+        need(gk, "galois_keys");
+        need(relin, "relin_keys");
+        need(out, "out");
+        boot::Bootstrapper &bs = ready(b, nullptr);
+        auto h = std::make_unique<shm_server>();
+        h->s = std::make_unique<boot::BootServer>(bs, gk->g, relin->k, (size_t)mem_budget_bytes, max_group,
+                                                  (size_t)workers);
+        *out = h.release();
+    });
+}
+
+int shm_server_submit_packed(shm_server *srv, const shm_ciphertext *const *in, size_t count, uint32_t slots,
+                             shm_ciphertext *const *out, uint64_t *request_id) {
+    return guard([&] {
+        need(srv, "server");
+        need(request_id, "request_id");
+        if (count == 0 || !in || !out) throw InvalidArgument("server: empty request");
+        std::vector<const Ciphertext *> ins(count);
+        std::vector<Ciphertext *> outs(count);
+        for (size_t i = 0; i < count; i++) {
+            need(in[i], "in[i]");
+            need(out[i], "out[i]");
+            ins[i] = &in[i]->ct;
+            outs[i] = &out[i]->ct;
+        }
+        *request_id = srv->s->submit_packed(ins, slots, outs);
+    });
+}
+
+int shm_server_wait_packed(shm_server *srv, uint64_t request_id, double *latency_ms, uint32_t *slot_offset) {
+    return guard([&] {
+        need(srv, "server");
+        size_t off = 0;
+        srv->s->wait(request_id, latency_ms, &off);
+        if (slot_offset) *slot_offset = (uint32_t)off;
+    });
+}
+
 int shm_server_destroy(shm_server *srv) {
     return guard([&] { delete srv; });
 }
@@ -1226,6 +1267,11 @@ int shm_server_get_stats(shm_server *srv, shm_server_stats *st) {
         st->in_flight_bytes = v.in_flight_bytes;
         st->p50_ms = v.p50_ms;
         st->p99_ms = v.p99_ms;
+        st->packed_requests = v.packed_requests;
+        st->shared_cts = v.shared_cts;
+        st->free_bytes = v.free_bytes;
+        st->admit_cap_bytes = v.admit_cap_bytes;
+        st->workers = v.workers;
         st->busy_s = v.busy_s;
     });
 }

@@ -70,6 +70,22 @@ def test_full_768_shared_a_pass(c3):
     assert all(o.chain_index == outs[0].chain_index for o in outs)
 
 
+def test_shared_a_n16_chunk(c3):
+    """A shared-a chunk of n = 16 (17 Sh-Rot outputs, 32 EvalMod members) equals the
+    chunked entry over the same 16 and decrypts within 2e-5."""
+    ctx, sk, boot, steps, rk, keys = c3
+    keys16 = shm.SharedAKeys(ctx, sk, 16, 192, 56, ctx.chain(4))
+    keys16.add_galois(steps, seed=17)
+    zs, cts = _inputs(ctx, sk, 16, 5)
+    single = shm.bootstrap_shared_a_s2c_first(boot, keys16, cts, rk)
+    chunked = shm.bootstrap_shared_a_s2c_first_chunks(boot, keys16, cts, rk)
+    for i in (0, 9, 15):
+        assert np.array_equal(chunked[i].numpy(), single[i].numpy()), f"member {i}"
+    dec, enc = shm.Decryptor(ctx, sk), shm.Encoder(ctx)
+    err = max(float(np.max(np.abs(enc.decode(dec.decrypt(single[i]), complex_out=True) - zs[i]))) for i in range(16))
+    assert err < 2e-5, err
+
+
 def test_graph_capture_is_refused(c3):
     ctx, sk, boot, steps, rk, keys = c3
     with pytest.raises(shm.capi.ShmLogicError):
