@@ -205,6 +205,15 @@ def serving_bench(shm, boot, gk, rk, cts, n_req: int = 12, seed: int = 9, worker
             ids.append(srv.submit([cts[(k + i) % len(cts)] for i in range(m)]))
             k += m
 
+    # one pass of the same sequence first: the group shapes it forms get autotuned there
+    warm = shm.BootstrapServer(boot, gk, rk, mem_budget=0, max_group=8, workers=workers)
+    wids, k = [], 0
+    for m in sizes:
+        wids.append(warm.submit([cts[(k + i) % len(cts)] for i in range(m)]))
+        k += m
+    for rid in wids:
+        warm.wait(rid)
+    warm.close()
     t0 = time.perf_counter()
     th = threading.Thread(target=client)
     th.start()
@@ -699,6 +708,13 @@ def main() -> int:
         del gk_packed
         gc.collect()
     boot768["keyswitch_counts_per_chunk_n8"] = {str(q): list(v) for q, v in counters.items()}
+    tuned = [ln for ln in shm.autotune_report().splitlines() if "N=65536" in ln]
+    picks = {}
+    for ln in tuned:
+        k = ln.split("-> slots ")[1].split(" ")[0]
+        picks[k] = picks.get(k, 0) + 1
+    boot768["autotune"] = {"shapes_tuned": len(tuned), "slots_chosen": picks,
+                           "examples": [ln for ln in tuned if "-> slots 1 " not in ln][:4]}
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -571,10 +571,19 @@ int shm_keyswitch_counters(uint64_t *modup, uint64_t *inner, uint64_t *moddown, 
 int shm_keyswitch_counters_reset(void);
 /* Launch-shape knobs (no counterpart in the reference; results never depend on them):
  * "ks_batch_slots_log2" and "ks_multi_slots_log2" set log2 of the member / output slots
- * per block of the batched and multi-output key inner products (0..4; 0 or -1 = one slot,
- * the default: each thread loops over all members / outputs). Process-wide. */
+ * per block of the batched and multi-output key inner products (0..4), or -1 (the
+ * default) for the autotuned choice. Process-wide. */
 int shm_set_tuning(const char *name, int32_t value);
 int shm_get_tuning(const char *name, int32_t *value);
+/* Autotuner (no counterpart in the reference): the first call of each inner-product
+ * shape (kernel, N, Ql, members or outputs, digits, permuted, ...) that writes its
+ * output outright times every candidate slot count on its own operands (it waits for
+ * its stream once) and caches the fastest; calls recorded into a graph use the cached
+ * choice or one slot. On by default. The report lists the cached shapes, the choice and
+ * the timings, one line each (needed = bytes including the terminating zero). */
+int shm_autotune_enable(int32_t on);
+int shm_autotune_clear(void);
+int shm_autotune_report(char *buf, size_t cap, size_t *needed);
 
 #ifdef __cplusplus
 }

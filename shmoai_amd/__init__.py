@@ -841,8 +841,26 @@ def keyswitch_counters_reset() -> None:
 
 
 def set_tuning(name: str, value: int) -> None:
-    """Set a launch-shape knob ("ks_batch_slots_log2", "ks_multi_slots_log2"; -1 = default)."""
+    """Set a launch-shape knob ("ks_batch_slots_log2", "ks_multi_slots_log2"; -1 = autotuned)."""
     check(_lib().shm_set_tuning(name.encode(), int(value)))
+
+
+def autotune_enable(on: bool = True) -> None:
+    """Switch the inner-product autotuner on or off (on by default)."""
+    check(_lib().shm_autotune_enable(1 if on else 0))
+
+
+def autotune_clear() -> None:
+    check(_lib().shm_autotune_clear())
+
+
+def autotune_report() -> str:
+    """The autotuned shapes, the chosen slot count and the candidates' timings."""
+    need = C.c_size_t()
+    check(_lib().shm_autotune_report(None, 0, C.byref(need)))
+    buf = C.create_string_buffer(need.value)
+    check(_lib().shm_autotune_report(buf, need.value, C.byref(need)))
+    return buf.value.decode()
 
 
 def get_tuning(name: str) -> int:

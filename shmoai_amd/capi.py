@@ -198,6 +198,9 @@ _SIGS = {
     "shm_keyswitch_counters_reset": (i32, []),
     "shm_set_tuning": (i32, [C.c_char_p, i32]),
     "shm_get_tuning": (i32, [C.c_char_p, C.POINTER(i32)]),
+    "shm_autotune_enable": (i32, [i32]),
+    "shm_autotune_clear": (i32, []),
+    "shm_autotune_report": (i32, [C.c_char_p, sz, C.POINTER(sz)]),
 }
 
 _lib = None

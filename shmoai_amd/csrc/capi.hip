@@ -1846,6 +1846,26 @@ int shm_set_tuning(const char *name, int32_t value) {
     });
 }
 
+int shm_autotune_enable(int32_t on) {
+    return guard([&] { ks_autotune_enable(on != 0); });
+}
+
+int shm_autotune_clear(void) {
+    return guard([&] { ks_autotune_clear(); });
+}
+
+int shm_autotune_report(char *buf, size_t cap, size_t *needed) {
+    return guard([&] {
+        const std::string r = ks_autotune_report();
+        if (needed) *needed = r.size() + 1;
+        if (buf && cap) {
+            const size_t n = std::min(cap - 1, r.size());
+            std::memcpy(buf, r.data(), n);
+            buf[n] = 0;
+        }
+    });
+}
+
 int shm_get_tuning(const char *name, int32_t *value) {
     return guard([&] {
         if (!name || !value) throw InvalidArgument("null argument");

@@ -242,12 +242,17 @@ void ks_counters(uint64_t *modup, uint64_t *inner, uint64_t *moddown, size_t cap
 void ks_counters_reset();
 // Launch-shape knobs of the key-switch inner products (results do not depend on
 // them): log2 of the member slots per block of the batched inner product and of
-// the output slots of the multi-output one (0 or -1: one slot, the default).
+// the output slots of the multi-output one (0..4; -1: the autotuned choice).
 struct KsTuning {
     int batch_slots_log2 = -1;
     int multi_slots_log2 = -1;
 };
 KsTuning &ks_tuning();
+// Autotuning of those slot counts on first use of each shape (on by default): switch
+// it on or off, forget the cached choices, or list them with their timings.
+void ks_autotune_enable(bool on);
+void ks_autotune_clear();
+std::string ks_autotune_report();
 // Encoder decompose + forward NTT in one: out[p][k] = NTT of the rounded FFT
 // output fft[p] (N/2 complex entries: real parts are coefficients 0..N/2-1,
 // imaginary parts N/2..N-1) reduced mod prime k, for k < n_limbs.

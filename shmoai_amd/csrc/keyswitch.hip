@@ -12,6 +12,11 @@
 #include <algorithm>
 
 #include <atomic>
+#include <cstdio>
+#include <functional>
+#include <map>
+#include <mutex>
+#include <string>
 
 #include "engine.h"
 
@@ -420,6 +425,94 @@ KsTuning &ks_tuning() {
     return t;
 }
 
+// Autotuner of the inner products' slot counts (the member / output grouping per
+// block). The first call of a shape (kernel, N, Ql, members or outputs, digits,
+// permuted, P d added, 52-bit primes) that writes its output outright times every
+// candidate slot count on the call's own operands (a warm-up, then each candidate
+// twice, the minimum kept) and caches the fastest; every candidate computes the
+// same words, so the call's result is the same whichever ran last. Calls recorded
+// into a graph, and accumulating calls, use the cached choice or one slot.
+namespace {
+struct TuneEntry {
+    int best = 0;
+    std::vector<float> us;
+};
+std::mutex g_tune_mu;
+std::map<std::string, TuneEntry> g_tune;
+std::atomic<bool> g_tune_on{true};
+
+bool capturing(hipStream_t s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) != hipSuccess) {
+        (void)hipGetLastError();
+        return true;
+    }
+    return st != hipStreamCaptureStatusNone;
+}
+
+// returns the slot count (log2) to use; `ran` tells whether the tuning runs
+// already produced the call's output
+int tuned_slots(const std::string &key, int max_lg, bool may_tune, hipStream_t s,
+                const std::function<void(int)> &launch, bool *ran) {
+    *ran = false;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) return std::min(it->second.best, max_lg);
+    }
+    if (!may_tune || max_lg == 0 || !g_tune_on.load() || capturing(s)) return 0;
+    const int nc = max_lg + 1;
+    std::vector<hipEvent_t> ev(2 * nc * 2);
+    for (auto &e : ev) SHM_HIP(hipEventCreate(&e));
+    launch(0);  // warm-up
+    for (int rep = 0; rep < 2; rep++)
+        for (int lg = 0; lg < nc; lg++) {
+            SHM_HIP(hipEventRecord(ev[2 * (rep * nc + lg)], s));
+            launch(lg);
+            SHM_HIP(hipEventRecord(ev[2 * (rep * nc + lg) + 1], s));
+        }
+    SHM_HIP(hipEventSynchronize(ev.back()));
+    TuneEntry te;
+    te.us.assign(nc, 1e30f);
+    for (int rep = 0; rep < 2; rep++)
+        for (int lg = 0; lg < nc; lg++) {
+            float ms = 0.f;
+            SHM_HIP(hipEventElapsedTime(&ms, ev[2 * (rep * nc + lg)], ev[2 * (rep * nc + lg) + 1]));
+            te.us[lg] = std::min(te.us[lg], ms * 1000.f);
+        }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    te.best = (int)(std::min_element(te.us.begin(), te.us.end()) - te.us.begin());
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[key] = te;
+    }
+    *ran = true;
+    return te.best;
+}
+}  // namespace
+
+void ks_autotune_enable(bool on) { g_tune_on.store(on); }
+
+void ks_autotune_clear() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune.clear();
+}
+
+std::string ks_autotune_report() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    std::string out;
+    for (auto &[k, e] : g_tune) {
+        out += k + " -> slots " + std::to_string(1 << e.best) + " (us:";
+        for (size_t i = 0; i < e.us.size(); i++) {
+            char b[32];
+            snprintf(b, sizeof b, " %d:%.1f", 1 << i, e.us[i]);
+            out += b;
+        }
+        out += ")\n";
+    }
+    return out;
+}
+
 void ks_counters_reset() {
     for (size_t i = 0; i < kCounterLevels; i++) {
         g_cnt_modup[i].store(0, std::memory_order_relaxed);
@@ -466,18 +559,30 @@ void ks_inner_multi(const Context &c, const uint64_t *d, size_t Ql, size_t ndig,
         return;
     }
     bump(g_cnt_inner, Ql, nout);
-    // one output slot by default: each thread loads its digits once and loops over the
-    // outputs (measured fastest on the shared-a bootstrap chunk)
-    int lgog = 0;
-    if (ks_tuning().multi_slots_log2 > 0)
-        while (lgog < ks_tuning().multi_slots_log2 && (size_t(1) << lgog) < nout) lgog++;
-    const unsigned blocks = (unsigned)((total + (256u >> lgog) - 1) >> (8 - lgog));
+    // output slots per block: the knob if set, else the autotuned choice (one slot,
+    // each thread loading its digits once and looping over the outputs, until tuned)
+    int max_lg = 0;
+    while (max_lg < 4 && (size_t(1) << max_lg) < nout) max_lg++;
     auto fn = c.primes52
                   ? (ndig <= 4 ? k_ks_inner_multi<4, true> : ndig <= 8 ? k_ks_inner_multi<8, true> : k_ks_inner_multi<16, true>)
                   : (ndig <= 4 ? k_ks_inner_multi<4, false> : ndig <= 8 ? k_ks_inner_multi<8, false> : k_ks_inner_multi<16, false>);
-    fn<<<blocks, kBlock, 0, s>>>(d, kr, (int)nout, cx, c.mods(), Ql, c.nQ, c.nP, (int)ndig, c.N, local ? 1 : 0, perm,
-                                 accumulate ? 1 : 0, lgog);
-    SHM_HIP(hipGetLastError());
+    auto launch = [&](int lgog) {
+        const unsigned blocks = (unsigned)((total + (256u >> lgog) - 1) >> (8 - lgog));
+        fn<<<blocks, kBlock, 0, s>>>(d, kr, (int)nout, cx, c.mods(), Ql, c.nQ, c.nP, (int)ndig, c.N, local ? 1 : 0,
+                                     perm, accumulate ? 1 : 0, lgog);
+        SHM_HIP(hipGetLastError());
+    };
+    if (ks_tuning().multi_slots_log2 >= 0) {
+        launch(std::min(ks_tuning().multi_slots_log2, max_lg));
+        return;
+    }
+    const std::string tk = "multi N=" + std::to_string(c.N) + " Ql=" + std::to_string(Ql) +
+                           " outputs=" + std::to_string(nout) + " digits=" + std::to_string(ndig) +
+                           " perm=" + std::to_string(perm ? 1 : 0) + " local=" + std::to_string(local ? 1 : 0) +
+                           " p52=" + std::to_string(c.primes52 ? 1 : 0);
+    bool ran = false;
+    const int lg = tuned_slots(tk, max_lg, !accumulate, s, launch, &ran);
+    if (!ran) launch(lg);
 }
 
 void ks_moddown(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, uint64_t *out, size_t out_stride,
@@ -565,24 +670,36 @@ void ks_inner_batch(const Context &c, const uint64_t *d, size_t m, size_t Ql, co
     if (beta > c.dnum) throw LogicError("key switching key has too few digits for this level");
     if (m == 0 || m > 64) throw InvalidArgument("ks_inner_batch: 1..64 members");
     bump(g_cnt_inner, Ql, 2 * m);
-    // one member slot by default: each thread loads its key words once and loops over
-    // the members (measured fastest on the shared-a bootstrap chunk)
-    int lgmg = 0;
-    if (ks_tuning().batch_slots_log2 > 0)
-        while (lgmg < ks_tuning().batch_slots_log2 && (size_t(1) << lgmg) < m) lgmg++;
+    // member slots per block: the knob if set, else the autotuned choice (one slot,
+    // each thread loading its key words once and looping over the members, until tuned)
+    int max_lg = 0;
+    while (max_lg < 4 && (size_t(1) << max_lg) < m) max_lg++;
     const size_t total = QlP * c.N;
-    const unsigned blocks = (unsigned)((total + (256u >> lgmg) - 1) >> (8 - lgmg));
     // digits in chunks of kMaxDigits; a later chunk adds its partial sums mod q
     for (size_t g0 = 0; g0 < beta; g0 += kMaxDigits) {
         const size_t nb = std::min<size_t>(kMaxDigits, beta - g0);
         auto fn = c.primes52
                       ? (nb <= 4 ? k_ks_inner_batch<4, true> : nb <= 8 ? k_ks_inner_batch<8, true> : k_ks_inner_batch<16, true>)
                       : (nb <= 4 ? k_ks_inner_batch<4, false> : nb <= 8 ? k_ks_inner_batch<8, false> : k_ks_inner_batch<16, false>);
-        fn<<<blocks, kBlock, 0, s>>>(d, key, cx, c.mods(), Ql, c.nQ, c.nQP(), c.nP, (int)beta, c.N, (int)m, lgmg,
-                                     perm, own ? *own : PolyPack{}, own ? 1 : 0, (int)g0, (int)nb, g0 > 0 ? 1 : 0,
-                                     add ? *add : PolyPack{}, add ? 1 : 0, c.pmod.as<uint64_t>(),
-                                     c.pmod_s.as<uint64_t>());
-        SHM_HIP(hipGetLastError());
+        auto launch = [&](int lgmg) {
+            const unsigned blocks = (unsigned)((total + (256u >> lgmg) - 1) >> (8 - lgmg));
+            fn<<<blocks, kBlock, 0, s>>>(d, key, cx, c.mods(), Ql, c.nQ, c.nQP(), c.nP, (int)beta, c.N, (int)m,
+                                         lgmg, perm, own ? *own : PolyPack{}, own ? 1 : 0, (int)g0, (int)nb,
+                                         g0 > 0 ? 1 : 0, add ? *add : PolyPack{}, add ? 1 : 0,
+                                         c.pmod.as<uint64_t>(), c.pmod_s.as<uint64_t>());
+            SHM_HIP(hipGetLastError());
+        };
+        if (ks_tuning().batch_slots_log2 >= 0) {
+            launch(std::min(ks_tuning().batch_slots_log2, max_lg));
+            continue;
+        }
+        const std::string tk = "batch N=" + std::to_string(c.N) + " Ql=" + std::to_string(Ql) +
+                               " m=" + std::to_string(m) + " digits=" + std::to_string(nb) +
+                               " perm=" + std::to_string(perm ? 1 : 0) + " add=" + std::to_string(add ? 1 : 0) +
+                               " p52=" + std::to_string(c.primes52 ? 1 : 0);
+        bool ran = false;
+        const int lg = tuned_slots(tk, max_lg, g0 == 0, s, launch, &ran);
+        if (!ran) launch(lg);
     }
 }
 
