@@ -565,6 +565,10 @@ int shm_boot_graph_destroy(shm_boot_graph *g);
  * combined operand read, plus the output read where it accumulates). */
 int shm_probe_fused_ntt_begin(void);
 int shm_probe_fused_ntt_end(uint64_t *launches, double *ms, double *bytes);
+/* The plain forward NTT tile passes (the ModUp digits' and other forward transforms' last
+ * pass) timed in the same window, with 16 N algorithmic bytes per limb (each limb read and
+ * written once); valid after shm_probe_fused_ntt_end. */
+int shm_probe_plain_ntt_last(uint64_t *launches, double *ms, double *bytes);
 /* Key-switching work since the last reset, per level Ql (entry Ql, cap entries): polynomials
  * ModUp'd, inner-product output polynomials and polynomials ModDown'd. */
 int shm_keyswitch_counters(uint64_t *modup, uint64_t *inner, uint64_t *moddown, size_t cap);

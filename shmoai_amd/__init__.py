@@ -829,6 +829,13 @@ def probe_fused_ntt_end() -> dict:
     return {"launches": n.value, "ms": ms.value, "bytes": by.value}
 
 
+def probe_plain_ntt_last() -> dict:
+    """The plain forward NTT tile passes of the last probe window: launches, ms, bytes."""
+    n, ms, by = C.c_uint64(), C.c_double(), C.c_double()
+    check(_lib().shm_probe_plain_ntt_last(C.byref(n), C.byref(ms), C.byref(by)))
+    return {"launches": n.value, "ms": ms.value, "bytes": by.value}
+
+
 def keyswitch_counters(cap: int = 128) -> dict:
     """Key-switching work per level since the last reset: {Ql: (modup, inner, moddown)}."""
     a, b, c = (np.zeros(cap, np.uint64) for _ in range(3))

@@ -194,6 +194,7 @@ _SIGS = {
     "shm_boot_graph_destroy": (i32, [vp]),
     "shm_probe_fused_ntt_begin": (i32, []),
     "shm_probe_fused_ntt_end": (i32, [C.POINTER(u64), C.POINTER(dbl), C.POINTER(dbl)]),
+    "shm_probe_plain_ntt_last": (i32, [C.POINTER(u64), C.POINTER(dbl), C.POINTER(dbl)]),
     "shm_keyswitch_counters": (i32, [pu64, pu64, pu64, sz]),
     "shm_keyswitch_counters_reset": (i32, []),
     "shm_set_tuning": (i32, [C.c_char_p, i32]),

@@ -1824,6 +1824,14 @@ int shm_probe_fused_ntt_end(uint64_t *launches, double *ms, double *bytes) {
     });
 }
 
+int shm_probe_plain_ntt_last(uint64_t *launches, double *ms, double *bytes) {
+    return guard([&] {
+        size_t n = 0;
+        probe_plain_ntt_last(&n, ms, bytes);
+        if (launches) *launches = n;
+    });
+}
+
 int shm_keyswitch_counters(uint64_t *modup, uint64_t *inner, uint64_t *moddown, size_t cap) {
     return guard([&] { ks_counters(modup, inner, moddown, cap); });
 }

@@ -236,6 +236,9 @@ void set_rescale_tile_events(hipEvent_t start, hipEvent_t stop);
 // launch count, the summed kernel time and the algorithmic bytes of those launches.
 void probe_fused_ntt_begin();
 void probe_fused_ntt_end(size_t *launches, double *ms, double *bytes);
+// The same window's plain forward tile passes (the ModUp digits' and other forward
+// NTTs' last pass), with 16 N algorithmic bytes per limb; valid after probe_fused_ntt_end.
+void probe_plain_ntt_last(size_t *launches, double *ms, double *bytes);
 // Key-switch work counters, per level Ql (index < 128): polynomials ModUp'd,
 // inner-product output polynomials, polynomials ModDown'd since the last reset.
 void ks_counters(uint64_t *modup, uint64_t *inner, uint64_t *moddown, size_t cap);

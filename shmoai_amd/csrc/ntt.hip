@@ -442,8 +442,12 @@ hipEvent_t g_rescale_tile_ev[2] = {nullptr, nullptr};
 struct FusedProbe {
     std::mutex mu;
     bool on = false;
-    std::vector<hipEvent_t> ev;  // start, stop per launch
+    std::vector<hipEvent_t> ev;  // start, stop per launch of the fused last pass
     double bytes = 0.0;
+    std::vector<hipEvent_t> ev_plain;  // the same for the plain forward tile pass
+    double bytes_plain = 0.0;
+    size_t last_plain_launches = 0;
+    double last_plain_ms = 0.0, last_plain_bytes = 0.0;
 } g_probe;
 
 template <bool FWD, int IOA, int IOB>
@@ -517,6 +521,18 @@ void launch_ntt(const Context &c, uint64_t *data, const LimbMap &mp, size_t tota
         const bool timed = IOB == IO_RESCALE && g_rescale_tile_ev[0] && g_rescale_tile_ev[1];
         if (timed) SHM_HIP(hipEventRecord(g_rescale_tile_ev[0], s));
         hipEvent_t pe[2] = {nullptr, nullptr};
+        if (IOB == IO_PLAIN) {
+            std::lock_guard<std::mutex> lk(g_probe.mu);
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            if (g_probe.on && hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) {
+                SHM_HIP(hipEventCreate(&pe[0]));
+                SHM_HIP(hipEventCreate(&pe[1]));
+                g_probe.ev_plain.push_back(pe[0]);
+                g_probe.ev_plain.push_back(pe[1]);
+                // the transform's algorithmic bytes: every limb read and written once
+                g_probe.bytes_plain += 16.0 * N * total_limbs;
+            }
+        }
         if (IOB == IO_RESCALE) {
             std::lock_guard<std::mutex> lk(g_probe.mu);
             hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -557,8 +573,11 @@ void ntt_forward(const Context &c, uint64_t *data, size_t n_polys, size_t n_limb
 void probe_fused_ntt_begin() {
     std::lock_guard<std::mutex> lk(g_probe.mu);
     for (hipEvent_t e : g_probe.ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : g_probe.ev_plain) (void)hipEventDestroy(e);
     g_probe.ev.clear();
+    g_probe.ev_plain.clear();
     g_probe.bytes = 0.0;
+    g_probe.bytes_plain = 0.0;
     g_probe.on = true;
 }
 
@@ -577,6 +596,24 @@ void probe_fused_ntt_end(size_t *launches, double *ms, double *bytes) {
     if (bytes) *bytes = g_probe.bytes;
     for (hipEvent_t e : g_probe.ev) (void)hipEventDestroy(e);
     g_probe.ev.clear();
+    double tp = 0.0;
+    for (size_t i = 0; i + 1 < g_probe.ev_plain.size(); i += 2) {
+        float t = 0.0f;
+        SHM_HIP(hipEventElapsedTime(&t, g_probe.ev_plain[i], g_probe.ev_plain[i + 1]));
+        tp += t;
+    }
+    g_probe.last_plain_launches = g_probe.ev_plain.size() / 2;
+    g_probe.last_plain_ms = tp;
+    g_probe.last_plain_bytes = g_probe.bytes_plain;
+    for (hipEvent_t e : g_probe.ev_plain) (void)hipEventDestroy(e);
+    g_probe.ev_plain.clear();
+}
+
+void probe_plain_ntt_last(size_t *launches, double *ms, double *bytes) {
+    std::lock_guard<std::mutex> lk(g_probe.mu);
+    if (launches) *launches = g_probe.last_plain_launches;
+    if (ms) *ms = g_probe.last_plain_ms;
+    if (bytes) *bytes = g_probe.last_plain_bytes;
 }
 
 void set_rescale_tile_events(hipEvent_t start, hipEvent_t stop) {
