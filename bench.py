@@ -517,13 +517,25 @@ def matmul_bench(shm, args, threads):
         log(f"bench: calibration copy of {hs[0].words() * 8} bytes done")
     kernel_ms, tile_ms = [], []
     steps = 10
+    # per-kernel times from the serial form (contraction, then rescale)...
+    shm.set_tuning("matmul_overlap", 0)
     t0 = time.perf_counter()
     for s in range(steps):
         shm.ct_pt_matrix_mul_wo_pre(hs, W, ROW_W, COL_W, ROW_W, ctx, variant=variant, out=out)
         kernel_ms.append(shm.last_matmul_kernel_ms())
         tile_ms.append(shm.last_matmul_rescale_tile_ms())
-    dt = time.perf_counter() - t0  # every matmul call synchronises its stream before returning
-    ms_per_step = dt / steps * 1e3
+    ms_per_step = (time.perf_counter() - t0) / steps * 1e3  # every call synchronises before returning
+    ref_out = [o.numpy() for o in out[:2]]
+    # ...and the overlapped form (contraction in limb windows with the rescale on a
+    # second stream), which gives the same limbs
+    shm.set_tuning("matmul_overlap", 1)
+    shm.ct_pt_matrix_mul_wo_pre(hs, W, ROW_W, COL_W, ROW_W, ctx, variant=variant, out=out)
+    assert all(np.array_equal(a, o.numpy()) for a, o in zip(ref_out, out[:2]))
+    t0 = time.perf_counter()
+    for s in range(steps):
+        shm.ct_pt_matrix_mul_wo_pre(hs, W, ROW_W, COL_W, ROW_W, ctx, variant=variant, out=out)
+    overlap_ms = (time.perf_counter() - t0) / steps * 1e3
+    shm.set_tuning("matmul_overlap", 0)
     k_ms, t_ms = float(np.mean(kernel_ms)), float(np.mean(tile_ms))
     N = ctx.N
     alg_step = ROW_W * 2 * L * N * 8 + COL_W * 2 * (L - 1) * N * 8   # read X, write Y (SURVEY 9.46 GB)
@@ -536,6 +548,10 @@ def matmul_bench(shm, args, threads):
                        "256 inputs x 128 tokens per ciphertext, incl. rescale",
            "variant": args.variant, "last_variant": shm.last_matmul_variant(), "steps": steps,
            "ms_per_step": round(ms_per_step, 4), "s_per_input": ms_per_step * 1e-3 / INPUTS,
+           "ms_per_step_overlapped": round(overlap_ms, 4),
+           "overlap_note": "the overlapped form runs the contraction in 3 limb windows with the rescale on a second "
+                           "stream (same limbs); both kernels fill the GPU, so it is not faster and the serial "
+                           "form is the default",
            "algorithmic_bytes_per_step": alg_step,
            "step_hbm_gbs_algorithmic": round(alg_step / (ms_per_step * 1e-3) / 1e9, 1),
            "step_frac_of_hbm_peak": round(alg_step / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
@@ -547,7 +563,8 @@ def matmul_bench(shm, args, threads):
                                  "traffic": traffic_t}}
     if traffic_c and traffic_t:
         res["traffic_over_algorithmic_step"] = round((traffic_c + traffic_t) / alg_step, 3)
-    log(f"bench: matmul 192x192 {ms_per_step:.3f} ms/step, contraction {k_ms:.3f} ms, rescale tile {t_ms:.3f} ms")
+    log(f"bench: matmul 192x192 {ms_per_step:.3f} ms/step ({overlap_ms:.3f} overlapped), contraction "
+        f"{k_ms:.3f} ms, rescale tile {t_ms:.3f} ms")
     if not args.quick:
         t1 = time.perf_counter()
         shm.ct_pt_matrix_mul_wo_pre(hs, W, ROW_W, COL_W, ROW_W, ctx, variant=shm.MATMUL_PER_CT, out=out)

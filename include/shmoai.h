@@ -576,7 +576,10 @@ int shm_keyswitch_counters_reset(void);
 /* Launch-shape knobs (no counterpart in the reference; results never depend on them):
  * "ks_batch_slots_log2" and "ks_multi_slots_log2" set log2 of the member / output slots
  * per block of the batched and multi-output key inner products (0..4), or -1 (the
- * default) for the autotuned choice. Process-wide. */
+ * default) for the autotuned choice; "matmul_overlap" (0, the default, or 1) runs the INT8
+ * ct-pt matmul's contraction and rescale one after the other, or overlaps the contraction
+ * (row_W <= 256, in limb windows) with the rescale on a second stream of the context.
+ * Process-wide. */
 int shm_set_tuning(const char *name, int32_t value);
 int shm_get_tuning(const char *name, int32_t *value);
 /* Autotuner (no counterpart in the reference): the first call of each inner-product

@@ -1844,6 +1844,11 @@ int shm_set_tuning(const char *name, int32_t value) {
     return guard([&] {
         if (!name) throw InvalidArgument("tuning knob name is null");
         const std::string n(name);
+        if (n == "matmul_overlap") {
+            if (value != 0 && value != 1) throw InvalidArgument("matmul_overlap: 0 or 1");
+            ks_tuning().matmul_overlap = value;
+            return;
+        }
         if (value < -1 || value > 4) throw InvalidArgument("tuning knob value: -1 (default) or 0..4");
         if (n == "ks_batch_slots_log2")
             ks_tuning().batch_slots_log2 = value;
@@ -1882,6 +1887,8 @@ int shm_get_tuning(const char *name, int32_t *value) {
             *value = ks_tuning().batch_slots_log2;
         else if (n == "ks_multi_slots_log2")
             *value = ks_tuning().multi_slots_log2;
+        else if (n == "matmul_overlap")
+            *value = ks_tuning().matmul_overlap;
         else
             throw InvalidArgument("unknown tuning knob: " + n);
     });

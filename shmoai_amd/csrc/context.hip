@@ -279,6 +279,19 @@ uint32_t Context::elt_from_step(int step) const {
     return (uint32_t)e;
 }
 
+Context::AuxStream::~AuxStream() {
+    if (s) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+    }
+}
+
+hipStream_t Context::aux_stream() {
+    std::lock_guard<std::mutex> lk(aux.mu);
+    if (!aux.s) SHM_HIP(hipStreamCreateWithFlags(&aux.s, hipStreamNonBlocking));
+    return aux.s;
+}
+
 const uint32_t *Context::galois_table(uint32_t elt, hipStream_t s) {
     std::lock_guard<std::mutex> lk(galois_mu);
     auto it = galois_tables.find(elt);

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -169,6 +170,15 @@ struct Context {
     const LevelConsts &level(size_t Ql) const { return *levels.at(Ql); }
     const uint32_t *galois_table(uint32_t elt, hipStream_t s);
     uint32_t elt_from_step(int step) const;
+
+    // A second stream of the context (non-blocking, default priority), created on
+    // first use, for work that overlaps the caller's stream inside one call.
+    struct AuxStream {
+        std::mutex mu;
+        hipStream_t s = nullptr;
+        ~AuxStream();
+    } aux;
+    hipStream_t aux_stream();
 };
 
 std::unique_ptr<Context> make_context(int logN, const std::vector<int> &bits, size_t nP);
@@ -249,6 +259,10 @@ void ks_counters_reset();
 struct KsTuning {
     int batch_slots_log2 = -1;
     int multi_slots_log2 = -1;
+    // the INT8 ct-pt matmul (row_W <= 256) overlaps its contraction with the rescale on
+    // the context's second stream (1) or runs them one after the other (0, the default:
+    // both kernels fill the GPU, and the overlapped form measured 7% slower)
+    int matmul_overlap = 0;
 };
 KsTuning &ks_tuning();
 // Autotuning of those slot counts on first use of each shape (on by default): switch
@@ -275,6 +289,10 @@ void ntt_inverse_qlp(const Context &c, uint64_t *data, size_t Ql, size_t lo, siz
 // NTT([last_p]_{q_k}); the block pass stores (acc[p][k] - that) * qinv_k to
 // outs[p / ppo] (or out_single) at ((p % ppo) * L1 + k) * N.
 // rounded: divide with rounding to nearest (centred last limb) instead of the floor.
+// rescale_ntt_fused over output limbs lo .. lo + n - 1 of L1 only (outputs keep L1 limbs per polynomial)
+void rescale_ntt_fused_window(const Context &c, const uint64_t *last, const uint64_t *acc, size_t acc_limbs,
+                              uint64_t *delta, uint64_t *const *outs, size_t ppo, size_t n_polys, size_t L1,
+                              size_t lo, size_t n, const uint64_t *qinv, const uint64_t *qinv_s, hipStream_t s);
 void rescale_ntt_fused(const Context &c, const uint64_t *last, const uint64_t *acc, size_t acc_limbs,
                        uint64_t *delta, uint64_t *const *outs, uint64_t *out_single, size_t ppo, size_t n_polys,
                        size_t L1, const uint64_t *qinv, const uint64_t *qinv_s, hipStream_t s, bool rounded = false);
@@ -427,6 +445,14 @@ void decode(const Context &c, const uint64_t *pt, size_t Ql, double scale, doubl
             hipStream_t s);
 
 // MOAI matmul (matmul.hip)
+// Limb windows of the INT8 contraction: the last limb is contracted first and
+// last_done() runs right after it is queued, then the other limbs in windows of
+// `window`, window_done(lo, n) after each (both on the caller's thread, in stream order).
+struct Int8WindowHook {
+    size_t window = 8;
+    std::function<void()> last_done;
+    std::function<void(size_t, size_t)> window_done;
+};
 void ctpt_matmul(const Context &c, const std::vector<const Ciphertext *> &X, const double *W, size_t row_w,
                  size_t col_w, const std::vector<Ciphertext *> &out, int variant, hipStream_t s);
 float last_matmul_rescale_tile_ms(); // tile pass of the last batched matmul's fused rescale NTT

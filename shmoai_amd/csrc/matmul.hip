@@ -206,7 +206,8 @@ void batched_rescale(const Context &c, const uint64_t *acc, const std::vector<Ci
 }  // namespace
 
 void ctpt_matmul_int8(const Context &c, const std::vector<const Ciphertext *> &X, const double *W, size_t row_w,
-                      size_t col_w, uint64_t *acc, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1);
+                      size_t col_w, uint64_t *acc, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1,
+                      const Int8WindowHook *hook = nullptr);
 bool int8_supported(const Context &c, size_t Ql, const double *W, const double *scales, size_t row_w, size_t col_w,
                     int *dx_out, int *dw_out);
 
@@ -307,7 +308,76 @@ void ctpt_matmul(const Context &c, const std::vector<const Ciphertext *> &X, con
         SHM_HIP(hipEventRecord(e1, s));
     } else {
         Scratch acc(col_w * 2 * S * sizeof(uint64_t), s);
-        if (variant == 2) {
+        if (variant == 2 && row_w <= 256 && ks_tuning().matmul_overlap) {
+            // Overlap: the last limb is contracted first and the context's second stream
+            // starts the rescale's correction from it (gather, inverse NTT); the other
+            // limbs are contracted in windows on s while the second stream rescales each
+            // finished window (the INT8 contraction is MFMA work, the rescale NTT VALU
+            // work). X is read once; the outputs equal the serial path's.
+            Context &cc = const_cast<Context &>(c);
+            const hipStream_t r = cc.aux_stream();
+            const size_t L1 = Ql - 1, nct = out.size(), np = 2 * nct;
+            std::vector<uint64_t *> ptrs(nct);
+            for (size_t i = 0; i < nct; i++) ptrs[i] = out[i]->data();
+            Scratch dptrs(nct * sizeof(uint64_t *), s);
+            SHM_HIP(hipMemcpyAsync(dptrs.as<void>(), ptrs.data(), nct * sizeof(uint64_t *), hipMemcpyHostToDevice, s));
+            Scratch last(np * N * sizeof(uint64_t), r);
+            Scratch delta(np * L1 * N * sizeof(uint64_t), r);
+            const LevelConsts &L = c.level(Ql);
+            std::vector<hipEvent_t> evs;
+            auto handoff = [&] {
+                hipEvent_t ev;
+                SHM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+                evs.push_back(ev);
+                SHM_HIP(hipEventRecord(ev, s));
+                SHM_HIP(hipStreamWaitEvent(r, ev, 0));
+            };
+            std::vector<std::pair<hipEvent_t, hipEvent_t>> tile_ev;
+            Int8WindowHook hook;
+            hook.window = (L1 + 2) / 3;
+            hook.last_done = [&] {
+                handoff();
+                k_gather_last<<<grid_for(np * N), kBlock, 0, r>>>(acc.as<uint64_t>(), last.as<uint64_t>(), Ql, N, np);
+                SHM_HIP(hipGetLastError());
+                ntt_inverse(c, last.as<uint64_t>(), np, 1, L1, r);
+            };
+            hook.window_done = [&](size_t lo, size_t n) {
+                handoff();
+                hipEvent_t t0, t1;
+                SHM_HIP(hipEventCreate(&t0));
+                SHM_HIP(hipEventCreate(&t1));
+                tile_ev.emplace_back(t0, t1);
+                set_rescale_tile_events(t0, t1);
+                rescale_ntt_fused_window(c, last.as<uint64_t>(), acc.as<uint64_t>(), Ql, delta.as<uint64_t>(),
+                                         dptrs.as<uint64_t *>(), 2, np, L1, lo, n, L.qlast_inv.as<uint64_t>(),
+                                         L.qlast_inv_s.as<uint64_t>(), r);
+                set_rescale_tile_events(nullptr, nullptr);
+            };
+            try {
+                ctpt_matmul_int8(c, X, W, row_w, col_w, acc.as<uint64_t>(), s, e0, e1, &hook);
+            } catch (...) {
+                set_rescale_tile_events(nullptr, nullptr);
+                (void)hipStreamSynchronize(r);
+                throw;
+            }
+            // s (which frees acc, and owns the outputs) continues after the rescale
+            hipEvent_t done;
+            SHM_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+            SHM_HIP(hipEventRecord(done, r));
+            SHM_HIP(hipStreamWaitEvent(s, done, 0));
+            SHM_HIP(hipEventSynchronize(done));
+            SHM_HIP(hipEventRecord(e1, s));  // the step: contraction and rescale together
+            g_last_tile_ms = 0.f;
+            for (auto &[t0, t1] : tile_ev) {
+                float ms = 0.f;
+                SHM_HIP(hipEventElapsedTime(&ms, t0, t1));
+                g_last_tile_ms += ms;
+                (void)hipEventDestroy(t0);
+                (void)hipEventDestroy(t1);
+            }
+            (void)hipEventDestroy(done);
+            for (hipEvent_t ev : evs) (void)hipEventDestroy(ev);
+        } else if (variant == 2) {
             ctpt_matmul_int8(c, X, W, row_w, col_w, acc.as<uint64_t>(), s, e0, e1);
         } else {
             // X rows staged in LDS per launch: at most 256 rows (64 KiB); longer
@@ -341,21 +411,23 @@ void ctpt_matmul(const Context &c, const std::vector<const Ciphertext *> &X, con
             SHM_HIP(hipEventRecord(e1, s));
             SHM_HIP(hipStreamSynchronize(s));
         }
-        hipEvent_t t0, t1;
-        SHM_HIP(hipEventCreate(&t0));
-        SHM_HIP(hipEventCreate(&t1));
-        set_rescale_tile_events(t0, t1);
-        try {
-            batched_rescale(c, acc.as<uint64_t>(), out, Ql, s);
-        } catch (...) {
+        if (!(variant == 2 && row_w <= 256 && ks_tuning().matmul_overlap)) {
+            hipEvent_t t0, t1;
+            SHM_HIP(hipEventCreate(&t0));
+            SHM_HIP(hipEventCreate(&t1));
+            set_rescale_tile_events(t0, t1);
+            try {
+                batched_rescale(c, acc.as<uint64_t>(), out, Ql, s);
+            } catch (...) {
+                set_rescale_tile_events(nullptr, nullptr);
+                throw;
+            }
             set_rescale_tile_events(nullptr, nullptr);
-            throw;
+            SHM_HIP(hipEventSynchronize(t1));
+            SHM_HIP(hipEventElapsedTime(&g_last_tile_ms, t0, t1));
+            SHM_HIP(hipEventDestroy(t0));
+            SHM_HIP(hipEventDestroy(t1));
         }
-        set_rescale_tile_events(nullptr, nullptr);
-        SHM_HIP(hipEventSynchronize(t1));
-        SHM_HIP(hipEventElapsedTime(&g_last_tile_ms, t0, t1));
-        SHM_HIP(hipEventDestroy(t0));
-        SHM_HIP(hipEventDestroy(t1));
     }
     SHM_HIP(hipEventSynchronize(e1));
     SHM_HIP(hipEventElapsedTime(&g_last_ms, e0, e1));

@@ -137,7 +137,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DX * KS <= 
                                                        const uint64_t *__restrict__ mu_arr,
                                                        uint64_t *__restrict__ acc_out,
                                                        const DevMod *__restrict__ mods, int row_w, int col_w,
-                                                       int MB, size_t Ql, size_t N, int accumulate, int radix) {
+                                                       int MB, size_t Ql, size_t N, int accumulate, int radix,
+                                                       int l_lo, int l_n) {
     constexpr int ND = DX + DW - 1;
     // The planes are recombined modulo 2^128, so S = sum_k x'_k w_k comes out exactly as
     // long as its true value satisfies |S| < 2^106 (x' the centred digits, |x'| < 2^(8 DX - 1)),
@@ -149,8 +150,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DX * KS <= 
     __shared__ uint64_t cl[RB * 16];  // row-sum corrections of this limb for the phase's rows
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const size_t tiles = N / 64;
-    const size_t tile = blockIdx.x % tiles, lp = blockIdx.x / tiles;  // lp = p * Ql + l
-    const size_t l = lp % Ql;
+    // the launch covers limbs l_lo .. l_lo + l_n - 1 of both polynomials
+    const size_t tile = blockIdx.x % tiles, lw = blockIdx.x / tiles;
+    const size_t l = (size_t)l_lo + lw % (size_t)l_n, lp = (lw / (size_t)l_n) * Ql + l;  // lp = p * Ql + l
     const size_t col = tile * 64 + 16 * wave + (lane & 15);
     const size_t off = lp * N + col;
     const uint64_t q = mods[l].q, mu = mu_arr[l];
@@ -264,7 +266,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DX * KS <= 
 }
 
 typedef void (*ContractFn)(const uint64_t *const *, const i8x16 *, const uint64_t *, const uint64_t *, uint64_t *,
-                           const DevMod *, int, int, int, size_t, size_t, int, int);
+                           const DevMod *, int, int, int, size_t, size_t, int, int, int, int);
 
 constexpr int rb_for(int KS) { return KS == 1 ? 4 : KS == 2 ? 4 : KS == 3 ? 3 : 2; }
 
@@ -328,7 +330,8 @@ bool int8_supported(const Context &c, size_t Ql, const double *W, const double *
 }
 
 void ctpt_matmul_int8(const Context &c, const std::vector<const Ciphertext *> &X, const double *W, size_t row_w,
-                      size_t col_w, uint64_t *acc, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+                      size_t col_w, uint64_t *acc, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1,
+                      const Int8WindowHook *hook) {
     const size_t Ql = X[0]->Ql(), N = c.N;
     std::vector<double> scales(row_w);
     for (size_t j = 0; j < row_w; j++) scales[j] = X[j]->scale;
@@ -338,6 +341,7 @@ void ctpt_matmul_int8(const Context &c, const std::vector<const Ciphertext *> &X
     // balanced row chunks of at most kChunkRows
     const size_t nch = (row_w + kChunkRows - 1) / kChunkRows;
     const size_t kc_max = (row_w + nch - 1) / nch;
+    if (hook && nch != 1) throw LogicError("limb windows need one row chunk");
     const int MB = (int)((col_w + 15) / 16);
     auto frag_count = [&](int KS) {
         const int RB = rb_for(KS), NPH = (MB + RB - 1) / RB;
@@ -354,7 +358,7 @@ void ctpt_matmul_int8(const Context &c, const std::vector<const Ciphertext *> &X
     SHM_HIP(hipMemcpyAsync(dW.as<void>(), W, row_w * col_w * sizeof(double), hipMemcpyHostToDevice, s));
     SHM_HIP(hipMemcpyAsync(dscale.as<void>(), scales.data(), row_w * sizeof(double), hipMemcpyHostToDevice, s));
     SHM_HIP(hipMemcpyAsync(xptr.as<void>(), xp.data(), row_w * sizeof(uint64_t *), hipMemcpyHostToDevice, s));
-    const unsigned blocks = (unsigned)(2 * Ql * (N / 64));
+    const size_t tiles = N / 64;
     size_t k0 = 0;
     for (size_t ch = 0; ch < nch; ch++) {
         // chunk rows [k0, k0 + kc): W rows are contiguous, so the chunk is a sub-matrix
@@ -368,11 +372,27 @@ void ctpt_matmul_int8(const Context &c, const std::vector<const Ciphertext *> &X
                                                               c.mods(), kc, col_w, Ql, DX);
         SHM_HIP(hipGetLastError());
         ContractFn fn = pick(DX, DW, KS);
+        auto launch = [&](size_t l_lo, size_t l_n) {
+            hipLaunchKernelGGL(fn, dim3((unsigned)(2 * l_n * tiles)), dim3(256), 0, s,
+                               xptr.as<const uint64_t *>() + k0, frag.as<i8x16>(), corr.as<uint64_t>(),
+                               mu.as<uint64_t>(), acc, c.mods(), (int)kc, (int)col_w, MB, Ql, N, ch > 0 ? 1 : 0, 256,
+                               (int)l_lo, (int)l_n);
+            SHM_HIP(hipGetLastError());
+        };
         if (ch == 0) SHM_HIP(hipEventRecord(ev0, s));
-        hipLaunchKernelGGL(fn, dim3(blocks), dim3(256), 0, s, xptr.as<const uint64_t *>() + k0, frag.as<i8x16>(),
-                           corr.as<uint64_t>(), mu.as<uint64_t>(), acc, c.mods(), (int)kc, (int)col_w, MB, Ql, N,
-                           ch > 0 ? 1 : 0, 256);
-        SHM_HIP(hipGetLastError());
+        if (hook) {
+            // the last limb first (the rescale's correction needs only it), then the
+            // other limbs in windows, each handed to the hook once it is on the stream
+            launch(Ql - 1, 1);
+            hook->last_done();
+            for (size_t w0 = 0; w0 < Ql - 1; w0 += hook->window) {
+                const size_t wn = std::min(hook->window, Ql - 1 - w0);
+                launch(w0, wn);
+                hook->window_done(w0, wn);
+            }
+        } else {
+            launch(0, Ql);
+        }
         k0 += kc;
     }
     SHM_HIP(hipEventRecord(ev1, s));

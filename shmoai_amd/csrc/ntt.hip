@@ -271,6 +271,7 @@ struct NttIO {
     uint32_t packed;          // IO_RESCALE: nonzero -> polynomial p reads acc_p[p], writes out_p[p]
     uint32_t out_packed;      // IO_RESCALE: nonzero -> polynomial p reads acc, writes out_p[p]
     uint64_t acc_mask;        // IO_RESCALE with out_packed: bit p set -> out_p[p] += result
+    uint32_t out_limbs;       // IO_RESCALE with outs: limbs per output polynomial (0: the launch's)
     PolyPack acc_p, out_p;
     const double2 *fft;       // IO_ENCODE: [n_polys][N/2]
 };
@@ -403,7 +404,7 @@ __global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ 
             out = io.out_p.p[poly] + (size_t)k * N + t0;
         } else {
             uint64_t *ob = io.outs ? io.outs[poly / io.ppo] : io.out_single;
-            out = ob + ((size_t)(poly % io.ppo) * mp.n_active + k) * N + t0;
+            out = ob + ((size_t)(poly % io.ppo) * (io.out_limbs ? io.out_limbs : mp.n_active) + k) * N + t0;
         }
         const uint64_t qi = io.qinv[prime], qis = io.qinv_s[prime];
         // unreduced outputs stay below (4 + 2 logN) q (< 2^64 for q < 2^58): acc - v is
@@ -671,6 +672,23 @@ void rescale_ntt_fused(const Context &c, const uint64_t *last, const uint64_t *a
     io.ppo = (uint32_t)ppo;
     io.center = rounded ? (c.q[L1] >> 1) : 0;
     launch_ntt<true, IO_REDUCE, IO_RESCALE>(c, delta, mp, n_polys * L1, s, io);
+}
+
+void rescale_ntt_fused_window(const Context &c, const uint64_t *last, const uint64_t *acc, size_t acc_limbs,
+                              uint64_t *delta, uint64_t *const *outs, size_t ppo, size_t n_polys, size_t L1,
+                              size_t lo, size_t n, const uint64_t *qinv, const uint64_t *qinv_s, hipStream_t s) {
+    if (lo + n > L1 || n == 0) throw InvalidArgument("rescale window outside the output limbs");
+    LimbMap mp{(uint32_t)n, (uint32_t)lo, 0xffffffffu, 0, 0xffffffffu, (uint32_t)c.nQ, 0, (uint64_t)L1 * c.N};
+    NttIO io{};
+    io.src = last;
+    io.acc = acc;
+    io.outs = outs;
+    io.qinv = qinv;
+    io.qinv_s = qinv_s;
+    io.acc_limbs = (uint32_t)acc_limbs;
+    io.ppo = (uint32_t)ppo;
+    io.out_limbs = (uint32_t)L1;
+    launch_ntt<true, IO_REDUCE, IO_RESCALE>(c, delta, mp, n_polys * n, s, io);
 }
 
 void moddown_ntt_fused(const Context &c, uint64_t *delta, const uint64_t *cx, size_t Ql, size_t cx_limbs,

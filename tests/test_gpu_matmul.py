@@ -182,3 +182,28 @@ def test_matmul_masked_rejects_short_mask():
     hs = [shm.Ciphertext(ctx, 2, 1) for _ in range(2)]
     with pytest.raises(shm.capi.ShmInvalidArgument):
         shm.ct_pt_matrix_mul_wo_pre_w_mask(hs, np.zeros((2, 2)), np.ones(10, np.int32), 2, 2, 2, ctx)
+
+
+def test_matmul_overlap_equals_serial(oracle_lib):
+    """The INT8 matmul with its contraction in limb windows overlapped with the rescale on
+    the context's second stream gives the limbs of the serial form (N = 2^13, 6 limbs,
+    40 x 12 weights); both equal the oracle."""
+    bits = [50] * 6 + [60]
+    ctx = shm.Context(13, bits, 1)
+    ref = oracle_lib.Ref(13, bits, 1)
+    rng = np.random.default_rng(17)
+    row_w, col_w, Ql = 40, 12, 6
+    xs, hs = _rand_cts(ctx, ref, rng, row_w, Ql)
+    W = np.random.default_rng(18).normal(0, 0.02, (row_w, col_w))
+    try:
+        shm.set_tuning("matmul_overlap", 0)
+        serial = [o.numpy() for o in shm.ct_pt_matrix_mul_wo_pre(hs, W, row_w, col_w, row_w, ctx)]
+        shm.set_tuning("matmul_overlap", 1)
+        over = [o.numpy() for o in shm.ct_pt_matrix_mul_wo_pre(hs, W, row_w, col_w, row_w, ctx)]
+    finally:
+        shm.set_tuning("matmul_overlap", 0)
+    assert shm.last_matmul_variant() == shm.MATMUL_INT8
+    exp = ref.ctpt_matmul(xs, W, Ql, np.full(row_w, 2.0 ** 40))
+    for i in range(col_w):
+        assert np.array_equal(serial[i], over[i]), f"output {i}"
+        assert np.array_equal(over[i], exp[i]), f"output {i}"
