@@ -37,6 +37,18 @@ __device__ __forceinline__ uint64_t mul_shoup_lazy(uint64_t x, uint64_t w, uint6
     return x * w - __umul64hi(x, ws) * q;
 }
 
+// x w mod q in [0, 4q) for any x < 2^64: the quotient estimate keeps x_hi ws_hi and the
+// high halves of the two middle partial products and drops the lowest one, so it is at
+// most 2 below Shoup's floor(x ws / 2^64) (two 32-bit high multiplies and one
+// 32 x 32 + 64 multiply-add instead of a full 64 x 64 high product).
+__device__ __forceinline__ uint64_t mul_shoup_lazy4(uint64_t x, uint64_t w, uint64_t ws, uint64_t q) {
+    const uint32_t xl = (uint32_t)x, xh = (uint32_t)(x >> 32);
+    const uint32_t sl = (uint32_t)ws, sh = (uint32_t)(ws >> 32);
+    const uint64_t mid = (uint64_t)__umulhi(xl, sh) + (uint64_t)__umulhi(xh, sl);
+    const uint64_t qe = (uint64_t)xh * sh + mid;
+    return x * w - qe * q;
+}
+
 __device__ __forceinline__ uint64_t mul_shoup(uint64_t x, uint64_t w, uint64_t ws, uint64_t q) {
     return csub(mul_shoup_lazy(x, w, ws, q), q);
 }

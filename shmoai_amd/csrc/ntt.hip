@@ -17,8 +17,9 @@
 //
 // Lazy ranges (Harvey): forward keeps [0, 4q) between stages, inverse [0, 2q);
 // primes are < 2^61 so nothing wraps. When every prime of a launch is below
-// 2^58 the forward stages skip the per-butterfly reduction of a altogether
-// (values stay below 36q). Outputs are canonical residues.
+// 2^64 / (5 + 4 log N) (about 2^57.9) the forward stages skip the per-butterfly
+// reduction of a altogether and take a cheaper quotient in the twiddle product
+// (values stay below (4 + 4 log N) q). Outputs are canonical residues.
 #include <algorithm>
 
 #include <mutex>
@@ -87,10 +88,11 @@ __device__ __forceinline__ void ct_unit(uint64_t *x, uint32_t G, const uint64_t 
     }
 }
 
-// Forward butterflies that leave a unreduced: with t = x w mod q in [0, 2q),
-// both outputs stay below a + 2q, so 16 stages from inputs below 4q end below
-// 36q, which fits 64 bits for q < 2^58. Saves the compare-select-subtract of
-// every butterfly; the final pass reduces once.
+// Forward butterflies that leave a unreduced: with t = x w mod q in [0, 4q) (the
+// quotient estimate of mul_shoup_lazy4, at most 2 short), both outputs stay below
+// a + 4q, so log N stages from inputs below 4q end below (4 + 4 log N) q, which fits
+// 64 bits for the primes the launcher allows here. Saves the compare-select-subtract
+// of every butterfly and part of the quotient's multiplies; the final pass reduces once.
 struct IntKLazy {
     uint64_t q;
 };
@@ -98,7 +100,7 @@ struct IntKLazy {
 template <int R>
 __device__ __forceinline__ void ct_unit(uint64_t *x, uint32_t G, const uint64_t *__restrict__ tw,
                                         const uint64_t *__restrict__ tws, IntKLazy k) {
-    const uint64_t q = k.q, q2 = q << 1;
+    const uint64_t q = k.q, q4 = q << 2;
 #pragma unroll
     for (int u = 0; u < R; u++) {
         const int h = 1 << (R - 1 - u);
@@ -107,9 +109,9 @@ __device__ __forceinline__ void ct_unit(uint64_t *x, uint32_t G, const uint64_t 
             if ((l & h) == 0) {
                 const uint32_t ti = (G << u) + (uint32_t)(l >> (R - u));
                 const uint64_t a = x[l];
-                const uint64_t t = mul_shoup_lazy(x[l + h], tw[ti], tws[ti], q);
+                const uint64_t t = mul_shoup_lazy4(x[l + h], tw[ti], tws[ti], q);
                 x[l] = a + t;
-                x[l + h] = a + q2 - t;
+                x[l + h] = a + q4 - t;
             }
         }
     }
@@ -392,7 +394,7 @@ __global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ 
     else
         run_rounds<LOGT, FWD>(lds, N1, 1u, (uint32_t)(t0 / T), tb.tw + (size_t)prime * N,
                               tb.tws + (size_t)prime * N, IntK{q}, load_acc);
-    // canonical residue of a forward output: below 4q, or below 36q when lazy
+    // canonical residue of a forward output: below 4q, or below (4 + 4 log N) q when lazy
     auto canon = [&](uint64_t v) -> uint64_t {
         if (LAZY) return reduce64(v, tb.mods[prime]);
         v = v >= q2 ? v - q2 : v;
@@ -407,9 +409,9 @@ __global__ __launch_bounds__(kThreads) void k_ntt_blocks(uint64_t *__restrict__ 
             out = ob + ((size_t)(poly % io.ppo) * (io.out_limbs ? io.out_limbs : mp.n_active) + k) * N + t0;
         }
         const uint64_t qi = io.qinv[prime], qis = io.qinv_s[prime];
-        // unreduced outputs stay below (4 + 2 logN) q (< 2^64 for q < 2^58): acc - v is
-        // taken as acc + (4 + 2 logN) q - v, which the Shoup product accepts as it is
-        const uint64_t kq = (uint64_t)(4 + 2 * logN) * q;
+        // unreduced outputs stay below (4 + 4 logN) q (< 2^64 for the lazy primes): acc - v
+        // is taken as acc + (4 + 4 logN) q - v, which the Shoup product accepts as it is
+        const uint64_t kq = (uint64_t)(4 + 4 * logN) * q;
         const bool addto = io.out_packed && poly < 64 && ((io.acc_mask >> poly) & 1);
 #pragma unroll
         for (int j = 0; j < PER; j++) {
@@ -474,13 +476,15 @@ void launch_ntt(const Context &c, uint64_t *data, const LimbMap &mp, size_t tota
                  FWD ? c.tw_s.as<uint64_t>() : c.itw_s.as<uint64_t>(),
                  c.ninv.as<uint64_t>(),
                  c.ninv_s.as<uint64_t>()};
-    // unreduced forward butterflies when every prime of the launch is below 2^58
+    // unreduced forward butterflies when (5 + 4 log N) q < 2^64 for every prime of the
+    // launch: values below (4 + 4 log N) q, plus one q of the fused epilogue's operand
     bool lazy = FWD;
+    const uint64_t lazy_max = ~uint64_t(0) / (uint64_t)(5 + 4 * c.logN);
     for (uint32_t j = 0; j < mp.n_active && lazy; j++) {
         uint32_t k = mp.lo + j;
         if (k >= mp.skip_lo) k += mp.skip_width;
         const uint32_t prime = k < mp.Ql ? mp.first_prime + k : mp.nQ + (k - mp.Ql);
-        lazy = c.q.at(prime) < (uint64_t(1) << 58);
+        lazy = c.q.at(prime) < lazy_max;
     }
 #define SHM_COLS(L)                                                                                    \
     case L:                                                                                            \
