@@ -82,6 +82,36 @@ __global__ void k_const_lincomb(uint64_t *acc, SrcPack sp, int nterms, const Dev
     }
 }
 
+// k_const_lincomb over up to 8 members sharing the constants and source levels:
+// row r is (member r / (2 Ql), polynomial (r / Ql) % 2, limb r % Ql).
+constexpr int kLeafMembers = 8, kLeafTerms = 8;
+struct LeafPack {
+    const uint64_t *x[kLeafMembers][kLeafTerms];
+    uint64_t *acc[kLeafMembers];
+    uint32_t ql[kLeafTerms];
+    double v[kLeafTerms];
+};
+
+__global__ void k_const_lincomb_batch(LeafPack lp, int nterms, int members, const DevMod *mods, size_t Ql, size_t N) {
+    __shared__ uint64_t cres[kLeafTerms];
+    for (size_t row = blockIdx.y; row < (size_t)members * 2 * Ql; row += gridDim.y) {
+        const size_t mb = row / (2 * Ql), p = (row / Ql) % 2, l = row % Ql;
+        const DevMod m = mods[l];
+        __syncthreads();  // the previous row's readers are done with cres
+        if (threadIdx.x < (unsigned)nterms) {
+            const uint64_t c = residue_of_rounded(lp.v[threadIdx.x], m);
+            cres[threadIdx.x] = c >= m.q ? c - m.q : c;
+        }
+        __syncthreads();
+        uint64_t *out = lp.acc[mb] + (p * Ql + l) * N;
+        for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
+            Acc128 a{0, 0};
+            for (int k = 0; k < nterms; k++) mac128(a, lp.x[mb][k][(p * lp.ql[k] + l) * N + i], cres[k]);
+            out[i] = reduce128(a, m);
+        }
+    }
+}
+
 void add_checked(Context &c, Ciphertext &a, const Ciphertext &b, hipStream_t s) {
     if (std::fabs(a.scale / b.scale - 1.0) > 1e-9) throw LogicError("scale mismatch");
     ct_add(c, a, b, s);
@@ -431,8 +461,7 @@ struct ChebEvalBatch {
         if (a != b) term = cptrs(get(a - b));
         Group out = fresh();
         mul_relin_rescale_batch(c, ptrs(out), cptrs(ta), cptrs(tb), rk, s, true, &term);
-        if (a == b)
-            for (auto &o : out) ct_add_const(c, *o, -1.0, s);
+        if (a == b) ct_add_const_batch(c, ptrs(out), -1.0, s);
         if (out[0]->chain != tchain(k)) throw LogicError("Chebyshev basis level bookkeeping");
         T[k] = std::move(out);
         return T[k];
@@ -458,26 +487,43 @@ struct ChebEvalBatch {
         const double ql = (double)c.q[Qc - 1];
         for (size_t k = 1; k < p.size(); k++)
             if (p[k] != 0.0) get((int)k);
-        for (size_t i = 0; i < m; i++) {
-            std::vector<const Ciphertext *> src;
-            std::vector<double> vals, sig;
-            for (size_t k = 1; k < p.size(); k++) {
-                if (p[k] == 0.0) continue;
-                const Ciphertext &t = *T[k][i];
-                if (t.chain > chain - 1) throw LogicError("Chebyshev basis element below the combination level");
-                src.push_back(&t);
-                vals.push_back(p[k]);
-                sig.push_back(scale * ql / t.scale);
+        // one launch per 8 members: the terms' constants and levels are the group's
+        std::vector<int> ks;
+        LeafPack lp{};
+        for (size_t k = 1; k < p.size(); k++) {
+            if (p[k] == 0.0) continue;
+            const Ciphertext &t = *T[k][0];
+            if (t.chain > chain - 1) throw LogicError("Chebyshev basis element below the combination level");
+            if (ks.size() == (size_t)kLeafTerms) throw LogicError("Chebyshev leaf with more than 8 terms");
+            lp.ql[ks.size()] = (uint32_t)t.Ql();
+            lp.v[ks.size()] = std::round(p[k] * (scale * ql / t.scale));
+            if (!(std::fabs(lp.v[ks.size()]) < 3.4e38)) throw InvalidArgument("constant out of range");
+            ks.push_back((int)k);
+        }
+        if (ks.empty()) throw LogicError("constant-only Chebyshev block");
+        for (size_t i0 = 0; i0 < m; i0 += kLeafMembers) {
+            const size_t mm = std::min<size_t>(kLeafMembers, m - i0);
+            for (size_t i = 0; i < mm; i++) {
+                Ciphertext &o = *out[i0 + i];
+                o.ctx = &c;
+                o.resize(2, c.nQ - Qc + 1, s, false);  // every word is written
+                o.ntt = true;
+                o.scale = scale * ql;
+                lp.acc[i] = o.data();
+                for (size_t j = 0; j < ks.size(); j++) {
+                    const Ciphertext &t = *T[ks[j]][i0 + i];
+                    if (t.chain != T[ks[j]][0]->chain || t.size != 2 || t.scale != T[ks[j]][0]->scale)
+                        throw LogicError("Chebyshev basis members at different levels or scales");
+                    lp.x[i][j] = t.data();
+                }
             }
-            if (src.empty()) throw LogicError("constant-only Chebyshev block");
-            const_lincomb(c, *out[i], src, vals, sig, Qc, s);
-            out[i]->scale = scale * ql;
+            k_const_lincomb_batch<<<rows_grid(2 * Qc * mm, c.N), kBlock, 0, s>>>(lp, (int)ks.size(), (int)mm, c.mods(),
+                                                                                Qc, c.N);
+            SHM_HIP(hipGetLastError());
         }
         rescale_batch(c, ptrs(out), s, true);
-        for (auto &o : out) {
-            o->scale = scale;
-            if (p[0] != 0.0) ct_add_const(c, *o, p[0], s);
-        }
+        for (auto &o : out) o->scale = scale;
+        if (p[0] != 0.0) ct_add_const_batch(c, ptrs(out), p[0], s);
     }
 
     void eval(const std::vector<double> &p, double target, const Group &out) {
@@ -499,8 +545,7 @@ struct ChebEvalBatch {
         bool q_terms = false;
         for (size_t k = 1; k < q.size(); k++) q_terms |= q[k] != 0.0;
         if (!q_terms) {
-            if (q[0] != 0.0)
-                for (auto &o : out) ct_add_const(c, *o, q[0], s);
+            if (q[0] != 0.0) ct_add_const_batch(c, ptrs(out), q[0], s);
             return;
         }
         Group Q = fresh();
@@ -513,7 +558,9 @@ struct ChebEvalBatch {
                 if (x->chain < out[0]->chain) low.push_back(x.get());
             align(low, out[0]->chain, target);
         }
-        for (size_t i = 0; i < m; i++) add_checked(c, *out[i], *Q[i], s);
+        for (size_t i = 0; i < m; i++)
+            if (std::fabs(out[i]->scale / Q[i]->scale - 1.0) > 1e-9) throw LogicError("scale mismatch");
+        ct_add_batch(c, ptrs(out), cptrs(Q), s);
     }
 };
 
@@ -537,8 +584,8 @@ void eval_mod_batch(Bootstrapper &b, const std::vector<Ciphertext *> &xs, const 
         ct_clone(*t1[i], *xs[i], s);
     }
     const double s1 = xs[0]->scale;
-    for (auto &t : t1)
-        ct_mul_const(c, *t, 1.0 / (double)b.prm.K, s1 * (double)c.q[t->Ql() - 1] / t->scale, s);
+    ct_mul_const_batch(c, ChebEvalBatch::ptrs(t1), 1.0 / (double)b.prm.K,
+                       s1 * (double)c.q[t1[0]->Ql() - 1] / t1[0]->scale, s);
     rescale_batch(c, ChebEvalBatch::ptrs(t1), s, true);
     for (auto &t : t1) t->scale = s1;
     ChebEvalBatch ev(c, rk, s, std::move(t1));
@@ -561,10 +608,8 @@ void eval_mod_batch(Bootstrapper &b, const std::vector<Ciphertext *> &xs, const 
         mul_relin_rescale_batch(c, ChebEvalBatch::ptrs(sq), ChebEvalBatch::cptrs(out), ChebEvalBatch::cptrs(out), rk,
                                 s, true);
         const double cst = j == r - 1 ? out_const : 1.0;
-        for (auto &x : sq) {
-            x->scale = sig[j + 1];  // = sig[j]^2 / (2 cst q_j): the square read as 2 cst c^2
-            ct_add_const(c, *x, -cst, s);
-        }
+        for (auto &x : sq) x->scale = sig[j + 1];  // = sig[j]^2 / (2 cst q_j): the square read as 2 cst c^2
+        ct_add_const_batch(c, ChebEvalBatch::ptrs(sq), -cst, s);
         out = std::move(sq);
     }
     for (size_t i = 0; i < m; i++) *xs[i] = std::move(*out[i]);
@@ -591,13 +636,12 @@ void chebyshev_eval_batch(Context &c, const std::vector<Ciphertext *> &xs, const
     // back onto the input scale, then the constant term
     const double s1 = xs[0]->scale;
     if (c.ql_of_chain(xs[0]->chain) < 2) throw LogicError("end of modulus switching chain reached");
-    for (auto &t : t1) ct_mul_const(c, *t, 2.0 / (hi - lo), s1 * (double)c.q[t->Ql() - 1] / t->scale, s);
+    ct_mul_const_batch(c, ChebEvalBatch::ptrs(t1), 2.0 / (hi - lo), s1 * (double)c.q[t1[0]->Ql() - 1] / t1[0]->scale,
+                       s);
     rescale_batch(c, ChebEvalBatch::ptrs(t1), s, true);
-    for (auto &t : t1) {
-        t->scale = s1;
-        const double shift = -(lo + hi) / (hi - lo);
-        if (shift != 0.0) ct_add_const(c, *t, shift, s);
-    }
+    for (auto &t : t1) t->scale = s1;
+    const double shift = -(lo + hi) / (hi - lo);
+    if (shift != 0.0) ct_add_const_batch(c, ChebEvalBatch::ptrs(t1), shift, s);
     ChebEvalBatch ev(c, rk, s, std::move(t1));
     const size_t cp = ev.eval_chain(coeffs);
     if (cp > c.nQ) throw LogicError("end of modulus switching chain reached");

@@ -343,6 +343,10 @@ void launch_gather_polys(const Context &c, const PolyPack &in, uint64_t *out, si
 // poly 0 (+)= [rounded]_q per limb; npolys polynomials (*)= [rounded]_q per limb
 // (rounded = round(value * scale), as encode_scalar computes it)
 void launch_add_scalar(const Context &c, uint64_t *a, double rounded, size_t Ql, hipStream_t s);
+// Batched elementwise ops over separate polynomials of Ql limbs: op 0 adds the
+// rounded constant, op 1 multiplies by it, op 2 adds b.p[i] onto a.p[i].
+void launch_pack_op(const Context &c, const PolyPack &a, const PolyPack &b, size_t npolys, double rounded, size_t Ql,
+                    int op, hipStream_t s);
 void launch_mul_scalar(const Context &c, uint64_t *a, double rounded, size_t Ql, size_t npolys, hipStream_t s);
 // zero fill by a kernel (a memset node recorded into a captured graph is avoided)
 void launch_fill_zero(uint64_t *out, size_t words, hipStream_t s);
@@ -513,6 +517,13 @@ void ct_clone(Ciphertext &dst, const Ciphertext &src, hipStream_t s);
 void ct_add(Context &c, Ciphertext &a, const Ciphertext &b, hipStream_t s);
 void ct_sub(Context &c, Ciphertext &a, const Ciphertext &b, hipStream_t s);
 void ct_add_const(Context &c, Ciphertext &a, double v, hipStream_t s);
+// ct_add_const / ct_mul_const / ct_add over ciphertexts of one level (and one scale
+// for the constant forms) in one launch each; each member gets exactly the words of
+// the single-ciphertext operation
+void ct_add_const_batch(Context &c, const std::vector<Ciphertext *> &a, double v, hipStream_t s);
+void ct_mul_const_batch(Context &c, const std::vector<Ciphertext *> &a, double v, double sigma, hipStream_t s);
+void ct_add_batch(Context &c, const std::vector<Ciphertext *> &a, const std::vector<const Ciphertext *> &b,
+                  hipStream_t s);
 void ct_mul_const_rescale(Context &c, Ciphertext &a, double v, double sigma, hipStream_t s, bool rounded = false);
 void ct_align(Context &c, Ciphertext &a, size_t chain, double scale, hipStream_t s, bool rounded = false);
 void ct_mul_relin_rescale(Context &c, Ciphertext &out, const Ciphertext &a, const Ciphertext &b,

@@ -335,6 +335,58 @@ void ct_add_const(Context &c, Ciphertext &a, double v, hipStream_t s) {
     launch_add_scalar(c, a.data(), round(v * a.scale), a.Ql(), s);
 }
 
+void ct_add_const_batch(Context &c, const std::vector<Ciphertext *> &a, double v, hipStream_t s) {
+    for (size_t i0 = 0; i0 < a.size(); i0 += kPolyPack) {
+        const size_t m = std::min<size_t>(kPolyPack, a.size() - i0);
+        PolyPack pa{};
+        for (size_t i = 0; i < m; i++) {
+            const Ciphertext &x = *a[i0 + i];
+            if (!(x.scale > 0)) throw InvalidArgument("scale out of bounds");
+            if (x.chain != a[i0]->chain || x.scale != a[i0]->scale)
+                throw InvalidArgument("batched constant: ciphertexts at different levels or scales");
+            pa.p[i] = a[i0 + i]->data();  // polynomial 0
+        }
+        launch_pack_op(c, pa, PolyPack{}, m, round(v * a[i0]->scale), a[i0]->Ql(), 0, s);
+    }
+}
+
+void ct_mul_const_batch(Context &c, const std::vector<Ciphertext *> &a, double v, double sigma, hipStream_t s) {
+    if (!(sigma > 0)) throw InvalidArgument("scale out of bounds");
+    for (size_t i0 = 0; i0 < a.size(); i0 += kPolyPack / 2) {
+        const size_t m = std::min<size_t>(kPolyPack / 2, a.size() - i0);
+        PolyPack pa{};
+        for (size_t i = 0; i < m; i++) {
+            Ciphertext &x = *a[i0 + i];
+            if (x.size != 2 || x.chain != a[i0]->chain)
+                throw InvalidArgument("batched constant: size-2 ciphertexts at one level");
+            pa.p[2 * i] = x.poly(0);
+            pa.p[2 * i + 1] = x.poly(1);
+        }
+        launch_pack_op(c, pa, PolyPack{}, 2 * m, round(v * sigma), a[i0]->Ql(), 1, s);
+        for (size_t i = 0; i < m; i++) a[i0 + i]->scale *= sigma;
+    }
+}
+
+void ct_add_batch(Context &c, const std::vector<Ciphertext *> &a, const std::vector<const Ciphertext *> &b,
+                  hipStream_t s) {
+    if (a.size() != b.size()) throw InvalidArgument("ct_add_batch: operand counts differ");
+    for (size_t i0 = 0; i0 < a.size(); i0 += kPolyPack / 2) {
+        const size_t m = std::min<size_t>(kPolyPack / 2, a.size() - i0);
+        PolyPack pa{}, pb{};
+        for (size_t i = 0; i < m; i++) {
+            Ciphertext &x = *a[i0 + i];
+            const Ciphertext &y = *b[i0 + i];
+            if (x.chain != y.chain || x.size != 2 || y.size != 2 || x.chain != a[i0]->chain)
+                throw InvalidArgument("ct_add_batch: size-2 ciphertexts at one level");
+            pa.p[2 * i] = x.poly(0);
+            pa.p[2 * i + 1] = x.poly(1);
+            pb.p[2 * i] = y.poly(0);
+            pb.p[2 * i + 1] = y.poly(1);
+        }
+        launch_pack_op(c, pa, pb, 2 * m, 0.0, a[i0]->Ql(), 2, s);
+    }
+}
+
 // a <- a * [v encoded at scale sigma]; scale becomes a.scale * sigma.
 void ct_mul_const(Context &c, Ciphertext &a, double v, double sigma, hipStream_t s) {
     // multiply_plain by encode_scalar(v, sigma) without materialising the plaintext

@@ -214,6 +214,30 @@ __global__ void k_add_scalar(uint64_t *a, const DevMod *mods, double v, size_t N
     }
 }
 
+// The same over separate polynomials: row r is limb r % Ql of polynomial a.p[r / Ql];
+// op 0: a += [v]_q, op 1: a *= [v]_q, op 2: a += b (b.p[r / Ql]).
+__global__ void k_pack_op(PolyPack a, PolyPack b, const DevMod *mods, double v, size_t Ql, size_t N, size_t rows,
+                          int op) {
+    SHM_ROWS_LOOP(row, rows) {
+        const size_t p = row / Ql, k = row % Ql;
+        const DevMod m = mods[k];
+        uint64_t *ar = a.p[p] + k * N;
+        if (op == 2) {
+            const uint64_t *br = b.p[p] + k * N;
+            SHM_COLS_LOOP(i, N) ar[i] = add_mod(ar[i], br[i], m.q);
+            continue;
+        }
+        uint64_t r = residue_of_rounded(v, m);
+        r = r >= m.q ? r - m.q : r;
+        if (op == 0) {
+            SHM_COLS_LOOP(i, N) ar[i] = add_mod(ar[i], r, m.q);
+        } else {
+            const uint64_t rs = shoup_companion(r, m);
+            SHM_COLS_LOOP(i, N) ar[i] = mul_shoup(ar[i], r, rs, m.q);
+        }
+    }
+}
+
 __global__ void k_mul_scalar(uint64_t *a, const DevMod *mods, double v, size_t Ql, size_t N, size_t rows) {
     SHM_ROWS_LOOP(row, rows) {
         const DevMod m = mods[row % Ql];
@@ -485,6 +509,14 @@ void launch_copy_words(uint64_t *dst, const uint64_t *src, size_t words, hipStre
 
 void launch_add_scalar(const Context &c, uint64_t *a, double rounded, size_t Ql, hipStream_t s) {
     k_add_scalar<<<rows_grid(Ql, c.N), kBlock, 0, s>>>(a, c.mods(), rounded, c.N, Ql);
+    SHM_HIP(hipGetLastError());
+}
+
+void launch_pack_op(const Context &c, const PolyPack &a, const PolyPack &b, size_t npolys, double rounded, size_t Ql,
+                    int op, hipStream_t s) {
+    if (npolys == 0) return;
+    if (npolys > (size_t)kPolyPack) throw InvalidArgument("at most 64 polynomials per batched operation");
+    k_pack_op<<<rows_grid(npolys * Ql, c.N), kBlock, 0, s>>>(a, b, c.mods(), rounded, Ql, c.N, npolys * Ql, op);
     SHM_HIP(hipGetLastError());
 }
 
