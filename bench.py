@@ -20,8 +20,9 @@ over all 768 ciphertexts (96 chunks in one call), end to end (no extrapolation);
 the plain per-ciphertext form (768 sequential single-ciphertext calls, each waited for, the
 reference's loop), ordinary groups of 8, shared-a n = 4 and shared-a n = 8, with decryption
 errors on a sample of outputs. `roofline` is the dominant kernel of a
-shared-a chunk, the fused last NTT pass (ModDown combine and rescale), from HIP events
-around each of its launches in one eager chunk. `cpu_baseline` is the oracle's key-switch
+shared-a chunk, whichever NTT tile pass takes more time per chunk (the plain forward pass
+or the fused last pass with the ModDown combine and rescale; the other one is
+`roofline_other_kernel`), from HIP events around each of its launches in one eager chunk. `cpu_baseline` is the oracle's key-switch
 primitives at the same parameters, weighted by the key-switch work the GPU path counts per
 chunk, extrapolated to 768 (a lower bound: transforms outside key switching and the
 elementwise work are not counted).
@@ -59,6 +60,12 @@ REF_SHARED_A_N8_S = 193.3      # reference, 768 cts shared-a n = 8, RTX PRO 6000
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "contract_int8_192x192_N65536_L24_pmc.txt")
 BOOT_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "ntt_tile_passes_shared_a_n8_c3_pmc.txt")
+# the NTT tile passes as the profiles name them: FP64 butterflies (every prime below
+# 2^51), 512-thread blocks; plain forward pass and the fused rescale / ModDown pass
+PLAIN_TILE_KERNEL = "k_ntt_blocks<12,true,0,2,512>"
+RESCALE_TILE_KERNEL = "k_ntt_blocks<12,true,2,2,512>"
+TILE_NOTE = ("12 stages of exact FP64 butterflies through LDS per 4096-word tile; bound by issue of the "
+             "butterflies and twiddle loads, not by HBM (the same tile's load and store alone run near peak)")
 
 
 def measured_traffic(kernel: str, path: str = TRAFFIC_PROFILE):
@@ -129,9 +136,6 @@ def cpu_baseline_matmul(xs, W, threads: int):
                       f"(N=2^16, L=24, incl. rescale) on {threads} threads = {dt:.2f} s, "
                       f"extrapolated x{COL_W / done:.2f}",
             "seconds_per_full_matmul": per_matmul}
-
-
-BOOT_BITS = [51] + [46] * 20 + [51] * 14 + [51] * 5
 
 
 def ntt_bench(shm, ctx, iters: int = 10):
@@ -543,7 +547,7 @@ def matmul_bench(shm, args, threads):
     macs = ROW_W * COL_W * 2 * L * N
     tops = 2 * 35 * macs / (k_ms * 1e-3) / 1e12
     traffic_c = measured_traffic("k_contract_int8")
-    traffic_t = measured_traffic("k_ntt_blocks<12,true,2,true>")
+    traffic_t = measured_traffic(RESCALE_TILE_KERNEL)
     res = {"workload": "quarter-width ct-pt matmul 192x192 (MOAI column packing), N=2^16, L=24 x 50-bit primes, "
                        "256 inputs x 128 tokens per ciphertext, incl. rescale",
            "variant": args.variant, "last_variant": shm.last_matmul_variant(), "steps": steps,
@@ -559,7 +563,7 @@ def matmul_bench(shm, args, threads):
                            "achieved": round(tops, 1), "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
                            "frac": round(tops / INT8_PEAK_TOPS, 4), "traffic": traffic_c,
                            "kernel_bytes_per_launch": contraction_bytes},
-           "rescale_tile_pass": {"kernel": "k_ntt_blocks<12,true,2,true>", "kernel_ms": round(t_ms, 4),
+           "rescale_tile_pass": {"kernel": RESCALE_TILE_KERNEL, "kernel_ms": round(t_ms, 4),
                                  "traffic": traffic_t}}
     if traffic_c and traffic_t:
         res["traffic_over_algorithmic_step"] = round((traffic_c + traffic_t) / alg_step, 3)
@@ -592,6 +596,8 @@ def main() -> int:
                     help="headline only: the shared-a n=8 form (steps and one 768 pass)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-matmul", action="store_true", help="skip the ct-pt matmul objects")
+    ap.add_argument("--matmul-only", action="store_true",
+                    help="only the ct-pt matmul objects (profiling); prints their JSON, not the headline line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one device-to-device copy of a ciphertext (profiler byte-counter calibration)")
     args = ap.parse_args()
@@ -603,6 +609,9 @@ def main() -> int:
 
     import shmoai_amd as shm
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    if args.matmul_only:
+        print(json.dumps({"matmul_192": matmul_bench(shm, args, threads)}), flush=True)
+        return 0
     cpu_info = host_cpu()
     t_setup = time.perf_counter()
     ctx, sk, boot, steps, gk, rk = boot_setup(shm, t_setup)
@@ -660,11 +669,10 @@ def main() -> int:
             r["traffic_over_algorithmic"] = round(traffic / alg, 3)
         return r
 
-    fused = tile_roofline(pr, "k_ntt_blocks<12,true,2,true>", "fused last NTT pass: ModDown combine and rescale",
-                          "VALU-bound on 64-bit modular butterflies; quoted against HBM")
-    plain = tile_roofline(pp, "k_ntt_blocks<12,true,0,true>",
-                          "forward NTT tile pass: ModUp digits and other forward transforms",
-                          "VALU-bound on 64-bit modular butterflies; quoted against HBM")
+    fused = tile_roofline(pr, RESCALE_TILE_KERNEL, "fused last NTT pass: ModDown combine and rescale",
+                          TILE_NOTE)
+    plain = tile_roofline(pp, PLAIN_TILE_KERNEL, "forward NTT tile pass: ModUp digits and other forward transforms",
+                          TILE_NOTE)
     # the dominant kernel of the chunk (most time) is the roofline object
     roofline, roofline_other = (plain, fused) if pp["ms"] >= pr["ms"] else (fused, plain)
     avg_ms, gbs = fused["kernel_ms_avg"], fused["achieved"]
@@ -773,7 +781,8 @@ def main() -> int:
         "vs_baseline": round(full_s / REF_SHARED_A_N8_S, 4),
         "vs_baseline_note": "value / 193.3 s, the reference's 768-ct shared-a n=8 figure on an RTX PRO 6000 "
                             "(README.md:32-34); below 1 is faster",
-        "dtype": "u64 (exact RNS limbs; FP64 only in the encoder)",
+        "dtype": "u64 (exact RNS limbs; the NTT butterflies run on integral FP64 values, exact for primes "
+                 "below 2^51)",
         "data": "synthetic (slots U(-1,1), random-init keys)",
         "config": {"workload": "768-ciphertext shared-a S2C-first bootstrap (one BERT-base layer's bootstrapping "
                                "segment), C3: N=2^16, Q=1x51+20x46+14x51, P=5x51, HW 192, scale 2^46; "

@@ -517,13 +517,13 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 8 : 1) void k_ntt_blocks(uint64_t *
     __syncthreads();
     if constexpr (F64)
         run_rounds<LOGT, FWD, NT>(lds, N1, 1u, (uint32_t)(t0 / T), tb.twd + (size_t)prime * N,
-                              tb.twq + (size_t)prime * N, F64K{qd, qi});
+                                  tb.twq + (size_t)prime * N, F64K{qd, qi});
     else if (AR == AR_LAZY)
         run_rounds<LOGT, FWD, NT>(lds, N1, 1u, (uint32_t)(t0 / T), tb.tw + (size_t)prime * N,
-                              tb.tws + (size_t)prime * N, IntKLazy{q}, load_acc);
+                                  tb.tws + (size_t)prime * N, IntKLazy{q}, load_acc);
     else
         run_rounds<LOGT, FWD, NT>(lds, N1, 1u, (uint32_t)(t0 / T), tb.tw + (size_t)prime * N,
-                              tb.tws + (size_t)prime * N, IntK{q}, load_acc);
+                                  tb.tws + (size_t)prime * N, IntK{q}, load_acc);
     // canonical residue of a forward output: below 4q, or below (4 + 4 log N) q when
     // lazy, or a double within 2.01 q
     auto canon = [&](E v) -> uint64_t {
