@@ -60,12 +60,13 @@ REF_SHARED_A_N8_S = 193.3      # reference, 768 cts shared-a n = 8, RTX PRO 6000
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "contract_int8_192x192_N65536_L24_pmc.txt")
 BOOT_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "ntt_tile_passes_shared_a_n8_c3_pmc.txt")
-# the NTT tile passes as the profiles name them: FP64 butterflies (every prime below
-# 2^51), 512-thread blocks; plain forward pass and the fused rescale / ModDown pass
-PLAIN_TILE_KERNEL = "k_ntt_blocks<12,true,0,2,512>"
-RESCALE_TILE_KERNEL = "k_ntt_blocks<12,true,2,2,512>"
-TILE_NOTE = ("12 stages of exact FP64 butterflies through LDS per 4096-word tile; bound by issue of the "
-             "butterflies and twiddle loads, not by HBM (the same tile's load and store alone run near peak)")
+# the NTT tile passes as the profiles name them (lazy integer butterflies, 512-thread
+# blocks): the plain forward pass and the fused rescale / ModDown pass
+PLAIN_TILE_KERNEL = "k_ntt_blocks<12,true,0,true,512>"
+RESCALE_TILE_KERNEL = "k_ntt_blocks<12,true,2,true,512>"
+TILE_NOTE = ("12 stages of 64-bit integer Harvey butterflies through LDS per 4096-word tile; bound by VALU issue "
+             "of the modular products and the twiddle loads, not by HBM (the same tile's load and store alone run "
+             "near peak)")
 
 
 def measured_traffic(kernel: str, path: str = TRAFFIC_PROFILE):
@@ -781,8 +782,7 @@ def main() -> int:
         "vs_baseline": round(full_s / REF_SHARED_A_N8_S, 4),
         "vs_baseline_note": "value / 193.3 s, the reference's 768-ct shared-a n=8 figure on an RTX PRO 6000 "
                             "(README.md:32-34); below 1 is faster",
-        "dtype": "u64 (exact RNS limbs; the NTT butterflies run on integral FP64 values, exact for primes "
-                 "below 2^51)",
+        "dtype": "u64 (exact RNS limbs; FP64 only in the encoder)",
         "data": "synthetic (slots U(-1,1), random-init keys)",
         "config": {"workload": "768-ciphertext shared-a S2C-first bootstrap (one BERT-base layer's bootstrapping "
                                "segment), C3: N=2^16, Q=1x51+20x46+14x51, P=5x51, HW 192, scale 2^46; "

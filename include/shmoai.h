@@ -579,10 +579,8 @@ int shm_keyswitch_counters_reset(void);
  * default) for the autotuned choice; "matmul_overlap" (0, the default, or 1) runs the INT8
  * ct-pt matmul's contraction and rescale one after the other, or overlaps the contraction
  * (row_W <= 256, in limb windows) with the rescale on a second stream of the context;
- * "ntt_f64" (1, the default, or 0) runs NTT launches whose primes are all below 2^51 on
- * exact FP64 butterflies or on the integer ones (the same output words either way);
- * "ntt_tile_threads" (512, the default, or 256) runs the FP64 tile pass in 512-thread
- * blocks of radix-8 rounds or 256-thread blocks of radix-16 rounds.
+ * "ntt_tile_threads" (512, the default, or 256) runs the NTT tile passes in 512-thread
+ * blocks of radix-8 rounds or 256-thread blocks of radix-16 rounds (the same words).
  * Process-wide. */
 int shm_set_tuning(const char *name, int32_t value);
 int shm_get_tuning(const char *name, int32_t *value);

@@ -851,8 +851,8 @@ def set_tuning(name: str, value: int) -> None:
     """Set a process-wide knob; results never depend on them.
 
     "ks_batch_slots_log2", "ks_multi_slots_log2": slots of the key inner products (0..4, -1 = autotuned);
-    "matmul_overlap": 0/1; "ntt_f64": FP64 (1) or integer (0) NTT butterflies for primes below 2^51;
-    "ntt_tile_threads": 512 or 256 threads per FP64 NTT tile block."""
+    "matmul_overlap": 0/1; "ntt_tile_threads": 512 (radix-8 rounds) or 256 (radix-16 rounds) threads per
+    NTT tile block."""
     check(_lib().shm_set_tuning(name.encode(), int(value)))
 
 

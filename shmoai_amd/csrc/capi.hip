@@ -1854,11 +1854,6 @@ int shm_set_tuning(const char *name, int32_t value) {
             ks_tuning().ntt_tile_threads = value;
             return;
         }
-        if (n == "ntt_f64") {
-            if (value != 0 && value != 1) throw InvalidArgument("ntt_f64: 0 or 1");
-            ks_tuning().ntt_f64 = value;
-            return;
-        }
         if (value < -1 || value > 4) throw InvalidArgument("tuning knob value: -1 (default) or 0..4");
         if (n == "ks_batch_slots_log2")
             ks_tuning().batch_slots_log2 = value;
@@ -1899,8 +1894,6 @@ int shm_get_tuning(const char *name, int32_t *value) {
             *value = ks_tuning().multi_slots_log2;
         else if (n == "matmul_overlap")
             *value = ks_tuning().matmul_overlap;
-        else if (n == "ntt_f64")
-            *value = ks_tuning().ntt_f64;
         else if (n == "ntt_tile_threads")
             *value = ks_tuning().ntt_tile_threads;
         else

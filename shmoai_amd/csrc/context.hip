@@ -372,29 +372,6 @@ std::unique_ptr<Context> make_context(int logN, const std::vector<int> &bits, si
     upload(c->itw_s, itws);
     upload(c->ninv, ninv);
     upload(c->ninv_s, ninvs);
-    {
-        // FP64 NTT tables: w < q < 2^51 converts exactly, and w / q is correctly rounded
-        std::vector<double> twd(QP * N), twq(QP * N), itwd(QP * N), itwq(QP * N), nd(QP), nq(QP), qr(QP);
-        for (size_t p = 0; p < QP; p++) {
-            const double qd = (double)c->q[p];
-            for (size_t i = 0; i < N; i++) {
-                twd[p * N + i] = (double)tw[p * N + i];
-                twq[p * N + i] = (double)tw[p * N + i] / qd;
-                itwd[p * N + i] = (double)itw[p * N + i];
-                itwq[p * N + i] = (double)itw[p * N + i] / qd;
-            }
-            nd[p] = (double)ninv[p];
-            nq[p] = (double)ninv[p] / qd;
-            qr[p] = 1.0 / qd;
-        }
-        upload(c->twd, twd);
-        upload(c->twq, twq);
-        upload(c->itwd, itwd);
-        upload(c->itwq, itwq);
-        upload(c->ninvd, nd);
-        upload(c->ninvq, nq);
-        upload(c->qrcp, qr);
-    }
 
     // ModDown constants (rns.cu:95-128, 187-190): P mod q_i, P^-1 mod q_i,
     // (P/p_j)^-1 mod p_j, (P/p_j) mod q_i.

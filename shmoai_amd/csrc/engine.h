@@ -144,10 +144,6 @@ struct Context {
     DevBuf tw, tw_s;        // [QP][N]
     DevBuf itw, itw_s;      // [QP][N]
     DevBuf ninv, ninv_s;    // [QP]
-    // FP64 NTT tables (used for primes below 2^51): twiddles as doubles and
-    // fl(w / q), [QP][N]; N^-1, fl(N^-1 / q) and fl(1 / q), [QP]
-    DevBuf twd, twq, itwd, itwq;
-    DevBuf ninvd, ninvq, qrcp;
     DevBuf pinv, pinv_s;    // P^-1 mod q_i, i < nQ
     DevBuf pmod, pmod_s;    // P mod q_i, i < nQ
     DevBuf phat_inv, phat_inv_s;  // [nP] (P/p_j)^-1 mod p_j
@@ -267,11 +263,8 @@ struct KsTuning {
     // the context's second stream (1) or runs them one after the other (0, the default:
     // both kernels fill the GPU, and the overlapped form measured 7% slower)
     int matmul_overlap = 0;
-    // NTT launches whose primes are all below 2^51 use exact FP64 butterflies (1, the
-    // default) or the integer ones (0); the results are the same words
-    int ntt_f64 = 1;
-    // threads per block of the FP64 NTT tile pass: 512 (radix-8 rounds, twice the
-    // waves per 32 KB tile; the default) or 256 (radix-16 rounds)
+    // threads per block of the NTT tile passes: 512 (radix-8 rounds, more waves per
+    // 32 KB tile; the default) or 256 (radix-16 rounds); the results are the same words
     int ntt_tile_threads = 512;
 };
 KsTuning &ks_tuning();

@@ -15,24 +15,14 @@
 // units). A sub-transform whose first-stage group id is G uses twiddle
 // tw[(G << u) + i'] at its local stage u for local group i'.
 //
-// Arithmetic, chosen per launch from its largest prime:
-// - every prime below 2^51 (the bootstrapping and BERT chains): exact FP64
-//   butterflies. Residues travel between stages as integral doubles below 2^53
-//   in magnitude; a twiddle product is h + l = x w (l the FMA residual) minus
-//   e q with e = rint(x fl(w / q)), every step exact. CDNA4 issues FP64 FMA at
-//   full rate but 32-bit integer multiplies at a quarter, so this halves the
-//   VALU time of a butterfly. Between the two passes the buffer holds the
-//   doubles' bit patterns.
-// - below 2^64 / (5 + 4 log N) (about 2^57.9): integer Harvey butterflies, the
-//   forward ones leaving a unreduced with a cheaper quotient (values stay below
-//   (4 + 4 log N) q);
-// - otherwise integer Harvey butterflies with lazy ranges [0, 4q) forward and
-//   [0, 2q) inverse (primes are < 2^61, so nothing wraps).
-// Outputs are canonical residues in every mode.
+// Lazy ranges (Harvey): forward keeps [0, 4q) between stages, inverse [0, 2q);
+// primes are < 2^61 so nothing wraps. When every prime of a launch is below
+// 2^64 / (5 + 4 log N) (about 2^57.9) the forward stages skip the per-butterfly
+// reduction of a altogether and take a cheaper quotient in the twiddle product
+// (values stay below (4 + 4 log N) q). Outputs are canonical residues.
 #include <algorithm>
 
 #include <mutex>
-#include <type_traits>
 #include <vector>
 
 #include "engine.h"
@@ -74,92 +64,7 @@ struct NttTables {
     const DevMod *mods;
     const uint64_t *tw, *tws;   // twiddles and their Shoup quotients
     const uint64_t *ninv, *ninv_s;
-    const double *twd, *twq;    // FP64 mode: twiddles and fl(w / q)
-    const double *ninvd, *ninvq, *qrcp;  // N^-1 and fl(N^-1 / q); fl(1 / q)
 };
-
-enum { AR_INT = 0, AR_LAZY = 1, AR_F64 = 2 };
-
-// ---- exact FP64 residue arithmetic (primes q < 2^51) -----------------------
-struct F64K {
-    double q, qi;  // q and fl(1 / q)
-};
-
-// r == x (mod q) with |r| <= q/2 (1 + 2^-50), for integral |x| < 2^53: the
-// quotient estimate rint(x fl(1/q)) is the nearest integer to x / q up to an
-// error far below 1/2^50 of a unit, and x - e q is an integer below 2^52, which
-// the FMA forms exactly.
-__device__ __forceinline__ double f64_reduce(double x, double q, double qi) {
-    return __fma_rn(-rint(__dmul_rn(x, qi)), q, x);
-}
-
-// x w mod q as an integral double in [-1.51 q, 1.51 q], for integral |x| <= 2.01 q,
-// 0 <= w < q < 2^51, wq = fl(w / q): h + l = x w exactly (h = fl(x w), l the FMA
-// residual, |l| <= 2^49); e = rint(fl(x wq)) is within 0.5 + |x| 2^-52 < 1.51 of
-// x w / q; h - e q (below 1.51 q + 2^49 < 2^53) and then + l are exact integers.
-__device__ __forceinline__ double f64_mulmod(double x, double w, double wq, double q) {
-    const double h = __dmul_rn(x, w);
-    const double l = __fma_rn(x, w, -h);
-    const double e = rint(__dmul_rn(x, wq));
-    return __dadd_rn(__fma_rn(-e, q, h), l);
-}
-
-// v < 2^52 as a double: the mantissa of 2^52 + v holds v
-__device__ __forceinline__ double u52_to_d(uint64_t v) {
-    return __dadd_rn(__longlong_as_double((long long)(v | 0x4330000000000000ull)), -4503599627370496.0);
-}
-
-// canonical residue in [0, q) of an integral double |x| < 2^53
-__device__ __forceinline__ uint64_t f64_canon(double x, double q, double qi) {
-    double r = f64_reduce(x, q, qi);
-    r = r < 0.0 ? __dadd_rn(r, q) : r;
-    return (uint64_t)__double_as_longlong(__dadd_rn(r, 4503599627370496.0)) & 0x000fffffffffffffull;
-}
-
-__device__ __forceinline__ uint64_t f64_bits(double x) { return (uint64_t)__double_as_longlong(x); }
-__device__ __forceinline__ double f64_of_bits(uint64_t v) { return __longlong_as_double((long long)v); }
-
-// Forward butterfly on doubles: a reduced to |a| <= q/2 (1 + 2^-50), t = b w in
-// [-1.51 q, 1.51 q], so both outputs stay within 2.01 q, the bound the next
-// stage's product accepts.
-template <int R>
-__device__ __forceinline__ void ct_unit(double *x, uint32_t G, const double *__restrict__ tw,
-                                        const double *__restrict__ twq, F64K k) {
-#pragma unroll
-    for (int u = 0; u < R; u++) {
-        const int h = 1 << (R - 1 - u);
-#pragma unroll
-        for (int l = 0; l < (1 << R); l++) {
-            if ((l & h) == 0) {
-                const uint32_t ti = (G << u) + (uint32_t)(l >> (R - u));
-                const double a = f64_reduce(x[l], k.q, k.qi);
-                const double t = f64_mulmod(x[l + h], tw[ti], twq[ti], k.q);
-                x[l] = __dadd_rn(a, t);
-                x[l + h] = __dadd_rn(a, -t);
-            }
-        }
-    }
-}
-
-// Inverse butterfly on doubles: a + b and a - b reduced (|.| <= q/2 (1 + 2^-50)),
-// the difference times w then below 0.63 q; inputs up to 2 q in magnitude.
-template <int R>
-__device__ __forceinline__ void gs_unit(double *x, uint32_t G, const double *__restrict__ tw,
-                                        const double *__restrict__ twq, F64K k) {
-#pragma unroll
-    for (int u = R - 1; u >= 0; u--) {
-        const int h = 1 << (R - 1 - u);
-#pragma unroll
-        for (int l = 0; l < (1 << R); l++) {
-            if ((l & h) == 0) {
-                const uint32_t ti = (G << u) + (uint32_t)(l >> (R - u));
-                const double a = x[l], b = x[l + h];
-                x[l] = f64_reduce(__dadd_rn(a, b), k.q, k.qi);
-                x[l + h] = f64_mulmod(f64_reduce(__dadd_rn(a, -b), k.q, k.qi), tw[ti], twq[ti], k.q);
-            }
-        }
-    }
-}
 
 template <int R>
 __device__ __forceinline__ void ct_unit(uint64_t *x, uint32_t G, const uint64_t *__restrict__ tw,
@@ -378,9 +283,9 @@ struct NttIO {
 // Phase A (columns, first log N1 stages, log N1 <= 5). Each thread owns one
 // column {c + N2 r} and runs all its stages in registers: no LDS, no barrier;
 // the 256 threads of a block read and write 256 consecutive words per row.
-// FWD: forward stages 0..logN1-1 (output stays lazy; doubles in FP64 mode).
+// FWD: forward stages 0..logN1-1 (output stays lazy).
 // !FWD: inverse stages logN1-1..0, then x N^-1, output canonical.
-template <int LOGT, bool FWD, int IO, int AR>
+template <int LOGT, bool FWD, int IO, bool LAZY>
 __global__ __launch_bounds__(kThreads) void k_ntt_cols(uint64_t *__restrict__ data, LimbMap mp, NttTables tb,
                                                        int logN, NttIO io) {
     constexpr int T = 1 << LOGT;
@@ -421,62 +326,39 @@ __global__ __launch_bounds__(kThreads) void k_ntt_cols(uint64_t *__restrict__ da
 #pragma unroll
             for (int r = 0; r < T; r++) x[r] = base[r * N2];
         }
-        if constexpr (AR == AR_F64) {
-            const F64K kd{(double)q, tb.qrcp[prime]};
-            const double *tw = tb.twd + (size_t)prime * N, *twq = tb.twq + (size_t)prime * N;
-            double xd[T];
-            if (FWD) {
+        const uint64_t *tw = tb.tw + (size_t)prime * N, *tws = tb.tws + (size_t)prime * N;
+        if (FWD) {
+            if (LAZY)
+                ct_unit<LOGT>(x, 1u, tw, tws, IntKLazy{q});
+            else
+                ct_unit<LOGT>(x, 1u, tw, tws, IntK{q});
+            // streaming store: the tile pass reads these words back long after
 #pragma unroll
-                for (int r = 0; r < T; r++) xd[r] = u52_to_d(x[r]);
-                ct_unit<LOGT>(xd, 1u, tw, twq, kd);
-#pragma unroll
-                for (int r = 0; r < T; r++) __builtin_nontemporal_store(f64_bits(xd[r]), &base[r * N2]);
-            } else {
-                // the tile pass left doubles
-#pragma unroll
-                for (int r = 0; r < T; r++) xd[r] = f64_of_bits(x[r]);
-                gs_unit<LOGT>(xd, 1u, tw, twq, kd);
-                const double ni = tb.ninvd[prime], niq = tb.ninvq[prime];
-#pragma unroll
-                for (int r = 0; r < T; r++) base[r * N2] = f64_canon(f64_mulmod(xd[r], ni, niq, kd.q), kd.q, kd.qi);
-            }
+            for (int r = 0; r < T; r++) __builtin_nontemporal_store(x[r], &base[r * N2]);
         } else {
-            const uint64_t *tw = tb.tw + (size_t)prime * N, *tws = tb.tws + (size_t)prime * N;
-            if (FWD) {
-                if (AR == AR_LAZY)
-                    ct_unit<LOGT>(x, 1u, tw, tws, IntKLazy{q});
-                else
-                    ct_unit<LOGT>(x, 1u, tw, tws, IntK{q});
-                // streaming store: the tile pass reads these words back long after
+            gs_unit<LOGT>(x, 1u, tw, tws, IntK{q});
+            const uint64_t ni = tb.ninv[prime], nis = tb.ninv_s[prime];
 #pragma unroll
-                for (int r = 0; r < T; r++) __builtin_nontemporal_store(x[r], &base[r * N2]);
-            } else {
-                gs_unit<LOGT>(x, 1u, tw, tws, IntK{q});
-                const uint64_t ni = tb.ninv[prime], nis = tb.ninv_s[prime];
-#pragma unroll
-                for (int r = 0; r < T; r++) base[r * N2] = mul_shoup(x[r], ni, nis, q);
-            }
+            for (int r = 0; r < T; r++) base[r * N2] = mul_shoup(x[r], ni, nis, q);
         }
     }
 }
 
 // Phase B (contiguous blocks of N2, last log N2 stages). LOGT = log N2.
 // FWD: input lazy, output canonical. !FWD: input canonical (< 2q accepted),
-// output lazy (doubles in FP64 mode).
+// output lazy.
 // Work order: workgroups are dispatched to the 8 XCDs round-robin, so block b
 // runs on XCD b % 8. Each XCD takes a contiguous range of the work index
 // O = (limb * tiles + tile) * n_polys + poly: the polynomials that share one
 // prime and tile (hence one twiddle slice) run back to back on the same L2.
-template <int LOGT, bool FWD, int IO, int AR, int NT = kThreads>
-// 512-thread blocks ask for 8 waves per SIMD (64 VGPRs): four 32 KB tiles per CU.
-__global__ __launch_bounds__(NT, NT >= 512 ? 8 : 1) void k_ntt_blocks(uint64_t *__restrict__ data, LimbMap mp, NttTables tb,
+// 512-thread blocks ask for 6 waves per SIMD (80 VGPRs), three 32 KB tiles per CU.
+template <int LOGT, bool FWD, int IO, bool LAZY, int NT = kThreads>
+__global__ __launch_bounds__(NT, NT >= 512 ? 6 : 1) void k_ntt_blocks(uint64_t *__restrict__ data, LimbMap mp, NttTables tb,
                                                          int logN, NttIO io, uint32_t n_polys, uint32_t per_xcd) {
-    constexpr bool F64 = AR == AR_F64;
-    typedef typename std::conditional<F64, double, uint64_t>::type E;
     constexpr int T = 1 << LOGT;
     constexpr int BLKS = kTile / T;
     constexpr int TP = row_stride<T>();
-    __shared__ E lds[BLKS * TP];
+    __shared__ uint64_t lds[BLKS * TP];
     const size_t N = size_t(1) << logN;
     const uint32_t tiles = (uint32_t)(N / kTile);
     const uint32_t O = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
@@ -490,24 +372,18 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 8 : 1) void k_ntt_blocks(uint64_t *
     // block b (global index t0/T + b) has group id N1 + b at its first stage
     const uint32_t N1 = (uint32_t)(N >> LOGT);
     const uint64_t q = tb.mods[prime].q, q2 = q << 1;
-    const double qd = (double)q, qi = F64 ? tb.qrcp[prime] : 0.0;
     for (int i = threadIdx.x; i < kTile; i += NT) {
         const int b = i / T, r = i % T;
-        if constexpr (F64)
-            // forward: the column pass's doubles; inverse: residues below 2q
-            lds[b * TP + swz<T>(r)] = FWD ? f64_of_bits(base[i]) : u52_to_d(base[i]);
-        else
-            lds[b * TP + swz<T>(r)] = base[i];
+        lds[b * TP + swz<T>(r)] = base[i];
     }
-    // rescale: the integer forms load the accumulator words before the last round,
-    // so they arrive during it (loading them before all rounds cost occupancy); the
-    // FP64 form reads them in the epilogue, which keeps it within 64 VGPRs
+    // rescale: the accumulator words are loaded before the last round, so they
+    // arrive during it (loading them before all rounds cost occupancy)
     constexpr int PER = kTile / NT;
     const uint64_t *accp = nullptr;
     if (FWD && IO == IO_RESCALE)
         accp = io.packed ? io.acc_p.p[poly] + (size_t)k * N + t0
                          : io.acc + ((size_t)poly * io.acc_limbs + k) * N + t0;
-    uint64_t accv[IO == IO_RESCALE && !F64 ? PER : 1];
+    uint64_t accv[IO == IO_RESCALE ? PER : 1];
     auto load_acc = [&] {
         if (FWD && IO == IO_RESCALE) {
 #pragma unroll
@@ -515,25 +391,17 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 8 : 1) void k_ntt_blocks(uint64_t *
         }
     };
     __syncthreads();
-    if constexpr (F64)
-        run_rounds<LOGT, FWD, NT>(lds, N1, 1u, (uint32_t)(t0 / T), tb.twd + (size_t)prime * N,
-                                  tb.twq + (size_t)prime * N, F64K{qd, qi});
-    else if (AR == AR_LAZY)
+    if (LAZY)
         run_rounds<LOGT, FWD, NT>(lds, N1, 1u, (uint32_t)(t0 / T), tb.tw + (size_t)prime * N,
                                   tb.tws + (size_t)prime * N, IntKLazy{q}, load_acc);
     else
         run_rounds<LOGT, FWD, NT>(lds, N1, 1u, (uint32_t)(t0 / T), tb.tw + (size_t)prime * N,
                                   tb.tws + (size_t)prime * N, IntK{q}, load_acc);
-    // canonical residue of a forward output: below 4q, or below (4 + 4 log N) q when
-    // lazy, or a double within 2.01 q
-    auto canon = [&](E v) -> uint64_t {
-        if constexpr (F64) {
-            return f64_canon(v, qd, qi);
-        } else {
-            if (AR == AR_LAZY) return reduce64(v, tb.mods[prime]);
-            v = v >= q2 ? v - q2 : v;
-            return v >= q ? v - q : v;
-        }
+    // canonical residue of a forward output: below 4q, or below (4 + 4 log N) q when lazy
+    auto canon = [&](uint64_t v) -> uint64_t {
+        if (LAZY) return reduce64(v, tb.mods[prime]);
+        v = v >= q2 ? v - q2 : v;
+        return v >= q ? v - q : v;
     };
     if (FWD && IO == IO_RESCALE) {
         uint64_t *out;
@@ -543,7 +411,7 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 8 : 1) void k_ntt_blocks(uint64_t *
             uint64_t *ob = io.outs ? io.outs[poly / io.ppo] : io.out_single;
             out = ob + ((size_t)(poly % io.ppo) * (io.out_limbs ? io.out_limbs : mp.n_active) + k) * N + t0;
         }
-        const uint64_t qinv = io.qinv[prime], qis = io.qinv_s[prime];
+        const uint64_t qi = io.qinv[prime], qis = io.qinv_s[prime];
         // unreduced outputs stay below (4 + 4 logN) q (< 2^64 for the lazy primes): acc - v
         // is taken as acc + (4 + 4 logN) q - v, which the Shoup product accepts as it is
         const uint64_t kq = (uint64_t)(4 + 4 * logN) * q;
@@ -552,13 +420,9 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 8 : 1) void k_ntt_blocks(uint64_t *
         for (int j = 0; j < PER; j++) {
             const int i = threadIdx.x + j * NT;
             const int b = i / T, r = i % T;
-            const E raw = lds[b * TP + swz<T>(r)];
-            uint64_t y;
-            if constexpr (F64)
-                y = mul_shoup(sub_mod(accp[i], canon(raw), q), qinv, qis, q);
-            else
-                y = AR == AR_LAZY ? mul_shoup(accv[j] + kq - raw, qinv, qis, q)
-                                  : mul_shoup(sub_mod(accv[j], canon(raw), q), qinv, qis, q);
+            const uint64_t raw = lds[b * TP + swz<T>(r)];
+            uint64_t y = LAZY ? mul_shoup(accv[j] + kq - raw, qi, qis, q)
+                              : mul_shoup(sub_mod(accv[j], canon(raw), q), qi, qis, q);
             if (addto) y = add_mod(y, out[i], q);
             __builtin_nontemporal_store(y, &out[i]);
         }
@@ -570,10 +434,7 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 8 : 1) void k_ntt_blocks(uint64_t *
     } else {
         for (int i = threadIdx.x; i < kTile; i += NT) {
             const int b = i / T, r = i % T;
-            if constexpr (F64)
-                base[i] = f64_bits(lds[b * TP + swz<T>(r)]);
-            else
-                base[i] = lds[b * TP + swz<T>(r)];
+            base[i] = lds[b * TP + swz<T>(r)];
         }
     }
 }
@@ -617,53 +478,42 @@ void launch_ntt(const Context &c, uint64_t *data, const LimbMap &mp, size_t tota
                  FWD ? c.tw.as<uint64_t>() : c.itw.as<uint64_t>(),
                  FWD ? c.tw_s.as<uint64_t>() : c.itw_s.as<uint64_t>(),
                  c.ninv.as<uint64_t>(),
-                 c.ninv_s.as<uint64_t>(),
-                 FWD ? c.twd.as<double>() : c.itwd.as<double>(),
-                 FWD ? c.twq.as<double>() : c.itwq.as<double>(),
-                 c.ninvd.as<double>(),
-                 c.ninvq.as<double>(),
-                 c.qrcp.as<double>()};
-    // FP64 butterflies when every prime of the launch is below 2^51; otherwise
-    // unreduced forward integer butterflies when (5 + 4 log N) q < 2^64 for every
-    // prime (values below (4 + 4 log N) q, plus one q of the fused epilogue's operand)
-    uint64_t qmax = 0;
-    for (uint32_t j = 0; j < mp.n_active; j++) {
+                 c.ninv_s.as<uint64_t>()};
+    // unreduced forward butterflies when (5 + 4 log N) q < 2^64 for every prime of the
+    // launch: values below (4 + 4 log N) q, plus one q of the fused epilogue's operand
+    bool lazy = FWD;
+    const uint64_t lazy_max = ~uint64_t(0) / (uint64_t)(5 + 4 * c.logN);
+    for (uint32_t j = 0; j < mp.n_active && lazy; j++) {
         uint32_t k = mp.lo + j;
         if (k >= mp.skip_lo) k += mp.skip_width;
         const uint32_t prime = k < mp.Ql ? mp.first_prime + k : mp.nQ + (k - mp.Ql);
-        qmax = std::max(qmax, c.q.at(prime));
+        lazy = c.q.at(prime) < lazy_max;
     }
-    const uint64_t lazy_max = ~uint64_t(0) / (uint64_t)(5 + 4 * c.logN);
-    const int ar = ks_tuning().ntt_f64 && qmax < (uint64_t(1) << 51) ? AR_F64
-                   : FWD && qmax < lazy_max                           ? AR_LAZY
-                                                                      : AR_INT;
-    // FP64 tile passes: 512 threads in radix-8 rounds (the default: eight waves per
-    // SIMD instead of five hide the tile's load and barrier latency) or 256 in
-    // radix-16 rounds
+    // tile passes in 512-thread blocks of radix-8 rounds (the default): the lazy forward
+    // passes and the inverse ones; the forward passes of primes above the lazy bound keep
+    // 256-thread radix-16 rounds (their fused form would spill at 80 registers)
     const bool tile512 = ks_tuning().ntt_tile_threads == 512;
-#define SHM_COLS(L)                                                                                          \
-    case L:                                                                                                  \
-        if (ar == AR_F64)                                                                                    \
-            k_ntt_cols<L, FWD, IOA, AR_F64><<<grid_cols, kThreads, 0, s>>>(data, mp, tb, logN, io);          \
-        else if (ar == AR_LAZY)                                                                              \
-            k_ntt_cols<L, FWD, IOA, AR_LAZY><<<grid_cols, kThreads, 0, s>>>(data, mp, tb, logN, io);         \
-        else                                                                                                 \
-            k_ntt_cols<L, FWD, IOA, AR_INT><<<grid_cols, kThreads, 0, s>>>(data, mp, tb, logN, io);          \
+#define SHM_COLS(L)                                                                                    \
+    case L:                                                                                            \
+        if (lazy)                                                                                      \
+            k_ntt_cols<L, FWD, IOA, true><<<grid_cols, kThreads, 0, s>>>(data, mp, tb, logN, io);      \
+        else                                                                                           \
+            k_ntt_cols<L, FWD, IOA, false><<<grid_cols, kThreads, 0, s>>>(data, mp, tb, logN, io);     \
         break;
 #define SHM_BLKS(L)                                                                                          \
     case L:                                                                                                  \
-        if (ar == AR_F64 && tile512)                                                                         \
-            k_ntt_blocks<L, FWD, IOB, AR_F64, 512><<<grid_blk, 512, 0, s>>>(data, mp, tb, logN, io, n_polys, \
-                                                                            per_xcd);                        \
-        else if (ar == AR_F64)                                                                               \
-            k_ntt_blocks<L, FWD, IOB, AR_F64><<<grid_blk, kThreads, 0, s>>>(data, mp, tb, logN, io, n_polys, \
-                                                                            per_xcd);                        \
-        else if (ar == AR_LAZY)                                                                              \
-            k_ntt_blocks<L, FWD, IOB, AR_LAZY><<<grid_blk, kThreads, 0, s>>>(data, mp, tb, logN, io,         \
-                                                                             n_polys, per_xcd);              \
+        if (lazy && tile512)                                                                                 \
+            k_ntt_blocks<L, FWD, IOB, true, 512><<<grid_blk, 512, 0, s>>>(data, mp, tb, logN, io, n_polys,   \
+                                                                          per_xcd);                          \
+        else if (lazy)                                                                                       \
+            k_ntt_blocks<L, FWD, IOB, true><<<grid_blk, kThreads, 0, s>>>(data, mp, tb, logN, io, n_polys,    \
+                                                                          per_xcd);                          \
+        else if (!FWD && tile512)                                                                            \
+            k_ntt_blocks<L, FWD, IOB, false, 512><<<grid_blk, 512, 0, s>>>(data, mp, tb, logN, io, n_polys,  \
+                                                                           per_xcd);                         \
         else                                                                                                 \
-            k_ntt_blocks<L, FWD, IOB, AR_INT><<<grid_blk, kThreads, 0, s>>>(data, mp, tb, logN, io, n_polys, \
-                                                                            per_xcd);                        \
+            k_ntt_blocks<L, FWD, IOB, false><<<grid_blk, kThreads, 0, s>>>(data, mp, tb, logN, io, n_polys,   \
+                                                                           per_xcd);                         \
         break;
     auto cols = [&]() {
         switch (log1) {

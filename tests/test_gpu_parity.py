@@ -79,12 +79,11 @@ def test_ntt_paths_bitexact(oracle_lib, logN, bits, n_special):
         assert np.array_equal(ct.numpy()[:x.size], x), f"inverse, primes {first}..{first + n}"
 
 
-def test_ntt_f64_mode_equals_integer_mode(oracle_lib):
-    """NTT launches whose primes are all below 2^51 run exact FP64 butterflies;
-    switching them to the integer butterflies, or the FP64 tile pass from 512-thread
-    radix-8 rounds to 256-thread radix-16 rounds, must give the same words, for the
-    plain transforms (with the residues 0 and q - 1) and for a multiply-relinearize-
-    rescale, whose ModDown and rescale fuse into the last NTT pass."""
+def test_ntt_tile_shapes_bitexact(oracle_lib):
+    """The NTT tile passes run in 512-thread blocks of radix-8 rounds by default;
+    the 256-thread radix-16 form must give the same words, for the plain transforms
+    (with the residues 0 and q - 1, and the oracle's forward NTT) and for a
+    multiply-relinearize-rescale, whose ModDown and rescale fuse into the last pass."""
     import shmoai_amd as shm
     bits = [51] + [46] * 4 + [51] * 3 + [51, 51]
     ctx = shm.Context(16, bits, 2)
@@ -101,10 +100,9 @@ def test_ntt_f64_mode_equals_integer_mode(oracle_lib):
     rk = sk.gen_relinkey(seed=2)
     ev = shm.Evaluator(ctx)
     outs = {}
-    assert shm.get_tuning("ntt_f64") == 1 and shm.get_tuning("ntt_tile_threads") == 512
+    assert shm.get_tuning("ntt_tile_threads") == 512
     try:
-        for mode, threads in ((1, 512), (1, 256), (0, 512)):
-            shm.set_tuning("ntt_f64", mode)
+        for threads in (512, 256):
             shm.set_tuning("ntt_tile_threads", threads)
             ct = shm.Ciphertext(ctx, 2, 1).load(x)
             ctx.ntt_forward(ct.data_ptr(), 2, nQ, 0)
@@ -117,15 +115,13 @@ def test_ntt_f64_mode_equals_integer_mode(oracle_lib):
             ev.multiply_inplace(a, b)
             ev.relinearize_inplace(a, rk)
             ev.rescale_to_next_inplace(a)
-            outs[mode, threads] = (fwd, inv, a.numpy())
+            outs[threads] = (fwd, inv, a.numpy())
     finally:
-        shm.set_tuning("ntt_f64", 1)
         shm.set_tuning("ntt_tile_threads", 512)
-    assert np.array_equal(outs[1, 512][1], x)
-    assert np.array_equal(outs[1, 512][0], ref.ntt(x, np.tile(ref.pidx(nQ), 2)))
-    for other in ((1, 256), (0, 512)):
-        for i, what in enumerate(("forward", "inverse", "multiply-relinearize-rescale")):
-            assert np.array_equal(outs[1, 512][i], outs[other][i]), (what, other)
+    assert np.array_equal(outs[512][1], x)
+    assert np.array_equal(outs[512][0], ref.ntt(x, np.tile(ref.pidx(nQ), 2)))
+    for i, what in enumerate(("forward", "inverse", "multiply-relinearize-rescale")):
+        assert np.array_equal(outs[512][i], outs[256][i]), what
 
 
 def test_secret_and_keys_bitexact(c1):
