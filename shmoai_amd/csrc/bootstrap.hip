@@ -48,7 +48,7 @@ __global__ void k_lt_mac(uint64_t *acc, PtPack pk, int nterms, const DevMod *mod
         for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
             Acc128 a{0, 0};
             for (int k = 0; k < nterms; k++) mac128(a, pk.x[k][row * N + i], pk.pt[k][limb * N + i]);
-            acc[row * N + i] = reduce128(a, m);
+            __builtin_nontemporal_store(reduce128(a, m), &acc[row * N + i]);
         }
     }
 }
@@ -77,7 +77,7 @@ __global__ void k_const_lincomb(uint64_t *acc, SrcPack sp, int nterms, const Dev
         for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
             Acc128 a{0, 0};
             for (int k = 0; k < nterms; k++) mac128(a, sp.x[k][(p * sp.ql[k] + l) * N + i], cres[k]);
-            acc[row * N + i] = reduce128(a, m);
+            __builtin_nontemporal_store(reduce128(a, m), &acc[row * N + i]);
         }
     }
 }
@@ -107,7 +107,7 @@ __global__ void k_const_lincomb_batch(LeafPack lp, int nterms, int members, cons
         for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
             Acc128 a{0, 0};
             for (int k = 0; k < nterms; k++) mac128(a, lp.x[mb][k][(p * lp.ql[k] + l) * N + i], cres[k]);
-            out[i] = reduce128(a, m);
+            __builtin_nontemporal_store(reduce128(a, m), &out[i]);
         }
     }
 }

@@ -108,7 +108,7 @@ __global__ void k_modup_bconv(const uint64_t *__restrict__ tcoef_all, uint64_t *
                 if (rounded) mac128(acc, mp.q - r, dig_mod[g * QlP + k]);
                 val = reduce128(acc, mp);
             }
-            o[k * N] = val;
+            __builtin_nontemporal_store(val, &o[k * N]);
         }
     }
 }
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(256) void k_ks_inner_multi(const uint64_t *__restri
         const size_t st = kr.stride[o];
         uint64_t kv[MAXB];
 #pragma unroll
-        for (int g = 0; g < MAXB; g++) kv[g] = g < ndig ? kb[(size_t)g * st + kl] : 0;
+        for (int g = 0; g < MAXB; g++) kv[g] = g < ndig ? __builtin_nontemporal_load(&kb[(size_t)g * st + kl]) : 0;
         uint64_t v;
         if (P52) {
             Acc52 a{0, 0, 0};
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(256) void k_ks_inner_multi(const uint64_t *__restri
                 if (g < ndig) mac128(a, dv[g], kv[g]);
             v = reduce128(a, m);
         }
-        cx[(size_t)o * total + t] = accumulate ? add_mod(cx[(size_t)o * total + t], v, m.q) : v;
+        __builtin_nontemporal_store(accumulate ? add_mod(cx[(size_t)o * total + t], v, m.q) : v, &cx[(size_t)o * total + t]);
     }
 }
 
@@ -240,7 +240,7 @@ __global__ void k_moddown_bconv(const uint64_t *__restrict__ cx, uint64_t *__res
                 if (rounded) mac128(acc, mq.q - r, pmod[i]);
                 val = reduce128(acc, mq);
             }
-            d[i * N] = val;
+            __builtin_nontemporal_store(val, &d[i * N]);
         }
     }
 }
@@ -284,7 +284,7 @@ __global__ void k_moddown_rescale_bconv(const uint64_t *__restrict__ cx, uint64_
                 for (int b = 0; b < kMaxAlpha + 1; b++)
                     if ((size_t)b < nb) mac52(acc, ys[b], bhat_mod[i * nb + b]);
                 mac52(acc, split26(mq.q - r), bmod[i]);
-                d[i * N] = reduce52(acc, mq);
+                __builtin_nontemporal_store(reduce52(acc, mq), &d[i * N]);
                 continue;
             }
             Acc128 acc{0, 0};
@@ -293,7 +293,7 @@ __global__ void k_moddown_rescale_bconv(const uint64_t *__restrict__ cx, uint64_
                 if ((size_t)b < nb) mac128(acc, y[b], bhat_mod[i * nb + b]);
             // - r B == (q - r) [B]_q (mod q), folded in before the one reduction
             mac128(acc, mq.q - r, bmod[i]);
-            d[i * N] = reduce128(acc, mq);
+            __builtin_nontemporal_store(reduce128(acc, mq), &d[i * N]);
         }
     }
 }
@@ -351,7 +351,8 @@ __global__ __launch_bounds__(256) void k_ks_inner_batch(const uint64_t *__restri
         const uint64_t *dm = d + ((size_t)mem * beta + g0) * total + src;
         uint64_t dv[MAXB];
 #pragma unroll
-        for (int g = 0; g < MAXB; g++) dv[g] = g < nb ? (g == own_g ? c2.p[mem][src] : dm[(size_t)g * total]) : 0;
+        for (int g = 0; g < MAXB; g++)
+            dv[g] = g < nb ? (g == own_g ? c2.p[mem][src] : __builtin_nontemporal_load(&dm[(size_t)g * total])) : 0;
         uint64_t v0, v1;
         if (P52) {
             Acc52 a0{0, 0, 0}, a1{0, 0, 0};
@@ -395,8 +396,8 @@ __global__ __launch_bounds__(256) void k_ks_inner_batch(const uint64_t *__restri
             v0 = add_mod(v0, mul_shoup(addp.p[2 * mem][k * N + j], pm, pms, m.q), m.q);
             v1 = add_mod(v1, mul_shoup(addp.p[2 * mem + 1][k * N + j], pm, pms, m.q), m.q);
         }
-        *o0 = v0;
-        *o1 = v1;
+        __builtin_nontemporal_store(v0, o0);
+        __builtin_nontemporal_store(v1, o1);
     }
 }
 
