@@ -339,7 +339,7 @@ __global__ __launch_bounds__(kThreads) void k_ntt_cols(uint64_t *__restrict__ da
             gs_unit<LOGT>(x, 1u, tw, tws, IntK{q});
             const uint64_t ni = tb.ninv[prime], nis = tb.ninv_s[prime];
 #pragma unroll
-            for (int r = 0; r < T; r++) base[r * N2] = mul_shoup(x[r], ni, nis, q);
+            for (int r = 0; r < T; r++) __builtin_nontemporal_store(mul_shoup(x[r], ni, nis, q), &base[r * N2]);
         }
     }
 }
@@ -429,7 +429,7 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 6 : 1) void k_ntt_blocks(uint64_t *
     } else if (FWD) {
         for (int i = threadIdx.x; i < kTile; i += NT) {
             const int b = i / T, r = i % T;
-            base[i] = canon(lds[b * TP + swz<T>(r)]);
+            __builtin_nontemporal_store(canon(lds[b * TP + swz<T>(r)]), &base[i]);
         }
     } else {
         for (int i = threadIdx.x; i < kTile; i += NT) {

@@ -90,9 +90,9 @@ __global__ void k_tensor(const uint64_t *a, const uint64_t *b, uint64_t *out, co
                 out[t + S] = add_mod(out[t + S], d1, m.q);
                 out[t + 2 * S] = add_mod(out[t + 2 * S], d2, m.q);
             } else {
-                out[t] = d0;
-                out[t + S] = d1;
-                out[t + 2 * S] = d2;
+                __builtin_nontemporal_store(d0, &out[t]);
+                __builtin_nontemporal_store(d1, &out[t + S]);
+                __builtin_nontemporal_store(d2, &out[t + 2 * S]);
             }
         }
     }
@@ -134,9 +134,9 @@ __global__ void k_tensor_cheb(const uint64_t *a, const uint64_t *b, const uint64
                 d0 = sub_mod(d0, mul_shoup(t[x], cm, cms, q), q);
                 d1 = sub_mod(d1, mul_shoup(t[x + St], cm, cms, q), q);
             }
-            out[x] = d0;
-            out[x + S] = d1;
-            out[x + 2 * S] = d2;
+            __builtin_nontemporal_store(d0, &out[x]);
+            __builtin_nontemporal_store(d1, &out[x + S]);
+            __builtin_nontemporal_store(d2, &out[x + 2 * S]);
         }
     }
 }
@@ -175,7 +175,7 @@ __global__ void k_gather_polys(PolyPack in, uint64_t *__restrict__ out, size_t w
     SHM_ROWS_LOOP(p, npolys) {
         const uint64_t *src = in.p[p];
         uint64_t *dst = out + p * words;
-        SHM_COLS_LOOP(i, words) dst[i] = src[i];
+        SHM_COLS_LOOP(i, words) __builtin_nontemporal_store(src[i], &dst[i]);
     }
 }
 
@@ -184,7 +184,7 @@ __global__ void k_permute(const uint64_t *in, uint64_t *out, const uint32_t *tab
     SHM_ROWS_LOOP(limb, nlimbs) {
         const uint64_t *src = in + limb * N;
         uint64_t *dst = out + limb * N;
-        SHM_COLS_LOOP(i, N) dst[i] = src[table[i]];
+        SHM_COLS_LOOP(i, N) __builtin_nontemporal_store(src[table[i]], &dst[i]);
     }
 }
 
