@@ -18,6 +18,8 @@
 #include <mutex>
 #include <string>
 
+#include <type_traits>
+
 #include "engine.h"
 
 namespace shm {
@@ -45,11 +47,31 @@ __device__ __forceinline__ uint32_t qlp_prime(size_t k, size_t Ql, size_t nQ) {
 // that otherwise lands on the slots nearest w = 1.
 constexpr int kMaxAlpha = 16;
 
+// The base conversions unroll their sums over the digit's (or the special primes')
+// limbs to a compile-time bound A >= that count, so the common sizes (5 special primes
+// at the bootstrapping parameters, 1 for the matmul's chain) carry no tests for absent
+// terms: A is 1, 2, 4, 5, 8 or 16.
+template <typename F>
+void with_alpha_bound(size_t alpha, F &&f) {
+    if (alpha <= 1)
+        f(std::integral_constant<int, 1>{});
+    else if (alpha <= 2)
+        f(std::integral_constant<int, 2>{});
+    else if (alpha <= 4)
+        f(std::integral_constant<int, 4>{});
+    else if (alpha == 5)
+        f(std::integral_constant<int, 5>{});
+    else if (alpha <= 8)
+        f(std::integral_constant<int, 8>{});
+    else
+        f(std::integral_constant<int, kMaxAlpha>{});
+}
+
 // Blocks never straddle two digits (N is a multiple of the block size), so the
 // digit index and every table address derived from it are wave-uniform.
 // `members` polynomials are converted at once: member m reads tcoef + m Ql N
 // and writes out + m beta QlP N.
-template <bool P52>
+template <bool P52, int A>
 __global__ void k_modup_bconv(const uint64_t *__restrict__ tcoef_all, uint64_t *__restrict__ out_all,
                               const DevMod *__restrict__ mods, const uint64_t *__restrict__ hat_inv,
                               const uint64_t *__restrict__ hat_inv_s, const uint64_t *__restrict__ hat_mod,
@@ -63,10 +85,10 @@ __global__ void k_modup_bconv(const uint64_t *__restrict__ tcoef_all, uint64_t *
         const uint64_t *tcoef = tcoef_all + mem * Ql * N;
         const size_t start = g * alpha, size = min(alpha, Ql - start);
         uint64_t *o = out_all + (mem * beta + g) * QlP * N + j;
-        uint64_t y[kMaxAlpha];
+        uint64_t y[A];
         double v = 0.0;
 #pragma unroll
-        for (int i = 0; i < kMaxAlpha; i++) {
+        for (int i = 0; i < A; i++) {
             y[i] = 0;
             if ((size_t)i < size) {
                 const size_t li = start + i;
@@ -80,9 +102,9 @@ __global__ void k_modup_bconv(const uint64_t *__restrict__ tcoef_all, uint64_t *
         }
         const uint64_t r = rounded ? (uint64_t)__dadd_rn(v, 0.5) : 0;
         const uint64_t qs = mods[start].q;
-        Split26 ys[kMaxAlpha];
+        Split26 ys[A];
 #pragma unroll
-        for (int i = 0; i < kMaxAlpha; i++) ys[i] = split26(P52 ? y[i] : 0);
+        for (int i = 0; i < A; i++) ys[i] = split26(P52 ? y[i] : 0);
         for (size_t k = 0; k < QlP; k++) {
             if (k >= start && k < start + size) continue;
             const DevMod mp = mods[qlp_prime(k, Ql, nQ)];
@@ -94,7 +116,7 @@ __global__ void k_modup_bconv(const uint64_t *__restrict__ tcoef_all, uint64_t *
                 Acc52 acc{0, 0, 0};
                 const uint64_t *hm = hat_mod + (g * alpha) * QlP + k;
 #pragma unroll
-                for (int i = 0; i < kMaxAlpha; i++)
+                for (int i = 0; i < A; i++)
                     if ((size_t)i < size) mac52(acc, ys[i], hm[i * QlP]);
                 if (rounded) mac52(acc, split26(mp.q - r), dig_mod[g * QlP + k]);
                 val = reduce52(acc, mp);
@@ -102,7 +124,7 @@ __global__ void k_modup_bconv(const uint64_t *__restrict__ tcoef_all, uint64_t *
                 Acc128 acc{0, 0};
                 const uint64_t *hm = hat_mod + (g * alpha) * QlP + k;
 #pragma unroll
-                for (int i = 0; i < kMaxAlpha; i++)
+                for (int i = 0; i < A; i++)
                     if ((size_t)i < size) mac128(acc, y[i], hm[i * QlP]);
                 // - r D_g == (p - r) [D_g]_p (mod p), folded in before the one reduction
                 if (rounded) mac128(acc, mp.q - r, dig_mod[g * QlP + k]);
@@ -192,7 +214,7 @@ __global__ __launch_bounds__(256) void k_ks_inner_multi(const uint64_t *__restri
 // q_i, one thread per (p, j). With `rounded`, the conversion is exact and
 // centred: it subtracts round(sum_j y_j / p_j) P, so delta is the representative
 // of cx mod P in (-P/2, P/2] and (cx - delta) / P rounds to nearest.
-template <bool P52>
+template <bool P52, int A>
 __global__ void k_moddown_bconv(const uint64_t *__restrict__ cx, uint64_t *__restrict__ delta,
                                 const DevMod *__restrict__ mods, const uint64_t *__restrict__ phat_inv,
                                 const uint64_t *__restrict__ phat_inv_s, const uint64_t *__restrict__ phat_mod,
@@ -203,10 +225,10 @@ __global__ void k_moddown_bconv(const uint64_t *__restrict__ cx, uint64_t *__res
         const size_t p = t >> __builtin_ctzll(N), j = t & (N - 1);
         const uint64_t *cxP = cx + (p * QlP + Ql) * N + j;
         uint64_t *d = delta + p * Ql * N + j;
-        uint64_t y[kMaxAlpha];
+        uint64_t y[A];
         double v = 0.0;
 #pragma unroll
-        for (int jj = 0; jj < kMaxAlpha; jj++) {
+        for (int jj = 0; jj < A; jj++) {
             y[jj] = 0;
             if ((size_t)jj < nP) {
                 const uint64_t pj = mods[nQ + jj].q;
@@ -215,16 +237,16 @@ __global__ void k_moddown_bconv(const uint64_t *__restrict__ cx, uint64_t *__res
             }
         }
         const uint64_t r = rounded ? (uint64_t)__dadd_rn(v, 0.5) : 0;
-        Split26 ys[kMaxAlpha];
+        Split26 ys[A];
 #pragma unroll
-        for (int jj = 0; jj < kMaxAlpha; jj++) ys[jj] = split26(P52 ? y[jj] : 0);
+        for (int jj = 0; jj < A; jj++) ys[jj] = split26(P52 ? y[jj] : 0);
         for (size_t i = 0; i < Ql; i++) {
             const DevMod mq = mods[i];
             uint64_t val;
             if (P52 && nP > 1) {
                 Acc52 acc{0, 0, 0};
 #pragma unroll
-                for (int jj = 0; jj < kMaxAlpha; jj++)
+                for (int jj = 0; jj < A; jj++)
                     if ((size_t)jj < nP) mac52(acc, ys[jj], phat_mod[i * nP + jj]);
                 if (rounded) mac52(acc, split26(mq.q - r), pmod[i]);
                 val = reduce52(acc, mq);
@@ -234,7 +256,7 @@ __global__ void k_moddown_bconv(const uint64_t *__restrict__ cx, uint64_t *__res
             } else {
                 Acc128 acc{0, 0};
 #pragma unroll
-                for (int jj = 0; jj < kMaxAlpha; jj++)
+                for (int jj = 0; jj < A; jj++)
                     if ((size_t)jj < nP) mac128(acc, y[jj], phat_mod[i * nP + jj]);
                 // - r P == (q - r) [P]_q (mod q), folded in before the one reduction
                 if (rounded) mac128(acc, mq.q - r, pmod[i]);
@@ -250,7 +272,7 @@ __global__ void k_moddown_bconv(const uint64_t *__restrict__ cx, uint64_t *__res
 // Ql+nP-1 of its QlP limbs) to q_i: sum_b y_b (B/b) - round(sum_b y_b / b) B with
 // y_b = [x_b (B/b)^-1]_b, the representative of the value mod B in (-B/2, B/2],
 // so that (x - delta) / B rounds x / (P q_last) to nearest. One thread per (p, j).
-template <bool P52>
+template <bool P52, int A>
 __global__ void k_moddown_rescale_bconv(const uint64_t *__restrict__ cx, uint64_t *__restrict__ delta,
                                         const DevMod *__restrict__ mods, const uint64_t *__restrict__ bhat_inv,
                                         const uint64_t *__restrict__ bhat_inv_s,
@@ -261,10 +283,10 @@ __global__ void k_moddown_rescale_bconv(const uint64_t *__restrict__ cx, uint64_
         const size_t p = t >> __builtin_ctzll(N), j = t & (N - 1);
         const uint64_t *xb = cx + (p * QlP + L1) * N + j;
         uint64_t *d = delta + p * L1 * N + j;
-        uint64_t y[kMaxAlpha + 1];
+        uint64_t y[A + 1];
         double v = 0.0;
 #pragma unroll
-        for (int b = 0; b < kMaxAlpha + 1; b++) {
+        for (int b = 0; b < A + 1; b++) {
             y[b] = 0;
             if ((size_t)b < nb) {
                 const uint64_t qb = mods[b == 0 ? L1 : nQ + (b - 1)].q;
@@ -273,15 +295,15 @@ __global__ void k_moddown_rescale_bconv(const uint64_t *__restrict__ cx, uint64_
             }
         }
         const uint64_t r = (uint64_t)__dadd_rn(v, 0.5);
-        Split26 ys[kMaxAlpha + 1];
+        Split26 ys[A + 1];
 #pragma unroll
-        for (int b = 0; b < kMaxAlpha + 1; b++) ys[b] = split26(P52 ? y[b] : 0);
+        for (int b = 0; b < A + 1; b++) ys[b] = split26(P52 ? y[b] : 0);
         for (size_t i = 0; i < L1; i++) {
             const DevMod mq = mods[i];
             if (P52) {
                 Acc52 acc{0, 0, 0};
 #pragma unroll
-                for (int b = 0; b < kMaxAlpha + 1; b++)
+                for (int b = 0; b < A + 1; b++)
                     if ((size_t)b < nb) mac52(acc, ys[b], bhat_mod[i * nb + b]);
                 mac52(acc, split26(mq.q - r), bmod[i]);
                 __builtin_nontemporal_store(reduce52(acc, mq), &d[i * N]);
@@ -289,7 +311,7 @@ __global__ void k_moddown_rescale_bconv(const uint64_t *__restrict__ cx, uint64_
             }
             Acc128 acc{0, 0};
 #pragma unroll
-            for (int b = 0; b < kMaxAlpha + 1; b++)
+            for (int b = 0; b < A + 1; b++)
                 if ((size_t)b < nb) mac128(acc, y[b], bhat_mod[i * nb + b]);
             // - r B == (q - r) [B]_q (mod q), folded in before the one reduction
             mac128(acc, mq.q - r, bmod[i]);
@@ -535,9 +557,12 @@ void ks_modup(const Context &c, const uint64_t *c2, size_t Ql, uint64_t *d, hipS
     launch_copy_words(tcoef.as<uint64_t>(), c2, Ql * N, s);
     ntt_inverse(c, tcoef.as<uint64_t>(), 1, Ql, 0, s);
     if (alpha > (size_t)kMaxAlpha) throw InvalidArgument("at most 16 special primes");
-    (c.primes52 ? k_modup_bconv<true> : k_modup_bconv<false>)<<<grid_for(beta * N), kBlock, 0, s>>>(
-        tcoef.as<uint64_t>(), d, c.mods(), L.hat_inv.as<uint64_t>(), L.hat_inv_s.as<uint64_t>(),
-        L.hat_mod.as<uint64_t>(), L.dig_mod.as<uint64_t>(), Ql, c.nQ, alpha, beta, N, 1, rounded ? 1 : 0, 1);
+    with_alpha_bound(alpha, [&](auto bound) {
+        constexpr int A = decltype(bound)::value;
+        (c.primes52 ? k_modup_bconv<true, A> : k_modup_bconv<false, A>)<<<grid_for(beta * N), kBlock, 0, s>>>(
+            tcoef.as<uint64_t>(), d, c.mods(), L.hat_inv.as<uint64_t>(), L.hat_inv_s.as<uint64_t>(),
+            L.hat_mod.as<uint64_t>(), L.dig_mod.as<uint64_t>(), Ql, c.nQ, alpha, beta, N, 1, rounded ? 1 : 0, 1);
+    });
     SHM_HIP(hipGetLastError());
     // one launch over every digit's QlP limbs (the own limbs were written in
     // coefficient form and transform back to their NTT-form input words)
@@ -602,9 +627,12 @@ void ks_moddown_pack(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, c
     Scratch delta(npolys * Ql * N * sizeof(uint64_t), s);
     ntt_inverse_qlp(c, cx, Ql, Ql, QlP, npolys, QlP * N, s);
     if (nP > (size_t)kMaxAlpha) throw InvalidArgument("at most 16 special primes");
-    (c.primes52 ? k_moddown_bconv<true> : k_moddown_bconv<false>)<<<grid_for(npolys * N), kBlock, 0, s>>>(
-        cx, delta.as<uint64_t>(), c.mods(), c.phat_inv.as<uint64_t>(), c.phat_inv_s.as<uint64_t>(),
-        c.phat_mod.as<uint64_t>(), c.pmod.as<uint64_t>(), Ql, c.nQ, nP, N, npolys, rounded ? 1 : 0);
+    with_alpha_bound(nP, [&](auto bound) {
+        constexpr int A = decltype(bound)::value;
+        (c.primes52 ? k_moddown_bconv<true, A> : k_moddown_bconv<false, A>)<<<grid_for(npolys * N), kBlock, 0, s>>>(
+            cx, delta.as<uint64_t>(), c.mods(), c.phat_inv.as<uint64_t>(), c.phat_inv_s.as<uint64_t>(),
+            c.phat_mod.as<uint64_t>(), c.pmod.as<uint64_t>(), Ql, c.nQ, nP, N, npolys, rounded ? 1 : 0);
+    });
     SHM_HIP(hipGetLastError());
     // the last NTT pass of delta writes (cx - delta) * P^-1 into the outputs
     moddown_ntt_fused(c, delta.as<uint64_t>(), cx, Ql, QlP, out, npolys, acc_mask, c.pinv.as<uint64_t>(),
@@ -650,10 +678,13 @@ void ks_modup_batch(const Context &c, const PolyPack &c2, size_t m, size_t Ql, u
     // fill the GPU, so the own limbs are written in coefficient form and one launch
     // transforms every limb (back to c2's words for the own ones).
     const bool per_digit = m >= 4;
-    (c.primes52 ? k_modup_bconv<true> : k_modup_bconv<false>)<<<grid_for(m * beta * N), kBlock, 0, s>>>(
-        tcoef.as<uint64_t>(), d, c.mods(), L.hat_inv.as<uint64_t>(), L.hat_inv_s.as<uint64_t>(),
-        L.hat_mod.as<uint64_t>(), L.dig_mod.as<uint64_t>(), Ql, c.nQ, alpha, beta, N, m, rounded ? 1 : 0,
-        per_digit ? 0 : 1);
+    with_alpha_bound(alpha, [&](auto bound) {
+        constexpr int A = decltype(bound)::value;
+        (c.primes52 ? k_modup_bconv<true, A> : k_modup_bconv<false, A>)<<<grid_for(m * beta * N), kBlock, 0, s>>>(
+            tcoef.as<uint64_t>(), d, c.mods(), L.hat_inv.as<uint64_t>(), L.hat_inv_s.as<uint64_t>(),
+            L.hat_mod.as<uint64_t>(), L.dig_mod.as<uint64_t>(), Ql, c.nQ, alpha, beta, N, m, rounded ? 1 : 0,
+            per_digit ? 0 : 1);
+    });
     SHM_HIP(hipGetLastError());
     if (per_digit) {
         for (size_t g = 0; g < beta; g++) {
@@ -735,9 +766,13 @@ void keyswitch_rescale_batch(const Context &c, const PolyPack &c2, const PolyPac
     bump(g_cnt_moddown, Ql, 2 * m);
     ntt_inverse_qlp(c, cx.as<uint64_t>(), Ql, Ql - 1, QlP, 2 * m, QlP * N, s);
     Scratch delta(2 * m * (Ql - 1) * N * sizeof(uint64_t), s);
-    (c.primes52 ? k_moddown_rescale_bconv<true> : k_moddown_rescale_bconv<false>)<<<grid_for(2 * m * N), kBlock, 0, s>>>(
-        cx.as<uint64_t>(), delta.as<uint64_t>(), c.mods(), L.bhat_inv.as<uint64_t>(), L.bhat_inv_s.as<uint64_t>(),
-        L.bhat_mod.as<uint64_t>(), L.bmod.as<uint64_t>(), Ql, c.nQ, nP, N, 2 * m);
+    with_alpha_bound(nP, [&](auto bound) {
+        constexpr int A = decltype(bound)::value;
+        (c.primes52 ? k_moddown_rescale_bconv<true, A>
+                    : k_moddown_rescale_bconv<false, A>)<<<grid_for(2 * m * N), kBlock, 0, s>>>(
+            cx.as<uint64_t>(), delta.as<uint64_t>(), c.mods(), L.bhat_inv.as<uint64_t>(),
+            L.bhat_inv_s.as<uint64_t>(), L.bhat_mod.as<uint64_t>(), L.bmod.as<uint64_t>(), Ql, c.nQ, nP, N, 2 * m);
+    });
     SHM_HIP(hipGetLastError());
     moddown_ntt_fused(c, delta.as<uint64_t>(), cx.as<uint64_t>(), Ql - 1, QlP, out, 2 * m, 0,
                       L.binv.as<uint64_t>(), L.binv_s.as<uint64_t>(), s);
