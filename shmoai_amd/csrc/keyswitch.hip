@@ -326,11 +326,12 @@ __global__ void k_moddown_rescale_bconv(const uint64_t *__restrict__ cx, uint64_
 // (hoisted rotations of every member by one step).
 // Layout: one thread per (member, limb k, coefficient j). A block covers
 // J = 256 / MG consecutive coefficients and MG = 2^lgmg member slots (MG <= 16,
-// so J >= 16 and the lanes of one member read whole 128-byte runs); the MG
-// threads of one coefficient load the same 2 beta key words, which L1 serves
-// after the first, so each key word leaves HBM once for the group. Every
-// thread has only its own beta digit loads to wait for, and the grid holds
-// members x QlP x N threads to hide their latency.
+// so J >= 16 and the lanes of one member read whole 128-byte runs). Member slot 0
+// loads the 2 beta key words of each coefficient once and stages them in LDS for
+// every slot and member, so each key word leaves HBM once for the group and no
+// thread holds the keys in registers (74 VGPRs at MAXB = 8 against 134, 5 waves
+// per SIMD against 3). Every thread has only its own beta digit loads to wait for,
+// and the grid holds members x QlP x N threads to hide their latency.
 template <int MAXB, bool P52>
 __global__ __launch_bounds__(256) void k_ks_inner_batch(const uint64_t *__restrict__ d, const uint64_t *__restrict__ key,
                                                         uint64_t *__restrict__ cx, const DevMod *__restrict__ mods,
@@ -342,29 +343,37 @@ __global__ __launch_bounds__(256) void k_ks_inner_batch(const uint64_t *__restri
                                                         const uint64_t *__restrict__ pmod_s) {
     const size_t QlP = Ql + alpha, total = QlP * N;
     const int lgj = 8 - lgmg;
-    const size_t t = ((size_t)blockIdx.x << lgj) + (threadIdx.x & ((1u << lgj) - 1));
+    const int lane = (int)(threadIdx.x & ((1u << lgj) - 1));
+    const size_t t = ((size_t)blockIdx.x << lgj) + (size_t)lane;
     const int ms = (int)(threadIdx.x >> lgj);
-    if (t >= total || ms >= members) return;
+    const bool in = t < total;
     const size_t k = t >> __builtin_ctzll(N), j = t & (N - 1);
-    const size_t src = perm ? k * N + perm[j] : t;
     const uint32_t pi = qlp_prime(k, Ql, nQ);
-    // digits g0 .. g0 + nb - 1 of the beta of each member (nb <= MAXB)
-    uint64_t kb[MAXB], ka[MAXB];
+    // digits g0 .. g0 + nb - 1 of the beta of each member (nb <= MAXB). Member slot 0
+    // loads the 2 nb key words of its coefficient once and leaves them in LDS for the
+    // other slots: 16 bytes per digit and coefficient, both words split into 26-bit
+    // halves (P52) or as they are.
+    __shared__ uint4 kl[MAXB * 256];
+    if (ms == 0 && in) {
 #pragma unroll
-    for (int g = 0; g < MAXB; g++) {
-        kb[g] = ka[g] = 0;
-        if (g < nb) {
-            const uint64_t *kg = key + (size_t)(g0 + g) * 2 * QP * N;
-            kb[g] = kg[(size_t)pi * N + j];
-            ka[g] = kg[(QP + pi) * N + j];
+        for (int g = 0; g < MAXB; g++) {
+            if (g < nb) {
+                const uint64_t *kg = key + (size_t)(g0 + g) * 2 * QP * N;
+                const uint64_t wb = kg[(size_t)pi * N + j], wa = kg[(QP + pi) * N + j];
+                uint4 w;
+                if (P52) {
+                    const Split26 bs = split26(wb), as = split26(wa);
+                    w = make_uint4(bs.lo, bs.hi, as.lo, as.hi);
+                } else {
+                    w = make_uint4((uint32_t)wb, (uint32_t)(wb >> 32), (uint32_t)wa, (uint32_t)(wa >> 32));
+                }
+                kl[g * 256 + lane] = w;
+            }
         }
     }
-    Split26 kbs[MAXB], kas[MAXB];
-#pragma unroll
-    for (int g = 0; g < MAXB; g++) {
-        kbs[g] = split26(P52 ? kb[g] : 0);
-        kas[g] = split26(P52 ? ka[g] : 0);
-    }
+    __syncthreads();
+    if (!in || ms >= members) return;
+    const size_t src = perm ? k * N + perm[j] : t;
     const DevMod m = mods[pi];
     // the digit among g0 .. g0 + nb - 1 whose own limbs hold k (digit dg's own limbs
     // k in [dg alpha, (dg + 1) alpha) are c2's limbs), or -1
@@ -382,15 +391,16 @@ __global__ __launch_bounds__(256) void k_ks_inner_batch(const uint64_t *__restri
             for (int g = 0; g < MAXB; g++) {
                 if (g < nb) {
                     // the key words are the pre-split operand, the digit is split once
+                    const uint4 w = kl[g * 256 + lane];
                     const Split26 ds = split26(dv[g]);
-                    a0.s0 += (uint64_t)ds.lo * kbs[g].lo;
-                    a0.s1 += (uint64_t)ds.hi * kbs[g].lo;
-                    a0.s1 += (uint64_t)ds.lo * kbs[g].hi;
-                    a0.s2 += (uint64_t)ds.hi * kbs[g].hi;
-                    a1.s0 += (uint64_t)ds.lo * kas[g].lo;
-                    a1.s1 += (uint64_t)ds.hi * kas[g].lo;
-                    a1.s1 += (uint64_t)ds.lo * kas[g].hi;
-                    a1.s2 += (uint64_t)ds.hi * kas[g].hi;
+                    a0.s0 += (uint64_t)ds.lo * w.x;
+                    a0.s1 += (uint64_t)ds.hi * w.x;
+                    a0.s1 += (uint64_t)ds.lo * w.y;
+                    a0.s2 += (uint64_t)ds.hi * w.y;
+                    a1.s0 += (uint64_t)ds.lo * w.z;
+                    a1.s1 += (uint64_t)ds.hi * w.z;
+                    a1.s1 += (uint64_t)ds.lo * w.w;
+                    a1.s2 += (uint64_t)ds.hi * w.w;
                 }
             }
             v0 = reduce52(a0, m);
@@ -400,8 +410,9 @@ __global__ __launch_bounds__(256) void k_ks_inner_batch(const uint64_t *__restri
 #pragma unroll
             for (int g = 0; g < MAXB; g++) {
                 if (g < nb) {
-                    mac128(a0, dv[g], kb[g]);
-                    mac128(a1, dv[g], ka[g]);
+                    const uint4 w = kl[g * 256 + lane];
+                    mac128(a0, dv[g], ((uint64_t)w.y << 32) | w.x);
+                    mac128(a1, dv[g], ((uint64_t)w.w << 32) | w.z);
                 }
             }
             v0 = reduce128(a0, m);
