@@ -285,8 +285,10 @@ struct NttIO {
 // the 256 threads of a block read and write 256 consecutive words per row.
 // FWD: forward stages 0..logN1-1 (output stays lazy).
 // !FWD: inverse stages logN1-1..0, then x N^-1, output canonical.
+// At most 80 VGPRs (6 waves per SIMD): the inverse form would otherwise take 99. The
+// encoder's form keeps its registers (it would spill).
 template <int LOGT, bool FWD, int IO, bool LAZY>
-__global__ __launch_bounds__(kThreads) void k_ntt_cols(uint64_t *__restrict__ data, LimbMap mp, NttTables tb,
+__global__ __launch_bounds__(kThreads, IO == IO_ENCODE ? 1 : 6) void k_ntt_cols(uint64_t *__restrict__ data, LimbMap mp, NttTables tb,
                                                        int logN, NttIO io) {
     constexpr int T = 1 << LOGT;
     const size_t N = size_t(1) << logN;
