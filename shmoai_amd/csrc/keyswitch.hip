@@ -186,27 +186,45 @@ __global__ __launch_bounds__(256) void k_ks_inner_multi(const uint64_t *__restri
 #pragma unroll
     for (int g = 0; g < MAXB; g++) ds[g] = split26(P52 ? dv[g] : 0);
     const DevMod m = mods[pi];
-    for (int o = os; o < nout; o += 1 << lgog) {
-        const uint64_t *kb = kr.base[o];
-        const size_t st = kr.stride[o];
-        uint64_t kv[MAXB];
+    // two outputs per step: their 2 ndig key loads are in flight together
+    const int step = 1 << lgog;
+    for (int o = os; o < nout; o += 2 * step) {
+        const int o2 = o + step;
+        const bool two = o2 < nout;
+        const uint64_t *kb = kr.base[o], *kb2 = kr.base[two ? o2 : o];
+        const size_t st = kr.stride[o], st2 = kr.stride[two ? o2 : o];
+        uint64_t kv[MAXB], kv2[MAXB];
 #pragma unroll
-        for (int g = 0; g < MAXB; g++) kv[g] = g < ndig ? __builtin_nontemporal_load(&kb[(size_t)g * st + kl]) : 0;
-        uint64_t v;
+        for (int g = 0; g < MAXB; g++) {
+            kv[g] = g < ndig ? __builtin_nontemporal_load(&kb[(size_t)g * st + kl]) : 0;
+            kv2[g] = g < ndig && two ? __builtin_nontemporal_load(&kb2[(size_t)g * st2 + kl]) : 0;
+        }
+        uint64_t v, v2;
         if (P52) {
-            Acc52 a{0, 0, 0};
+            Acc52 a{0, 0, 0}, a2{0, 0, 0};
 #pragma unroll
             for (int g = 0; g < MAXB; g++)
-                if (g < ndig) mac52(a, ds[g], kv[g]);
+                if (g < ndig) {
+                    mac52(a, ds[g], kv[g]);
+                    mac52(a2, ds[g], kv2[g]);
+                }
             v = reduce52(a, m);
+            v2 = reduce52(a2, m);
         } else {
-            Acc128 a{0, 0};
+            Acc128 a{0, 0}, a2{0, 0};
 #pragma unroll
             for (int g = 0; g < MAXB; g++)
-                if (g < ndig) mac128(a, dv[g], kv[g]);
+                if (g < ndig) {
+                    mac128(a, dv[g], kv[g]);
+                    mac128(a2, dv[g], kv2[g]);
+                }
             v = reduce128(a, m);
+            v2 = reduce128(a2, m);
         }
         __builtin_nontemporal_store(accumulate ? add_mod(cx[(size_t)o * total + t], v, m.q) : v, &cx[(size_t)o * total + t]);
+        if (two)
+            __builtin_nontemporal_store(accumulate ? add_mod(cx[(size_t)o2 * total + t], v2, m.q) : v2,
+                                        &cx[(size_t)o2 * total + t]);
     }
 }
 
