@@ -396,21 +396,46 @@ __global__ __launch_bounds__(256) void k_ks_inner_batch(const uint64_t *__restri
     // the digit among g0 .. g0 + nb - 1 whose own limbs hold k (digit dg's own limbs
     // k in [dg alpha, (dg + 1) alpha) are c2's limbs), or -1
     const int own_g = own_c2 && k < Ql ? (int)(k / alpha) - g0 : -1;
-    for (int mem = ms; mem < members; mem += 1 << lgmg) {
+    // accumulate, or add P (d0, d1) on the data limbs (the merged ModDown + rescale
+    // divides P d + cx by P q_last in one step), then store member mem's two outputs
+    auto finish_member = [&](int mem, uint64_t v0, uint64_t v1) {
+        uint64_t *o0 = cx + (size_t)(2 * mem) * total + t, *o1 = cx + (size_t)(2 * mem + 1) * total + t;
+        if (accumulate) {
+            v0 = add_mod(v0, *o0, m.q);
+            v1 = add_mod(v1, *o1, m.q);
+        } else if (add_on && k < Ql) {
+            const uint64_t pm = pmod[pi], pms = pmod_s[pi];
+            v0 = add_mod(v0, mul_shoup(addp.p[2 * mem][k * N + j], pm, pms, m.q), m.q);
+            v1 = add_mod(v1, mul_shoup(addp.p[2 * mem + 1][k * N + j], pm, pms, m.q), m.q);
+        }
+        __builtin_nontemporal_store(v0, o0);
+        __builtin_nontemporal_store(v1, o1);
+    };
+    // two members per step: their digit loads are in flight together
+    const int mstep = 1 << lgmg;
+    for (int mem = ms; mem < members; mem += 2 * mstep) {
+        const int mem2 = mem + mstep;
+        const bool two = mem2 < members;
+        const int m2 = two ? mem2 : mem;
         const uint64_t *dm = d + ((size_t)mem * beta + g0) * total + src;
-        uint64_t dv[MAXB];
+        const uint64_t *dm2 = d + ((size_t)m2 * beta + g0) * total + src;
+        uint64_t dv[MAXB], dv2[MAXB];
 #pragma unroll
-        for (int g = 0; g < MAXB; g++)
+        for (int g = 0; g < MAXB; g++) {
             dv[g] = g < nb ? (g == own_g ? c2.p[mem][src] : __builtin_nontemporal_load(&dm[(size_t)g * total])) : 0;
-        uint64_t v0, v1;
+            dv2[g] = g < nb && two
+                         ? (g == own_g ? c2.p[m2][src] : __builtin_nontemporal_load(&dm2[(size_t)g * total]))
+                         : 0;
+        }
+        uint64_t v0, v1, w0, w1;
         if (P52) {
-            Acc52 a0{0, 0, 0}, a1{0, 0, 0};
+            Acc52 a0{0, 0, 0}, a1{0, 0, 0}, b0{0, 0, 0}, b1{0, 0, 0};
 #pragma unroll
             for (int g = 0; g < MAXB; g++) {
                 if (g < nb) {
-                    // the key words are the pre-split operand, the digit is split once
+                    // the key words are the pre-split operand, the digits are split once
                     const uint4 w = kl[g * 256 + lane];
-                    const Split26 ds = split26(dv[g]);
+                    const Split26 ds = split26(dv[g]), es = split26(dv2[g]);
                     a0.s0 += (uint64_t)ds.lo * w.x;
                     a0.s1 += (uint64_t)ds.hi * w.x;
                     a0.s1 += (uint64_t)ds.lo * w.y;
@@ -419,36 +444,40 @@ __global__ __launch_bounds__(256) void k_ks_inner_batch(const uint64_t *__restri
                     a1.s1 += (uint64_t)ds.hi * w.z;
                     a1.s1 += (uint64_t)ds.lo * w.w;
                     a1.s2 += (uint64_t)ds.hi * w.w;
+                    b0.s0 += (uint64_t)es.lo * w.x;
+                    b0.s1 += (uint64_t)es.hi * w.x;
+                    b0.s1 += (uint64_t)es.lo * w.y;
+                    b0.s2 += (uint64_t)es.hi * w.y;
+                    b1.s0 += (uint64_t)es.lo * w.z;
+                    b1.s1 += (uint64_t)es.hi * w.z;
+                    b1.s1 += (uint64_t)es.lo * w.w;
+                    b1.s2 += (uint64_t)es.hi * w.w;
                 }
             }
             v0 = reduce52(a0, m);
             v1 = reduce52(a1, m);
+            w0 = reduce52(b0, m);
+            w1 = reduce52(b1, m);
         } else {
-            Acc128 a0{0, 0}, a1{0, 0};
+            Acc128 a0{0, 0}, a1{0, 0}, b0{0, 0}, b1{0, 0};
 #pragma unroll
             for (int g = 0; g < MAXB; g++) {
                 if (g < nb) {
                     const uint4 w = kl[g * 256 + lane];
-                    mac128(a0, dv[g], ((uint64_t)w.y << 32) | w.x);
-                    mac128(a1, dv[g], ((uint64_t)w.w << 32) | w.z);
+                    const uint64_t kbw = ((uint64_t)w.y << 32) | w.x, kaw = ((uint64_t)w.w << 32) | w.z;
+                    mac128(a0, dv[g], kbw);
+                    mac128(a1, dv[g], kaw);
+                    mac128(b0, dv2[g], kbw);
+                    mac128(b1, dv2[g], kaw);
                 }
             }
             v0 = reduce128(a0, m);
             v1 = reduce128(a1, m);
+            w0 = reduce128(b0, m);
+            w1 = reduce128(b1, m);
         }
-        uint64_t *o0 = cx + (size_t)(2 * mem) * total + t, *o1 = cx + (size_t)(2 * mem + 1) * total + t;
-        if (accumulate) {
-            v0 = add_mod(v0, *o0, m.q);
-            v1 = add_mod(v1, *o1, m.q);
-        } else if (add_on && k < Ql) {
-            // + P (d0, d1) on the data limbs: the merged ModDown + rescale
-            // divides P d + cx by P q_last in one step
-            const uint64_t pm = pmod[pi], pms = pmod_s[pi];
-            v0 = add_mod(v0, mul_shoup(addp.p[2 * mem][k * N + j], pm, pms, m.q), m.q);
-            v1 = add_mod(v1, mul_shoup(addp.p[2 * mem + 1][k * N + j], pm, pms, m.q), m.q);
-        }
-        __builtin_nontemporal_store(v0, o0);
-        __builtin_nontemporal_store(v1, o1);
+        finish_member(mem, v0, v1);
+        if (two) finish_member(mem2, w0, w1);
     }
 }
 
