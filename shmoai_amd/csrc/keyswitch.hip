@@ -115,9 +115,16 @@ __global__ void k_modup_bconv(const uint64_t *__restrict__ tcoef_all, uint64_t *
             } else if (P52) {
                 Acc52 acc{0, 0, 0};
                 const uint64_t *hm = hat_mod + (g * alpha) * QlP + k;
+                if (size == (size_t)A) {
+                    // a full digit (every digit but possibly the last): no per-term tests,
+                    // so the term constants are fetched together
 #pragma unroll
-                for (int i = 0; i < A; i++)
-                    if ((size_t)i < size) mac52(acc, ys[i], hm[i * QlP]);
+                    for (int i = 0; i < A; i++) mac52(acc, ys[i], hm[i * QlP]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < A; i++)
+                        if ((size_t)i < size) mac52(acc, ys[i], hm[i * QlP]);
+                }
                 if (rounded) mac52(acc, split26(mp.q - r), dig_mod[g * QlP + k]);
                 val = reduce52(acc, mp);
             } else {
@@ -263,9 +270,14 @@ __global__ void k_moddown_bconv(const uint64_t *__restrict__ cx, uint64_t *__res
             uint64_t val;
             if (P52 && nP > 1) {
                 Acc52 acc{0, 0, 0};
+                if (nP == (size_t)A) {
 #pragma unroll
-                for (int jj = 0; jj < A; jj++)
-                    if ((size_t)jj < nP) mac52(acc, ys[jj], phat_mod[i * nP + jj]);
+                    for (int jj = 0; jj < A; jj++) mac52(acc, ys[jj], phat_mod[i * nP + jj]);
+                } else {
+#pragma unroll
+                    for (int jj = 0; jj < A; jj++)
+                        if ((size_t)jj < nP) mac52(acc, ys[jj], phat_mod[i * nP + jj]);
+                }
                 if (rounded) mac52(acc, split26(mq.q - r), pmod[i]);
                 val = reduce52(acc, mq);
             } else if (nP == 1) {
@@ -320,9 +332,14 @@ __global__ void k_moddown_rescale_bconv(const uint64_t *__restrict__ cx, uint64_
             const DevMod mq = mods[i];
             if (P52) {
                 Acc52 acc{0, 0, 0};
+                if (nb == (size_t)A + 1) {
 #pragma unroll
-                for (int b = 0; b < A + 1; b++)
-                    if ((size_t)b < nb) mac52(acc, ys[b], bhat_mod[i * nb + b]);
+                    for (int b = 0; b < A + 1; b++) mac52(acc, ys[b], bhat_mod[i * nb + b]);
+                } else {
+#pragma unroll
+                    for (int b = 0; b < A + 1; b++)
+                        if ((size_t)b < nb) mac52(acc, ys[b], bhat_mod[i * nb + b]);
+                }
                 mac52(acc, split26(mq.q - r), bmod[i]);
                 __builtin_nontemporal_store(reduce52(acc, mq), &d[i * N]);
                 continue;
