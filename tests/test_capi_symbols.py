@@ -48,3 +48,26 @@ def test_cpp_facade_header_compiles(tmp_path):
                         "-I", os.path.join(root, "include"), os.path.join(root, "tests", "cpp", "facade_c1.cpp")],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_tuning_knobs_validate():
+    """The process-wide knobs (no GPU work): defaults, accepted values, and refusals."""
+    import pytest
+
+    import shmoai_amd as shm
+    from shmoai_amd.capi import ShmInvalidArgument
+    assert shm.get_tuning("ntt_tile_threads") == 512
+    assert shm.get_tuning("matmul_overlap") == 0
+    assert shm.get_tuning("ks_batch_slots_log2") == -1
+    try:
+        shm.set_tuning("ntt_tile_threads", 256)
+        assert shm.get_tuning("ntt_tile_threads") == 256
+    finally:
+        shm.set_tuning("ntt_tile_threads", 512)
+    for name, value in (("ntt_tile_threads", 384), ("matmul_overlap", 2), ("ks_batch_slots_log2", 5),
+                        ("ks_multi_slots_log2", -2), ("no_such_knob", 0)):
+        with pytest.raises(ShmInvalidArgument):
+            shm.set_tuning(name, value)
+    with pytest.raises(ShmInvalidArgument):
+        shm.get_tuning("no_such_knob")
+    assert shm.get_tuning("ntt_tile_threads") == 512
