@@ -306,14 +306,19 @@ constexpr int kPolyPack = 64;
 struct PolyPack {
     uint64_t *p[kPolyPack];
 };
-// The same rescale over n_polys polynomials in separate buffers: input p at
-// in.p[p] ([L1 + 1][N]), output p at out.p[p] ([L1][N]).
 // Forward NTT of the ModDown correction delta (n_polys x Ql limbs, in place)
 // whose last pass writes (cx - NTT(delta)) * pinv to out.p[p] (adding onto it
 // when bit p of acc_mask is set); cx holds cx_limbs limbs per polynomial.
 void moddown_ntt_fused(const Context &c, uint64_t *delta, const uint64_t *cx, size_t Ql, size_t cx_limbs,
                        const PolyPack &out, size_t n_polys, uint64_t acc_mask, const uint64_t *pinv,
                        const uint64_t *pinv_s, hipStream_t s);
+// ntt_inverse of polynomials held in separate buffers: limb k of polynomial p is read
+// from in.p[p] + k N (n_limbs limbs each, at most 64 polynomials) and the result is
+// written to out ([n_polys][n_limbs][N]); the inputs are left as they are.
+void ntt_inverse_gather(const Context &c, const PolyPack &in, uint64_t *out, size_t n_polys, size_t n_limbs,
+                        size_t first_prime, hipStream_t s);
+// The fused rescale over n_polys polynomials in separate buffers: input p at
+// in.p[p] ([L1 + 1][N]), output p at out.p[p] ([L1][N]).
 void rescale_ntt_fused_pack(const Context &c, const uint64_t *last, const PolyPack &in, const PolyPack &out,
                             uint64_t *delta, size_t n_polys, size_t L1, const uint64_t *qinv, const uint64_t *qinv_s,
                             hipStream_t s, bool rounded);

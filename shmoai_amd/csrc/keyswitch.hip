@@ -743,10 +743,9 @@ void ks_modup_batch(const Context &c, const PolyPack &c2, size_t m, size_t Ql, u
     if (beta > c.dnum) throw LogicError("key switching key has too few digits for this level");
     if (alpha > (size_t)kMaxAlpha) throw InvalidArgument("at most 16 special primes");
     bump(g_cnt_modup, Ql, m);
-    // gather + iNTT, conversion, one NTT over m beta QlP limbs
+    // iNTT of the members into one buffer, conversion, NTT over m beta QlP limbs
     Scratch tcoef(m * Ql * N * sizeof(uint64_t), s);
-    launch_gather_polys(c, c2, tcoef.as<uint64_t>(), Ql * N, m, s);
-    ntt_inverse(c, tcoef.as<uint64_t>(), m, Ql, 0, s);
+    ntt_inverse_gather(c, c2, tcoef.as<uint64_t>(), m, Ql, 0, s);
     // From 4 members on, each digit gets its own NTT launch over the m members that
     // skips the digit's own limbs: they are c2's NTT-form words, which the inner
     // product reads from c2. With fewer members such launches are too small to

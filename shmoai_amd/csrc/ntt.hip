@@ -256,11 +256,14 @@ __device__ __forceinline__ void run_rounds(E *lds, uint32_t gmul, uint32_t gadd_
 //                out_packed, polynomial p reads acc at its acc_limbs stride and
 //                writes out_p[p], adding onto it when bit p of acc_mask is set
 //                (the ModDown combine, rns_bconv.cu:680-690 and 776-828, fused the same way).
+//   IO_GATHER  : (phase B, inverse) limb k of polynomial p is read from
+//                acc_p.p[p] + k N (separate input buffers) and the transform is
+//                written to `data`: the gather copy fused into the first pass.
 //   IO_ENCODE  : (phase A, forward) the input of limb k of polynomial p is the
 //                rounded coefficient of the encoder's FFT output fft[p] (index
 //                i < N/2: round(re), i >= N/2: round(im) of entry i - N/2) as a
 //                residue, i.e. the encoder's decompose step fused into the load.
-enum { IO_PLAIN = 0, IO_REDUCE = 1, IO_RESCALE = 2, IO_ENCODE = 3 };
+enum { IO_PLAIN = 0, IO_REDUCE = 1, IO_RESCALE = 2, IO_ENCODE = 3, IO_GATHER = 4 };
 
 struct NttIO {
     const uint64_t *src;      // IO_REDUCE: [n_polys][N]
@@ -276,7 +279,7 @@ struct NttIO {
     uint32_t out_packed;      // IO_RESCALE: nonzero -> polynomial p reads acc, writes out_p[p]
     uint64_t acc_mask;        // IO_RESCALE with out_packed: bit p set -> out_p[p] += result
     uint32_t out_limbs;       // IO_RESCALE with outs: limbs per output polynomial (0: the launch's)
-    PolyPack acc_p, out_p;
+    PolyPack acc_p, out_p;    // IO_RESCALE with packed / out_packed; IO_GATHER inputs in acc_p
     const double2 *fft;       // IO_ENCODE: [n_polys][N/2]
 };
 
@@ -374,9 +377,10 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 6 : 1) void k_ntt_blocks(uint64_t *
     // block b (global index t0/T + b) has group id N1 + b at its first stage
     const uint32_t N1 = (uint32_t)(N >> LOGT);
     const uint64_t q = tb.mods[prime].q, q2 = q << 1;
+    const uint64_t *ld = IO == IO_GATHER ? io.acc_p.p[poly] + (size_t)k * N + t0 : base;
     for (int i = threadIdx.x; i < kTile; i += NT) {
         const int b = i / T, r = i % T;
-        lds[b * TP + swz<T>(r)] = base[i];
+        lds[b * TP + swz<T>(r)] = ld[i];
     }
     // rescale: the accumulator words are loaded before the last round, so they
     // arrive during it (loading them before all rounds cost occupancy)
@@ -654,6 +658,17 @@ void ntt_inverse(const Context &c, uint64_t *data, size_t n_polys, size_t n_limb
     LimbMap mp{(uint32_t)n_limbs, 0, 0xffffffffu, 0, 0xffffffffu, (uint32_t)c.nQ, (uint32_t)first_prime,
                (uint64_t)n_limbs * c.N};
     launch_ntt<false, IO_PLAIN, IO_PLAIN>(c, data, mp, n_polys * n_limbs, s);
+}
+
+void ntt_inverse_gather(const Context &c, const PolyPack &in, uint64_t *out, size_t n_polys, size_t n_limbs,
+                        size_t first_prime, hipStream_t s) {
+    if (n_polys == 0) return;
+    if (n_polys > (size_t)kPolyPack) throw InvalidArgument("inverse NTT: 1..64 polynomials per batch");
+    LimbMap mp{(uint32_t)n_limbs, 0, 0xffffffffu, 0, 0xffffffffu, (uint32_t)c.nQ, (uint32_t)first_prime,
+               (uint64_t)n_limbs * c.N};
+    NttIO io{};
+    io.acc_p = in;
+    launch_ntt<false, IO_PLAIN, IO_GATHER>(c, out, mp, n_polys * n_limbs, s, io);
 }
 
 void ntt_forward_qlp(const Context &c, uint64_t *data, size_t Ql, size_t lo, size_t hi, size_t skip_lo,

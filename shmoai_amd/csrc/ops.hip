@@ -431,8 +431,7 @@ void rescale_batch(Context &c, const std::vector<Ciphertext *> &cts, hipStream_t
     }
     Scratch last(np * N * sizeof(uint64_t), s);
     Scratch delta(np * L1 * N * sizeof(uint64_t), s);
-    launch_gather_polys(c, lastp, last.as<uint64_t>(), N, np, s);
-    ntt_inverse(c, last.as<uint64_t>(), np, 1, L1, s);
+    ntt_inverse_gather(c, lastp, last.as<uint64_t>(), np, 1, L1, s);
     const LevelConsts &L = c.level(Ql);
     rescale_ntt_fused_pack(c, last.as<uint64_t>(), in, out, delta.as<uint64_t>(), np, L1,
                            L.qlast_inv.as<uint64_t>(), L.qlast_inv_s.as<uint64_t>(), s, rounded);
