@@ -311,7 +311,8 @@ struct PolyPack {
 // when bit p of acc_mask is set); cx holds cx_limbs limbs per polynomial.
 void moddown_ntt_fused(const Context &c, uint64_t *delta, const uint64_t *cx, size_t Ql, size_t cx_limbs,
                        const PolyPack &out, size_t n_polys, uint64_t acc_mask, const uint64_t *pinv,
-                       const uint64_t *pinv_s, hipStream_t s);
+                       const uint64_t *pinv_s, hipStream_t s, const PolyPack *add_src = nullptr,
+                       const uint32_t *add_perm = nullptr);
 // ntt_inverse of polynomials held in separate buffers: limb k of polynomial p is read
 // from in.p[p] + k N (n_limbs limbs each, at most 64 polynomials) and the result is
 // written to out ([n_polys][n_limbs][N]); the inputs are left as they are.
@@ -409,9 +410,12 @@ void ks_inner_multi(const Context &c, const uint64_t *d, size_t Ql, size_t ndig,
 // ModDown of npolys [QlP][N] polynomials into out + p * out_stride (bit p of acc_mask: accumulate)
 void ks_moddown(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, uint64_t *out, size_t out_stride,
                 uint64_t acc_mask, hipStream_t s, bool rounded = false);
-// The same with polynomial p written to out.p[p]
+// The same with polynomial p written to out.p[p]. With add_src and perm, a polynomial p
+// whose acc_mask bit is set gets add_src.p[p] read through the NTT-domain permutation
+// perm added instead of out.p[p] (a Galois permutation fused into the accumulation).
 void ks_moddown_pack(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, const PolyPack &out,
-                     uint64_t acc_mask, hipStream_t s, bool rounded = false);
+                     uint64_t acc_mask, hipStream_t s, bool rounded = false, const PolyPack *add_src = nullptr,
+                     const uint32_t *perm = nullptr);
 // ModUp of m polynomials c2.p[i] (Ql limbs, NTT) into d [m][beta][QlP][N]; from 4
 // members on the digits' own limbs are not written (they are c2's limbs: pass c2
 // as `own` to ks_inner_batch, which reads them from there in every case)

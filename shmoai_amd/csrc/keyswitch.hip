@@ -629,8 +629,9 @@ void ks_modup(const Context &c, const uint64_t *c2, size_t Ql, uint64_t *d, hipS
     if (beta > c.dnum) throw LogicError("key switching key has too few digits for this level");
     bump(g_cnt_modup, Ql, 1);
     Scratch tcoef(Ql * N * sizeof(uint64_t), s);
-    launch_copy_words(tcoef.as<uint64_t>(), c2, Ql * N, s);
-    ntt_inverse(c, tcoef.as<uint64_t>(), 1, Ql, 0, s);
+    PolyPack in{};
+    in.p[0] = const_cast<uint64_t *>(c2);  // read only
+    ntt_inverse_gather(c, in, tcoef.as<uint64_t>(), 1, Ql, 0, s);
     if (alpha > (size_t)kMaxAlpha) throw InvalidArgument("at most 16 special primes");
     with_alpha_bound(alpha, [&](auto bound) {
         constexpr int A = decltype(bound)::value;
@@ -695,7 +696,7 @@ void ks_moddown(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, uint64
 }
 
 void ks_moddown_pack(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, const PolyPack &out,
-                     uint64_t acc_mask, hipStream_t s, bool rounded) {
+                     uint64_t acc_mask, hipStream_t s, bool rounded, const PolyPack *add_src, const uint32_t *perm) {
     const size_t N = c.N, nP = c.nP, QlP = Ql + nP;
     if (npolys == 0 || npolys > 64) throw InvalidArgument("ks_moddown: 1..64 polynomials");
     bump(g_cnt_moddown, Ql, npolys);
@@ -711,7 +712,7 @@ void ks_moddown_pack(const Context &c, uint64_t *cx, size_t npolys, size_t Ql, c
     SHM_HIP(hipGetLastError());
     // the last NTT pass of delta writes (cx - delta) * P^-1 into the outputs
     moddown_ntt_fused(c, delta.as<uint64_t>(), cx, Ql, QlP, out, npolys, acc_mask, c.pinv.as<uint64_t>(),
-                      c.pinv_s.as<uint64_t>(), s);
+                      c.pinv_s.as<uint64_t>(), s, add_src, perm);
 }
 
 void keyswitch(const Context &c, const uint64_t *c2, const uint64_t *key, size_t Ql, uint64_t *ct0, uint64_t *ct1,

@@ -280,6 +280,8 @@ struct NttIO {
     uint64_t acc_mask;        // IO_RESCALE with out_packed: bit p set -> out_p[p] += result
     uint32_t out_limbs;       // IO_RESCALE with outs: limbs per output polynomial (0: the launch's)
     PolyPack acc_p, out_p;    // IO_RESCALE with packed / out_packed; IO_GATHER inputs in acc_p
+    const uint32_t *add_perm; // IO_RESCALE with out_packed: add add_src.p[p] read through this
+    PolyPack add_src;         // NTT-domain permutation instead of out_p[p] (acc_mask bit p set)
     const double2 *fft;       // IO_ENCODE: [n_polys][N/2]
 };
 
@@ -429,7 +431,8 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 6 : 1) void k_ntt_blocks(uint64_t *
             const uint64_t raw = lds[b * TP + swz<T>(r)];
             uint64_t y = LAZY ? mul_shoup(accv[j] + kq - raw, qi, qis, q)
                               : mul_shoup(sub_mod(accv[j], canon(raw), q), qi, qis, q);
-            if (addto) y = add_mod(y, out[i], q);
+            if (addto)
+                y = add_mod(y, io.add_perm ? io.add_src.p[poly][(size_t)k * N + io.add_perm[t0 + i]] : out[i], q);
             __builtin_nontemporal_store(y, &out[i]);
         }
     } else if (FWD) {
@@ -727,8 +730,9 @@ void rescale_ntt_fused_window(const Context &c, const uint64_t *last, const uint
 
 void moddown_ntt_fused(const Context &c, uint64_t *delta, const uint64_t *cx, size_t Ql, size_t cx_limbs,
                        const PolyPack &out, size_t n_polys, uint64_t acc_mask, const uint64_t *pinv,
-                       const uint64_t *pinv_s, hipStream_t s) {
+                       const uint64_t *pinv_s, hipStream_t s, const PolyPack *add_src, const uint32_t *add_perm) {
     if (n_polys == 0 || n_polys > (size_t)kPolyPack) throw InvalidArgument("ModDown: 1..64 polynomials per batch");
+    if ((add_src == nullptr) != (add_perm == nullptr)) throw LogicError("ModDown: permuted addend needs both parts");
     LimbMap mp{(uint32_t)Ql, 0, 0xffffffffu, 0, 0xffffffffu, (uint32_t)c.nQ, 0, (uint64_t)Ql * c.N};
     NttIO io{};
     io.acc = cx;
@@ -738,6 +742,10 @@ void moddown_ntt_fused(const Context &c, uint64_t *delta, const uint64_t *cx, si
     io.out_packed = 1;
     io.acc_mask = acc_mask;
     io.out_p = out;
+    if (add_src) {
+        io.add_src = *add_src;
+        io.add_perm = add_perm;
+    }
     launch_ntt<true, IO_PLAIN, IO_RESCALE>(c, delta, mp, n_polys * Ql, s, io);
 }
 

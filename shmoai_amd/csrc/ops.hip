@@ -174,13 +174,17 @@ void rotate_hoisted_group(Context &c, const Ciphertext &g, const std::vector<int
         const uint32_t *table = c.galois_table(elt, s);
         ks_inner_batch(c, d.as<uint64_t>(), m, Ql, it->second->k.as<uint64_t>(), cx.as<uint64_t>(), s, table, &a);
         o.ctx = &c;
-        o.resize(g.size, g.chain, s, false);  // permute writes every b_i, ModDown every a_i
+        o.resize(g.size, g.chain, s, false);  // the fused ModDown writes every word
         o.scale = g.scale;
         o.ntt = true;
-        launch_permute(c, g.data(), o.data(), table, g.size * Ql, s);
-        PolyPack op{};
-        for (size_t k = 0; k < g.size; k++) op.p[k] = o.poly(k);
-        ks_moddown_pack(c, cx.as<uint64_t>(), g.size, Ql, op, even_bits(g.size), s, rounded);
+        // ModDown's last pass adds each b_i read through the Galois permutation (no
+        // permuted copy in between) and writes each a_i
+        PolyPack op{}, src{};
+        for (size_t k = 0; k < g.size; k++) {
+            op.p[k] = o.poly(k);
+            src.p[k] = g.poly(k);
+        }
+        ks_moddown_pack(c, cx.as<uint64_t>(), g.size, Ql, op, even_bits(g.size), s, rounded, &src, table);
     }
 }
 
@@ -212,11 +216,15 @@ void rotate_hoisted(Context &c, const Ciphertext &ct, const std::vector<int> &st
         kr.stride[1] = 2 * QP * N;
         ks_inner_multi(c, d.as<uint64_t>(), Ql, beta, kr, 2, false, cx.as<uint64_t>(), s, table);
         o.ctx = &c;
-        o.resize(2, ct.chain, s, false);  // permute writes c0, ModDown writes c1
+        o.resize(2, ct.chain, s, false);  // the fused ModDown writes every word
         o.scale = ct.scale;
         o.ntt = true;
-        launch_permute(c, ct.poly(0), o.poly(0), table, Ql, s);
-        ks_moddown(c, cx.as<uint64_t>(), 2, Ql, o.data(), Ql * N, 1, s, rounded);
+        // c0 is added through the Galois permutation by ModDown's last pass
+        PolyPack op{}, src{};
+        op.p[0] = o.poly(0);
+        op.p[1] = o.poly(1);
+        src.p[0] = ct.poly(0);
+        ks_moddown_pack(c, cx.as<uint64_t>(), 2, Ql, op, 1, s, rounded, &src, table);
     }
 }
 

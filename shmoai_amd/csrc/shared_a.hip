@@ -363,11 +363,15 @@ void shared_rotate_hoisted(SharedAKeys &K, const Ciphertext &x, const std::vecto
         kr.stride[n] = QP * N;
         ks_inner_multi(c, d.as<uint64_t>(), Ql, beta, kr, n + 1, false, cx.as<uint64_t>(), s, table);
         o.ctx = &c;
-        o.resize(n + 1, x.chain, s);
+        o.resize(n + 1, x.chain, s, false);  // the fused ModDown writes every word
         o.scale = x.scale;
         o.ntt = true;
-        launch_permute(c, x.data(), o.data(), table, n * Ql, s);
-        ks_moddown(c, cx.as<uint64_t>(), n + 1, Ql, o.data(), Ql * N, (1ull << n) - 1, s, rounded);
+        // the members' b parts are added through the Galois permutation by the ModDown's
+        // last pass (no permuted copy in between)
+        PolyPack outp{}, srcp{};
+        for (size_t p = 0; p <= n; p++) outp.p[p] = o.poly(p);
+        for (size_t p = 0; p < n; p++) srcp.p[p] = x.poly(p);
+        ks_moddown_pack(c, cx.as<uint64_t>(), n + 1, Ql, outp, (1ull << n) - 1, s, rounded, &srcp, table);
     }
 }
 
