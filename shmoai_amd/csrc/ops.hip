@@ -64,6 +64,33 @@ void apply_galois(Context &c, Ciphertext &ct, uint32_t elt, const GaloisKeys &gk
     if (it == gk.keys.end()) throw InvalidArgument("Galois elt not present");
     const size_t Ql = ct.Ql(), N = c.N;
     const uint32_t *table = c.galois_table(elt, s);
+    if (rounded) {
+        // the exact centred ModUp commutes with the Galois permutation, so the
+        // unpermuted c1 is raised and its digits are read through the permutation,
+        // and c0 is added through it by ModDown's last pass (the hoisted form of one
+        // step: the same words, no permuted copies)
+        const size_t QlP = Ql + c.nP, QP = c.nQP(), beta = c.level(Ql).beta;
+        Scratch d(beta * QlP * N * sizeof(uint64_t), s), cx(2 * QlP * N * sizeof(uint64_t), s);
+        ks_modup(c, ct.poly(1), Ql, d.as<uint64_t>(), s, true);
+        KeyRefs kr{};
+        kr.base[0] = it->second->k.as<uint64_t>();
+        kr.stride[0] = 2 * QP * N;
+        kr.base[1] = kr.base[0] + QP * N;
+        kr.stride[1] = 2 * QP * N;
+        ks_inner_multi(c, d.as<uint64_t>(), Ql, beta, kr, 2, false, cx.as<uint64_t>(), s, table);
+        Ciphertext o;
+        o.ctx = &c;
+        o.resize(2, ct.chain, s, false);  // the fused ModDown writes every word
+        o.scale = ct.scale;
+        o.ntt = true;
+        PolyPack op{}, src{};
+        op.p[0] = o.poly(0);
+        op.p[1] = o.poly(1);
+        src.p[0] = ct.poly(0);
+        ks_moddown_pack(c, cx.as<uint64_t>(), 2, Ql, op, 1, s, true, &src, table);
+        ct = std::move(o);
+        return;
+    }
     Scratch t0(Ql * N * sizeof(uint64_t), s), t1(Ql * N * sizeof(uint64_t), s);
     launch_permute(c, ct.poly(0), t0.as<uint64_t>(), table, Ql, s);
     launch_permute(c, ct.poly(1), t1.as<uint64_t>(), table, Ql, s);
@@ -113,6 +140,30 @@ void apply_galois_group(Context &c, Ciphertext &g, uint32_t elt, const GaloisKey
     if (it == gk.keys.end()) throw InvalidArgument("Galois elt not present");
     const size_t m = g.size / 2, Ql = g.Ql(), N = c.N;
     const uint32_t *table = c.galois_table(elt, s);
+    if (rounded) {
+        // as in apply_galois: the group's unpermuted a_i are raised once, the inner
+        // product reads their digits through the permutation, and ModDown's last pass
+        // adds every b_i through it (the hoisted group form of one step)
+        const size_t QlP = Ql + c.nP, beta = c.level(Ql).beta;
+        Scratch d(m * beta * QlP * N * sizeof(uint64_t), s), cx(g.size * QlP * N * sizeof(uint64_t), s);
+        PolyPack a{};
+        for (size_t k = 0; k < m; k++) a.p[k] = g.poly(2 * k + 1);
+        ks_modup_batch(c, a, m, Ql, d.as<uint64_t>(), s, true);
+        ks_inner_batch(c, d.as<uint64_t>(), m, Ql, it->second->k.as<uint64_t>(), cx.as<uint64_t>(), s, table, &a);
+        Ciphertext o;
+        o.ctx = &c;
+        o.resize(g.size, g.chain, s, false);  // the fused ModDown writes every word
+        o.scale = g.scale;
+        o.ntt = true;
+        PolyPack op{}, src{};
+        for (size_t k = 0; k < g.size; k++) {
+            op.p[k] = o.poly(k);
+            src.p[k] = g.poly(k);
+        }
+        ks_moddown_pack(c, cx.as<uint64_t>(), g.size, Ql, op, even_bits(g.size), s, true, &src, table);
+        g = std::move(o);
+        return;
+    }
     Scratch t(g.size * Ql * N * sizeof(uint64_t), s);
     launch_permute(c, g.data(), t.as<uint64_t>(), table, g.size * Ql, s);
     // b_i := sigma(b_i); a_i is overwritten by the switched part below

@@ -309,6 +309,32 @@ void shared_apply_galois(SharedAKeys &K, SharedABatch &x, uint32_t elt, hipStrea
     const size_t n = x.n, N = c.N, Ql = x.ct.Ql(), QlP = Ql + c.nP, QP = c.nQP(), beta = c.level(Ql).beta;
     if (x.ct.size != n + 1 || n != K.n) throw InvalidArgument("shared_a_rotate: batch size does not match the Galois key");
     const uint32_t *table = c.galois_table(elt, s);
+    if (rounded) {
+        // the exact centred ModUp commutes with the Galois permutation: raise the
+        // unpermuted a, read its digits through the permutation, and add the members'
+        // b parts through it in ModDown's last pass (the hoisted form of one step)
+        Scratch d(beta * QlP * N * sizeof(uint64_t), s), cx((n + 1) * QlP * N * sizeof(uint64_t), s);
+        ks_modup(c, x.ct.poly(n), Ql, d.as<uint64_t>(), s, true);
+        KeyRefs kr{};
+        for (size_t i = 0; i < n; i++) {
+            kr.base[i] = key.B.as<uint64_t>() + i * QP * N;
+            kr.stride[i] = n * QP * N;
+        }
+        kr.base[n] = key.A.as<uint64_t>();
+        kr.stride[n] = QP * N;
+        ks_inner_multi(c, d.as<uint64_t>(), Ql, beta, kr, n + 1, false, cx.as<uint64_t>(), s, table);
+        Ciphertext o;
+        o.ctx = &c;
+        o.resize(n + 1, x.ct.chain, s, false);  // the fused ModDown writes every word
+        o.scale = x.ct.scale;
+        o.ntt = true;
+        PolyPack outp{}, srcp{};
+        for (size_t p = 0; p <= n; p++) outp.p[p] = o.poly(p);
+        for (size_t p = 0; p < n; p++) srcp.p[p] = x.ct.poly(p);
+        ks_moddown_pack(c, cx.as<uint64_t>(), n + 1, Ql, outp, (1ull << n) - 1, s, true, &srcp, table);
+        x.ct = std::move(o);
+        return;
+    }
     Ciphertext rot;
     rot.ctx = &c;
     rot.resize(n + 1, x.ct.chain, s);
