@@ -192,7 +192,10 @@ __device__ __forceinline__ void lds_round(E *lds, int s0, uint32_t gmul, uint32_
         const int kk = w2 / ups;
         const int o = w2 - kk * ups;
         // group id of the whole phase for this sub-transform, shifted by s0
-        const uint32_t G = ((gmul + gadd0 + gadd_step * (uint32_t)sub) << s0) + (uint32_t)kk;
+        uint32_t G = ((gmul + gadd0 + gadd_step * (uint32_t)sub) << s0) + (uint32_t)kk;
+        // one sub-transform of at least 64 units: the 64 lanes of a wave share kk,
+        // so G is wave-uniform and the twiddles come through scalar loads
+        if (SUBS == 1 && ups >= 64) G = __builtin_amdgcn_readfirstlane(G);
         E *row = lds + sub * TP;
         const int i0 = kk * T0 + o;
         E x[1 << R];
@@ -359,13 +362,26 @@ __global__ __launch_bounds__(kThreads, IO == IO_ENCODE ? 1 : 6) void k_ntt_cols(
 // O = (limb * tiles + tile) * n_polys + poly: the polynomials that share one
 // prime and tile (hence one twiddle slice) run back to back on the same L2.
 // 512-thread blocks ask for 6 waves per SIMD (80 VGPRs), three 32 KB tiles per CU.
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+// Pointers the kernel reads out of device memory or selects between are known to
+// address global memory: the cast lets their accesses use global instead of flat
+// instructions (flat ones also hold the LDS wait counter).
+template <typename V>
+__device__ __forceinline__ __attribute__((address_space(1))) V *global_ptr(V *p) {
+    return (__attribute__((address_space(1))) V *)p;
+}
+template <typename V>
+__device__ __forceinline__ const __attribute__((address_space(1))) V *global_ptr(const V *p) {
+    return (const __attribute__((address_space(1))) V *)p;
+}
+
 template <int LOGT, bool FWD, int IO, bool LAZY, int NT = kThreads>
 __global__ __launch_bounds__(NT, NT >= 512 ? 6 : 1) void k_ntt_blocks(uint64_t *__restrict__ data, LimbMap mp, NttTables tb,
                                                          int logN, NttIO io, uint32_t n_polys, uint32_t per_xcd) {
     constexpr int T = 1 << LOGT;
     constexpr int BLKS = kTile / T;
     constexpr int TP = row_stride<T>();
-    __shared__ uint64_t lds[BLKS * TP];
+    __shared__ __align__(16) uint64_t lds[BLKS * TP];
     const size_t N = size_t(1) << logN;
     const uint32_t tiles = (uint32_t)(N / kTile);
     const uint32_t O = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
@@ -380,22 +396,80 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 6 : 1) void k_ntt_blocks(uint64_t *
     const uint32_t N1 = (uint32_t)(N >> LOGT);
     const uint64_t q = tb.mods[prime].q, q2 = q << 1;
     const uint64_t *ld = IO == IO_GATHER ? io.acc_p.p[poly] + (size_t)k * N + t0 : base;
-    for (int i = threadIdx.x; i < kTile; i += NT) {
-        const int b = i / T, r = i % T;
-        lds[b * TP + swz<T>(r)] = ld[i];
+    // LDS slot of tile word i
+    auto slot = [](int i) { return (i / T) * TP + swz<T>(i % T); };
+    // Word pair (i, i + 1), i even, to and from LDS. In a full-width tile the
+    // swizzle keeps the pair in one aligned 16-byte slot, swapped when bit 4 of i
+    // is set, so a lane moves it with one 16-byte LDS access (a wave covers 1 KB
+    // of consecutive slots: no bank conflicts).
+    auto lds_put = [&](int i, u64x2 v) {
+        if (T >= kTile) {
+            const bool sw = (i >> 4) & 1;
+            u64x2 o;
+            o.x = sw ? v.y : v.x;
+            o.y = sw ? v.x : v.y;
+            *reinterpret_cast<u64x2 *>(&lds[swz<T>(i) & ~1]) = o;
+        } else {
+            lds[slot(i)] = v.x;
+            lds[slot(i + 1)] = v.y;
+        }
+    };
+    auto lds_get = [&](int i) -> u64x2 {
+        u64x2 v;
+        if (T >= kTile) {
+            const bool sw = (i >> 4) & 1;
+            const u64x2 o = *reinterpret_cast<const u64x2 *>(&lds[swz<T>(i) & ~1]);
+            v.x = sw ? o.y : o.x;
+            v.y = sw ? o.x : o.y;
+        } else {
+            v.x = lds[slot(i)];
+            v.y = lds[slot(i + 1)];
+        }
+        return v;
+    };
+    // the tile comes in two words per lane (16-byte loads; the launcher checks the
+    // alignment). Plain outputs leave one word per lane: 16-byte stores measured slower.
+    constexpr int PAIRS = kTile / (2 * NT);  // pairs per lane
+#pragma unroll
+    for (int j = 0; j < PAIRS; j++) {
+        const int i = 2 * (threadIdx.x + j * NT);
+        lds_put(i, *reinterpret_cast<const u64x2 *>(ld + i));
     }
     // rescale: the accumulator words are loaded before the last round, so they
     // arrive during it (loading them before all rounds cost occupancy)
     constexpr int PER = kTile / NT;
     const uint64_t *accp = nullptr;
-    if (FWD && IO == IO_RESCALE)
+    uint64_t *out = nullptr;
+    if (FWD && IO == IO_RESCALE) {
         accp = io.packed ? io.acc_p.p[poly] + (size_t)k * N + t0
                          : io.acc + ((size_t)poly * io.acc_limbs + k) * N + t0;
+        if (io.packed || io.out_packed) {
+            out = io.out_p.p[poly] + (size_t)k * N + t0;
+        } else {
+            uint64_t *ob = io.outs ? io.outs[poly / io.ppo] : io.out_single;
+            out = ob + ((size_t)(poly % io.ppo) * (io.out_limbs ? io.out_limbs : mp.n_active) + k) * N + t0;
+        }
+    }
+    // accumulator and output in 16-byte pairs when both are aligned (block-uniform);
+    // lane word j is then tile word 2 (tid + (j / 2) NT) + j % 2, else tid + j NT
+    const bool vec = (((uintptr_t)accp | (uintptr_t)out) & 15) == 0;
+    auto word_of = [&](int j) {
+        return vec ? 2 * ((int)threadIdx.x + (j >> 1) * NT) + (j & 1) : (int)threadIdx.x + j * NT;
+    };
     uint64_t accv[IO == IO_RESCALE ? PER : 1];
     auto load_acc = [&] {
         if (FWD && IO == IO_RESCALE) {
+            if (vec) {
 #pragma unroll
-            for (int j = 0; j < PER; j++) accv[j] = accp[threadIdx.x + j * NT];
+                for (int j = 0; j < PER; j += 2) {
+                    const u64x2 v = *global_ptr(reinterpret_cast<const u64x2 *>(accp + word_of(j)));
+                    accv[j] = v.x;
+                    accv[j + 1] = v.y;
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < PER; j++) accv[j] = *global_ptr(accp + word_of(j));
+            }
         }
     };
     __syncthreads();
@@ -412,38 +486,57 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 6 : 1) void k_ntt_blocks(uint64_t *
         return v >= q ? v - q : v;
     };
     if (FWD && IO == IO_RESCALE) {
-        uint64_t *out;
-        if (io.packed || io.out_packed) {
-            out = io.out_p.p[poly] + (size_t)k * N + t0;
-        } else {
-            uint64_t *ob = io.outs ? io.outs[poly / io.ppo] : io.out_single;
-            out = ob + ((size_t)(poly % io.ppo) * (io.out_limbs ? io.out_limbs : mp.n_active) + k) * N + t0;
-        }
         const uint64_t qi = io.qinv[prime], qis = io.qinv_s[prime];
         // unreduced outputs stay below (4 + 4 logN) q (< 2^64 for the lazy primes): acc - v
         // is taken as acc + (4 + 4 logN) q - v, which the Shoup product accepts as it is
         const uint64_t kq = (uint64_t)(4 + 4 * logN) * q;
         const bool addto = io.out_packed && poly < 64 && ((io.acc_mask >> poly) & 1);
+        auto result = [&](uint64_t acc, uint64_t raw) -> uint64_t {
+            return LAZY ? mul_shoup(acc + kq - raw, qi, qis, q) : mul_shoup(sub_mod(acc, canon(raw), q), qi, qis, q);
+        };
+        auto addend = [&](int i) -> uint64_t {
+            return *global_ptr(io.add_src.p[poly] + (size_t)k * N + io.add_perm[t0 + i]);
+        };
+        if (vec) {
 #pragma unroll
-        for (int j = 0; j < PER; j++) {
-            const int i = threadIdx.x + j * NT;
-            const int b = i / T, r = i % T;
-            const uint64_t raw = lds[b * TP + swz<T>(r)];
-            uint64_t y = LAZY ? mul_shoup(accv[j] + kq - raw, qi, qis, q)
-                              : mul_shoup(sub_mod(accv[j], canon(raw), q), qi, qis, q);
-            if (addto)
-                y = add_mod(y, io.add_perm ? io.add_src.p[poly][(size_t)k * N + io.add_perm[t0 + i]] : out[i], q);
-            __builtin_nontemporal_store(y, &out[i]);
+            for (int j = 0; j < PER; j += 2) {
+                const int i = word_of(j);
+                const u64x2 raw = lds_get(i);
+                u64x2 y;
+                y.x = result(accv[j], raw.x);
+                y.y = result(accv[j + 1], raw.y);
+                if (addto) {
+                    if (io.add_perm) {
+                        y.x = add_mod(y.x, addend(i), q);
+                        y.y = add_mod(y.y, addend(i + 1), q);
+                    } else {
+                        const u64x2 o = *global_ptr(reinterpret_cast<const u64x2 *>(out + i));
+                        y.x = add_mod(y.x, o.x, q);
+                        y.y = add_mod(y.y, o.y, q);
+                    }
+                }
+                __builtin_nontemporal_store(y, global_ptr(reinterpret_cast<u64x2 *>(out + i)));
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < PER; j++) {
+                const int i = word_of(j);
+                uint64_t y = result(accv[j], lds[slot(i)]);
+                if (addto) y = add_mod(y, io.add_perm ? addend(i) : *global_ptr(out + i), q);
+                __builtin_nontemporal_store(y, global_ptr(out + i));
+            }
         }
     } else if (FWD) {
-        for (int i = threadIdx.x; i < kTile; i += NT) {
-            const int b = i / T, r = i % T;
-            __builtin_nontemporal_store(canon(lds[b * TP + swz<T>(r)]), &base[i]);
+#pragma unroll
+        for (int j = 0; j < kTile / NT; j++) {
+            const int i = threadIdx.x + j * NT;
+            __builtin_nontemporal_store(canon(lds[slot(i)]), &base[i]);
         }
     } else {
-        for (int i = threadIdx.x; i < kTile; i += NT) {
-            const int b = i / T, r = i % T;
-            base[i] = lds[b * TP + swz<T>(r)];
+#pragma unroll
+        for (int j = 0; j < kTile / NT; j++) {
+            const int i = threadIdx.x + j * NT;
+            base[i] = lds[slot(i)];
         }
     }
 }
@@ -502,6 +595,11 @@ void launch_ntt(const Context &c, uint64_t *data, const LimbMap &mp, size_t tota
     // passes and the inverse ones; the forward passes of primes above the lazy bound keep
     // 256-thread radix-16 rounds (their fused form would spill at 80 registers)
     const bool tile512 = ks_tuning().ntt_tile_threads == 512;
+    // the tile pass reads its input in 16-byte pairs of words
+    bool aligned = ((uintptr_t)data & 15) == 0 && mp.poly_stride % 2 == 0;
+    if (IOB == IO_GATHER)
+        for (uint32_t p = 0; p < n_polys; p++) aligned = aligned && ((uintptr_t)io.acc_p.p[p] & 15) == 0;
+    if (!aligned) throw LogicError("NTT operands must be 16-byte aligned");
 #define SHM_COLS(L)                                                                                    \
     case L:                                                                                            \
         if (lazy)                                                                                      \
