@@ -64,9 +64,9 @@ BOOT_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "ntt_tile_passes_shared_a_
 # blocks): the plain forward pass and the fused rescale / ModDown pass
 PLAIN_TILE_KERNEL = "k_ntt_blocks<12,true,0,true,512>"
 RESCALE_TILE_KERNEL = "k_ntt_blocks<12,true,2,true,512>"
-TILE_NOTE = ("12 stages of 64-bit integer Harvey butterflies through LDS per 4096-word tile; bound by VALU issue "
-             "of the modular products and the twiddle loads, not by HBM (the same tile's load and store alone run "
-             "near peak)")
+TILE_NOTE = ("12 stages of 64-bit integer Harvey butterflies through LDS per 4096-word tile (4 rounds of radix-8 "
+             "units; the two wave-uniform rounds read their twiddles through scalar loads); bound by VALU issue of "
+             "the 64-bit modular products, not by HBM (the same tile's load and store alone run near peak)")
 
 
 def measured_traffic(kernel: str, path: str = TRAFFIC_PROFILE):
