@@ -111,7 +111,8 @@ int shm_galois_elt_from_step(const shm_context *ctx, int step, uint32_t *elt);
 /* ---- raw limb transforms ------------------------------------------------------
  * Replace nwt_2d_radix8_forward_inplace / nwt_2d_radix8_backward_inplace
  * (fntt_2d.cu:620, intt_2d.cu:724): n_polys x n_limbs limbs, limb k uses prime
- * first_prime + k. `data` is a device pointer. */
+ * first_prime + k. `data` is a device pointer aligned to 16 bytes (the tile pass
+ * reads pairs of words; SHM_ERR_INVALID_ARGUMENT otherwise, before any launch). */
 int shm_ntt_forward(shm_context *ctx, uint64_t *data, uint32_t n_polys, uint32_t n_limbs, uint32_t first_prime,
                     shm_stream_t stream);
 int shm_ntt_inverse(shm_context *ctx, uint64_t *data, uint32_t n_polys, uint32_t n_limbs, uint32_t first_prime,

@@ -54,6 +54,19 @@ namespace {
 constexpr int kTile = 4096;
 constexpr int kThreads = 256;
 
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+// Pointers the kernel reads out of device memory or selects between are known to
+// address global memory: the cast lets their accesses use global instead of flat
+// instructions (flat ones also hold the LDS wait counter).
+template <typename V>
+__device__ __forceinline__ __attribute__((address_space(1))) V *global_ptr(V *p) {
+    return (__attribute__((address_space(1))) V *)p;
+}
+template <typename V>
+__device__ __forceinline__ const __attribute__((address_space(1))) V *global_ptr(const V *p) {
+    return (const __attribute__((address_space(1))) V *)p;
+}
+
 // Per-prime constant as the butterflies see it.
 struct IntK {
     uint64_t q;
@@ -362,19 +375,6 @@ __global__ __launch_bounds__(kThreads, IO == IO_ENCODE ? 1 : 6) void k_ntt_cols(
 // O = (limb * tiles + tile) * n_polys + poly: the polynomials that share one
 // prime and tile (hence one twiddle slice) run back to back on the same L2.
 // 512-thread blocks ask for 6 waves per SIMD (80 VGPRs), three 32 KB tiles per CU.
-typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
-// Pointers the kernel reads out of device memory or selects between are known to
-// address global memory: the cast lets their accesses use global instead of flat
-// instructions (flat ones also hold the LDS wait counter).
-template <typename V>
-__device__ __forceinline__ __attribute__((address_space(1))) V *global_ptr(V *p) {
-    return (__attribute__((address_space(1))) V *)p;
-}
-template <typename V>
-__device__ __forceinline__ const __attribute__((address_space(1))) V *global_ptr(const V *p) {
-    return (const __attribute__((address_space(1))) V *)p;
-}
-
 template <int LOGT, bool FWD, int IO, bool LAZY, int NT = kThreads>
 __global__ __launch_bounds__(NT, NT >= 512 ? 6 : 1) void k_ntt_blocks(uint64_t *__restrict__ data, LimbMap mp, NttTables tb,
                                                          int logN, NttIO io, uint32_t n_polys, uint32_t per_xcd) {
@@ -599,7 +599,7 @@ void launch_ntt(const Context &c, uint64_t *data, const LimbMap &mp, size_t tota
     bool aligned = ((uintptr_t)data & 15) == 0 && mp.poly_stride % 2 == 0;
     if (IOB == IO_GATHER)
         for (uint32_t p = 0; p < n_polys; p++) aligned = aligned && ((uintptr_t)io.acc_p.p[p] & 15) == 0;
-    if (!aligned) throw LogicError("NTT operands must be 16-byte aligned");
+    if (!aligned) throw InvalidArgument("NTT operands must be 16-byte aligned");
 #define SHM_COLS(L)                                                                                    \
     case L:                                                                                            \
         if (lazy)                                                                                      \

@@ -46,6 +46,21 @@ def test_ntt_roundtrip_bitexact(c1):
     assert np.array_equal(ct.numpy(), x)
 
 
+def test_ntt_rejects_misaligned_buffer(c1):
+    """The tile pass reads 16-byte pairs of words: a buffer 8 bytes off that alignment is
+    refused on the host, before any launch, and the words stay as they were."""
+    import shmoai_amd as shm
+    from shmoai_amd import capi
+    ctx, ref = c1
+    x = _rand_limbs(ref, np.random.default_rng(12), 2, 8)
+    ct = shm.Ciphertext(ctx, 2, 1).load(x)
+    with pytest.raises(capi.ShmInvalidArgument, match="16-byte aligned"):
+        ctx.ntt_forward(ct.data_ptr() + 8, 1, 1, 0)
+    with pytest.raises(capi.ShmInvalidArgument, match="16-byte aligned"):
+        ctx.ntt_inverse(ct.data_ptr() + 8, 1, 1, 0)
+    assert np.array_equal(ct.numpy(), x)
+
+
 @pytest.mark.parametrize("logN,bits,n_special", [
     (12, [51, 51, 30, 20, 61], 1),
     (16, [51, 50, 46, 60], 1),
