@@ -33,6 +33,25 @@ inline dim3 rows_grid(size_t rows, size_t N) {
     return dim3((unsigned)bx, (unsigned)by);
 }
 
+// a += sum over k < nterms of x(k) * y(k) (128-bit), the operands of up to 8 terms
+// loaded before their products so their loads are in flight together (nterms is
+// uniform over the block)
+template <typename LX, typename LY>
+__device__ __forceinline__ void mac_terms(Acc128 &a, int nterms, LX x, LY y) {
+    for (int k = 0; k < nterms; k += 8) {
+        uint64_t xv[8], yv[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (k + j < nterms) {
+                xv[j] = x(k + j);
+                yv[j] = y(k + j);
+            }
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (k + j < nterms) mac128(a, xv[j], yv[j]);
+    }
+}
+
 struct PtPack {
     const uint64_t *x[kMaxTerms];
     const uint64_t *pt[kMaxTerms];
@@ -47,7 +66,8 @@ __global__ void k_lt_mac(uint64_t *acc, PtPack pk, int nterms, const DevMod *mod
         const DevMod m = mods[limb];
         for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
             Acc128 a{0, 0};
-            for (int k = 0; k < nterms; k++) mac128(a, pk.x[k][row * N + i], pk.pt[k][limb * N + i]);
+            mac_terms(a, nterms, [&](int k) { return pk.x[k][row * N + i]; },
+                      [&](int k) { return pk.pt[k][limb * N + i]; });
             __builtin_nontemporal_store(reduce128(a, m), &acc[row * N + i]);
         }
     }
@@ -76,7 +96,8 @@ __global__ void k_const_lincomb(uint64_t *acc, SrcPack sp, int nterms, const Dev
         __syncthreads();
         for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
             Acc128 a{0, 0};
-            for (int k = 0; k < nterms; k++) mac128(a, sp.x[k][(p * sp.ql[k] + l) * N + i], cres[k]);
+            mac_terms(a, nterms, [&](int k) { return sp.x[k][(p * sp.ql[k] + l) * N + i]; },
+                      [&](int k) { return cres[k]; });
             __builtin_nontemporal_store(reduce128(a, m), &acc[row * N + i]);
         }
     }
@@ -106,7 +127,8 @@ __global__ void k_const_lincomb_batch(LeafPack lp, int nterms, int members, cons
         uint64_t *out = lp.acc[mb] + (p * Ql + l) * N;
         for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
             Acc128 a{0, 0};
-            for (int k = 0; k < nterms; k++) mac128(a, lp.x[mb][k][(p * lp.ql[k] + l) * N + i], cres[k]);
+            mac_terms(a, nterms, [&](int k) { return lp.x[mb][k][(p * lp.ql[k] + l) * N + i]; },
+                      [&](int k) { return cres[k]; });
             __builtin_nontemporal_store(reduce128(a, m), &out[i]);
         }
     }
