@@ -390,17 +390,27 @@ __global__ __launch_bounds__(256) void k_ks_inner_batch(const uint64_t *__restri
     // halves (P52) or as they are.
     __shared__ uint4 kl[MAXB * 256];
     if (ms == 0 && in) {
+        // every digit's two key words are requested before the first is used
+        uint64_t wb[MAXB], wa[MAXB];
+#pragma unroll
+        for (int g = 0; g < MAXB; g++) {
+            wb[g] = wa[g] = 0;
+            if (g < nb) {
+                const uint64_t *kg = key + (size_t)(g0 + g) * 2 * QP * N;
+                wb[g] = kg[(size_t)pi * N + j];
+                wa[g] = kg[(QP + pi) * N + j];
+            }
+        }
 #pragma unroll
         for (int g = 0; g < MAXB; g++) {
             if (g < nb) {
-                const uint64_t *kg = key + (size_t)(g0 + g) * 2 * QP * N;
-                const uint64_t wb = kg[(size_t)pi * N + j], wa = kg[(QP + pi) * N + j];
                 uint4 w;
                 if (P52) {
-                    const Split26 bs = split26(wb), as = split26(wa);
+                    const Split26 bs = split26(wb[g]), as = split26(wa[g]);
                     w = make_uint4(bs.lo, bs.hi, as.lo, as.hi);
                 } else {
-                    w = make_uint4((uint32_t)wb, (uint32_t)(wb >> 32), (uint32_t)wa, (uint32_t)(wa >> 32));
+                    w = make_uint4((uint32_t)wb[g], (uint32_t)(wb[g] >> 32), (uint32_t)wa[g],
+                                   (uint32_t)(wa[g] >> 32));
                 }
                 kl[g * 256 + lane] = w;
             }
