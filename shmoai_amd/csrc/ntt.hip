@@ -433,7 +433,7 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 6 : 1) void k_ntt_blocks(uint64_t *
 #pragma unroll
     for (int j = 0; j < PAIRS; j++) {
         const int i = 2 * (threadIdx.x + j * NT);
-        lds_put(i, *reinterpret_cast<const u64x2 *>(ld + i));
+        lds_put(i, __builtin_nontemporal_load(reinterpret_cast<const u64x2 *>(ld + i)));
     }
     // rescale: the accumulator words are loaded before the last round, so they
     // arrive during it (loading them before all rounds cost occupancy)
@@ -462,7 +462,7 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 6 : 1) void k_ntt_blocks(uint64_t *
             if (vec) {
 #pragma unroll
                 for (int j = 0; j < PER; j += 2) {
-                    const u64x2 v = *global_ptr(reinterpret_cast<const u64x2 *>(accp + word_of(j)));
+                    const u64x2 v = __builtin_nontemporal_load(global_ptr(reinterpret_cast<const u64x2 *>(accp + word_of(j))));
                     accv[j] = v.x;
                     accv[j + 1] = v.y;
                 }
