@@ -543,7 +543,7 @@ def matmul_bench(shm, args, threads):
     shm.set_tuning("matmul_overlap", 0)
     k_ms, t_ms = float(np.mean(kernel_ms)), float(np.mean(tile_ms))
     N = ctx.N
-    alg_step = ROW_W * 2 * L * N * 8 + COL_W * 2 * (L - 1) * N * 8   # read X, write Y (SURVEY 9.46 GB)
+    alg_step = ROW_W * 2 * L * N * 8 + COL_W * 2 * (L - 1) * N * 8   # read X, write Y (9.46 GB)
     contraction_bytes = ROW_W * 2 * L * N * 8 + COL_W * 2 * L * N * 8  # what the contraction kernel moves
     macs = ROW_W * COL_W * 2 * L * N
     tops = 2 * 35 * macs / (k_ms * 1e-3) / 1e12
@@ -604,7 +604,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported: the 768 ciphertexts "
-              "shard by chunk across GPUs with keys replicated (DESIGN.md), and only one GPU is measured here",
+              "would shard by chunk across GPUs with the keys replicated on each, and only one GPU is measured here",
               file=sys.stderr)
         return 2
 
