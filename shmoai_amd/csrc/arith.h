@@ -137,6 +137,13 @@ __device__ __forceinline__ uint64_t reduce52(const Acc52 &a, const DevMod &m) {
     return csub(csub(r, m.q << 1), m.q);
 }
 
+// a b mod q for a, b < 2^53 and 2^45 < q < 2^52: the four 26-bit split products (S < 2^106)
+// and reduce52, about half the multiplies of mul_mod; the same canonical residue.
+__device__ __forceinline__ uint64_t mul_mod52(uint64_t a, uint64_t b, const DevMod &m) {
+    Acc52 acc{0, 0, 0};
+    mac52(acc, split26(a), b);
+    return reduce52(acc, m);
+}
 // [v]_q of an integral double v with |v| < 2^128; negative v maps to q - (|v| mod q),
 // which is q itself when |v| is a multiple of q (callers reduce when they need [0, q)).
 __device__ __forceinline__ uint64_t residue_of_rounded(double v, const DevMod &m) {

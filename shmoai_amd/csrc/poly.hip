@@ -73,15 +73,17 @@ __global__ void k_mul_plain(uint64_t *ct, const uint64_t *pt, const DevMod *mods
 // the three products are added onto out (a sum of tensor products, as
 // multiply + add_inplace of 3-polynomial ciphertexts).
 __global__ void k_tensor(const uint64_t *a, const uint64_t *b, uint64_t *out, const DevMod *mods, size_t Ql,
-                         size_t N, size_t a_limbs, size_t b_limbs, int accumulate) {
+                         size_t N, size_t a_limbs, size_t b_limbs, int accumulate, int p52) {
     const size_t S = Ql * N, Sa = a_limbs * N, Sb = b_limbs * N;
     SHM_ROWS_LOOP(row, Ql) {
         const DevMod m = mods[row];
         SHM_COLS_LOOP(i, N) {
             const size_t t = row * N + i;
             const uint64_t a0 = a[t], a1 = a[t + Sa], b0 = b[t], b1 = b[t + Sb];
-            const uint64_t d0 = mul_mod(a0, b0, m), d2 = mul_mod(a1, b1, m);
-            uint64_t d1 = mul_mod(a0 + a1, b0 + b1, m);
+            // 52-bit primes: split products (operands below 2q < 2^53), same residues
+            auto mm = [&](uint64_t x, uint64_t y) { return p52 ? mul_mod52(x, y, m) : mul_mod(x, y, m); };
+            const uint64_t d0 = mm(a0, b0), d2 = mm(a1, b1);
+            uint64_t d1 = mm(a0 + a1, b0 + b1);
             d1 = d1 + 2 * m.q - d0 - d2;
             d1 = csub(d1, m.q);
             d1 = csub(d1, m.q);
@@ -105,7 +107,7 @@ __global__ void k_tensor(const uint64_t *a, const uint64_t *b, uint64_t *out, co
 // limb; every operand enters through its first Ql limbs.
 __global__ void k_tensor_cheb(const uint64_t *a, const uint64_t *b, const uint64_t *t, uint64_t *out,
                               const DevMod *mods, size_t Ql, size_t N, size_t a_limbs, size_t b_limbs, size_t t_limbs,
-                              uint64_t cval) {
+                              uint64_t cval, int p52) {
     const size_t S = Ql * N, Sa = a_limbs * N, Sb = b_limbs * N, St = t_limbs * N;
     SHM_ROWS_LOOP(row, Ql) {
         const DevMod m = mods[row];
@@ -114,17 +116,23 @@ __global__ void k_tensor_cheb(const uint64_t *a, const uint64_t *b, const uint64
         SHM_COLS_LOOP(i, N) {
             const size_t x = row * N + i;
             const uint64_t a0 = a[x], a1 = a[x + Sa];
+            // 52-bit primes: split products (operands below 2q < 2^53), same residues
+            auto mm = [&](uint64_t u, uint64_t w) { return p52 ? mul_mod52(u, w, m) : mul_mod(u, w, m); };
             uint64_t d0, d1, d2;
             if (a == b) {
-                d0 = mul_mod(a0, a0, m);
-                d2 = mul_mod(a1, a1, m);
-                const uint64_t lo = a0 * a1, hi = __umul64hi(a0, a1);
-                d1 = barrett128((hi << 1) | (lo >> 63), lo << 1, m);
+                d0 = mm(a0, a0);
+                d2 = mm(a1, a1);
+                if (p52) {
+                    d1 = mul_mod52(a0, a1 << 1, m);
+                } else {
+                    const uint64_t lo = a0 * a1, hi = __umul64hi(a0, a1);
+                    d1 = barrett128((hi << 1) | (lo >> 63), lo << 1, m);
+                }
             } else {
                 const uint64_t b0 = b[x], b1 = b[x + Sb];
-                d0 = mul_mod(a0, b0, m);
-                d2 = mul_mod(a1, b1, m);
-                d1 = mul_mod(a0 + a1, b0 + b1, m);
+                d0 = mm(a0, b0);
+                d2 = mm(a1, b1);
+                d1 = mm(a0 + a1, b0 + b1);
                 d1 = csub(csub(d1 + 2 * q - d0 - d2, q), q);
             }
             d0 = add_mod(d0, d0, q);
@@ -143,16 +151,25 @@ __global__ void k_tensor_cheb(const uint64_t *a, const uint64_t *b, const uint64
 
 // polymath.cu:505-550 tensor_square_2x2_rns_poly. One row per limb.
 __global__ void k_square(const uint64_t *a, uint64_t *out, const DevMod *mods, size_t Ql, size_t N,
-                         size_t a_limbs) {
+                         size_t a_limbs, int p52) {
     const size_t S = Ql * N, Sa = a_limbs * N;
     SHM_ROWS_LOOP(row, Ql) {
         const DevMod m = mods[row];
         SHM_COLS_LOOP(i, N) {
             const size_t t = row * N + i;
             const uint64_t c0 = a[t], c1 = a[t + Sa];
-            const uint64_t d0 = mul_mod(c0, c0, m), d2 = mul_mod(c1, c1, m);
-            const uint64_t lo = c0 * c1, hi = __umul64hi(c0, c1);
-            const uint64_t d1 = barrett128((hi << 1) | (lo >> 63), lo << 1, m);
+            uint64_t d0, d1, d2;
+            if (p52) {
+                // split products (2 c1 < 2^53), same residues
+                d0 = mul_mod52(c0, c0, m);
+                d2 = mul_mod52(c1, c1, m);
+                d1 = mul_mod52(c0, c1 << 1, m);
+            } else {
+                d0 = mul_mod(c0, c0, m);
+                d2 = mul_mod(c1, c1, m);
+                const uint64_t lo = c0 * c1, hi = __umul64hi(c0, c1);
+                d1 = barrett128((hi << 1) | (lo >> 63), lo << 1, m);
+            }
             out[t] = d0;
             out[t + S] = d1;
             out[t + 2 * S] = d2;
@@ -423,7 +440,7 @@ void launch_tensor(const Context &c, const uint64_t *a, const uint64_t *b, uint6
     if (!b_limbs) b_limbs = Ql;
     if (a_limbs < Ql || b_limbs < Ql) throw LogicError("tensor operand below the product level");
     k_tensor<<<rows_grid(Ql, c.N), kBlock, 0, s>>>(a, b, out3, c.mods(), Ql, c.N, a_limbs, b_limbs,
-                                                   accumulate ? 1 : 0);
+                                                   accumulate ? 1 : 0, c.primes52 ? 1 : 0);
     SHM_HIP(hipGetLastError());
 }
 
@@ -431,14 +448,14 @@ void launch_tensor_cheb(const Context &c, const uint64_t *a, const uint64_t *b, 
                         uint64_t *out3, size_t Ql, hipStream_t s, size_t a_limbs, size_t b_limbs, size_t t_limbs) {
     if (a_limbs < Ql || b_limbs < Ql || (t && t_limbs < Ql)) throw LogicError("Chebyshev operand below the product level");
     k_tensor_cheb<<<rows_grid(Ql, c.N), kBlock, 0, s>>>(a, b, t, out3, c.mods(), Ql, c.N, a_limbs, b_limbs, t_limbs,
-                                                        cval);
+                                                        cval, c.primes52 ? 1 : 0);
     SHM_HIP(hipGetLastError());
 }
 
 void launch_square(const Context &c, const uint64_t *a, uint64_t *out3, size_t Ql, hipStream_t s, size_t a_limbs) {
     if (!a_limbs) a_limbs = Ql;
     if (a_limbs < Ql) throw LogicError("square operand below the product level");
-    k_square<<<rows_grid(Ql, c.N), kBlock, 0, s>>>(a, out3, c.mods(), Ql, c.N, a_limbs);
+    k_square<<<rows_grid(Ql, c.N), kBlock, 0, s>>>(a, out3, c.mods(), Ql, c.N, a_limbs, c.primes52 ? 1 : 0);
     SHM_HIP(hipGetLastError());
 }
 
