@@ -33,11 +33,16 @@ inline dim3 rows_grid(size_t rows, size_t N) {
     return dim3((unsigned)bx, (unsigned)by);
 }
 
-// a += sum over k < nterms of x(k) * y(k) (128-bit), the operands of up to 8 terms
-// loaded before their products so their loads are in flight together (nterms is
-// uniform over the block)
+// [sum over k < nterms of x(k) * y(k)]_q for residues x, y < q and nterms <= 16, the
+// operands of up to 8 terms loaded before their products so their loads are in flight
+// together (nterms and p52 are uniform over the block). With p52 (every prime in
+// (2^45, 2^52)) the products are 26-bit split multiply-adds into an Acc52 (sum below
+// 16 * 2^104 < 2^109) reduced by reduce52, else 128-bit products and Barrett; the same
+// canonical residue either way.
 template <typename LX, typename LY>
-__device__ __forceinline__ void mac_terms(Acc128 &a, int nterms, LX x, LY y) {
+__device__ __forceinline__ uint64_t sum_terms(int nterms, int p52, const DevMod &m, LX x, LY y) {
+    Acc128 a{0, 0};
+    Acc52 b{0, 0, 0};
     for (int k = 0; k < nterms; k += 8) {
         uint64_t xv[8], yv[8];
 #pragma unroll
@@ -46,10 +51,17 @@ __device__ __forceinline__ void mac_terms(Acc128 &a, int nterms, LX x, LY y) {
                 xv[j] = x(k + j);
                 yv[j] = y(k + j);
             }
+        if (p52) {
 #pragma unroll
-        for (int j = 0; j < 8; j++)
-            if (k + j < nterms) mac128(a, xv[j], yv[j]);
+            for (int j = 0; j < 8; j++)
+                if (k + j < nterms) mac52(b, split26(xv[j]), yv[j]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                if (k + j < nterms) mac128(a, xv[j], yv[j]);
+        }
     }
+    return p52 ? reduce52(b, m) : reduce128(a, m);
 }
 
 struct PtPack {
@@ -60,15 +72,14 @@ struct PtPack {
 // acc[p][l][i] = sum_t x_t[p][l][i] * pt_t[l][i] mod q_l over npolys polynomials;
 // products accumulate in 128 bits (at most 16 terms below 2^122).
 __global__ void k_lt_mac(uint64_t *acc, PtPack pk, int nterms, const DevMod *mods, size_t Ql, size_t N,
-                         size_t npolys) {
+                         size_t npolys, int p52) {
     for (size_t row = blockIdx.y; row < npolys * Ql; row += gridDim.y) {
         const size_t limb = row % Ql;
         const DevMod m = mods[limb];
         for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
-            Acc128 a{0, 0};
-            mac_terms(a, nterms, [&](int k) { return pk.x[k][row * N + i]; },
-                      [&](int k) { return pk.pt[k][limb * N + i]; });
-            __builtin_nontemporal_store(reduce128(a, m), &acc[row * N + i]);
+            const uint64_t v = sum_terms(nterms, p52, m, [&](int k) { return pk.x[k][row * N + i]; },
+                                         [&](int k) { return pk.pt[k][limb * N + i]; });
+            __builtin_nontemporal_store(v, &acc[row * N + i]);
         }
     }
 }
@@ -83,7 +94,8 @@ struct SrcPack {
 // ciphertexts at different levels, each read over the first Ql limbs. One row
 // (p, l) per blockIdx.y; the constant residues of the row are formed once, from
 // the integral doubles, into LDS.
-__global__ void k_const_lincomb(uint64_t *acc, SrcPack sp, int nterms, const DevMod *mods, size_t Ql, size_t N) {
+__global__ void k_const_lincomb(uint64_t *acc, SrcPack sp, int nterms, const DevMod *mods, size_t Ql, size_t N,
+                                int p52) {
     __shared__ uint64_t cres[kMaxTerms];
     for (size_t row = blockIdx.y; row < 2 * Ql; row += gridDim.y) {
         const size_t p = row / Ql, l = row % Ql;
@@ -95,10 +107,9 @@ __global__ void k_const_lincomb(uint64_t *acc, SrcPack sp, int nterms, const Dev
         }
         __syncthreads();
         for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
-            Acc128 a{0, 0};
-            mac_terms(a, nterms, [&](int k) { return sp.x[k][(p * sp.ql[k] + l) * N + i]; },
-                      [&](int k) { return cres[k]; });
-            __builtin_nontemporal_store(reduce128(a, m), &acc[row * N + i]);
+            const uint64_t v = sum_terms(nterms, p52, m, [&](int k) { return sp.x[k][(p * sp.ql[k] + l) * N + i]; },
+                                         [&](int k) { return cres[k]; });
+            __builtin_nontemporal_store(v, &acc[row * N + i]);
         }
     }
 }
@@ -113,7 +124,8 @@ struct LeafPack {
     double v[kLeafTerms];
 };
 
-__global__ void k_const_lincomb_batch(LeafPack lp, int nterms, int members, const DevMod *mods, size_t Ql, size_t N) {
+__global__ void k_const_lincomb_batch(LeafPack lp, int nterms, int members, const DevMod *mods, size_t Ql, size_t N,
+                                      int p52) {
     __shared__ uint64_t cres[kLeafTerms];
     for (size_t row = blockIdx.y; row < (size_t)members * 2 * Ql; row += gridDim.y) {
         const size_t mb = row / (2 * Ql), p = (row / Ql) % 2, l = row % Ql;
@@ -126,10 +138,9 @@ __global__ void k_const_lincomb_batch(LeafPack lp, int nterms, int members, cons
         __syncthreads();
         uint64_t *out = lp.acc[mb] + (p * Ql + l) * N;
         for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
-            Acc128 a{0, 0};
-            mac_terms(a, nterms, [&](int k) { return lp.x[mb][k][(p * lp.ql[k] + l) * N + i]; },
-                      [&](int k) { return cres[k]; });
-            __builtin_nontemporal_store(reduce128(a, m), &out[i]);
+            const uint64_t v = sum_terms(nterms, p52, m, [&](int k) { return lp.x[mb][k][(p * lp.ql[k] + l) * N + i]; },
+                                         [&](int k) { return cres[k]; });
+            __builtin_nontemporal_store(v, &out[i]);
         }
     }
 }
@@ -166,7 +177,8 @@ void const_lincomb(Context &c, Ciphertext &acc, const std::vector<const Cipherte
     acc.ctx = &c;
     acc.resize(2, c.nQ - Ql + 1, s, false);  // k_const_lincomb writes every word
     acc.ntt = true;
-    k_const_lincomb<<<rows_grid(2 * Ql, c.N), kBlock, 0, s>>>(acc.data(), sp, (int)src.size(), c.mods(), Ql, c.N);
+    k_const_lincomb<<<rows_grid(2 * Ql, c.N), kBlock, 0, s>>>(acc.data(), sp, (int)src.size(), c.mods(), Ql, c.N,
+                                                              c.primes52 ? 1 : 0);
     SHM_HIP(hipGetLastError());
 }
 
@@ -214,7 +226,8 @@ void lt_apply(Context &c, const LtLayer &L, Ciphertext &ct, const Rotator &r, hi
         inner.resize(ct.size, ct.chain, s, false);  // k_lt_mac writes every word
         inner.ntt = true;
         inner.scale = ct.scale;
-        k_lt_mac<<<rows_grid(ct.size * Ql, c.N), kBlock, 0, s>>>(inner.data(), pk, nt, c.mods(), Ql, c.N, ct.size);
+        k_lt_mac<<<rows_grid(ct.size * Ql, c.N), kBlock, 0, s>>>(inner.data(), pk, nt, c.mods(), Ql, c.N, ct.size,
+                                                                 c.primes52 ? 1 : 0);
         SHM_HIP(hipGetLastError());
         const long G = ((L.giant_offset(i) % (long)n) + (long)n) % (long)n;
         if (G) r.rot(inner, (int)G);
@@ -540,7 +553,7 @@ struct ChebEvalBatch {
                 }
             }
             k_const_lincomb_batch<<<rows_grid(2 * Qc * mm, c.N), kBlock, 0, s>>>(lp, (int)ks.size(), (int)mm, c.mods(),
-                                                                                Qc, c.N);
+                                                                                Qc, c.N, c.primes52 ? 1 : 0);
             SHM_HIP(hipGetLastError());
         }
         rescale_batch(c, ptrs(out), s, true);
